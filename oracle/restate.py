@@ -1,0 +1,1089 @@
+"""NumPy restatement of G3M^3's per-timestep hot-path subroutines.
+
+TEST INFRASTRUCTURE ONLY -- this is the oracle.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may
+import it, and only as the checker (or as the CPU-port baseline when the
+reference build is absent).  The product (``g3m3_amd``) never imports it.
+
+Every function keeps the reference subroutine's name and positional
+argument list, so a test can hand one argument list to the reference build
+(``oracle.ref``) and to this module.  Conventions:
+
+* a field ``q(nx,ny,nz,n_grids)`` is a float32 numpy array of shape
+  ``(n_grids, nz, ny, nx)`` (C order, byte-identical to the Fortran array);
+  a box-local scratch ``q(nx,ny,nz)`` has shape ``(nz, ny, nx)``;
+  ``box`` is 1-based, as in the reference;
+* arrays are updated in place, as the Fortran dummies are;
+* arithmetic is IEEE single precision, with each expression evaluated in
+  the reference's left-to-right order and with no fused multiply-add, so
+  results match a serial ``-ffp-contract=off`` build of the reference bit
+  for bit (the pinning tests check exactly that);
+* scalar outputs (``set_speed_agrd``'s maxima, ``sparse_solver``'s
+  ``residual2``, ``trilin_interp``'s ``scdata``) are 1-element arrays.
+
+Parity status per function is recorded in the test files that pin it
+(tests/test_oracle_vs_ref.py, tests/test_oracle_invariants.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F = np.float32
+I32 = np.int32
+
+# ---------------------------------------------------------------------------
+# stencil views of the interior 2..n-1 (1-based) of a (nz, ny, nx) array
+
+
+def _sh(dz, dy, dx):
+    return (slice(1 + dz, (-1 + dz) or None),
+            slice(1 + dy, (-1 + dy) or None),
+            slice(1 + dx, (-1 + dx) or None))
+
+
+C = _sh(0, 0, 0)
+XP, XM = _sh(0, 0, 1), _sh(0, 0, -1)
+YP, YM = _sh(0, 1, 0), _sh(0, -1, 0)
+ZP, ZM = _sh(1, 0, 0), _sh(-1, 0, 0)
+
+
+def _f(*xs):
+    return [F(x) for x in xs]
+
+
+# ---------------------------------------------------------------------------
+# code/sub_store_array.f90:1-20
+
+
+def store_array(bx, bx0, nx, ny, nz, n_grids, box):
+    bx[box - 1][...] = bx0[box - 1]
+
+
+# ---------------------------------------------------------------------------
+# code/sub_calcur.f90:5-189  J = curl(b), face/edge/corner rules
+
+
+def calcur(bx, by, bz, nx, ny, nz, n_grids, box, curx, cury, curz, rx, ry, rz):
+    b = box - 1
+    rx, ry, rz = _f(rx, ry, rz)
+    dxt, dyt, dzt = F(2) * rx, F(2) * ry, F(2) * rz
+    X, Y, Z = bx[b], by[b], bz[b]
+    # interior (:36-44)
+    curx[C] = (Z[YP] - Z[YM]) / dyt - (Y[ZP] - Y[ZM]) / dzt
+    cury[C] = (X[ZP] - X[ZM]) / dzt - (Z[XP] - Z[XM]) / dxt
+    curz[C] = (Y[XP] - Y[XM]) / dxt - (X[YP] - X[YM]) / dyt
+    for c in (curx, cury, curz):
+        # bottom/top panels (:58-72)
+        c[0, 1:-1, 1:-1] = c[1, 1:-1, 1:-1]
+        c[-1, 1:-1, 1:-1] = c[-2, 1:-1, 1:-1]
+        # front/back (:76-88)
+        c[1:-1, 1:-1, 0] = c[1:-1, 1:-1, 1]
+        c[1:-1, 1:-1, -1] = c[1:-1, 1:-1, -2]
+        # left/right (:92-104)
+        c[1:-1, 0, 1:-1] = c[1:-1, 1, 1:-1]
+        c[1:-1, -1, 1:-1] = c[1:-1, -2, 1:-1]
+        # corner lines along x (:109-121); note (i,ny,nz) <- (i,ny1,nz)
+        c[0, 0, 1:-1] = c[1, 0, 1:-1]
+        c[-1, 0, 1:-1] = c[-2, 0, 1:-1]
+        c[0, -1, 1:-1] = c[1, -1, 1:-1]
+        c[-1, -1, 1:-1] = c[-1, -2, 1:-1]
+        # corner lines along y (:124-138)
+        c[0, 1:-1, 0] = c[1, 1:-1, 0]
+        c[-1, 1:-1, 0] = c[-2, 1:-1, 0]
+        c[0, 1:-1, -1] = c[1, 1:-1, -1]
+        c[-1, 1:-1, -1] = c[-2, 1:-1, -1]
+        # corner lines along z (:141-155)
+        c[1:-1, 0, 0] = c[1:-1, 1, 0]
+        c[1:-1, 0, -1] = c[1:-1, 1, -1]
+        c[1:-1, -1, 0] = c[1:-1, -2, 0]
+        c[1:-1, -1, -1] = c[1:-1, -2, -1]
+        # corner points (:157-186)
+        c[0, 0, 0] = c[1, 0, 0]
+        c[0, -1, 0] = c[1, -1, 0]
+        c[-1, 0, 0] = (c[-2, 0, 0] + c[-1, 1, 0]) / F(2)
+        c[-1, -1, 0] = (c[-2, -1, 0] + c[-1, -2, 0]) / F(2)
+        c[0, 0, -1] = (c[1, 0, -1] + c[0, 1, -1] + c[0, 0, -2]) / F(3)
+        c[0, -1, -1] = (c[1, -1, -1] + c[0, -2, -1] + c[0, -1, -2]) / F(3)
+        c[-1, 0, -1] = (c[-2, 0, -1] + c[-1, 1, -1] + c[-1, 0, -2]) / F(3)
+        c[-1, -1, -1] = (c[-2, -1, -1] + c[-1, -2, -1] + c[-1, -1, -2]) / F(3)
+
+
+# ---------------------------------------------------------------------------
+# code/sub_fnd_evel.f90:4-57
+
+
+def fnd_evel(qpx, qpy, qpz, qrho, hpx, hpy, hpz, hrho, opx, opy, opz, orho,
+             curx, cury, curz, evx, evy, evz, tvx, tvy, tvz,
+             nx, ny, nz, n_grids, box, rmassq, rmassh, rmasso, reynolds):
+    b = box - 1
+    rmassq, rmassh, rmasso, reynolds = _f(rmassq, rmassh, rmasso, reynolds)
+    d = F(0.000001)
+    qden = (qrho[b] + d) / rmassq
+    hden = (hrho[b] + d) / rmassh
+    oden = (orho[b] + d) / rmasso
+    tot = qden + hden + oden
+    tden = qrho[b] + hrho[b] + orho[b]
+    tvx[...] = (qpx[b] + hpx[b] + opx[b]) / tden
+    evx[...] = tvx - curx / tot / reynolds
+    tvy[...] = (qpy[b] + hpy[b] + opy[b]) / tden
+    evy[...] = tvy - cury / tot / reynolds
+    tvz[...] = (qpz[b] + hpz[b] + opz[b]) / tden
+    evz[...] = tvz - curz / tot / reynolds
+
+
+# ---------------------------------------------------------------------------
+# code/sub_BandE.f90:5-162  Ohm's law
+
+
+def BandE(efldx, efldy, efldz, bsx, bsy, bsz, curx, cury, curz, evx, evy, evz,
+          btot, epres, qrho, hrho, orho, rst, resist, reynolds,
+          nx, ny, nz, n_grids, box, rmassq, rmassh, rmasso,
+          ijmid, nummid, ijzero, mbndry, numzero, mmid, mzero, rx, ry, rz):
+    b = box - 1
+    rmassq, rmassh, rmasso, reynolds, resist = _f(rmassq, rmassh, rmasso, reynolds, resist)
+    rx, ry, rz = _f(rx, ry, rz)
+    # -(ve x B) everywhere (:32-51)
+    efldx[...] = -(evy * bsz - evz * bsy)
+    efldy[...] = -(evz * bsx - evx * bsz)
+    efldz[...] = -(evx * bsy - evy * bsx)
+    # grad pe on the interior (:55-84)
+    dxt, dyt, dzt = F(2) * rx, F(2) * ry, F(2) * rz
+    q, h, o, ep = qrho[b], hrho[b], orho[b], epres[b]
+    arho = (q[C] / rmassq + h[C] / rmassh + o[C] / rmasso) * reynolds
+    efldx[C] = efldx[C] - ((ep[XP] - ep[XM]) / dxt) / arho
+    efldy[C] = efldy[C] - ((ep[YP] - ep[YM]) / dyt) / arho
+    efldz[C] = efldz[C] - ((ep[ZP] - ep[ZM]) / dzt) / arho
+    # resistive term on the inner-boundary lists (:88-116)
+    if box <= mbndry and resist < F(5000.):
+        for lst, num in ((ijmid, nummid), (ijzero, numzero)):
+            n = int(num[b])
+            if n == 0:
+                continue
+            ii = lst[:n, 0, b] - 1
+            jj = lst[:n, 1, b] - 1
+            kk = lst[:n, 2, b] - 1
+            eden = (q[kk, jj, ii] / rmassq + h[kk, jj, ii] / rmassh
+                    + o[kk, jj, ii] / rmasso)
+            r = rst[b][kk, jj, ii]
+            efldx[kk, jj, ii] = efldx[kk, jj, ii] + r * curx[kk, jj, ii] / eden
+            efldy[kk, jj, ii] = efldy[kk, jj, ii] + r * cury[kk, jj, ii] / eden
+            efldz[kk, jj, ii] = efldz[kk, jj, ii] + r * curz[kk, jj, ii] / eden
+    # flank copies, x then z then y (:125-159)
+    for e in (efldx, efldy, efldz):
+        e[:, :, 0] = e[:, :, 1]
+        e[:, :, -1] = e[:, :, -2]
+    for e in (efldx, efldy, efldz):
+        e[0] = e[1]
+        e[-1] = e[-2]
+    for e in (efldx, efldy, efldz):
+        e[:, 0, :] = e[:, 1, :]
+        e[:, -1, :] = e[:, -2, :]
+
+
+# ---------------------------------------------------------------------------
+# code/sub_push_elec.f90:4-53
+
+
+def push_elec(wrkepres, oldepres, epres, evx, evy, evz, gamma, gamma1,
+              nx, ny, nz, n_grids, box, delt, rx, ry, rz):
+    b = box - 1
+    gamma, gamma1, delt, rx, ry, rz = _f(gamma, gamma1, delt, rx, ry, rz)
+    dxt, dyt, dzt = F(2) * rx, F(2) * ry, F(2) * rz
+    w = wrkepres[b]
+    egx = (w[XP] - w[XM]) / dxt
+    egy = (w[YP] - w[YM]) / dyt
+    egz = (w[ZP] - w[ZM]) / dzt
+    e = oldepres[b][C] - delt * gamma * (
+        ((w[XP] * evx[XP] - w[XM] * evx[XM]) / dxt)
+        + ((w[YP] * evy[YP] - w[YM] * evy[YM]) / dyt)
+        + ((w[ZP] * evz[ZP] - w[ZM] * evz[ZM]) / dzt))
+    epres[b][C] = e + delt * gamma1 * (evx[C] * egx + evy[C] * egy + evz[C] * egz)
+
+
+# ---------------------------------------------------------------------------
+# code/sub_push_bfld.f90:4-50  induction
+
+
+def push_bfld(bx, by, bz, oldbx, oldby, oldbz, efldx, efldy, efldz,
+              nx, ny, nz, n_grids, box, delt, rx, ry, rz):
+    b = box - 1
+    delt, rx, ry, rz = _f(delt, rx, ry, rz)
+    dxt, dyt, dzt = F(2) * rx, F(2) * ry, F(2) * rz
+    ex, ey, ez = efldx, efldy, efldz
+    bx[b][C] = oldbx[b][C] - delt * (((ez[YP] - ez[YM]) / dyt) - ((ey[ZP] - ey[ZM]) / dzt))
+    by[b][C] = oldby[b][C] + delt * (((ez[XP] - ez[XM]) / dxt) - ((ex[ZP] - ex[ZM]) / dzt))
+    bz[b][C] = oldbz[b][C] - delt * (((ey[XP] - ey[XM]) / dxt) - ((ex[YP] - ex[YM]) / dyt))
+
+
+# ---------------------------------------------------------------------------
+# code/sub_push_ion.f90:4-362  two-step Lax-Wendroff ion push, one species
+
+
+def push_ion(qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz,
+             oldqrho, oldqpresx, oldqpresy, oldqpresz, oldqpx, oldqpy, oldqpz,
+             wrkqrho, wrkqpresx, wrkqpresy, wrkqpresz, wrkqpx, wrkqpy, wrkqpz,
+             qpresxy, qpresxz, qpresyz, oldqpresxy, oldqpresxz, oldqpresyz,
+             wrkqpresxy, wrkqpresxz, wrkqpresyz,
+             bsx, bsy, bsz, btot, efldx, efldy, efldz, rmassq,
+             qvx, qvy, qvz, tvx, tvy, tvz, gamma, gamma1,
+             nx, ny, nz, n_grids, box, delt, grav, re_equiv, reynolds,
+             grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax, isotropic):
+    b = box - 1
+    rmassq, delt, grav, re_equiv, reynolds = _f(rmassq, delt, grav, re_equiv, reynolds)
+    ddx = (F(grd_xmax[b]) - F(grd_xmin[b])) / F(nx - 1)
+    ddy = (F(grd_ymax[b]) - F(grd_ymin[b])) / F(ny - 1)
+    ddz = (F(grd_zmax[b]) - F(grd_zmin[b])) / F(nz - 1)
+    dxt, dyt, dzt = F(2) * ddx, F(2) * ddy, F(2) * ddz
+    d_min = F(1e-6)
+    rmass = rmassq
+    bmax = F(0.20) * rmassq / delt / reynolds
+
+    wr, wpx, wpy, wpz = wrkqrho[b], wrkqpx[b], wrkqpy[b], wrkqpz[b]
+    wx, wy, wz = wrkqpresx[b], wrkqpresy[b], wrkqpresz[b]
+    wxy, wxz, wyz = wrkqpresxy[b], wrkqpresxz[b], wrkqpresyz[b]
+
+    # velocities (:67-81)
+    aq = np.maximum(wr, d_min)
+    qvx[...] = wpx / aq
+    qvy[...] = wpy / aq
+    qvz[...] = wpz / aq
+
+    def flux(w):
+        return (((w[XP] * qvx[XP] - w[XM] * qvx[XM]) / dxt)
+                + ((w[YP] * qvy[YP] - w[YM] * qvy[YM]) / dyt)
+                + ((w[ZP] * qvz[ZP] - w[ZM] * qvz[ZM]) / dzt))
+
+    # continuity (:107-110)
+    qrho[b][C] = oldqrho[b][C] - delt * (((wpx[XP] - wpx[XM]) / dxt)
+                                        + ((wpy[YP] - wpy[YM]) / dyt)
+                                        + ((wpz[ZP] - wpz[ZM]) / dzt))
+    aqpx = oldqpx[b][C] - delt * flux(wpx)
+    aqpy = oldqpy[b][C] - delt * flux(wpy)
+    aqpz = oldqpz[b][C] - delt * flux(wpz)
+
+    abx, aby, abz = bsx[C], bsy[C], bsz[C]
+    bmag = btot[C]
+    weight = np.maximum(F(1), bmag / bmax)
+    skin_depth = reynolds / weight
+
+    tx, ty, tz = tvx[C], tvy[C], tvz[C]
+    vx, vy, vz = qvx[C], qvy[C], qvz[C]
+    tcx = (ty * abz - tz * aby)
+    tcy = -(tx * abz - tz * abx)
+    tcz = (tx * aby - ty * abx)
+    qcx = (vy * abz - vz * aby)
+    qcy = -(vx * abz - vz * abx)
+    qcz = (vx * aby - vy * abx)
+
+    gpx = (wx[XP] - wx[XM]) / dxt + (wxy[YP] - wxy[YM]) / dxt + (wxz[ZP] - wxz[ZM]) / dxt
+    gpy = (wxy[XP] - wxy[XM]) / dxt + (wy[YP] - wy[YM]) / dyt + (wyz[ZP] - wyz[ZM]) / dxt
+    gpz = (wxz[XP] - wxz[XM]) / dxt + (wyz[YP] - wyz[YM]) / dyt + (wz[ZP] - wz[ZM]) / dzt
+
+    dex = (efldx[C] + tcx) * reynolds
+    dey = (efldy[C] + tcy) * reynolds
+    dez = (efldz[C] + tcz) * reynolds
+    dvx = (qcx - tcx) * skin_depth
+    dvy = (qcy - tcy) * skin_depth
+    dvz = (qcz - tcz) * skin_depth
+
+    qden = wr[C] / rmassq
+    px_ = aqpx + delt * (qden * (dex + dvx) - gpx)
+    py_ = aqpy + delt * (qden * (dey + dvy) - gpy)
+    pz_ = aqpz + delt * (qden * (dez + dvz) - gpz)
+
+    # gravity (:202-206)
+    ax = (F(grd_xmin[b]) + ddx * np.arange(1, nx - 1, dtype=F))[None, None, :]
+    ay = (F(grd_ymin[b]) + ddy * np.arange(1, ny - 1, dtype=F))[None, :, None]
+    az = (F(grd_zmin[b]) + ddz * np.arange(1, nz - 1, dtype=F))[:, None, None]
+    radius = np.sqrt(ax * ax + ay * ay + az * az) + F(0.0000001)
+    t = re_equiv * radius
+    g = delt * wr[C] * grav / (t * t)
+    qpx[b][C] = px_ - g * ax / radius
+    qpy[b][C] = py_ - g * ay / radius
+    qpz[b][C] = pz_ - g * az / radius
+
+    two = F(2)
+    # diagonal pressures, isotropic part (:210-238)
+    px0 = oldqpresx[b][C] - delt * flux(wx) - two * delt * wx[C] * (qvx[XP] - qvx[XM]) / dxt
+    py0 = oldqpresy[b][C] - delt * flux(wy) - two * delt * wy[C] * (qvy[YP] - qvy[YM]) / dyt
+    pz0 = oldqpresz[b][C] - delt * flux(wz) - two * delt * wz[C] * (qvz[ZP] - qvz[ZM]) / dzt
+
+    # smoothed B for the gyrotropic terms (:242-254)
+    abx = (F(6) * bsx[C] + bsx[XP] + bsx[XM] + bsx[YP] + bsx[YM] + bsx[ZP] + bsx[ZM]) / F(12)
+    aby = (F(6) * bsy[C] + bsy[XP] + bsy[XM] + bsy[YP] + bsy[YM] + bsy[ZP] + bsy[ZM]) / F(12)
+    abz = (F(6) * bsz[C] + bsz[XP] + bsz[XM] + bsz[YP] + bsz[YM] + bsz[ZP] + bsz[ZM]) / F(12)
+    bmag = np.sqrt(abx * abx + aby * aby + abz * abz)
+    sf = np.where(bmag > bmax, reynolds * (bmax / bmag), reynolds).astype(F)
+
+    cxy, cxz, cyz = wxy[C], wxz[C], wyz[C]
+    cx, cy, cz = wx[C], wy[C], wz[C]
+    qpresx[b][C] = (px0 - two * delt * cxy * (qvx[YP] - qvx[YM]) / dyt
+                    - two * delt * cxz * (qvx[ZP] - qvx[ZM]) / dzt
+                    + two * delt * sf / rmass * (cxy * abz - cxz * aby))
+    qpresy[b][C] = (py0 - two * delt * cxy * (qvy[XP] - qvy[XM]) / dxt
+                    - two * delt * cyz * (qvy[ZP] - qvy[ZM]) / dzt
+                    + two * delt * sf / rmass * (cyz * abx - cxy * abz))
+    qpresz[b][C] = (pz0 - two * delt * cxz * (qvz[XP] - qvz[XM]) / dxt
+                    - two * delt * cyz * (qvz[YP] - qvz[YM]) / dyt
+                    + two * delt * sf / rmass * (cxz * aby - cyz * abx))
+
+    # off-diagonal pressures (:285-356)
+    pxy = oldqpresxy[b][C] - delt * flux(wxy)
+    qpresxy[b][C] = (pxy - delt * cxy * ((qvx[XP] - qvx[XM]) / dxt + (qvy[YP] - qvy[YM]) / dyt)
+                     - delt * cy * (qvx[YP] - qvx[YM]) / dyt
+                     - delt * cx * (qvy[XP] - qvy[XM]) / dxt
+                     - delt * cyz * (qvx[ZP] - qvx[ZM]) / dzt
+                     - delt * cxz * (qvy[ZP] - qvy[ZM]) / dzt
+                     + delt * sf / rmass * (abz * (cy - cx) - aby * cyz + abx * cxz))
+    pxz = oldqpresxz[b][C] - delt * flux(wxz)
+    qpresxz[b][C] = (pxz - delt * cxz * ((qvx[XP] - qvx[XM]) / dxt + (qvz[ZP] - qvz[ZM]) / dzt)
+                     - delt * cz * (qvx[ZP] - qvx[ZM]) / dzt
+                     - delt * cx * (qvz[XP] - qvz[XM]) / dxt
+                     - delt * cyz * (qvx[YP] - qvx[YM]) / dyt
+                     - delt * cxy * (qvz[YP] - qvz[YM]) / dyt
+                     + delt * sf / rmass * (aby * (cx - cz) - abx * cxy + abz * cyz))
+    pyz = oldqpresyz[b][C] - delt * flux(wyz)
+    qpresyz[b][C] = (pyz - delt * cyz * ((qvy[YP] - qvy[YM]) / dyt + (qvz[ZP] - qvz[ZM]) / dzt)
+                     - delt * cy * (qvz[YP] - qvz[YM]) / dyt
+                     - delt * cz * (qvy[ZP] - qvy[ZM]) / dzt
+                     - delt * cxy * (qvz[XP] - qvz[XM]) / dxt
+                     - delt * cxz * (qvy[XP] - qvy[XM]) / dxt
+                     + delt * sf / rmass * (abx * (cz - cy) - abz * cxz + aby * cxy))
+
+
+# ---------------------------------------------------------------------------
+# code/sub_set_rho.f90:5-104  density floors and negative-pressure reset
+
+
+def set_rho(qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, rmassq,
+            hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, rmassh,
+            orho, opresx, opresy, opresz, opresxy, opresxz, opresyz, rmasso,
+            epres, nx, ny, nz, n_grids, box, o_conc):
+    b = box - 1
+    mins = (F(2.e-4), F(1.e-4), F(2.e-5))
+    groups = ((qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz),
+              (hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz),
+              (orho, opresx, opresy, opresz, opresxy, opresxz, opresyz))
+    for (r, px, py, pz, pxy, pxz, pyz), dmin in zip(groups, mins):
+        R, X, Y, Z = r[b], px[b], py[b], pz[b]
+        m = (R < dmin) | (X < 0) | (Y < 0) | (Z < 0)
+        if not m.any():
+            continue
+        floor = F(1.05) * dmin
+        R[m] = np.maximum(np.abs(R[m]), floor)
+        ap = (np.abs(X[m]) + np.abs(Y[m]) + np.abs(Z[m])) / F(3)
+        X[m] = ap
+        Y[m] = ap
+        Z[m] = ap
+        pxy[b][m] = 0
+        pxz[b][m] = 0
+        pyz[b][m] = 0
+    np.abs(epres[b], out=epres[b])
+
+
+# ---------------------------------------------------------------------------
+# code/sub_set_speed_agrd.f90:6-435  speed limits, neighbour averaging, CFL maxima
+
+_ISO_FLAG = [F(1)]   # the reference's SAVEd iso_flag (:68,75)
+
+
+def set_speed_agrd(nx, ny, nz, n_grids, box, rmassq, rmassh, rmasso,
+                   isotropic, smallbit, aniso_limit, vlim, qvlim, hvlim, ovlim,
+                   qclim, hclim, oclim, cslim,
+                   bx, by, bz, bx0, by0, bz0,
+                   qrho, qpx, qpy, qpz, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz,
+                   hrho, hpx, hpy, hpz, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz,
+                   orho, opx, opy, opz, opresx, opresy, opresz, opresxy, opresxz, opresyz,
+                   epres, btot, pxmax, pymax, pzmax, csmax, alfmax, fastest):
+    b = box - 1
+    smallbit, aniso, vlim, cslim = _f(smallbit, aniso_limit, vlim, cslim)
+    if isotropic:
+        _ISO_FLAG[0] = F(0)
+    iso = _ISO_FLAG[0]
+    mx = [F(0)] * 4   # pxmax, pymax, pzmax, csmax
+    species = ((qrho, qpx, qpy, qpz, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, F(qvlim), F(qclim)),
+               (hrho, hpx, hpy, hpz, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, F(hvlim), F(hclim)),
+               (orho, opx, opy, opz, opresx, opresy, opresz, opresxy, opresxz, opresyz, F(ovlim), F(oclim)))
+    for (r, px, py, pz, Px, Py, Pz, Pxy, Pxz, Pyz, svl, scl) in species:
+        R, X, Y, Z = r[b], px[b], py[b], pz[b]
+        PX, PY, PZ = Px[b], Py[b], Pz[b]
+        # snapshot with anisotropy limit (:88-121)
+        s_rho, s_px, s_py, s_pz = R.copy(), X.copy(), Y.copy(), Z.copy()
+        ap = (PX + PY + PZ) / F(3.0)
+        one_m = F(1.) - aniso
+        s_prx = aniso * PX + one_m * ap
+        s_pry = aniso * PY + one_m * ap
+        s_prz = aniso * PZ + one_m * ap
+        s_pxy = aniso * Pxy[b]
+        s_pxz = aniso * Pxz[b]
+        s_pyz = aniso * Pyz[b]
+        if isotropic:
+            s_prx = PX.copy()
+            s_pry = PX.copy()
+            s_prz = PX.copy()
+            s_pxy = np.zeros_like(PX)
+            s_pxz = np.zeros_like(PX)
+            s_pyz = np.zeros_like(PX)
+        # checks on the interior (:123-184)
+        arho = R[C] + smallbit
+        avx = np.abs(X[C] / arho)
+        avy = np.abs(Y[C] / arho)
+        avz = np.abs(Z[C] / arho)
+        v = np.sqrt(avx * avx + avy * avy + avz * avz)
+        cs = np.sqrt(F(0.33) * (PX[C] + PY[C] + PZ[C]) / arho)
+        flag = (v > (vlim * svl)) | (cs > vlim * svl * scl)
+        if flag.any():
+            def avg(s):
+                return (s[XP] + s[XM] + s[YP] + s[YM] + s[ZP] + s[ZM]) / F(6.)
+
+            def avg_iso(s):
+                return iso * (s[XP] + s[XM] + s[YP] + s[YM] + s[ZP] + s[ZM]) / F(6.)
+            for dst, snap, fn in ((R, s_rho, avg), (X, s_px, avg), (Y, s_py, avg), (Z, s_pz, avg),
+                                  (PX, s_prx, avg), (PY, s_pry, avg), (PZ, s_prz, avg),
+                                  (Pxy[b], s_pxy, avg_iso), (Pxz[b], s_pxz, avg_iso),
+                                  (Pyz[b], s_pyz, avg_iso)):
+                interior = dst[C]
+                interior[flag] = fn(snap)[flag]
+        mx[0] = max(mx[0], avx.max())
+        mx[1] = max(mx[1], avy.max())
+        mx[2] = max(mx[2], avz.max())
+        mx[3] = max(mx[3], cs.max())
+    # total field and electron sound-speed cap (:385-429)
+    bsx = bx0[b] + bx[b]
+    bsy = by0[b] + by[b]
+    bsz = bz0[b] + bz[b]
+    btot[...] = np.maximum(np.sqrt(bsx * bsx + bsy * bsy + bsz * bsz), smallbit)
+    arho = qrho[b][C] + hrho[b][C] + orho[b][C] + smallbit
+    E = epres[b]
+    cs = np.sqrt(E[C] / arho)
+    cap = cs > cslim
+    if cap.any():
+        ec = E[C]
+        ec[cap] = ec[cap] * cslim / cs[cap]
+    amax = (btot[C] / np.sqrt(arho)).max()
+    pxmax[0], pymax[0], pzmax[0], csmax[0] = mx
+    alfmax[0] = max(F(0), amax)
+    fastest[0] = max(F(0), np.sqrt(mx[0] * mx[0] + mx[1] * mx[1] + mx[2] * mx[2]
+                                   + mx[3] * mx[3] + alfmax[0] * alfmax[0]))
+
+
+# ---------------------------------------------------------------------------
+# Lapidus smoothing: code/sub_lap_plasma.f90:4-147, sub_lap_elec.f90:4-51,
+# sub_lap_bfld.f90:4-67
+
+
+def _lap(q, w, vx, vy, vz, chi, delt):
+    uxp = np.abs(vx[XP] - vx[C])
+    uxm = np.abs(vx[C] - vx[XM])
+    uyp = np.abs(vy[YP] - vy[C])
+    uym = np.abs(vy[C] - vy[YM])
+    uzp = np.abs(vz[ZP] - vz[C])
+    uzm = np.abs(vz[C] - vz[ZM])
+    c = q[C]
+    w[C] = c + chi * (delt * (uxp * (q[XP] - c) - uxm * (c - q[XM]))
+                      + delt * (uyp * (q[YP] - c) - uym * (c - q[YM]))
+                      + delt * (uzp * (q[ZP] - c) - uzm * (c - q[ZM])))
+
+
+def lap_plasma(rho, px, py, pz, presx, presy, presz, presxy, presxz, presyz,
+               wrkrho, wrkpx, wrkpy, wrkpz, wrkpresx, wrkpresy, wrkpresz,
+               wrkpresxy, wrkpresxz, wrkpresyz, vx, vy, vz,
+               nx, ny, nz, n_grids, box, chirho, chipxyz, chierg, delt, isotropic):
+    b = box - 1
+    chirho, chipxyz, delt = _f(chirho, chipxyz, delt)
+    _lap(rho[b], wrkrho[b], vx, vy, vz, chirho, delt)
+    for q, w in ((px, wrkpx), (py, wrkpy), (pz, wrkpz), (presx, wrkpresx),
+                 (presy, wrkpresy), (presz, wrkpresz), (presxy, wrkpresxy),
+                 (presxz, wrkpresxz), (presyz, wrkpresyz)):
+        _lap(q[b], w[b], vx, vy, vz, chipxyz, delt)
+
+
+def lap_elec(ppres, wrkppres, vx, vy, vz, nx, ny, nz, n_grids, box,
+             chirho, chipxyz, chierg, delt):
+    b = box - 1
+    _lap(ppres[b], wrkppres[b], vx, vy, vz, F(chipxyz), F(delt))
+
+
+def lap_bfld(bx, by, bz, wrkbx, wrkby, wrkbz, vx, vy, vz, nx, ny, nz, n_grids, box,
+             chirho, chipxyz, chierg, delt):
+    b = box - 1
+    for q, w in ((bx, wrkbx), (by, wrkby), (bz, wrkbz)):
+        _lap(q[b], w[b], vx, vy, vz, F(chierg), F(delt))
+
+
+# code/sub_fnd_vtot.f90:4-41
+def fnd_vtot(qpx, qpy, qpz, qrho, hpx, hpy, hpz, hrho, opx, opy, opz, orho,
+             vx, vy, vz, nx, ny, nz, n_grids, box, rmassq, rmassh, rmasso):
+    b = box - 1
+    tden = qrho[b] + hrho[b] + orho[b]
+    vx[...] = (qpx[b] + hpx[b] + opx[b]) / tden
+    vy[...] = (qpy[b] + hpy[b] + opy[b]) / tden
+    vz[...] = (qpz[b] + hpz[b] + opz[b]) / tden
+
+
+# ---------------------------------------------------------------------------
+# code/sub_fcsmooth.f90:7-242  flux-corrected transport, one field
+
+
+def _limit(s, d1m, chiabs, d1p):
+    return s * np.maximum(F(0), np.minimum(np.minimum(s * d1m, chiabs), s * d1p))
+
+
+def fcsmooth(px, oldpx, wrkpx, nx, ny, nz, n_grids, box, chipx, tempx, tempy, tempz):
+    b = box - 1
+    chipx = F(chipx)
+    P, O, W = px[b], oldpx[b], wrkpx[b]
+    tempx[...] = W
+    # diffusion (:23-33)
+    P[C] = W[C] + chipx * (O[XP] + O[XM] + O[YP] + O[YM] + O[ZP] + O[ZM] - F(6.) * O[C])
+    # boundary points (:37-65)
+    P[:, :, 0] = W[:, :, 0]
+    P[:, :, -1] = W[:, :, -1]
+    P[0] = W[0]
+    P[-1] = W[-1]
+    P[:, 0, :] = W[:, 0, :]
+    P[:, -1, :] = W[:, -1, :]
+    # x antidiffusion (:69-116)
+    dx = np.empty_like(W)
+    d1x = np.empty_like(W)
+    dx[:, :, :-1] = W[:, :, 1:] - W[:, :, :-1]
+    d1x[:, :, :-1] = P[:, :, 1:] - P[:, :, :-1]
+    dx[:, :, -1] = dx[:, :, -2]
+    d1x[:, :, -1] = d1x[:, :, -2]
+    fn = np.empty_like(W)
+    s = np.copysign(F(1), dx[:, :, 1:-1])
+    fn[:, :, 1:-1] = _limit(s, d1x[:, :, :-2], chipx * np.abs(dx[:, :, 1:-1]), d1x[:, :, 2:])
+    fn[:, :, 0] = fn[:, :, 1]
+    fn[:, :, -1] = fn[:, :, -2]
+    tempx[:, :, 1:-1] = P[:, :, 1:-1] - fn[:, :, 1:-1] + fn[:, :, :-2]
+    # y antidiffusion (:120-167)
+    dy = np.empty_like(W)
+    d1y = np.empty_like(W)
+    dy[:, :-1, :] = W[:, 1:, :] - W[:, :-1, :]
+    d1y[:, :-1, :] = P[:, 1:, :] - P[:, :-1, :]
+    dy[:, -1, :] = dy[:, -2, :]
+    d1y[:, -1, :] = d1y[:, -2, :]
+    s = np.copysign(F(1), dy[:, 1:-1, :])
+    fn[:, 1:-1, :] = _limit(s, d1y[:, :-2, :], chipx * np.abs(dy[:, 1:-1, :]), d1y[:, 2:, :])
+    fn[:, 0, :] = fn[:, 1, :]
+    fn[:, -1, :] = fn[:, -2, :]
+    tempx[:, 1:-1, :] = tempx[:, 1:-1, :] - fn[:, 1:-1, :] + fn[:, :-2, :]
+    # z antidiffusion (:171-239)
+    dz = np.empty_like(W)
+    d1z = np.empty_like(W)
+    dz[:-1] = W[1:] - W[:-1]
+    d1z[:-1] = P[1:] - P[:-1]
+    dz[-1] = dz[-2]
+    d1z[-1] = d1z[-2]
+    s = np.copysign(F(1), dz[1:-1])
+    fn[1:-1] = _limit(s, d1z[:-2], chipx * np.abs(dz[1:-1]), d1z[2:])
+    fn[0] = fn[1]
+    fn[-1] = fn[-2]
+    P[1:-1] = tempx[1:-1] - fn[1:-1] + fn[:-2]
+
+
+def flux_correct(qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, qpx, qpy, qpz,
+                 wrkqrho, wrkqpresx, wrkqpresy, wrkqpresz, wrkqpresxy, wrkqpresxz, wrkqpresyz,
+                 wrkqpx, wrkqpy, wrkqpz,
+                 oldqrho, oldqpresx, oldqpresy, oldqpresz, oldqpresxy, oldqpresxz, oldqpresyz,
+                 oldqpx, oldqpy, oldqpz,
+                 hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, hpx, hpy, hpz,
+                 wrkhrho, wrkhpresx, wrkhpresy, wrkhpresz, wrkhpresxy, wrkhpresxz, wrkhpresyz,
+                 wrkhpx, wrkhpy, wrkhpz,
+                 oldhrho, oldhpresx, oldhpresy, oldhpresz, oldhpresxy, oldhpresxz, oldhpresyz,
+                 oldhpx, oldhpy, oldhpz,
+                 orho, opresx, opresy, opresz, opresxy, opresxz, opresyz, opx, opy, opz,
+                 wrkorho, wrkopresx, wrkopresy, wrkopresz, wrkopresxy, wrkopresxz, wrkopresyz,
+                 wrkopx, wrkopy, wrkopz,
+                 oldorho, oldopresx, oldopresy, oldopresz, oldopresxy, oldopresxz, oldopresyz,
+                 oldopx, oldopy, oldopz,
+                 epres, wrkepres, oldepres, bx, by, bz, wrkbx, wrkby, wrkbz, oldbx, oldby, oldbz,
+                 vvx, vvy, vvz, nx, ny, nz, n_grids, box, difrho, diferg, xspac, isotropic):
+    """code/sub_flux_correct.f90:4-192: fcsmooth on all 34 fields."""
+    a = locals()
+    chi = F(difrho)
+    for sp in "qho":
+        for nm in ("rho", "presx", "presy", "presz", "presxy", "presxz", "presyz", "px", "py", "pz"):
+            fcsmooth(a[sp + nm], a["old" + sp + nm], a["wrk" + sp + nm],
+                     nx, ny, nz, n_grids, box, chi, vvx, vvy, vvz)
+    fcsmooth(epres, oldepres, wrkepres, nx, ny, nz, n_grids, box, chi, vvx, vvy, vvz)
+    chi = F(diferg)
+    for c in "xyz":
+        fcsmooth(a["b" + c], a["oldb" + c], a["wrkb" + c], nx, ny, nz, n_grids, box, chi,
+                 vvx, vvy, vvz)
+
+
+# ---------------------------------------------------------------------------
+# code/sub_bndry_outer.f90:5-731  outer-box faces, edges and corners
+
+# kinds: which sign rule applies on which face
+_RHO, _PRES, _PX, _PY, _PZ, _OFF, _EP, _BX, _BY, _BZ = range(10)
+
+
+def bndry_outer(qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz,
+                hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz,
+                orho, opresx, opresy, opresz, opx, opy, opz,
+                epres, qpresxy, qpresxz, qpresyz, hpresxy, hpresxz, hpresyz,
+                opresxy, opresxz, opresyz, rmassq, rmassh, rmasso,
+                bx, by, bz, bx0, by0, bz0, nx, ny, nz, n_grids,
+                srho, rho_frac, o_conc, spress, spx, spy, spz,
+                sbx_wind, sby_wind, sbz_wind, ti_te, isotropic):
+    b = n_grids - 1
+    rmassq, rmassh, rmasso = _f(rmassq, rmassh, rmasso)
+    srho, rho_frac, o_conc, spress, spx, spy, spz = _f(srho, rho_frac, o_conc, spress, spx, spy, spz)
+    sbx, sby, sbz, ti_te = _f(sbx_wind, sby_wind, sbz_wind, ti_te)
+    ofrac = rho_frac * o_conc
+    frac_h = rmassh / rmassq
+    frac_o = rmasso / rmassq
+    half = F(0.5)
+    pq = half * spress
+    # (field, kind, inflow value or None) -- inflow values (:50-103)
+    fields = [
+        (qrho, _RHO, srho), (qpx, _PX, spx), (qpy, _PY, spy), (qpz, _PZ, spz),
+        (qpresx, _PRES, pq), (qpresy, _PRES, pq), (qpresz, _PRES, pq),
+        (hrho, _RHO, srho * rho_frac * frac_h), (hpx, _PX, spx * rho_frac * frac_h),
+        (hpy, _PY, spy * rho_frac * frac_h), (hpz, _PZ, spz * rho_frac * frac_h),
+        (hpresx, _PRES, half * spress * rho_frac), (hpresy, _PRES, half * spress * rho_frac),
+        (hpresz, _PRES, half * spress * rho_frac),
+        (orho, _RHO, srho * ofrac * frac_o), (opx, _PX, spx * ofrac * frac_o),
+        (opy, _PY, spy * ofrac * frac_o), (opz, _PZ, spz * ofrac * frac_o),
+        (opresx, _PRES, half * spress * ofrac), (opresy, _PRES, half * spress * ofrac),
+        (opresz, _PRES, half * spress * ofrac),
+        (epres, _EP, pq / ti_te),
+        (qpresxy, _OFF, F(0)), (qpresxz, _OFF, F(0)), (qpresyz, _OFF, F(0)),
+        (hpresxy, _OFF, F(0)), (hpresxz, _OFF, F(0)), (hpresyz, _OFF, F(0)),
+        (opresxy, _OFF, F(0)), (opresxz, _OFF, F(0)), (opresyz, _OFF, F(0)),
+        (bx, _BX, None), (by, _BY, None), (bz, _BZ, None),
+    ]
+    two = F(2.)
+    for arr, kind, val in fields:
+        f = arr[b]
+        # inflow face i=1
+        if kind == _BX:
+            f[:, :, 0] = sbx - bx0[b][:, :, 0]
+        elif kind == _BY:
+            f[:, :, 0] = sby - by0[b][:, :, 0]
+        elif kind == _BZ:
+            f[:, :, 0] = sbz - bz0[b][:, :, 0]
+        else:
+            f[:, :, 0] = val
+        # back wall i=nx (:107-154)
+        f[:, :, -1] = np.abs(f[:, :, -2]) if kind == _PX else f[:, :, -2]
+        # y faces, i=2..nx (:158-252)
+        if kind == _PY:
+            f[:, 0, 1:] = -np.abs(f[:, 1, 1:])
+            f[:, -1, 1:] = np.abs(f[:, -2, 1:])
+        else:
+            f[:, 0, 1:] = f[:, 1, 1:]
+            f[:, -1, 1:] = f[:, -2, 1:]
+        # z top, i=2..nx (:256-307)
+        f[-1, :, 1:] = np.abs(f[-2, :, 1:]) if kind == _PZ else f[-2, :, 1:]
+        # z bottom (:311-359)
+        f[0, :, 1:] = -np.abs(f[1, :, 1:]) if kind == _PZ else f[1, :, 1:]
+        # corner lines of the back wall, k=2..nz-1 (:363-466)
+        if kind == _BX:
+            f[1:-1, 0, -1] = f[1:-1, 0, -2]
+            f[1:-1, -1, -1] = f[1:-1, -1, -2]
+        else:
+            f[1:-1, 0, -1] = (f[1:-1, 0, -2] + f[1:-1, 1, -1]) / two
+            f[1:-1, -1, -1] = (f[1:-1, -1, -2] + f[1:-1, -2, -1]) / two
+        # top and bottom lines of the back wall, j=2..ny-1 (:470-566)
+        f[0, 1:-1, -1] = f[1, 1:-1, -1]
+        if kind != _EP:
+            f[-1, 1:-1, -1] = f[-1, 1:-1, -2]
+        # x-directed edge lines, i=2..nx (:570-728)
+        f[0, 0, 1:] = f[1, 0, 1:]
+        f[0, -1, 1:] = f[1, -1, 1:]
+        f[-1, 0, 1:] = f[-2, 0, 1:]
+        f[-1, -1, 1:] = f[-2, -1, 1:]
+
+
+# ---------------------------------------------------------------------------
+# code/sub_bndry_inner.f90:5-268  ionosphere / atmosphere / interior lists
+
+
+def bndry_inner(qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, rmassq,
+                hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz, rmassh,
+                orho, opresx, opresy, opresz, opx, opy, opz, rmasso,
+                epres, bx, by, bz, qpresxy, qpresxz, qpresyz,
+                hpresxy, hpresxz, hpresyz, opresxy, opresxz, opresyz,
+                nx, ny, nz, n_grids, parm_srf, parm_mid, parm_zero,
+                ijsrf, numsrf, ijmid, nummid, ijzero, numzero,
+                mbndry, msrf, mmid, mzero, erho, epress, alpha_e, ti_te, o_conc,
+                re_equiv, r_inner, sbx_wind, spress,
+                grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax,
+                v_rot=0.0, r_rot=0.0):
+    """``v_rot``/``r_rot`` come from the reference's common /rotation/."""
+    b = mbndry - 1
+    re_equiv, v_rot = _f(re_equiv, v_rot)
+    dx = (F(grd_xmax[b]) - F(grd_xmin[b])) / F(nx - 1.)
+    dy = (F(grd_ymax[b]) - F(grd_ymin[b])) / F(ny - 1.)
+    one = F(1.)
+    rhos = (qrho[b], hrho[b], orho[b])
+    pres = ((qpresx[b], qpresy[b], qpresz[b]), (hpresx[b], hpresy[b], hpresz[b]),
+            (opresx[b], opresy[b], opresz[b]))
+    moms = ((qpx[b], qpy[b], qpz[b]), (hpx[b], hpy[b], hpz[b]), (opx[b], opy[b], opz[b]))
+    offs = (qpresxy[b], qpresxz[b], qpresyz[b], hpresxy[b], hpresxz[b], hpresyz[b],
+            opresxy[b], opresxz[b], opresyz[b])
+
+    def coords(lst, n):
+        ii = lst[:n, 0, b] - 1
+        jj = lst[:n, 1, b] - 1
+        kk = lst[:n, 2, b] - 1
+        ay = F(grd_ymin[b]) + dy * jj.astype(F)
+        ax = F(grd_xmin[b]) + dx * ii.astype(F)
+        rx = ax * re_equiv
+        ry = ay * re_equiv
+        rvy = rx * v_rot * one
+        rvx = -ry * v_rot * one
+        return (kk, jj, ii), rvx, rvy
+
+    # surface (:78-140): densities floored at the stored values
+    n = int(numsrf[b])
+    if n:
+        idx, rvx, rvy = coords(ijsrf, n)
+        p = parm_srf[:n, :, b]
+        dens = [np.maximum(p[:, s], rhos[s][idx]) for s in range(3)]
+        for s in range(3):
+            rhos[s][idx] = dens[s]
+        for s in range(3):
+            for c in range(3):
+                pres[s][c][idx] = p[:, 3 + s]
+        epres[b][idx] = p[:, 6]
+        for s in (1, 2, 0):
+            moms[s][0][idx] = dens[s] * rvx
+            moms[s][1][idx] = dens[s] * rvy
+            moms[s][2][idx] = dens[s] * F(0)
+        for o in offs:
+            o[idx] = 0
+    # atmosphere (:143-201) and interior (:204-265)
+    for lst, num, parm, zero_b in ((ijmid, nummid, parm_mid, False),
+                                   (ijzero, numzero, parm_zero, True)):
+        n = int(num[b])
+        if not n:
+            continue
+        idx, rvx, rvy = coords(lst, n)
+        p = parm[:n, :, b]
+        for s in range(3):
+            rhos[s][idx] = p[:, s]
+        for s in range(3):
+            for c in range(3):
+                pres[s][c][idx] = p[:, 3 + s]
+        epres[b][idx] = p[:, 6]
+        for s in (2, 1, 0):
+            r = rhos[s][idx]
+            moms[s][0][idx] = r * rvx
+            moms[s][1][idx] = r * rvy
+            moms[s][2][idx] = r * F(0)
+        for o in offs:
+            o[idx] = 0
+        if zero_b:
+            bx[b][idx] = 0
+            by[b][idx] = 0
+            bz[b][idx] = 0
+
+
+# ---------------------------------------------------------------------------
+# Nested-grid coupling
+
+
+def _axis_weights(gmin_s, gmax_s, gmin_b, gmax_b, n):
+    """Coarse index and weights of each fine index along one axis
+    (code/sub_flanks.f90:24-30, 50-55)."""
+    dl = (F(gmax_b) - F(gmin_b)) / F(n - 1.)
+    sl = (F(gmax_s) - F(gmin_s)) / F(n - 1.)
+    a = F(gmin_s) + sl * np.arange(n, dtype=F)
+    ai = F(1.) + (a - F(gmin_b)) / dl
+    ib = ai.astype(np.int64)              # truncation toward zero
+    d = ai - ib.astype(F)
+    return ib - 1, d, F(1.) - d           # 0-based lower index
+
+
+def flanks(wrkrho, rho, oldrho, nx, ny, nz, n_grids, box, t_new, t_old, t, work,
+           grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax):
+    """code/sub_flanks.f90:5-303: coarse->fine boundary, 2 layers per face,
+    trilinear in space and linear in time; panels in reference order."""
+    mb, ms = box, box - 1          # 0-based coarse / fine
+    t = float(t.v if hasattr(t, "v") else t)
+    t_new = np.asarray(t_new, dtype=np.float64)
+    t_old = np.asarray(t_old, dtype=np.float64)
+    t1 = F((t_new[mb] - t) / (t_new[mb] - t_old[mb]))
+    t2 = F(1.) - t1
+    work[...] = oldrho[mb] * t1 + rho[mb] * t2
+    ib, dxv, ddxv = _axis_weights(grd_xmin[ms], grd_xmax[ms], grd_xmin[mb], grd_xmax[mb], nx)
+    jb, dyv, ddyv = _axis_weights(grd_ymin[ms], grd_ymax[ms], grd_ymin[mb], grd_ymax[mb], ny)
+    kb, dzv, ddzv = _axis_weights(grd_zmin[ms], grd_zmax[ms], grd_zmin[mb], grd_zmax[mb], nz)
+    W = wrkrho[ms]
+
+    def interp(ks, js, is_):
+        K, J, I = np.ix_(ks, js, is_)
+        k0, j0, i0 = kb[K], jb[J], ib[I]
+        dz, ddz = dzv[K], ddzv[K]
+        dy, ddy = dyv[J], ddyv[J]
+        dx, ddx = dxv[I], ddxv[I]
+        w = work
+        return (w[k0, j0, i0] * ddx * ddy * ddz
+                + w[k0 + 1, j0, i0] * ddx * ddy * dz
+                + w[k0, j0 + 1, i0] * ddx * dy * ddz
+                + w[k0 + 1, j0 + 1, i0] * ddx * dy * dz
+                + w[k0, j0, i0 + 1] * dx * ddy * ddz
+                + w[k0 + 1, j0, i0 + 1] * dx * ddy * dz
+                + w[k0, j0 + 1, i0 + 1] * dx * dy * ddz
+                + w[k0 + 1, j0 + 1, i0 + 1] * dx * dy * dz)
+
+    allk, allj, alli = np.arange(nz), np.arange(ny), np.arange(nx)
+
+    # Layer l=0 is the outer layer (frac 0: replaced), l=1 the inner one
+    # (frac 0.5: blended).  Panels are applied in the reference's order --
+    # bottom, top, front, back, left, right -- so edge cells see the same
+    # sequence of blends.
+    def blend(sel_k, sel_j, sel_i, frac):
+        frac = F(frac)
+        frac1 = F(1.) - frac
+        K, J, I = np.ix_(sel_k, sel_j, sel_i)
+        W[K, J, I] = frac * W[K, J, I] + frac1 * interp(sel_k, sel_j, sel_i)
+
+    for l in (1, 0):
+        blend([l], allj, alli, l / 2.)
+    for l in (1, 0):
+        blend([nz - 1 - l], allj, alli, l / 2.)
+    for l in (1, 0):
+        blend(allk, [l], alli, l / 2.)
+    for l in (1, 0):
+        blend(allk, [ny - 1 - l], alli, l / 2.)
+    for l in (1, 0):
+        blend(allk, allj, [l], l / 2.)
+    for l in (1, 0):
+        blend(allk, allj, [nx - 1 - l], l / 2.)
+
+
+def bndry_flanks(*args):
+    """code/sub_bndry_flanks.f90:5-212: ``flanks`` on all 34 fields.
+
+    Positional layout as the reference: 34 targets (wrk), 34 current (q),
+    34 old, then work, nx, ny, nz, n_grids, box, t_old, t_new, t, grid."""
+    tgt, cur, old = args[0:34], args[34:68], args[68:102]
+    work, nx, ny, nz, n_grids, box, t_old, t_new, t = args[102:111]
+    grid = args[111:117]
+    # the reference calls flanks field by field in this same order
+    for w, q, o in zip(tgt, cur, old):
+        flanks(w, q, o, nx, ny, nz, n_grids, box, t_new, t_old, t, work, *grid)
+
+
+def corer(rho, nx, ny, nz, n_grids, box, grd_xmin, grd_xmax, grd_ymin, grd_ymax,
+          grd_zmin, grd_zmax):
+    """code/sub_corer.f90:5-55: fine->coarse core restriction."""
+    mb, ms = box, box - 1
+    sx = (F(grd_xmax[ms]) - F(grd_xmin[ms])) / F(nx - 1.)
+    dx = (F(grd_xmax[mb]) - F(grd_xmin[mb])) / F(nx - 1.)
+    dy = (F(grd_ymax[mb]) - F(grd_ymin[mb])) / F(ny - 1.)
+    dz = (F(grd_zmax[mb]) - F(grd_zmin[mb])) / F(nz - 1.)
+    nskip = int(dx / sx)
+    nbx1 = int(F(1.) + (F(grd_xmin[ms]) - F(grd_xmin[mb])) / dx + F(.5))
+    nby1 = int(F(1.) + (F(grd_ymin[ms]) - F(grd_ymin[mb])) / dy + F(.5))
+    nbz1 = int(F(1.) + (F(grd_zmin[ms]) - F(grd_zmin[mb])) / dz + F(.5))
+    ks = np.arange(1 + nskip, nz - nskip + 1, nskip)   # 1-based
+    js = np.arange(1 + nskip, ny - nskip + 1, nskip)
+    is_ = np.arange(1 + nskip, nx - nskip + 1, nskip)
+    kb = nbz1 + (ks - 1) // nskip
+    jb = nby1 + (js - 1) // nskip
+    ib = nbx1 + (is_ - 1) // nskip
+    f = rho[ms]
+    K, J, I = np.ix_(ks - 1, js - 1, is_ - 1)
+    val = (F(6.) * f[K, J, I] + f[K, J, I + 1] + f[K, J, I - 1] + f[K, J + 1, I]
+           + f[K, J - 1, I] + f[K + 1, J, I] + f[K - 1, J, I]) / F(12.)
+    rho[mb][np.ix_(kb - 1, jb - 1, ib - 1)] = val
+
+
+_CORER_ORDER = None
+
+
+def bndry_corer(qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz,
+                hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz,
+                orho, opresx, opresy, opresz, opx, opy, opz,
+                epres, bx, by, bz, qpresxy, qpresxz, qpresyz,
+                hpresxy, hpresxz, hpresyz, opresxy, opresxz, opresyz,
+                nx, ny, nz, n_grids, box, *grid):
+    """code/sub_bndry_corer.f90:5-114: ``corer`` on all 34 fields."""
+    for f in (qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, qpx, qpy, qpz,
+              hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, hpx, hpy, hpz,
+              orho, opresx, opresy, opresz, opresxy, opresxz, opresyz, opx, opy, opz,
+              epres, bx, by, bz):
+        corer(f, nx, ny, nz, n_grids, box, *grid)
+
+
+def flanks_synced(rho, nx, ny, nz, n_grids, box, grd_xmin, grd_xmax, grd_ymin,
+                  grd_ymax, grd_zmin, grd_zmax):
+    """code/sub_flanks_synced.f90:5-141: single-layer coarse->fine faces,
+    bilinear; top/bottom, front/back, left/right in order."""
+    mb, ms = box, box - 1
+    delx = (F(grd_xmax[mb]) - F(grd_xmin[mb])) / F(nx - 1.)
+    dely = (F(grd_ymax[mb]) - F(grd_ymin[mb])) / F(ny - 1.)
+    delz = (F(grd_zmax[mb]) - F(grd_zmin[mb])) / F(nz - 1.)
+    kb_top = int(F(1.) + (F(grd_zmax[ms]) - F(grd_zmin[mb])) / delz) - 1
+    kb_bot = int(F(1.) + (F(grd_zmin[ms]) - F(grd_zmin[mb])) / delz) - 1
+    jb_top = int(F(1.) + (F(grd_ymax[ms]) - F(grd_ymin[mb])) / dely) - 1
+    jb_bot = int(F(1.) + (F(grd_ymin[ms]) - F(grd_ymin[mb])) / dely) - 1
+    ib_top = int(F(1.) + (F(grd_xmax[ms]) - F(grd_xmin[mb])) / delx) - 1
+    ib_bot = int(F(1.) + (F(grd_xmin[ms]) - F(grd_xmin[mb])) / delx) - 1
+    ib, dx, ddx = _axis_weights(grd_xmin[ms], grd_xmax[ms], grd_xmin[mb], grd_xmax[mb], nx)
+    jb, dy, ddy = _axis_weights(grd_ymin[ms], grd_ymax[ms], grd_ymin[mb], grd_ymax[mb], ny)
+    kb, dz, ddz = _axis_weights(grd_zmin[ms], grd_zmax[ms], grd_zmin[mb], grd_zmax[mb], nz)
+    S, B = rho[ms], rho[mb]
+    # top and bottom panels (:43-72): arrays over (js, is)
+    J, I = np.ix_(jb, ib)
+    DY, DDY = dy[:, None], ddy[:, None]
+    DX, DDX = dx[None, :], ddx[None, :]
+    for ks, kbb in ((-1, kb_top), (0, kb_bot)):
+        S[ks] = (B[kbb, J, I] * DDX * DDY + B[kbb, J + 1, I] * DDX * DY
+                 + B[kbb, J, I + 1] * DX * DDY + B[kbb, J + 1, I + 1] * DX * DY)
+    # front and back panels (:76-105): arrays over (ks, is)
+    K, I = np.ix_(kb, ib)
+    DZ, DDZ = dz[:, None], ddz[:, None]
+    for js, jbb in ((-1, jb_top), (0, jb_bot)):
+        S[:, js, :] = (B[K, jbb, I] * DDX * DDZ + B[K + 1, jbb, I] * DDX * DZ
+                       + B[K, jbb, I + 1] * DX * DDZ + B[K + 1, jbb, I + 1] * DX * DZ)
+    # left and right panels (:109-138): arrays over (ks, js)
+    K, J = np.ix_(kb, jb)
+    DY2, DDY2 = dy[None, :], ddy[None, :]
+    for is_, ibb in ((-1, ib_top), (0, ib_bot)):
+        S[:, :, is_] = (B[K, J, ibb] * DDY2 * DDZ + B[K, J + 1, ibb] * DY2 * DDZ
+                        + B[K + 1, J, ibb] * DDY2 * DZ + B[K + 1, J + 1, ibb] * DY2 * DZ)
+
+
+# ---------------------------------------------------------------------------
+# divB clean-up: code/sub_banded_tpose.f90:6-33, sub_sparse_solver.f90:19-159,
+# sub_divB_correct.f90:25-175
+
+
+def seq_sum(x):
+    """Sequential single-precision sum, as the reference's DO loops add."""
+    return np.add.accumulate(x.astype(F), dtype=F)[-1] if x.size else F(0)
+
+
+def banded_tpose(nbands, npts, divB_band_pos, banded_matrix, x_vec, tp_result):
+    """y_i = sum_b M(b, i+pos_b) x(i+pos_b), accumulated in band order."""
+    M = banded_matrix.reshape(npts, nbands)      # Fortran M(nbands, npts)
+    y = np.zeros(npts, dtype=F)
+    idx = np.arange(npts)
+    for band in range(nbands):
+        ii = idx + int(divB_band_pos[band])
+        ok = (ii >= 0) & (ii < npts)
+        t = np.zeros(npts, dtype=F)
+        t[ok] = M[ii[ok], band] * x_vec[ii[ok]]
+        y = y + t
+    tp_result[...] = y
+
+
+_REDOS = [0]   # the reference's SAVEd `redos` (:45)
+
+
+def sparse_solver(nbands, npts, divB_band_pos, divB_errlim, recurse, redo_lim,
+                  divB, soln_vec, residual2, banded_matrix):
+    """Numerical-Recipes CGNR with restart recursion, all single precision."""
+    errlim2 = F(npts) * (F(divB_errlim) * F(divB_errlim))
+    _REDOS[0] = int(redo_lim) if recurse else 0
+    _REDOS[0] += 1
+    xi = np.empty(npts, F)
+    xj = np.empty(npts, F)
+    g = np.empty(npts, F)
+    banded_tpose(nbands, npts, divB_band_pos, banded_matrix, soln_vec, xi)
+    sum_b = seq_sum(divB * divB)
+    xi[...] = xi - divB
+    resd = seq_sum(xi * xi)
+    banded_tpose(nbands, npts, divB_band_pos, banded_matrix, xi, g)
+    g[...] = -g
+    h = g.copy()
+    for _ in range(10 * npts):
+        banded_tpose(nbands, npts, divB_band_pos, banded_matrix, h, xi)
+        anum = seq_sum(g * h)
+        aden = seq_sum(xi * xi)
+        anum = anum / aden
+        xi[...] = soln_vec
+        soln_vec[...] = soln_vec + anum * h
+        banded_tpose(nbands, npts, divB_band_pos, banded_matrix, soln_vec, xj)
+        xj[...] = xj - divB
+        r2 = seq_sum(xj * xj)
+        residual2[0] = r2
+        if r2 == resd or r2 <= sum_b * errlim2:
+            return
+        if r2 > resd:
+            soln_vec[...] = xi
+            if _REDOS[0] >= 3:
+                return
+            sparse_solver(nbands, npts, divB_band_pos, divB_errlim, True, _REDOS[0],
+                          divB, soln_vec, residual2, banded_matrix)
+        resd = residual2[0]
+        banded_tpose(nbands, npts, divB_band_pos, banded_matrix, xj, xi)
+        gg = seq_sum(g * g)
+        dgg = seq_sum((xi + g) * xi)
+        if gg == 0:
+            return
+        gam = dgg / gg
+        g[...] = -xi
+        h[...] = g + gam * h
+
+
+def divb_matrix(nx, ny, nz):
+    """The 7-band matrix of code/sub_divB_correct.f90:85-123, as a
+    C-order (npts, 7) array (= Fortran banded_matrix(7, npts))."""
+    npts = nx * ny * nz
+    M = np.zeros((npts, 7), dtype=F)
+    M[:, 3] = -6.
+    n = np.arange(npts)
+    i = n % nx
+    jp = n % (nx * ny)
+    M[i >= 1, 2] = 1.                 # upper1: i = 2..nx
+    M[i <= nx - 2, 4] = 1.            # lower1: i = 1..nx-1
+    M[jp >= nx, 1] = 1.               # upper2: in-plane index nx+1..nx*ny
+    M[jp < nx * ny - nx, 5] = 1.      # lower2
+    M[n >= nx * ny, 0] = 1.           # upper3
+    M[n < npts - nx * ny, 6] = 1.     # lower3
+    return M
+
+
+def divB_correct(nx, ny, nz, n_grids, box, nbands, divB_band_pos, xspac, divB_errlim,
+                 bx, by, bz, soln_init=None):
+    """The reference leaves ``soln_vec`` (an automatic array) uninitialised
+    before the solve; the oracle starts from ``soln_init`` (zeros by
+    default), the only defined choice."""
+    b = box - 1
+    rx = F(xspac[b])
+    npts = nx * ny * nz
+    M = divb_matrix(nx, ny, nz)
+    divB = np.zeros((nz, ny, nx), F)
+    X, Y, Z = bx[b], by[b], bz[b]
+    divB[C] = ((X[XP] - X[XM]) + (Y[YP] - Y[YM]) + (Z[ZP] - Z[ZM])) / (F(2.) * rx)
+    soln = np.zeros(npts, F) if soln_init is None else soln_init.astype(F).copy()
+    res = np.zeros(1, F)
+    sparse_solver(nbands, npts, divB_band_pos, divB_errlim, False, 0, divB.reshape(-1),
+                  soln, res, M)
+    poten = (rx * rx) * soln.reshape(nz, ny, nx)
+    X[C] = X[C] - (poten[XP] - poten[XM]) / (F(2.) * rx)
+    Y[C] = Y[C] - (poten[YP] - poten[YM]) / (F(2.) * rx)
+    Z[C] = Z[C] - (poten[ZP] - poten[ZM]) / (F(2.) * rx)
+    return soln, res[0]
+
+
+# ---------------------------------------------------------------------------
+# code/cft_trilin_interp.f90:10-66
+
+
+def trilin_interp(sxyz, gridpts, qty, scdata):
+    """``gridpts`` is Fortran gridpts(3,2) -> numpy (2,3); ``qty(2,2,2)``
+    -> numpy (2,2,2) indexed [z][y][x]."""
+    s = np.asarray(sxyz, F)
+    x0, y0, z0 = gridpts[0]
+    x1, y1, z1 = gridpts[1]
+    xd = (s[0] - x0) / (x1 - x0)
+    yd = (s[1] - y0) / (y1 - y0)
+    zd = (s[2] - z0) / (z1 - z0)
+    q = qty
+    one = F(1)
+    e00 = q[0, 0, 0] * (one - xd) + q[0, 0, 1] * xd
+    e01 = q[1, 0, 0] * (one - xd) + q[1, 0, 1] * xd
+    e10 = q[0, 1, 0] * (one - xd) + q[0, 1, 1] * xd
+    e11 = q[1, 1, 0] * (one - xd) + q[1, 1, 1] * xd
+    f0 = e00 * (one - yd) + e10 * yd
+    f1 = e01 * (one - yd) + e11 * yd
+    scdata[0] = f0 * (one - zd) + f1 * zd

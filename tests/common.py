@@ -1,0 +1,240 @@
+"""Shared fixtures for the parity tests: seeded synthetic inputs laid out
+exactly as the reference's Fortran arrays, and the reference argument
+lists of each hot-path subroutine, so one list can be handed to the
+reference build, to the oracle and to the HIP C-ABI."""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+
+F = np.float32
+SPECIES = "qho"
+SP_FIELDS = ("rho", "presx", "presy", "presz", "px", "py", "pz",
+             "presxy", "presxz", "presyz")
+# the 34 per-cell state fields
+FIELDS = [s + f for s in SPECIES for f in SP_FIELDS] + ["epres", "bx", "by", "bz"]
+
+
+class World:
+    """Named arrays + scalars of one synthetic run."""
+
+    def __init__(self, nx, ny, nz, ngrd, seed=0, nested=None, lists=True):
+        self.nx, self.ny, self.nz, self.ngrd = nx, ny, nz, ngrd
+        self.rng = np.random.default_rng(seed)
+        self.a = {}
+        self.s = dict(nx=nx, ny=ny, nz=nz, n_grids=ngrd, rmassq=F(1.), rmassh=F(1.),
+                      rmasso=F(16.), gamma=F(1.6666), gamma1=F(0.6666), reynolds=F(256.),
+                      resist=F(64.), grav=F(0.05), re_equiv=F(0.2), isotropic=False,
+                      smallbit=F(1e-5), aniso_limit=F(0.9995), qvlim=F(1.), hvlim=F(1.4),
+                      ovlim=F(2.), qclim=F(1.), hclim=F(1.), oclim=F(4.), cslim=F(1.),
+                      chirho=F(2.), chipxyz=F(2.), chierg=F(0.), difrho=F(0.008),
+                      diferg=F(0.004), o_conc=F(0.125), rho_frac=F(0.001), ti_te=F(5.),
+                      mbndry=1)
+        # grid extents: box b spacing 2^(b-1), centred on the origin
+        L = (nx - 1) // 2
+        xs = np.array([2.0 ** b for b in range(ngrd)], dtype=F)
+        self.a["xspac"] = xs
+        self.a["grd_xmin"] = (-(nx - 1) / 2 * xs).astype(F)
+        self.a["grd_xmax"] = ((nx - 1) / 2 * xs).astype(F)
+        self.a["grd_ymin"] = (-(ny - 1) / 2 * xs).astype(F)
+        self.a["grd_ymax"] = ((ny - 1) / 2 * xs).astype(F)
+        self.a["grd_zmin"] = (-(nz - 1) / 2 * xs).astype(F)
+        self.a["grd_zmax"] = ((nz - 1) / 2 * xs).astype(F)
+        del L
+        shp4 = (ngrd, nz, ny, nx)
+        r = self.rng
+        for pre in ("", "old", "wrk"):
+            for s in SPECIES:
+                self.a[pre + s + "rho"] = r.uniform(0.5, 2.0, shp4).astype(F)
+                for c in "xyz":
+                    self.a[pre + s + "p" + c] = r.uniform(-0.3, 0.3, shp4).astype(F)
+                    self.a[pre + s + "pres" + c] = r.uniform(0.1, 1.0, shp4).astype(F)
+                for c in ("xy", "xz", "yz"):
+                    self.a[pre + s + "pres" + c] = r.uniform(-0.05, 0.05, shp4).astype(F)
+            self.a[pre + "epres"] = r.uniform(0.1, 0.5, shp4).astype(F)
+            for c in "xyz":
+                self.a[pre + "b" + c] = r.uniform(-0.3, 0.3, shp4).astype(F)
+        for c in "xyz":
+            self.a["b" + c + "0"] = r.uniform(-1.0, 1.0, shp4).astype(F)
+        shp3 = (nz, ny, nx)
+        for nm in ("curx", "cury", "curz", "evx", "evy", "evz", "tvx", "tvy", "tvz",
+                   "efldx", "efldy", "efldz", "bsx", "bsy", "bsz", "btot",
+                   "vvx", "vvy", "vvz", "work"):
+            self.a[nm] = r.uniform(-1, 1, shp3).astype(F)
+        for nm in ("bsx", "bsy", "bsz", "evx", "evy", "evz", "tvx", "tvy", "tvz",
+                   "efldx", "efldy", "efldz"):
+            self.a[nm] = r.uniform(-0.5, 0.5, shp3).astype(F)
+        self.a["btot"] = np.maximum(np.sqrt(self.a["bsx"] ** 2 + self.a["bsy"] ** 2
+                                            + self.a["bsz"] ** 2), F(1e-5)).astype(F)
+        self.a["rst"] = r.uniform(0.0, 0.01, (1, nz, ny, nx)).astype(F)
+        if lists:
+            self._lists()
+        # one-element outputs
+        for nm in ("pxmax", "pymax", "pzmax", "csmax", "alfmax", "fastest",
+                   "residual2", "scdata"):
+            self.a[nm] = np.zeros(1, F)
+
+    def _lists(self):
+        """Inner-boundary index lists on box 1: spherical shells about the
+        box centre (the layout code/3d_main.f90:1383-1455 builds)."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        r = np.sqrt((i - (nx - 1) / 2) ** 2 + (j - (ny - 1) / 2) ** 2 + (k - (nz - 1) / 2) ** 2)
+        rin = min(nx, ny, nz) / 4.0
+        zero = r < rin - 1.5
+        mid = (r >= rin - 1.5) & (r < rin - 0.5)
+        srf = (r >= rin - 0.5) & (r <= rin + 0.6)
+        for nm, m in (("srf", srf), ("mid", mid), ("zero", zero)):
+            kk, jj, ii = np.nonzero(m)
+            n = len(ii)
+            lst = np.zeros((max(n, 1), 3, 1), dtype=np.int32)
+            lst[:n, 0, 0] = ii + 1
+            lst[:n, 1, 0] = jj + 1
+            lst[:n, 2, 0] = kk + 1
+            self.a["ij" + nm] = lst
+            self.a["num" + nm] = np.array([n], dtype=np.int32)
+            self.s["m" + nm] = max(n, 1)
+            parm = np.zeros((max(n, 1), 7, 1), dtype=F)
+            parm[:n, 0:3, 0] = self.rng.uniform(0.5, 2.0, (n, 3))
+            parm[:n, 3:7, 0] = self.rng.uniform(0.1, 1.0, (n, 4))
+            self.a["parm_" + nm] = parm
+
+    def copy(self):
+        w = copy.copy(self)
+        w.a = {k: v.copy() for k, v in self.a.items()}
+        w.s = dict(self.s)
+        return w
+
+    def args(self, names):
+        out = []
+        for n in names:
+            if isinstance(n, str):
+                if n in self.a:
+                    out.append(self.a[n])
+                elif n in self.s:
+                    out.append(self.s[n])
+                else:
+                    raise KeyError(n)
+            else:
+                out.append(n)
+        return out
+
+
+GRID = ["grd_xmin", "grd_xmax", "grd_ymin", "grd_ymax", "grd_zmin", "grd_zmax"]
+
+
+def push_ion_names(out, old, wrk, sp, rmass):
+    """Argument names of code/sub_push_ion.f90:4-17."""
+    def g(pre, names):
+        return [pre + sp + n for n in names]
+    return (g(out, ["rho", "presx", "presy", "presz", "px", "py", "pz"])
+            + g(old, ["rho", "presx", "presy", "presz", "px", "py", "pz"])
+            + g(wrk, ["rho", "presx", "presy", "presz", "px", "py", "pz"])
+            + g(out, ["presxy", "presxz", "presyz"])
+            + g(old, ["presxy", "presxz", "presyz"])
+            + g(wrk, ["presxy", "presxz", "presyz"])
+            + ["bsx", "bsy", "bsz", "btot", "efldx", "efldy", "efldz", rmass,
+               "vvx", "vvy", "vvz", "tvx", "tvy", "tvz", "gamma", "gamma1",
+               "nx", "ny", "nz", "n_grids", "box", "delt", "grav", "re_equiv", "reynolds"]
+            + GRID + ["isotropic"])
+
+
+def set_rho_names(pre):
+    """code/sub_set_rho.f90:5-11."""
+    n = []
+    for s, m in zip(SPECIES, ("rmassq", "rmassh", "rmasso")):
+        n += [pre + s + f for f in ("rho", "presx", "presy", "presz", "presxy", "presxz",
+                                    "presyz")] + [m]
+    return n + [pre + "epres", "nx", "ny", "nz", "n_grids", "box", "o_conc"]
+
+
+def set_speed_names(pre):
+    """code/sub_set_speed_agrd.f90:6-22."""
+    n = ["nx", "ny", "nz", "n_grids", "box", "rmassq", "rmassh", "rmasso", "isotropic",
+         "smallbit", "aniso_limit", "vlim", "qvlim", "hvlim", "ovlim", "qclim", "hclim",
+         "oclim", "cslim", pre + "bx", pre + "by", pre + "bz", "bx0", "by0", "bz0"]
+    for s in SPECIES:
+        n += [pre + s + f for f in ("rho", "px", "py", "pz", "presx", "presy", "presz",
+                                    "presxy", "presxz", "presyz")]
+    return n + [pre + "epres", "btot", "pxmax", "pymax", "pzmax", "csmax", "alfmax",
+                "fastest"]
+
+
+def bndry_outer_names(pre):
+    """code/sub_bndry_outer.f90:5-16."""
+    n = []
+    for s in SPECIES:
+        n += [pre + s + f for f in ("rho", "presx", "presy", "presz", "px", "py", "pz")]
+    n += [pre + "epres"]
+    for s in SPECIES:
+        n += [pre + s + f for f in ("presxy", "presxz", "presyz")]
+    n += ["rmassq", "rmassh", "rmasso", pre + "bx", pre + "by", pre + "bz",
+          "bx0", "by0", "bz0", "nx", "ny", "nz", "n_grids",
+          "srho", "rho_frac", "o_conc", "spress", "spx", "spy", "spz",
+          "sbx_wind", "sby_wind", "sbz_wind", "ti_te", "isotropic"]
+    return n
+
+
+def bndry_inner_names(pre):
+    """code/sub_bndry_inner.f90:5-19."""
+    n = []
+    for s, m in zip(SPECIES, ("rmassq", "rmassh", "rmasso")):
+        n += [pre + s + f for f in ("rho", "presx", "presy", "presz", "px", "py", "pz")] + [m]
+    n += [pre + "epres", pre + "bx", pre + "by", pre + "bz"]
+    for s in SPECIES:
+        n += [pre + s + f for f in ("presxy", "presxz", "presyz")]
+    n += ["nx", "ny", "nz", "n_grids", "parm_srf", "parm_mid", "parm_zero",
+          "ijsrf", "numsrf", "ijmid", "nummid", "ijzero", "numzero",
+          "mbndry", "msrf", "mmid", "mzero", "erho", "epress", "alpha_e", "ti_te",
+          "o_conc", "re_equiv", "r_inner", "sbx_wind", "spress"] + GRID
+    return n
+
+
+def lap_plasma_names(sp, src, dst):
+    """code/sub_lap_plasma.f90:4-10 (v in vvx..vvz)."""
+    f = ("rho", "px", "py", "pz", "presx", "presy", "presz", "presxy", "presxz", "presyz")
+    return ([src + sp + x for x in f] + [dst + sp + x for x in f]
+            + ["vvx", "vvy", "vvz", "nx", "ny", "nz", "n_grids", "box",
+               "chirho", "chipxyz", "chierg", "delt", "isotropic"])
+
+
+def flux_correct_names():
+    """code/sub_flux_correct.f90:4-35."""
+    f = ("rho", "presx", "presy", "presz", "presxy", "presxz", "presyz", "px", "py", "pz")
+    n = []
+    for s in SPECIES:
+        n += [s + x for x in f] + ["wrk" + s + x for x in f] + ["old" + s + x for x in f]
+    n += ["epres", "wrkepres", "oldepres", "bx", "by", "bz", "wrkbx", "wrkby", "wrkbz",
+          "oldbx", "oldby", "oldbz", "vvx", "vvy", "vvz", "nx", "ny", "nz", "n_grids", "box",
+          "difrho", "diferg", "xspac", "isotropic"]
+    return n
+
+
+def flanks_names(tgt, cur, old):
+    """code/sub_bndry_flanks.f90:5-35 argument layout."""
+    order = []
+    for s in SPECIES:
+        order += [s + x for x in ("rho", "px", "py", "pz", "presx", "presy", "presz",
+                                  "presxy", "presxz", "presyz")]
+    order += ["epres", "bx", "by", "bz"]
+    return ([tgt + x for x in order] + [cur + x for x in order] + [old + x for x in order]
+            + ["work", "nx", "ny", "nz", "n_grids", "box", "t_old", "t_new", "t"] + GRID)
+
+
+def corer_names(pre):
+    """code/sub_bndry_corer.f90:5-14."""
+    n = []
+    for s in SPECIES:
+        n += [pre + s + f for f in ("rho", "presx", "presy", "presz", "px", "py", "pz")]
+    n += [pre + "epres", pre + "bx", pre + "by", pre + "bz"]
+    for s in SPECIES:
+        n += [pre + s + f for f in ("presxy", "presxz", "presyz")]
+    return n + ["nx", "ny", "nz", "n_grids", "box"] + GRID
+
+
+def max_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-30)
+    return float(np.abs(a - b).max() / scale)

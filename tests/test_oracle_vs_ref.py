@@ -1,0 +1,251 @@
+"""Pins the NumPy oracle to the reference's own Fortran, compiled unchanged
+(serial, -ffp-contract=off) into oracle/_ref: every hot-path subroutine is
+run on the same seeded inputs by both, and every array they can touch must
+agree BIT FOR BIT.  Skipped where the reference build is absent (the GPU
+box); the golden fixtures in tests/golden carry the same pinning there."""
+import numpy as np
+import pytest
+
+from oracle import ref as R
+from oracle import restate as O
+from common import (FIELDS, World, push_ion_names, set_rho_names, set_speed_names,
+                    bndry_outer_names, bndry_inner_names, lap_plasma_names,
+                    flux_correct_names, flanks_names, corer_names, GRID)
+
+pytestmark = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = R.RefLib(omp=False)
+    return _LIB
+
+
+def run_both(w, name, names, pyname=None, **kw):
+    wr, wo = w.copy(), w.copy()
+    lib().call(name, *wr.args(names))
+    getattr(O, pyname or name)(*wo.args(names), **kw)
+    return wr, wo
+
+
+def assert_changed(w0, w1, keys):
+    """Guards against a vacuous pass: the call must have written these."""
+    for k in keys:
+        assert not np.array_equal(w0.a[k], w1.a[k]), f"{k} unchanged"
+
+
+def assert_same(wr, wo, keys=None):
+    bad = []
+    for k in (keys or wr.a.keys()):
+        a, b = wr.a[k], wo.a[k]
+        if not np.array_equal(a, b, equal_nan=True):
+            d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+            bad.append((k, float(d.max()), int((d > 0).sum())))
+    assert not bad, f"oracle differs from reference: {bad[:8]}"
+
+
+@pytest.fixture(params=[(13, 11, 9), (9, 12, 7)])
+def world(request):
+    nx, ny, nz = request.param
+    w = World(nx, ny, nz, 2, seed=sum(request.param))
+    w.s.update(box=2, delt=np.float32(0.037), vlim=np.float32(1.2), rx=np.float32(2.),
+               ry=np.float32(2.), rz=np.float32(2.))
+    return w
+
+
+def test_calcur(world):
+    names = ["bx", "by", "bz", "nx", "ny", "nz", "n_grids", "box", "curx", "cury", "curz",
+             "rx", "ry", "rz"]
+    assert_same(*run_both(world, "calcur", names))
+
+
+def test_fnd_evel(world):
+    names = ["qpx", "qpy", "qpz", "qrho", "hpx", "hpy", "hpz", "hrho", "opx", "opy", "opz",
+             "orho", "curx", "cury", "curz", "evx", "evy", "evz", "tvx", "tvy", "tvz",
+             "nx", "ny", "nz", "n_grids", "box", "rmassq", "rmassh", "rmasso", "reynolds"]
+    assert_same(*run_both(world, "fnd_evel", names))
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_BandE(world, box):
+    world.s["box"] = box
+    names = ["efldx", "efldy", "efldz", "bsx", "bsy", "bsz", "curx", "cury", "curz",
+             "evx", "evy", "evz", "btot", "epres", "qrho", "hrho", "orho", "rst", "resist",
+             "reynolds", "nx", "ny", "nz", "n_grids", "box", "rmassq", "rmassh", "rmasso",
+             "ijmid", "nummid", "ijzero", "mbndry", "numzero", "mmid", "mzero",
+             "rx", "ry", "rz"]
+    assert_same(*run_both(world, "BandE", names))
+
+
+def test_push_elec(world):
+    names = ["wrkepres", "oldepres", "epres", "evx", "evy", "evz", "gamma", "gamma1",
+             "nx", "ny", "nz", "n_grids", "box", "delt", "rx", "ry", "rz"]
+    assert_same(*run_both(world, "push_elec", names))
+
+
+def test_push_bfld(world):
+    names = ["bx", "by", "bz", "oldbx", "oldby", "oldbz", "efldx", "efldy", "efldz",
+             "nx", "ny", "nz", "n_grids", "box", "delt", "rx", "ry", "rz"]
+    assert_same(*run_both(world, "push_bfld", names))
+
+
+@pytest.mark.parametrize("sp,rm", [("q", "rmassq"), ("o", "rmasso")])
+def test_push_ion(world, sp, rm):
+    # strong field in part of the box so both skin-factor branches run
+    world.a["bsx"][:, :, : world.nx // 2] *= 40
+    world.a["btot"][...] = np.sqrt(world.a["bsx"] ** 2 + world.a["bsy"] ** 2
+                                   + world.a["bsz"] ** 2)
+    assert_same(*run_both(world, "push_ion", push_ion_names("", "old", "wrk", sp, rm)))
+
+
+def test_set_rho(world):
+    r = world.rng
+    for f in ("qrho", "hrho", "orho"):
+        m = r.random(world.a[f].shape) < 0.1
+        world.a[f][m] = r.uniform(-1e-4, 1e-4, m.sum())
+    for f in ("qpresx", "hpresy", "opresz", "epres"):
+        m = r.random(world.a[f].shape) < 0.1
+        world.a[f][m] *= -1
+    wr, wo = run_both(world, "set_rho", set_rho_names(""))
+    assert_same(wr, wo)
+    assert_changed(world, wr, ["qrho", "hpresx", "opresxy", "epres"])
+
+
+def test_set_speed_agrd(world):
+    r = world.rng
+    for f in ("qpx", "hpy", "opz"):
+        m = r.random(world.a[f].shape) < 0.15
+        world.a[f][m] *= 20
+    m = r.random(world.a["epres"].shape) < 0.2
+    world.a["epres"][m] *= 30
+    world.s["vlim"] = np.float32(0.6 * 2)
+    wr, wo = run_both(world, "set_speed_agrd", set_speed_names(""))
+    assert_same(wr, wo)
+    assert_changed(world, wr, ["qrho", "hpresxy", "opz", "epres"])
+    assert wr.a["fastest"][0] > 0
+
+
+@pytest.mark.parametrize("sp", ["q", "h"])
+def test_lap_plasma(world, sp):
+    assert_same(*run_both(world, "lap_plasma", lap_plasma_names(sp, "", "wrk")))
+
+
+def test_lap_elec_bfld(world):
+    names = ["epres", "wrkepres", "vvx", "vvy", "vvz", "nx", "ny", "nz", "n_grids", "box",
+             "chirho", "chipxyz", "chierg", "delt"]
+    assert_same(*run_both(world, "lap_elec", names))
+    names = ["bx", "by", "bz", "wrkbx", "wrkby", "wrkbz", "vvx", "vvy", "vvz", "nx", "ny",
+             "nz", "n_grids", "box", "chirho", "chipxyz", "chierg", "delt"]
+    world.s["chierg"] = np.float32(0.7)
+    assert_same(*run_both(world, "lap_bfld", names))
+
+
+def test_fnd_vtot(world):
+    names = ["qpx", "qpy", "qpz", "qrho", "hpx", "hpy", "hpz", "hrho", "opx", "opy", "opz",
+             "orho", "vvx", "vvy", "vvz", "nx", "ny", "nz", "n_grids", "box",
+             "rmassq", "rmassh", "rmasso"]
+    assert_same(*run_both(world, "fnd_vtot", names))
+
+
+def test_fcsmooth(world):
+    names = ["qrho", "oldqrho", "wrkqrho", "nx", "ny", "nz", "n_grids", "box",
+             np.float32(0.3), "vvx", "vvy", "vvz"]
+    assert_same(*run_both(world, "fcsmooth", names))
+
+
+def test_flux_correct(world):
+    assert_same(*run_both(world, "flux_correct", flux_correct_names()))
+
+
+def test_bndry_outer(world):
+    world.s.update(srho=np.float32(25.), spress=np.float32(0.02395), spx=np.float32(10.),
+                   spy=np.float32(0.), spz=np.float32(-0.1), sbx_wind=np.float32(0.),
+                   sby_wind=np.float32(0.01), sbz_wind=np.float32(-0.244))
+    assert_same(*run_both(world, "bndry_outer", bndry_outer_names("")))
+
+
+def test_bndry_inner(world):
+    world.s.update(erho=np.float32(1.), epress=np.float32(1.), alpha_e=np.float32(6.),
+                   r_inner=np.float32(3.), sbx_wind=np.float32(0.), spress=np.float32(0.02))
+    rot = lib().rotation()
+    rot.v_rot, rot.r_rot = 0.3, 2.0
+    assert_same(*run_both(world, "bndry_inner", bndry_inner_names(""), v_rot=0.3, r_rot=2.0))
+
+
+@pytest.fixture
+def nested():
+    w = World(13, 13, 7, 3, seed=7, lists=False)
+    w.a["t_old"] = np.array([0.0, 0.1, 0.1], dtype=np.float64)
+    w.a["t_new"] = np.array([0.05, 0.3, 0.5], dtype=np.float64)
+    w.s["t"] = R.F64(0.17)
+    return w
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_bndry_flanks(nested, box):
+    nested.s["box"] = box
+    assert_same(*run_both(nested, "bndry_flanks", flanks_names("wrk", "", "old")))
+
+
+def test_bndry_flanks_aliased(nested):
+    """The reference's 4th call passes the state as target and source
+    (code/3d_main.f90:2945-2961)."""
+    nested.s["box"] = 1
+    assert_same(*run_both(nested, "bndry_flanks", flanks_names("", "", "old")))
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_bndry_corer(nested, box):
+    nested.s["box"] = box
+    assert_same(*run_both(nested, "bndry_corer", corer_names("")))
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_flanks_synced(nested, box):
+    nested.s["box"] = box
+    assert_same(*run_both(nested, "flanks_synced",
+                          ["bx", "nx", "ny", "nz", "n_grids", "box"] + GRID))
+
+
+def test_banded_tpose():
+    nx, ny, nz = 7, 6, 5
+    npts = nx * ny * nz
+    M = O.divb_matrix(nx, ny, nz)
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    x = np.random.default_rng(3).uniform(-1, 1, npts).astype(np.float32)
+    y1 = np.zeros(npts, np.float32)
+    y2 = np.zeros(npts, np.float32)
+    lib().call("banded_tpose", 7, npts, pos, M, x, y1)
+    O.banded_tpose(7, npts, pos, M, x, y2)
+    assert np.array_equal(y1, y2)
+
+
+@pytest.mark.parametrize("errlim", [1e-4, 1e-6])
+def test_sparse_solver(errlim):
+    nx, ny, nz = 9, 8, 7
+    npts = nx * ny * nz
+    M = O.divb_matrix(nx, ny, nz)
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    rhs = np.random.default_rng(5).uniform(-1, 1, npts).astype(np.float32)
+    s1 = np.zeros(npts, np.float32)
+    s2 = np.zeros(npts, np.float32)
+    r1 = np.zeros(1, np.float32)
+    r2 = np.zeros(1, np.float32)
+    lib().call("sparse_solver", 7, npts, pos, np.float32(errlim), False, 0, rhs, s1, r1, M)
+    O.sparse_solver(7, npts, pos, np.float32(errlim), False, 0, rhs, s2, r2, M)
+    assert np.array_equal(s1, s2) and r1[0] == r2[0]
+
+
+def test_trilin_interp():
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        g = np.sort(rng.uniform(-5, 5, (2, 3)), axis=0).astype(np.float32)
+        s = (g[0] + rng.uniform(0, 1, 3) * (g[1] - g[0])).astype(np.float32)
+        q = rng.uniform(-1, 1, (2, 2, 2)).astype(np.float32)
+        a, b = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        lib().call("trilin_interp", s, g, q, a)
+        O.trilin_interp(s, g, q, b)
+        assert a[0] == b[0]
