@@ -1,0 +1,840 @@
+// extern "C" entry points of include/g3m3.h: argument unpacking, pointer
+// translation for registered host arrays, workspace, error latch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "../../include/g3m3.h"
+#include "g3m3_launch.h"
+
+using namespace g3m3;
+
+namespace {
+
+hipStream_t g_stream = nullptr;
+int g_err = 0;
+std::string g_msg;
+float g_iso_flag = 1.0f;   // the reference's SAVEd iso_flag (sub_set_speed_agrd.f90:68)
+
+void latch(int code, const char* what) {
+    if (g_err == 0) {
+        g_err = code;
+        g_msg = what;
+    }
+}
+
+void check_launch(const char* where) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::string m = std::string(where) + ": " + hipGetErrorString(e);
+        latch((int)e, m.c_str());
+    }
+}
+
+// --- host-array registry -------------------------------------------------------
+struct Mirror {
+    char* dev;
+    size_t bytes;
+};
+std::map<uintptr_t, Mirror> g_reg;   // keyed by host base address
+
+template <typename T>
+T* R(T* p) {
+    if (!p || g_reg.empty()) return p;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    auto it = g_reg.upper_bound(a);
+    if (it == g_reg.begin()) return p;
+    --it;
+    if (a >= it->first && a < it->first + it->second.bytes)
+        return reinterpret_cast<T*>(it->second.dev + (a - it->first));
+    return p;
+}
+
+// --- grow-only device workspace ---------------------------------------------------
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need) {
+        if (need > bytes) {
+            if (p) hipFree(p);
+            p = nullptr;
+            if (hipMalloc(&p, need) != hipSuccess) {
+                latch(2, "workspace allocation failed");
+                bytes = 0;
+                return nullptr;
+            }
+            bytes = need;
+        }
+        return p;
+    }
+};
+Buf g_fct, g_speed_list, g_small, g_divb_f, g_divb_d, g_tmp;
+
+inline Box box_of(const int* nx, const int* ny, const int* nz) { return Box{*nx, *ny, *nz}; }
+inline size_t off(const Box& B, const int* box) { return (size_t)(*box - 1) * B.V(); }
+
+template <typename T>
+T* S(T* base, const Box& B, const int* box) {
+    return R(base) + off(B, box);
+}
+
+}  // namespace
+
+extern "C" {
+
+int g3m3_init(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        latch(1, "no HIP device");
+        return 1;
+    }
+    hipSetDevice(0);
+    return 0;
+}
+
+int g3m3_last_error(void) {
+    const int e = g_err;
+    g_err = 0;
+    return e;
+}
+
+const char* g3m3_last_error_msg(void) { return g_msg.c_str(); }
+
+void g3m3_set_stream(void* s) { g_stream = (hipStream_t)s; }
+
+void g3m3_sync(void) {
+    if (hipStreamSynchronize(g_stream) != hipSuccess) latch(3, "stream synchronize failed");
+    check_launch("sync");
+}
+
+int g3m3_register(void* host, long long bytes) {
+    if (!host || bytes <= 0) return 1;
+    void* d = nullptr;
+    if (hipMalloc(&d, (size_t)bytes) != hipSuccess) {
+        latch(2, "register: allocation failed");
+        return 2;
+    }
+    g_reg[reinterpret_cast<uintptr_t>(host)] = Mirror{(char*)d, (size_t)bytes};
+    return 0;
+}
+
+int g3m3_unregister(void* host) {
+    auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
+    if (it == g_reg.end()) return 1;
+    hipStreamSynchronize(g_stream);
+    hipFree(it->second.dev);
+    g_reg.erase(it);
+    return 0;
+}
+
+int g3m3_to_device(void* host) {
+    auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
+    if (it == g_reg.end()) return 1;
+    hipMemcpyAsync(it->second.dev, host, it->second.bytes, hipMemcpyHostToDevice, g_stream);
+    return 0;
+}
+
+int g3m3_to_host(void* host) {
+    auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
+    if (it == g_reg.end()) return 1;
+    hipMemcpyAsync(host, it->second.dev, it->second.bytes, hipMemcpyDeviceToHost, g_stream);
+    hipStreamSynchronize(g_stream);
+    return 0;
+}
+
+// -----------------------------------------------------------------------------
+void g3m3_store_array(float* bx, const float* bx0, const int* nx, const int* ny, const int* nz,
+                      const int* n_grids, const int* box) {
+    const Box B = box_of(nx, ny, nz);
+    FieldPtrs d{}, s{};
+    d.p[0] = S(bx, B, box);
+    s.p[0] = const_cast<float*>(S(bx0, B, box));
+    launch_copy_fields(B.V(), 1, d, s, g_stream);
+    check_launch("store_array");
+}
+
+void g3m3_calcur(const float* bx, const float* by, const float* bz, const int* nx, const int* ny,
+                 const int* nz, const int* n_grids, const int* box, float* curx, float* cury,
+                 float* curz, const float* rx, const float* ry, const float* rz) {
+    const Box B = box_of(nx, ny, nz);
+    launch_calcur(B, S(bx, B, box), S(by, B, box), S(bz, B, box), R(curx), R(cury), R(curz), *rx,
+                  *ry, *rz, g_stream);
+    check_launch("calcur");
+}
+
+void g3m3_fnd_evel(const float* qpx, const float* qpy, const float* qpz, const float* qrho,
+                   const float* hpx, const float* hpy, const float* hpz, const float* hrho,
+                   const float* opx, const float* opy, const float* opz, const float* orho,
+                   const float* curx, const float* cury, const float* curz, float* evx,
+                   float* evy, float* evz, float* tvx, float* tvy, float* tvz, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box,
+                   const float* rmassq, const float* rmassh, const float* rmasso,
+                   const float* reynolds) {
+    const Box B = box_of(nx, ny, nz);
+    const float* mom[9] = {S(qpx, B, box), S(qpy, B, box), S(qpz, B, box),
+                           S(hpx, B, box), S(hpy, B, box), S(hpz, B, box),
+                           S(opx, B, box), S(opy, B, box), S(opz, B, box)};
+    const float* rho[3] = {S(qrho, B, box), S(hrho, B, box), S(orho, B, box)};
+    const float* cur[3] = {R(curx), R(cury), R(curz)};
+    float* ev[3] = {R(evx), R(evy), R(evz)};
+    float* tv[3] = {R(tvx), R(tvy), R(tvz)};
+    const float* nb[3] = {nullptr, nullptr, nullptr};
+    float* nbs[3] = {nullptr, nullptr, nullptr};
+    launch_bfield_evel(B, nb, nb, nbs, nullptr, mom, rho, cur, ev, tv, *rmassq, *rmassh, *rmasso,
+                       *reynolds, 0.0f, 0, 1, g_stream);
+    check_launch("fnd_evel");
+}
+
+void g3m3_bfield_total(const float* bx, const float* by, const float* bz, const float* bx0,
+                       const float* by0, const float* bz0, float* bsx, float* bsy, float* bsz,
+                       float* btot, const int* nx, const int* ny, const int* nz,
+                       const int* n_grids, const int* box, const float* smallbit) {
+    const Box B = box_of(nx, ny, nz);
+    const float* b0[3] = {S(bx0, B, box), S(by0, B, box), S(bz0, B, box)};
+    const float* b[3] = {S(bx, B, box), S(by, B, box), S(bz, B, box)};
+    float* bs[3] = {R(bsx), R(bsy), R(bsz)};
+    const float* nm[9] = {};
+    const float* nr[3] = {};
+    float* nv[3] = {};
+    launch_bfield_evel(B, b0, b, bs, R(btot), nm, nr, nr, nv, nv, 1.f, 1.f, 1.f, 1.f, *smallbit, 1,
+                       0, g_stream);
+    check_launch("bfield_total");
+}
+
+static void bande_impl(float* efldx, float* efldy, float* efldz, const float* bsx,
+                       const float* bsy, const float* bsz, const float* curx, const float* cury,
+                       const float* curz, const float* evx, const float* evy, const float* evz,
+                       const float* epres, const float* qrho, const float* hrho,
+                       const float* orho, const float* rst, const float* resist,
+                       const float* reynolds, const int* nx, const int* ny, const int* nz,
+                       const int* box, const float* rmassq, const float* rmassh,
+                       const float* rmasso, const int* ijmid, const int* nummid,
+                       const int* ijzero, const int* mbndry, const int* numzero, const float* rx,
+                       const float* ry, const float* rz, float eta) {
+    const Box B = box_of(nx, ny, nz);
+    float* e[3] = {R(efldx), R(efldy), R(efldz)};
+    const float* bs[3] = {R(bsx), R(bsy), R(bsz)};
+    const float* cur[3] = {R(curx), R(cury), R(curz)};
+    const float* ev[3] = {R(evx), R(evy), R(evz)};
+    const float* rho[3] = {S(qrho, B, box), S(hrho, B, box), S(orho, B, box)};
+    const int b = *box - 1;
+    const int nm = (*box <= *mbndry) ? nummid[b] : 0;
+    const int nz0 = (*box <= *mbndry) ? numzero[b] : 0;
+    launch_bande(B, e, bs, cur, ev, S(epres, B, box), rho, R(rst), *resist, *reynolds, *box,
+                 *rmassq, *rmassh, *rmasso, R(ijmid), nm, R(ijzero), nz0, *mbndry, *rx, *ry, *rz,
+                 eta, g_stream);
+    check_launch("bande");
+}
+
+void g3m3_bande(float* efldx, float* efldy, float* efldz, const float* bsx, const float* bsy,
+                const float* bsz, const float* curx, const float* cury, const float* curz,
+                const float* evx, const float* evy, const float* evz, const float* btot,
+                const float* epres, const float* qrho, const float* hrho, const float* orho,
+                const float* rst, const float* resist, const float* reynolds, const int* nx,
+                const int* ny, const int* nz, const int* n_grids, const int* box,
+                const float* rmassq, const float* rmassh, const float* rmasso, const int* ijmid,
+                const int* nummid, const int* ijzero, const int* mbndry, const int* numzero,
+                const int* mmid, const int* mzero, const float* rx, const float* ry,
+                const float* rz) {
+    bande_impl(efldx, efldy, efldz, bsx, bsy, bsz, curx, cury, curz, evx, evy, evz, epres, qrho,
+               hrho, orho, rst, resist, reynolds, nx, ny, nz, box, rmassq, rmassh, rmasso, ijmid,
+               nummid, ijzero, mbndry, numzero, rx, ry, rz, 0.0f);
+}
+
+void g3m3_bande_eta(float* efldx, float* efldy, float* efldz, const float* bsx, const float* bsy,
+                    const float* bsz, const float* curx, const float* cury, const float* curz,
+                    const float* evx, const float* evy, const float* evz, const float* btot,
+                    const float* epres, const float* qrho, const float* hrho, const float* orho,
+                    const float* rst, const float* resist, const float* reynolds, const int* nx,
+                    const int* ny, const int* nz, const int* n_grids, const int* box,
+                    const float* rmassq, const float* rmassh, const float* rmasso,
+                    const int* ijmid, const int* nummid, const int* ijzero, const int* mbndry,
+                    const int* numzero, const int* mmid, const int* mzero, const float* rx,
+                    const float* ry, const float* rz, const float* eta) {
+    bande_impl(efldx, efldy, efldz, bsx, bsy, bsz, curx, cury, curz, evx, evy, evz, epres, qrho,
+               hrho, orho, rst, resist, reynolds, nx, ny, nz, box, rmassq, rmassh, rmasso, ijmid,
+               nummid, ijzero, mbndry, numzero, rx, ry, rz, *eta);
+}
+
+void g3m3_push_elec(const float* wrkepres, const float* oldepres, float* epres, const float* evx,
+                    const float* evy, const float* evz, const float* gamma, const float* gamma1,
+                    const int* nx, const int* ny, const int* nz, const int* n_grids,
+                    const int* box, const float* delt, const float* rx, const float* ry,
+                    const float* rz) {
+    const Box B = box_of(nx, ny, nz);
+    const float* ev[3] = {R(evx), R(evy), R(evz)};
+    launch_push_elec(B, S(wrkepres, B, box), S(oldepres, B, box), S(epres, B, box), ev, *gamma,
+                     *gamma1, *delt, *rx, *ry, *rz, g_stream);
+    check_launch("push_elec");
+}
+
+void g3m3_push_ion(float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx,
+                   float* qpy, float* qpz, const float* oldqrho, const float* oldqpresx,
+                   const float* oldqpresy, const float* oldqpresz, const float* oldqpx,
+                   const float* oldqpy, const float* oldqpz, const float* wrkqrho,
+                   const float* wrkqpresx, const float* wrkqpresy, const float* wrkqpresz,
+                   const float* wrkqpx, const float* wrkqpy, const float* wrkqpz,
+                   float* qpresxy, float* qpresxz, float* qpresyz, const float* oldqpresxy,
+                   const float* oldqpresxz, const float* oldqpresyz, const float* wrkqpresxy,
+                   const float* wrkqpresxz, const float* wrkqpresyz, const float* bsx,
+                   const float* bsy, const float* bsz, const float* btot, const float* efldx,
+                   const float* efldy, const float* efldz, const float* rmassq, float* qvx,
+                   float* qvy, float* qvz, const float* tvx, const float* tvy, const float* tvz,
+                   const float* gamma, const float* gamma1, const int* nx, const int* ny,
+                   const int* nz, const int* n_grids, const int* box, const float* delt,
+                   const float* grav, const float* re_equiv, const float* reynolds,
+                   const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+                   const float* grd_ymax, const float* grd_zmin, const float* grd_zmax,
+                   const int* isotropic) {
+    const Box B = box_of(nx, ny, nz);
+    IonPtrs P;
+    float* out[10] = {qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz};
+    const float* old[10] = {oldqrho, oldqpresx, oldqpresy, oldqpresz, oldqpx, oldqpy, oldqpz,
+                            oldqpresxy, oldqpresxz, oldqpresyz};
+    const float* wrk[10] = {wrkqrho, wrkqpresx, wrkqpresy, wrkqpresz, wrkqpx, wrkqpy, wrkqpz,
+                            wrkqpresxy, wrkqpresxz, wrkqpresyz};
+    for (int f = 0; f < 10; ++f) {
+        P.out[f] = S(out[f], B, box);
+        P.old[f] = S(old[f], B, box);
+        P.wrk[f] = S(wrk[f], B, box);
+    }
+    P.bs[0] = R(bsx);
+    P.bs[1] = R(bsy);
+    P.bs[2] = R(bsz);
+    P.btot = R(btot);
+    P.e[0] = R(efldx);
+    P.e[1] = R(efldy);
+    P.e[2] = R(efldz);
+    P.qv[0] = R(qvx);
+    P.qv[1] = R(qvy);
+    P.qv[2] = R(qvz);
+    P.tv[0] = R(tvx);
+    P.tv[1] = R(tvy);
+    P.tv[2] = R(tvz);
+    const int b = *box - 1;
+    const float gmin[3] = {grd_xmin[b], grd_ymin[b], grd_zmin[b]};
+    const float gmax[3] = {grd_xmax[b], grd_ymax[b], grd_zmax[b]};
+    launch_push_ion(B, P, *rmassq, *delt, *grav, *re_equiv, *reynolds, gmin, gmax, g_stream);
+    check_launch("push_ion");
+}
+
+void g3m3_push_bfld(float* bx, float* by, float* bz, const float* oldbx, const float* oldby,
+                    const float* oldbz, const float* efldx, const float* efldy, const float* efldz,
+                    const int* nx, const int* ny, const int* nz, const int* n_grids,
+                    const int* box, const float* delt, const float* rx, const float* ry,
+                    const float* rz) {
+    const Box B = box_of(nx, ny, nz);
+    float* b[3] = {S(bx, B, box), S(by, B, box), S(bz, B, box)};
+    const float* o[3] = {S(oldbx, B, box), S(oldby, B, box), S(oldbz, B, box)};
+    const float* e[3] = {R(efldx), R(efldy), R(efldz)};
+    launch_push_bfld(B, b, o, e, *delt, *rx, *ry, *rz, g_stream);
+    check_launch("push_bfld");
+}
+
+void g3m3_set_rho(float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpresxy,
+                  float* qpresxz, float* qpresyz, const float* rmassq, float* hrho, float* hpresx,
+                  float* hpresy, float* hpresz, float* hpresxy, float* hpresxz, float* hpresyz,
+                  const float* rmassh, float* orho, float* opresx, float* opresy, float* opresz,
+                  float* opresxy, float* opresxz, float* opresyz, const float* rmasso,
+                  float* epres, const int* nx, const int* ny, const int* nz, const int* n_grids,
+                  const int* box, const float* o_conc) {
+    const Box B = box_of(nx, ny, nz);
+    SpeciesPtrs SP{};
+    float* g[3][7] = {{qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz},
+                      {hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz},
+                      {orho, opresx, opresy, opresz, opresxy, opresxz, opresyz}};
+    for (int s = 0; s < 3; ++s) {
+        SP.f[s][0] = S(g[s][0], B, box);
+        for (int d = 0; d < 3; ++d) SP.f[s][1 + d] = S(g[s][1 + d], B, box);
+        for (int d = 0; d < 3; ++d) SP.f[s][7 + d] = S(g[s][4 + d], B, box);
+    }
+    launch_set_rho(B.V(), SP, S(epres, B, box), g_stream);
+    check_launch("set_rho");
+}
+
+void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                         const int* box, const float* rmassq, const float* rmassh,
+                         const float* rmasso, const int* isotropic, const float* smallbit,
+                         const float* aniso_limit, const float* vlim, const float* qvlim,
+                         const float* hvlim, const float* ovlim, const float* qclim,
+                         const float* hclim, const float* oclim, const float* cslim,
+                         const float* bx, const float* by, const float* bz, const float* bx0,
+                         const float* by0, const float* bz0, float* qrho, float* qpx, float* qpy,
+                         float* qpz, float* qpresx, float* qpresy, float* qpresz, float* qpresxy,
+                         float* qpresxz, float* qpresyz, float* hrho, float* hpx, float* hpy,
+                         float* hpz, float* hpresx, float* hpresy, float* hpresz, float* hpresxy,
+                         float* hpresxz, float* hpresyz, float* orho, float* opx, float* opy,
+                         float* opz, float* opresx, float* opresy, float* opresz,
+                         float* opresxy, float* opresxz, float* opresyz, float* epres,
+                         float* btot, float* pxmax, float* pymax, float* pzmax, float* csmax,
+                         float* alfmax, float* fastest) {
+    const Box B = box_of(nx, ny, nz);
+    SpeciesPtrs SP{};
+    float* g[3][10] = {{qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz},
+                       {hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz, hpresxy, hpresxz, hpresyz},
+                       {orho, opresx, opresy, opresz, opx, opy, opz, opresxy, opresxz, opresyz}};
+    for (int s = 0; s < 3; ++s)
+        for (int f = 0; f < 10; ++f) SP.f[s][f] = S(g[s][f], B, box);
+    if (*isotropic) g_iso_flag = 0.0f;
+    SpeedParams Q;
+    Q.aniso = *aniso_limit;
+    Q.smallbit = *smallbit;
+    Q.vlim = *vlim;
+    Q.cslim = *cslim;
+    Q.iso_flag = g_iso_flag;
+    Q.isotropic = *isotropic ? 1 : 0;
+    Q.svlim[0] = *qvlim;
+    Q.svlim[1] = *hvlim;
+    Q.svlim[2] = *ovlim;
+    Q.sclim[0] = *qclim;
+    Q.sclim[1] = *hclim;
+    Q.sclim[2] = *oclim;
+    const float* b0[3] = {S(bx0, B, box), S(by0, B, box), S(bz0, B, box)};
+    const float* b[3] = {S(bx, B, box), S(by, B, box), S(bz, B, box)};
+    void* list = g_speed_list.get(speed_list_bytes(B.V()));
+    char* small = (char*)g_small.get(256);
+    if (!list || !small) return;
+    float* maxima = (float*)small;
+    int* count = (int*)(small + 64);
+    launch_set_speed(B, SP, Q, b0, b, R(btot), S(epres, B, box), maxima, list, count, g_stream);
+    check_launch("set_speed_agrd");
+    float h[6];
+    hipMemcpyAsync(h, maxima, sizeof(h), hipMemcpyDeviceToHost, g_stream);
+    if (hipStreamSynchronize(g_stream) != hipSuccess) latch(3, "set_speed_agrd sync failed");
+    *pxmax = h[0];
+    *pymax = h[1];
+    *pzmax = h[2];
+    *csmax = h[3];
+    *alfmax = h[4];
+    const float f = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2] + h[3] * h[3] + h[4] * h[4]);
+    *fastest = std::max(0.0f, f);
+}
+
+void g3m3_species_velocity(const float* rho, const float* px, const float* py, const float* pz,
+                           float* vx, float* vy, float* vz, const int* nx, const int* ny,
+                           const int* nz, const int* n_grids, const int* box,
+                           const float* smallbit) {
+    const Box B = box_of(nx, ny, nz);
+    const float* p[3] = {S(px, B, box), S(py, B, box), S(pz, B, box)};
+    float* v[3] = {R(vx), R(vy), R(vz)};
+    launch_velocity(B.V(), S(rho, B, box), p, v, *smallbit, g_stream);
+    check_launch("species_velocity");
+}
+
+static void lap_impl(const Box& B, const float* const* src, float* const* dst, const float* chi,
+                     int nf, const float* vx, const float* vy, const float* vz, float delt,
+                     const int* box) {
+    LapPtrs L{};
+    for (int f = 0; f < nf; ++f) {
+        L.src[f] = S(src[f], B, box);
+        L.dst[f] = S(dst[f], B, box);
+        L.chi[f] = chi[f];
+    }
+    const float* v[3] = {R(vx), R(vy), R(vz)};
+    launch_lapidus(B, L, nf, v, delt, g_stream);
+    check_launch("lapidus");
+}
+
+void g3m3_lap_plasma(const float* rho, const float* px, const float* py, const float* pz,
+                     const float* presx, const float* presy, const float* presz,
+                     const float* presxy, const float* presxz, const float* presyz,
+                     float* wrkrho, float* wrkpx, float* wrkpy, float* wrkpz, float* wrkpresx,
+                     float* wrkpresy, float* wrkpresz, float* wrkpresxy, float* wrkpresxz,
+                     float* wrkpresyz, const float* vx, const float* vy, const float* vz,
+                     const int* nx, const int* ny, const int* nz, const int* n_grids,
+                     const int* box, const float* chirho, const float* chipxyz,
+                     const float* chierg, const float* delt, const int* isotropic) {
+    const Box B = box_of(nx, ny, nz);
+    const float* src[10] = {rho, px, py, pz, presx, presy, presz, presxy, presxz, presyz};
+    float* dst[10] = {wrkrho, wrkpx, wrkpy, wrkpz, wrkpresx, wrkpresy, wrkpresz,
+                      wrkpresxy, wrkpresxz, wrkpresyz};
+    float chi[10];
+    chi[0] = *chirho;
+    for (int f = 1; f < 10; ++f) chi[f] = *chipxyz;
+    lap_impl(B, src, dst, chi, 10, vx, vy, vz, *delt, box);
+}
+
+void g3m3_lap_elec(const float* ppres, float* wrkppres, const float* vx, const float* vy,
+                   const float* vz, const int* nx, const int* ny, const int* nz,
+                   const int* n_grids, const int* box, const float* chirho,
+                   const float* chipxyz, const float* chierg, const float* delt) {
+    const Box B = box_of(nx, ny, nz);
+    const float* src[1] = {ppres};
+    float* dst[1] = {wrkppres};
+    lap_impl(B, src, dst, chipxyz, 1, vx, vy, vz, *delt, box);
+}
+
+void g3m3_lap_bfld(const float* bx, const float* by, const float* bz, float* wrkbx, float* wrkby,
+                   float* wrkbz, const float* vx, const float* vy, const float* vz,
+                   const int* nx, const int* ny, const int* nz, const int* n_grids,
+                   const int* box, const float* chirho, const float* chipxyz,
+                   const float* chierg, const float* delt) {
+    const Box B = box_of(nx, ny, nz);
+    const float* src[3] = {bx, by, bz};
+    float* dst[3] = {wrkbx, wrkby, wrkbz};
+    const float chi[3] = {*chierg, *chierg, *chierg};
+    lap_impl(B, src, dst, chi, 3, vx, vy, vz, *delt, box);
+}
+
+void g3m3_fnd_vtot(const float* qpx, const float* qpy, const float* qpz, const float* qrho,
+                   const float* hpx, const float* hpy, const float* hpz, const float* hrho,
+                   const float* opx, const float* opy, const float* opz, const float* orho,
+                   float* vx, float* vy, float* vz, const int* nx, const int* ny, const int* nz,
+                   const int* n_grids, const int* box, const float* rmassq,
+                   const float* rmassh, const float* rmasso) {
+    const Box B = box_of(nx, ny, nz);
+    const float* mom[9] = {S(qpx, B, box), S(qpy, B, box), S(qpz, B, box),
+                           S(hpx, B, box), S(hpy, B, box), S(hpz, B, box),
+                           S(opx, B, box), S(opy, B, box), S(opz, B, box)};
+    const float* rho[3] = {S(qrho, B, box), S(hrho, B, box), S(orho, B, box)};
+    float* v[3] = {R(vx), R(vy), R(vz)};
+    launch_vtot(B.V(), mom, rho, v, g_stream);
+    check_launch("fnd_vtot");
+}
+
+static void fct_impl(const Box& B, float* const* out, const float* const* old,
+                     const float* const* wrk, const float* chi, int nf, const int* box) {
+    FctPtrs F{};
+    for (int f = 0; f < nf; ++f) {
+        F.out[f] = S(out[f], B, box);
+        F.old[f] = S(old[f], B, box);
+        F.wrk[f] = S(wrk[f], B, box);
+        F.chi[f] = chi[f];
+    }
+    float* scratch = (float*)g_fct.get(fct_scratch_floats(B.V(), nf) * sizeof(float));
+    if (!scratch) return;
+    launch_fct(B, F, nf, scratch, g_stream);
+    check_launch("fct");
+}
+
+void g3m3_fcsmooth(float* px, const float* oldpx, const float* wrkpx, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box,
+                   const float* chipx, float* tempx, float* tempy, float* tempz) {
+    const Box B = box_of(nx, ny, nz);
+    float* out[1] = {px};
+    const float* old[1] = {oldpx};
+    const float* wrk[1] = {wrkpx};
+    fct_impl(B, out, old, wrk, chipx, 1, box);
+}
+
+void g3m3_flux_correct(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpresxy, float* qpresxz,
+    float* qpresyz, float* qpx, float* qpy, float* qpz, const float* wrkqrho,
+    const float* wrkqpresx, const float* wrkqpresy, const float* wrkqpresz,
+    const float* wrkqpresxy, const float* wrkqpresxz, const float* wrkqpresyz,
+    const float* wrkqpx, const float* wrkqpy, const float* wrkqpz, const float* oldqrho,
+    const float* oldqpresx, const float* oldqpresy, const float* oldqpresz,
+    const float* oldqpresxy, const float* oldqpresxz, const float* oldqpresyz,
+    const float* oldqpx, const float* oldqpy, const float* oldqpz, float* hrho, float* hpresx,
+    float* hpresy, float* hpresz, float* hpresxy, float* hpresxz, float* hpresyz, float* hpx,
+    float* hpy, float* hpz, const float* wrkhrho, const float* wrkhpresx,
+    const float* wrkhpresy, const float* wrkhpresz, const float* wrkhpresxy,
+    const float* wrkhpresxz, const float* wrkhpresyz, const float* wrkhpx, const float* wrkhpy,
+    const float* wrkhpz, const float* oldhrho, const float* oldhpresx, const float* oldhpresy,
+    const float* oldhpresz, const float* oldhpresxy, const float* oldhpresxz,
+    const float* oldhpresyz, const float* oldhpx, const float* oldhpy, const float* oldhpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opresxy, float* opresxz,
+    float* opresyz, float* opx, float* opy, float* opz, const float* wrkorho,
+    const float* wrkopresx, const float* wrkopresy, const float* wrkopresz,
+    const float* wrkopresxy, const float* wrkopresxz, const float* wrkopresyz,
+    const float* wrkopx, const float* wrkopy, const float* wrkopz, const float* oldorho,
+    const float* oldopresx, const float* oldopresy, const float* oldopresz,
+    const float* oldopresxy, const float* oldopresxz, const float* oldopresyz,
+    const float* oldopx, const float* oldopy, const float* oldopz, float* epres,
+    const float* wrkepres, const float* oldepres, float* bx, float* by, float* bz,
+    const float* wrkbx, const float* wrkby, const float* wrkbz, const float* oldbx,
+    const float* oldby, const float* oldbz, float* vvx, float* vvy, float* vvz, const int* nx,
+    const int* ny, const int* nz, const int* n_grids, const int* box, const float* difrho,
+    const float* diferg, const float* xspac, const int* isotropic) {
+    const Box B = box_of(nx, ny, nz);
+    float* out[34] = {qrho, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, qpx, qpy, qpz,
+                      hrho, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, hpx, hpy, hpz,
+                      orho, opresx, opresy, opresz, opresxy, opresxz, opresyz, opx, opy, opz,
+                      epres, bx, by, bz};
+    const float* wrk[34] = {wrkqrho, wrkqpresx, wrkqpresy, wrkqpresz, wrkqpresxy, wrkqpresxz,
+                            wrkqpresyz, wrkqpx, wrkqpy, wrkqpz, wrkhrho, wrkhpresx, wrkhpresy,
+                            wrkhpresz, wrkhpresxy, wrkhpresxz, wrkhpresyz, wrkhpx, wrkhpy, wrkhpz,
+                            wrkorho, wrkopresx, wrkopresy, wrkopresz, wrkopresxy, wrkopresxz,
+                            wrkopresyz, wrkopx, wrkopy, wrkopz, wrkepres, wrkbx, wrkby, wrkbz};
+    const float* old[34] = {oldqrho, oldqpresx, oldqpresy, oldqpresz, oldqpresxy, oldqpresxz,
+                            oldqpresyz, oldqpx, oldqpy, oldqpz, oldhrho, oldhpresx, oldhpresy,
+                            oldhpresz, oldhpresxy, oldhpresxz, oldhpresyz, oldhpx, oldhpy, oldhpz,
+                            oldorho, oldopresx, oldopresy, oldopresz, oldopresxy, oldopresxz,
+                            oldopresyz, oldopx, oldopy, oldopz, oldepres, oldbx, oldby, oldbz};
+    float chi[34];
+    for (int f = 0; f < 31; ++f) chi[f] = *difrho;
+    for (int f = 31; f < 34; ++f) chi[f] = *diferg;
+    fct_impl(B, out, old, wrk, chi, 34, box);
+}
+
+void g3m3_bndry_outer(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx, float* hpy, float* hpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opx, float* opy, float* opz,
+    float* epres, float* qpresxy, float* qpresxz, float* qpresyz, float* hpresxy,
+    float* hpresxz, float* hpresyz, float* opresxy, float* opresxz, float* opresyz,
+    const float* rmassq, const float* rmassh, const float* rmasso, float* bx, float* by,
+    float* bz, const float* bx0, const float* by0, const float* bz0, const int* nx,
+    const int* ny, const int* nz, const int* n_grids, const float* srho, const float* rho_frac,
+    const float* o_conc, const float* spress, const float* spx, const float* spy,
+    const float* spz, const float* sbx_wind, const float* sby_wind, const float* sbz_wind,
+    const float* ti_te, const int* isotropic) {
+    const Box B = box_of(nx, ny, nz);
+    const int* box = n_grids;
+    const float ofrac = *rho_frac * *o_conc;
+    const float frac_h = *rmassh / *rmassq;
+    const float frac_o = *rmasso / *rmassq;
+    const float pq = 0.5f * *spress;
+    const float ph = 0.5f * *spress * *rho_frac;
+    const float po = 0.5f * *spress * ofrac;
+    struct E {
+        float* f;
+        int kind;
+        float val;
+    } fs[34] = {
+        {qrho, kRHO, *srho}, {qpx, kPX, *spx}, {qpy, kPY, *spy}, {qpz, kPZ, *spz},
+        {qpresx, kPRES, pq}, {qpresy, kPRES, pq}, {qpresz, kPRES, pq},
+        {hrho, kRHO, *srho * *rho_frac * frac_h}, {hpx, kPX, *spx * *rho_frac * frac_h},
+        {hpy, kPY, *spy * *rho_frac * frac_h}, {hpz, kPZ, *spz * *rho_frac * frac_h},
+        {hpresx, kPRES, ph}, {hpresy, kPRES, ph}, {hpresz, kPRES, ph},
+        {orho, kRHO, *srho * ofrac * frac_o}, {opx, kPX, *spx * ofrac * frac_o},
+        {opy, kPY, *spy * ofrac * frac_o}, {opz, kPZ, *spz * ofrac * frac_o},
+        {opresx, kPRES, po}, {opresy, kPRES, po}, {opresz, kPRES, po},
+        {epres, kEP, pq / *ti_te},
+        {qpresxy, kOFF, 0.f}, {qpresxz, kOFF, 0.f}, {qpresyz, kOFF, 0.f},
+        {hpresxy, kOFF, 0.f}, {hpresxz, kOFF, 0.f}, {hpresyz, kOFF, 0.f},
+        {opresxy, kOFF, 0.f}, {opresxz, kOFF, 0.f}, {opresyz, kOFF, 0.f},
+        {bx, kBX, *sbx_wind}, {by, kBY, *sby_wind}, {bz, kBZ, *sbz_wind}};
+    OuterPtrs O{};
+    for (int f = 0; f < 34; ++f) {
+        O.f[f] = S(fs[f].f, B, box);
+        O.kind[f] = fs[f].kind;
+        O.val[f] = fs[f].val;
+    }
+    O.b0[0] = S(bx0, B, box);
+    O.b0[1] = S(by0, B, box);
+    O.b0[2] = S(bz0, B, box);
+    launch_bndry_outer(B, O, 34, g_stream);
+    check_launch("bndry_outer");
+}
+
+void g3m3_bndry_inner(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    const float* rmassq, float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx,
+    float* hpy, float* hpz, const float* rmassh, float* orho, float* opresx, float* opresy,
+    float* opresz, float* opx, float* opy, float* opz, const float* rmasso, float* epres,
+    float* bx, float* by, float* bz, float* qpresxy, float* qpresxz, float* qpresyz,
+    float* hpresxy, float* hpresxz, float* hpresyz, float* opresxy, float* opresxz,
+    float* opresyz, const int* nx, const int* ny, const int* nz, const int* n_grids,
+    const float* parm_srf, const float* parm_mid, const float* parm_zero, const int* ijsrf,
+    const int* numsrf, const int* ijmid, const int* nummid, const int* ijzero,
+    const int* numzero, const int* mbndry, const int* msrf, const int* mmid, const int* mzero,
+    const float* erho, const float* epress, const float* alpha_e, const float* ti_te,
+    const float* o_conc, const float* re_equiv, const float* r_inner, const float* sbx_wind,
+    const float* spress, const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+    const float* grd_ymax, const float* grd_zmin, const float* grd_zmax, const float* v_rot,
+    const float* r_rot) {
+    const Box B = box_of(nx, ny, nz);
+    const int* box = mbndry;
+    float* g[34] = {qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz,
+                    hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz, hpresxy, hpresxz, hpresyz,
+                    orho, opresx, opresy, opresz, opx, opy, opz, opresxy, opresxz, opresyz,
+                    epres, bx, by, bz};
+    InnerPtrs I{};
+    for (int f = 0; f < 34; ++f) I.f[f] = S(g[f], B, box);
+    const int b = *mbndry - 1;
+    const int* ij[3] = {R(ijsrf), R(ijmid), R(ijzero)};
+    const float* parm[3] = {R(parm_srf), R(parm_mid), R(parm_zero)};
+    const int num[3] = {numsrf[b], nummid[b], numzero[b]};
+    const float dx = (grd_xmax[b] - grd_xmin[b]) / (float)(*nx - 1.0f);
+    const float dy = (grd_ymax[b] - grd_ymin[b]) / (float)(*ny - 1.0f);
+    launch_bndry_inner(B, I, ij, parm, num, *mbndry, *mbndry, grd_xmin[b], grd_ymin[b], dx, dy,
+                       *re_equiv, *v_rot, g_stream);
+    check_launch("bndry_inner");
+}
+
+static FlankGeom flank_geom(int nx, int ny, int nz, int box, const float* gx0, const float* gx1,
+                            const float* gy0, const float* gy1, const float* gz0,
+                            const float* gz1) {
+    const int mb = box, ms = box - 1;   // 0-based coarse, fine
+    FlankGeom G;
+    G.delx = (gx1[mb] - gx0[mb]) / (nx - 1.0f);
+    G.dely = (gy1[mb] - gy0[mb]) / (ny - 1.0f);
+    G.delz = (gz1[mb] - gz0[mb]) / (nz - 1.0f);
+    G.sx = (gx1[ms] - gx0[ms]) / (nx - 1.0f);
+    G.sy = (gy1[ms] - gy0[ms]) / (ny - 1.0f);
+    G.sz = (gz1[ms] - gz0[ms]) / (nz - 1.0f);
+    G.sxmin = gx0[ms];
+    G.symin = gy0[ms];
+    G.szmin = gz0[ms];
+    G.bxmin = gx0[mb];
+    G.bymin = gy0[mb];
+    G.bzmin = gz0[mb];
+    return G;
+}
+
+void g3m3_bndry_flanks_v(float* const* tgt, const float* const* cur, const float* const* old,
+                         float* work, const int* nx, const int* ny, const int* nz,
+                         const int* n_grids, const int* box, const double* t_old,
+                         const double* t_new, const double* t, const float* grd_xmin,
+                         const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                         const float* grd_zmin, const float* grd_zmax) {
+    const Box B = box_of(nx, ny, nz);
+    const int mb = *box;   // 0-based index of box+1
+    const float t1 = (float)((t_new[mb] - *t) / (t_new[mb] - t_old[mb]));
+    const float t2 = 1.0f - t1;
+    const int cbox = *box + 1;
+    FlankPtrs P{};
+    for (int f = 0; f < 34; ++f) {
+        P.tgt[f] = S(tgt[f], B, box);
+        P.cur[f] = S(cur[f], B, &cbox);
+        P.old[f] = S(old[f], B, &cbox);
+    }
+    const FlankGeom G = flank_geom(*nx, *ny, *nz, *box, grd_xmin, grd_xmax, grd_ymin, grd_ymax,
+                                   grd_zmin, grd_zmax);
+    launch_flanks(B, P, 34, G, t1, t2, g_stream);
+    check_launch("bndry_flanks");
+}
+
+void g3m3_flanks(float* wrkrho, const float* rho, const float* oldrho, const int* nx,
+                 const int* ny, const int* nz, const int* n_grids, const int* box,
+                 const double* t_new, const double* t_old, const double* t, float* work,
+                 const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+                 const float* grd_ymax, const float* grd_zmin, const float* grd_zmax) {
+    const Box B = box_of(nx, ny, nz);
+    const int mb = *box;
+    const float t1 = (float)((t_new[mb] - *t) / (t_new[mb] - t_old[mb]));
+    const float t2 = 1.0f - t1;
+    const int cbox = *box + 1;
+    FlankPtrs P{};
+    P.tgt[0] = S(wrkrho, B, box);
+    P.cur[0] = S(rho, B, &cbox);
+    P.old[0] = S(oldrho, B, &cbox);
+    const FlankGeom G = flank_geom(*nx, *ny, *nz, *box, grd_xmin, grd_xmax, grd_ymin, grd_ymax,
+                                   grd_zmin, grd_zmax);
+    launch_flanks(B, P, 1, G, t1, t2, g_stream);
+    check_launch("flanks");
+}
+
+static void corer_impl(float* const* fields, int nf, const int* nx, const int* ny, const int* nz,
+                       const int* box, const float* gx0, const float* gx1, const float* gy0,
+                       const float* gy1, const float* gz0, const float* gz1) {
+    const Box B = box_of(nx, ny, nz);
+    const int mb = *box, ms = *box - 1;
+    const float sx = (gx1[ms] - gx0[ms]) / (*nx - 1.0f);
+    const float dx = (gx1[mb] - gx0[mb]) / (*nx - 1.0f);
+    const float dy = (gy1[mb] - gy0[mb]) / (*ny - 1.0f);
+    const float dz = (gz1[mb] - gz0[mb]) / (*nz - 1.0f);
+    const int nskip = (int)(dx / sx);
+    const int nbx1 = (int)(1.0f + (gx0[ms] - gx0[mb]) / dx + 0.5f);
+    const int nby1 = (int)(1.0f + (gy0[ms] - gy0[mb]) / dy + 0.5f);
+    const int nbz1 = (int)(1.0f + (gz0[ms] - gz0[mb]) / dz + 0.5f);
+    const int cbox = *box + 1;
+    FieldPtrs fine{}, coarse{};
+    for (int f = 0; f < nf; ++f) {
+        fine.p[f] = S(fields[f], B, box);
+        coarse.p[f] = S(fields[f], B, &cbox);
+    }
+    launch_corer(B, fine, coarse, nf, nskip, nbx1, nby1, nbz1, g_stream);
+    check_launch("corer");
+}
+
+void g3m3_corer(float* rho, const int* nx, const int* ny, const int* nz, const int* n_grids,
+                const int* box, const float* grd_xmin, const float* grd_xmax,
+                const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+                const float* grd_zmax) {
+    float* f[1] = {rho};
+    corer_impl(f, 1, nx, ny, nz, box, grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax);
+}
+
+void g3m3_bndry_corer_v(float* const* fields, const int* nx, const int* ny, const int* nz,
+                        const int* n_grids, const int* box, const float* grd_xmin,
+                        const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                        const float* grd_zmin, const float* grd_zmax) {
+    corer_impl(fields, 34, nx, ny, nz, box, grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin,
+               grd_zmax);
+}
+
+void g3m3_flanks_synced(float* rho, const int* nx, const int* ny, const int* nz,
+                        const int* n_grids, const int* box, const float* gx0, const float* gx1,
+                        const float* gy0, const float* gy1, const float* gz0, const float* gz1) {
+    const Box B = box_of(nx, ny, nz);
+    const int mb = *box, ms = *box - 1;
+    const FlankGeom G = flank_geom(*nx, *ny, *nz, *box, gx0, gx1, gy0, gy1, gz0, gz1);
+    const int kbi[6] = {(int)(1.0f + (gz1[ms] - gz0[mb]) / G.delz),
+                        (int)(1.0f + (gz0[ms] - gz0[mb]) / G.delz),
+                        (int)(1.0f + (gy1[ms] - gy0[mb]) / G.dely),
+                        (int)(1.0f + (gy0[ms] - gy0[mb]) / G.dely),
+                        (int)(1.0f + (gx1[ms] - gx0[mb]) / G.delx),
+                        (int)(1.0f + (gx0[ms] - gx0[mb]) / G.delx)};
+    const int cbox = *box + 1;
+    FieldPtrs fine{}, coarse{};
+    fine.p[0] = S(rho, B, box);
+    coarse.p[0] = S(rho, B, &cbox);
+    launch_flanks_synced(B, fine, coarse, 1, G, kbi, g_stream);
+    check_launch("flanks_synced");
+}
+
+void g3m3_banded_tpose(const int* nbands, const int* npts, const int* divB_band_pos,
+                       const float* banded_matrix, const float* x_vec, float* tp_result) {
+    if (*nbands != 7) {
+        latch(4, "banded_tpose: only the 7-band layout is supported");
+        return;
+    }
+    int* pos = (int*)g_tmp.get(64);
+    if (!pos) return;
+    hipMemcpyAsync(pos, divB_band_pos, 7 * sizeof(int), hipMemcpyHostToDevice, g_stream);
+    launch_banded_matvec(*npts, pos, R(banded_matrix), R(x_vec), R(tp_result), g_stream);
+    check_launch("banded_tpose");
+}
+
+int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                         const int* box, const float* xspac, const float* divB_errlim,
+                         float* bx, float* by, float* bz, const int* mode,
+                         const long long* max_iters, long long* iters, double* residual2) {
+    const Box B = box_of(nx, ny, nz);
+    float* w = (float*)g_divb_f.get(divb_work_floats(B.V()) * sizeof(float));
+    double* d = (double*)g_divb_d.get(divb_work_doubles(B.V()) * sizeof(double));
+    if (!w || !d) return -1;
+    long it = 0;
+    double r2 = 0.0;
+    const int st = divb_correct(B, S(bx, B, box), S(by, B, box), S(bz, B, box), xspac[*box - 1],
+                                *divB_errlim, mode ? *mode : kDivbRef,
+                                max_iters ? (long)*max_iters : 0, w, d, &it, &r2, g_stream);
+    check_launch("divb_correct");
+    if (iters) *iters = it;
+    if (residual2) *residual2 = r2;
+    return st;
+}
+
+void g3m3_divb_correct(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                       const int* box, const int* nbands, const int* divB_band_pos,
+                       const float* xspac, const float* divB_errlim, float* bx, float* by,
+                       float* bz) {
+    const int mode = kDivbRef;
+    const long long cap = 0;
+    g3m3_divb_correct_ex(nx, ny, nz, n_grids, box, xspac, divB_errlim, bx, by, bz, &mode, &cap,
+                         nullptr, nullptr);
+}
+
+void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qty,
+                        float* scdata) {
+    float* d = (float*)g_tmp.get(256);
+    if (!d) return;
+    float h[3 + 6 + 8];
+    std::memcpy(h, sxyz, 3 * sizeof(float));
+    std::memcpy(h + 3, gridpts, 6 * sizeof(float));
+    std::memcpy(h + 9, qty, 8 * sizeof(float));
+    hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, g_stream);
+    launch_trilin(1, d, d + 3, d + 9, d + 17, g_stream);
+    hipMemcpyAsync(scdata, d + 17, sizeof(float), hipMemcpyDeviceToHost, g_stream);
+    hipStreamSynchronize(g_stream);
+    check_launch("trilin_interp");
+}
+
+}  // extern "C"

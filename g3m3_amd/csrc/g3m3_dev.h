@@ -1,0 +1,53 @@
+// Device-side helpers shared by the G3M^3 hot-path kernels (gfx950).
+//
+// Layout: a field is the reference's Fortran array q(nx,ny,nz,n_grids),
+// column-major, i fastest.  A kernel always receives the base address of
+// one box slice, q + (box-1)*V, and indexes it with IX(i,j,k) (0-based).
+//
+// All arithmetic is IEEE single precision with no contraction into fused
+// multiply-adds (the library is built with -ffp-contract=off), and each
+// expression keeps the reference's left-to-right order, so a kernel gives
+// the same bits as a serial build of the reference subroutine it replaces.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace g3m3 {
+
+constexpr int kMaxFields = 34;
+
+struct Box {
+    int nx, ny, nz;
+    __host__ __device__ int V() const { return nx * ny * nz; }
+    __host__ __device__ int IX(int i, int j, int k) const { return i + nx * (j + ny * k); }
+    __device__ bool interior(int i, int j, int k) const {
+        return i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1;
+    }
+};
+
+// Flat thread id over a box -> (i,j,k); false past the end.
+__device__ inline bool cell_of(const Box& b, int gid, int& i, int& j, int& k) {
+    if (gid >= b.V()) return false;
+    i = gid % b.nx;
+    int t = gid / b.nx;
+    j = t % b.ny;
+    k = t / b.ny;
+    return true;
+}
+
+__device__ inline int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// A table of field pointers passed by value (kernel argument space).
+struct FieldPtrs {
+    float* p[kMaxFields];
+};
+
+__device__ inline float sgn1(float x) { return copysignf(1.0f, x); }
+
+// Non-negative float max via integer compare (maxima here are >= 0).
+__device__ inline void atomic_max_nonneg(float* addr, float v) {
+    if (!(v >= 0.0f)) return;
+    atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+
+}  // namespace g3m3

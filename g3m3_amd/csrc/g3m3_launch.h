@@ -1,0 +1,126 @@
+// Host-side launchers of the G3M^3 kernels and the pointer tables they take.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "g3m3_dev.h"
+
+namespace g3m3 {
+
+// one ion species: index 0 rho, 1..3 presx/y/z, 4..6 px/py/pz, 7..9 presxy/xz/yz
+struct IonPtrs {
+    float* out[10];
+    const float* old[10];
+    const float* wrk[10];
+    const float* bs[3];
+    const float* btot;
+    const float* e[3];
+    float* qv[3];
+    const float* tv[3];
+};
+
+struct SpeciesPtrs {
+    float* f[3][10];   // [species q,h,o][same index as IonPtrs]
+};
+
+struct SpeedParams {
+    float aniso, smallbit, vlim, cslim, iso_flag;
+    int isotropic;
+    float svlim[3], sclim[3];
+};
+
+struct LapPtrs {
+    const float* src[10];
+    float* dst[10];
+    float chi[10];
+};
+
+struct FctPtrs {
+    float* out[kMaxFields];
+    const float* old[kMaxFields];
+    const float* wrk[kMaxFields];
+    float chi[kMaxFields];
+};
+
+enum OuterKind { kRHO = 0, kPRES, kPX, kPY, kPZ, kOFF, kEP, kBX, kBY, kBZ };
+struct OuterPtrs {
+    float* f[kMaxFields];
+    int kind[kMaxFields];
+    float val[kMaxFields];
+    const float* b0[3];
+};
+
+// q(10), h(10), o(10), epres, bx, by, bz -- IonPtrs order within a species
+struct InnerPtrs {
+    float* f[kMaxFields];
+};
+
+struct FlankPtrs {
+    float* tgt[kMaxFields];
+    const float* cur[kMaxFields];
+    const float* old[kMaxFields];
+};
+
+// fine box (s*) and coarse box (b*) origins and spacings
+struct FlankGeom {
+    float sxmin, sx, bxmin, delx;
+    float symin, sy, bymin, dely;
+    float szmin, sz, bzmin, delz;
+};
+
+enum { kDivbRef = 0, kDivbF64 = 1 };
+enum { kScAnum = 0, kScGam = 1, kScTmp = 4 };
+
+void launch_calcur(const Box& B, const float* bx, const float* by, const float* bz, float* cx,
+                   float* cy, float* cz, float rx, float ry, float rz, hipStream_t s);
+void launch_bfield_evel(const Box& B, const float* b0[3], const float* b[3], float* bs[3],
+                        float* btot, const float* mom[9], const float* rho[3], const float* cur[3],
+                        float* ev[3], float* tv[3], float rmq, float rmh, float rmo,
+                        float reynolds, float smallbit, int do_b, int do_v, hipStream_t s);
+void launch_bande(const Box& B, float* e[3], const float* bs[3], const float* cur[3],
+                  const float* ev[3], const float* ep, const float* rho[3], const float* rst,
+                  float resist, float reynolds, int box, float rmq, float rmh, float rmo,
+                  const int* ijmid, int nummid, const int* ijzero, int numzero, int mbndry,
+                  float rx, float ry, float rz, float eta, hipStream_t s);
+void launch_push_elec(const Box& B, const float* w, const float* old, float* ep, const float* ev[3],
+                      float gamma, float gamma1, float delt, float rx, float ry, float rz,
+                      hipStream_t s);
+void launch_push_bfld(const Box& B, float* b[3], const float* old[3], const float* e[3], float delt,
+                      float rx, float ry, float rz, hipStream_t s);
+void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& src, hipStream_t s);
+void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, float grav,
+                     float re_equiv, float reynolds, const float gmin[3], const float gmax[3],
+                     hipStream_t s);
+void launch_set_rho(int n, const SpeciesPtrs& S, float* ep, hipStream_t s);
+void launch_set_speed(const Box& B, const SpeciesPtrs& S, const SpeedParams& Q, const float* b0[3],
+                      const float* b[3], float* btot, float* ep, float* maxima, void* list,
+                      int* count, hipStream_t s);
+size_t speed_list_bytes(int ncells);
+void launch_velocity(int n, const float* r, const float* p[3], float* v[3], float dmin,
+                     hipStream_t s);
+void launch_vtot(int n, const float* mom[9], const float* rho[3], float* v[3], hipStream_t s);
+void launch_lapidus(const Box& B, const LapPtrs& L, int nf, const float* v[3], float delt,
+                    hipStream_t s);
+void launch_fct(const Box& B, const FctPtrs& F, int nf, float* scratch, hipStream_t s);
+size_t fct_scratch_floats(int ncells, int nf);
+void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s);
+void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
+                        const int num[3], int mbndry, int box, float xmin, float ymin, float dx,
+                        float dy, float re_equiv, float v_rot, hipStream_t s);
+void launch_flanks(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
+                   float t2, hipStream_t s);
+void launch_corer(const Box& B, const FieldPtrs& fine, const FieldPtrs& coarse, int nf, int nskip,
+                  int nbx1, int nby1, int nbz1, hipStream_t s);
+void launch_flanks_synced(const Box& B, const FieldPtrs& fine, const FieldPtrs& coarse, int nf,
+                          const FlankGeom& G, const int kbi[6], hipStream_t s);
+int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float errlim, int mode,
+                 long max_iters, float* work, double* dwork, long* iters_out, double* res2_out,
+                 hipStream_t s);
+size_t divb_work_floats(int ncells);
+size_t divb_work_doubles(int ncells);
+void launch_banded_matvec(int npts, const int* pos, const float* M, const float* x, float* y,
+                          hipStream_t s);
+void launch_trilin(int npts, const float* sxyz, const float* gridpts, const float* qty,
+                   float* out, hipStream_t s);
+
+}  // namespace g3m3

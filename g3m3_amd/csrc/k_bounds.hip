@@ -1,0 +1,293 @@
+// Boundary conditions and nested-grid coupling:
+//   bndry_outer   (code/sub_bndry_outer.f90:5-731)  outermost-box faces/edges
+//   bndry_inner   (code/sub_bndry_inner.f90:5-268)  planet index lists
+//   flanks        (code/sub_flanks.f90:5-303)       coarse -> fine 2-layer shell
+//   corer         (code/sub_corer.f90:5-55)         fine -> coarse core
+//   flanks_synced (code/sub_flanks_synced.f90:5-141) coarse -> fine faces
+#include "g3m3_dev.h"
+#include "g3m3_launch.h"
+
+namespace g3m3 {
+
+// ---------------------------------------------------------------------------
+// bndry_outer as eight ordered passes over all fields (blockIdx.y = field).
+// Pass p covers a 2-D index range (a, b); see outer_pass_extent().
+__global__ void k_outer_pass(Box B, OuterPtrs O, int pass) {
+    const int f = blockIdx.y;
+    const int t = gtid();
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    float* q = O.f[f];
+    const int kind = O.kind[f];
+    auto at = [&](int i, int j, int k) -> float& { return q[B.IX(i, j, k)]; };
+    switch (pass) {
+    case 0: {  // inflow face i=1, all j,k (:50-103)
+        if (t >= ny * nz) return;
+        const int j = t % ny, k = t / ny;
+        if (kind >= kBX) at(0, j, k) = O.val[f] - O.b0[kind - kBX][B.IX(0, j, k)];
+        else at(0, j, k) = O.val[f];
+        break;
+    }
+    case 1: {  // back wall i=nx (:107-154)
+        if (t >= ny * nz) return;
+        const int j = t % ny, k = t / ny;
+        const float v = at(nx - 2, j, k);
+        at(nx - 1, j, k) = (kind == kPX) ? fabsf(v) : v;
+        break;
+    }
+    case 2: {  // y faces, i=2..nx, all k (:158-252)
+        if (t >= (nx - 1) * nz) return;
+        const int i = 1 + t % (nx - 1), k = t / (nx - 1);
+        const float lo = at(i, 1, k), hi = at(i, ny - 2, k);
+        at(i, 0, k) = (kind == kPY) ? -fabsf(lo) : lo;
+        at(i, ny - 1, k) = (kind == kPY) ? fabsf(hi) : hi;
+        break;
+    }
+    case 3: {  // z top, i=2..nx, all j (:256-307)
+        if (t >= (nx - 1) * ny) return;
+        const int i = 1 + t % (nx - 1), j = t / (nx - 1);
+        const float v = at(i, j, nz - 2);
+        at(i, j, nz - 1) = (kind == kPZ) ? fabsf(v) : v;
+        break;
+    }
+    case 4: {  // z bottom (:311-359)
+        if (t >= (nx - 1) * ny) return;
+        const int i = 1 + t % (nx - 1), j = t / (nx - 1);
+        const float v = at(i, j, 1);
+        at(i, j, 0) = (kind == kPZ) ? -fabsf(v) : v;
+        break;
+    }
+    case 5: {  // corner lines of the back wall, k=2..nz-1 (:363-466)
+        if (t >= nz - 2) return;
+        const int k = 1 + t;
+        if (kind == kBX) {
+            at(nx - 1, 0, k) = at(nx - 2, 0, k);
+            at(nx - 1, ny - 1, k) = at(nx - 2, ny - 1, k);
+        } else {
+            at(nx - 1, 0, k) = (at(nx - 2, 0, k) + at(nx - 1, 1, k)) / 2.0f;
+            at(nx - 1, ny - 1, k) = (at(nx - 2, ny - 1, k) + at(nx - 1, ny - 2, k)) / 2.0f;
+        }
+        break;
+    }
+    case 6: {  // top/bottom lines of the back wall, j=2..ny-1 (:470-566)
+        if (t >= ny - 2) return;
+        const int j = 1 + t;
+        at(nx - 1, j, 0) = at(nx - 1, j, 1);
+        if (kind != kEP) at(nx - 1, j, nz - 1) = at(nx - 2, j, nz - 1);
+        break;
+    }
+    case 7: {  // x-directed edge lines, i=2..nx (:570-728)
+        if (t >= nx - 1) return;
+        const int i = 1 + t;
+        at(i, 0, 0) = at(i, 0, 1);
+        at(i, ny - 1, 0) = at(i, ny - 1, 1);
+        at(i, 0, nz - 1) = at(i, 0, nz - 2);
+        at(i, ny - 1, nz - 1) = at(i, ny - 1, nz - 2);
+        break;
+    }
+    }
+}
+
+void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s) {
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const int ext[8] = {ny * nz, ny * nz, (nx - 1) * nz, (nx - 1) * ny, (nx - 1) * ny,
+                        nz - 2, ny - 2, nx - 1};
+    for (int p = 0; p < 8; ++p)
+        hipLaunchKernelGGL(k_outer_pass, dim3((ext[p] + 255) / 256, nf), dim3(256), 0, s, B, O, p);
+}
+
+// ---------------------------------------------------------------------------
+// bndry_inner.  which: 0 = surface (:78-140), 1 = atmosphere (:143-201),
+// 2 = interior (:204-265).  Field order in I.f: q,h,o x
+// (rho,presx,presy,presz,px,py,pz,presxy,presxz,presyz), epres, bx, by, bz.
+__global__ void k_inner_list(Box B, InnerPtrs I, int which, const int* __restrict__ ij,
+                             const float* __restrict__ parm, int num, int mbndry, int b,
+                             float xmin, float ymin, float dx, float dy, float re_equiv,
+                             float v_rot) {
+    const int n = gtid();
+    if (n >= num) return;
+    const int i = ij[b + mbndry * (0 + 3 * n)] - 1;
+    const int j = ij[b + mbndry * (1 + 3 * n)] - 1;
+    const int k = ij[b + mbndry * (2 + 3 * n)] - 1;
+    const int c = B.IX(i, j, k);
+    auto P = [&](int comp) { return parm[b + mbndry * (comp + 7 * n)]; };
+    const float ay = ymin + dy * (float)j;
+    const float ax = xmin + dx * (float)i;
+    const float rx = ax * re_equiv;
+    const float ry = ay * re_equiv;
+    const float vfrac = 1.0f;
+    const float rvy = rx * v_rot * vfrac;
+    const float rvx = -ry * v_rot * vfrac;
+    const float rvz = 0.0f;
+    float den[3];
+    for (int s = 0; s < 3; ++s) {
+        float* const* f = I.f + 10 * s;
+        den[s] = (which == 0) ? fmaxf(P(s), f[0][c]) : P(s);
+        f[0][c] = den[s];
+        f[1][c] = P(3 + s);
+        f[2][c] = P(3 + s);
+        f[3][c] = P(3 + s);
+        f[4][c] = den[s] * rvx;
+        f[5][c] = den[s] * rvy;
+        f[6][c] = den[s] * rvz;
+        f[7][c] = 0.0f;
+        f[8][c] = 0.0f;
+        f[9][c] = 0.0f;
+    }
+    I.f[30][c] = P(6);
+    if (which == 2) {
+        I.f[31][c] = 0.0f;
+        I.f[32][c] = 0.0f;
+        I.f[33][c] = 0.0f;
+    }
+}
+
+void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
+                        const int num[3], int mbndry, int box, float xmin, float ymin, float dx,
+                        float dy, float re_equiv, float v_rot, hipStream_t s) {
+    for (int w = 0; w < 3; ++w)
+        if (num[w] > 0)
+            hipLaunchKernelGGL(k_inner_list, dim3((num[w] + 255) / 256), dim3(256), 0, s, B, I, w,
+                               ij[w], parm[w], num[w], mbndry, box - 1, xmin, ymin, dx, dy,
+                               re_equiv, v_rot);
+}
+
+// ---------------------------------------------------------------------------
+// flanks: coarse box mb -> fine box ms, two layers on every face.
+// The time interpolation work = old*t1 + cur*t2 is evaluated at the eight
+// coarse corners of each shell cell instead of over the whole coarse box;
+// the panels' blends are applied in the reference's order (bottom, top,
+// front, back, left, right), so edge cells see the same sequence.
+__global__ void k_flanks(Box B, FlankPtrs Fp, FlankGeom G, float t1, float t2) {
+    int i, j, k;
+    const int f = blockIdx.y;
+    if (!cell_of(B, gtid(), i, j, k)) return;
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const bool bot = k <= 1, top = k >= nz - 2, fro = j <= 1, bac = j >= ny - 2, lef = i <= 1,
+               rig = i >= nx - 2;
+    if (!(bot || top || fro || bac || lef || rig)) return;
+    // coarse position of this fine cell (:50-55, 60-65, 68-73)
+    const float ax = G.sxmin + G.sx * (float)i;
+    const float ai = 1.0f + (ax - G.bxmin) / G.delx;
+    const int ib = (int)ai;
+    const float dx = ai - (float)ib, ddx = 1.0f - dx;
+    const float ay = G.symin + G.sy * (float)j;
+    const float aj = 1.0f + (ay - G.bymin) / G.dely;
+    const int jb = (int)aj;
+    const float dy = aj - (float)jb, ddy = 1.0f - dy;
+    const float az = G.szmin + G.sz * (float)k;
+    const float ak = 1.0f + (az - G.bzmin) / G.delz;
+    const int kb = (int)ak;
+    const float dz = ak - (float)kb, ddz = 1.0f - dz;
+    const float* __restrict__ o = Fp.old[f];
+    const float* __restrict__ q = Fp.cur[f];
+    auto W = [&](int ii, int jj, int kk) {
+        const int cc = B.IX(ii - 1, jj - 1, kk - 1);
+        return o[cc] * t1 + q[cc] * t2;
+    };
+    const float I0 = W(ib, jb, kb) * ddx * ddy * ddz + W(ib, jb, kb + 1) * ddx * ddy * dz +
+                     W(ib, jb + 1, kb) * ddx * dy * ddz + W(ib, jb + 1, kb + 1) * ddx * dy * dz +
+                     W(ib + 1, jb, kb) * dx * ddy * ddz + W(ib + 1, jb, kb + 1) * dx * ddy * dz +
+                     W(ib + 1, jb + 1, kb) * dx * dy * ddz +
+                     W(ib + 1, jb + 1, kb + 1) * dx * dy * dz;
+    float* tgt = Fp.tgt[f];
+    const int c = B.IX(i, j, k);
+    float w = tgt[c];
+    auto blend = [&](bool inner) {
+        const float frac = inner ? 0.5f : 0.0f;
+        const float frac1 = 1.0f - frac;
+        w = frac * w + frac1 * I0;
+    };
+    if (bot) blend(k == 1);
+    if (top) blend(k == nz - 2);
+    if (fro) blend(j == 1);
+    if (bac) blend(j == ny - 2);
+    if (lef) blend(i == 1);
+    if (rig) blend(i == nx - 2);
+    tgt[c] = w;
+}
+
+void launch_flanks(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
+                   float t2, hipStream_t s) {
+    const int n = B.V();
+    hipLaunchKernelGGL(k_flanks, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp, G, t1, t2);
+}
+
+// ---------------------------------------------------------------------------
+// corer: for every nskip-th fine cell, coarse(ib,jb,kb) = (6c + 6 nbrs)/12
+__global__ void k_corer(Box B, FieldPtrs fine, FieldPtrs coarse, int nskip, int nbx1, int nby1,
+                        int nbz1, int ni, int nj, int nk) {
+    const int f = blockIdx.y;
+    const int t = gtid();
+    if (t >= ni * nj * nk) return;
+    const int a = t % ni, bq = (t / ni) % nj, cq = t / (ni * nj);
+    const int is = 1 + nskip + a * nskip, js = 1 + nskip + bq * nskip, ks = 1 + nskip + cq * nskip;
+    const int ib = nbx1 + (is - 1) / nskip, jb = nby1 + (js - 1) / nskip,
+              kb = nbz1 + (ks - 1) / nskip;
+    const float* r = fine.p[f];
+    const int c = B.IX(is - 1, js - 1, ks - 1), X = 1, Y = B.nx, Z = B.nx * B.ny;
+    coarse.p[f][B.IX(ib - 1, jb - 1, kb - 1)] =
+        (6.0f * r[c] + r[c + X] + r[c - X] + r[c + Y] + r[c - Y] + r[c + Z] + r[c - Z]) / 12.0f;
+}
+
+void launch_corer(const Box& B, const FieldPtrs& fine, const FieldPtrs& coarse, int nf, int nskip,
+                  int nbx1, int nby1, int nbz1, hipStream_t s) {
+    // number of fine indices 1+nskip, 1+2*nskip, ... <= n-nskip (1-based)
+    auto cnt = [&](int n) { return (n - nskip - (1 + nskip)) / nskip + 1; };
+    const int ci = cnt(B.nx), cj = cnt(B.ny), ck = cnt(B.nz);
+    if (ci <= 0 || cj <= 0 || ck <= 0) return;
+    const int tot = ci * cj * ck;
+    hipLaunchKernelGGL(k_corer, dim3((tot + 255) / 256, nf), dim3(256), 0, s, B, fine, coarse,
+                       nskip, nbx1, nby1, nbz1, ci, cj, ck);
+}
+
+// ---------------------------------------------------------------------------
+// flanks_synced: a fine face cell takes the bilinear value of the last
+// panel that contains it (top/bottom, then front/back, then left/right).
+__global__ void k_flanks_synced(Box B, FieldPtrs fine, FieldPtrs coarse, FlankGeom G, int kb_top,
+                                int kb_bot, int jb_top, int jb_bot, int ib_top, int ib_bot) {
+    int i, j, k;
+    const int f = blockIdx.y;
+    if (!cell_of(B, gtid(), i, j, k)) return;
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const bool xf = (i == 0 || i == nx - 1), yf = (j == 0 || j == ny - 1),
+               zf = (k == 0 || k == nz - 1);
+    if (!(xf || yf || zf)) return;
+    const float ax = G.sxmin + G.sx * (float)i;
+    const float ai = 1.0f + (ax - G.bxmin) / G.delx;
+    const int ib = (int)ai;
+    const float dx = ai - (float)ib, ddx = 1.0f - dx;
+    const float ay = G.symin + G.sy * (float)j;
+    const float aj = 1.0f + (ay - G.bymin) / G.dely;
+    const int jb = (int)aj;
+    const float dy = aj - (float)jb, ddy = 1.0f - dy;
+    const float az = G.szmin + G.sz * (float)k;
+    const float ak = 1.0f + (az - G.bzmin) / G.delz;
+    const int kb = (int)ak;
+    const float dz = ak - (float)kb, ddz = 1.0f - dz;
+    const float* Bc = coarse.p[f];
+    auto at = [&](int ii, int jj, int kk) { return Bc[B.IX(ii - 1, jj - 1, kk - 1)]; };
+    float v;
+    if (xf) {
+        const int ibb = (i == nx - 1) ? ib_top : ib_bot;
+        v = at(ibb, jb, kb) * ddy * ddz + at(ibb, jb + 1, kb) * dy * ddz +
+            at(ibb, jb, kb + 1) * ddy * dz + at(ibb, jb + 1, kb + 1) * dy * dz;
+    } else if (yf) {
+        const int jbb = (j == ny - 1) ? jb_top : jb_bot;
+        v = at(ib, jbb, kb) * ddx * ddz + at(ib, jbb, kb + 1) * ddx * dz +
+            at(ib + 1, jbb, kb) * dx * ddz + at(ib + 1, jbb, kb + 1) * dx * dz;
+    } else {
+        const int kbb = (k == nz - 1) ? kb_top : kb_bot;
+        v = at(ib, jb, kbb) * ddx * ddy + at(ib, jb + 1, kbb) * ddx * dy +
+            at(ib + 1, jb, kbb) * dx * ddy + at(ib + 1, jb + 1, kbb) * dx * dy;
+    }
+    fine.p[f][B.IX(i, j, k)] = v;
+}
+
+void launch_flanks_synced(const Box& B, const FieldPtrs& fine, const FieldPtrs& coarse, int nf,
+                          const FlankGeom& G, const int kbi[6], hipStream_t s) {
+    const int n = B.V();
+    hipLaunchKernelGGL(k_flanks_synced, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, fine, coarse,
+                       G, kbi[0], kbi[1], kbi[2], kbi[3], kbi[4], kbi[5]);
+}
+
+}  // namespace g3m3

@@ -1,0 +1,284 @@
+"""Host driver of the MI355X hot path: the reference's box-step and
+full-step sequence (code/3d_main.f90:1925-3095) over the C-ABI.
+
+All per-cell arrays live in HBM for the whole run (torch tensors used as
+plain device allocations); only the small arrays the library reads on the
+host (grid extents, xspac, list counts, t_old/t_new) are numpy.  The state,
+``old`` and ``wrk`` sets are each one contiguous [34, n_grids, nz, ny, nx]
+allocation; a field is a contiguous view of it, laid out as the
+reference's Fortran ``q(nx,ny,nz,n_grids)``.
+
+One device->host transfer per full step: the CFL maxima of each box, read
+after its last validation pass (the reference reads them into
+``t_stepnew``, code/3d_main.f90:3038).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from . import signatures as sig
+from .workload import FIELDS, Workload
+
+F = np.float32
+SCRATCH3 = ("curx", "cury", "curz", "evx", "evy", "evz", "tvx", "tvy", "tvz",
+            "efldx", "efldy", "efldz", "bsx", "bsy", "bsz", "btot", "vvx", "vvy", "vvz")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.G3M3Error("no GPU visible: the G3M3 hot path runs only on the device")
+    return torch
+
+
+class Run:
+    """One nested-box run resident on the GPU."""
+
+    def __init__(self, wl: Workload):
+        torch = _torch()
+        self.torch = torch
+        self.wl = wl
+        self.nx, self.ny, self.nz, self.ngrd = wl.nx, wl.ny, wl.nz, wl.ngrd
+        dev = torch.device("cuda")
+        shp = (len(FIELDS), self.ngrd, self.nz, self.ny, self.nx)
+        self.sets = {}
+        a = {}
+        for pre in ("", "old", "wrk"):
+            t = torch.zeros(shp, dtype=torch.float32, device=dev)
+            self.sets[pre] = t
+            for f, k in enumerate(FIELDS):
+                a[pre + k] = t[f]
+        host = np.stack([wl.fields[k] for k in FIELDS])
+        self.sets[""].copy_(torch.from_numpy(host))
+        for k in ("bx0", "by0", "bz0", "rst"):
+            a[k] = torch.from_numpy(np.ascontiguousarray(wl.fields[k])).to(dev)
+        for k in SCRATCH3:
+            a[k] = torch.zeros((self.nz, self.ny, self.nx), dtype=torch.float32, device=dev)
+        for k, v in wl.grid.items():
+            a[k] = np.ascontiguousarray(v, dtype=F)
+        for k in ("ijsrf", "ijmid", "ijzero", "parm_srf", "parm_mid", "parm_zero"):
+            a[k] = torch.from_numpy(np.ascontiguousarray(wl.lists[k])).to(dev)
+        for k in ("numsrf", "nummid", "numzero"):
+            a[k] = np.ascontiguousarray(wl.lists[k], dtype=np.int32)
+        self.t_old = np.zeros(self.ngrd, np.float64)
+        self.t_new = np.zeros(self.ngrd, np.float64)
+        a["t_old"], a["t_new"] = self.t_old, self.t_new
+        for k in ("pxmax", "pymax", "pzmax", "csmax", "alfmax", "fastest"):
+            a[k] = np.zeros(1, F)
+        self.a = a
+        s = dict(wl.scal)
+        s.update(nx=self.nx, ny=self.ny, nz=self.nz, n_grids=self.ngrd,
+                 msrf=int(wl.lists["msrf"]), mmid=int(wl.lists["mmid"]),
+                 mzero=int(wl.lists["mzero"]))
+        self.s = s
+        self.t_step = np.zeros(self.ngrd, np.float64)
+        self.t_stepnew = np.zeros(self.ngrd, np.float64)
+        self.t = 0.0
+        self.cell_updates = 0
+        self.box_steps = 0
+        # host arrays of device pointers for the 34-field entry points
+        self._ptr_cache = {}
+        _lib.lib()
+
+    # -- plumbing -------------------------------------------------------------
+    def args(self, names):
+        out = []
+        for nm in names:
+            if isinstance(nm, str):
+                out.append(self.a[nm] if nm in self.a else self.s[nm])
+            else:
+                out.append(nm)
+        return out
+
+    def call(self, name, names):
+        _lib.call(name, *self.args(names))
+
+    def ptrs(self, pre, order):
+        key = (pre, tuple(order))
+        p = self._ptr_cache.get(key)
+        if p is None:
+            p = np.array([self.a[pre + k].data_ptr() for k in order], dtype=np.uint64)
+            self._ptr_cache[key] = p
+        return p
+
+    # -- pieces ------------------------------------------------------------------
+    def _efield(self, pre):
+        self.call("calcur", [pre + "bx", pre + "by", pre + "bz", "nx", "ny", "nz", "n_grids",
+                             "box", "curx", "cury", "curz", "rx", "ry", "rz"])
+        self.call("bfield_total", [pre + "bx", pre + "by", pre + "bz", "bx0", "by0", "bz0",
+                                   "bsx", "bsy", "bsz", "btot", "nx", "ny", "nz", "n_grids",
+                                   "box", "smallbit"])
+        self.call("fnd_evel", sig.fnd_evel(pre))
+        if float(self.s["eta"]) != 0.0:
+            self.call("bande_eta", sig.bande(pre, eta=True))
+        else:
+            self.call("bande", sig.bande(pre))
+
+    def _push(self, out, old, src, delt):
+        self.s["delt"] = F(delt)
+        self.call("push_elec", ["wrkepres", "oldepres", "epres", "evx", "evy", "evz", "gamma",
+                                "gamma1", "nx", "ny", "nz", "n_grids", "box", "delt",
+                                "rx", "ry", "rz"])
+        for sp, rm in zip("qho", ("rmassq", "rmassh", "rmasso")):
+            self.call("push_ion", sig.push_ion(out, old, src, sp, rm))
+        self.call("push_bfld", [out + "bx", out + "by", out + "bz", old + "bx", old + "by",
+                                old + "bz", "efldx", "efldy", "efldz", "nx", "ny", "nz",
+                                "n_grids", "box", "delt", "rx", "ry", "rz"])
+
+    def _bounds(self, pre, box, tgt, src, t_grid=None):
+        if box == self.ngrd:
+            self.call("bndry_outer", sig.bndry_outer(pre))
+        else:
+            _lib.call("bndry_flanks_v", self.ptrs(tgt, sig.FLANK_ORDER),
+                      self.ptrs(src, sig.FLANK_ORDER), self.ptrs("old", sig.FLANK_ORDER),
+                      self.a["vvx"], self.nx, self.ny, self.nz, self.ngrd, box,
+                      self.t_old, self.t_new, _lib.F64(t_grid), *[self.a[g] for g in sig.GRID])
+        if box <= self.s["mbndry"]:
+            self.call("bndry_inner", sig.bndry_inner(pre))
+
+    def _validate(self, pre, box):
+        self.call("set_rho", sig.set_rho(pre))
+        self.s["vlim"] = F(self.s["vlim_factor"]) * F(box)
+        self.call("set_speed_agrd", sig.set_speed(pre))
+
+    def _store_old(self, b):
+        # store_array x34 (code/3d_main.f90:2193-2229): one copy of box b
+        self.sets["old"][:, b].copy_(self.sets[""][:, b])
+
+    # -- code/3d_main.f90:2163-3065 ---------------------------------------------------
+    def box_step(self, box):
+        a, s = self.a, self.s
+        b = box - 1
+        self.t_old[b] = self.t_new[b]
+        self.t_new[b] = self.t_old[b] + self.t_step[b]
+        delt = F(self.t_step[b])
+        delt2 = delt / F(2.)
+        s["box"] = box
+        self._store_old(b)
+        s["rx"] = s["ry"] = s["rz"] = F(a["xspac"][b])
+
+        def clamp(tg):
+            if tg > self.t_new[b + 1]:
+                tg = self.t_new[b + 1]
+            if tg < self.t_old[b + 1]:
+                tg = self.t_old[b + 1]
+            return tg
+
+        # step 1
+        self._efield("")
+        self._push("wrk", "old", "", F(0.5) * delt)
+        tg = clamp(self.t_old[b] + float(delt2)) if box < self.ngrd else None
+        self._bounds("wrk", box, "wrk", "", tg)
+        self._validate("wrk", box)
+        # step 2
+        self._efield("wrk")
+        self._push("", "old", "wrk", delt)
+        tg = clamp(self.t_old[b] + float(delt)) if box < self.ngrd else None
+        self._bounds("", box, "wrk", "", tg)
+        self._validate("", box)
+        # Lapidus
+        s["delt"] = delt
+        for sp in "qho":
+            self.call("species_velocity", [sp + "rho", sp + "px", sp + "py", sp + "pz",
+                                           "curx", "cury", "curz", "nx", "ny", "nz",
+                                           "n_grids", "box", "smallbit"])
+            self.call("lap_plasma", sig.lap_plasma(sp, "", "wrk", ("curx", "cury", "curz")))
+        self.call("fnd_vtot", ["qpx", "qpy", "qpz", "qrho", "hpx", "hpy", "hpz", "hrho",
+                               "opx", "opy", "opz", "orho", "curx", "cury", "curz",
+                               "nx", "ny", "nz", "n_grids", "box", "rmassq", "rmassh", "rmasso"])
+        self.call("lap_elec", ["epres", "wrkepres", "curx", "cury", "curz", "nx", "ny", "nz",
+                               "n_grids", "box", "chirho", "chipxyz", "chierg", "delt"])
+        self.call("lap_bfld", ["bx", "by", "bz", "wrkbx", "wrkby", "wrkbz", "curx", "cury",
+                               "curz", "nx", "ny", "nz", "n_grids", "box", "chirho",
+                               "chipxyz", "chierg", "delt"])
+        self._bounds("wrk", box, "wrk", "", tg)
+        self._validate("wrk", box)
+        # FCT
+        self.call("flux_correct", sig.flux_correct())
+        self._bounds("", box, "", "", tg)
+        self._validate("", box)
+        fastest = a["fastest"][0]
+        self.t_stepnew[b] = min(F(s["stepsz"]) * F(a["xspac"][b]) / fastest, F(s["t_step_max"]))
+        if box < self.ngrd:
+            if abs(self.t_new[b] - self.t_new[b + 1]) <= 0.005 * self.t_step[b]:
+                _lib.call("bndry_corer_v", self.ptrs("", sig.CORER_ORDER), self.nx, self.ny,
+                          self.nz, self.ngrd, box, *[a[g] for g in sig.GRID])
+                self.t_old[b + 1] = self.t_new[b + 1]
+        self.cell_updates += self.nx * self.ny * self.nz
+        self.box_steps += 1
+
+    # -- code/3d_main.f90:1861-1903 --------------------------------------------------
+    def init_dt(self):
+        s = self.s
+        for box in range(self.ngrd, 0, -1):
+            b = box - 1
+            self.t_old[b] = 0.
+            self.t_new[b] = 0.
+            s["box"] = box
+            self._validate("", box)
+            self.t_stepnew[b] = min(F(s["stepsz"]) * F(self.a["xspac"][b])
+                                    / self.a["fastest"][0], F(s["t_step_max"]))
+        _lib.check("init_dt")
+
+    # -- code/3d_main.f90:1925-3095 ----------------------------------------------------
+    def full_step(self, dt_override=None):
+        n = self.ngrd
+        ts = self.t_step
+        smallest = F(0)
+        m_small = n
+        for box in range(n, 0, -1):
+            b = box - 1
+            ts[b] = self.t_stepnew[b]
+            self.t_old[b] = 0.
+            self.t_new[b] = 0.
+            if box == n:
+                smallest = F(ts[b])
+                m_small = box
+            else:
+                if ts[b] > ts[b + 1]:
+                    ts[b] = ts[b + 1]
+                if float(smallest) > ts[b]:
+                    smallest = F(ts[b])
+                    m_small = box
+        for box in range(1, n + 1):
+            b = box - 1
+            if box <= m_small:
+                ts[b] = float(smallest)
+            else:
+                astep = F(ts[b] / ts[b - 1] + 0.50)
+                nsteps = min(max(int(astep), 1), 2)
+                ts[b] = ts[b - 1] * nsteps
+        if dt_override is not None:
+            ts[:] = dt_override
+        m_step = int(ts[n - 1] / ts[m_small - 1])
+        self.t += float(F(ts[n - 1]))
+        for _ in range(m_step):
+            for box in range(n, 0, -1):
+                b = box - 1
+                yes = box == 1 or (self.t_old[b] == self.t_new[b]
+                                   and self.t_new[b] < ts[n - 1]
+                                   and abs(self.t_new[b] - self.t_new[0]) <= 0.005 * ts[0])
+                if yes:
+                    self.box_step(box)
+        self.t_old[0] = self.t_new[0]
+        for box in range(1, n):
+            self.s["box"] = box
+            _lib.call("bndry_corer_v", self.ptrs("", sig.CORER_ORDER), self.nx, self.ny,
+                      self.nz, self.ngrd, box, *[self.a[g] for g in sig.GRID])
+            self.call("set_rho", sig.set_rho(""))
+        _lib.check("full_step")
+
+    def divb(self, box, errlim, mode=0, max_iters=0):
+        """divB_correct on one box (code/sub_divB_correct.f90:25-175)."""
+        import ctypes
+        it = ctypes.c_longlong(0)
+        r2 = ctypes.c_double(0.0)
+        st = _lib.call("divb_correct_ex", self.nx, self.ny, self.nz, self.ngrd, box,
+                       self.a["xspac"], F(errlim), self.a["bx"], self.a["by"], self.a["bz"],
+                       int(mode), ctypes.c_longlong(max_iters), it, r2)
+        _lib.check("divb")
+        return st, it.value, r2.value
+
+    def state_host(self):
+        return {k: self.a[k].cpu().numpy() for k in FIELDS}

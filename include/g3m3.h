@@ -1,0 +1,339 @@
+/*
+ * g3m3.h -- C-ABI of the MI355X-native G3M^3 per-timestep hot path.
+ *
+ * Every entry point mirrors one reference subroutine: same name (prefixed
+ * g3m3_), same dummy-argument order and meaning, every argument passed by
+ * reference exactly as the reference's implicit interfaces receive them,
+ * so a Fortran caller binds each one with an ISO_C_BINDING interface
+ * (bind(c), no VALUE attributes) and keeps its call sites unchanged.
+ *
+ * Arrays:
+ *   - a field q(nx,ny,nz,n_grids) is float[n_grids][nz][ny][nx] (Fortran
+ *     column-major, i fastest); box-local scratch q(nx,ny,nz) likewise;
+ *   - pointers may be device pointers (hipMalloc / torch tensors) or host
+ *     addresses inside a range registered with g3m3_register(); the latter
+ *     are translated to their device mirror (the mirror is the data the
+ *     kernels read and write -- use g3m3_to_device / g3m3_to_host to move
+ *     values between a host array and its mirror).
+ *   - integer dummies are int32, real dummies float, real(dp) are double,
+ *     LOGICAL dummies are int32 (nonzero = .true.).
+ *
+ * Execution: calls are ordered on the library stream (g3m3_set_stream,
+ * default the null stream).  A call returns before its kernels finish,
+ * except calls with host scalar outputs (g3m3_set_speed_agrd,
+ * g3m3_divb_correct_ex), which synchronise.  No entry point returns a
+ * status: HIP and argument errors are latched and read with
+ * g3m3_last_error().  Not reentrant: one host thread per process.
+ *
+ * Arithmetic is IEEE single precision, evaluated in the reference's order
+ * with no fused multiply-add, so results agree with a serial build of the
+ * reference subroutine bit for bit (the parity tests check this), except
+ * where noted (divB dot products accumulate in double).
+ */
+#ifndef G3M3_H
+#define G3M3_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library control ---------------------------------------------------- */
+int g3m3_init(void);                          /* 0 on success; selects device 0 */
+int g3m3_last_error(void);                    /* 0 = none; clears the latch */
+const char* g3m3_last_error_msg(void);
+void g3m3_set_stream(void* hip_stream);       /* hipStream_t, NULL = null stream */
+void g3m3_sync(void);                         /* wait for the library stream */
+/* device mirrors of host arrays (drop-in use from an unchanged Fortran driver) */
+int g3m3_register(void* host_ptr, long long bytes);
+int g3m3_unregister(void* host_ptr);
+int g3m3_to_device(void* host_ptr);           /* host array -> its mirror */
+int g3m3_to_host(void* host_ptr);             /* mirror -> host array */
+
+/* ---- box-step pieces ------------------------------------------------------ */
+/* code/sub_store_array.f90:1 -- bx(:,:,:,box) = bx0(:,:,:,box) */
+void g3m3_store_array(float* bx, const float* bx0, const int* nx, const int* ny,
+                      const int* nz, const int* n_grids, const int* box);
+
+/* code/sub_calcur.f90:5-6 -- J = curl(b) with the reference's face rules */
+void g3m3_calcur(const float* bx, const float* by, const float* bz, const int* nx,
+                 const int* ny, const int* nz, const int* n_grids, const int* box,
+                 float* curx, float* cury, float* curz, const float* rx, const float* ry,
+                 const float* rz);
+
+/* code/sub_fnd_evel.f90:4-7 -- ion (tv) and electron (ev) bulk velocity */
+void g3m3_fnd_evel(const float* qpx, const float* qpy, const float* qpz, const float* qrho,
+                   const float* hpx, const float* hpy, const float* hpz, const float* hrho,
+                   const float* opx, const float* opy, const float* opz, const float* orho,
+                   const float* curx, const float* cury, const float* curz, float* evx,
+                   float* evy, float* evz, float* tvx, float* tvy, float* tvz, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box,
+                   const float* rmassq, const float* rmassh, const float* rmasso,
+                   const float* reynolds);
+
+/* code/sub_BandE.f90:5-10 -- Ohm's law E = -ve x B - grad pe/(n Re) + rst J/n */
+void g3m3_bande(float* efldx, float* efldy, float* efldz, const float* bsx, const float* bsy,
+                const float* bsz, const float* curx, const float* cury, const float* curz,
+                const float* evx, const float* evy, const float* evz, const float* btot,
+                const float* epres, const float* qrho, const float* hrho, const float* orho,
+                const float* rst, const float* resist, const float* reynolds, const int* nx,
+                const int* ny, const int* nz, const int* n_grids, const int* box,
+                const float* rmassq, const float* rmassh, const float* rmasso, const int* ijmid,
+                const int* nummid, const int* ijzero, const int* mbndry, const int* numzero,
+                const int* mmid, const int* mzero, const float* rx, const float* ry,
+                const float* rz);
+
+/* No reference counterpart: g3m3_bande plus a uniform resistivity eta*J/n
+ * on every interior cell (applied after the list term). */
+void g3m3_bande_eta(float* efldx, float* efldy, float* efldz, const float* bsx, const float* bsy,
+                    const float* bsz, const float* curx, const float* cury, const float* curz,
+                    const float* evx, const float* evy, const float* evz, const float* btot,
+                    const float* epres, const float* qrho, const float* hrho, const float* orho,
+                    const float* rst, const float* resist, const float* reynolds, const int* nx,
+                    const int* ny, const int* nz, const int* n_grids, const int* box,
+                    const float* rmassq, const float* rmassh, const float* rmasso,
+                    const int* ijmid, const int* nummid, const int* ijzero, const int* mbndry,
+                    const int* numzero, const int* mmid, const int* mzero, const float* rx,
+                    const float* ry, const float* rz, const float* eta);
+
+/* code/3d_main.f90:2248-2254 -- bs = b0 + b, btot = max(|bs|, smallbit) */
+void g3m3_bfield_total(const float* bx, const float* by, const float* bz, const float* bx0,
+                       const float* by0, const float* bz0, float* bsx, float* bsy, float* bsz,
+                       float* btot, const int* nx, const int* ny, const int* nz,
+                       const int* n_grids, const int* box, const float* smallbit);
+
+/* code/sub_push_elec.f90:4-5 */
+void g3m3_push_elec(const float* wrkepres, const float* oldepres, float* epres, const float* evx,
+                    const float* evy, const float* evz, const float* gamma, const float* gamma1,
+                    const int* nx, const int* ny, const int* nz, const int* n_grids,
+                    const int* box, const float* delt, const float* rx, const float* ry,
+                    const float* rz);
+
+/* code/sub_push_ion.f90:4-17 -- one ion species (qvx..qvz are outputs) */
+void g3m3_push_ion(float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx,
+                   float* qpy, float* qpz, const float* oldqrho, const float* oldqpresx,
+                   const float* oldqpresy, const float* oldqpresz, const float* oldqpx,
+                   const float* oldqpy, const float* oldqpz, const float* wrkqrho,
+                   const float* wrkqpresx, const float* wrkqpresy, const float* wrkqpresz,
+                   const float* wrkqpx, const float* wrkqpy, const float* wrkqpz,
+                   float* qpresxy, float* qpresxz, float* qpresyz, const float* oldqpresxy,
+                   const float* oldqpresxz, const float* oldqpresyz, const float* wrkqpresxy,
+                   const float* wrkqpresxz, const float* wrkqpresyz, const float* bsx,
+                   const float* bsy, const float* bsz, const float* btot, const float* efldx,
+                   const float* efldy, const float* efldz, const float* rmassq, float* qvx,
+                   float* qvy, float* qvz, const float* tvx, const float* tvy, const float* tvz,
+                   const float* gamma, const float* gamma1, const int* nx, const int* ny,
+                   const int* nz, const int* n_grids, const int* box, const float* delt,
+                   const float* grav, const float* re_equiv, const float* reynolds,
+                   const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+                   const float* grd_ymax, const float* grd_zmin, const float* grd_zmax,
+                   const int* isotropic);
+
+/* code/sub_push_bfld.f90:4-6 -- induction */
+void g3m3_push_bfld(float* bx, float* by, float* bz, const float* oldbx, const float* oldby,
+                    const float* oldbz, const float* efldx, const float* efldy, const float* efldz,
+                    const int* nx, const int* ny, const int* nz, const int* n_grids,
+                    const int* box, const float* delt, const float* rx, const float* ry,
+                    const float* rz);
+
+/* code/sub_set_rho.f90:5-11 -- density floors and negative-pressure reset */
+void g3m3_set_rho(float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpresxy,
+                  float* qpresxz, float* qpresyz, const float* rmassq, float* hrho, float* hpresx,
+                  float* hpresy, float* hpresz, float* hpresxy, float* hpresxz, float* hpresyz,
+                  const float* rmassh, float* orho, float* opresx, float* opresy, float* opresz,
+                  float* opresxy, float* opresxz, float* opresyz, const float* rmasso,
+                  float* epres, const int* nx, const int* ny, const int* nz, const int* n_grids,
+                  const int* box, const float* o_conc);
+
+/* code/sub_set_speed_agrd.f90:6-22 -- speed limits, averaging, CFL maxima */
+void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                         const int* box, const float* rmassq, const float* rmassh,
+                         const float* rmasso, const int* isotropic, const float* smallbit,
+                         const float* aniso_limit, const float* vlim, const float* qvlim,
+                         const float* hvlim, const float* ovlim, const float* qclim,
+                         const float* hclim, const float* oclim, const float* cslim,
+                         const float* bx, const float* by, const float* bz, const float* bx0,
+                         const float* by0, const float* bz0, float* qrho, float* qpx, float* qpy,
+                         float* qpz, float* qpresx, float* qpresy, float* qpresz, float* qpresxy,
+                         float* qpresxz, float* qpresyz, float* hrho, float* hpx, float* hpy,
+                         float* hpz, float* hpresx, float* hpresy, float* hpresz, float* hpresxy,
+                         float* hpresxz, float* hpresyz, float* orho, float* opx, float* opy,
+                         float* opz, float* opresx, float* opresy, float* opresz,
+                         float* opresxy, float* opresxz, float* opresyz, float* epres,
+                         float* btot, float* pxmax, float* pymax, float* pzmax, float* csmax,
+                         float* alfmax, float* fastest);
+
+/* code/sub_lap_plasma.f90:4-10 -- Lapidus smoothing, one species */
+void g3m3_lap_plasma(const float* rho, const float* px, const float* py, const float* pz,
+                     const float* presx, const float* presy, const float* presz,
+                     const float* presxy, const float* presxz, const float* presyz,
+                     float* wrkrho, float* wrkpx, float* wrkpy, float* wrkpz, float* wrkpresx,
+                     float* wrkpresy, float* wrkpresz, float* wrkpresxy, float* wrkpresxz,
+                     float* wrkpresyz, const float* vx, const float* vy, const float* vz,
+                     const int* nx, const int* ny, const int* nz, const int* n_grids,
+                     const int* box, const float* chirho, const float* chipxyz,
+                     const float* chierg, const float* delt, const int* isotropic);
+
+/* code/sub_lap_elec.f90:4-5 */
+void g3m3_lap_elec(const float* ppres, float* wrkppres, const float* vx, const float* vy,
+                   const float* vz, const int* nx, const int* ny, const int* nz,
+                   const int* n_grids, const int* box, const float* chirho,
+                   const float* chipxyz, const float* chierg, const float* delt);
+
+/* code/sub_lap_bfld.f90:4-5 */
+void g3m3_lap_bfld(const float* bx, const float* by, const float* bz, float* wrkbx, float* wrkby,
+                   float* wrkbz, const float* vx, const float* vy, const float* vz,
+                   const int* nx, const int* ny, const int* nz, const int* n_grids,
+                   const int* box, const float* chirho, const float* chipxyz,
+                   const float* chierg, const float* delt);
+
+/* code/sub_fnd_vtot.f90:4-6 */
+void g3m3_fnd_vtot(const float* qpx, const float* qpy, const float* qpz, const float* qrho,
+                   const float* hpx, const float* hpy, const float* hpz, const float* hrho,
+                   const float* opx, const float* opy, const float* opz, const float* orho,
+                   float* vx, float* vy, float* vz, const int* nx, const int* ny,
+                   const int* nz, const int* n_grids, const int* box, const float* rmassq,
+                   const float* rmassh, const float* rmasso);
+
+/* code/3d_main.f90:2698-2748 -- v = p/max(rho, smallbit) for Lapidus */
+void g3m3_species_velocity(const float* rho, const float* px, const float* py, const float* pz,
+                           float* vx, float* vy, float* vz, const int* nx, const int* ny,
+                           const int* nz, const int* n_grids, const int* box,
+                           const float* smallbit);
+
+/* code/sub_fcsmooth.f90:7-8 -- FCT on one field (tempx..tempz are workspace
+ * in the reference and are not written here) */
+void g3m3_fcsmooth(float* px, const float* oldpx, const float* wrkpx, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box,
+                   const float* chipx, float* tempx, float* tempy, float* tempz);
+
+/* code/sub_flux_correct.f90:4-35 -- FCT on all 34 fields, one launch set */
+void g3m3_flux_correct(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpresxy, float* qpresxz,
+    float* qpresyz, float* qpx, float* qpy, float* qpz, const float* wrkqrho,
+    const float* wrkqpresx, const float* wrkqpresy, const float* wrkqpresz,
+    const float* wrkqpresxy, const float* wrkqpresxz, const float* wrkqpresyz,
+    const float* wrkqpx, const float* wrkqpy, const float* wrkqpz, const float* oldqrho,
+    const float* oldqpresx, const float* oldqpresy, const float* oldqpresz,
+    const float* oldqpresxy, const float* oldqpresxz, const float* oldqpresyz,
+    const float* oldqpx, const float* oldqpy, const float* oldqpz, float* hrho, float* hpresx,
+    float* hpresy, float* hpresz, float* hpresxy, float* hpresxz, float* hpresyz, float* hpx,
+    float* hpy, float* hpz, const float* wrkhrho, const float* wrkhpresx,
+    const float* wrkhpresy, const float* wrkhpresz, const float* wrkhpresxy,
+    const float* wrkhpresxz, const float* wrkhpresyz, const float* wrkhpx, const float* wrkhpy,
+    const float* wrkhpz, const float* oldhrho, const float* oldhpresx, const float* oldhpresy,
+    const float* oldhpresz, const float* oldhpresxy, const float* oldhpresxz,
+    const float* oldhpresyz, const float* oldhpx, const float* oldhpy, const float* oldhpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opresxy, float* opresxz,
+    float* opresyz, float* opx, float* opy, float* opz, const float* wrkorho,
+    const float* wrkopresx, const float* wrkopresy, const float* wrkopresz,
+    const float* wrkopresxy, const float* wrkopresxz, const float* wrkopresyz,
+    const float* wrkopx, const float* wrkopy, const float* wrkopz, const float* oldorho,
+    const float* oldopresx, const float* oldopresy, const float* oldopresz,
+    const float* oldopresxy, const float* oldopresxz, const float* oldopresyz,
+    const float* oldopx, const float* oldopy, const float* oldopz, float* epres,
+    const float* wrkepres, const float* oldepres, float* bx, float* by, float* bz,
+    const float* wrkbx, const float* wrkby, const float* wrkbz, const float* oldbx,
+    const float* oldby, const float* oldbz, float* vvx, float* vvy, float* vvz, const int* nx,
+    const int* ny, const int* nz, const int* n_grids, const int* box, const float* difrho,
+    const float* diferg, const float* xspac, const int* isotropic);
+
+/* code/sub_bndry_outer.f90:5-16 -- outermost box (box = n_grids) */
+void g3m3_bndry_outer(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx, float* hpy, float* hpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opx, float* opy, float* opz,
+    float* epres, float* qpresxy, float* qpresxz, float* qpresyz, float* hpresxy,
+    float* hpresxz, float* hpresyz, float* opresxy, float* opresxz, float* opresyz,
+    const float* rmassq, const float* rmassh, const float* rmasso, float* bx, float* by,
+    float* bz, const float* bx0, const float* by0, const float* bz0, const int* nx,
+    const int* ny, const int* nz, const int* n_grids, const float* srho, const float* rho_frac,
+    const float* o_conc, const float* spress, const float* spx, const float* spy,
+    const float* spz, const float* sbx_wind, const float* sby_wind, const float* sbz_wind,
+    const float* ti_te, const int* isotropic);
+
+/* code/sub_bndry_inner.f90:5-19 -- planet lists on box mbndry.  v_rot and
+ * r_rot are the reference's common /rotation/ values (code/3d_main.f90:633). */
+void g3m3_bndry_inner(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    const float* rmassq, float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx,
+    float* hpy, float* hpz, const float* rmassh, float* orho, float* opresx, float* opresy,
+    float* opresz, float* opx, float* opy, float* opz, const float* rmasso, float* epres,
+    float* bx, float* by, float* bz, float* qpresxy, float* qpresxz, float* qpresyz,
+    float* hpresxy, float* hpresxz, float* hpresyz, float* opresxy, float* opresxz,
+    float* opresyz, const int* nx, const int* ny, const int* nz, const int* n_grids,
+    const float* parm_srf, const float* parm_mid, const float* parm_zero, const int* ijsrf,
+    const int* numsrf, const int* ijmid, const int* nummid, const int* ijzero,
+    const int* numzero, const int* mbndry, const int* msrf, const int* mmid, const int* mzero,
+    const float* erho, const float* epress, const float* alpha_e, const float* ti_te,
+    const float* o_conc, const float* re_equiv, const float* r_inner, const float* sbx_wind,
+    const float* spress, const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+    const float* grd_ymax, const float* grd_zmin, const float* grd_zmax, const float* v_rot,
+    const float* r_rot);
+
+/* code/sub_flanks.f90:5-7 -- coarse box+1 -> fine box shell, one field */
+void g3m3_flanks(float* wrkrho, const float* rho, const float* oldrho, const int* nx,
+                 const int* ny, const int* nz, const int* n_grids, const int* box,
+                 const double* t_new, const double* t_old, const double* t, float* work,
+                 const float* grd_xmin, const float* grd_xmax, const float* grd_ymin,
+                 const float* grd_ymax, const float* grd_zmin, const float* grd_zmax);
+
+/* code/sub_bndry_flanks.f90:5-35 -- flanks on all 34 fields, one launch.
+ * tgt/cur/old are arrays of 34 field pointers in the reference's argument
+ * order (q: rho,px,py,pz,presx,presy,presz,presxy,presxz,presyz; h; o;
+ * epres, bx, by, bz). */
+void g3m3_bndry_flanks_v(float* const* tgt, const float* const* cur, const float* const* old,
+                         float* work, const int* nx, const int* ny, const int* nz,
+                         const int* n_grids, const int* box, const double* t_old,
+                         const double* t_new, const double* t, const float* grd_xmin,
+                         const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                         const float* grd_zmin, const float* grd_zmax);
+
+/* code/sub_corer.f90:5-6 -- fine box -> coarse box+1 core, one field */
+void g3m3_corer(float* rho, const int* nx, const int* ny, const int* nz, const int* n_grids,
+                const int* box, const float* grd_xmin, const float* grd_xmax,
+                const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+                const float* grd_zmax);
+
+/* code/sub_bndry_corer.f90:5-14 -- corer on 34 fields given as a pointer
+ * array in the reference's argument order */
+void g3m3_bndry_corer_v(float* const* fields, const int* nx, const int* ny, const int* nz,
+                        const int* n_grids, const int* box, const float* grd_xmin,
+                        const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                        const float* grd_zmin, const float* grd_zmax);
+
+/* code/sub_flanks_synced.f90:5-6 -- coarse -> fine faces after divB */
+void g3m3_flanks_synced(float* rho, const int* nx, const int* ny, const int* nz,
+                        const int* n_grids, const int* box, const float* grd_xmin,
+                        const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                        const float* grd_zmin, const float* grd_zmax);
+
+/* code/sub_banded_tpose.f90:6 -- y = M^T x for the 7-band matrix M(7,npts) */
+void g3m3_banded_tpose(const int* nbands, const int* npts, const int* divB_band_pos,
+                       const float* banded_matrix, const float* x_vec, float* tp_result);
+
+/* code/sub_divB_correct.f90:25-27 -- divB clean-up by CGNR (matrix-free).
+ * The reference starts from an uninitialised solution vector; this starts
+ * from zero. */
+void g3m3_divb_correct(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                       const int* box, const int* nbands, const int* divB_band_pos,
+                       const float* xspac, const float* divB_errlim, float* bx, float* by,
+                       float* bz);
+
+/* No reference counterpart: g3m3_divb_correct with a precision mode
+ * (0 = single-precision scalars as the reference, 1 = double-precision
+ * scalars and updates), an iteration cap (0 = the reference's 10*npts), and
+ * the iteration count and final squared residual returned.  Returns the
+ * solver status: 1 converged, 2 stopped after restarts, 3 iteration cap. */
+int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int* n_grids,
+                         const int* box, const float* xspac, const float* divB_errlim,
+                         float* bx, float* by, float* bz, const int* mode,
+                         const long long* max_iters, long long* iters, double* residual2);
+
+/* code/cft_trilin_interp.f90:10 -- one point (sxyz(3), gridpts(3,2), qty(2,2,2)) */
+void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qty,
+                        float* scdata);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G3M3_H */

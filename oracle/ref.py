@@ -105,3 +105,32 @@ class RefLib:
             f = self.lib.omp_set_num_threads if hasattr(self.lib, "omp_set_num_threads") \
                 else ctypes.CDLL("libomp.so").omp_set_num_threads
             f(ctypes.c_int(n))
+
+
+def with_big_stack(fn, *args, stack_mb: int = 1024, **kw):
+    """Run ``fn`` on a thread with a large stack.
+
+    The reference keeps full-box automatic arrays on the stack (13 of them
+    in code/sub_set_speed_agrd.f90:60-61, 9 in code/sub_fcsmooth.f90:207-208,
+    plus the 7-band matrix of code/sub_divB_correct.f90:47-49); at
+    121x121x61 they overflow the default 8 MB main-thread stack."""
+    import threading
+    out, err = [], []
+
+    def body():
+        try:
+            out.append(fn(*args, **kw))
+        except BaseException as e:  # re-raised on the caller's thread
+            err.append(e)
+
+    old = threading.stack_size()
+    threading.stack_size(stack_mb << 20)
+    try:
+        t = threading.Thread(target=body)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if err:
+        raise err[0]
+    return out[0] if out else None
