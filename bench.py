@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X G3M^3 hot path: full steps of the nested
+121x121x61 workload, inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+A "step" is one full step of the reference's outer loop
+(code/3d_main.f90:1925-3095): dt synchronisation, every box-step the
+subcycling schedule asks for (31 for five levels at a CFL ratio of 2),
+nested-grid sync.  ``value`` = cell-updates/s = sum over executed
+box-steps of nx*ny*nz, divided by the wall time of the K timed steps.
+The divB clean-up (off-step in the reference, at checkpoints) is timed
+separately and reported in ``divb``.
+
+One JSON line is printed.  ``roofline`` is for the kernel named by
+--prof-kernel (timed with HIP events inside the library over the timed
+region); ``cpu_baseline`` is the reference's own OpenMP build timed on this
+host over one full step of the same workload (or, where that build is
+absent, the NumPy restatement on one box-step: kind "port").
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s, full step, 5 nested 121x121x61 grids; adjoint/forward cost"
+
+# Algorithmic HBM bytes per unit of each instrumented kernel (unit: one
+# interior cell, one field for the FCT kernels), counting each input and
+# output array once -- what the kernel must move if nothing is re-read.
+KERNEL_BYTES = {
+    # wrk 10 + old 10 + qv 3 + bs 3 + btot + E 3 + tv 3 reads, 10 writes
+    "k_push_ion": (43 * 4, "interior"),
+    # v 3 + 10 fields read, 10 written
+    "k_lapidus": (23 * 4, "interior"),
+    # per field: P and w read, 3 fluxes written
+    "k_fct_flux": (5 * 4, "all_x_fields"),
+    # per field: P, w, 3 fluxes read, out written
+    "k_fct_apply": (6 * 4, "all_x_fields"),
+    # per field: w, old read, P written
+    "k_fct_diffuse": (3 * 4, "all_x_fields"),
+    # 30 species fields read
+    "k_speed_flag": (30 * 4, "interior"),
+    "k_copy_fields": (2 * 4, "all_x_fields"),
+}
+
+
+def units_per_launch(kind, wl, nfields=34):
+    nx, ny, nz = wl.nx, wl.ny, wl.nz
+    if kind == "interior":
+        return (nx - 2) * (ny - 2) * (nz - 2)
+    return nx * ny * nz * nfields
+
+
+def cpu_baseline(workload: str, threads: int):
+    """The reference (OpenMP build) on one full step of the same workload."""
+    try:
+        from g3m3_amd.workload import make_workload
+        from oracle import ref
+        from oracle.boxstep import OracleRun
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "error": str(e)}
+    wl = make_workload(workload)
+    if ref.available(omp=True):
+        out = {}
+
+        def go():
+            r = OracleRun(wl, "ref_omp", threads=threads)
+            r.init_dt()
+            t0 = time.perf_counter()
+            r.full_step()
+            dt = time.perf_counter() - t0
+            out.update(cu=r.cell_updates, dt=dt, bs=r.box_steps)
+        ref.with_big_stack(go)
+        return {"value": out["cu"] / out["dt"], "unit": "cell-updates/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"1 full step ({out['bs']} box-steps, {out['cu']} cell-updates) of "
+                          f"{workload}, reference Fortran sub_*.f90 built with flang -O2 "
+                          f"-fopenmp, {threads} threads, {out['dt']:.1f} s"}
+    r = OracleRun(wl, "restate")
+    r.init_dt()
+    r.t_step[:] = r.t_stepnew
+    r.t_step[:] = np.minimum.accumulate(r.t_step[::-1])[::-1]
+    t0 = time.perf_counter()
+    r.box_step(wl.ngrd)
+    dt = time.perf_counter() - t0
+    return {"value": r.cell_updates / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 box-step of the outermost box of {workload} by the NumPy "
+                      f"restatement, {dt:.1f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="nested121")
+    ap.add_argument("--prof-kernel", default="k_push_ion")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-divb", action="store_true")
+    ap.add_argument("--cpu-threads", type=int,
+                    default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        return 2
+
+    import torch
+    from g3m3_amd import _lib
+    from g3m3_amd.driver import Run
+    from g3m3_amd.workload import make_workload
+
+    wl = make_workload(a.workload)
+    run = Run(wl)
+    run.init_dt()
+    for _ in range(a.warmup):
+        run.full_step()
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    L.g3m3_prof_select(a.prof_kernel.encode())
+    cu0, bs0 = run.cell_updates, run.box_steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run.full_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    import ctypes
+    ms = ctypes.c_double(0)
+    nl = ctypes.c_longlong(0)
+    L.g3m3_prof_read(ctypes.byref(ms), ctypes.byref(nl))
+    L.g3m3_prof_select(None)
+    _lib.check("bench")
+    cu = run.cell_updates - cu0
+    wall = t1 - t0
+    value = cu / wall
+    for k in ("qrho", "bx", "epres"):
+        v = run.a[k]
+        assert bool(torch.isfinite(v).all()), f"non-finite {k} after the timed steps"
+
+    roof = None
+    if nl.value and a.prof_kernel in KERNEL_BYTES:
+        per_unit, kind = KERNEL_BYTES[a.prof_kernel]
+        per_launch = per_unit * units_per_launch(kind, wl)
+        avg_s = ms.value / 1e3 / nl.value
+        achieved = per_launch / avg_s / 1e9
+        roof = {"kernel": a.prof_kernel, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": per_launch,
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl.value}
+
+    divb = None
+    if not a.no_divb:
+        # reference criterion: res^2 <= |b|^2 * npts * errlim^2; rel-residual 1e-6
+        npts = wl.nx * wl.ny * wl.nz
+        errlim = 1e-6 / np.sqrt(npts)
+        torch.cuda.synchronize()
+        d0 = time.perf_counter()
+        info = []
+        for box in range(1, wl.ngrd + 1):
+            st, it, r2 = run.divb(box, errlim, mode=1, max_iters=20000)
+            info.append({"box": box, "status": st, "iterations": it})
+        torch.cuda.synchronize()
+        dsec = time.perf_counter() - d0
+        per_step = wall / a.steps
+        divb = {"seconds_all_boxes": round(dsec, 3), "boxes": info, "rel_residual": 1e-6,
+                "mode": "fp32 matvec, fp64 dots/scalars",
+                "cell_updates_per_s_incl_divb_every_10_steps":
+                    cu / a.steps * 10 / (per_step * 10 + dsec)}
+
+    cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dipole + solar-wind initial state)",
+        "config": {"workload": f"{a.workload}: {wl.ngrd} nested {wl.nx}x{wl.ny}x{wl.nz} boxes, "
+                               f"spacing {list(map(float, wl.grid['xspac']))}, "
+                               f"eta={float(wl.scal['eta'])}",
+                   "box_steps": run.box_steps - bs0, "cell_updates": cu},
+        "roofline": roof, "cpu_baseline": cpu, "divb": divb,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/g3m3.h"
 #include "g3m3_launch.h"
@@ -83,9 +84,62 @@ T* S(T* base, const Box& B, const int* box) {
     return R(base) + off(B, box);
 }
 
+// --- per-kernel event timing ------------------------------------------------------
+std::string g_prof_name;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
+hipEvent_t g_prof_open = nullptr;
+double g_prof_ms = 0.0;
+long long g_prof_count = 0;
+
+void prof_drain() {
+    for (auto& e : g_prof_events) {
+        hipEventSynchronize(e.second);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        g_prof_ms += ms;
+        ++g_prof_count;
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    g_prof_events.clear();
+}
+
 }  // namespace
 
+namespace g3m3 {
+void prof_begin(const char* kernel, hipStream_t s) {
+    if (g_prof_name.empty() || g_prof_name != kernel) return;
+    hipEventCreate(&g_prof_open);
+    hipEventRecord(g_prof_open, s);
+}
+void prof_end(const char* kernel, hipStream_t s) {
+    if (g_prof_name.empty() || g_prof_name != kernel || !g_prof_open) return;
+    hipEvent_t stop;
+    hipEventCreate(&stop);
+    hipEventRecord(stop, s);
+    g_prof_events.emplace_back(g_prof_open, stop);
+    g_prof_open = nullptr;
+    if (g_prof_events.size() > 4096) prof_drain();
+}
+}  // namespace g3m3
+
 extern "C" {
+
+/* Select the kernel whose launches are timed with HIP events (NULL or "" =
+ * none) and reset the totals. */
+void g3m3_prof_select(const char* kernel) {
+    prof_drain();
+    g_prof_name = kernel ? kernel : "";
+    g_prof_ms = 0.0;
+    g_prof_count = 0;
+}
+
+/* Total milliseconds and launch count of the selected kernel so far. */
+void g3m3_prof_read(double* ms, long long* launches) {
+    prof_drain();
+    if (ms) *ms = g_prof_ms;
+    if (launches) *launches = g_prof_count;
+}
 
 int g3m3_init(void) {
     int n = 0;

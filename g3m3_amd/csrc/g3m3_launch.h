@@ -69,6 +69,17 @@ struct FlankGeom {
 };
 
 enum { kDivbRef = 0, kDivbF64 = 1 };
+
+// Optional per-kernel timing: when the kernel named by g3m3_prof_select()
+// is launched, HIP events are recorded around it on its stream.
+void prof_begin(const char* kernel, hipStream_t s);
+void prof_end(const char* kernel, hipStream_t s);
+struct ProfScope {
+    const char* k;
+    hipStream_t s;
+    ProfScope(const char* kernel, hipStream_t st) : k(kernel), s(st) { prof_begin(k, s); }
+    ~ProfScope() { prof_end(k, s); }
+};
 enum { kScAnum = 0, kScGam = 1, kScTmp = 4 };
 
 void launch_calcur(const Box& B, const float* bx, const float* by, const float* bz, float* cx,

@@ -209,7 +209,10 @@ __global__ void k_flanks(Box B, FlankPtrs Fp, FlankGeom G, float t1, float t2) {
 void launch_flanks(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
                    float t2, hipStream_t s) {
     const int n = B.V();
-    hipLaunchKernelGGL(k_flanks, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp, G, t1, t2);
+    {
+        ProfScope prof_("k_flanks", s);
+        hipLaunchKernelGGL(k_flanks, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp, G, t1, t2);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -319,7 +319,10 @@ __global__ void k_copy_fields(int n, FieldPtrs dst, FieldPtrs src) {
 
 void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& src, hipStream_t s) {
     if (nf <= 0) return;
-    hipLaunchKernelGGL(k_copy_fields, dim3((n + 255) / 256, nf), dim3(256), 0, s, n, dst, src);
+    {
+        ProfScope prof_("k_copy_fields", s);
+        hipLaunchKernelGGL(k_copy_fields, dim3((n + 255) / 256, nf), dim3(256), 0, s, n, dst, src);
+    }
 }
 
 }  // namespace g3m3

@@ -164,8 +164,11 @@ void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, fl
     const float ddx = (gmax[0] - gmin[0]) / (float)(B.nx - 1);
     const float ddy = (gmax[1] - gmin[1]) / (float)(B.ny - 1);
     const float ddz = (gmax[2] - gmin[2]) / (float)(B.nz - 1);
-    hipLaunchKernelGGL(k_push_ion, g, t, 0, s, B, P, rmass, delt, grav, re_equiv, reynolds, ddx,
+    {
+        ProfScope prof_("k_push_ion", s);
+        hipLaunchKernelGGL(k_push_ion, g, t, 0, s, B, P, rmass, delt, grav, re_equiv, reynolds, ddx,
                        ddy, ddz, gmin[0], gmin[1], gmin[2]);
+    }
 }
 
 }  // namespace g3m3

@@ -74,8 +74,11 @@ __global__ void k_lapidus(Box B, LapPtrs L, int nf, const float* __restrict__ vx
 void launch_lapidus(const Box& B, const LapPtrs& L, int nf, const float* v[3], float delt,
                     hipStream_t s) {
     const int n = B.V();
-    hipLaunchKernelGGL(k_lapidus, dim3((n + 255) / 256), dim3(256), 0, s, B, L, nf, v[0], v[1],
+    {
+        ProfScope prof_("k_lapidus", s);
+        hipLaunchKernelGGL(k_lapidus, dim3((n + 255) / 256), dim3(256), 0, s, B, L, nf, v[0], v[1],
                        v[2], delt);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -165,9 +168,18 @@ void launch_fct(const Box& B, const FctPtrs& F, int nf, float* scratch, hipStrea
     float* fny = scratch + (size_t)nf * fs;
     float* fnz = scratch + (size_t)2 * nf * fs;
     const dim3 g((n + 255) / 256, nf), t(256);
-    hipLaunchKernelGGL(k_fct_diffuse, g, t, 0, s, B, F, nf);
-    hipLaunchKernelGGL(k_fct_flux, g, t, 0, s, B, F, nf, fnx, fny, fnz, fs);
-    hipLaunchKernelGGL(k_fct_apply, g, t, 0, s, B, F, nf, fnx, fny, fnz, fs);
+    {
+        ProfScope prof_("k_fct_diffuse", s);
+        hipLaunchKernelGGL(k_fct_diffuse, g, t, 0, s, B, F, nf);
+    }
+    {
+        ProfScope prof_("k_fct_flux", s);
+        hipLaunchKernelGGL(k_fct_flux, g, t, 0, s, B, F, nf, fnx, fny, fnz, fs);
+    }
+    {
+        ProfScope prof_("k_fct_apply", s);
+        hipLaunchKernelGGL(k_fct_apply, g, t, 0, s, B, F, nf, fnx, fny, fnz, fs);
+    }
 }
 
 size_t fct_scratch_floats(int ncells, int nf) { return (size_t)3 * nf * ncells; }

@@ -194,8 +194,11 @@ void launch_set_speed(const Box& B, const SpeciesPtrs& S, const SpeedParams& Q, 
     const int n = B.V();
     hipLaunchKernelGGL(k_zero_floats, dim3(1), dim3(64), 0, s, maxima, 6);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, count, 1);
-    hipLaunchKernelGGL(k_speed_flag, dim3((n + 255) / 256), dim3(256), 0, s, B, S, Q,
+    {
+        ProfScope prof_("k_speed_flag", s);
+        hipLaunchKernelGGL(k_speed_flag, dim3((n + 255) / 256), dim3(256), 0, s, B, S, Q,
                        (AvgEntry*)list, count, maxima);
+    }
     hipLaunchKernelGGL(k_speed_apply, dim3(256), dim3(256), 0, s, S, (const AvgEntry*)list, count);
     hipLaunchKernelGGL(k_speed_elec, dim3((n + 255) / 256), dim3(256), 0, s, B, b0[0], b0[1], b0[2],
                        b[0], b[1], b[2], btot, S.f[0][0], S.f[1][0], S.f[2][0], ep, Q.smallbit,

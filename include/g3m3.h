@@ -43,6 +43,11 @@ int g3m3_last_error(void);                    /* 0 = none; clears the latch */
 const char* g3m3_last_error_msg(void);
 void g3m3_set_stream(void* hip_stream);       /* hipStream_t, NULL = null stream */
 void g3m3_sync(void);                         /* wait for the library stream */
+/* No reference counterpart: per-kernel timing for measurement harnesses.
+ * Launches of the named kernel are bracketed by HIP events on the stream
+ * they run on; g3m3_prof_read returns the summed time and launch count. */
+void g3m3_prof_select(const char* kernel);
+void g3m3_prof_read(double* ms, long long* launches);
 /* device mirrors of host arrays (drop-in use from an unchanged Fortran driver) */
 int g3m3_register(void* host_ptr, long long bytes);
 int g3m3_unregister(void* host_ptr);
