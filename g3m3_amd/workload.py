@@ -74,10 +74,21 @@ def _grid(L: int, ngrd: int, wind_adjust: float = 5.0 / 4.0):
     return g
 
 
+def _check_mating(g, ngrd):
+    """Each box's lower corner must fall on a grid plane of the next
+    coarser box (the reference stops otherwise, code/3d_main.f90:757-789)."""
+    for b in range(ngrd - 1):
+        for ax in "xyz":
+            d = 1.0 + (float(g[f"grd_{ax}min"][b]) - float(g[f"grd_{ax}min"][b + 1])) \
+                / float(g["xspac"][b + 1])
+            if abs(d - int(d)) > 0.001:
+                raise ValueError(f"box {b + 1} does not mate with box {b + 2} along {ax}")
+
+
 def make_workload(name: str = "nested121", seed: int = 20261203, **over) -> Workload:
     presets = {
-        "tiny": dict(L=6, ngrd=2, steps=3, divb_every=0, r_inner=1.6, eta=0.0),
-        "small": dict(L=10, ngrd=2, steps=4, divb_every=0, r_inner=2.2, eta=0.0),
+        "tiny": dict(L=8, ngrd=2, steps=3, divb_every=0, r_inner=1.8, eta=0.0),
+        "small": dict(L=12, ngrd=2, steps=4, divb_every=0, r_inner=2.2, eta=0.0),
         "single61": dict(L=30, ngrd=1, steps=100, divb_every=10, r_inner=3.0, eta=0.0),
         "nested121": dict(L=60, ngrd=5, steps=50, divb_every=10, r_inner=3.0, eta=1e-3),
     }
@@ -86,6 +97,7 @@ def make_workload(name: str = "nested121", seed: int = 20261203, **over) -> Work
     L, ngrd = int(p["L"]), int(p["ngrd"])
     nx, ny, nz = 2 * L + 1, 2 * L + 1, L + 1
     grid = _grid(L, ngrd)
+    _check_mating(grid, ngrd)
     rng = np.random.default_rng(seed)
 
     # --- upstream conditions (solar wind along +x in the simulation frame)

@@ -1,0 +1,175 @@
+"""Checks of the oracle that the reference comparison leaves open, from what
+the mathematics fixes:
+
+* the uniform-resistivity extension (no reference counterpart) equals the
+  reference's own list term with rst = eta on every interior cell;
+* the divB solve: the CGNR iterate solves the same 7-point Laplacian as a
+  direct sparse solve (scipy), and the correction removes the divergence;
+* the full box-step composition keeps a uniform state at rest exactly
+  uniform (a fixed point of every stage) -- a planted index or sign error in
+  the composition breaks it;
+* interpolation stencils are exact on linear fields.
+"""
+import numpy as np
+import pytest
+
+from common import World, GRID
+from oracle import restate as O
+
+F = np.float32
+
+
+def test_eta_extension_matches_reference_list_term():
+    """E += eta*J/n on the interior == BandE's rst*J/n on a list of every
+    interior cell with rst = eta (code/sub_BandE.f90:88-116)."""
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built")
+    w = World(11, 9, 8, 1, seed=4, lists=False)
+    nx, ny, nz = 11, 9, 8
+    eta = F(1e-3)
+    k, j, i = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), np.arange(1, nx - 1),
+                          indexing="ij")
+    n = i.size
+    lst = np.zeros((n, 3, 1), np.int32)
+    lst[:, 0, 0], lst[:, 1, 0], lst[:, 2, 0] = i.ravel() + 1, j.ravel() + 1, k.ravel() + 1
+    w.a.update(ijmid=lst, nummid=np.array([n], np.int32), ijzero=np.zeros((1, 3, 1), np.int32),
+               numzero=np.array([0], np.int32), rst=np.full((1, nz, ny, nx), eta, F))
+    w.s.update(box=1, mmid=n, mzero=1, rx=F(1.), ry=F(1.), rz=F(1.))
+    names = ["efldx", "efldy", "efldz", "bsx", "bsy", "bsz", "curx", "cury", "curz", "evx",
+             "evy", "evz", "btot", "epres", "qrho", "hrho", "orho", "rst", "resist", "reynolds",
+             "nx", "ny", "nz", "n_grids", "box", "rmassq", "rmassh", "rmasso", "ijmid", "nummid",
+             "ijzero", "mbndry", "numzero", "mmid", "mzero", "rx", "ry", "rz"]
+    wr = w.copy()
+    R.RefLib().call("BandE", *wr.args(names))
+    # the extension path: BandE without lists, then eta on the interior
+    we = w.copy()
+    we.a["nummid"] = np.array([0], np.int32)
+    O.BandE(*we.args(names))
+    C = O.C
+    eden = (we.a["qrho"][0][C] / F(1.) + we.a["hrho"][0][C] / F(1.)
+            + we.a["orho"][0][C] / F(16.))
+    for e, c in (("efldx", "curx"), ("efldy", "cury"), ("efldz", "curz")):
+        we.a[e][C] = we.a[e][C] + eta * we.a[c][C] / eden
+        E = we.a[e]
+        E[:, :, 0], E[:, :, -1] = E[:, :, 1], E[:, :, -2]
+    for e in ("efldx", "efldy", "efldz"):
+        E = we.a[e]
+        E[0], E[-1] = E[1], E[-2]
+    for e in ("efldx", "efldy", "efldz"):
+        E = we.a[e]
+        E[:, 0, :], E[:, -1, :] = E[:, 1, :], E[:, -2, :]
+    for e in ("efldx", "efldy", "efldz"):
+        assert np.array_equal(we.a[e], wr.a[e]), e
+
+
+def test_divb_solution_matches_direct_solve():
+    sp = pytest.importorskip("scipy.sparse")
+    spl = pytest.importorskip("scipy.sparse.linalg")
+    nx, ny, nz = 9, 8, 7
+    n = nx * ny * nz
+    M = O.divb_matrix(nx, ny, nz)
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny])
+    rows, cols, vals = [], [], []
+    for b in range(7):
+        for r in range(n):
+            c = r + pos[b]
+            if 0 <= c < n and M[c, b] != 0:
+                rows.append(r)
+                cols.append(c)
+                vals.append(float(M[c, b]))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    assert abs(A - A.T).max() == 0                        # symmetric, as banded_tpose assumes
+    rhs = np.random.default_rng(2).uniform(-1, 1, n).astype(F)
+    x = np.zeros(n, F)
+    r2 = np.zeros(1, F)
+    O.sparse_solver(7, n, pos, F(1e-6), False, 0, rhs, x, r2, M)
+    xd = spl.spsolve(A.tocsc(), rhs.astype(np.float64))
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-3
+
+
+def test_divb_correct_removes_smooth_divergence():
+    """The reference solves the compact 7-point Laplacian but corrects B with
+    wide (2-cell) central differences, so only smooth divergence is removed;
+    take B = grad(psi) for a smooth psi."""
+    nx, ny, nz = 25, 23, 21
+    w = World(nx, ny, nz, 1, seed=8, lists=False)
+    z, y, x = np.meshgrid(np.arange(nz) - nz // 2, np.arange(ny) - ny // 2,
+                          np.arange(nx) - nx // 2, indexing="ij")
+    s2 = 4.0 ** 2
+    psi = np.exp(-(x * x + y * y + z * z) / s2)
+    w.a["bx"][0] = (-2 * x / s2 * psi).astype(F)
+    w.a["by"][0] = (-2 * y / s2 * psi).astype(F)
+    w.a["bz"][0] = (-2 * z / s2 * psi).astype(F)
+    w.a["xspac"][0] = F(1.0)
+    X, Y, Z = w.a["bx"][0], w.a["by"][0], w.a["bz"][0]
+    XP, XM, YP, YM, ZP, ZM = O.XP, O.XM, O.YP, O.YM, O.ZP, O.ZM
+
+    def div():
+        return (X[XP] - X[XM] + Y[YP] - Y[YM] + Z[ZP] - Z[ZM]) / 2.0
+    d0 = np.linalg.norm(div())
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    O.divB_correct(nx, ny, nz, 1, 1, 7, pos, w.a["xspac"], F(1e-5), w.a["bx"], w.a["by"],
+                   w.a["bz"])
+    assert np.linalg.norm(div()) < 0.2 * d0
+
+
+def _uniform_run():
+    from g3m3_amd.workload import make_workload
+    from oracle.boxstep import OracleRun, FIELDS
+    wl = make_workload("tiny", ngrd=1)
+    sc = wl.scal
+    sc.update(spx=F(0.), sbx_wind=F(0.), sby_wind=F(0.), sbz_wind=F(-0.25), grav=F(0.))
+    ofrac = sc["rho_frac"] * sc["o_conc"]
+    pq = F(0.5) * sc["spress"]
+    val = {"qrho": sc["srho"], "hrho": sc["srho"] * sc["rho_frac"] * (sc["rmassh"] / sc["rmassq"]),
+           "orho": sc["srho"] * ofrac * (sc["rmasso"] / sc["rmassq"]),
+           "qpres": pq, "hpres": F(0.5) * sc["spress"] * sc["rho_frac"],
+           "opres": F(0.5) * sc["spress"] * ofrac, "epres": pq / sc["ti_te"]}
+    for k in FIELDS:
+        f = wl.fields[k]
+        if k in val:
+            f[...] = val[k]
+        elif k[1:5] == "pres" and len(k) == 6:
+            f[...] = val[k[0] + "pres"]
+        else:
+            f[...] = 0
+    wl.fields["bz"][...] = F(-0.25)
+    for c in ("bx0", "by0", "bz0"):
+        wl.fields[c][...] = 0
+    for nm in ("srf", "mid", "zero"):
+        wl.lists["num" + nm][...] = 0
+    return OracleRun(wl, "restate"), FIELDS
+
+
+def test_uniform_state_is_a_fixed_point_of_the_step():
+    r, fields = _uniform_run()
+    # the electron fluxes use the previous box-step's wrkepres
+    # (code/3d_main.f90:2271, 2517); start it where a previous step leaves it
+    r.a["wrkepres"][...] = r.a["epres"]
+    before = {k: r.a[k].copy() for k in fields}
+    r.init_dt()
+    for _ in range(2):
+        r.full_step()
+    for k in fields:
+        assert np.array_equal(r.a[k], before[k]), k
+
+
+def test_interpolation_exact_on_linear_fields():
+    # fine faces on coarse planes, as the reference's mating check requires
+    # (code/3d_main.f90:757-789)
+    w = World(13, 13, 9, 2, seed=1, lists=False)
+    xs = w.a["xspac"]
+    for b in range(2):
+        gx = w.a["grd_xmin"][b] + xs[b] * np.arange(13)
+        gy = w.a["grd_ymin"][b] + xs[b] * np.arange(13)
+        gz = w.a["grd_zmin"][b] + xs[b] * np.arange(9)
+        Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
+        w.a["bx"][b] = (0.25 * X - 0.5 * Y + 0.125 * Z + 1.0).astype(F)
+    fine = w.a["bx"][0].copy()
+    O.flanks_synced(w.a["bx"], 13, 13, 9, 2, 1, *[w.a[g] for g in GRID])
+    assert np.abs(w.a["bx"][0] - fine).max() < 1e-5
+    # corer is the (6c + sum of 6)/12 average: exact on linear fields
+    coarse = w.a["bx"][1].copy()
+    O.corer(w.a["bx"], 13, 13, 9, 2, 1, *[w.a[g] for g in GRID])
+    assert np.abs(w.a["bx"][1] - coarse).max() < 1e-5
