@@ -35,6 +35,18 @@ METRIC = "cell-updates/s, full step, 5 nested 121x121x61 grids; adjoint/forward 
 # interior cell, one field for the FCT kernels), counting each input and
 # output array once -- what the kernel must move if nothing is re-read.
 KERNEL_BYTES = {
+    # per interior cell, 3 species + fields: per species wrk 10 + old 10 +
+    # out 10; shared bs 3 + tv 3 + E 3 + ev 3, b old 3 + b out 3, epres
+    # wrk/old/out 3 -> 3*30 + 12 + 6 + 3
+    "k_push": (111 * 4, "interior"),
+    # per cell: 30 species fields + epres + b 3 + b0 3 read (maxima pass)
+    "k_validate": (37 * 4, "all"),
+    # per interior cell: 34 fields read, 34 written
+    "k_lap": (68 * 4, "interior"),
+    # per cell and field: w, old read, P written
+    "k_fct_diffuse2": (3 * 4, "all_x_fields"),
+    # per cell and field: P, w read, out written
+    "k_fct_limit": (3 * 4, "all_x_fields"),
     # wrk 10 + old 10 + qv 3 + bs 3 + btot + E 3 + tv 3 reads, 10 writes
     "k_push_ion": (43 * 4, "interior"),
     # v 3 + 10 fields read, 10 written
@@ -55,6 +67,8 @@ def units_per_launch(kind, wl, nfields=34):
     nx, ny, nz = wl.nx, wl.ny, wl.nz
     if kind == "interior":
         return (nx - 2) * (ny - 2) * (nz - 2)
+    if kind == "all":
+        return nx * ny * nz
     return nx * ny * nz * nfields
 
 
@@ -101,7 +115,10 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="nested121")
-    ap.add_argument("--prof-kernel", default="k_push_ion")
+    ap.add_argument("--prof-kernel", default="k_push")
+    ap.add_argument("--driver", choices=("native", "persub"), default="native",
+                    help="native: fused kernels issued by the C++ runtime; persub: one C-ABI "
+                         "call per reference subroutine")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-divb", action="store_true")
     ap.add_argument("--cpu-threads", type=int,
@@ -114,11 +131,11 @@ def main(argv=None):
 
     import torch
     from g3m3_amd import _lib
-    from g3m3_amd.driver import Run
+    from g3m3_amd.driver import FastRun, Run
     from g3m3_amd.workload import make_workload
 
     wl = make_workload(a.workload)
-    run = Run(wl)
+    run = (FastRun if a.driver == "native" else Run)(wl)
     run.init_dt()
     for _ in range(a.warmup):
         run.full_step()
@@ -180,6 +197,7 @@ def main(argv=None):
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded dipole + solar-wind initial state)",
+        "driver": a.driver,
         "config": {"workload": f"{a.workload}: {wl.ngrd} nested {wl.nx}x{wl.ny}x{wl.nz} boxes, "
                                f"spacing {list(map(float, wl.grid['xspac']))}, "
                                f"eta={float(wl.scal['eta'])}",

@@ -1,0 +1,101 @@
+// Parameter blocks of the fused box-step kernels (passed by value).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "g3m3_dev.h"
+#include "g3m3_launch.h"
+
+namespace g3m3 {
+
+// one ion species: 0 rho, 1..3 presx/y/z, 4..6 px/py/pz, 7..9 presxy/xz/yz
+struct SpeciesSet {
+    float* f[10];
+};
+
+struct EParams {
+    Box B;
+    const float* b[3];
+    const float* b0[3];
+    const float* mom[9];     // q px,py,pz; h px,py,pz; o px,py,pz
+    const float* rho[3];
+    const float* ep;
+    const float* rst;        // rst on BandE's list points (box <= mbndry), else null
+    float* e[3];
+    float* ev[3];
+    float* tv[3];
+    float* bs[3];
+    float rmq, rmh, rmo, reynolds, dxt, dyt, dzt, eta;
+};
+
+struct PushParams {
+    Box B;
+    SpeciesSet src[3], old[3], out[3];
+    const float* bs[3];
+    const float* tv[3];
+    const float* e[3];
+    const float* ev[3];
+    float* bout[3];
+    const float* bold[3];
+    const float* wrkepres;
+    const float* oldepres;
+    float* epres;
+    float rmass[3];
+    float delt, reynolds, smallbit;
+    float dxt, dyt, dzt;        // ions: 2 * grid spacing of the box extents
+    float fdxt, fdyt, fdzt;     // fields and electrons: 2 * xspac
+    float xmin, ymin, zmin, ddx, ddy, ddz, re_equiv, grav, gamma, gamma1;
+    int do_elec;
+};
+
+struct ValEntry {
+    int cell;
+    int species;
+    float v[10];
+};
+
+struct ValParams {
+    Box B;
+    SpeciesSet sp[3];
+    float* ep;
+    const float* b[3];
+    const float* b0[3];
+    ValEntry* list;
+    int* count;
+    float* partials;         // per-block maxima (null: not needed this pass)
+    float aniso, smallbit, vlim, cslim, iso_flag;
+    int isotropic;
+    float svlim[3], sclim[3];
+};
+
+struct LapParams {
+    Box B;
+    SpeciesSet src[3], dst[3];
+    const float* ep_src;
+    float* ep_dst;
+    const float* b_src[3];
+    float* b_dst[3];
+    float chirho, chipxyz, chierg, delt, smallbit;
+};
+
+struct FctParams {
+    Box B;
+    float* out[kMaxFields];
+    const float* old[kMaxFields];
+    const float* wrk[kMaxFields];
+    float chi[kMaxFields];
+    float* P;
+    size_t pstride;
+};
+
+void launch_efield(const EParams& P, hipStream_t s);
+void launch_push(const PushParams& P, hipStream_t s);
+void launch_validate(const ValParams& Q, float* maxima, hipStream_t s);
+int validate_blocks(const Box& B);
+void launch_lap(const LapParams& L, hipStream_t s);
+void launch_fct2(const FctParams& F, int nf, hipStream_t s);
+void launch_flanks_shell(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
+                         float t2, hipStream_t s);
+int shell_cells(const Box& B);
+
+}  // namespace g3m3

@@ -1,0 +1,559 @@
+// Native runtime of the hot path: one run (all boxes, all fields) resident
+// in HBM, advanced a full step per call with the fused kernels.
+//
+// The host logic restates code/3d_main.f90:1925-3095 -- dt
+// synchronisation across boxes, the subcycling schedule, the box-step call
+// sequence with the reference's boundary quirks, nested-grid sync -- and
+// issues only kernel launches; the one device->host read per full step is
+// the CFL maxima of each box, for t_stepnew (code/3d_main.f90:3038).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/g3m3.h"
+#include "g3m3_fused.h"
+
+using namespace g3m3;
+
+namespace {
+
+enum Fld {   // canonical field order
+    QRHO = 0, HRHO = 10, ORHO = 20, EPRES = 30, BX = 31, BY = 32, BZ = 33
+};
+constexpr int NF = 34;
+
+struct Run {
+    g3m3_run_desc d;
+    Box B;
+    int n;        // cells per box
+    hipStream_t s = nullptr;
+    // scratch (one box)
+    float* scr = nullptr;         // e(3) ev(3) tv(3) bs(3)
+    float* fctP = nullptr;        // 34 * n
+    ValEntry* list = nullptr;
+    int* count = nullptr;
+    float* partials = nullptr;
+    float* maxima = nullptr;      // 5 per box
+    float* rst_eff = nullptr;     // rst on BandE's list points of box mbndry
+    double t = 0.0;
+    std::vector<double> t_old, t_new, t_step, t_stepnew;
+    long long cell_updates = 0, box_steps = 0;
+    float iso_flag = 1.0f;
+    int err = 0;
+
+    float* fld(float* const* set, int f, int box) const { return set[f] + (size_t)(box - 1) * n; }
+};
+
+void check(Run* R, const char* where) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess && R->err == 0) {
+        R->err = (int)e;
+        fprintf(stderr, "g3m3 run: %s: %s\n", where, hipGetErrorString(e));
+    }
+}
+
+SpeciesSet species(const Run* R, float* const* set, int s, int box) {
+    SpeciesSet S;
+    for (int m = 0; m < 10; ++m) S.f[m] = R->fld(set, 10 * s + m, box);
+    return S;
+}
+
+// -- stage pieces ---------------------------------------------------------------
+void efield(Run* R, float* const* src, int box) {
+    const g3m3_run_desc& d = R->d;
+    EParams P;
+    P.B = R->B;
+    for (int c = 0; c < 3; ++c) {
+        P.b[c] = R->fld(src, BX + c, box);
+        P.b0[c] = (c == 0 ? d.bx0 : c == 1 ? d.by0 : d.bz0) + (size_t)(box - 1) * R->n;
+        P.rho[c] = R->fld(src, 10 * c, box);
+        for (int m = 0; m < 3; ++m) P.mom[3 * c + m] = R->fld(src, 10 * c + 4 + m, box);
+        P.e[c] = R->scr + (size_t)(0 + c) * R->n;
+        P.ev[c] = R->scr + (size_t)(3 + c) * R->n;
+        P.tv[c] = R->scr + (size_t)(6 + c) * R->n;
+        P.bs[c] = R->scr + (size_t)(9 + c) * R->n;
+    }
+    P.ep = R->fld(src, EPRES, box);
+    P.rst = (box <= d.mbndry && d.resist < 5000.0f) ? R->rst_eff : nullptr;
+    P.rmq = d.rmassq;
+    P.rmh = d.rmassh;
+    P.rmo = d.rmasso;
+    P.reynolds = d.reynolds;
+    const float rx = d.xspac[box - 1];
+    P.dxt = 2.0f * rx;
+    P.dyt = 2.0f * rx;
+    P.dzt = 2.0f * rx;
+    P.eta = d.eta;
+    launch_efield(P, R->s);
+}
+
+void push(Run* R, float* const* out, float* const* old, float* const* src, int box, float delt,
+          int do_elec) {
+    const g3m3_run_desc& d = R->d;
+    PushParams P;
+    P.B = R->B;
+    for (int s = 0; s < 3; ++s) {
+        P.src[s] = species(R, src, s, box);
+        P.old[s] = species(R, old, s, box);
+        P.out[s] = species(R, out, s, box);
+    }
+    for (int c = 0; c < 3; ++c) {
+        P.e[c] = R->scr + (size_t)(0 + c) * R->n;
+        P.ev[c] = R->scr + (size_t)(3 + c) * R->n;
+        P.tv[c] = R->scr + (size_t)(6 + c) * R->n;
+        P.bs[c] = R->scr + (size_t)(9 + c) * R->n;
+        P.bout[c] = R->fld(out, BX + c, box);
+        P.bold[c] = R->fld(old, BX + c, box);
+    }
+    P.wrkepres = R->fld(R->d.wrk, EPRES, box);
+    P.oldepres = R->fld(old, EPRES, box);
+    P.epres = R->fld(out, EPRES, box);
+    P.rmass[0] = d.rmassq;
+    P.rmass[1] = d.rmassh;
+    P.rmass[2] = d.rmasso;
+    P.delt = delt;
+    P.reynolds = d.reynolds;
+    P.smallbit = d.smallbit;
+    const int b = box - 1;
+    P.ddx = (d.grd_xmax[b] - d.grd_xmin[b]) / (float)(d.nx - 1);
+    P.ddy = (d.grd_ymax[b] - d.grd_ymin[b]) / (float)(d.ny - 1);
+    P.ddz = (d.grd_zmax[b] - d.grd_zmin[b]) / (float)(d.nz - 1);
+    P.dxt = 2.0f * P.ddx;
+    P.dyt = 2.0f * P.ddy;
+    P.dzt = 2.0f * P.ddz;
+    const float rx = d.xspac[b];
+    P.fdxt = 2.0f * rx;
+    P.fdyt = 2.0f * rx;
+    P.fdzt = 2.0f * rx;
+    P.xmin = d.grd_xmin[b];
+    P.ymin = d.grd_ymin[b];
+    P.zmin = d.grd_zmin[b];
+    P.re_equiv = d.re_equiv;
+    P.grav = d.grav;
+    P.gamma = d.gamma;
+    P.gamma1 = d.gamma1;
+    P.do_elec = do_elec;
+    launch_push(P, R->s);
+}
+
+void outer_bc(Run* R, float* const* set) {
+    const g3m3_run_desc& d = R->d;
+    const int box = d.n_grids;
+    const float ofrac = d.rho_frac * d.o_conc;
+    const float frac_h = d.rmassh / d.rmassq;
+    const float frac_o = d.rmasso / d.rmassq;
+    const float pq = 0.5f * d.spress;
+    OuterPtrs O{};
+    const float rf = d.rho_frac;
+    const float sval[3][10] = {
+        {d.srho, pq, pq, pq, d.spx, d.spy, d.spz, 0.f, 0.f, 0.f},
+        {d.srho * rf * frac_h, 0.5f * d.spress * rf, 0.5f * d.spress * rf, 0.5f * d.spress * rf,
+         d.spx * rf * frac_h, d.spy * rf * frac_h, d.spz * rf * frac_h, 0.f, 0.f, 0.f},
+        {d.srho * ofrac * frac_o, 0.5f * d.spress * ofrac, 0.5f * d.spress * ofrac,
+         0.5f * d.spress * ofrac, d.spx * ofrac * frac_o, d.spy * ofrac * frac_o,
+         d.spz * ofrac * frac_o, 0.f, 0.f, 0.f}};
+    const int kinds[10] = {kRHO, kPRES, kPRES, kPRES, kPX, kPY, kPZ, kOFF, kOFF, kOFF};
+    for (int s = 0; s < 3; ++s)
+        for (int m = 0; m < 10; ++m) {
+            O.f[10 * s + m] = R->fld(set, 10 * s + m, box);
+            O.kind[10 * s + m] = kinds[m];
+            O.val[10 * s + m] = sval[s][m];
+        }
+    O.f[EPRES] = R->fld(set, EPRES, box);
+    O.kind[EPRES] = kEP;
+    O.val[EPRES] = pq / d.ti_te;
+    const float sb[3] = {d.sbx_wind, d.sby_wind, d.sbz_wind};
+    for (int c = 0; c < 3; ++c) {
+        O.f[BX + c] = R->fld(set, BX + c, box);
+        O.kind[BX + c] = kBX + c;
+        O.val[BX + c] = sb[c];
+    }
+    O.b0[0] = d.bx0 + (size_t)(box - 1) * R->n;
+    O.b0[1] = d.by0 + (size_t)(box - 1) * R->n;
+    O.b0[2] = d.bz0 + (size_t)(box - 1) * R->n;
+    launch_bndry_outer(R->B, O, NF, R->s);
+}
+
+FlankGeom geom(const g3m3_run_desc& d, int box) {
+    const int mb = box, ms = box - 1;
+    FlankGeom G;
+    G.delx = (d.grd_xmax[mb] - d.grd_xmin[mb]) / (d.nx - 1.0f);
+    G.dely = (d.grd_ymax[mb] - d.grd_ymin[mb]) / (d.ny - 1.0f);
+    G.delz = (d.grd_zmax[mb] - d.grd_zmin[mb]) / (d.nz - 1.0f);
+    G.sx = (d.grd_xmax[ms] - d.grd_xmin[ms]) / (d.nx - 1.0f);
+    G.sy = (d.grd_ymax[ms] - d.grd_ymin[ms]) / (d.ny - 1.0f);
+    G.sz = (d.grd_zmax[ms] - d.grd_zmin[ms]) / (d.nz - 1.0f);
+    G.sxmin = d.grd_xmin[ms];
+    G.symin = d.grd_ymin[ms];
+    G.szmin = d.grd_zmin[ms];
+    G.bxmin = d.grd_xmin[mb];
+    G.bymin = d.grd_ymin[mb];
+    G.bzmin = d.grd_zmin[mb];
+    return G;
+}
+
+void flanks(Run* R, float* const* tgt, int box, double t_grid) {
+    const int mb = box;   // 0-based index of box+1
+    const float t1 = (float)((R->t_new[mb] - t_grid) / (R->t_new[mb] - R->t_old[mb]));
+    const float t2 = 1.0f - t1;
+    FlankPtrs P{};
+    for (int f = 0; f < NF; ++f) {
+        P.tgt[f] = R->fld(tgt, f, box);
+        P.cur[f] = R->fld(R->d.state, f, box + 1);
+        P.old[f] = R->fld(R->d.old, f, box + 1);
+    }
+    launch_flanks_shell(R->B, P, NF, geom(R->d, box), t1, t2, R->s);
+}
+
+void inner_bc(Run* R, float* const* set) {
+    const g3m3_run_desc& d = R->d;
+    const int box = d.mbndry;
+    InnerPtrs I{};
+    // InnerPtrs order: q,h,o x (rho,presx,presy,presz,px,py,pz,presxy,presxz,presyz),
+    // epres, bx, by, bz -- the canonical order
+    for (int f = 0; f < NF; ++f) I.f[f] = R->fld(set, f, box);
+    const int* ij[3] = {d.ijsrf, d.ijmid, d.ijzero};
+    const float* parm[3] = {d.parm_srf, d.parm_mid, d.parm_zero};
+    const int num[3] = {d.numsrf, d.nummid, d.numzero};
+    const int b = box - 1;
+    const float dx = (d.grd_xmax[b] - d.grd_xmin[b]) / (float)(d.nx - 1.0f);
+    const float dy = (d.grd_ymax[b] - d.grd_ymin[b]) / (float)(d.ny - 1.0f);
+    launch_bndry_inner(R->B, I, ij, parm, num, d.mbndry, box, d.grd_xmin[b], d.grd_ymin[b], dx, dy,
+                       d.re_equiv, d.v_rot, R->s);
+}
+
+void validate(Run* R, float* const* set, int box, bool want_max) {
+    const g3m3_run_desc& d = R->d;
+    ValParams Q;
+    Q.B = R->B;
+    for (int s = 0; s < 3; ++s) Q.sp[s] = species(R, set, s, box);
+    Q.ep = R->fld(set, EPRES, box);
+    for (int c = 0; c < 3; ++c) {
+        Q.b[c] = R->fld(set, BX + c, box);
+        Q.b0[c] = (c == 0 ? d.bx0 : c == 1 ? d.by0 : d.bz0) + (size_t)(box - 1) * R->n;
+    }
+    Q.list = R->list;
+    Q.count = R->count;
+    Q.partials = want_max ? R->partials : nullptr;
+    if (d.isotropic) R->iso_flag = 0.0f;
+    Q.aniso = d.aniso_limit;
+    Q.smallbit = d.smallbit;
+    Q.vlim = d.vlim_factor * (float)box;
+    Q.cslim = d.cslim;
+    Q.iso_flag = R->iso_flag;
+    Q.isotropic = d.isotropic ? 1 : 0;
+    Q.svlim[0] = d.qvlim;
+    Q.svlim[1] = d.hvlim;
+    Q.svlim[2] = d.ovlim;
+    Q.sclim[0] = d.qclim;
+    Q.sclim[1] = d.hclim;
+    Q.sclim[2] = d.oclim;
+    launch_validate(Q, want_max ? R->maxima + 5 * (box - 1) : nullptr, R->s);
+}
+
+void set_rho_only(Run* R, float* const* set, int box) {
+    SpeciesPtrs S{};
+    for (int s = 0; s < 3; ++s)
+        for (int m = 0; m < 10; ++m) S.f[s][m] = R->fld(set, 10 * s + m, box);
+    launch_set_rho(R->n, S, R->fld(set, EPRES, box), R->s);
+}
+
+void bounds(Run* R, float* const* set, float* const* flank_tgt, int box, double t_grid) {
+    if (box == R->d.n_grids) outer_bc(R, set);
+    else flanks(R, flank_tgt, box, t_grid);
+    if (box <= R->d.mbndry) inner_bc(R, set);
+}
+
+void lapidus(Run* R, int box, float delt) {
+    const g3m3_run_desc& d = R->d;
+    LapParams L;
+    L.B = R->B;
+    for (int s = 0; s < 3; ++s) {
+        L.src[s] = species(R, d.state, s, box);
+        L.dst[s] = species(R, d.wrk, s, box);
+    }
+    L.ep_src = R->fld(d.state, EPRES, box);
+    L.ep_dst = R->fld(d.wrk, EPRES, box);
+    for (int c = 0; c < 3; ++c) {
+        L.b_src[c] = R->fld(d.state, BX + c, box);
+        L.b_dst[c] = R->fld(d.wrk, BX + c, box);
+    }
+    L.chirho = d.chirho;
+    L.chipxyz = d.chipxyz;
+    L.chierg = d.chierg;
+    L.delt = delt;
+    L.smallbit = d.smallbit;
+    launch_lap(L, R->s);
+}
+
+void fct(Run* R, int box) {
+    const g3m3_run_desc& d = R->d;
+    FctParams F;
+    F.B = R->B;
+    for (int f = 0; f < NF; ++f) {
+        F.out[f] = R->fld(d.state, f, box);
+        F.old[f] = R->fld(d.old, f, box);
+        F.wrk[f] = R->fld(d.wrk, f, box);
+        F.chi[f] = (f >= BX) ? d.diferg : d.difrho;
+    }
+    F.P = R->fctP;
+    F.pstride = (size_t)R->n;
+    launch_fct2(F, NF, R->s);
+}
+
+void corer(Run* R, int box) {
+    const g3m3_run_desc& d = R->d;
+    const int mb = box, ms = box - 1;
+    const float sx = (d.grd_xmax[ms] - d.grd_xmin[ms]) / (d.nx - 1.0f);
+    const float dx = (d.grd_xmax[mb] - d.grd_xmin[mb]) / (d.nx - 1.0f);
+    const float dy = (d.grd_ymax[mb] - d.grd_ymin[mb]) / (d.ny - 1.0f);
+    const float dz = (d.grd_zmax[mb] - d.grd_zmin[mb]) / (d.nz - 1.0f);
+    const int nskip = (int)(dx / sx);
+    const int nbx1 = (int)(1.0f + (d.grd_xmin[ms] - d.grd_xmin[mb]) / dx + 0.5f);
+    const int nby1 = (int)(1.0f + (d.grd_ymin[ms] - d.grd_ymin[mb]) / dy + 0.5f);
+    const int nbz1 = (int)(1.0f + (d.grd_zmin[ms] - d.grd_zmin[mb]) / dz + 0.5f);
+    FieldPtrs fine{}, coarse{};
+    for (int f = 0; f < NF; ++f) {
+        fine.p[f] = R->fld(d.state, f, box);
+        coarse.p[f] = R->fld(d.state, f, box + 1);
+    }
+    launch_corer(R->B, fine, coarse, NF, nskip, nbx1, nby1, nbz1, R->s);
+}
+
+// -- code/3d_main.f90:2163-3065 ----------------------------------------------------
+void box_step(Run* R, int box) {
+    g3m3_run_desc& d = R->d;
+    const int b = box - 1;
+    R->t_old[b] = R->t_new[b];
+    R->t_new[b] = R->t_old[b] + R->t_step[b];
+    const float delt = (float)R->t_step[b];
+    const float delt2 = delt / 2.0f;
+    // store_array x34 (:2193-2229)
+    {
+        FieldPtrs dst{}, src{};
+        for (int f = 0; f < NF; ++f) {
+            dst.p[f] = R->fld(d.old, f, box);
+            src.p[f] = R->fld(d.state, f, box);
+        }
+        launch_copy_fields(R->n, NF, dst, src, R->s);
+    }
+    auto clamp_t = [&](double tg) {
+        if (tg > R->t_new[b + 1]) tg = R->t_new[b + 1];
+        if (tg < R->t_old[b + 1]) tg = R->t_old[b + 1];
+        return tg;
+    };
+    const bool inner_box = box < d.n_grids;
+    // step 1 (:2243-2458); its push_elec only writes the state epres interior,
+    // which step 2 overwrites before any read, so it is not issued
+    efield(R, d.state, box);
+    push(R, d.wrk, d.old, d.state, box, 0.5f * delt, 0);
+    double tg = inner_box ? clamp_t(R->t_old[b] + (double)delt2) : 0.0;
+    bounds(R, d.wrk, d.wrk, box, tg);
+    validate(R, d.wrk, box, false);
+    // step 2 (:2476-2684); inner boxes put their flanks into wrk
+    efield(R, d.wrk, box);
+    push(R, d.state, d.old, d.wrk, box, delt, 1);
+    tg = inner_box ? clamp_t(R->t_old[b] + (double)delt) : 0.0;
+    bounds(R, d.state, d.wrk, box, tg);
+    validate(R, d.state, box, false);
+    // Lapidus (:2692-2890), same t_grid
+    lapidus(R, box, delt);
+    bounds(R, d.wrk, d.wrk, box, tg);
+    validate(R, d.wrk, box, false);
+    // FCT (:2896-3030)
+    fct(R, box);
+    bounds(R, d.state, d.state, box, tg);
+    validate(R, d.state, box, true);
+    // nested sync (:3045-3063)
+    if (inner_box && std::fabs(R->t_new[b] - R->t_new[b + 1]) <= 0.005 * R->t_step[b]) {
+        corer(R, box);
+        R->t_old[b + 1] = R->t_new[b + 1];
+    }
+    R->cell_updates += R->n;
+    R->box_steps += 1;
+    check(R, "box_step");
+}
+
+void read_tstepnew(Run* R, const std::vector<int>& boxes) {
+    std::vector<float> h(5 * R->d.n_grids);
+    hipMemcpyAsync(h.data(), R->maxima, h.size() * sizeof(float), hipMemcpyDeviceToHost, R->s);
+    hipStreamSynchronize(R->s);
+    for (int box : boxes) {
+        const float* m = &h[5 * (box - 1)];
+        float f = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3] + m[4] * m[4]);
+        f = std::max(0.0f, f);
+        const float st = R->d.stepsz * R->d.xspac[box - 1] / f;
+        R->t_stepnew[box - 1] = (double)std::min(st, R->d.t_step_max);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* g3m3_run_create(const g3m3_run_desc* desc) {
+    if (!desc || desc->n_grids < 1 || desc->n_grids > G3M3_MAX_GRIDS) return nullptr;
+    Run* R = new Run();
+    R->d = *desc;
+    R->B = Box{desc->nx, desc->ny, desc->nz};
+    R->n = R->B.V();
+    const int n = R->n, ng = desc->n_grids;
+    bool ok = hipMalloc(&R->scr, sizeof(float) * 12 * (size_t)n) == hipSuccess &&
+              hipMalloc(&R->fctP, sizeof(float) * NF * (size_t)n) == hipSuccess &&
+              hipMalloc(&R->list, sizeof(ValEntry) * 3 * (size_t)n) == hipSuccess &&
+              hipMalloc(&R->count, 64) == hipSuccess &&
+              hipMalloc(&R->partials, sizeof(float) * 5 * (size_t)validate_blocks(R->B)) ==
+                  hipSuccess &&
+              hipMalloc(&R->maxima, sizeof(float) * 5 * ng) == hipSuccess;
+    if (ok && desc->mbndry >= 1 && desc->rst) {
+        // rst restricted to BandE's lists (mid and zero) of box mbndry
+        ok = hipMalloc(&R->rst_eff, sizeof(float) * (size_t)n) == hipSuccess;
+        if (ok) {
+            std::vector<float> h(n, 0.0f), rst(n);
+            hipMemcpy(rst.data(), desc->rst + (size_t)(desc->mbndry - 1) * n, sizeof(float) * n,
+                      hipMemcpyDeviceToHost);
+            const int mb = desc->mbndry, b = desc->mbndry - 1;
+            const int* lists[2] = {desc->ijmid, desc->ijzero};
+            const int nums[2] = {desc->nummid, desc->numzero};
+            for (int l = 0; l < 2; ++l) {
+                if (nums[l] <= 0) continue;
+                std::vector<int> ij((size_t)mb * 3 * nums[l]);
+                hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(), hipMemcpyDeviceToHost);
+                for (int p = 0; p < nums[l]; ++p) {
+                    const int i = ij[b + mb * (0 + 3 * p)] - 1, j = ij[b + mb * (1 + 3 * p)] - 1,
+                              k = ij[b + mb * (2 + 3 * p)] - 1;
+                    const int c = R->B.IX(i, j, k);
+                    h[c] = rst[c];
+                }
+            }
+            hipMemcpy(R->rst_eff, h.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        }
+    }
+    if (!ok) {
+        fprintf(stderr, "g3m3_run_create: device allocation failed\n");
+        g3m3_run_destroy(R);
+        return nullptr;
+    }
+    hipMemset(R->count, 0, 64);
+    hipMemset(R->maxima, 0, sizeof(float) * 5 * ng);
+    R->t_old.assign(ng, 0.0);
+    R->t_new.assign(ng, 0.0);
+    R->t_step.assign(ng, 0.0);
+    R->t_stepnew.assign(ng, 0.0);
+    return R;
+}
+
+void g3m3_run_destroy(void* run) {
+    Run* R = (Run*)run;
+    if (!R) return;
+    hipStreamSynchronize(R->s);
+    for (void* p : {(void*)R->scr, (void*)R->fctP, (void*)R->list, (void*)R->count,
+                    (void*)R->partials, (void*)R->maxima, (void*)R->rst_eff})
+        if (p) hipFree(p);
+    delete R;
+}
+
+void g3m3_run_set_stream(void* run, void* stream) { ((Run*)run)->s = (hipStream_t)stream; }
+
+int g3m3_run_init_dt(void* run) {
+    Run* R = (Run*)run;
+    std::vector<int> boxes;
+    for (int box = R->d.n_grids; box >= 1; --box) {
+        R->t_old[box - 1] = 0.0;
+        R->t_new[box - 1] = 0.0;
+        validate(R, R->d.state, box, true);
+        boxes.push_back(box);
+    }
+    read_tstepnew(R, boxes);
+    check(R, "init_dt");
+    return R->err;
+}
+
+int g3m3_run_full_step(void* run, const double* dt_override) {
+    Run* R = (Run*)run;
+    const int n = R->d.n_grids;
+    std::vector<double>& ts = R->t_step;
+    // dt synchronisation (code/3d_main.f90:1929-1968)
+    float smallest = 0.0f;
+    int m_small = n;
+    for (int box = n; box >= 1; --box) {
+        const int b = box - 1;
+        ts[b] = R->t_stepnew[b];
+        R->t_old[b] = 0.0;
+        R->t_new[b] = 0.0;
+        if (box == n) {
+            smallest = (float)ts[b];
+            m_small = box;
+        } else {
+            if (ts[b] > ts[b + 1]) ts[b] = ts[b + 1];
+            if ((double)smallest > ts[b]) {
+                smallest = (float)ts[b];
+                m_small = box;
+            }
+        }
+    }
+    for (int box = 1; box <= n; ++box) {
+        const int b = box - 1;
+        if (box <= m_small) {
+            ts[b] = (double)smallest;
+        } else {
+            const float astep = (float)(ts[b] / ts[b - 1] + 0.50);
+            int nsteps = (int)astep;
+            nsteps = std::min(std::max(nsteps, 1), 2);
+            ts[b] = ts[b - 1] * nsteps;
+        }
+    }
+    if (dt_override)
+        for (int b = 0; b < n; ++b) ts[b] = dt_override[b];
+    const int m_step = (int)(ts[n - 1] / ts[m_small - 1]);
+    R->t += (double)(float)ts[n - 1];
+    std::vector<int> stepped;
+    for (int ms = 0; ms < m_step; ++ms) {
+        for (int box = n; box >= 1; --box) {
+            const int b = box - 1;
+            const bool yes = box == 1 || (R->t_old[b] == R->t_new[b] && R->t_new[b] < ts[n - 1] &&
+                                          std::fabs(R->t_new[b] - R->t_new[0]) <= 0.005 * ts[0]);
+            if (yes) {
+                box_step(R, box);
+                if (std::find(stepped.begin(), stepped.end(), box) == stepped.end())
+                    stepped.push_back(box);
+            }
+        }
+    }
+    // final sync (:3069-3095)
+    R->t_old[0] = R->t_new[0];
+    for (int box = 1; box < n; ++box) {
+        corer(R, box);
+        set_rho_only(R, R->d.state, box);
+    }
+    read_tstepnew(R, stepped);
+    check(R, "full_step");
+    return R->err;
+}
+
+int g3m3_run_box_step(void* run, int box) {
+    Run* R = (Run*)run;
+    box_step(R, box);
+    return R->err;
+}
+
+void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
+                   double* t_new, long long* cell_updates, long long* box_steps) {
+    Run* R = (Run*)run;
+    const int n = R->d.n_grids;
+    if (t) *t = R->t;
+    for (int b = 0; b < n; ++b) {
+        if (t_step) t_step[b] = R->t_step[b];
+        if (t_stepnew) t_stepnew[b] = R->t_stepnew[b];
+        if (t_old) t_old[b] = R->t_old[b];
+        if (t_new) t_new[b] = R->t_new[b];
+    }
+    if (cell_updates) *cell_updates = R->cell_updates;
+    if (box_steps) *box_steps = R->box_steps;
+}
+
+}  // extern "C"

@@ -282,3 +282,108 @@ class Run:
 
     def state_host(self):
         return {k: self.a[k].cpu().numpy() for k in FIELDS}
+
+
+# -----------------------------------------------------------------------------
+# Native runtime (g3m3_run_*): the same full step, issued from C++ with the
+# fused kernels; Python makes one call per full step.
+
+import ctypes as _ct
+
+_MAXG = 8
+_SCALARS = ("rmassq", "rmassh", "rmasso", "gamma", "gamma1", "reynolds", "resist", "grav",
+            "re_equiv", "smallbit", "aniso_limit", "qvlim", "hvlim", "ovlim", "qclim", "hclim",
+            "oclim", "cslim", "chirho", "chipxyz", "chierg", "difrho", "diferg", "o_conc",
+            "rho_frac", "ti_te", "stepsz", "t_step_max", "vlim_factor", "srho", "spx", "spy",
+            "spz", "spress", "sbx_wind", "sby_wind", "sbz_wind", "v_rot", "r_rot", "eta")
+
+
+class RunDesc(_ct.Structure):
+    """Mirror of g3m3_run_desc (include/g3m3.h)."""
+    _fields_ = ([("nx", _ct.c_int), ("ny", _ct.c_int), ("nz", _ct.c_int), ("n_grids", _ct.c_int),
+                 ("state", _ct.c_void_p * 34), ("old", _ct.c_void_p * 34),
+                 ("wrk", _ct.c_void_p * 34)]
+                + [(k, _ct.c_void_p) for k in ("bx0", "by0", "bz0", "rst", "ijsrf", "ijmid",
+                                               "ijzero", "parm_srf", "parm_mid", "parm_zero")]
+                + [(k, _ct.c_int) for k in ("numsrf", "nummid", "numzero", "mbndry")]
+                + [(k, _ct.c_float * _MAXG) for k in ("grd_xmin", "grd_xmax", "grd_ymin",
+                                                      "grd_ymax", "grd_zmin", "grd_zmax",
+                                                      "xspac")]
+                + [(k, _ct.c_float) for k in _SCALARS]
+                + [("isotropic", _ct.c_int)])
+
+
+class FastRun(Run):
+    """A run advanced by the native runtime (one C-ABI call per full step)."""
+
+    def __init__(self, wl: Workload):
+        super().__init__(wl)
+        L = _lib.lib()
+        L.g3m3_run_create.restype = _ct.c_void_p
+        L.g3m3_run_create.argtypes = [_ct.POINTER(RunDesc)]
+        for nm in ("g3m3_run_init_dt", "g3m3_run_full_step", "g3m3_run_box_step"):
+            getattr(L, nm).restype = _ct.c_int
+        d = RunDesc()
+        d.nx, d.ny, d.nz, d.n_grids = self.nx, self.ny, self.nz, self.ngrd
+        for f, k in enumerate(FIELDS):
+            d.state[f] = self.a[k].data_ptr()
+            d.old[f] = self.a["old" + k].data_ptr()
+            d.wrk[f] = self.a["wrk" + k].data_ptr()
+        for k in ("bx0", "by0", "bz0", "rst", "ijsrf", "ijmid", "ijzero", "parm_srf",
+                  "parm_mid", "parm_zero"):
+            setattr(d, k, self.a[k].data_ptr())
+        d.numsrf = int(self.a["numsrf"][0])
+        d.nummid = int(self.a["nummid"][0])
+        d.numzero = int(self.a["numzero"][0])
+        d.mbndry = int(self.s["mbndry"])
+        for k in ("grd_xmin", "grd_xmax", "grd_ymin", "grd_ymax", "grd_zmin", "grd_zmax",
+                  "xspac"):
+            arr = getattr(d, k)
+            for b in range(self.ngrd):
+                arr[b] = float(self.a[k][b])
+        for k in _SCALARS:
+            setattr(d, k, float(self.s[k]))
+        d.isotropic = 1 if self.s["isotropic"] else 0
+        self._desc = d
+        self.h = L.g3m3_run_create(_ct.byref(d))
+        if not self.h:
+            raise _lib.G3M3Error("g3m3_run_create failed")
+        self._L = L
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._L.g3m3_run_destroy(_ct.c_void_p(h))
+            self.h = None
+
+    def _pull(self):
+        n = self.ngrd
+        t = _ct.c_double(0)
+        ts = (_ct.c_double * n)()
+        tsn = (_ct.c_double * n)()
+        to = (_ct.c_double * n)()
+        tn = (_ct.c_double * n)()
+        cu = _ct.c_longlong(0)
+        bs = _ct.c_longlong(0)
+        self._L.g3m3_run_info(_ct.c_void_p(self.h), _ct.byref(t), ts, tsn, to, tn,
+                              _ct.byref(cu), _ct.byref(bs))
+        self.t = t.value
+        self.t_step[:] = list(ts)
+        self.t_stepnew[:] = list(tsn)
+        self.t_old[:] = list(to)
+        self.t_new[:] = list(tn)
+        self.cell_updates = cu.value
+        self.box_steps = bs.value
+
+    def init_dt(self):
+        if self._L.g3m3_run_init_dt(_ct.c_void_p(self.h)):
+            raise _lib.G3M3Error("g3m3_run_init_dt failed")
+        self._pull()
+
+    def full_step(self, dt_override=None):
+        arg = None
+        if dt_override is not None:
+            arg = (_ct.c_double * self.ngrd)(*[float(x) for x in dt_override])
+        if self._L.g3m3_run_full_step(_ct.c_void_p(self.h), arg):
+            raise _lib.G3M3Error("g3m3_run_full_step failed")
+        self._pull()

@@ -338,6 +338,56 @@ int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int*
 void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qty,
                         float* scdata);
 
+/* ---- native runtime ----------------------------------------------------------
+ * No reference counterpart: the whole run resident in HBM, advanced a full
+ * step (code/3d_main.f90:1925-3095) per call with fused kernels.  Results
+ * equal the per-subroutine entry points called in the reference's order.
+ *
+ * Field pointer arrays use this order: for each ion species q, h, o --
+ * rho, presx, presy, presz, px, py, pz, presxy, presxz, presyz -- then
+ * epres, bx, by, bz; each a device array q(nx,ny,nz,n_grids).  wrk and
+ * old must start zeroed (the reference's static arrays).  Index lists and
+ * parm arrays are device arrays in the reference's layout; extents and
+ * scalars are copied at creation. */
+#define G3M3_MAX_GRIDS 8
+typedef struct {
+    int nx, ny, nz, n_grids;
+    float* state[34];
+    float* old[34];
+    float* wrk[34];
+    const float* bx0;
+    const float* by0;
+    const float* bz0;
+    const float* rst;                  /* rst(nx,ny,nz,mbndry) */
+    const int* ijsrf;
+    const int* ijmid;
+    const int* ijzero;
+    const float* parm_srf;
+    const float* parm_mid;
+    const float* parm_zero;
+    int numsrf, nummid, numzero, mbndry;
+    float grd_xmin[G3M3_MAX_GRIDS], grd_xmax[G3M3_MAX_GRIDS];
+    float grd_ymin[G3M3_MAX_GRIDS], grd_ymax[G3M3_MAX_GRIDS];
+    float grd_zmin[G3M3_MAX_GRIDS], grd_zmax[G3M3_MAX_GRIDS];
+    float xspac[G3M3_MAX_GRIDS];
+    float rmassq, rmassh, rmasso, gamma, gamma1, reynolds, resist, grav, re_equiv, smallbit;
+    float aniso_limit, qvlim, hvlim, ovlim, qclim, hclim, oclim, cslim;
+    float chirho, chipxyz, chierg, difrho, diferg, o_conc, rho_frac, ti_te;
+    float stepsz, t_step_max, vlim_factor;
+    float srho, spx, spy, spz, spress, sbx_wind, sby_wind, sbz_wind, v_rot, r_rot;
+    float eta;                         /* uniform resistivity (0 = reference) */
+    int isotropic;
+} g3m3_run_desc;
+
+void* g3m3_run_create(const g3m3_run_desc* desc);   /* NULL on failure */
+void g3m3_run_destroy(void* run);
+void g3m3_run_set_stream(void* run, void* hip_stream);
+int g3m3_run_init_dt(void* run);                     /* code/3d_main.f90:1861-1903 */
+int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt */
+int g3m3_run_box_step(void* run, int box);
+void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
+                   double* t_new, long long* cell_updates, long long* box_steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,11 +28,11 @@ def oracle_backend():
     return "ref" if ref.available() else "restate"
 
 
-def run_pair(name, steps, checkpoints, **over):
-    from g3m3_amd.driver import Run
+def run_pair(name, steps, checkpoints, native=False, **over):
+    from g3m3_amd.driver import FastRun, Run
     from oracle.ref import with_big_stack
     wl = make_workload(name, **over)
-    g = Run(wl)
+    g = (FastRun if native else Run)(wl)
     g.init_dt()
     be = oracle_backend()
     o = OracleRun(wl, be)
@@ -46,7 +46,8 @@ def run_pair(name, steps, checkpoints, **over):
             gh = g.state_host()
             errs = {k: rel_l2(gh[k], o.a[k]) for k in FIELDS}
             report[st] = errs
-            print(f"{name} step {st} ({be}): worst field "
+            print(f"{name} step {st} ({be}, {'native' if native else 'per-subroutine'}): "
+                  f"worst field "
                   f"{max(errs, key=errs.get)} rel-L2 {max(errs.values()):.3e}", flush=True)
             bad = {k: v for k, v in errs.items() if not v <= TOL}
             assert not bad, f"step {st}: fields over {TOL}: {bad}"
@@ -67,3 +68,11 @@ def test_small_nested_eta():
 def test_single61_1_10():
     """The single 61x61x31 box (the reference's CPU-runnable case)."""
     run_pair("single61", 10, {1, 10})
+
+
+@pytest.mark.parametrize("name,steps,over", [("tiny", 3, {}), ("small", 4, {"eta": 1e-3}),
+                                             ("single61", 10, {})])
+def test_native_runtime(name, steps, over):
+    """The native runtime (fused kernels, one call per full step) against
+    the oracle: same bar."""
+    run_pair(name, steps, {1, steps}, native=True, **over)
