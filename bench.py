@@ -43,6 +43,8 @@ KERNEL_BYTES = {
     "k_validate": (37 * 4, "all"),
     # per interior cell: 34 fields read, 34 written
     "k_lap": (68 * 4, "interior"),
+    # per cell and field: wrk, old read, out written (P stays in LDS)
+    "k_fct_fused": (3 * 4, "all_x_fields"),
     # per cell and field: w, old read, P written
     "k_fct_diffuse2": (3 * 4, "all_x_fields"),
     # per cell and field: P, w read, out written
@@ -157,6 +159,8 @@ def main(argv=None):
     cu = run.cell_updates - cu0
     wall = t1 - t0
     value = cu / wall
+    if hasattr(run, "sync_arrays"):
+        run.sync_arrays()
     for k in ("qrho", "bx", "epres"):
         v = run.a[k]
         assert bool(torch.isfinite(v).all()), f"non-finite {k} after the timed steps"

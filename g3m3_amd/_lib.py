@@ -54,7 +54,17 @@ def _is_tensor(a):
     return type(a).__module__.startswith("torch") and hasattr(a, "data_ptr")
 
 
+class ByValue:
+    """A ctypes scalar passed by value (the C-ABI's few value arguments, such
+    as the hidden CHARACTER length of g3m3_findgrid)."""
+
+    def __init__(self, c):
+        self.c = c
+
+
 def marshal(a, keep):
+    if isinstance(a, ByValue):
+        return a.c
     if _is_tensor(a):
         if not a.is_cuda:
             raise G3M3Error("device arrays must be CUDA/HIP tensors")

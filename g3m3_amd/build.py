@@ -8,6 +8,10 @@ g3m3_amd/libg3m3.so.  Flags:
                          kernels reproduce the reference's single-precision
                          expression order);
   -target-feature -packed-fp32-ops   no packed FP32 instructions.
+
+When flang is present the Fortran driver fragment g3m3_amd/fortran/
+drive_fields.f90 (reference-style calls through implicit interfaces) is
+linked against the library as g3m3_amd/fortran/drive_fields.
 """
 from __future__ import annotations
 
@@ -54,7 +58,25 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT} from {len(srcs)} sources")
+    _build_fortran(verbose)
     return OUT
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def _build_fortran(verbose: bool) -> None:
+    src = PKG / "fortran" / "drive_fields.f90"
+    exe = src.with_suffix("")
+    if not (src.exists() and os.access(FLANG, os.X_OK)):
+        return
+    cmd = [FLANG, "-O2", str(src), "-L", str(PKG), "-lg3m3", "-Wl,-rpath,$ORIGIN/..",
+           "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(OBJ))
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed on {src.name}:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {exe}")
 
 
 if __name__ == "__main__":

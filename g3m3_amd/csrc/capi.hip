@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/g3m3.h"
+#include "g3m3_craft.h"
 #include "g3m3_launch.h"
 
 using namespace g3m3;
@@ -891,4 +892,139 @@ void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qt
     check_launch("trilin_interp");
 }
 
+void g3m3_sparse_solver(const int* nbands, const int* npts, const int* divB_band_pos,
+                        const float* divB_errlim, const int* recurse, const int* redo_lim,
+                        const float* divB, float* soln_vec, float* residual2,
+                        const float* banded_matrix) {
+    const int n = *npts;
+    float* w = (float*)g_divb_f.get(sparse_work_floats(n) * sizeof(float));
+    double* d = (double*)g_divb_d.get(divb_work_doubles(n) * sizeof(double));
+    if (!w || !d) return;
+    double r2 = 0.0;
+    const int st = sparse_solve(n, *nbands, divB_band_pos, R(banded_matrix), R(divB),
+                                R(soln_vec), *divB_errlim, *recurse != 0, *redo_lim, kDivbRef, 0,
+                                w, d, nullptr, &r2, g_stream);
+    if (st < 0) latch(4, "sparse_solver: nbands must be 1..8");
+    check_launch("sparse_solver");
+    if (residual2) *residual2 = (float)r2;
+}
+
+void g3m3_fnd_pres(const float* px, const float* py, const float* pz, const float* pxy,
+                   const float* pxz, const float* pyz, float* p_para, float* p_perp,
+                   float* p_cross, const float* vvx, const float* vvy, const float* vvz,
+                   const float* bxt, const float* byt, const float* bzt, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box) {
+    (void)pxy;
+    (void)pxz;
+    (void)pyz;
+    (void)n_grids;
+    const Box B = box_of(nx, ny, nz);
+    const float* p[3] = {S(px, B, box), S(py, B, box), S(pz, B, box)};
+    const float* v[3] = {R(vvx), R(vvy), R(vvz)};
+    const float* b[3] = {R(bxt), R(byt), R(bzt)};
+    launch_fnd_pres(B.V(), p, v, b, S(p_para, B, box), S(p_perp, B, box), S(p_cross, B, box),
+                    g_stream);
+    check_launch("fnd_pres");
+}
+
+static bool craft_grid(CraftGrid& G, const int* n_grids, const float* grid_minvals,
+                       const float* grid_maxvals, const float* xspac, const float* re_equiv) {
+    if (*n_grids < 1 || *n_grids > kCraftMaxGrids) {
+        latch(4, "craft sampling: n_grids out of range");
+        return false;
+    }
+    G.n_grids = *n_grids;
+    for (int b = 0; b < G.n_grids; ++b) {
+        for (int a = 0; a < 3; ++a) {
+            G.mn[b][a] = grid_minvals[a + 3 * b];
+            G.mx[b][a] = grid_maxvals[a + 3 * b];
+        }
+        G.xspac[b] = xspac[b];
+    }
+    G.re_equiv = *re_equiv;
+    return true;
+}
+
+void g3m3_findgrid(const float* xyz, const int* n_grids, const float* grid_minvals,
+                   const float* grid_maxvals, const float* xspac, int* gridpt,
+                   const float* re_equiv, const char* cname, size_t cname_len) {
+    CraftGrid G;
+    if (!craft_grid(G, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv)) return;
+    if (!craft_findgrid(G, xyz, gridpt)) {
+        std::string name = cname ? std::string(cname, cname_len) : std::string();
+        printf(" Craft outside of grid limits. Name, xyz: %s %g %g %g\n", name.c_str(),
+               (double)xyz[0], (double)xyz[1], (double)xyz[2]);
+    }
+}
+
+void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
+                        const float* grid_minvals, const float* grid_maxvals, const float* xspac,
+                        const float* re_equiv, const int* nx, const int* ny, const int* nz,
+                        const int* nq, const float* const* fields, float* scdata, int* gridpt) {
+    CraftGrid G;
+    if (!craft_grid(G, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv)) return;
+    if (*nq < 0 || *nq > kMaxFields) {
+        latch(4, "sample_fields: nq out of range");
+        return;
+    }
+    FieldPtrs F{};
+    for (int q = 0; q < *nq; ++q) F.p[q] = const_cast<float*>(R(fields[q]));
+    launch_sample(*npts, R(sxyz), G, *nx, *ny, *nz, *nq, F, R(scdata), R(gridpt), g_stream);
+    check_launch("sample_fields");
+}
+
 }  // extern "C"
+
+// -----------------------------------------------------------------------------
+// Fortran implicit-interface names.  A Fortran caller that declares no
+// interface (as the reference's own call sites do) calls g3m3_push_ion as
+// the symbol g3m3_push_ion_ with every argument by reference; those names
+// are aliases of the entry points above.  The registry and control calls,
+// whose C forms take values, get by-reference forms.
+extern "C" {
+int g3m3_init_(void) { return g3m3_init(); }
+int g3m3_last_error_(void) { return g3m3_last_error(); }
+void g3m3_sync_(void) { g3m3_sync(); }
+int g3m3_register_(void* host, const long long* bytes) { return g3m3_register(host, *bytes); }
+int g3m3_unregister_(void* host) { return g3m3_unregister(host); }
+int g3m3_to_device_(void* host) { return g3m3_to_device(host); }
+int g3m3_to_host_(void* host) { return g3m3_to_host(host); }
+}
+
+#ifndef __HIP_DEVICE_COMPILE__
+#define G3M3_FORTRAN_ALIAS(n) extern "C" void g3m3_##n##_() __attribute__((alias("g3m3_" #n)));
+G3M3_FORTRAN_ALIAS(store_array)
+G3M3_FORTRAN_ALIAS(calcur)
+G3M3_FORTRAN_ALIAS(fnd_evel)
+G3M3_FORTRAN_ALIAS(bfield_total)
+G3M3_FORTRAN_ALIAS(bande)
+G3M3_FORTRAN_ALIAS(bande_eta)
+G3M3_FORTRAN_ALIAS(push_elec)
+G3M3_FORTRAN_ALIAS(push_ion)
+G3M3_FORTRAN_ALIAS(push_bfld)
+G3M3_FORTRAN_ALIAS(set_rho)
+G3M3_FORTRAN_ALIAS(set_speed_agrd)
+G3M3_FORTRAN_ALIAS(species_velocity)
+G3M3_FORTRAN_ALIAS(lap_plasma)
+G3M3_FORTRAN_ALIAS(lap_elec)
+G3M3_FORTRAN_ALIAS(lap_bfld)
+G3M3_FORTRAN_ALIAS(fnd_vtot)
+G3M3_FORTRAN_ALIAS(fcsmooth)
+G3M3_FORTRAN_ALIAS(flux_correct)
+G3M3_FORTRAN_ALIAS(bndry_outer)
+G3M3_FORTRAN_ALIAS(bndry_inner)
+G3M3_FORTRAN_ALIAS(bndry_flanks_v)
+G3M3_FORTRAN_ALIAS(flanks)
+G3M3_FORTRAN_ALIAS(corer)
+G3M3_FORTRAN_ALIAS(bndry_corer_v)
+G3M3_FORTRAN_ALIAS(flanks_synced)
+G3M3_FORTRAN_ALIAS(banded_tpose)
+G3M3_FORTRAN_ALIAS(divb_correct)
+G3M3_FORTRAN_ALIAS(divb_correct_ex)
+G3M3_FORTRAN_ALIAS(trilin_interp)
+G3M3_FORTRAN_ALIAS(fnd_pres)
+G3M3_FORTRAN_ALIAS(sparse_solver)
+G3M3_FORTRAN_ALIAS(findgrid)
+G3M3_FORTRAN_ALIAS(sample_fields)
+#undef G3M3_FORTRAN_ALIAS
+#endif

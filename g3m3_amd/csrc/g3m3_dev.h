@@ -37,6 +37,31 @@ __device__ inline bool cell_of(const Box& b, int gid, int& i, int& j, int& k) {
 
 __device__ inline int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Largest box volume for cell_of_fast (host launchers check it).
+constexpr int kFastDivMax = 1 << 23;
+
+// n / d for 0 <= n < 2^23, d >= 1: a float-reciprocal estimate (within 1
+// of the quotient at these sizes) and two exact integer corrections --
+// a few instructions instead of a software integer division.
+__device__ inline int udiv_small(int n, int d) {
+    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    int r = n - q * d;
+    q += (r >= d) - (r < 0);
+    r = n - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+}
+
+// cell_of for boxes of fewer than 2^23 cells
+__device__ inline bool cell_of_fast(const Box& b, int gid, int& i, int& j, int& k) {
+    if (gid >= b.V()) return false;
+    const int t = udiv_small(gid, b.nx);
+    i = gid - t * b.nx;
+    k = udiv_small(t, b.ny);
+    j = t - k * b.ny;
+    return true;
+}
+
 // A table of field pointers passed by value (kernel argument space).
 struct FieldPtrs {
     float* p[kMaxFields];

@@ -46,6 +46,7 @@ struct PushParams {
     float fdxt, fdyt, fdzt;     // fields and electrons: 2 * xspac
     float xmin, ymin, zmin, ddx, ddy, ddz, re_equiv, grav, gamma, gamma1;
     int do_elec;
+    int copy_shell;             // copy non-interior cells old -> out
 };
 
 struct ValEntry {
@@ -61,7 +62,8 @@ struct ValParams {
     const float* b[3];
     const float* b0[3];
     ValEntry* list;
-    int* count;
+    int* count;              // entries of this pass (zero on entry)
+    int* count_next;         // the next pass's counter, cleared by this pass
     float* partials;         // per-block maxima (null: not needed this pass)
     float aniso, smallbit, vlim, cslim, iso_flag;
     int isotropic;
@@ -84,18 +86,33 @@ struct FctParams {
     const float* old[kMaxFields];
     const float* wrk[kMaxFields];
     float chi[kMaxFields];
-    float* P;
-    size_t pstride;
+    int nf;                  // fields in use (k_fct_fused)
 };
+
+// One cell of the 2-layer flank shell of a fine box with its precomputed
+// trilinear weights (code/sub_flanks.f90:25-45): c = the fine cell, cb =
+// the coarse cell (ib-1, jb-1, kb-1), dx/dy/dz = the fractional offsets,
+// flags = bits 0-5 panels bottom, top, front, back, left, right containing
+// the cell, bits 8-13 the same panels when the cell is on their inner layer.
+struct ShellPt {
+    int c, cb;
+    float dx, dy, dz;
+    int flags;
+};
+
+// the shell of a box in panel-slab order, weights computed as the kernel
+// of the reference computes them (host, IEEE single precision)
+void build_shell(const Box& B, const FlankGeom& G, ShellPt* out);
+int shell_count(const Box& B);
+void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const FlankPtrs& Fp, int nf,
+                         float t1, float t2, hipStream_t s);
 
 void launch_efield(const EParams& P, hipStream_t s);
 void launch_push(const PushParams& P, hipStream_t s);
 void launch_validate(const ValParams& Q, float* maxima, hipStream_t s);
 int validate_blocks(const Box& B);
 void launch_lap(const LapParams& L, hipStream_t s);
-void launch_fct2(const FctParams& F, int nf, hipStream_t s);
-void launch_flanks_shell(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
-                         float t2, hipStream_t s);
-int shell_cells(const Box& B);
+void launch_fct_fused(const FctParams& F, int nf, hipStream_t s);
+
 
 }  // namespace g3m3

@@ -99,6 +99,13 @@ void launch_push_elec(const Box& B, const float* w, const float* old, float* ep,
 void launch_push_bfld(const Box& B, float* b[3], const float* old[3], const float* e[3], float delt,
                       float rx, float ry, float rz, hipStream_t s);
 void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& src, hipStream_t s);
+void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],
+                     const float* const b[3], float* para, float* perp, float* cross,
+                     hipStream_t s);
+struct CraftGrid;
+void launch_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
+                   int nq, const FieldPtrs& F, float* out, int* gridpt, hipStream_t s);
+void launch_swap_fields(int n, int nf, const FieldPtrs& a, const FieldPtrs& b, hipStream_t s);
 void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, float grav,
                      float re_equiv, float reynolds, const float gmin[3], const float gmax[3],
                      hipStream_t s);
@@ -128,6 +135,10 @@ int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float 
                  long max_iters, float* work, double* dwork, long* iters_out, double* res2_out,
                  hipStream_t s);
 size_t divb_work_floats(int ncells);
+int sparse_solve(int npts, int nbands, const int* pos, const float* M, const float* rhs,
+                 float* soln, float errlim, bool recurse, int redo_lim, int mode, long max_iters,
+                 float* work, double* dwork, long* iters_out, double* res2_out, hipStream_t s);
+size_t sparse_work_floats(int npts);
 size_t divb_work_doubles(int ncells);
 void launch_banded_matvec(int npts, const int* pos, const float* M, const float* x, float* y,
                           hipStream_t s);

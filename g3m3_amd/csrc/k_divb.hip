@@ -13,6 +13,7 @@
 #include "g3m3_dev.h"
 #include "g3m3_launch.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace g3m3 {
@@ -45,23 +46,58 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
+// Device-side control of the iteration: the scalars of code/
+// sub_sparse_solver.f90 live in device memory and each kernel of an
+// iteration first reads the status word, so the host can queue a batch of
+// iterations and look at the outcome once per batch.
+struct DivbCtl {
+    int status;          // 0 running, 1 converged / exact, 2 restart wanted
+    int pad;
+    long long iters;     // completed iterations (all levels)
+};
+
+enum { kScResd = 2, kScSumB = 3, kScErr2 = 8, kScR2 = 9 };
+
+__device__ inline double as_mode_d(double v, int mode) {
+    return mode == kDivbRef ? (double)(float)v : v;
+}
+
+// An explicit banded matrix M(nbands, npts) with band offsets pos (the
+// general form of code/sub_banded_tpose.f90); M == nullptr selects the
+// matrix-free divB Laplacian of the box.
+struct Bands {
+    const float* M;
+    int nbands, npts;
+    int pos[8];
+};
+
 // y = A x (- b), partial sums of y*y and of u*v (either optional)
-__global__ void k_matvec(Box B, const float* __restrict__ x, float* y, const float* __restrict__ b,
-                         double* part_yy, const float* __restrict__ u, const float* __restrict__ v,
-                         double* part_uv, const float* __restrict__ gplus, double* part_gy) {
+__global__ void k_matvec(Box B, Bands Bd, const float* __restrict__ x, float* y,
+                         const float* __restrict__ b, double* part_yy, const float* __restrict__ u,
+                         const float* __restrict__ v, double* part_uv,
+                         const float* __restrict__ gplus, double* part_gy, const DivbCtl* ctl) {
+    if (ctl && ctl->status != 0) return;
     int i, j, k;
     const int c = gtid();
     double yy = 0.0, uv = 0.0, gy = 0.0;
-    if (cell_of(B, c, i, j, k)) {
-        const int X = 1, Y = B.nx, Z = B.nx * B.ny;
+    const bool mine = Bd.M ? c < Bd.npts : cell_of(B, c, i, j, k);
+    if (mine) {
         float t = 0.0f;
-        if (k < B.nz - 1) t = t + x[c + Z];
-        if (j < B.ny - 1) t = t + x[c + Y];
-        if (i < B.nx - 1) t = t + x[c + X];
-        t = t + (-6.0f) * x[c];
-        if (i > 0) t = t + x[c - X];
-        if (j > 0) t = t + x[c - Y];
-        if (k > 0) t = t + x[c - Z];
+        if (Bd.M) {
+            for (int q = 0; q < Bd.nbands; ++q) {
+                const int ii = c + Bd.pos[q];
+                if (ii >= 0 && ii < Bd.npts) t = t + Bd.M[q + Bd.nbands * (size_t)ii] * x[ii];
+            }
+        } else {
+            const int X = 1, Y = B.nx, Z = B.nx * B.ny;
+            if (k < B.nz - 1) t = t + x[c + Z];
+            if (j < B.ny - 1) t = t + x[c + Y];
+            if (i < B.nx - 1) t = t + x[c + X];
+            t = t + (-6.0f) * x[c];
+            if (i > 0) t = t + x[c - X];
+            if (j > 0) t = t + x[c - Y];
+            if (k > 0) t = t + x[c - Z];
+        }
         if (b) t = t - b[c];
         y[c] = t;
         yy = (double)t * (double)t;
@@ -80,6 +116,76 @@ __global__ void k_matvec(Box B, const float* __restrict__ x, float* y, const flo
     }
 }
 
+__device__ inline double sum_parts(const double* __restrict__ part, int nb) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kRedThreads) s += part[b];
+    return block_sum(s);
+}
+
+// start of a level (:61-71): resd = |A soln - b|^2, sum_b = |b|^2
+__global__ void k_fin_start(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl,
+                            int mode) {
+    const double r = sum_parts(part, nb);
+    const double b2 = sum_parts(part + nb, nb);
+    if (threadIdx.x == 0) {
+        sc[kScResd] = as_mode_d(r, mode);
+        sc[kScSumB] = as_mode_d(b2, mode);
+        ctl->status = 0;
+    }
+}
+
+// anum = (g.h) / |A h|^2 (:84-95)
+__global__ void k_fin_anum(const double* __restrict__ part, int nb, double* sc,
+                           const DivbCtl* ctl, int mode) {
+    if (ctl->status != 0) return;
+    const double aden = as_mode_d(sum_parts(part, nb), mode);
+    const double anum = as_mode_d(sum_parts(part + nb, nb), mode);
+    if (threadIdx.x == 0) sc[kScAnum] = as_mode_d(anum / aden, mode);
+}
+
+// residual2 and the exit / restart tests (:101-129)
+__global__ void k_fin_r2(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl,
+                         int mode) {
+    if (ctl->status != 0) return;
+    const double r2 = as_mode_d(sum_parts(part, nb), mode);
+    if (threadIdx.x == 0) {
+        sc[kScR2] = r2;
+        ctl->iters += 1;
+        const double resd = sc[kScResd];
+        if (r2 == resd || r2 <= as_mode_d(sc[kScSumB] * sc[kScErr2], mode)) ctl->status = 1;
+        else if (r2 > resd) ctl->status = 2;
+        else sc[kScResd] = r2;
+    }
+}
+
+// gam = dgg / gg (:134-146)
+__global__ void k_fin_gam(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl,
+                          int mode) {
+    if (ctl->status != 0) return;
+    const double gg = as_mode_d(sum_parts(part + nb, nb), mode);
+    const double dgg = as_mode_d(sum_parts(part + 2 * nb, nb), mode);
+    if (threadIdx.x == 0) {
+        if (gg == 0.0) ctl->status = 1;
+        else sc[kScGam] = as_mode_d(dgg / gg, mode);
+    }
+}
+
+__global__ void k_divb_init(double* sc, DivbCtl* ctl, double errlim2) {
+    if (threadIdx.x == 0) {
+        sc[kScErr2] = errlim2;
+        ctl->status = 0;
+        ctl->iters = 0;
+    }
+}
+
+// resd = residual2 of the restarted solve, status cleared (after a restart)
+__global__ void k_resume(double* sc, DivbCtl* ctl) {
+    if (threadIdx.x == 0) {
+        sc[kScResd] = sc[kScR2];
+        ctl->status = 0;
+    }
+}
+
 // generic dot: partial sums of u*v
 __global__ void k_dot(int n, const float* __restrict__ u, const float* __restrict__ v, double* part) {
     const int c = gtid();
@@ -89,19 +195,10 @@ __global__ void k_dot(int n, const float* __restrict__ u, const float* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// out[q] = sum of part[q*stride .. q*stride+nb), fixed order
-__global__ void k_finalize(const double* __restrict__ part, int nb, int stride, int nq, double* out) {
-    for (int q = 0; q < nq; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kRedThreads) s += part[q * stride + b];
-        const double r = block_sum(s);
-        if (threadIdx.x == 0) out[q] = r;
-    }
-}
-
 // xi = soln; soln = soln + anum*h
 __global__ void k_update_soln(int n, float* soln, float* xi, const float* __restrict__ h,
-                              const double* __restrict__ sc, int mode) {
+                              const double* __restrict__ sc, int mode, const DivbCtl* ctl) {
+    if (ctl->status != 0) return;
     const int c = gtid();
     if (c >= n) return;
     const float s = soln[c];
@@ -112,7 +209,8 @@ __global__ void k_update_soln(int n, float* soln, float* xi, const float* __rest
 
 // g = -xi; h = g + gam*h
 __global__ void k_update_dir(int n, float* g, float* h, const float* __restrict__ xi,
-                             const double* __restrict__ sc, int mode) {
+                             const double* __restrict__ sc, int mode, const DivbCtl* ctl) {
+    if (ctl->status != 0) return;
     const int c = gtid();
     if (c >= n) return;
     const float gn = -xi[c];
@@ -158,12 +256,14 @@ __global__ void k_divb_apply(Box B, const float* __restrict__ soln, float* bx, f
 }
 
 // ---------------------------------------------------------------------------
-// Host side: the iteration of sparse_solver, recursion included.
+// Host side: the iteration of sparse_solver, recursion included, queued in
+// batches of kBatch iterations between looks at the status word.
 
 namespace {
 struct Level {
     float *g, *h, *xi, *xj;
 };
+constexpr int kBatch = 32;
 }  // namespace
 
 struct DivbSolver {
@@ -173,91 +273,101 @@ struct DivbSolver {
     const float* rhs;
     double* part;    // 3 * nb partial sums
     double* sc;      // device scalars
+    DivbCtl* ctl;    // device control word
     Level lv[4];
+    Bands bands{};
     hipStream_t s;
     int redos = 0;
-    long iters = 0;
-    long max_iters;
-    double errlim2, sum_b;
+    long long iters = 0;
+    long long max_iters;
     double residual2 = 0.0;
-    double host_sc[8];
     int status = 0;
 
-    dim3 grid() const { return dim3(nb); }
-
-    void read(int nq) {
-        hipMemcpyAsync(host_sc, sc, sizeof(double) * nq, hipMemcpyDeviceToHost, s);
+    DivbCtl read_ctl() {
+        DivbCtl h;
+        hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
         hipStreamSynchronize(s);
+        return h;
     }
-    double as_mode(double v) const { return mode == kDivbRef ? (double)(float)v : v; }
+    double read_sc(int q) {
+        double v = 0.0;
+        hipMemcpyAsync(&v, sc + q, sizeof(double), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        return v;
+    }
 
-    // returns when converged / restarted out / iteration limit
+    // the tail of an iteration (:131-146): xi = A xj, gg, dgg, new direction
+    void tail(const Level& L) {
+        const dim3 g(nb), t(kRedThreads);
+        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, bands, L.xj, L.xi, nullptr, part, L.g, L.g,
+                           part + nb, L.g, part + 2 * nb, ctl);
+        hipLaunchKernelGGL(k_fin_gam, dim3(1), t, 0, s, part, nb, sc, ctl, mode);
+        hipLaunchKernelGGL(k_update_dir, g, t, 0, s, n, L.g, L.h, L.xi, sc, mode, ctl);
+    }
+
     void solve(int depth, bool recurse, int redo_lim) {
         Level& L = lv[depth];
         redos = recurse ? redo_lim : 0;
         redos += 1;
         const dim3 g(nb), t(kRedThreads);
-        // xi = A soln - b, |b|^2, |xi|^2 (:61-71)
-        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, soln, L.xi, rhs, part, rhs, rhs, part + nb,
-                           nullptr, nullptr);
-        hipLaunchKernelGGL(k_finalize, dim3(1), t, 0, s, part, nb, nb, 2, sc + kScTmp);
-        read(kScTmp + 2);
-        double resd = as_mode(host_sc[kScTmp]);
-        sum_b = as_mode(host_sc[kScTmp + 1]);
-        // g = -A xi, h = g (:73-75)
-        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, L.xi, L.g, nullptr, part, nullptr, nullptr,
+        // xi = A soln - b, |xi|^2, |b|^2 (:61-71)
+        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, bands, soln, L.xi, rhs, part, rhs, rhs, part + nb,
                            nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_fin_start, dim3(1), t, 0, s, part, nb, sc, ctl, mode);
+        // g = -A xi, h = g (:73-75)
+        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, bands, L.xi, L.g, nullptr, part, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr);
         hipLaunchKernelGGL(k_sub_neg, g, t, 0, s, n, L.h, nullptr, L.g, 1);
-        for (long it = 0; it < 10L * n && iters < max_iters; ++it) {
-            ++iters;
-            // xi = A h; anum = g.h / |xi|^2 (:82-95)
-            hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, L.h, L.xi, nullptr, part, L.g, L.h,
-                               part + nb, nullptr, nullptr);
-            hipLaunchKernelGGL(k_finalize, dim3(1), t, 0, s, part, nb, nb, 2, sc + kScTmp);
-            read(kScTmp + 2);
-            const double aden = as_mode(host_sc[kScTmp]);
-            double anum = as_mode(host_sc[kScTmp + 1]);
-            anum = as_mode(anum / aden);
-            host_sc[kScAnum] = anum;
-            hipMemcpyAsync(sc + kScAnum, &host_sc[kScAnum], sizeof(double), hipMemcpyHostToDevice, s);
-            hipLaunchKernelGGL(k_update_soln, g, t, 0, s, n, soln, L.xi, L.h, sc, mode);
-            // xj = A soln - b, residual2 (:101-107)
-            hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, soln, L.xj, rhs, part, nullptr, nullptr,
-                               nullptr, nullptr, nullptr);
-            hipLaunchKernelGGL(k_finalize, dim3(1), t, 0, s, part, nb, nb, 1, sc + kScTmp);
-            read(kScTmp + 1);
-            const double r2 = as_mode(host_sc[kScTmp]);
-            residual2 = r2;
-            if (r2 == resd || r2 <= as_mode(sum_b * errlim2)) {
+        const long long level_cap = 10LL * n;
+        long long level_done = 0;
+        for (;;) {
+            const long long start = iters;
+            const long long room = std::min(level_cap - level_done, max_iters - iters);
+            if (room <= 0) {
+                status = 3;   // iteration limit
+                return;
+            }
+            const int nbatch = (int)std::min<long long>(kBatch, room);
+            for (int b = 0; b < nbatch; ++b) {
+                // xi = A h; anum = g.h / |xi|^2; xi = soln; soln += anum h (:82-99)
+                hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, bands, L.h, L.xi, nullptr, part, L.g, L.h,
+                                   part + nb, nullptr, nullptr, ctl);
+                hipLaunchKernelGGL(k_fin_anum, dim3(1), t, 0, s, part, nb, sc, ctl, mode);
+                hipLaunchKernelGGL(k_update_soln, g, t, 0, s, n, soln, L.xi, L.h, sc, mode, ctl);
+                // xj = A soln - b, residual2, exit / restart tests (:101-129)
+                hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, bands, soln, L.xj, rhs, part, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, ctl);
+                hipLaunchKernelGGL(k_fin_r2, dim3(1), t, 0, s, part, nb, sc, ctl, mode);
+                tail(L);
+            }
+            const DivbCtl c = read_ctl();
+            iters = c.iters;
+            level_done += iters - start;
+            if (c.status == 0) continue;
+            residual2 = read_sc(kScR2);
+            if (c.status == 1) {
                 status = 1;
                 return;
             }
-            if (r2 > resd) {   // restart from the previous iterate (:118-129)
-                hipLaunchKernelGGL(k_copy1, g, t, 0, s, n, soln, L.xi);
-                if (redos >= 3 || depth + 1 >= 4) {
-                    status = 2;
-                    return;
-                }
-                solve(depth + 1, true, redos);
-                if (status != 0) { /* the reference keeps iterating here */ }
+            // restart from the previous iterate (:118-129); the gg test of a
+            // finished iteration can also end the solve (status 1 above)
+            hipLaunchKernelGGL(k_copy1, g, t, 0, s, n, soln, L.xi);
+            if (redos >= 3 || depth + 1 >= 4) {
+                status = 2;
+                return;
             }
-            resd = residual2;
-            // xi = A xj; gg = |g|^2, dgg = (xi+g).xi (:134-141)
-            hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, L.xj, L.xi, nullptr, part, L.g, L.g,
-                               part + nb, L.g, part + 2 * nb);
-            hipLaunchKernelGGL(k_finalize, dim3(1), t, 0, s, part, nb, nb, 3, sc + kScTmp);
-            read(kScTmp + 3);
-            const double gg = as_mode(host_sc[kScTmp + 1]);
-            const double dgg = as_mode(host_sc[kScTmp + 2]);
-            if (gg == 0.0) {
+            solve(depth + 1, true, redos);
+            // the reference continues this level after the nested solve:
+            // resd = its residual2, then the rest of the iteration
+            hipLaunchKernelGGL(k_resume, dim3(1), t, 0, s, sc, ctl);
+            tail(L);
+            const DivbCtl c2 = read_ctl();
+            if (c2.status == 1) {
                 status = 1;
                 return;
             }
-            host_sc[kScGam] = as_mode(dgg / gg);
-            hipMemcpyAsync(sc + kScGam, &host_sc[kScGam], sizeof(double), hipMemcpyHostToDevice, s);
-            hipLaunchKernelGGL(k_update_dir, g, t, 0, s, n, L.g, L.h, L.xi, sc, mode);
+            status = 0;
         }
-        status = 3;   // iteration limit
     }
 };
 
@@ -279,22 +389,65 @@ int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float 
         S.lv[l] = Level{base, base + n, base + 2 * (size_t)n, base + 3 * (size_t)n};
     }
     S.sc = dwork;
+    S.ctl = reinterpret_cast<DivbCtl*>(dwork + 12);
     S.part = dwork + 16;
     S.s = s;
-    S.max_iters = max_iters > 0 ? max_iters : 10L * n;
-    S.errlim2 = (double)((float)n * (errlim * errlim));
-    if (mode != kDivbRef) S.errlim2 = (double)n * (double)errlim * (double)errlim;
+    S.max_iters = max_iters > 0 ? max_iters : 10LL * n;
+    const double errlim2 = mode == kDivbRef ? (double)((float)n * (errlim * errlim))
+                                            : (double)n * (double)errlim * (double)errlim;
     const dim3 g(nb), t(kRedThreads);
+    hipLaunchKernelGGL(k_divb_init, dim3(1), dim3(64), 0, s, S.sc, S.ctl, errlim2);
     hipLaunchKernelGGL(k_divb_rhs, g, t, 0, s, B, bx, by, bz, rhs, 2.0f * rx);
     // the reference's soln_vec is an uninitialised automatic array; start from 0
     hipLaunchKernelGGL(k_zero1, g, t, 0, s, n, S.soln);
     S.solve(0, false, 0);
+    S.residual2 = S.read_sc(kScR2);
     hipLaunchKernelGGL(k_divb_apply, g, t, 0, s, B, S.soln, bx, by, bz, rx * rx, 2.0f * rx,
                        2.0f * rx, 2.0f * rx);
     if (iters_out) *iters_out = S.iters;
     if (res2_out) *res2_out = S.residual2;
     return S.status;
 }
+
+// code/sub_sparse_solver.f90:19-159 on an explicit banded matrix: soln is
+// the initial guess on entry and the solution on return
+int sparse_solve(int npts, int nbands, const int* pos, const float* M, const float* rhs,
+                 float* soln, float errlim, bool recurse, int redo_lim, int mode, long max_iters,
+                 float* work, double* dwork, long* iters_out, double* res2_out, hipStream_t s) {
+    if (nbands < 1 || nbands > 8) return -1;
+    const int n = npts;
+    const int nb = (n + kRedThreads - 1) / kRedThreads;
+    DivbSolver S;
+    S.B = Box{n, 1, 1};
+    S.bands.M = M;
+    S.bands.nbands = nbands;
+    S.bands.npts = npts;
+    for (int q = 0; q < nbands; ++q) S.bands.pos[q] = pos[q];
+    S.n = n;
+    S.nb = nb;
+    S.mode = mode;
+    S.soln = soln;
+    S.rhs = rhs;
+    for (int l = 0; l < 4; ++l) {
+        float* base = work + (size_t)(4 * l) * n;
+        S.lv[l] = Level{base, base + n, base + 2 * (size_t)n, base + 3 * (size_t)n};
+    }
+    S.sc = dwork;
+    S.ctl = reinterpret_cast<DivbCtl*>(dwork + 12);
+    S.part = dwork + 16;
+    S.s = s;
+    S.max_iters = max_iters > 0 ? max_iters : 10LL * n;
+    const double errlim2 = mode == kDivbRef ? (double)((float)n * (errlim * errlim))
+                                            : (double)n * (double)errlim * (double)errlim;
+    hipLaunchKernelGGL(k_divb_init, dim3(1), dim3(64), 0, s, S.sc, S.ctl, errlim2);
+    S.solve(0, recurse, redo_lim);
+    S.residual2 = S.read_sc(kScR2);
+    if (iters_out) *iters_out = S.iters;
+    if (res2_out) *res2_out = S.residual2;
+    return S.status;
+}
+
+size_t sparse_work_floats(int npts) { return (size_t)16 * npts; }
 
 size_t divb_work_floats(int ncells) { return (size_t)18 * ncells; }
 size_t divb_work_doubles(int ncells) { return 16 + 3 * (size_t)((ncells + kRedThreads - 1) / kRedThreads); }

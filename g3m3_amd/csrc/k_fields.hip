@@ -325,4 +325,20 @@ void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& sr
     }
 }
 
+// exchange the contents of two field sets (blockIdx.y = field)
+__global__ void k_swap_fields(int n, FieldPtrs a, FieldPtrs b) {
+    const int c = gtid();
+    if (c >= n) return;
+    float* pa = a.p[blockIdx.y];
+    float* pb = b.p[blockIdx.y];
+    const float t = pa[c];
+    pa[c] = pb[c];
+    pb[c] = t;
+}
+
+void launch_swap_fields(int n, int nf, const FieldPtrs& a, const FieldPtrs& b, hipStream_t s) {
+    if (nf <= 0) return;
+    hipLaunchKernelGGL(k_swap_fields, dim3((n + 255) / 256, nf), dim3(256), 0, s, n, a, b);
+}
+
 }  // namespace g3m3

@@ -6,17 +6,38 @@
 // about once per stage instead of once per reference loop nest:
 //   k_efield    calcur + total B + fnd_evel + BandE (code/sub_calcur.f90,
 //               code/3d_main.f90:2248-2254, sub_fnd_evel.f90, sub_BandE.f90)
-//   k_push      push_ion x3 species, push_bfld, push_elec (blockIdx.y picks)
+//   k_push      push_ion x3 species, push_bfld, push_elec (launch group picks)
 //   k_validate  set_rho + set_speed_agrd in one read of the fields
-//   k_lap       Lapidus for the 3 species, electrons and B (blockIdx.y)
-//   k_fct_*     flux-corrected transport, two passes for all 34 fields
-//   k_flanks_shell  coarse -> fine boundary on the 2-layer shell only
+//   k_lap       Lapidus for the 3 species, electrons and B (launch group)
+//   k_fct_fused flux-corrected transport of all 34 fields, one pass
+//   k_flanks_table  coarse -> fine boundary on the 2-layer shell only
 #include "g3m3_dev.h"
 #include "g3m3_fused.h"
 
 namespace g3m3 {
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Launch shape of the fused stage kernels: 256 consecutive cells per block
+// (so every wave reads whole, aligned 256-byte runs of a field) and
+// blockIdx.y = the group (species / field / B-and-electron group).
+//
+// XCD-aware order: workgroups are dealt to the 8 XCDs round-robin in
+// dispatch order, and each XCD has its own L2.  Block b is given the
+// cells of logical block (b % 8) * (nb / 8) + b / 8, so each XCD sweeps a
+// contiguous eighth of the box and finds the neighbouring planes a stencil
+// reads in its own L2.  The grid is padded to a multiple of 8 blocks.
+__device__ inline int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+
+__device__ inline bool cell3(const Box& B, int& i, int& j, int& k, int& g) {
+    g = blockIdx.y;
+    return cell_of_fast(B, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x, i, j, k);
+}
+inline dim3 grid3(const Box& B, int groups) {
+    const int nb = (B.V() + 255) / 256;
+    return dim3((nb + 7) & ~7, groups, 1);
+}
+inline dim3 block3() { return dim3(256, 1, 1); }
 
 // ---------------------------------------------------------------------------
 // E field.  cur at a boundary cell is calcur's face/edge/corner rule; those
@@ -108,9 +129,9 @@ __device__ inline void efield_interior(const EParams& P, const CurlCtx& C, int i
 }
 
 __global__ void __launch_bounds__(256) k_efield(EParams P) {
-    int i, j, k;
+    int i, j, k, g;
     const Box& B = P.B;
-    if (!cell_of(B, gtid(), i, j, k)) return;
+    if (!cell3(B, i, j, k, g)) return;
     const int c = B.IX(i, j, k);
     CurlCtx C{B, P.b[0], P.b[1], P.b[2], P.dxt, P.dyt, P.dzt};
     float Ex, Ey, Ez;
@@ -155,16 +176,16 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Push.  blockIdx.y = 0,1,2: ion species q,h,o; 3: push_bfld (+ push_elec
+// Push.  group = 0,1,2: ion species q,h,o; 3: push_bfld (+ push_elec
 // when P.do_elec).  The species velocity wrk p / max(wrk rho, 1e-6) is
 // evaluated where it is needed instead of stored (same bits).
 
 __device__ inline void push_species(const PushParams& P, int s, int i, int j, int k) {
     const Box& B = P.B;
     const int c = B.IX(i, j, k), X = 1, Y = B.nx, Z = B.nx * B.ny;
+    auto put = [&](int m, float v) { P.out[s].f[m][c] = v; };
     const SpeciesSet& W = P.src[s];
     const SpeciesSet& O = P.old[s];
-    const SpeciesSet& U = P.out[s];
     const float rmass = P.rmass[s];
     const float delt = P.delt, reynolds = P.reynolds;
     const float dxt = P.dxt, dyt = P.dyt, dzt = P.dzt;
@@ -196,9 +217,9 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
                ((w[c + Y] * vy[3] - w[c - Y] * vy[4]) / dyt) +
                ((w[c + Z] * vz[5] - w[c - Z] * vz[6]) / dzt);
     };
-    U.f[0][c] = O.f[0][c] - delt * (((wpx[c + X] - wpx[c - X]) / dxt) +
+    put(0, O.f[0][c] - delt * (((wpx[c + X] - wpx[c - X]) / dxt) +
                                     ((wpy[c + Y] - wpy[c - Y]) / dyt) +
-                                    ((wpz[c + Z] - wpz[c - Z]) / dzt));
+                                    ((wpz[c + Z] - wpz[c - Z]) / dzt)));
     const float aqpx = O.f[4][c] - delt * flux(wpx);
     const float aqpy = O.f[5][c] - delt * flux(wpy);
     const float aqpz = O.f[6][c] - delt * flux(wpz);
@@ -239,9 +260,9 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     const float radius = sqrtf(ax * ax + ay * ay + az * az) + 0.0000001f;
     const float tr = P.re_equiv * radius;
     const float g = delt * wrc * P.grav / (tr * tr);
-    U.f[4][c] = px - g * ax / radius;
-    U.f[5][c] = py - g * ay / radius;
-    U.f[6][c] = pz - g * az / radius;
+    put(4, px - g * ax / radius);
+    put(5, py - g * ay / radius);
+    put(6, pz - g * az / radius);
 
     const float cx = wx[c], cy = wy[c], cz = wz[c];
     const float cxy = wxy[c], cxz = wxz[c], cyz = wyz[c];
@@ -258,27 +279,27 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     float sf = reynolds;
     if (bmag > bmax) sf = reynolds * (bmax / bmag);
     const float t2 = 2.0f * delt;
-    U.f[1][c] = px0 - t2 * cxy * (vx[3] - vx[4]) / dyt - t2 * cxz * (vx[5] - vx[6]) / dzt +
-                t2 * sf / rmass * (cxy * abz - cxz * aby);
-    U.f[2][c] = py0 - t2 * cxy * (vy[1] - vy[2]) / dxt - t2 * cyz * (vy[5] - vy[6]) / dzt +
-                t2 * sf / rmass * (cyz * abx - cxy * abz);
-    U.f[3][c] = pz0 - t2 * cxz * (vz[1] - vz[2]) / dxt - t2 * cyz * (vz[3] - vz[4]) / dyt +
-                t2 * sf / rmass * (cxz * aby - cyz * abx);
+    put(1, px0 - t2 * cxy * (vx[3] - vx[4]) / dyt - t2 * cxz * (vx[5] - vx[6]) / dzt +
+                t2 * sf / rmass * (cxy * abz - cxz * aby));
+    put(2, py0 - t2 * cxy * (vy[1] - vy[2]) / dxt - t2 * cyz * (vy[5] - vy[6]) / dzt +
+                t2 * sf / rmass * (cyz * abx - cxy * abz));
+    put(3, pz0 - t2 * cxz * (vz[1] - vz[2]) / dxt - t2 * cyz * (vz[3] - vz[4]) / dyt +
+                t2 * sf / rmass * (cxz * aby - cyz * abx));
     const float pxy = O.f[7][c] - delt * flux(wxy);
-    U.f[7][c] = pxy - delt * cxy * ((vx[1] - vx[2]) / dxt + (vy[3] - vy[4]) / dyt) -
+    put(7, pxy - delt * cxy * ((vx[1] - vx[2]) / dxt + (vy[3] - vy[4]) / dyt) -
                 delt * cy * (vx[3] - vx[4]) / dyt - delt * cx * (vy[1] - vy[2]) / dxt -
                 delt * cyz * (vx[5] - vx[6]) / dzt - delt * cxz * (vy[5] - vy[6]) / dzt +
-                delt * sf / rmass * (abz * (cy - cx) - aby * cyz + abx * cxz);
+                delt * sf / rmass * (abz * (cy - cx) - aby * cyz + abx * cxz));
     const float pxz = O.f[8][c] - delt * flux(wxz);
-    U.f[8][c] = pxz - delt * cxz * ((vx[1] - vx[2]) / dxt + (vz[5] - vz[6]) / dzt) -
+    put(8, pxz - delt * cxz * ((vx[1] - vx[2]) / dxt + (vz[5] - vz[6]) / dzt) -
                 delt * cz * (vx[5] - vx[6]) / dzt - delt * cx * (vz[1] - vz[2]) / dxt -
                 delt * cyz * (vx[3] - vx[4]) / dyt - delt * cxy * (vz[3] - vz[4]) / dyt +
-                delt * sf / rmass * (aby * (cx - cz) - abx * cxy + abz * cyz);
+                delt * sf / rmass * (aby * (cx - cz) - abx * cxy + abz * cyz));
     const float pyz = O.f[9][c] - delt * flux(wyz);
-    U.f[9][c] = pyz - delt * cyz * ((vy[3] - vy[4]) / dyt + (vz[5] - vz[6]) / dzt) -
+    put(9, pyz - delt * cyz * ((vy[3] - vy[4]) / dyt + (vz[5] - vz[6]) / dzt) -
                 delt * cy * (vz[3] - vz[4]) / dyt - delt * cz * (vy[5] - vy[6]) / dzt -
                 delt * cxy * (vz[1] - vz[2]) / dxt - delt * cxz * (vy[1] - vy[2]) / dxt +
-                delt * sf / rmass * (abx * (cz - cy) - abz * cxz + aby * cxy);
+                delt * sf / rmass * (abx * (cz - cy) - abz * cxz + aby * cxy));
 }
 
 __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k) {
@@ -288,13 +309,14 @@ __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k)
     const float* __restrict__ ex = P.e[0];
     const float* __restrict__ ey = P.e[1];
     const float* __restrict__ ez = P.e[2];
+    auto put_b = [&](int m, float v) { P.bout[m][c] = v; };
     // code/sub_push_bfld.f90:87-97
-    P.bout[0][c] = P.bold[0][c] - delt * (((ez[c + sy] - ez[c - sy]) / dyt) -
-                                          ((ey[c + sz] - ey[c - sz]) / dzt));
-    P.bout[1][c] = P.bold[1][c] + delt * (((ez[c + 1] - ez[c - 1]) / dxt) -
-                                          ((ex[c + sz] - ex[c - sz]) / dzt));
-    P.bout[2][c] = P.bold[2][c] - delt * (((ey[c + 1] - ey[c - 1]) / dxt) -
-                                          ((ex[c + sy] - ex[c - sy]) / dyt));
+    put_b(0, P.bold[0][c] - delt * (((ez[c + sy] - ez[c - sy]) / dyt) -
+                                    ((ey[c + sz] - ey[c - sz]) / dzt)));
+    put_b(1, P.bold[1][c] + delt * (((ez[c + 1] - ez[c - 1]) / dxt) -
+                                    ((ex[c + sz] - ex[c - sz]) / dzt)));
+    put_b(2, P.bold[2][c] - delt * (((ey[c + 1] - ey[c - 1]) / dxt) -
+                                    ((ex[c + sy] - ex[c - sy]) / dyt)));
     if (P.do_elec) {   // code/sub_push_elec.f90:31-47
         const float* __restrict__ w = P.wrkepres;
         const float* __restrict__ evx = P.ev[0];
@@ -311,10 +333,27 @@ __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k)
     }
 }
 
+// Cells the push does not write keep their pre-step values.  With the
+// runtime's buffer rotation the output buffer holds stale data there, so
+// when copy_shell is set the boundary threads copy them from old.
+__device__ inline void push_shell_copy(const PushParams& P, int s, int c) {
+    if (s < 3) {
+#pragma unroll
+        for (int m = 0; m < 10; ++m) P.out[s].f[m][c] = P.old[s].f[m][c];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) P.bout[m][c] = P.bold[m][c];
+        P.epres[c] = P.oldepres[c];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_push(PushParams P) {
-    int i, j, k;
-    if (!cell_of(P.B, gtid(), i, j, k) || !P.B.interior(i, j, k)) return;
-    const int s = blockIdx.y;
+    int i, j, k, s;
+    if (!cell3(P.B, i, j, k, s)) return;
+    if (!P.B.interior(i, j, k)) {
+        if (P.copy_shell) push_shell_copy(P, s, P.B.IX(i, j, k));
+        return;
+    }
     if (s < 3) push_species(P, s, i, j, k);
     else push_field_elec(P, i, j, k);
 }
@@ -349,10 +388,10 @@ __device__ inline float wave_max(float v) {
 
 __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
     const Box& B = Q.B;
-    int i, j, k;
+    int i, j, k, g;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, mc = 0.0f, ma = 0.0f;
     const float dmins[3] = {2.e-4f, 1.e-4f, 2.e-5f};
-    if (cell_of(B, gtid(), i, j, k)) {
+    if (cell3(B, i, j, k, g)) {
         const int c = B.IX(i, j, k);
         const bool inner = B.interior(i, j, k);
         float rho_final[3];
@@ -476,6 +515,9 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
 
 __global__ void k_validate_apply(ValParams Q) {
     const int n = *Q.count;
+    // the next pass appends to the other counter; clear it here (no pass
+    // is using it now) so no separate zeroing launch is needed
+    if (blockIdx.x == 0 && threadIdx.x == 0) *Q.count_next = 0;
     for (int e = gtid(); e < n; e += gridDim.x * blockDim.x) {
         const ValEntry& a = Q.list[e];
         float* const* f = Q.sp[a.species].f;
@@ -505,19 +547,12 @@ __global__ void k_validate_reduce(const float* __restrict__ partials, int nblock
     }
 }
 
-__global__ void k_zero_count(int* count) {
-    if (threadIdx.x == 0) *count = 0;
-}
-
 // ---------------------------------------------------------------------------
-// Lapidus.  blockIdx.y = 0,1,2: a species' 10 fields with its own velocity
+// Lapidus.  group = 0,1,2: a species' 10 fields with its own velocity
 // p/max(rho, smallbit); 3: epres and b with the total ion velocity.
-__global__ void __launch_bounds__(256) k_lap(LapParams L) {
-    int i, j, k;
+__device__ inline void lap_body(const LapParams& L, int g, int i, int j, int k) {
     const Box& B = L.B;
-    if (!cell_of(B, gtid(), i, j, k) || !B.interior(i, j, k)) return;
     const int c = B.IX(i, j, k), X = 1, Y = B.nx, Z = B.nx * B.ny;
-    const int g = blockIdx.y;
     const int off[7] = {0, X, -X, Y, -Y, Z, -Z};
     float vx[7], vy[7], vz[7];
     if (g < 3) {
@@ -547,241 +582,359 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
     const float uzp = fabsf(vz[5] - vz[0]);
     const float uzm = fabsf(vz[0] - vz[6]);
     const float delt = L.delt;
-    auto lap = [&](const float* __restrict__ q, float* w, float chi) {
+    auto lap = [&](const float* __restrict__ q, float* w, float chi, int m) {
         const float q0 = q[c];
-        w[c] = q0 + chi * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +
-                           delt * (uyp * (q[c + Y] - q0) - uym * (q0 - q[c - Y])) +
-                           delt * (uzp * (q[c + Z] - q0) - uzm * (q0 - q[c - Z])));
+        const float v = q0 + chi * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +
+                                    delt * (uyp * (q[c + Y] - q0) - uym * (q0 - q[c - Y])) +
+                                    delt * (uzp * (q[c + Z] - q0) - uzm * (q0 - q[c - Z])));
+        (void)m;
+        w[c] = v;
     };
     if (g < 3) {
-        lap(L.src[g].f[0], L.dst[g].f[0], L.chirho);
+        lap(L.src[g].f[0], L.dst[g].f[0], L.chirho, 0);
         // the reference's field order within lap_plasma does not matter
         // (each output depends only on its own input)
 #pragma unroll
-        for (int m = 1; m < 10; ++m) lap(L.src[g].f[m], L.dst[g].f[m], L.chipxyz);
+        for (int m = 1; m < 10; ++m) lap(L.src[g].f[m], L.dst[g].f[m], L.chipxyz, m);
     } else {
-        lap(L.ep_src, L.ep_dst, L.chipxyz);
+        lap(L.ep_src, L.ep_dst, L.chipxyz, 0);
 #pragma unroll
-        for (int m = 0; m < 3; ++m) lap(L.b_src[m], L.b_dst[m], L.chierg);
+        for (int m = 0; m < 3; ++m) lap(L.b_src[m], L.b_dst[m], L.chierg, 1 + m);
     }
+}
+
+__global__ void __launch_bounds__(256) k_lap(LapParams L) {
+    int i, j, k, g;
+    const Box& B = L.B;
+    if (!cell3(B, i, j, k, g)) return;
+    if (!B.interior(i, j, k)) return;
+    lap_body(L, g, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
-// FCT, two passes over all 34 fields (blockIdx.y = field):
-//   k_fct_diffuse2: P = w + chi*lap7(old) (interior), w (faces) -> scratch
-//   k_fct_limit:    the three limited antidiffusive flux divergences of
-//                   code/sub_fcsmooth.f90:279-431 evaluated at each cell
-//                   from P and w, written to the output field.
+// FCT in one pass (k_fct_fused): a block owns a 64 x 4 (i, j) tile of one
+// field and a run of up to 16 k-planes, and marches it along k.  LDS rings
+// hold the planes the stencils need -- old with a 3-cell (i, j) halo (3
+// planes), wrk and the diffused field P with a 2-cell halo (4 and 5
+// planes) -- so P never leaves the CU and each input value comes from HBM
+// about once.  The global loads of the next plane are issued into
+// registers before the current plane is processed.  The arithmetic is
+// k_fct_diffuse2 + k_fct_limit's, expression for expression.
 
-__global__ void __launch_bounds__(256) k_fct_diffuse2(FctParams F) {
-    int i, j, k;
-    const Box& B = F.B;
-    const int f = blockIdx.y;
-    if (!cell_of(B, gtid(), i, j, k)) return;
-    const int c = B.IX(i, j, k);
-    const float* __restrict__ w = F.wrk[f];
-    const float* __restrict__ o = F.old[f];
-    float* P = F.P + (size_t)f * F.pstride;
-    if (B.interior(i, j, k)) {
-        const int X = 1, Y = B.nx, Z = B.nx * B.ny;
-        P[c] = w[c] + F.chi[f] * (o[c + X] + o[c - X] + o[c + Y] + o[c - Y] + o[c + Z] +
-                                  o[c - Z] - 6.0f * o[c]);
-    } else {
-        P[c] = w[c];
-    }
-}
+constexpr int FTX = 64, FTY = 4, FKC = 16;
+constexpr int FOX = FTX + 6, FOY = FTY + 6;    // old plane with halo 3
+constexpr int FPX = FTX + 4, FPY = FTY + 4;    // wrk / P planes with halo 2
+constexpr int FO_N = FOX * FOY, FP_N = FPX * FPY;
+constexpr int FO_PT = (FO_N + 255) / 256, FP_PT = (FP_N + 255) / 256;
 
-__device__ inline float fct_flux2(const float* __restrict__ w, const float* __restrict__ P, int c,
-                                  int m, int n, int st, float chi) {
+struct FctLds {
+    float o[3][FOY][FOX];
+    float w[4][FPY][FPX];
+    float p[5][FPY][FPX];
+};
+
+// limited antidiffusive flux of one axis at the face above the cell `base`
+// cells along that axis from the output cell (m = its index along the
+// axis, n the extent); W(t) / P(t) read wrk / P t cells along the axis
+template <class WA, class PA>
+__device__ inline float fct_flux_ax(const WA& W, const PA& P, int base, int m, int n, float chi) {
+    int c = base;
     if (m == 0) {
-        c += st;
+        c += 1;
         m = 1;
     } else if (m == n - 1) {
-        c -= st;
+        c -= 1;
         m = n - 2;
     }
-    const float dq = w[c + st] - w[c];
-    const float d1m = P[c] - P[c - st];
-    const float d1p = (m + 1 == n - 1) ? (P[c + st] - P[c]) : (P[c + 2 * st] - P[c + st]);
+    const float dq = W(c + 1) - W(c);
+    const float d1m = P(c) - P(c - 1);
+    const float d1p = (m + 1 == n - 1) ? (P(c + 1) - P(c)) : (P(c + 2) - P(c + 1));
     const float s = copysignf(1.0f, dq);
     return s * fmaxf(0.0f, fminf(fminf(s * d1m, chi * fabsf(dq)), s * d1p));
 }
 
-__global__ void __launch_bounds__(256) k_fct_limit(FctParams F) {
-    int i, j, k;
+__global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
+    __shared__ FctLds L;
     const Box& B = F.B;
-    const int f = blockIdx.y;
-    if (!cell_of(B, gtid(), i, j, k)) return;
-    const int c = B.IX(i, j, k);
-    const float* __restrict__ w = F.wrk[f];
-    const float* __restrict__ P = F.P + (size_t)f * F.pstride;
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    // logical block order: field, k-run, tile row, tile column (fastest)
+    const int per_f = ntx * nty * nkc;
+    const int bl = xcd_block(blockIdx.x, gridDim.x);   // neighbouring tiles on one XCD
+    if (bl >= per_f * F.nf) return;
+    const int f = bl / per_f;
+    const int r = bl % per_f;
+    const int kc = r / (ntx * nty);
+    const int t = r % (ntx * nty);
+    const int i0 = (t % ntx) * FTX, j0 = (t / ntx) * FTY;
+    const int kb = kc * FKC, ke = min(nz, kb + FKC);
+    const float* __restrict__ wg = F.wrk[f];
+    const float* __restrict__ og = F.old[f];
     float* out = F.out[f];
-    if (k == 0 || k == B.nz - 1) {     // z faces keep w
-        out[c] = w[c];
-        return;
-    }
     const float chi = F.chi[f];
-    const int Y = B.nx, Z = B.nx * B.ny;
-    float t;
-    if (i > 0 && i < B.nx - 1) {
-        const float fx = fct_flux2(w, P, c, i, B.nx, 1, chi);
-        const float fxm = fct_flux2(w, P, c - 1, i - 1, B.nx, 1, chi);
-        t = P[c] - fx + fxm;
-    } else {
-        t = w[c];
+    const int tid = threadIdx.x + FTX * threadIdx.y;
+    const int plane = nx * ny;
+
+    float rw[FP_PT], ro[FO_PT];
+    auto fetch_w = [&](int k) {
+#pragma unroll
+        for (int m = 0; m < FP_PT; ++m) {
+            const int e = tid + 256 * m;
+            const int ii = e % FPX, jj = e / FPX;
+            const int i = i0 - 2 + ii, j = j0 - 2 + jj;
+            rw[m] = (e < FP_N && i >= 0 && i < nx && j >= 0 && j < ny)
+                        ? wg[(size_t)k * plane + j * nx + i] : 0.0f;
+        }
+    };
+    auto store_w = [&](int k) {
+        float* dst = &L.w[k % 4][0][0];
+#pragma unroll
+        for (int m = 0; m < FP_PT; ++m) {
+            const int e = tid + 256 * m;
+            if (e < FP_N) dst[e] = rw[m];
+        }
+    };
+    auto fetch_o = [&](int k) {
+#pragma unroll
+        for (int m = 0; m < FO_PT; ++m) {
+            const int e = tid + 256 * m;
+            const int ii = e % FOX, jj = e / FOX;
+            const int i = i0 - 3 + ii, j = j0 - 3 + jj;
+            ro[m] = (e < FO_N && i >= 0 && i < nx && j >= 0 && j < ny)
+                        ? og[(size_t)k * plane + j * nx + i] : 0.0f;
+        }
+    };
+    auto store_o = [&](int k) {
+        float* dst = &L.o[k % 3][0][0];
+#pragma unroll
+        for (int m = 0; m < FO_PT; ++m) {
+            const int e = tid + 256 * m;
+            if (e < FO_N) dst[e] = ro[m];
+        }
+    };
+    // P of plane k on the tile + 2-cell halo (code/sub_fcsmooth.f90:44-83)
+    auto make_p = [&](int k) {
+        float(*dst)[FPX] = L.p[k % 5];
+        const float(*w)[FPX] = L.w[k % 4];
+        const bool kface = (k == 0 || k == nz - 1);
+        const float(*om)[FOX] = L.o[(k + 2) % 3];   // plane k - 1
+        const float(*oc)[FOX] = L.o[k % 3];
+        const float(*op)[FOX] = L.o[(k + 1) % 3];
+        for (int e = tid; e < FP_N; e += 256) {
+            const int ii = e % FPX, jj = e / FPX;
+            const int i = i0 - 2 + ii, j = j0 - 2 + jj;
+            float v = 0.0f;
+            if (i >= 0 && i < nx && j >= 0 && j < ny) {
+                const float wc = w[jj][ii];
+                if (kface || i == 0 || i == nx - 1 || j == 0 || j == ny - 1) {
+                    v = wc;
+                } else {
+                    const int oi = ii + 1, oj = jj + 1;
+                    v = wc + chi * (oc[oj][oi + 1] + oc[oj][oi - 1] + oc[oj + 1][oi] +
+                                    oc[oj - 1][oi] + op[oj][oi] + om[oj][oi] - 6.0f * oc[oj][oi]);
+                }
+            }
+            dst[jj][ii] = v;
+        }
+    };
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx, j = j0 + ty;
+    const int pi = tx + 2, pj = ty + 2;
+    // flanks into the output (inner boxes): the cell's final FCT value is
+    // blended with the coarse box before it is stored
+
+    auto output = [&](int k) {
+        if (i >= nx || j >= ny) return;
+        const float(*wk)[FPX] = L.w[k % 4];
+        const float(*pk)[FPX] = L.p[k % 5];
+        const size_t c = (size_t)k * plane + (size_t)j * nx + i;
+        if (k == 0 || k == nz - 1) {     // z faces keep w
+            out[c] = wk[pj][pi];
+            return;
+        }
+        float tv;
+        if (i > 0 && i < nx - 1) {
+            auto Wx = [&](int o) { return wk[pj][pi + o]; };
+            auto Px = [&](int o) { return pk[pj][pi + o]; };
+            const float fx = fct_flux_ax(Wx, Px, 0, i, nx, chi);
+            const float fxm = fct_flux_ax(Wx, Px, -1, i - 1, nx, chi);
+            tv = pk[pj][pi] - fx + fxm;
+        } else {
+            tv = wk[pj][pi];
+        }
+        if (j > 0 && j < ny - 1) {
+            auto Wy = [&](int o) { return wk[pj + o][pi]; };
+            auto Py = [&](int o) { return pk[pj + o][pi]; };
+            const float fy = fct_flux_ax(Wy, Py, 0, j, ny, chi);
+            const float fym = fct_flux_ax(Wy, Py, -1, j - 1, ny, chi);
+            tv = tv - fy + fym;
+        }
+        auto Wz = [&](int o) { return L.w[(k + o + 4) % 4][pj][pi]; };
+        auto Pz = [&](int o) { return L.p[(k + o + 5) % 5][pj][pi]; };
+        const float fz = fct_flux_ax(Wz, Pz, 0, k, nz, chi);
+        const float fzm = fct_flux_ax(Wz, Pz, -1, k - 1, nz, chi);
+        out[c] = tv - fz + fzm;
+    };
+
+    // P planes pstart..pend are made in order; making P(s) needs wrk(s) and
+    // old(s-1..s+1); output plane k needs P(k-2..k+2) and wrk(k-1..k+1)
+    const int pstart = max(0, kb - 2), pend = min(nz - 1, ke + 1);
+    if (pstart >= 1) {
+        fetch_o(pstart - 1);
+        store_o(pstart - 1);
     }
-    if (j > 0 && j < B.ny - 1) {
-        const float fy = fct_flux2(w, P, c, j, B.ny, Y, chi);
-        const float fym = fct_flux2(w, P, c - Y, j - 1, B.ny, Y, chi);
-        t = t - fy + fym;
+    fetch_o(pstart);
+    store_o(pstart);
+    fetch_w(pstart);
+    if (pstart + 1 <= nz - 1) fetch_o(pstart + 1);
+    for (int s = pstart; s <= pend; ++s) {
+        __syncthreads();   // readers of the ring slots reused below are done
+        store_w(s);
+        if (s + 1 <= nz - 1) store_o(s + 1);
+        if (s + 1 <= pend) {   // next plane's loads in flight during this one
+            fetch_w(s + 1);
+            if (s + 2 <= nz - 1) fetch_o(s + 2);
+        }
+        __syncthreads();
+        make_p(s);
+        __syncthreads();
+        const int k = s - 2;
+        if (k >= kb && k < ke) output(k);
     }
-    const float fz = fct_flux2(w, P, c, k, B.nz, Z, chi);
-    const float fzm = fct_flux2(w, P, c - Z, k - 1, B.nz, Z, chi);
-    out[c] = t - fz + fzm;
+    for (int k = max(kb, pend - 1); k < ke; ++k) output(k);   // planes past the last P made
+}
+
+void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
+    const int ntx = (F.B.nx + FTX - 1) / FTX, nty = (F.B.ny + FTY - 1) / FTY;
+    const int nkc = (F.B.nz + FKC - 1) / FKC;
+    FctParams G = F;
+    G.nf = nf;
+    const int nb = ntx * nty * nkc * nf;
+    ProfScope prof_("k_fct_fused", s);
+    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(FTX, FTY), 0, s, G, ntx, nty, nkc);
 }
 
 // ---------------------------------------------------------------------------
-// flanks on the 2-layer shell only.  Shell cells are enumerated slab by
-// slab: the four z-planes k in {0,1,nz-2,nz-1}, then for 2<=k<=nz-3 the four
-// y-rows j in {0,1,ny-2,ny-1}, then the four x-columns.
-__device__ inline bool shell_cell(const Box& B, int t, int& i, int& j, int& k) {
+// flanks from a precomputed shell table: one thread per (shell cell, field)
+int shell_count(const Box& B) {
     const int nx = B.nx, ny = B.ny, nz = B.nz;
-    const int plane = nx * ny;
-    if (t < 4 * plane) {
-        const int s = t / plane, r = t % plane;
-        k = (s < 2) ? s : nz - 4 + s;
-        j = r / nx;
-        i = r % nx;
-        return true;
-    }
-    t -= 4 * plane;
-    const int nzi = nz - 4;
-    const int rows = 4 * nx * nzi;
-    if (t < rows) {
-        const int kk = t / (4 * nx), r = t % (4 * nx);
-        const int s = r / nx;
-        i = r % nx;
-        j = (s < 2) ? s : ny - 4 + s;
-        k = 2 + kk;
-        return true;
-    }
-    t -= rows;
-    const int nyi = ny - 4;
-    if (t < 4 * nyi * nzi) {
-        const int kk = t / (4 * nyi), r = t % (4 * nyi);
-        const int s = r / nyi;
-        j = 2 + r % nyi;
-        i = (s < 2) ? s : nx - 4 + s;
-        k = 2 + kk;
-        return true;
-    }
-    return false;
+    return 4 * nx * ny + 4 * nx * (nz - 4) + 4 * (ny - 4) * (nz - 4);
 }
 
-__global__ void __launch_bounds__(256) k_flanks_shell(Box B, FlankPtrs Fp, FlankGeom G, float t1,
-                                                      float t2) {
-    int i, j, k;
-    const int f = blockIdx.y;
-    if (!shell_cell(B, gtid(), i, j, k)) return;
+void build_shell(const Box& B, const FlankGeom& G, ShellPt* out) {
     const int nx = B.nx, ny = B.ny, nz = B.nz;
-    const bool bot = k <= 1, top = k >= nz - 2, fro = j <= 1, bac = j >= ny - 2, lef = i <= 1,
-               rig = i >= nx - 2;
-    const float ax = G.sxmin + G.sx * (float)i;
-    const float ai = 1.0f + (ax - G.bxmin) / G.delx;
-    const int ib = (int)ai;
-    const float dx = ai - (float)ib, ddx = 1.0f - dx;
-    const float ay = G.symin + G.sy * (float)j;
-    const float aj = 1.0f + (ay - G.bymin) / G.dely;
-    const int jb = (int)aj;
-    const float dy = aj - (float)jb, ddy = 1.0f - dy;
-    const float az = G.szmin + G.sz * (float)k;
-    const float ak = 1.0f + (az - G.bzmin) / G.delz;
-    const int kb = (int)ak;
-    const float dz = ak - (float)kb, ddz = 1.0f - dz;
+    int n = 0;
+    auto add = [&](int i, int j, int k) {
+        ShellPt p;
+        p.c = B.IX(i, j, k);
+        const float ax = G.sxmin + G.sx * (float)i;
+        const float ai = 1.0f + (ax - G.bxmin) / G.delx;
+        const int ib = (int)ai;
+        const float ay = G.symin + G.sy * (float)j;
+        const float aj = 1.0f + (ay - G.bymin) / G.dely;
+        const int jb = (int)aj;
+        const float az = G.szmin + G.sz * (float)k;
+        const float ak = 1.0f + (az - G.bzmin) / G.delz;
+        const int kb = (int)ak;
+        p.dx = ai - (float)ib;
+        p.dy = aj - (float)jb;
+        p.dz = ak - (float)kb;
+        p.cb = B.IX(ib - 1, jb - 1, kb - 1);
+        int f = 0;
+        if (k <= 1) f |= 1 | (k == 1 ? 1 << 8 : 0);
+        if (k >= nz - 2) f |= 2 | (k == nz - 2 ? 2 << 8 : 0);
+        if (j <= 1) f |= 4 | (j == 1 ? 4 << 8 : 0);
+        if (j >= ny - 2) f |= 8 | (j == ny - 2 ? 8 << 8 : 0);
+        if (i <= 1) f |= 16 | (i == 1 ? 16 << 8 : 0);
+        if (i >= nx - 2) f |= 32 | (i == nx - 2 ? 32 << 8 : 0);
+        p.flags = f;
+        out[n++] = p;
+    };
+    const int ks[4] = {0, 1, nz - 2, nz - 1};
+    const int js[4] = {0, 1, ny - 2, ny - 1};
+    const int is[4] = {0, 1, nx - 2, nx - 1};
+    for (int s = 0; s < 4; ++s)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) add(i, j, ks[s]);
+    for (int k = 2; k <= nz - 3; ++k)
+        for (int s = 0; s < 4; ++s)
+            for (int i = 0; i < nx; ++i) add(i, js[s], k);
+    for (int k = 2; k <= nz - 3; ++k)
+        for (int s = 0; s < 4; ++s)
+            for (int j = 2; j <= ny - 3; ++j) add(is[s], j, k);
+}
+
+__global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict__ tab, int npts,
+                                                      FlankPtrs Fp, float t1, float t2, int X,
+                                                      int Y, int Z) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= npts) return;
+    const int f = blockIdx.y;
+    const ShellPt p = tab[t];
+    const float dx = p.dx, dy = p.dy, dz = p.dz;
+    const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
     const float* __restrict__ o = Fp.old[f];
     const float* __restrict__ q = Fp.cur[f];
-    auto W = [&](int ii, int jj, int kk) {
-        const int cc = B.IX(ii - 1, jj - 1, kk - 1);
-        return o[cc] * t1 + q[cc] * t2;
-    };
-    const float I0 = W(ib, jb, kb) * ddx * ddy * ddz + W(ib, jb, kb + 1) * ddx * ddy * dz +
-                     W(ib, jb + 1, kb) * ddx * dy * ddz + W(ib, jb + 1, kb + 1) * ddx * dy * dz +
-                     W(ib + 1, jb, kb) * dx * ddy * ddz + W(ib + 1, jb, kb + 1) * dx * ddy * dz +
-                     W(ib + 1, jb + 1, kb) * dx * dy * ddz +
-                     W(ib + 1, jb + 1, kb + 1) * dx * dy * dz;
+    const int c0 = p.cb;
+    auto W = [&](int c) { return o[c] * t1 + q[c] * t2; };
+    const float I0 = W(c0) * ddx * ddy * ddz + W(c0 + Z) * ddx * ddy * dz +
+                     W(c0 + Y) * ddx * dy * ddz + W(c0 + Y + Z) * ddx * dy * dz +
+                     W(c0 + X) * dx * ddy * ddz + W(c0 + X + Z) * dx * ddy * dz +
+                     W(c0 + X + Y) * dx * dy * ddz + W(c0 + X + Y + Z) * dx * dy * dz;
     float* tgt = Fp.tgt[f];
-    const int c = B.IX(i, j, k);
-    float w = tgt[c];
-    auto blend = [&](bool inner) {
-        const float frac = inner ? 0.5f : 0.0f;
-        const float frac1 = 1.0f - frac;
-        w = frac * w + frac1 * I0;
-    };
-    if (bot) blend(k == 1);
-    if (top) blend(k == nz - 2);
-    if (fro) blend(j == 1);
-    if (bac) blend(j == ny - 2);
-    if (lef) blend(i == 1);
-    if (rig) blend(i == nx - 2);
-    tgt[c] = w;
+    float w = tgt[p.c];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {   // bottom, top, front, back, left, right
+        if (p.flags & (1 << b)) {
+            const float frac = (p.flags & (256 << b)) ? 0.5f : 0.0f;
+            const float frac1 = 1.0f - frac;
+            w = frac * w + frac1 * I0;
+        }
+    }
+    tgt[p.c] = w;
 }
 
-int shell_cells(const Box& B) {
-    return B.V() - (B.nx - 4) * (B.ny - 4) * (B.nz - 4);
+void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const FlankPtrs& Fp, int nf,
+                         float t1, float t2, hipStream_t s) {
+    if (npts <= 0 || nf <= 0) return;
+    ProfScope prof_("k_flanks_table", s);
+    hipLaunchKernelGGL(k_flanks_table, dim3((npts + 255) / 256, nf), dim3(256), 0, s, tab, npts,
+                       Fp, t1, t2, 1, B.nx, B.nx * B.ny);
 }
+
 
 // ---------------------------------------------------------------------------
 // host launchers
 void launch_efield(const EParams& P, hipStream_t s) {
-    const int n = P.B.V();
-    hipLaunchKernelGGL(k_efield, dim3((n + 255) / 256), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_efield, grid3(P.B, 1), block3(), 0, s, P);
 }
 
 void launch_push(const PushParams& P, hipStream_t s) {
-    const int n = P.B.V();
     ProfScope prof_("k_push", s);
-    hipLaunchKernelGGL(k_push, dim3((n + 255) / 256, 4), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_push, grid3(P.B, 4), block3(), 0, s, P);
 }
 
-int validate_blocks(const Box& B) { return (B.V() + 255) / 256; }
+int validate_blocks(const Box& B) {
+    const dim3 g = grid3(B, 1);
+    return (int)(g.x * g.y * g.z);
+}
 
 void launch_validate(const ValParams& Q, float* maxima, hipStream_t s) {
-    const int n = Q.B.V();
-    const int nb = (n + 255) / 256;
+    const int nb = validate_blocks(Q.B);
     {
         ProfScope prof_("k_validate", s);
-        hipLaunchKernelGGL(k_validate, dim3(nb), dim3(256), 0, s, Q);
+        hipLaunchKernelGGL(k_validate, grid3(Q.B, 1), block3(), 0, s, Q);
     }
     hipLaunchKernelGGL(k_validate_apply, dim3(128), dim3(256), 0, s, Q);
     if (Q.partials && maxima)
         hipLaunchKernelGGL(k_validate_reduce, dim3(1), dim3(256), 0, s, Q.partials, nb, maxima,
                            Q.count);
-    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, Q.count);
 }
 
 void launch_lap(const LapParams& L, hipStream_t s) {
-    const int n = L.B.V();
     ProfScope prof_("k_lap", s);
-    hipLaunchKernelGGL(k_lap, dim3((n + 255) / 256, 4), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(k_lap, grid3(L.B, 4), block3(), 0, s, L);
 }
 
-void launch_fct2(const FctParams& F, int nf, hipStream_t s) {
-    const int n = F.B.V();
-    const dim3 g((n + 255) / 256, nf), t(256);
-    {
-        ProfScope prof_("k_fct_diffuse2", s);
-        hipLaunchKernelGGL(k_fct_diffuse2, g, t, 0, s, F);
-    }
-    ProfScope prof_("k_fct_limit", s);
-    hipLaunchKernelGGL(k_fct_limit, g, t, 0, s, F);
-}
 
-void launch_flanks_shell(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
-                         float t2, hipStream_t s) {
-    const int n = shell_cells(B);
-    ProfScope prof_("k_flanks_shell", s);
-    hipLaunchKernelGGL(k_flanks_shell, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp, G, t1,
-                       t2);
-}
 
 }  // namespace g3m3

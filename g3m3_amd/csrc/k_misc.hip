@@ -1,6 +1,7 @@
 // Small kernels: the general banded product of code/sub_banded_tpose.f90
 // (for the drop-in entry point; the divB solve itself is matrix-free) and
 // batched trilinear sampling of code/cft_trilin_interp.f90:10-66.
+#include "g3m3_craft.h"
 #include "g3m3_dev.h"
 #include "g3m3_launch.h"
 
@@ -52,6 +53,88 @@ void launch_trilin(int npts, const float* sxyz, const float* gridpts, const floa
     if (npts <= 0) return;
     hipLaunchKernelGGL(k_trilin, dim3((npts + 255) / 256), dim3(256), 0, s, npts, sxyz, gridpts,
                        qty, out);
+}
+
+// ---------------------------------------------------------------------------
+// code/sub_fnd_pres.f90:29-62: parallel / perpendicular / cross pressures
+__global__ void k_fnd_pres(int n, const float* __restrict__ px, const float* __restrict__ py,
+                           const float* __restrict__ pz, const float* __restrict__ vx,
+                           const float* __restrict__ vy, const float* __restrict__ vz,
+                           const float* __restrict__ bx, const float* __restrict__ by,
+                           const float* __restrict__ bz, float* para, float* perp, float* cross) {
+    const int c = gtid();
+    if (c >= n) return;
+    const float x = px[c], y = py[c], z = pz[c];
+    const float presmag = sqrtf(x * x + y * y + z * z) + 1.e-11f;
+    const float abx = bx[c], aby = by[c], abz = bz[c];
+    const float bmag = sqrtf(abx * abx + aby * aby + abz * abz) + 1.e-12f;
+    const float avx = vx[c], avy = vy[c], avz = vz[c];
+    const float vcbx = (avy * abz - avz * aby) / bmag;
+    const float vcby = -(avx * abz - avz * abx) / bmag;
+    const float vcbz = (avx * aby - avy * abx) / bmag;
+    const float vcmag = sqrtf(vcbx * vcbx + vcby * vcby + vcbz * vcbz) + 1.e-12f;
+    const float ax = abx * x, ay = aby * y, az = abz * z;
+    const float pa = sqrtf(ax * ax + ay * ay + az * az) / bmag;
+    const float cx = vcbx * x, cy = vcby * y, cz = vcbz * z;
+    const float pc = sqrtf(cx * cx + cy * cy + cz * cz) / vcmag;
+    para[c] = pa;
+    cross[c] = pc;
+    perp[c] = sqrtf(fabsf(presmag * presmag - (pa * pa + pc * pc)));
+}
+
+void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],
+                     const float* const b[3], float* para, float* perp, float* cross,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_fnd_pres, dim3((n + 255) / 256), dim3(256), 0, s, n, p[0], p[1], p[2],
+                       v[0], v[1], v[2], b[0], b[1], b[2], para, perp, cross);
+}
+
+// ---------------------------------------------------------------------------
+// Craft sampling at many points (code/cft_craft_handling.f90:165-195 with
+// code/cft_record_aux_data.f90:27-30): findgrid, the aux cube, the 2x2x2
+// cube of every field and trilin_interp, one point per thread.  A point
+// whose cube would leave its box gets gridpt[0] = 0 and zero samples.
+__global__ void k_sample(int npts, const float* __restrict__ sxyz, CraftGrid G, int nx, int ny,
+                         int nz, int nq, FieldPtrs F, float* out, int* gridpt) {
+    const int p = gtid();
+    if (p >= npts) return;
+    const float s[3] = {sxyz[3 * p], sxyz[3 * p + 1], sxyz[3 * p + 2]};
+    int gp[4];
+    craft_findgrid(G, s, gp);
+    const bool ok = gp[0] >= 1 && gp[0] <= nx - 1 && gp[1] >= 1 && gp[1] <= ny - 1 &&
+                    gp[2] >= 1 && gp[2] <= nz - 1;
+    if (!ok) gp[0] = 0;
+    for (int a = 0; a < 4; ++a) gridpt[4 * p + a] = gp[a];
+    if (!ok) {
+        for (int q = 0; q < nq; ++q) out[q + nq * p] = 0.0f;
+        return;
+    }
+    float v0[3], v1[3];
+    craft_cube(G, gp, v0, v1);
+    const float xd = (s[0] - v0[0]) / (v1[0] - v0[0]);
+    const float yd = (s[1] - v0[1]) / (v1[1] - v0[1]);
+    const float zd = (s[2] - v0[2]) / (v1[2] - v0[2]);
+    const size_t V = (size_t)nx * ny * nz;
+    const size_t c = (size_t)(gp[3] - 1) * V + (size_t)(gp[2] - 1) * nx * ny +
+                     (size_t)(gp[1] - 1) * nx + (gp[0] - 1);
+    const size_t Y = nx, Z = (size_t)nx * ny;
+    for (int q = 0; q < nq; ++q) {
+        const float* f = F.p[q];
+        const float e00 = f[c] * (1.0f - xd) + f[c + 1] * xd;
+        const float e01 = f[c + Z] * (1.0f - xd) + f[c + Z + 1] * xd;
+        const float e10 = f[c + Y] * (1.0f - xd) + f[c + Y + 1] * xd;
+        const float e11 = f[c + Y + Z] * (1.0f - xd) + f[c + Y + Z + 1] * xd;
+        const float f0 = e00 * (1.0f - yd) + e10 * yd;
+        const float f1 = e01 * (1.0f - yd) + e11 * yd;
+        out[q + nq * p] = f0 * (1.0f - zd) + f1 * zd;
+    }
+}
+
+void launch_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
+                   int nq, const FieldPtrs& F, float* out, int* gridpt, hipStream_t s) {
+    if (npts <= 0) return;
+    hipLaunchKernelGGL(k_sample, dim3((npts + 255) / 256), dim3(256), 0, s, npts, sxyz, G, nx, ny,
+                       nz, nq, F, out, gridpt);
 }
 
 }  // namespace g3m3

@@ -26,6 +26,11 @@ enum Fld {   // canonical field order
 };
 constexpr int NF = 34;
 
+// one box's slice of every field
+struct Set {
+    float* f[NF];
+};
+
 struct Run {
     g3m3_run_desc d;
     Box B;
@@ -33,7 +38,6 @@ struct Run {
     hipStream_t s = nullptr;
     // scratch (one box)
     float* scr = nullptr;         // e(3) ev(3) tv(3) bs(3)
-    float* fctP = nullptr;        // 34 * n
     ValEntry* list = nullptr;
     int* count = nullptr;
     float* partials = nullptr;
@@ -44,8 +48,24 @@ struct Run {
     long long cell_updates = 0, box_steps = 0;
     float iso_flag = 1.0f;
     int err = 0;
+    // Buffer rotation: instead of copying the state into old at the start of
+    // a box-step (code/3d_main.f90:2193-2229) the two slices swap roles.
+    // rot[b] set: box b's state lives in the old array and its old state in
+    // the state array.
+    std::vector<char> rot;
+    int vpass = 0;                // which validation counter the next pass uses
+    // flank shell tables of the inner boxes (geometry is fixed for a run)
+    ShellPt* shell[G3M3_MAX_GRIDS] = {};
+    int nshell = 0;
 
-    float* fld(float* const* set, int f, int box) const { return set[f] + (size_t)(box - 1) * n; }
+    Set slice(float* const* arrays, int box) const {
+        Set S;
+        for (int f = 0; f < NF; ++f) S.f[f] = arrays[f] + (size_t)(box - 1) * n;
+        return S;
+    }
+    Set state(int box) const { return slice(rot[box - 1] ? d.old : d.state, box); }
+    Set old(int box) const { return slice(rot[box - 1] ? d.state : d.old, box); }
+    Set wrk(int box) const { return slice(d.wrk, box); }
 };
 
 void check(Run* R, const char* where) {
@@ -56,28 +76,28 @@ void check(Run* R, const char* where) {
     }
 }
 
-SpeciesSet species(const Run* R, float* const* set, int s, int box) {
-    SpeciesSet S;
-    for (int m = 0; m < 10; ++m) S.f[m] = R->fld(set, 10 * s + m, box);
-    return S;
+SpeciesSet species(const Set& S, int s) {
+    SpeciesSet Q;
+    for (int m = 0; m < 10; ++m) Q.f[m] = S.f[10 * s + m];
+    return Q;
 }
 
 // -- stage pieces ---------------------------------------------------------------
-void efield(Run* R, float* const* src, int box) {
+void efield(Run* R, const Set& src, int box) {
     const g3m3_run_desc& d = R->d;
     EParams P;
     P.B = R->B;
     for (int c = 0; c < 3; ++c) {
-        P.b[c] = R->fld(src, BX + c, box);
+        P.b[c] = src.f[BX + c];
         P.b0[c] = (c == 0 ? d.bx0 : c == 1 ? d.by0 : d.bz0) + (size_t)(box - 1) * R->n;
-        P.rho[c] = R->fld(src, 10 * c, box);
-        for (int m = 0; m < 3; ++m) P.mom[3 * c + m] = R->fld(src, 10 * c + 4 + m, box);
+        P.rho[c] = src.f[10 * c];
+        for (int m = 0; m < 3; ++m) P.mom[3 * c + m] = src.f[10 * c + 4 + m];
         P.e[c] = R->scr + (size_t)(0 + c) * R->n;
         P.ev[c] = R->scr + (size_t)(3 + c) * R->n;
         P.tv[c] = R->scr + (size_t)(6 + c) * R->n;
         P.bs[c] = R->scr + (size_t)(9 + c) * R->n;
     }
-    P.ep = R->fld(src, EPRES, box);
+    P.ep = src.f[EPRES];
     P.rst = (box <= d.mbndry && d.resist < 5000.0f) ? R->rst_eff : nullptr;
     P.rmq = d.rmassq;
     P.rmh = d.rmassh;
@@ -91,27 +111,27 @@ void efield(Run* R, float* const* src, int box) {
     launch_efield(P, R->s);
 }
 
-void push(Run* R, float* const* out, float* const* old, float* const* src, int box, float delt,
-          int do_elec) {
+void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float delt,
+          int do_elec, int copy_shell) {
     const g3m3_run_desc& d = R->d;
     PushParams P;
     P.B = R->B;
     for (int s = 0; s < 3; ++s) {
-        P.src[s] = species(R, src, s, box);
-        P.old[s] = species(R, old, s, box);
-        P.out[s] = species(R, out, s, box);
+        P.src[s] = species(src, s);
+        P.old[s] = species(old, s);
+        P.out[s] = species(out, s);
     }
     for (int c = 0; c < 3; ++c) {
         P.e[c] = R->scr + (size_t)(0 + c) * R->n;
         P.ev[c] = R->scr + (size_t)(3 + c) * R->n;
         P.tv[c] = R->scr + (size_t)(6 + c) * R->n;
         P.bs[c] = R->scr + (size_t)(9 + c) * R->n;
-        P.bout[c] = R->fld(out, BX + c, box);
-        P.bold[c] = R->fld(old, BX + c, box);
+        P.bout[c] = out.f[BX + c];
+        P.bold[c] = old.f[BX + c];
     }
-    P.wrkepres = R->fld(R->d.wrk, EPRES, box);
-    P.oldepres = R->fld(old, EPRES, box);
-    P.epres = R->fld(out, EPRES, box);
+    P.wrkepres = R->wrk(box).f[EPRES];
+    P.oldepres = old.f[EPRES];
+    P.epres = out.f[EPRES];
     P.rmass[0] = d.rmassq;
     P.rmass[1] = d.rmassh;
     P.rmass[2] = d.rmasso;
@@ -137,10 +157,11 @@ void push(Run* R, float* const* out, float* const* old, float* const* src, int b
     P.gamma = d.gamma;
     P.gamma1 = d.gamma1;
     P.do_elec = do_elec;
+    P.copy_shell = copy_shell;
     launch_push(P, R->s);
 }
 
-void outer_bc(Run* R, float* const* set) {
+void outer_bc(Run* R, const Set& set) {
     const g3m3_run_desc& d = R->d;
     const int box = d.n_grids;
     const float ofrac = d.rho_frac * d.o_conc;
@@ -159,16 +180,16 @@ void outer_bc(Run* R, float* const* set) {
     const int kinds[10] = {kRHO, kPRES, kPRES, kPRES, kPX, kPY, kPZ, kOFF, kOFF, kOFF};
     for (int s = 0; s < 3; ++s)
         for (int m = 0; m < 10; ++m) {
-            O.f[10 * s + m] = R->fld(set, 10 * s + m, box);
+            O.f[10 * s + m] = set.f[10 * s + m];
             O.kind[10 * s + m] = kinds[m];
             O.val[10 * s + m] = sval[s][m];
         }
-    O.f[EPRES] = R->fld(set, EPRES, box);
+    O.f[EPRES] = set.f[EPRES];
     O.kind[EPRES] = kEP;
     O.val[EPRES] = pq / d.ti_te;
     const float sb[3] = {d.sbx_wind, d.sby_wind, d.sbz_wind};
     for (int c = 0; c < 3; ++c) {
-        O.f[BX + c] = R->fld(set, BX + c, box);
+        O.f[BX + c] = set.f[BX + c];
         O.kind[BX + c] = kBX + c;
         O.val[BX + c] = sb[c];
     }
@@ -196,26 +217,27 @@ FlankGeom geom(const g3m3_run_desc& d, int box) {
     return G;
 }
 
-void flanks(Run* R, float* const* tgt, int box, double t_grid) {
+void flanks(Run* R, const Set& tgt, int box, double t_grid) {
     const int mb = box;   // 0-based index of box+1
     const float t1 = (float)((R->t_new[mb] - t_grid) / (R->t_new[mb] - R->t_old[mb]));
     const float t2 = 1.0f - t1;
     FlankPtrs P{};
+    const Set cur = R->state(box + 1), old = R->old(box + 1);
     for (int f = 0; f < NF; ++f) {
-        P.tgt[f] = R->fld(tgt, f, box);
-        P.cur[f] = R->fld(R->d.state, f, box + 1);
-        P.old[f] = R->fld(R->d.old, f, box + 1);
+        P.tgt[f] = tgt.f[f];
+        P.cur[f] = cur.f[f];
+        P.old[f] = old.f[f];
     }
-    launch_flanks_shell(R->B, P, NF, geom(R->d, box), t1, t2, R->s);
+    launch_flanks_table(R->B, R->shell[box - 1], R->nshell, P, NF, t1, t2, R->s);
 }
 
-void inner_bc(Run* R, float* const* set) {
+void inner_bc(Run* R, const Set& set) {
     const g3m3_run_desc& d = R->d;
     const int box = d.mbndry;
     InnerPtrs I{};
     // InnerPtrs order: q,h,o x (rho,presx,presy,presz,px,py,pz,presxy,presxz,presyz),
     // epres, bx, by, bz -- the canonical order
-    for (int f = 0; f < NF; ++f) I.f[f] = R->fld(set, f, box);
+    for (int f = 0; f < NF; ++f) I.f[f] = set.f[f];
     const int* ij[3] = {d.ijsrf, d.ijmid, d.ijzero};
     const float* parm[3] = {d.parm_srf, d.parm_mid, d.parm_zero};
     const int num[3] = {d.numsrf, d.nummid, d.numzero};
@@ -226,18 +248,20 @@ void inner_bc(Run* R, float* const* set) {
                        d.re_equiv, d.v_rot, R->s);
 }
 
-void validate(Run* R, float* const* set, int box, bool want_max) {
+void validate(Run* R, const Set& set, int box, bool want_max) {
     const g3m3_run_desc& d = R->d;
     ValParams Q;
     Q.B = R->B;
-    for (int s = 0; s < 3; ++s) Q.sp[s] = species(R, set, s, box);
-    Q.ep = R->fld(set, EPRES, box);
+    for (int s = 0; s < 3; ++s) Q.sp[s] = species(set, s);
+    Q.ep = set.f[EPRES];
     for (int c = 0; c < 3; ++c) {
-        Q.b[c] = R->fld(set, BX + c, box);
+        Q.b[c] = set.f[BX + c];
         Q.b0[c] = (c == 0 ? d.bx0 : c == 1 ? d.by0 : d.bz0) + (size_t)(box - 1) * R->n;
     }
     Q.list = R->list;
-    Q.count = R->count;
+    Q.count = R->count + 16 * R->vpass;           // separate 64-byte lines
+    Q.count_next = R->count + 16 * (R->vpass ^ 1);
+    R->vpass ^= 1;
     Q.partials = want_max ? R->partials : nullptr;
     if (d.isotropic) R->iso_flag = 0.0f;
     Q.aniso = d.aniso_limit;
@@ -255,16 +279,17 @@ void validate(Run* R, float* const* set, int box, bool want_max) {
     launch_validate(Q, want_max ? R->maxima + 5 * (box - 1) : nullptr, R->s);
 }
 
-void set_rho_only(Run* R, float* const* set, int box) {
+void set_rho_only(Run* R, const Set& set) {
     SpeciesPtrs S{};
     for (int s = 0; s < 3; ++s)
-        for (int m = 0; m < 10; ++m) S.f[s][m] = R->fld(set, 10 * s + m, box);
-    launch_set_rho(R->n, S, R->fld(set, EPRES, box), R->s);
+        for (int m = 0; m < 10; ++m) S.f[s][m] = set.f[10 * s + m];
+    launch_set_rho(R->n, S, set.f[EPRES], R->s);
 }
 
-void bounds(Run* R, float* const* set, float* const* flank_tgt, int box, double t_grid) {
+void bounds(Run* R, const Set& set, const Set& flank_tgt, int box, double t_grid,
+            bool do_flanks = true) {
     if (box == R->d.n_grids) outer_bc(R, set);
-    else flanks(R, flank_tgt, box, t_grid);
+    else if (do_flanks) flanks(R, flank_tgt, box, t_grid);
     if (box <= R->d.mbndry) inner_bc(R, set);
 }
 
@@ -272,15 +297,16 @@ void lapidus(Run* R, int box, float delt) {
     const g3m3_run_desc& d = R->d;
     LapParams L;
     L.B = R->B;
+    const Set st = R->state(box), wk = R->wrk(box);
     for (int s = 0; s < 3; ++s) {
-        L.src[s] = species(R, d.state, s, box);
-        L.dst[s] = species(R, d.wrk, s, box);
+        L.src[s] = species(st, s);
+        L.dst[s] = species(wk, s);
     }
-    L.ep_src = R->fld(d.state, EPRES, box);
-    L.ep_dst = R->fld(d.wrk, EPRES, box);
+    L.ep_src = st.f[EPRES];
+    L.ep_dst = wk.f[EPRES];
     for (int c = 0; c < 3; ++c) {
-        L.b_src[c] = R->fld(d.state, BX + c, box);
-        L.b_dst[c] = R->fld(d.wrk, BX + c, box);
+        L.b_src[c] = st.f[BX + c];
+        L.b_dst[c] = wk.f[BX + c];
     }
     L.chirho = d.chirho;
     L.chipxyz = d.chipxyz;
@@ -294,15 +320,14 @@ void fct(Run* R, int box) {
     const g3m3_run_desc& d = R->d;
     FctParams F;
     F.B = R->B;
+    const Set st = R->state(box), ol = R->old(box), wk = R->wrk(box);
     for (int f = 0; f < NF; ++f) {
-        F.out[f] = R->fld(d.state, f, box);
-        F.old[f] = R->fld(d.old, f, box);
-        F.wrk[f] = R->fld(d.wrk, f, box);
+        F.out[f] = st.f[f];
+        F.old[f] = ol.f[f];
+        F.wrk[f] = wk.f[f];
         F.chi[f] = (f >= BX) ? d.diferg : d.difrho;
     }
-    F.P = R->fctP;
-    F.pstride = (size_t)R->n;
-    launch_fct2(F, NF, R->s);
+    launch_fct_fused(F, NF, R->s);
 }
 
 void corer(Run* R, int box) {
@@ -317,9 +342,10 @@ void corer(Run* R, int box) {
     const int nby1 = (int)(1.0f + (d.grd_ymin[ms] - d.grd_ymin[mb]) / dy + 0.5f);
     const int nbz1 = (int)(1.0f + (d.grd_zmin[ms] - d.grd_zmin[mb]) / dz + 0.5f);
     FieldPtrs fine{}, coarse{};
+    const Set fs = R->state(box), cs = R->state(box + 1);
     for (int f = 0; f < NF; ++f) {
-        fine.p[f] = R->fld(d.state, f, box);
-        coarse.p[f] = R->fld(d.state, f, box + 1);
+        fine.p[f] = fs.f[f];
+        coarse.p[f] = cs.f[f];
     }
     launch_corer(R->B, fine, coarse, NF, nskip, nbx1, nby1, nbz1, R->s);
 }
@@ -332,42 +358,44 @@ void box_step(Run* R, int box) {
     R->t_new[b] = R->t_old[b] + R->t_step[b];
     const float delt = (float)R->t_step[b];
     const float delt2 = delt / 2.0f;
-    // store_array x34 (:2193-2229)
-    {
-        FieldPtrs dst{}, src{};
-        for (int f = 0; f < NF; ++f) {
-            dst.p[f] = R->fld(d.old, f, box);
-            src.p[f] = R->fld(d.state, f, box);
-        }
-        launch_copy_fields(R->n, NF, dst, src, R->s);
-    }
+    // store_array x34 (:2193-2229) by rotation: the state becomes old and
+    // step 2 writes the new state into the other slice (its push copies the
+    // cells it does not compute from old)
+    R->rot[b] ^= 1;
+    const Set st = R->state(box), ol = R->old(box), wk = R->wrk(box);
     auto clamp_t = [&](double tg) {
         if (tg > R->t_new[b + 1]) tg = R->t_new[b + 1];
         if (tg < R->t_old[b + 1]) tg = R->t_old[b + 1];
         return tg;
     };
     const bool inner_box = box < d.n_grids;
-    // step 1 (:2243-2458); its push_elec only writes the state epres interior,
-    // which step 2 overwrites before any read, so it is not issued
-    efield(R, d.state, box);
-    push(R, d.wrk, d.old, d.state, box, 0.5f * delt, 0);
+    // step 1 (:2243-2458), reading the state from old (same values); its
+    // push_elec only writes the state epres interior, which step 2
+    // overwrites before any read, so it is not issued
+    efield(R, ol, box);
+    push(R, wk, ol, ol, box, 0.5f * delt, 0, 0);
     double tg = inner_box ? clamp_t(R->t_old[b] + (double)delt2) : 0.0;
-    bounds(R, d.wrk, d.wrk, box, tg);
-    validate(R, d.wrk, box, false);
-    // step 2 (:2476-2684); inner boxes put their flanks into wrk
-    efield(R, d.wrk, box);
-    push(R, d.state, d.old, d.wrk, box, delt, 1);
+    bounds(R, wk, wk, box, tg);
+    validate(R, wk, box, false);
+    // step 2 (:2476-2684).  Inner boxes put these flanks into wrk, where
+    // nothing reads them: the Lapidus stage overwrites the wrk interior
+    // (layer 2 of the shell) and its own flanks, same t_grid, set layer 1
+    // to the same interpolant with weight 1 (w = 0*w + I0) before any read.
+    // So the call is not issued (results equal up to the sign of an exact
+    // zero).
+    efield(R, wk, box);
+    push(R, st, ol, wk, box, delt, 1, 1);
     tg = inner_box ? clamp_t(R->t_old[b] + (double)delt) : 0.0;
-    bounds(R, d.state, d.wrk, box, tg);
-    validate(R, d.state, box, false);
+    bounds(R, st, wk, box, tg, false);
+    validate(R, st, box, false);
     // Lapidus (:2692-2890), same t_grid
     lapidus(R, box, delt);
-    bounds(R, d.wrk, d.wrk, box, tg);
-    validate(R, d.wrk, box, false);
+    bounds(R, wk, wk, box, tg);
+    validate(R, wk, box, false);
     // FCT (:2896-3030)
     fct(R, box);
-    bounds(R, d.state, d.state, box, tg);
-    validate(R, d.state, box, true);
+    bounds(R, st, st, box, tg);
+    validate(R, st, box, true);
     // nested sync (:3045-3063)
     if (inner_box && std::fabs(R->t_new[b] - R->t_new[b + 1]) <= 0.005 * R->t_step[b]) {
         corer(R, box);
@@ -402,10 +430,14 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
     R->B = Box{desc->nx, desc->ny, desc->nz};
     R->n = R->B.V();
     const int n = R->n, ng = desc->n_grids;
+    if (n >= kFastDivMax || desc->nx < 5 || desc->ny < 5 || desc->nz < 5) {
+        fprintf(stderr, "g3m3_run_create: box extents out of range\n");
+        delete R;
+        return nullptr;
+    }
     bool ok = hipMalloc(&R->scr, sizeof(float) * 12 * (size_t)n) == hipSuccess &&
-              hipMalloc(&R->fctP, sizeof(float) * NF * (size_t)n) == hipSuccess &&
               hipMalloc(&R->list, sizeof(ValEntry) * 3 * (size_t)n) == hipSuccess &&
-              hipMalloc(&R->count, 64) == hipSuccess &&
+              hipMalloc(&R->count, 128) == hipSuccess &&
               hipMalloc(&R->partials, sizeof(float) * 5 * (size_t)validate_blocks(R->B)) ==
                   hipSuccess &&
               hipMalloc(&R->maxima, sizeof(float) * 5 * ng) == hipSuccess;
@@ -433,17 +465,26 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
             hipMemcpy(R->rst_eff, h.data(), sizeof(float) * n, hipMemcpyHostToDevice);
         }
     }
+    R->nshell = shell_count(R->B);
+    for (int box = 1; ok && box < ng; ++box) {
+        std::vector<ShellPt> h(R->nshell);
+        build_shell(R->B, geom(R->d, box), h.data());
+        ok = hipMalloc(&R->shell[box - 1], sizeof(ShellPt) * h.size()) == hipSuccess &&
+             hipMemcpy(R->shell[box - 1], h.data(), sizeof(ShellPt) * h.size(),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    }
     if (!ok) {
         fprintf(stderr, "g3m3_run_create: device allocation failed\n");
         g3m3_run_destroy(R);
         return nullptr;
     }
-    hipMemset(R->count, 0, 64);
+    hipMemset(R->count, 0, 128);
     hipMemset(R->maxima, 0, sizeof(float) * 5 * ng);
     R->t_old.assign(ng, 0.0);
     R->t_new.assign(ng, 0.0);
     R->t_step.assign(ng, 0.0);
     R->t_stepnew.assign(ng, 0.0);
+    R->rot.assign(ng, 0);
     return R;
 }
 
@@ -451,8 +492,10 @@ void g3m3_run_destroy(void* run) {
     Run* R = (Run*)run;
     if (!R) return;
     hipStreamSynchronize(R->s);
-    for (void* p : {(void*)R->scr, (void*)R->fctP, (void*)R->list, (void*)R->count,
+    for (void* p : {(void*)R->scr, (void*)R->list, (void*)R->count,
                     (void*)R->partials, (void*)R->maxima, (void*)R->rst_eff})
+        if (p) hipFree(p);
+    for (ShellPt* p : R->shell)
         if (p) hipFree(p);
     delete R;
 }
@@ -465,7 +508,7 @@ int g3m3_run_init_dt(void* run) {
     for (int box = R->d.n_grids; box >= 1; --box) {
         R->t_old[box - 1] = 0.0;
         R->t_new[box - 1] = 0.0;
-        validate(R, R->d.state, box, true);
+        validate(R, R->state(box), box, true);
         boxes.push_back(box);
     }
     read_tstepnew(R, boxes);
@@ -528,10 +571,27 @@ int g3m3_run_full_step(void* run, const double* dt_override) {
     R->t_old[0] = R->t_new[0];
     for (int box = 1; box < n; ++box) {
         corer(R, box);
-        set_rho_only(R, R->d.state, box);
+        set_rho_only(R, R->state(box));
     }
     read_tstepnew(R, stepped);
     check(R, "full_step");
+    return R->err;
+}
+
+int g3m3_run_sync_arrays(void* run) {
+    Run* R = (Run*)run;
+    for (int box = 1; box <= R->d.n_grids; ++box) {
+        if (!R->rot[box - 1]) continue;
+        const Set a = R->slice(R->d.state, box), o = R->slice(R->d.old, box);
+        FieldPtrs pa{}, po{};
+        for (int f = 0; f < NF; ++f) {
+            pa.p[f] = a.f[f];
+            po.p[f] = o.f[f];
+        }
+        launch_swap_fields(R->n, NF, pa, po, R->s);
+        R->rot[box - 1] = 0;
+    }
+    check(R, "sync_arrays");
     return R->err;
 }
 

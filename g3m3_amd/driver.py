@@ -280,8 +280,9 @@ class Run:
         _lib.check("divb")
         return st, it.value, r2.value
 
-    def state_host(self):
-        return {k: self.a[k].cpu().numpy() for k in FIELDS}
+    def state_host(self, prefix=""):
+        """The 34 fields (or their old / wrk arrays: prefix "old", "wrk")."""
+        return {k: self.a[prefix + k].cpu().numpy() for k in FIELDS}
 
 
 # -----------------------------------------------------------------------------
@@ -321,7 +322,8 @@ class FastRun(Run):
         L = _lib.lib()
         L.g3m3_run_create.restype = _ct.c_void_p
         L.g3m3_run_create.argtypes = [_ct.POINTER(RunDesc)]
-        for nm in ("g3m3_run_init_dt", "g3m3_run_full_step", "g3m3_run_box_step"):
+        for nm in ("g3m3_run_init_dt", "g3m3_run_full_step", "g3m3_run_box_step",
+                   "g3m3_run_sync_arrays"):
             getattr(L, nm).restype = _ct.c_int
         d = RunDesc()
         d.nx, d.ny, d.nz, d.n_grids = self.nx, self.ny, self.nz, self.ngrd
@@ -387,3 +389,17 @@ class FastRun(Run):
         if self._L.g3m3_run_full_step(_ct.c_void_p(self.h), arg):
             raise _lib.G3M3Error("g3m3_run_full_step failed")
         self._pull()
+
+    def sync_arrays(self):
+        """Put every box's state back in the state tensors and its old state
+        in the old tensors (the runtime swaps the two per box-step)."""
+        if self._L.g3m3_run_sync_arrays(_ct.c_void_p(self.h)):
+            raise _lib.G3M3Error("g3m3_run_sync_arrays failed")
+
+    def divb(self, box, errlim, mode=0, max_iters=0):
+        self.sync_arrays()
+        return super().divb(box, errlim, mode, max_iters)
+
+    def state_host(self, prefix=""):
+        self.sync_arrays()
+        return super().state_host(prefix)

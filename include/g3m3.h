@@ -33,6 +33,8 @@
 #ifndef G3M3_H
 #define G3M3_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -338,6 +340,43 @@ int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int*
 void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qty,
                         float* scdata);
 
+/* code/sub_sparse_solver.f90:19-20 -- CGNR on an explicit banded matrix
+ * banded_matrix(nbands,npts) (device or registered), band offsets
+ * divB_band_pos(nbands) (host), soln_vec = initial guess in, solution out;
+ * residual2 is a host scalar, written before return.  Dot products
+ * accumulate in double (the reference: single). */
+void g3m3_sparse_solver(const int* nbands, const int* npts, const int* divB_band_pos,
+                        const float* divB_errlim, const int* recurse, const int* redo_lim,
+                        const float* divB, float* soln_vec, float* residual2,
+                        const float* banded_matrix);
+
+/* code/sub_fnd_pres.f90:5-6 -- parallel / perpendicular / cross pressure of
+ * one species (pxy, pxz, pyz are taken, as by the reference, but unused) */
+void g3m3_fnd_pres(const float* px, const float* py, const float* pz, const float* pxy,
+                   const float* pxz, const float* pyz, float* p_para, float* p_perp,
+                   float* p_cross, const float* vvx, const float* vvy, const float* vvz,
+                   const float* bxt, const float* byt, const float* bzt, const int* nx,
+                   const int* ny, const int* nz, const int* n_grids, const int* box);
+
+/* code/cft_findgrid.f90:4 -- host arrays; grid_minvals/maxvals are (3,n_grids);
+ * cname_len is the CHARACTER*8 hidden length a Fortran caller appends.
+ * Prints the reference's warning and clamps when the point is in no box. */
+void g3m3_findgrid(const float* xyz, const int* n_grids, const float* grid_minvals,
+                   const float* grid_maxvals, const float* xspac, int* gridpt,
+                   const float* re_equiv, const char* cname, size_t cname_len);
+
+/* Craft sampling at npts points in one launch: for each point sxyz(3,p),
+ * findgrid + the cube of code/cft_find_aux_cube.f90:62-65 + trilin_interp
+ * of each of nq fields q(nx,ny,nz,n_grids) (code/cft_craft_handling.f90:
+ * 165-195, code/cft_record_aux_data.f90:27-30).  scdata(nq,npts),
+ * gridpt(4,npts) are device (or registered) arrays; grid_minvals,
+ * grid_maxvals, xspac are host arrays.  A point whose cube would leave its
+ * box gets gridpt(1,p) = 0 and zero samples. */
+void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
+                        const float* grid_minvals, const float* grid_maxvals, const float* xspac,
+                        const float* re_equiv, const int* nx, const int* ny, const int* nz,
+                        const int* nq, const float* const* fields, float* scdata, int* gridpt);
+
 /* ---- native runtime ----------------------------------------------------------
  * No reference counterpart: the whole run resident in HBM, advanced a full
  * step (code/3d_main.f90:1925-3095) per call with fused kernels.  Results
@@ -348,7 +387,14 @@ void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qt
  * epres, bx, by, bz; each a device array q(nx,ny,nz,n_grids).  wrk and
  * old must start zeroed (the reference's static arrays).  Index lists and
  * parm arrays are device arrays in the reference's layout; extents and
- * scalars are copied at creation. */
+ * scalars are copied at creation.
+ *
+ * A box-step swaps the roles of the box's state and old slices instead of
+ * copying the state into old, so between calls a box's state may live in
+ * the old array.  g3m3_run_sync_arrays() puts every box's state back into
+ * the state array and its old state into the old array (the reference's
+ * arrays, bit for bit); call it before reading or changing the arrays
+ * outside the runtime. */
 #define G3M3_MAX_GRIDS 8
 typedef struct {
     int nx, ny, nz, n_grids;
@@ -385,8 +431,22 @@ void g3m3_run_set_stream(void* run, void* hip_stream);
 int g3m3_run_init_dt(void* run);                     /* code/3d_main.f90:1861-1903 */
 int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt */
 int g3m3_run_box_step(void* run, int box);
+int g3m3_run_sync_arrays(void* run);
 void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
                    double* t_new, long long* cell_updates, long long* box_steps);
+
+/* ---- Fortran implicit-interface names ------------------------------------------
+ * A Fortran call with no interface block (the reference's own style) names
+ * g3m3_<name>_ and passes everything by reference.  Every subroutine entry
+ * point above is also exported under that name (an alias); the control and
+ * registry calls have by-reference forms: */
+int g3m3_init_(void);
+int g3m3_last_error_(void);
+void g3m3_sync_(void);
+int g3m3_register_(void* host_ptr, const long long* bytes);
+int g3m3_unregister_(void* host_ptr);
+int g3m3_to_device_(void* host_ptr);
+int g3m3_to_host_(void* host_ptr);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ CPU baseline.
 ``oracle/Makefile`` compiles the reference's ``code/sub_*.f90`` files,
 unchanged, with flang into ``oracle/_ref/libg3m3_ref.so`` (serial) and
 ``libg3m3_ref_omp.so`` (OpenMP).  The subroutines have implicit interfaces
-(SURVEY: every argument by reference), so each call here passes:
+(every argument passed by reference), so each call here passes:
 
 * numpy arrays (C-contiguous, float32 or int32) as their first-element
   address -- a (ngrd, nz, ny, nx) C-order array is byte-identical to the
@@ -38,6 +38,16 @@ class F64:
 
     def __init__(self, v):
         self.v = float(v)
+
+
+class Chars:
+    """A ``CHARACTER*n`` actual argument: its bytes by reference, and its
+    length appended after the last argument (the hidden-length convention
+    of flang and gfortran)."""
+
+    def __init__(self, text: str, n: int):
+        self.b = ctypes.create_string_buffer(text.ljust(n)[:n].encode(), n)
+        self.n = n
 
 
 class _Rotation(ctypes.Structure):
@@ -90,7 +100,14 @@ class RefLib:
     def call(self, name: str, *args):
         fn = getattr(self.lib, name.lower() + "_")
         fn.restype = None
-        cargs = [self._arg(a) for a in args]
+        cargs, hidden = [], []
+        for a in args:
+            if isinstance(a, Chars):
+                cargs.append(ctypes.cast(a.b, ctypes.c_void_p))
+                hidden.append(ctypes.c_size_t(a.n))
+            else:
+                cargs.append(self._arg(a))
+        cargs += hidden
         try:
             fn(*cargs)
         finally:

@@ -1087,3 +1087,93 @@ def trilin_interp(sxyz, gridpts, qty, scdata):
     f0 = e00 * (one - yd) + e10 * yd
     f1 = e01 * (one - yd) + e11 * yd
     scdata[0] = f0 * (one - zd) + f1 * zd
+
+
+# ---------------------------------------------------------------------------
+# code/sub_fnd_pres.f90:5-67 (pxy, pxz, pyz are read by the reference but do
+# not enter the result)
+
+
+def fnd_pres(px, py, pz, pxy, pxz, pyz, p_para, p_perp, p_cross, vvx, vvy, vvz, bxt, byt, bzt,
+             nx, ny, nz, n_grids, box):
+    b = box - 1
+    x, y, z = px[b], py[b], pz[b]
+    presmag = np.sqrt(x * x + y * y + z * z) + F(1.e-11)
+    bmag = np.sqrt(bxt * bxt + byt * byt + bzt * bzt) + F(1.e-12)
+    vcbx = (vvy * bzt - vvz * byt) / bmag
+    vcby = -(vvx * bzt - vvz * bxt) / bmag
+    vcbz = (vvx * byt - vvy * bxt) / bmag
+    vcmag = np.sqrt(vcbx * vcbx + vcby * vcby + vcbz * vcbz) + F(1.e-12)
+    ax, ay, az = bxt * x, byt * y, bzt * z
+    p_para[b] = np.sqrt(ax * ax + ay * ay + az * az) / bmag
+    cx, cy, cz = vcbx * x, vcby * y, vcbz * z
+    p_cross[b] = np.sqrt(cx * cx + cy * cy + cz * cz) / vcmag
+    pa, pc = p_para[b], p_cross[b]
+    p_perp[b] = np.sqrt(np.abs(presmag * presmag - (pa * pa + pc * pc)))
+
+
+# ---------------------------------------------------------------------------
+# code/cft_findgrid.f90:4-60.  grid_minvals(3, n_grids) -> numpy
+# (n_grids, 3); gridpt(4) -> int32 array (1-based indices and box).
+
+
+def findgrid(xyz, n_grids, grid_minvals, grid_maxvals, xspac, gridpt, re_equiv, cname=None):
+    abit = F(0.001)
+    xa = (np.asarray(xyz, F) / F(re_equiv)).astype(F)
+    gridpt[3] = 0
+    for box in range(1, n_grids + 1):
+        mn, mx = grid_minvals[box - 1], grid_maxvals[box - 1]
+        if bool(np.all(xa > mn)) and bool(np.all(xa < mx)):
+            gridpt[3] = box
+            break
+    if gridpt[3] == 0:
+        gridpt[3] = n_grids
+        xa = np.maximum(xa, grid_minvals[n_grids - 1] + abit)
+        xa = np.minimum(xa, grid_maxvals[n_grids - 1] - abit)
+    bb = int(gridpt[3]) - 1
+    for axis in range(3):
+        delta = F(xa[axis] - grid_minvals[bb, axis])
+        gridpt[axis] = 1 + int(F(delta / xspac[bb]))
+
+
+def aux_cube(gridpt, grid_minvals, xspac, re_equiv):
+    """The cube_vertices(3,2) of code/cft_find_aux_cube.f90:62-65, as
+    numpy (2, 3)."""
+    bb = int(gridpt[3]) - 1
+    cv = np.zeros((2, 3), F)
+    for axis in range(3):
+        cv[0, axis] = (grid_minvals[bb, axis] + (F(gridpt[axis]) - F(1.)) * xspac[bb]) \
+            * F(re_equiv)
+        cv[1, axis] = cv[0, axis] + xspac[bb] * F(re_equiv)
+    return cv
+
+
+def sample_fields(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv, fields):
+    """Craft sampling (code/cft_craft_handling.f90:165-195 with
+    code/cft_record_aux_data.f90:27-30): for each point, findgrid, the aux
+    cube, the 2x2x2 cube of each field and trilin_interp.  ``sxyz`` is
+    (npts, 3); fields are (n_grids, nz, ny, nx); returns (scdata (npts,
+    nq), gridpt (npts, 4)).  A point whose cube would leave its box is
+    marked gridpt[0] = 0 with zero samples (the reference's 'gridding
+    problem' indicator)."""
+    npts = sxyz.shape[0]
+    nq = len(fields)
+    out = np.zeros((npts, nq), F)
+    gps = np.zeros((npts, 4), np.int32)
+    for p in range(npts):
+        gp = np.zeros(4, np.int32)
+        findgrid(sxyz[p], n_grids, grid_minvals, grid_maxvals, xspac, gp, re_equiv)
+        nz, ny, nx = fields[0].shape[1:]
+        if not (1 <= gp[0] <= nx - 1 and 1 <= gp[1] <= ny - 1 and 1 <= gp[2] <= nz - 1):
+            gp[0] = 0
+            gps[p] = gp
+            continue
+        cv = aux_cube(gp, grid_minvals, xspac, re_equiv)
+        i, j, k, bb = gp[0] - 1, gp[1] - 1, gp[2] - 1, gp[3] - 1
+        for q in range(nq):
+            cube = np.ascontiguousarray(fields[q][bb, k:k + 2, j:j + 2, i:i + 2])
+            r = np.zeros(1, F)
+            trilin_interp(np.asarray(sxyz[p], F), cv, cube, r)
+            out[p, q] = r[0]
+        gps[p] = gp
+    return out, gps

@@ -38,3 +38,19 @@ def test_header_matches_reference_signatures():
     assert nargs("g3m3_bndry_inner") == len(A.bndry_inner_names("")) + 2   # + v_rot, r_rot
     assert nargs("g3m3_flux_correct") == len(A.flux_correct_names())
     assert nargs("g3m3_lap_plasma") == len(A.lap_plasma_names("q", "", "wrk"))
+
+
+def test_fortran_names_exported():
+    """Each subroutine entry point is also exported under the name an
+    implicit-interface Fortran call uses (lower case + '_'), as an alias of
+    the same code."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libg3m3.so not built")
+    so = ctypes.CDLL(str(_lib.LIB_PATH), mode=os.RTLD_LOCAL)
+    for n in ("g3m3_push_ion", "g3m3_calcur", "g3m3_bande", "g3m3_flux_correct",
+              "g3m3_bndry_flanks_v", "g3m3_divb_correct", "g3m3_trilin_interp"):
+        a = ctypes.cast(getattr(so, n), ctypes.c_void_p).value
+        b = ctypes.cast(getattr(so, n + "_"), ctypes.c_void_p).value
+        assert a == b, n
+    for n in ("g3m3_init_", "g3m3_register_", "g3m3_to_device_", "g3m3_to_host_"):
+        assert hasattr(so, n), n

@@ -313,3 +313,104 @@ def test_divb_solves_poisson(dev):
         got = d[k].cpu().numpy()
         ref = wo.a[k]
         assert np.abs(got - ref).max() <= 0.05 * np.abs(w.a[k]).max()
+
+
+def test_fnd_pres(dev, world):
+    w = world.copy()
+    shp = w.a["qrho"].shape
+    for k in ("p_para", "p_perp", "p_cross"):
+        w.a[k] = np.zeros(shp, np.float32)
+    names = ["qpresx", "qpresy", "qpresz", "qpresxy", "qpresxz", "qpresyz", "p_para", "p_perp",
+             "p_cross", "vvx", "vvy", "vvz", "bsx", "bsy", "bsz", "nx", "ny", "nz", "n_grids",
+             "box"]
+    wg, _ = gpu_call(dev, w, "fnd_pres", names)
+    wo = oracle_call(w, "fnd_pres", names)
+    assert_bits(wg, wo, ["p_para", "p_perp", "p_cross"])
+
+
+def test_findgrid_host(dev):
+    """The host findgrid entry point against the oracle (inside, on a face,
+    outside every box)."""
+    from g3m3_amd import _lib
+    import ctypes
+    ngrd, re = 3, np.float32(0.5)
+    xs = np.array([1.0, 2.0, 4.0], np.float32)
+    mn = np.ascontiguousarray(np.stack([-6 * xs, -6 * xs, -3 * xs], 1).astype(np.float32))
+    mx = np.ascontiguousarray(np.stack([6 * xs, 6 * xs, 3 * xs], 1).astype(np.float32))
+    rng = np.random.default_rng(12)
+    for p in list(rng.uniform(-30, 30, (30, 3))) + [np.array([3.0, 0, 0])]:
+        p = np.asarray(p, np.float32)
+        a, b = np.zeros(4, np.int32), np.zeros(4, np.int32)
+        name = ctypes.create_string_buffer(b"craft01 ", 8)
+        _lib.call("findgrid", p, ngrd, mn, mx, xs, a, re, name, _lib.ByValue(ctypes.c_size_t(8)))
+        O.findgrid(p, ngrd, mn, mx, xs, b, re)
+        assert np.array_equal(a, b), (p, a, b)
+
+
+def test_sample_fields(dev):
+    """Craft sampling of several fields at many points in one launch against
+    the oracle's findgrid + aux cube + trilin_interp per point."""
+    from g3m3_amd import _lib
+    nx, ny, nz, ngrd = 13, 13, 7, 3
+    re = np.float32(0.5)
+    xs = np.array([1.0, 2.0, 4.0], np.float32)
+    mn = np.ascontiguousarray(np.stack([-6 * xs, -6 * xs, -3 * xs], 1).astype(np.float32))
+    mx = np.ascontiguousarray(np.stack([6 * xs, 6 * xs, 3 * xs], 1).astype(np.float32))
+    rng = np.random.default_rng(21)
+    fields = [rng.uniform(-1, 1, (ngrd, nz, ny, nx)).astype(np.float32) for _ in range(4)]
+    pts = rng.uniform(-14, 14, (500, 3)).astype(np.float32)
+    pts[:, 2] *= 0.5
+    pts[-3:] = np.array([[3.0, 0.0, 0.0], [40.0, 0.0, 0.0], [0.0, 0.0, -30.0]], np.float32)
+    want, wgp = O.sample_fields(pts, ngrd, mn, mx, xs, re, fields)
+    assert (wgp[:, 0] > 0).sum() > 400   # most points are sampled
+    dfs = [dev.from_numpy(f).cuda() for f in fields]
+    ptr = (np.array([f.data_ptr() for f in dfs], np.uint64))
+    import ctypes
+    parr = (ctypes.c_void_p * len(dfs))(*[int(x) for x in ptr])
+    dp = dev.from_numpy(np.ascontiguousarray(pts)).cuda()
+    out = dev.zeros((len(pts), len(fields)), dtype=dev.float32, device="cuda")
+    gp = dev.zeros((len(pts), 4), dtype=dev.int32, device="cuda")
+    _lib.call("sample_fields", len(pts), dp, ngrd, mn, mx, xs, re, nx, ny, nz, len(fields), parr,
+              out, gp)
+    _lib.sync()
+    _lib.check("sample_fields")
+    assert np.array_equal(gp.cpu().numpy(), wgp)
+    assert np.array_equal(out.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("errlim", [1e-4, 1e-6])
+def test_sparse_solver(dev, errlim):
+    """The explicit-matrix CGNR entry point: same system and start as the
+    oracle; the solutions agree to the solve's own accuracy (dots in double
+    on the GPU vs sequential single in the reference), and the GPU meets
+    the reference's stopping criterion |r|^2 <= |b|^2 npts errlim^2."""
+    from g3m3_amd import _lib
+    nx, ny, nz = 9, 8, 7
+    npts = nx * ny * nz
+    M = O.divb_matrix(nx, ny, nz)
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    rhs = np.random.default_rng(5).uniform(-1, 1, npts).astype(np.float32)
+    so = np.zeros(npts, np.float32)
+    ro = np.zeros(1, np.float32)
+    O.sparse_solver(7, npts, pos, np.float32(errlim), False, 0, rhs, so, ro, M)
+    dM = dev.from_numpy(np.ascontiguousarray(M)).cuda()
+    drhs = dev.from_numpy(rhs).cuda()
+    ds = dev.zeros(npts, dtype=dev.float32, device="cuda")
+    rg = np.zeros(1, np.float32)
+    _lib.call("sparse_solver", 7, npts, pos, np.float32(errlim), False, 0, drhs, ds, rg, dM)
+    _lib.sync()
+    _lib.check("sparse_solver")
+    sg = ds.cpu().numpy()
+    # the GPU's own residual, recomputed here in double
+    Ad = np.zeros(npts)
+    for b in range(7):
+        ii = np.arange(npts) + pos[b]
+        ok = (ii >= 0) & (ii < npts)
+        Ad[ok] += M[ii[ok], b].astype(np.float64) * sg[ii[ok]]
+    r2 = float(((Ad - rhs) ** 2).sum())
+    b2 = float((rhs.astype(np.float64) ** 2).sum())
+    assert r2 <= 1.5 * b2 * npts * errlim ** 2
+    assert abs(rg[0] - r2) <= 1e-3 * max(r2, 1e-30) + 1e-12
+    scale = float(np.abs(so).max())
+    tol = max(1e-3, 20 * np.sqrt(npts) * errlim)
+    assert float(np.abs(sg - so).max()) <= tol * scale

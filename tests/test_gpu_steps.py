@@ -51,6 +51,14 @@ def run_pair(name, steps, checkpoints, native=False, **over):
                   f"{max(errs, key=errs.get)} rel-L2 {max(errs.values()):.3e}", flush=True)
             bad = {k: v for k, v in errs.items() if not v <= TOL}
             assert not bad, f"step {st}: fields over {TOL}: {bad}"
+            if native:
+                # the runtime rotates state/old; after sync both arrays and
+                # wrk must hold what the reference's arrays hold
+                for pre in ("old", "wrk"):
+                    gp = g.state_host(pre)
+                    e2 = {k: rel_l2(gp[k], o.a[pre + k]) for k in FIELDS}
+                    bad = {k: v for k, v in e2.items() if not v <= TOL}
+                    assert not bad, f"step {st}: {pre} arrays over {TOL}: {bad}"
     assert g.box_steps == o.box_steps
     return report
 

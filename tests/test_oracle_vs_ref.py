@@ -249,3 +249,41 @@ def test_trilin_interp():
         lib().call("trilin_interp", s, g, q, a)
         O.trilin_interp(s, g, q, b)
         assert a[0] == b[0]
+
+
+def test_fnd_pres(world):
+    w = world.copy()
+    shp = w.a["qrho"].shape
+    for k in ("p_para", "p_perp", "p_cross"):
+        w.a[k] = np.zeros(shp, np.float32)
+    names = ["qpresx", "qpresy", "qpresz", "qpresxy", "qpresxz", "qpresyz", "p_para", "p_perp",
+             "p_cross", "vvx", "vvy", "vvz", "bsx", "bsy", "bsz", "nx", "ny", "nz", "n_grids",
+             "box"]
+    wr, wo = run_both(w, "fnd_pres", names)
+    assert_changed(w, wr, ["p_para", "p_perp", "p_cross"])
+    assert_same(wr, wo)
+
+
+def _grid_extents(ngrd, L=6):
+    xs = np.array([2.0 ** b for b in range(ngrd)], np.float32)
+    mn = np.stack([-L * xs, -L * xs, -L / 2 * xs], axis=1).astype(np.float32)
+    mx = np.stack([L * xs, L * xs, L / 2 * xs], axis=1).astype(np.float32)
+    return xs, np.ascontiguousarray(mn), np.ascontiguousarray(mx)
+
+
+def test_findgrid():
+    """Inside the smallest box, inside only outer boxes, on a box face
+    (strict inequalities), and outside every box (clamped to the outermost,
+    with the reference's message)."""
+    ngrd, re = 3, np.float32(0.5)
+    xs, mn, mx = _grid_extents(ngrd)
+    rng = np.random.default_rng(5)
+    pts = [rng.uniform(-30, 30, 3) for _ in range(40)]
+    pts += [np.array([6 * 0.5, 0.0, 0.0]), np.array([100.0, -100.0, 3.0]),
+            np.array([0.0, 0.0, -200.0])]
+    for p in pts:
+        p = np.asarray(p, np.float32)
+        a, b = np.zeros(4, np.int32), np.zeros(4, np.int32)
+        lib().call("findgrid", p, ngrd, mn, mx, xs, a, re, R.Chars("craft01", 8))
+        O.findgrid(p, ngrd, mn, mx, xs, b, re)
+        assert np.array_equal(a, b), (p, a, b)
