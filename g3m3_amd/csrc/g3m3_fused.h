@@ -41,9 +41,13 @@ struct PushParams {
     const float* oldepres;
     float* epres;
     float rmass[3];
+    float rrmass[3];            // 1/rmass (used when pow2)
+    float bmax[3];              // 0.20*rmass/delt/reynolds per species
     float delt, reynolds, smallbit;
     float dxt, dyt, dzt;        // ions: 2 * grid spacing of the box extents
     float fdxt, fdyt, fdzt;     // fields and electrons: 2 * xspac
+    float rdxt, rdyt, rdzt, rfdxt, rfdyt, rfdzt;   // reciprocals (used when pow2)
+    int pow2;                   // every step and mass above is a power of two
     float xmin, ymin, zmin, ddx, ddy, ddz, re_equiv, grav, gamma, gamma1;
     int do_elec;
     int copy_shell;             // copy non-interior cells old -> out

@@ -180,16 +180,29 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
 // when P.do_elec).  The species velocity wrk p / max(wrk rho, 1e-6) is
 // evaluated where it is needed instead of stored (same bits).
 
+// Division by a grid step or a species mass.  When every such divisor is a
+// power of two (P2), x / d is computed as x * (1/d): the reciprocal is
+// exact and both operations are correctly rounded, so the result is the
+// same float; otherwise a true division.
+template <bool P2>
+__device__ inline float dvp(float x, float d, float r) {
+    return P2 ? x * r : x / d;
+}
+
+template <bool P2>
 __device__ inline void push_species(const PushParams& P, int s, int i, int j, int k) {
     const Box& B = P.B;
     const int c = B.IX(i, j, k), X = 1, Y = B.nx, Z = B.nx * B.ny;
     auto put = [&](int m, float v) { P.out[s].f[m][c] = v; };
     const SpeciesSet& W = P.src[s];
     const SpeciesSet& O = P.old[s];
-    const float rmass = P.rmass[s];
+    const float rmass = P.rmass[s], rrmass = P.rrmass[s];
     const float delt = P.delt, reynolds = P.reynolds;
-    const float dxt = P.dxt, dyt = P.dyt, dzt = P.dzt;
-    const float bmax = 0.20f * rmass / delt / reynolds;
+    auto Dx = [&](float x) { return dvp<P2>(x, P.dxt, P.rdxt); };
+    auto Dy = [&](float x) { return dvp<P2>(x, P.dyt, P.rdyt); };
+    auto Dz = [&](float x) { return dvp<P2>(x, P.dzt, P.rdzt); };
+    auto Dm = [&](float x) { return dvp<P2>(x, rmass, rrmass); };
+    const float bmax = P.bmax[s];    // 0.20*rmass/delt/reynolds (code/sub_push_ion.f90:64)
     const float* __restrict__ wr = W.f[0];
     const float* __restrict__ wx = W.f[1];
     const float* __restrict__ wy = W.f[2];
@@ -213,13 +226,11 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     }
     // index into vx etc: 0 c, 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z
     auto flux = [&](const float* __restrict__ w) {
-        return ((w[c + X] * vx[1] - w[c - X] * vx[2]) / dxt) +
-               ((w[c + Y] * vy[3] - w[c - Y] * vy[4]) / dyt) +
-               ((w[c + Z] * vz[5] - w[c - Z] * vz[6]) / dzt);
+        return Dx(w[c + X] * vx[1] - w[c - X] * vx[2]) + Dy(w[c + Y] * vy[3] - w[c - Y] * vy[4]) +
+               Dz(w[c + Z] * vz[5] - w[c - Z] * vz[6]);
     };
-    put(0, O.f[0][c] - delt * (((wpx[c + X] - wpx[c - X]) / dxt) +
-                                    ((wpy[c + Y] - wpy[c - Y]) / dyt) +
-                                    ((wpz[c + Z] - wpz[c - Z]) / dzt)));
+    put(0, O.f[0][c] - delt * (Dx(wpx[c + X] - wpx[c - X]) + Dy(wpy[c + Y] - wpy[c - Y]) +
+                               Dz(wpz[c + Z] - wpz[c - Z])));
     const float aqpx = O.f[4][c] - delt * flux(wpx);
     const float aqpy = O.f[5][c] - delt * flux(wpy);
     const float aqpz = O.f[6][c] - delt * flux(wpz);
@@ -237,12 +248,14 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     const float qcx = (vy[0] * abz - vz[0] * aby);
     const float qcy = -(vx[0] * abz - vz[0] * abx);
     const float qcz = (vx[0] * aby - vy[0] * abx);
-    const float gpx = (wx[c + X] - wx[c - X]) / dxt + (wxy[c + Y] - wxy[c - Y]) / dxt +
-                      (wxz[c + Z] - wxz[c - Z]) / dxt;
-    const float gpy = (wxy[c + X] - wxy[c - X]) / dxt + (wy[c + Y] - wy[c - Y]) / dyt +
-                      (wyz[c + Z] - wyz[c - Z]) / dxt;
-    const float gpz = (wxz[c + X] - wxz[c - X]) / dxt + (wyz[c + Y] - wyz[c - Y]) / dyt +
-                      (wz[c + Z] - wz[c - Z]) / dzt;
+    // the reference divides the y and z cross terms by dxt as well
+    // (code/sub_push_ion.f90:155-164)
+    const float gpx = Dx(wx[c + X] - wx[c - X]) + Dx(wxy[c + Y] - wxy[c - Y]) +
+                      Dx(wxz[c + Z] - wxz[c - Z]);
+    const float gpy = Dx(wxy[c + X] - wxy[c - X]) + Dy(wy[c + Y] - wy[c - Y]) +
+                      Dx(wyz[c + Z] - wyz[c - Z]);
+    const float gpz = Dx(wxz[c + X] - wxz[c - X]) + Dy(wyz[c + Y] - wyz[c - Y]) +
+                      Dz(wz[c + Z] - wz[c - Z]);
     const float dex = (P.e[0][c] + tcx) * reynolds;
     const float dey = (P.e[1][c] + tcy) * reynolds;
     const float dez = (P.e[2][c] + tcz) * reynolds;
@@ -250,7 +263,7 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     const float dvy = (qcy - tcy) * skin_depth;
     const float dvz = (qcz - tcz) * skin_depth;
     const float wrc = wr[c];
-    const float qden = wrc / rmass;
+    const float qden = Dm(wrc);
     const float px = aqpx + delt * (qden * (dex + dvx) - gpx);
     const float py = aqpy + delt * (qden * (dey + dvy) - gpy);
     const float pz = aqpz + delt * (qden * (dez + dvz) - gpz);
@@ -266,9 +279,9 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
 
     const float cx = wx[c], cy = wy[c], cz = wz[c];
     const float cxy = wxy[c], cxz = wxz[c], cyz = wyz[c];
-    const float px0 = O.f[1][c] - delt * flux(wx) - 2.0f * delt * cx * (vx[1] - vx[2]) / dxt;
-    const float py0 = O.f[2][c] - delt * flux(wy) - 2.0f * delt * cy * (vy[3] - vy[4]) / dyt;
-    const float pz0 = O.f[3][c] - delt * flux(wz) - 2.0f * delt * cz * (vz[5] - vz[6]) / dzt;
+    const float px0 = O.f[1][c] - delt * flux(wx) - Dx(2.0f * delt * cx * (vx[1] - vx[2]));
+    const float py0 = O.f[2][c] - delt * flux(wy) - Dy(2.0f * delt * cy * (vy[3] - vy[4]));
+    const float pz0 = O.f[3][c] - delt * flux(wz) - Dz(2.0f * delt * cz * (vz[5] - vz[6]));
     abx = (6.0f * bsx[c] + bsx[c + X] + bsx[c - X] + bsx[c + Y] + bsx[c - Y] + bsx[c + Z] +
            bsx[c - Z]) / 12.0f;
     aby = (6.0f * bsy[c] + bsy[c + X] + bsy[c - X] + bsy[c + Y] + bsy[c - Y] + bsy[c + Z] +
@@ -279,55 +292,55 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     float sf = reynolds;
     if (bmag > bmax) sf = reynolds * (bmax / bmag);
     const float t2 = 2.0f * delt;
-    put(1, px0 - t2 * cxy * (vx[3] - vx[4]) / dyt - t2 * cxz * (vx[5] - vx[6]) / dzt +
-                t2 * sf / rmass * (cxy * abz - cxz * aby));
-    put(2, py0 - t2 * cxy * (vy[1] - vy[2]) / dxt - t2 * cyz * (vy[5] - vy[6]) / dzt +
-                t2 * sf / rmass * (cyz * abx - cxy * abz));
-    put(3, pz0 - t2 * cxz * (vz[1] - vz[2]) / dxt - t2 * cyz * (vz[3] - vz[4]) / dyt +
-                t2 * sf / rmass * (cxz * aby - cyz * abx));
+    put(1, px0 - Dy(t2 * cxy * (vx[3] - vx[4])) - Dz(t2 * cxz * (vx[5] - vx[6])) +
+               Dm(t2 * sf) * (cxy * abz - cxz * aby));
+    put(2, py0 - Dx(t2 * cxy * (vy[1] - vy[2])) - Dz(t2 * cyz * (vy[5] - vy[6])) +
+               Dm(t2 * sf) * (cyz * abx - cxy * abz));
+    put(3, pz0 - Dx(t2 * cxz * (vz[1] - vz[2])) - Dy(t2 * cyz * (vz[3] - vz[4])) +
+               Dm(t2 * sf) * (cxz * aby - cyz * abx));
     const float pxy = O.f[7][c] - delt * flux(wxy);
-    put(7, pxy - delt * cxy * ((vx[1] - vx[2]) / dxt + (vy[3] - vy[4]) / dyt) -
-                delt * cy * (vx[3] - vx[4]) / dyt - delt * cx * (vy[1] - vy[2]) / dxt -
-                delt * cyz * (vx[5] - vx[6]) / dzt - delt * cxz * (vy[5] - vy[6]) / dzt +
-                delt * sf / rmass * (abz * (cy - cx) - aby * cyz + abx * cxz));
+    put(7, pxy - delt * cxy * (Dx(vx[1] - vx[2]) + Dy(vy[3] - vy[4])) -
+               Dy(delt * cy * (vx[3] - vx[4])) - Dx(delt * cx * (vy[1] - vy[2])) -
+               Dz(delt * cyz * (vx[5] - vx[6])) - Dz(delt * cxz * (vy[5] - vy[6])) +
+               Dm(delt * sf) * (abz * (cy - cx) - aby * cyz + abx * cxz));
     const float pxz = O.f[8][c] - delt * flux(wxz);
-    put(8, pxz - delt * cxz * ((vx[1] - vx[2]) / dxt + (vz[5] - vz[6]) / dzt) -
-                delt * cz * (vx[5] - vx[6]) / dzt - delt * cx * (vz[1] - vz[2]) / dxt -
-                delt * cyz * (vx[3] - vx[4]) / dyt - delt * cxy * (vz[3] - vz[4]) / dyt +
-                delt * sf / rmass * (aby * (cx - cz) - abx * cxy + abz * cyz));
+    put(8, pxz - delt * cxz * (Dx(vx[1] - vx[2]) + Dz(vz[5] - vz[6])) -
+               Dz(delt * cz * (vx[5] - vx[6])) - Dx(delt * cx * (vz[1] - vz[2])) -
+               Dy(delt * cyz * (vx[3] - vx[4])) - Dy(delt * cxy * (vz[3] - vz[4])) +
+               Dm(delt * sf) * (aby * (cx - cz) - abx * cxy + abz * cyz));
     const float pyz = O.f[9][c] - delt * flux(wyz);
-    put(9, pyz - delt * cyz * ((vy[3] - vy[4]) / dyt + (vz[5] - vz[6]) / dzt) -
-                delt * cy * (vz[3] - vz[4]) / dyt - delt * cz * (vy[5] - vy[6]) / dzt -
-                delt * cxy * (vz[1] - vz[2]) / dxt - delt * cxz * (vy[1] - vy[2]) / dxt +
-                delt * sf / rmass * (abx * (cz - cy) - abz * cxz + aby * cxy));
+    put(9, pyz - delt * cyz * (Dy(vy[3] - vy[4]) + Dz(vz[5] - vz[6])) -
+               Dy(delt * cy * (vz[3] - vz[4])) - Dz(delt * cz * (vy[5] - vy[6])) -
+               Dx(delt * cxy * (vz[1] - vz[2])) - Dx(delt * cxz * (vy[1] - vy[2])) +
+               Dm(delt * sf) * (abx * (cz - cy) - abz * cxz + aby * cxy));
 }
 
+template <bool P2>
 __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k) {
     const Box& B = P.B;
     const int c = B.IX(i, j, k), sy = B.nx, sz = B.nx * B.ny;
-    const float delt = P.delt, dxt = P.fdxt, dyt = P.fdyt, dzt = P.fdzt;
+    const float delt = P.delt;
+    auto Dx = [&](float x) { return dvp<P2>(x, P.fdxt, P.rfdxt); };
+    auto Dy = [&](float x) { return dvp<P2>(x, P.fdyt, P.rfdyt); };
+    auto Dz = [&](float x) { return dvp<P2>(x, P.fdzt, P.rfdzt); };
     const float* __restrict__ ex = P.e[0];
     const float* __restrict__ ey = P.e[1];
     const float* __restrict__ ez = P.e[2];
-    auto put_b = [&](int m, float v) { P.bout[m][c] = v; };
     // code/sub_push_bfld.f90:87-97
-    put_b(0, P.bold[0][c] - delt * (((ez[c + sy] - ez[c - sy]) / dyt) -
-                                    ((ey[c + sz] - ey[c - sz]) / dzt)));
-    put_b(1, P.bold[1][c] + delt * (((ez[c + 1] - ez[c - 1]) / dxt) -
-                                    ((ex[c + sz] - ex[c - sz]) / dzt)));
-    put_b(2, P.bold[2][c] - delt * (((ey[c + 1] - ey[c - 1]) / dxt) -
-                                    ((ex[c + sy] - ex[c - sy]) / dyt)));
+    P.bout[0][c] = P.bold[0][c] - delt * (Dy(ez[c + sy] - ez[c - sy]) - Dz(ey[c + sz] - ey[c - sz]));
+    P.bout[1][c] = P.bold[1][c] + delt * (Dx(ez[c + 1] - ez[c - 1]) - Dz(ex[c + sz] - ex[c - sz]));
+    P.bout[2][c] = P.bold[2][c] - delt * (Dx(ey[c + 1] - ey[c - 1]) - Dy(ex[c + sy] - ex[c - sy]));
     if (P.do_elec) {   // code/sub_push_elec.f90:31-47
         const float* __restrict__ w = P.wrkepres;
         const float* __restrict__ evx = P.ev[0];
         const float* __restrict__ evy = P.ev[1];
         const float* __restrict__ evz = P.ev[2];
-        const float gx = (w[c + 1] - w[c - 1]) / dxt;
-        const float gy = (w[c + sy] - w[c - sy]) / dyt;
-        const float gz = (w[c + sz] - w[c - sz]) / dzt;
-        const float div = ((w[c + 1] * evx[c + 1] - w[c - 1] * evx[c - 1]) / dxt) +
-                          ((w[c + sy] * evy[c + sy] - w[c - sy] * evy[c - sy]) / dyt) +
-                          ((w[c + sz] * evz[c + sz] - w[c - sz] * evz[c - sz]) / dzt);
+        const float gx = Dx(w[c + 1] - w[c - 1]);
+        const float gy = Dy(w[c + sy] - w[c - sy]);
+        const float gz = Dz(w[c + sz] - w[c - sz]);
+        const float div = Dx(w[c + 1] * evx[c + 1] - w[c - 1] * evx[c - 1]) +
+                          Dy(w[c + sy] * evy[c + sy] - w[c - sy] * evy[c - sy]) +
+                          Dz(w[c + sz] * evz[c + sz] - w[c - sz] * evz[c - sz]);
         const float e = P.oldepres[c] - delt * P.gamma * div;
         P.epres[c] = e + delt * P.gamma1 * (evx[c] * gx + evy[c] * gy + evz[c] * gz);
     }
@@ -347,6 +360,7 @@ __device__ inline void push_shell_copy(const PushParams& P, int s, int c) {
     }
 }
 
+template <bool P2>
 __global__ void __launch_bounds__(256) k_push(PushParams P) {
     int i, j, k, s;
     if (!cell3(P.B, i, j, k, s)) return;
@@ -354,8 +368,8 @@ __global__ void __launch_bounds__(256) k_push(PushParams P) {
         if (P.copy_shell) push_shell_copy(P, s, P.B.IX(i, j, k));
         return;
     }
-    if (s < 3) push_species(P, s, i, j, k);
-    else push_field_elec(P, i, j, k);
+    if (s < 3) push_species<P2>(P, s, i, j, k);
+    else push_field_elec<P2>(P, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,6 +647,13 @@ struct FctLds {
     float p[5][FPY][FPX];
 };
 
+// the limiter of code/sub_fcsmooth.f90 on one face: dq = wrk difference,
+// d1m / d1p = diffused differences below / above
+__device__ inline float fct_lim(float dq, float d1m, float d1p, float chi) {
+    const float s = copysignf(1.0f, dq);
+    return s * fmaxf(0.0f, fminf(fminf(s * d1m, chi * fabsf(dq)), s * d1p));
+}
+
 // limited antidiffusive flux of one axis at the face above the cell `base`
 // cells along that axis from the output cell (m = its index along the
 // axis, n the extent); W(t) / P(t) read wrk / P t cells along the axis
@@ -649,8 +670,7 @@ __device__ inline float fct_flux_ax(const WA& W, const PA& P, int base, int m, i
     const float dq = W(c + 1) - W(c);
     const float d1m = P(c) - P(c - 1);
     const float d1p = (m + 1 == n - 1) ? (P(c + 1) - P(c)) : (P(c + 2) - P(c + 1));
-    const float s = copysignf(1.0f, dq);
-    return s * fmaxf(0.0f, fminf(fminf(s * d1m, chi * fabsf(dq)), s * d1p));
+    return fct_lim(dq, d1m, d1p, chi);
 }
 
 __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
@@ -674,6 +694,12 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     const int tid = threadIdx.x + FTX * threadIdx.y;
     const int plane = nx * ny;
 
+    // plane rings: old(k) in L.o[k % 3], wrk(k) in L.w[k % 4], P(k) in
+    // L.p[k % 5]; slot pointers are formed once per stage
+    auto oslot = [&](int k) { return &L.o[k % 3][0][0]; };
+    auto wslot = [&](int k) { return &L.w[k & 3][0][0]; };
+    auto pslot = [&](int k) { return &L.p[k % 5][0][0]; };
+
     float rw[FP_PT], ro[FO_PT];
     auto fetch_w = [&](int k) {
 #pragma unroll
@@ -685,8 +711,7 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
                         ? wg[(size_t)k * plane + j * nx + i] : 0.0f;
         }
     };
-    auto store_w = [&](int k) {
-        float* dst = &L.w[k % 4][0][0];
+    auto store_w = [&](float* dst) {
 #pragma unroll
         for (int m = 0; m < FP_PT; ++m) {
             const int e = tid + 256 * m;
@@ -703,8 +728,7 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
                         ? og[(size_t)k * plane + j * nx + i] : 0.0f;
         }
     };
-    auto store_o = [&](int k) {
-        float* dst = &L.o[k % 3][0][0];
+    auto store_o = [&](float* dst) {
 #pragma unroll
         for (int m = 0; m < FO_PT; ++m) {
             const int e = tid + 256 * m;
@@ -713,83 +737,106 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     };
     // P of plane k on the tile + 2-cell halo (code/sub_fcsmooth.f90:44-83)
     auto make_p = [&](int k) {
-        float(*dst)[FPX] = L.p[k % 5];
-        const float(*w)[FPX] = L.w[k % 4];
+        float* dst = pslot(k);
+        const float* w = wslot(k);
+        const float* om = oslot(k + 2);   // plane k - 1
+        const float* oc = oslot(k);
+        const float* op = oslot(k + 1);
         const bool kface = (k == 0 || k == nz - 1);
-        const float(*om)[FOX] = L.o[(k + 2) % 3];   // plane k - 1
-        const float(*oc)[FOX] = L.o[k % 3];
-        const float(*op)[FOX] = L.o[(k + 1) % 3];
-        for (int e = tid; e < FP_N; e += 256) {
+#pragma unroll
+        for (int m = 0; m < FP_PT; ++m) {
+            const int e = tid + 256 * m;
+            if (e >= FP_N) break;
             const int ii = e % FPX, jj = e / FPX;
             const int i = i0 - 2 + ii, j = j0 - 2 + jj;
             float v = 0.0f;
             if (i >= 0 && i < nx && j >= 0 && j < ny) {
-                const float wc = w[jj][ii];
+                const float wc = w[e];
                 if (kface || i == 0 || i == nx - 1 || j == 0 || j == ny - 1) {
                     v = wc;
                 } else {
-                    const int oi = ii + 1, oj = jj + 1;
-                    v = wc + chi * (oc[oj][oi + 1] + oc[oj][oi - 1] + oc[oj + 1][oi] +
-                                    oc[oj - 1][oi] + op[oj][oi] + om[oj][oi] - 6.0f * oc[oj][oi]);
+                    const int oc0 = (jj + 1) * FOX + ii + 1;
+                    v = wc + chi * (oc[oc0 + 1] + oc[oc0 - 1] + oc[oc0 + FOX] + oc[oc0 - FOX] +
+                                    op[oc0] + om[oc0] - 6.0f * oc[oc0]);
                 }
             }
-            dst[jj][ii] = v;
+            dst[e] = v;
         }
     };
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = i0 + tx, j = j0 + ty;
-    const int pi = tx + 2, pj = ty + 2;
-    // flanks into the output (inner boxes): the cell's final FCT value is
-    // blended with the coarse box before it is stored
-
+    const int pc = (ty + 2) * FPX + tx + 2;   // this thread's cell in a P / wrk plane
+    float fz_prev = 0.0f;   // flux at the face below the plane being output
+    int fz_plane = -1;      // the plane whose upper-face flux fz_prev is
     auto output = [&](int k) {
-        if (i >= nx || j >= ny) return;
-        const float(*wk)[FPX] = L.w[k % 4];
-        const float(*pk)[FPX] = L.p[k % 5];
+        const float* wk = wslot(k);
+        const float* pk = pslot(k);
+        const bool mine = i < nx && j < ny;
         const size_t c = (size_t)k * plane + (size_t)j * nx + i;
         if (k == 0 || k == nz - 1) {     // z faces keep w
-            out[c] = wk[pj][pi];
+            if (mine) out[c] = wk[pc];
             return;
         }
-        float tv;
-        if (i > 0 && i < nx - 1) {
-            auto Wx = [&](int o) { return wk[pj][pi + o]; };
-            auto Px = [&](int o) { return pk[pj][pi + o]; };
-            const float fx = fct_flux_ax(Wx, Px, 0, i, nx, chi);
-            const float fxm = fct_flux_ax(Wx, Px, -1, i - 1, nx, chi);
-            tv = pk[pj][pi] - fx + fxm;
-        } else {
-            tv = wk[pj][pi];
+        // x: each lane's face flux, the lower one from the lane to the left
+        // (the tile's first lane computes its own)
+        float fx = 0.0f;
+        if (i < nx) {
+            auto Wx = [&](int o) { return wk[pc + o]; };
+            auto Px = [&](int o) { return pk[pc + o]; };
+            fx = fct_flux_ax(Wx, Px, 0, min(i, nx - 1), nx, chi);
         }
+        float fxm = __shfl_up(fx, 1, 64);
+        if (tx == 0 && i > 0 && i < nx) {
+            auto Wx = [&](int o) { return wk[pc + o]; };
+            auto Px = [&](int o) { return pk[pc + o]; };
+            fxm = fct_flux_ax(Wx, Px, -1, i - 1, nx, chi);
+        }
+        // z: this cell's column, P(k-2..k+2) and wrk(k-1..k+1), in registers;
+        // the flux below is the previous plane's flux above when it was made
+        const float Wm1 = wslot(k + 3)[pc], W0 = wk[pc], W1 = wslot(k + 1)[pc];
+        const float Pm2 = pslot(k + 3)[pc], Pm1 = pslot(k + 4)[pc], P0 = pk[pc];
+        const float P1 = pslot(k + 1)[pc], P2 = pslot(k + 2)[pc];   // P2 unused at k = nz-2
+        // face above (fct_flux_ax, base 0, m = k: no end shift for 1 <= k <= nz-2)
+        const float fz = fct_lim(W1 - W0, P0 - Pm1, (k + 1 == nz - 1) ? (P1 - P0) : (P2 - P1), chi);
+        float fzm;
+        if (fz_plane == k - 1) {
+            fzm = fz_prev;
+        } else if (k == 1) {   // base -1, m = 0: shifted to the face above plane 1
+            fzm = fct_lim(W1 - W0, P0 - Pm1, (nz == 3) ? (P1 - P0) : (P2 - P1), chi);
+        } else {               // base -1, m = k - 1
+            fzm = fct_lim(W0 - Wm1, Pm1 - Pm2, P1 - P0, chi);
+        }
+        fz_prev = fz;
+        fz_plane = k;
+        if (!mine) return;
+        float tv;
+        if (i > 0 && i < nx - 1) tv = pk[pc] - fx + fxm;
+        else tv = wk[pc];
         if (j > 0 && j < ny - 1) {
-            auto Wy = [&](int o) { return wk[pj + o][pi]; };
-            auto Py = [&](int o) { return pk[pj + o][pi]; };
+            auto Wy = [&](int o) { return wk[pc + o * FPX]; };
+            auto Py = [&](int o) { return pk[pc + o * FPX]; };
             const float fy = fct_flux_ax(Wy, Py, 0, j, ny, chi);
             const float fym = fct_flux_ax(Wy, Py, -1, j - 1, ny, chi);
             tv = tv - fy + fym;
         }
-        auto Wz = [&](int o) { return L.w[(k + o + 4) % 4][pj][pi]; };
-        auto Pz = [&](int o) { return L.p[(k + o + 5) % 5][pj][pi]; };
-        const float fz = fct_flux_ax(Wz, Pz, 0, k, nz, chi);
-        const float fzm = fct_flux_ax(Wz, Pz, -1, k - 1, nz, chi);
         out[c] = tv - fz + fzm;
     };
-
-    // P planes pstart..pend are made in order; making P(s) needs wrk(s) and
-    // old(s-1..s+1); output plane k needs P(k-2..k+2) and wrk(k-1..k+1)
+    // P planes pstart..pend are made in order; stage s stores wrk(s) and
+    // old(s+1) (fetched during stage s-1), makes P(s) and outputs plane
+    // s-2, which needs P(s-4..s) and wrk(s-3..s-1)
     const int pstart = max(0, kb - 2), pend = min(nz - 1, ke + 1);
     if (pstart >= 1) {
         fetch_o(pstart - 1);
-        store_o(pstart - 1);
+        store_o(oslot(pstart - 1));
     }
     fetch_o(pstart);
-    store_o(pstart);
+    store_o(oslot(pstart));
     fetch_w(pstart);
     if (pstart + 1 <= nz - 1) fetch_o(pstart + 1);
     for (int s = pstart; s <= pend; ++s) {
         __syncthreads();   // readers of the ring slots reused below are done
-        store_w(s);
-        if (s + 1 <= nz - 1) store_o(s + 1);
+        store_w(wslot(s));
+        if (s + 1 <= nz - 1) store_o(oslot(s + 1));
         if (s + 1 <= pend) {   // next plane's loads in flight during this one
             fetch_w(s + 1);
             if (s + 2 <= nz - 1) fetch_o(s + 2);
@@ -910,7 +957,10 @@ void launch_efield(const EParams& P, hipStream_t s) {
 
 void launch_push(const PushParams& P, hipStream_t s) {
     ProfScope prof_("k_push", s);
-    hipLaunchKernelGGL(k_push, grid3(P.B, 4), block3(), 0, s, P);
+    if (P.pow2)
+        hipLaunchKernelGGL(k_push<true>, grid3(P.B, 4), block3(), 0, s, P);
+    else
+        hipLaunchKernelGGL(k_push<false>, grid3(P.B, 4), block3(), 0, s, P);
 }
 
 int validate_blocks(const Box& B) {

@@ -76,6 +76,12 @@ void check(Run* R, const char* where) {
     }
 }
 
+// x is +-2^n (so 1/x is exact and x / y == x * (1/y) bit for bit)
+bool is_pow2(float x) {
+    int e = 0;
+    return x != 0.0f && std::isfinite(x) && std::fabs(std::frexp(x, &e)) == 0.5f;
+}
+
 SpeciesSet species(const Set& S, int s) {
     SpeciesSet Q;
     for (int m = 0; m < 10; ++m) Q.f[m] = S.f[10 * s + m];
@@ -156,6 +162,19 @@ void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float
     P.grav = d.grav;
     P.gamma = d.gamma;
     P.gamma1 = d.gamma1;
+    bool p2 = true;
+    for (int q = 0; q < 3; ++q) {
+        P.bmax[q] = 0.20f * P.rmass[q] / delt / d.reynolds;
+        P.rrmass[q] = 1.0f / P.rmass[q];
+        p2 = p2 && is_pow2(P.rmass[q]);
+    }
+    const float ds[6] = {P.dxt, P.dyt, P.dzt, P.fdxt, P.fdyt, P.fdzt};
+    float* rs[6] = {&P.rdxt, &P.rdyt, &P.rdzt, &P.rfdxt, &P.rfdyt, &P.rfdzt};
+    for (int q = 0; q < 6; ++q) {
+        *rs[q] = 1.0f / ds[q];
+        p2 = p2 && is_pow2(ds[q]);
+    }
+    P.pow2 = p2 ? 1 : 0;
     P.do_elec = do_elec;
     P.copy_shell = copy_shell;
     launch_push(P, R->s);
