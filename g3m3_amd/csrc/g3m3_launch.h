@@ -55,6 +55,12 @@ struct InnerPtrs {
     float* f[kMaxFields];
 };
 
+struct InnerLists {
+    const int* ij[3];
+    const float* parm[3];
+    int num[3];
+};
+
 struct FlankPtrs {
     float* tgt[kMaxFields];
     const float* cur[kMaxFields];
@@ -124,7 +130,7 @@ size_t fct_scratch_floats(int ncells, int nf);
 void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s);
 void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
                         const int num[3], int mbndry, int box, float xmin, float ymin, float dx,
-                        float dy, float re_equiv, float v_rot, hipStream_t s);
+                        float dy, float re_equiv, float v_rot, hipStream_t s, bool disjoint = false);
 void launch_flanks(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G, float t1,
                    float t2, hipStream_t s);
 void launch_corer(const Box& B, const FieldPtrs& fine, const FieldPtrs& coarse, int nf, int nskip,

@@ -99,12 +99,10 @@ void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s)
 // bndry_inner.  which: 0 = surface (:78-140), 1 = atmosphere (:143-201),
 // 2 = interior (:204-265).  Field order in I.f: q,h,o x
 // (rho,presx,presy,presz,px,py,pz,presxy,presxz,presyz), epres, bx, by, bz.
-__global__ void k_inner_list(Box B, InnerPtrs I, int which, const int* __restrict__ ij,
-                             const float* __restrict__ parm, int num, int mbndry, int b,
-                             float xmin, float ymin, float dx, float dy, float re_equiv,
-                             float v_rot) {
-    const int n = gtid();
-    if (n >= num) return;
+__device__ inline void inner_point(const Box& B, const InnerPtrs& I, int which,
+                                   const int* __restrict__ ij, const float* __restrict__ parm,
+                                   int n, int mbndry, int b, float xmin, float ymin, float dx,
+                                   float dy, float re_equiv, float v_rot) {
     const int i = ij[b + mbndry * (0 + 3 * n)] - 1;
     const int j = ij[b + mbndry * (1 + 3 * n)] - 1;
     const int k = ij[b + mbndry * (2 + 3 * n)] - 1;
@@ -141,9 +139,42 @@ __global__ void k_inner_list(Box B, InnerPtrs I, int which, const int* __restric
     }
 }
 
+__global__ void k_inner_list(Box B, InnerPtrs I, int which, const int* __restrict__ ij,
+                             const float* __restrict__ parm, int num, int mbndry, int b,
+                             float xmin, float ymin, float dx, float dy, float re_equiv,
+                             float v_rot) {
+    const int n = gtid();
+    if (n >= num) return;
+    inner_point(B, I, which, ij, parm, n, mbndry, b, xmin, ymin, dx, dy, re_equiv, v_rot);
+}
+
+// the three lists in one launch (blockIdx.y = list); only for lists with
+// no cell in common, where their order does not matter
+__global__ void k_inner_lists(Box B, InnerPtrs I, InnerLists Ls, int mbndry, int b, float xmin,
+                              float ymin, float dx, float dy, float re_equiv, float v_rot) {
+    const int w = blockIdx.y;
+    const int n = gtid();
+    if (n >= Ls.num[w]) return;
+    inner_point(B, I, w, Ls.ij[w], Ls.parm[w], n, mbndry, b, xmin, ymin, dx, dy, re_equiv, v_rot);
+}
+
 void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
                         const int num[3], int mbndry, int box, float xmin, float ymin, float dx,
-                        float dy, float re_equiv, float v_rot, hipStream_t s) {
+                        float dy, float re_equiv, float v_rot, hipStream_t s, bool disjoint) {
+    if (disjoint) {
+        InnerLists Ls;
+        int nmax = 0;
+        for (int w = 0; w < 3; ++w) {
+            Ls.ij[w] = ij[w];
+            Ls.parm[w] = parm[w];
+            Ls.num[w] = num[w] > 0 ? num[w] : 0;
+            nmax = num[w] > nmax ? num[w] : nmax;
+        }
+        if (nmax > 0)
+            hipLaunchKernelGGL(k_inner_lists, dim3((nmax + 255) / 256, 3), dim3(256), 0, s, B, I,
+                               Ls, mbndry, box - 1, xmin, ymin, dx, dy, re_equiv, v_rot);
+        return;
+    }
     for (int w = 0; w < 3; ++w)
         if (num[w] > 0)
             hipLaunchKernelGGL(k_inner_list, dim3((num[w] + 255) / 256), dim3(256), 0, s, B, I, w,

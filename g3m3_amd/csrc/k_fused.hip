@@ -905,9 +905,11 @@ void build_shell(const Box& B, const FlankGeom& G, ShellPt* out) {
     for (int k = 2; k <= nz - 3; ++k)
         for (int s = 0; s < 4; ++s)
             for (int i = 0; i < nx; ++i) add(i, js[s], k);
+    // the x faces: the four cells of a row (i = 0, 1, nx-2, nx-1) are
+    // adjacent, so the two cache lines of the row are fetched once
     for (int k = 2; k <= nz - 3; ++k)
-        for (int s = 0; s < 4; ++s)
-            for (int j = 2; j <= ny - 3; ++j) add(is[s], j, k);
+        for (int j = 2; j <= ny - 3; ++j)
+            for (int s = 0; s < 4; ++s) add(is[s], j, k);
 }
 
 __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict__ tab, int npts,

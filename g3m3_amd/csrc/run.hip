@@ -54,6 +54,7 @@ struct Run {
     // the state array.
     std::vector<char> rot;
     int vpass = 0;                // which validation counter the next pass uses
+    bool lists_disjoint = false;  // the planet lists share no cell: one launch for all three
     // flank shell tables of the inner boxes (geometry is fixed for a run)
     ShellPt* shell[G3M3_MAX_GRIDS] = {};
     int nshell = 0;
@@ -264,7 +265,7 @@ void inner_bc(Run* R, const Set& set) {
     const float dx = (d.grd_xmax[b] - d.grd_xmin[b]) / (float)(d.nx - 1.0f);
     const float dy = (d.grd_ymax[b] - d.grd_ymin[b]) / (float)(d.ny - 1.0f);
     launch_bndry_inner(R->B, I, ij, parm, num, d.mbndry, box, d.grd_xmin[b], d.grd_ymin[b], dx, dy,
-                       d.re_equiv, d.v_rot, R->s);
+                       d.re_equiv, d.v_rot, R->s, R->lists_disjoint);
 }
 
 void validate(Run* R, const Set& set, int box, bool want_max) {
@@ -483,6 +484,24 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
             }
             hipMemcpy(R->rst_eff, h.data(), sizeof(float) * n, hipMemcpyHostToDevice);
         }
+    }
+    if (ok && desc->mbndry >= 1) {
+        // do the planet lists of box mbndry share a cell?  (their order
+        // matters only if they do)
+        const int mb = desc->mbndry, b = desc->mbndry - 1;
+        const int* lists[3] = {desc->ijsrf, desc->ijmid, desc->ijzero};
+        const int nums[3] = {desc->numsrf, desc->nummid, desc->numzero};
+        std::vector<int> cells;
+        for (int l = 0; l < 3; ++l) {
+            if (nums[l] <= 0) continue;
+            std::vector<int> ij((size_t)mb * 3 * nums[l]);
+            hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(), hipMemcpyDeviceToHost);
+            for (int p = 0; p < nums[l]; ++p)
+                cells.push_back(R->B.IX(ij[b + mb * (0 + 3 * p)] - 1, ij[b + mb * (1 + 3 * p)] - 1,
+                                        ij[b + mb * (2 + 3 * p)] - 1));
+        }
+        std::sort(cells.begin(), cells.end());
+        R->lists_disjoint = std::adjacent_find(cells.begin(), cells.end()) == cells.end();
     }
     R->nshell = shell_count(R->B);
     for (int box = 1; ok && box < ng; ++box) {
