@@ -74,6 +74,22 @@ def units_per_launch(kind, wl, nfields=34):
     return nx * ny * nz * nfields
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters, calibrated on a plain copy (the commands and the
+    calibration are in profiles/traffic_native_<workload>.md)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_native_{workload}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        data = json.load(fh)
+    for name, rec in data.get("kernels", {}).items():
+        if name == kernel or name.startswith(kernel + "<"):
+            return rec["traffic_bytes"], f"profiles/traffic_native_{workload}.md"
+    return None, None
+
+
 def cpu_baseline(workload: str, threads: int):
     """The reference (OpenMP build) on one full step of the same workload."""
     try:
@@ -166,6 +182,7 @@ def main(argv=None):
         assert bool(torch.isfinite(v).all()), f"non-finite {k} after the timed steps"
 
     roof = None
+    traffic, tsrc = measured_traffic(a.workload, a.prof_kernel)
     if nl.value and a.prof_kernel in KERNEL_BYTES:
         per_unit, kind = KERNEL_BYTES[a.prof_kernel]
         per_launch = per_unit * units_per_launch(kind, wl)
@@ -173,7 +190,7 @@ def main(argv=None):
         achieved = per_launch / avg_s / 1e9
         roof = {"kernel": a.prof_kernel, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": per_launch,
+                "traffic": traffic, "traffic_source": tsrc, "bytes_per_launch": per_launch,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl.value}
 
     divb = None
