@@ -139,6 +139,9 @@ def main(argv=None):
                          "call per reference subroutine")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-divb", action="store_true")
+    ap.add_argument("--divb-mode", type=int, default=2,
+                    help="0/1: the reference's CGNR (single / double scalars); 2: conjugate "
+                         "gradients on the same system (FP64 dots)")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -202,13 +205,16 @@ def main(argv=None):
         d0 = time.perf_counter()
         info = []
         for box in range(1, wl.ngrd + 1):
-            st, it, r2 = run.divb(box, errlim, mode=1, max_iters=20000)
+            st, it, r2 = run.divb(box, errlim, mode=a.divb_mode, max_iters=20000)
             info.append({"box": box, "status": st, "iterations": it})
         torch.cuda.synchronize()
         dsec = time.perf_counter() - d0
         per_step = wall / a.steps
+        modes = {0: "CGNR (the reference's solver), fp32 scalars",
+                 1: "CGNR (the reference's solver), fp32 vectors, fp64 dots/scalars",
+                 2: "CG on the same system, fp32 vectors, fp64 dots/scalars"}
         divb = {"seconds_all_boxes": round(dsec, 3), "boxes": info, "rel_residual": 1e-6,
-                "mode": "fp32 matvec, fp64 dots/scalars",
+                "mode": modes.get(a.divb_mode, str(a.divb_mode)),
                 "cell_updates_per_s_incl_divb_every_10_steps":
                     cu / a.steps * 10 / (per_step * 10 + dsec)}
 

@@ -371,6 +371,159 @@ struct DivbSolver {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Conjugate gradients on the same system (mode kDivbCG).  The 7-band
+// matrix A of code/sub_divB_correct.f90:85-123 is symmetric negative
+// definite, so plain CG on M = -A, c = -b converges in about the square
+// root of CGNR's iteration count.  Vectors FP32, dot products and scalars
+// FP64, stopping on the reference's criterion |r|^2 <= |b|^2 npts errlim^2.
+// Two fused kernels and two one-block reductions per iteration; the new
+// search direction p = r + beta p is formed inside the next matrix product.
+
+enum { kCgRR = 0, kCgAlpha = 1, kCgBeta = 2, kCgB2 = 3, kCgErr2 = 8, kCgTmp = 9 };
+
+// p_new = r + beta p_old (r alone on the first iteration) at the 7 points,
+// q = M p_new, partial sums of p_new.q
+__global__ void k_cg_matvec(Box B, const float* __restrict__ r, const float* __restrict__ pold,
+                            float* pnew, float* q, double* part, const double* sc,
+                            const DivbCtl* ctl, int first) {
+    if (ctl->status != 0) return;
+    int i, j, k;
+    const int c = gtid();
+    double pq = 0.0;
+    if (cell_of(B, c, i, j, k)) {
+        const double beta = first ? 0.0 : sc[kCgBeta];
+        auto P = [&](int n) {
+            return first ? r[n] : (float)((double)r[n] + beta * (double)pold[n]);
+        };
+        const int X = 1, Y = B.nx, Z = B.nx * B.ny;
+        const float pc = P(c);
+        float t = 6.0f * pc;
+        if (k < B.nz - 1) t = t - P(c + Z);
+        if (j < B.ny - 1) t = t - P(c + Y);
+        if (i < B.nx - 1) t = t - P(c + X);
+        if (i > 0) t = t - P(c - X);
+        if (j > 0) t = t - P(c - Y);
+        if (k > 0) t = t - P(c - Z);
+        pnew[c] = pc;
+        q[c] = t;
+        pq = (double)pc * (double)t;
+    }
+    const double s1 = block_sum(pq);
+    if (threadIdx.x == 0) part[blockIdx.x] = s1;
+}
+
+// alpha = rr / p.q
+__global__ void k_cg_alpha(const double* __restrict__ part, int nb, double* sc,
+                           const DivbCtl* ctl) {
+    if (ctl->status != 0) return;
+    const double pq = sum_parts(part, nb);
+    if (threadIdx.x == 0) sc[kCgAlpha] = sc[kCgRR] / pq;
+}
+
+// x += alpha p, r -= alpha q, partial sums of r.r
+__global__ void k_cg_update(int n, float* x, float* r, const float* __restrict__ p,
+                            const float* __restrict__ q, double* part, const double* sc,
+                            const DivbCtl* ctl) {
+    if (ctl->status != 0) return;
+    const int c = gtid();
+    double rr = 0.0;
+    if (c < n) {
+        const double a = sc[kCgAlpha];
+        x[c] = (float)((double)x[c] + a * (double)p[c]);
+        const float rn = (float)((double)r[c] - a * (double)q[c]);
+        r[c] = rn;
+        rr = (double)rn * (double)rn;
+    }
+    const double s1 = block_sum(rr);
+    if (threadIdx.x == 0) part[blockIdx.x] = s1;
+}
+
+// rr_new, the stopping test, beta = rr_new / rr
+__global__ void k_cg_beta(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl) {
+    if (ctl->status != 0) return;
+    const double rn = sum_parts(part, nb);
+    if (threadIdx.x == 0) {
+        ctl->iters += 1;
+        if (rn <= sc[kCgB2] * sc[kCgErr2] || rn == 0.0) ctl->status = 1;
+        sc[kCgBeta] = rn / sc[kCgRR];
+        sc[kCgRR] = rn;
+    }
+}
+
+// r = c = -b, x = 0, |c|^2 partials
+__global__ void k_cg_init(int n, const float* __restrict__ b, float* x, float* r, double* part) {
+    const int c = gtid();
+    double bb = 0.0;
+    if (c < n) {
+        const float v = -b[c];
+        r[c] = v;
+        x[c] = 0.0f;
+        bb = (double)v * (double)v;
+    }
+    const double s1 = block_sum(bb);
+    if (threadIdx.x == 0) part[blockIdx.x] = s1;
+}
+
+__global__ void k_sum1(const double* __restrict__ part, int nb, double* out) {
+    const double v = sum_parts(part, nb);
+    if (threadIdx.x == 0) *out = v;
+}
+
+__global__ void k_cg_start(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl,
+                           double err2) {
+    const double bb = sum_parts(part, nb);
+    if (threadIdx.x == 0) {
+        sc[kCgRR] = bb;
+        sc[kCgB2] = bb;
+        sc[kCgErr2] = err2;
+        ctl->status = bb == 0.0 ? 1 : 0;
+        ctl->iters = 0;
+    }
+}
+
+// CG solve of A soln = rhs from zero; work: x lives in soln, 4 more vectors
+// (r, p0, p1, q) in `vec`.  Returns 1 converged, 3 iteration cap.
+static int cg_solve(const Box& B, const float* rhs, float* soln, float* vec, double* sc,
+                    DivbCtl* ctl, double* part, double err2, long long max_iters,
+                    long long* iters_out, hipStream_t s) {
+    const int n = B.V();
+    const int nb = (n + kRedThreads - 1) / kRedThreads;
+    float* r = vec;
+    float* p[2] = {vec + (size_t)n, vec + 2 * (size_t)n};
+    float* q = vec + 3 * (size_t)n;
+    const dim3 g(nb), t(kRedThreads);
+    hipLaunchKernelGGL(k_cg_init, g, t, 0, s, n, rhs, soln, r, part);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), t, 0, s, part, nb, sc, ctl, err2);
+    long long it = 0;
+    int cur = 0;
+    for (;;) {
+        const int nbatch = (int)std::min<long long>(kBatch, max_iters - it);
+        if (nbatch <= 0) break;
+        for (int q2 = 0; q2 < nbatch; ++q2) {
+            hipLaunchKernelGGL(k_cg_matvec, g, t, 0, s, B, r, p[cur], p[cur ^ 1], q, part, sc, ctl,
+                               (it + q2) == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), t, 0, s, part, nb, sc, ctl);
+            hipLaunchKernelGGL(k_cg_update, g, t, 0, s, n, soln, r, p[cur ^ 1], q, part, sc, ctl);
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), t, 0, s, part, nb, sc, ctl);
+            cur ^= 1;
+        }
+        it += nbatch;
+        DivbCtl h;
+        hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        if (h.status != 0) {
+            if (iters_out) *iters_out = h.iters;
+            return 1;
+        }
+    }
+    DivbCtl h;
+    hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    if (iters_out) *iters_out = h.iters;
+    return h.status == 1 ? 1 : 3;
+}
+
 int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float errlim, int mode,
                  long max_iters, float* work, double* dwork, long* iters_out, double* res2_out,
                  hipStream_t s) {
@@ -398,10 +551,22 @@ int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float 
     const dim3 g(nb), t(kRedThreads);
     hipLaunchKernelGGL(k_divb_init, dim3(1), dim3(64), 0, s, S.sc, S.ctl, errlim2);
     hipLaunchKernelGGL(k_divb_rhs, g, t, 0, s, B, bx, by, bz, rhs, 2.0f * rx);
-    // the reference's soln_vec is an uninitialised automatic array; start from 0
-    hipLaunchKernelGGL(k_zero1, g, t, 0, s, n, S.soln);
-    S.solve(0, false, 0);
-    S.residual2 = S.read_sc(kScR2);
+    if (mode == kDivbCG) {
+        // plain CG from zero; the true residual |A soln - b|^2 at the end
+        long long it = 0;
+        S.status = cg_solve(B, rhs, S.soln, work + 2 * (size_t)n, S.sc, S.ctl, S.part,
+                            (double)n * (double)errlim * (double)errlim, S.max_iters, &it, s);
+        S.iters = it;
+        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, Bands{}, S.soln, S.lv[3].xi, rhs, S.part,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_sum1, dim3(1), t, 0, s, S.part, nb, S.sc + kCgTmp);
+        S.residual2 = S.read_sc(kCgTmp);
+    } else {
+        // the reference's soln_vec is an uninitialised automatic array; start from 0
+        hipLaunchKernelGGL(k_zero1, g, t, 0, s, n, S.soln);
+        S.solve(0, false, 0);
+        S.residual2 = S.read_sc(kScR2);
+    }
     hipLaunchKernelGGL(k_divb_apply, g, t, 0, s, B, S.soln, bx, by, bz, rx * rx, 2.0f * rx,
                        2.0f * rx, 2.0f * rx);
     if (iters_out) *iters_out = S.iters;

@@ -326,11 +326,15 @@ void g3m3_divb_correct(const int* nx, const int* ny, const int* nz, const int* n
                        const float* xspac, const float* divB_errlim, float* bx, float* by,
                        float* bz);
 
-/* No reference counterpart: g3m3_divb_correct with a precision mode
- * (0 = single-precision scalars as the reference, 1 = double-precision
- * scalars and updates), an iteration cap (0 = the reference's 10*npts), and
- * the iteration count and final squared residual returned.  Returns the
- * solver status: 1 converged, 2 stopped after restarts, 3 iteration cap. */
+/* No reference counterpart: g3m3_divb_correct with a solver mode
+ *   0 = the reference's CGNR, single-precision scalars,
+ *   1 = the reference's CGNR, double-precision scalars and updates,
+ *   2 = conjugate gradients on the same (symmetric negative definite)
+ *       system, FP32 vectors with FP64 dot products and scalars, stopping
+ *       on the reference's criterion -- about sqrt of CGNR's iterations,
+ * an iteration cap (0 = the reference's 10*npts), and the iteration count
+ * and final squared residual |A x - b|^2 returned.  Returns the solver
+ * status: 1 converged, 2 stopped after restarts, 3 iteration cap. */
 int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int* n_grids,
                          const int* box, const float* xspac, const float* divB_errlim,
                          float* bx, float* by, float* bz, const int* mode,
