@@ -90,6 +90,42 @@ def measured_traffic(workload: str, kernel: str):
     return None, None
 
 
+def precision_case(fast_mode: int):
+    """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
+    FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
+    FP32-vector CG alone (mode 2) at 1e-6 beside it."""
+    import ctypes
+    import torch
+    from g3m3_amd import _lib
+    from g3m3_amd.workload import precision_field
+    host = precision_field()
+    nz, ny, nx = host[0].shape[1:]
+    npts = nx * ny * nz
+    xs = np.array([1.0], np.float32)
+    # |b|^2 of the system's right side (central-difference divergence / 2)
+    X, Y, Z = (h[0].astype(np.float64) for h in host)
+    rhs = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2]) + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
+           + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / 2.0
+    b2 = float((rhs * rhs).sum())
+    del X, Y, Z, rhs
+    res = {"grid": f"{nx}x{ny}x{nz}"}
+    for mode, rel in ((2, 1e-6), (3, 1e-8)):
+        dev = [torch.from_numpy(h).cuda() for h in host]
+        torch.cuda.synchronize()
+        it = ctypes.c_longlong(0)
+        r2 = ctypes.c_double(0)
+        t0 = time.perf_counter()
+        st = _lib.call("divb_correct_ex", nx, ny, nz, 1, 1, xs, np.float32(rel / np.sqrt(npts)),
+                       dev[0], dev[1], dev[2], mode, ctypes.c_longlong(0), it, r2)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        _lib.check("divb precision")
+        res[f"mode{mode}"] = {"rel_residual_target": rel, "status": st, "iterations": it.value,
+                              "seconds": round(dt, 3),
+                              "achieved_rel_residual": float(np.sqrt(max(r2.value, 0.0) / b2))}
+    return res
+
+
 def cpu_baseline(workload: str, threads: int):
     """The reference (OpenMP build) on one full step of the same workload."""
     try:
@@ -142,6 +178,9 @@ def main(argv=None):
     ap.add_argument("--divb-mode", type=int, default=2,
                     help="0/1: the reference's CGNR (single / double scalars); 2: conjugate "
                          "gradients on the same system (FP64 dots)")
+    ap.add_argument("--precision", action="store_true",
+                    help="also run the divB precision case: 241x241x121 dipole + 1%% noise, "
+                         "mode 3 (FP64 residual refinement) to rel-residual 1e-8")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -218,6 +257,7 @@ def main(argv=None):
                 "cell_updates_per_s_incl_divb_every_10_steps":
                     cu / a.steps * 10 / (per_step * 10 + dsec)}
 
+    prec = precision_case(a.divb_mode) if a.precision else None
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -231,6 +271,8 @@ def main(argv=None):
                    "box_steps": run.box_steps - bs0, "cell_updates": cu},
         "roofline": roof, "cpu_baseline": cpu, "divb": divb,
     }
+    if prec is not None:
+        out["divb_precision"] = prec
     print(json.dumps(out))
     return 0
 

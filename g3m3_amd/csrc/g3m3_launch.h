@@ -74,7 +74,7 @@ struct FlankGeom {
     float szmin, sz, bzmin, delz;
 };
 
-enum { kDivbRef = 0, kDivbF64 = 1, kDivbCG = 2 };
+enum { kDivbRef = 0, kDivbF64 = 1, kDivbCG = 2, kDivbRefine = 3 };
 
 // Optional per-kernel timing: when the kernel named by g3m3_prof_select()
 // is launched, HIP events are recorded around it on its stream.

@@ -524,6 +524,105 @@ static int cg_solve(const Box& B, const float* rhs, float* soln, float* vec, dou
     return h.status == 1 ? 1 : 3;
 }
 
+// ---------------------------------------------------------------------------
+// Residual refinement (mode kDivbRefine): the solution is kept in FP64; each
+// outer pass forms r = b - A x in FP64, solves A d = r by the FP32-vector CG
+// above to a loose tolerance and adds d in FP64, until the FP64 residual
+// meets the reference's criterion.  Reaches residuals FP32 products alone
+// cannot (the matrix entries are small integers, exact in FP32).
+
+// r = b - A x in FP64 (x FP64), r rounded to FP32 for the inner solve,
+// partial sums of r^2 and (first pass) of b^2
+__global__ void k_dres(Box B, const double* __restrict__ x, const float* __restrict__ b,
+                       float* rf, double* part, int want_b2) {
+    int i, j, k;
+    const int c = gtid();
+    double rr = 0.0, bb = 0.0;
+    if (cell_of(B, c, i, j, k)) {
+        const int X = 1, Y = B.nx, Z = B.nx * B.ny;
+        double t = 0.0;
+        if (k < B.nz - 1) t += x[c + Z];
+        if (j < B.ny - 1) t += x[c + Y];
+        if (i < B.nx - 1) t += x[c + X];
+        t += -6.0 * x[c];
+        if (i > 0) t += x[c - X];
+        if (j > 0) t += x[c - Y];
+        if (k > 0) t += x[c - Z];
+        const double r = (double)b[c] - t;
+        rf[c] = (float)r;
+        rr = r * r;
+        bb = (double)b[c] * (double)b[c];
+    }
+    const double s1 = block_sum(rr);
+    if (threadIdx.x == 0) part[blockIdx.x] = s1;
+    if (want_b2) {
+        const double s2 = block_sum(bb);
+        if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+__global__ void k_dzero(int n, double* x) {
+    const int c = gtid();
+    if (c < n) x[c] = 0.0;
+}
+
+__global__ void k_dadd(int n, double* x, const float* __restrict__ d) {
+    const int c = gtid();
+    if (c < n) x[c] += (double)d[c];
+}
+
+__global__ void k_d2f(int n, const double* __restrict__ x, float* y) {
+    const int c = gtid();
+    if (c < n) y[c] = (float)x[c];
+}
+
+__global__ void k_sum2(const double* __restrict__ part, int nb, double* out) {
+    const double a = sum_parts(part, nb);
+    const double b = sum_parts(part + nb, nb);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// x (FP64, `xd`) solving A x = b to |b - A x|^2 <= |b|^2 * err2; soln gets
+// x rounded to FP32.  Returns 1 converged, 3 pass / iteration cap.
+static int refine_solve(const Box& B, const float* rhs, float* soln, double* xd, float* rf,
+                        float* d, float* vec, double* sc, DivbCtl* ctl, double* part, double err2,
+                        long long max_iters, long long* iters_out, double* res2_out,
+                        hipStream_t s) {
+    const int n = B.V();
+    const int nb = (n + kRedThreads - 1) / kRedThreads;
+    const dim3 g(nb), t(kRedThreads);
+    hipLaunchKernelGGL(k_dzero, g, t, 0, s, n, xd);
+    long long total = 0;
+    double b2 = 0.0, r2 = 0.0;
+    int status = 3;
+    for (int pass = 0; pass < 20; ++pass) {
+        hipLaunchKernelGGL(k_dres, g, t, 0, s, B, xd, rhs, rf, part, pass == 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_sum2, dim3(1), t, 0, s, part, nb, sc + kCgTmp);
+        double h[2];
+        hipMemcpyAsync(h, sc + kCgTmp, sizeof(h), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        r2 = h[0];
+        if (pass == 0) b2 = h[1];
+        if (r2 <= b2 * err2) {
+            status = 1;
+            break;
+        }
+        if (total >= max_iters) break;
+        // inner solve to 1e-4 of the current residual (FP32 vectors)
+        long long it = 0;
+        cg_solve(B, rf, d, vec, sc, ctl, part, 1e-8, max_iters - total, &it, s);
+        total += it;
+        hipLaunchKernelGGL(k_dadd, g, t, 0, s, n, xd, d);
+    }
+    hipLaunchKernelGGL(k_d2f, g, t, 0, s, n, xd, soln);
+    if (iters_out) *iters_out = total;
+    if (res2_out) *res2_out = r2;
+    return status;
+}
+
 int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float errlim, int mode,
                  long max_iters, float* work, double* dwork, long* iters_out, double* res2_out,
                  hipStream_t s) {
@@ -551,7 +650,22 @@ int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float 
     const dim3 g(nb), t(kRedThreads);
     hipLaunchKernelGGL(k_divb_init, dim3(1), dim3(64), 0, s, S.sc, S.ctl, errlim2);
     hipLaunchKernelGGL(k_divb_rhs, g, t, 0, s, B, bx, by, bz, rhs, 2.0f * rx);
-    if (mode == kDivbCG) {
+    if (mode == kDivbRefine) {
+        // FP64 solution and residual, FP32-vector inner CG; layout after
+        // rhs, soln: CG vectors (4n), inner correction d (n), FP32
+        // residual (n), then FP64 x (2n floats)
+        float* vec = work + 2 * (size_t)n;
+        float* dvec = work + 6 * (size_t)n;
+        float* rf = work + 7 * (size_t)n;
+        double* xd = reinterpret_cast<double*>(work + 8 * (size_t)n);
+        long long it = 0;
+        double r2 = 0.0;
+        S.status = refine_solve(B, rhs, S.soln, xd, rf, dvec, vec, S.sc, S.ctl, S.part,
+                                (double)n * (double)errlim * (double)errlim, S.max_iters, &it,
+                                &r2, s);
+        S.iters = it;
+        S.residual2 = r2;
+    } else if (mode == kDivbCG) {
         // plain CG from zero; the true residual |A soln - b|^2 at the end
         long long it = 0;
         S.status = cg_solve(B, rhs, S.soln, work + 2 * (size_t)n, S.sc, S.ctl, S.part,

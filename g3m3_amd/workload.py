@@ -200,3 +200,26 @@ def make_workload(name: str = "nested121", seed: int = 20261203, **over) -> Work
     return Workload(name=name, nx=nx, ny=ny, nz=nz, ngrd=ngrd, steps=int(p["steps"]),
                     divb_every=int(p["divb_every"]), divb_tol=float(p.get("divb_tol", 1e-6)),
                     grid=grid, scal=sc, fields=f, lists=lists)
+
+
+def precision_field(L: int = 120, seed: int = 20261205, r_inner: float = 3.0):
+    """The divB precision case: one (2L+1, 2L+1, L+1) box of unit spacing
+    (241x241x121 at L=120) holding a dipole field plus Gaussian noise of
+    sigma = 1% of |B| on each component (seeded), as float32 arrays of shape
+    (1, nz, ny, nx) -- the field whose divergence divB_correct removes."""
+    nx, ny, nz = 2 * L + 1, 2 * L + 1, L + 1
+    rng = np.random.default_rng(seed)
+    ax = (np.arange(nx, dtype=F) - F(L) - F(0.00001))[None, None, :]
+    ay = (np.arange(ny, dtype=F) - F(L))[None, :, None]
+    az = (np.arange(nz, dtype=F) - F(L // 2))[:, None, None]
+    ar = np.sqrt(ax * ax + ay * ay + az * az).astype(F)
+    ar5 = np.maximum(ar, F(r_inner - 1.5)) ** 5
+    bmag = F(-10.0 * r_inner ** 3) / ar5
+    b = [(-3.0 * bmag * ax * az).astype(F), (-3.0 * bmag * ay * az).astype(F),
+         (bmag * (ax * ax + ay * ay - 2.0 * az * az)).astype(F)]
+    mag = np.sqrt(b[0] ** 2 + b[1] ** 2 + b[2] ** 2)
+    out = []
+    for c in b:
+        noisy = c + F(0.01) * mag * rng.standard_normal(c.shape).astype(F)
+        out.append(np.ascontiguousarray(noisy.astype(F)[None]))
+    return out

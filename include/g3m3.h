@@ -332,6 +332,10 @@ void g3m3_divb_correct(const int* nx, const int* ny, const int* nz, const int* n
  *   2 = conjugate gradients on the same (symmetric negative definite)
  *       system, FP32 vectors with FP64 dot products and scalars, stopping
  *       on the reference's criterion -- about sqrt of CGNR's iterations,
+ *   3 = mode 2 inside FP64 residual refinement: the solution and the
+ *       residual b - A x in FP64, corrections by the FP32-vector CG, until
+ *       the FP64 residual meets the criterion (relative residuals well
+ *       below FP32's reach, e.g. 1e-8),
  * an iteration cap (0 = the reference's 10*npts), and the iteration count
  * and final squared residual |A x - b|^2 returned.  Returns the solver
  * status: 1 converged, 2 stopped after restarts, 3 iteration cap. */

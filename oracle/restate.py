@@ -1177,3 +1177,41 @@ def sample_fields(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv, fi
             out[p, q] = r[0]
         gps[p] = gp
     return out, gps
+
+
+def divB_correct_f64(bx, by, bz, rx):
+    """All-FP64 divB_correct (code/sub_divB_correct.f90:130-172) of one box
+    slice (nz, ny, nx): the central-difference right side, an exact sparse
+    solve of the same 7-band system (scipy), phi = rx^2 * x and B -= grad
+    phi on the interior, all in double.  Returns the three corrected
+    fields (float64)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    nz, ny, nx = bx.shape
+    n = nx * ny * nz
+    M = divb_matrix(nx, ny, nz).astype(np.float64)
+    pos = [nx * ny, nx, 1, 0, -1, -nx, -nx * ny]
+    rows, cols, vals = [], [], []
+    for b in range(7):
+        r = np.arange(n)
+        c = r + pos[b]
+        ok = (c >= 0) & (c < n)
+        v = M[c[ok], b]
+        keep = v != 0
+        rows.append(r[ok][keep])
+        cols.append(c[ok][keep])
+        vals.append(v[keep])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n))
+    X, Y, Z = (a.astype(np.float64) for a in (bx, by, bz))
+    rx = float(rx)
+    rhs = np.zeros((nz, ny, nx))
+    rhs[1:-1, 1:-1, 1:-1] = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2])
+                             + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
+                             + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / (2.0 * rx)
+    phi = (spl.spsolve(A.tocsc(), rhs.ravel()) * rx * rx).reshape(nz, ny, nx)
+    out = [X.copy(), Y.copy(), Z.copy()]
+    out[0][1:-1, 1:-1, 1:-1] -= (phi[1:-1, 1:-1, 2:] - phi[1:-1, 1:-1, :-2]) / (2.0 * rx)
+    out[1][1:-1, 1:-1, 1:-1] -= (phi[1:-1, 2:, 1:-1] - phi[1:-1, :-2, 1:-1]) / (2.0 * rx)
+    out[2][1:-1, 1:-1, 1:-1] -= (phi[2:, 1:-1, 1:-1] - phi[:-2, 1:-1, 1:-1]) / (2.0 * rx)
+    return out

@@ -84,3 +84,9 @@ def test_native_runtime(name, steps, over):
     """The native runtime (fused kernels, one call per full step) against
     the oracle: same bar."""
     run_pair(name, steps, {1, steps}, native=True, **over)
+
+
+def test_native_single61_100_steps():
+    """The 61x61x31 single-grid case over 100 steps, every field checked
+    after 1, 10 and 100 steps (Lapidus and FCT every step)."""
+    run_pair("single61", 100, {1, 10, 100}, native=True)

@@ -173,3 +173,23 @@ def test_interpolation_exact_on_linear_fields():
     coarse = w.a["bx"][1].copy()
     O.corer(w.a["bx"], 13, 13, 9, 2, 1, *[w.a[g] for g in GRID])
     assert np.abs(w.a["bx"][1] - coarse).max() < 1e-5
+
+
+def test_divb_fp64_oracle_matches_single_precision_solve():
+    """The all-FP64 divB oracle and the single-precision restatement of the
+    reference's divB_correct (CGNR to a tight tolerance) agree to the
+    tolerance: same right side, matrix, scaling and correction stencil (a
+    sign or index slip in either shows up as an O(1) difference)."""
+    pytest.importorskip("scipy.sparse")
+    w = World(9, 8, 7, 1, seed=12, lists=False)
+    nx, ny, nz = 9, 8, 7
+    ref = O.divB_correct_f64(w.a["bx"][0], w.a["by"][0], w.a["bz"][0], 2.0)
+    wo = w.copy()
+    wo.a["xspac"] = np.array([2.0], F)
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    O.divB_correct(nx, ny, nz, 1, 1, 7, pos, wo.a["xspac"], F(1e-7), wo.a["bx"], wo.a["by"],
+                   wo.a["bz"])
+    for k, r in zip(("bx", "by", "bz"), ref):
+        d = np.abs(wo.a[k][0].astype(np.float64) - r).max()
+        assert d <= 1e-4 * np.abs(r).max(), (k, d)
+        assert not np.allclose(r, w.a[k][0].astype(np.float64))   # the correction did something
