@@ -90,6 +90,41 @@ def measured_traffic(workload: str, kernel: str):
     return None, None
 
 
+def batch_case(n: int = 32, steps: int = 20, streams: int = 4):
+    """Throughput of n independent runs stepped one after another (each run
+    alone on the GPU) and stepped together (g3m3_amd.driver.Batch)."""
+    import torch
+    from g3m3_amd.driver import Batch, FastRun
+    from g3m3_amd.workload import make_batch_workloads
+    wls = make_batch_workloads(n)
+    runs = [FastRun(wl) for wl in wls]
+    for r in runs:
+        r.init_dt()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in runs:
+        for _ in range(steps):
+            r.full_step()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    cu_plain = sum(r.cell_updates for r in runs)
+    del runs
+    b = Batch(wls, streams=streams)
+    b.init_dt()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.full_step()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    cu_batch = b.cell_updates
+    return {"runs": n, "steps": steps, "streams": streams,
+            "cells_min_max": [min(w.cells_per_box for w in wls), max(w.cells_per_box for w in wls)],
+            "plain_cell_updates_per_s": cu_plain / t_plain,
+            "batched_cell_updates_per_s": cu_batch / t_batch,
+            "speedup": (cu_batch / t_batch) / (cu_plain / t_plain)}
+
+
 def precision_case(fast_mode: int):
     """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
     FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
@@ -181,6 +216,10 @@ def main(argv=None):
     ap.add_argument("--precision", action="store_true",
                     help="also run the divB precision case: 241x241x121 dipole + 1%% noise, "
                          "mode 3 (FP64 residual refinement) to rel-residual 1e-8")
+    ap.add_argument("--batch", action="store_true",
+                    help="also run the batching case: 32 independent single-grid runs "
+                         "(41x41x21 .. 121x121x61), 20 full steps each, one at a time and "
+                         "then together on 4 streams")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -258,6 +297,7 @@ def main(argv=None):
                     cu / a.steps * 10 / (per_step * 10 + dsec)}
 
     prec = precision_case(a.divb_mode) if a.precision else None
+    batch = batch_case() if a.batch else None
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -273,6 +313,8 @@ def main(argv=None):
     }
     if prec is not None:
         out["divb_precision"] = prec
+    if batch is not None:
+        out["batch"] = batch
     print(json.dumps(out))
     return 0
 

@@ -54,6 +54,7 @@ struct Run {
     // the state array.
     std::vector<char> rot;
     int vpass = 0;                // which validation counter the next pass uses
+    std::vector<int> stepped;     // boxes stepped by the last issued full step
     bool lists_disjoint = false;  // the planet lists share no cell: one launch for all three
     // flank shell tables of the inner boxes (geometry is fixed for a run)
     ShellPt* shell[G3M3_MAX_GRIDS] = {};
@@ -555,6 +556,12 @@ int g3m3_run_init_dt(void* run) {
 }
 
 int g3m3_run_full_step(void* run, const double* dt_override) {
+    const int e = g3m3_run_full_step_async(run, dt_override);
+    if (e) return e;
+    return g3m3_run_finish(run);
+}
+
+int g3m3_run_full_step_async(void* run, const double* dt_override) {
     Run* R = (Run*)run;
     const int n = R->d.n_grids;
     std::vector<double>& ts = R->t_step;
@@ -611,8 +618,16 @@ int g3m3_run_full_step(void* run, const double* dt_override) {
         corer(R, box);
         set_rho_only(R, R->state(box));
     }
-    read_tstepnew(R, stepped);
+    R->stepped = stepped;
     check(R, "full_step");
+    return R->err;
+}
+
+int g3m3_run_finish(void* run) {
+    Run* R = (Run*)run;
+    read_tstepnew(R, R->stepped);
+    R->stepped.clear();
+    check(R, "finish");
     return R->err;
 }
 
@@ -629,6 +644,8 @@ int g3m3_run_sync_arrays(void* run) {
         launch_swap_fields(R->n, NF, pa, po, R->s);
         R->rot[box - 1] = 0;
     }
+    // the caller reads or writes the arrays next, possibly on another stream
+    hipStreamSynchronize(R->s);
     check(R, "sync_arrays");
     return R->err;
 }

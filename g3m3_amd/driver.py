@@ -323,7 +323,7 @@ class FastRun(Run):
         L.g3m3_run_create.restype = _ct.c_void_p
         L.g3m3_run_create.argtypes = [_ct.POINTER(RunDesc)]
         for nm in ("g3m3_run_init_dt", "g3m3_run_full_step", "g3m3_run_box_step",
-                   "g3m3_run_sync_arrays"):
+                   "g3m3_run_sync_arrays", "g3m3_run_full_step_async", "g3m3_run_finish"):
             getattr(L, nm).restype = _ct.c_int
         d = RunDesc()
         d.nx, d.ny, d.nz, d.n_grids = self.nx, self.ny, self.nz, self.ngrd
@@ -390,6 +390,24 @@ class FastRun(Run):
             raise _lib.G3M3Error("g3m3_run_full_step failed")
         self._pull()
 
+    def set_stream(self, stream):
+        """Issue this run's kernels on `stream` (a torch.cuda.Stream)."""
+        self._L.g3m3_run_set_stream(_ct.c_void_p(self.h), _ct.c_void_p(stream.cuda_stream))
+        self._stream = stream
+
+    def full_step_async(self, dt_override=None):
+        """Issue a full step's kernels and return (finish() completes it)."""
+        arg = None
+        if dt_override is not None:
+            arg = (_ct.c_double * self.ngrd)(*[float(x) for x in dt_override])
+        if self._L.g3m3_run_full_step_async(_ct.c_void_p(self.h), arg):
+            raise _lib.G3M3Error("g3m3_run_full_step_async failed")
+
+    def finish(self):
+        if self._L.g3m3_run_finish(_ct.c_void_p(self.h)):
+            raise _lib.G3M3Error("g3m3_run_finish failed")
+        self._pull()
+
     def sync_arrays(self):
         """Put every box's state back in the state tensors and its old state
         in the old tensors (the runtime swaps the two per box-step)."""
@@ -403,3 +421,35 @@ class FastRun(Run):
     def state_host(self, prefix=""):
         self.sync_arrays()
         return super().state_host(prefix)
+
+
+# -----------------------------------------------------------------------------
+# Many independent runs advanced together: each run's full step is issued on
+# one of a few HIP streams, then all are finished -- small boxes that leave
+# most of the chip idle one at a time overlap on the GPU.
+
+
+class Batch:
+    """Independent runs (any sizes) stepped together on `streams` streams."""
+
+    def __init__(self, workloads, streams: int = 4):
+        torch = _torch()
+        self.runs = [FastRun(wl) for wl in workloads]
+        torch.cuda.synchronize()   # uploads done before other streams touch the arrays
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
+        for i, r in enumerate(self.runs):
+            r.set_stream(self.streams[i % len(self.streams)])
+
+    def init_dt(self):
+        for r in self.runs:
+            r.init_dt()
+
+    def full_step(self):
+        for r in self.runs:
+            r.full_step_async()
+        for r in self.runs:
+            r.finish()
+
+    @property
+    def cell_updates(self) -> int:
+        return sum(r.cell_updates for r in self.runs)

@@ -223,3 +223,24 @@ def precision_field(L: int = 120, seed: int = 20261205, r_inner: float = 3.0):
         noisy = c + F(0.01) * mag * rng.standard_normal(c.shape).astype(F)
         out.append(np.ascontiguousarray(noisy.astype(F)[None]))
     return out
+
+
+def make_batch_workloads(n: int = 32, seed: int = 20261204):
+    """Independent single-grid runs for batching: box sizes 41x41x21 ..
+    121x121x61 (L uniform in 20..60), upstream density uniform in [1, 20]
+    cm^-3, speed in [300, 800] km/s, IMF direction uniform on the sphere
+    with magnitude in [2, 10] nT -- all drawn from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        L = int(rng.integers(20, 61))
+        n_sw = float(rng.uniform(1.0, 20.0))
+        v_sw = float(rng.uniform(300.0, 800.0))
+        d = rng.standard_normal(3)
+        d = d / np.linalg.norm(d)
+        bmag = float(rng.uniform(2.0, 10.0))
+        wl = make_workload("single61", seed=seed + 1 + i, L=L, n_sw=n_sw, v_sw=v_sw,
+                           imf=tuple(float(x) for x in bmag * d))
+        wl.name = f"batch{i:02d}_L{L}"
+        out.append(wl)
+    return out

@@ -401,8 +401,8 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
  * copying the state into old, so between calls a box's state may live in
  * the old array.  g3m3_run_sync_arrays() puts every box's state back into
  * the state array and its old state into the old array (the reference's
- * arrays, bit for bit); call it before reading or changing the arrays
- * outside the runtime. */
+ * arrays, bit for bit) and waits for the run's stream; call it before
+ * reading or changing the arrays outside the runtime. */
 #define G3M3_MAX_GRIDS 8
 typedef struct {
     int nx, ny, nz, n_grids;
@@ -438,6 +438,12 @@ void g3m3_run_destroy(void* run);
 void g3m3_run_set_stream(void* run, void* hip_stream);
 int g3m3_run_init_dt(void* run);                     /* code/3d_main.f90:1861-1903 */
 int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt */
+/* g3m3_run_full_step in two halves, so that many runs on different streams
+ * can be in flight together: _async issues the full step's kernels on the
+ * run's stream and returns; _finish waits for them and reads the CFL maxima
+ * that set the next step (the one host synchronisation of a full step). */
+int g3m3_run_full_step_async(void* run, const double* dt_override);
+int g3m3_run_finish(void* run);
 int g3m3_run_box_step(void* run, int box);
 int g3m3_run_sync_arrays(void* run);
 void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
