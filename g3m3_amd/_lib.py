@@ -65,6 +65,9 @@ class ByValue:
 def marshal(a, keep):
     if isinstance(a, ByValue):
         return a.c
+    if isinstance(a, bytes):    # a C string (file paths)
+        keep.append(a)
+        return ctypes.c_char_p(a)
     if _is_tensor(a):
         if not a.is_cuda:
             raise G3M3Error("device arrays must be CUDA/HIP tensors")

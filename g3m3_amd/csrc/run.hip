@@ -650,6 +650,8 @@ int g3m3_run_sync_arrays(void* run) {
     return R->err;
 }
 
+void g3m3_run_set_time(void* run, double t) { ((Run*)run)->t = t; }
+
 int g3m3_run_box_step(void* run, int box) {
     Run* R = (Run*)run;
     box_step(R, box);

@@ -314,6 +314,14 @@ class RunDesc(_ct.Structure):
                 + [("isotropic", _ct.c_int)])
 
 
+# the reference's fluid-file array order (code/3d_main.f90:3312-3348)
+FLUID_ORDER = [sp + f for sp in "qho" for f in ("rho", "px", "py", "pz", "presx", "presy",
+                                                   "presz", "presxy", "presxz", "presyz")] + \
+    ["bx", "by", "bz", "epres", "bx0", "by0", "bz0"]
+_LIST_KEYS = ["parm_srf", "parm_mid", "parm_zero", "ijsrf", "ijmid", "ijzero", "numsrf",
+              "nummid", "numzero"]
+
+
 class FastRun(Run):
     """A run advanced by the native runtime (one C-ABI call per full step)."""
 
@@ -417,6 +425,72 @@ class FastRun(Run):
     def divb(self, box, errlim, mode=0, max_iters=0):
         self.sync_arrays()
         return super().divb(box, errlim, mode, max_iters)
+
+    # -- fluid (restart) files in the reference's format -------------------------
+    @staticmethod
+    def _host(a):
+        return a.cpu().numpy() if hasattr(a, "cpu") else a
+
+    def _fluid_arrays(self):
+        return [self.a[k] for k in FLUID_ORDER]
+
+    def _list_caps(self):
+        return tuple(int(self.a["ij" + nm].shape[0]) for nm in ("srf", "mid", "zero"))
+
+    def write_fluid(self, path):
+        """Write the run (t, the 37 arrays, the planet lists) as a fluid file
+        the reference reads at restart (code/3d_main.f90:972-1011)."""
+        self.sync_arrays()
+        host = [t.cpu().numpy() for t in self._fluid_arrays()]
+        lists = {k: np.ascontiguousarray(self._host(self.a[k])) for k in _LIST_KEYS}
+        msrf, mmid, mzero = self._list_caps()
+        rc = _lib.call("fluid_write", path.encode() if isinstance(path, str) else path,
+                       _lib.F64(self.t), self.nx, self.ny, self.nz, self.ngrd,
+                       (_ct.c_void_p * len(host))(*[h.ctypes.data for h in host]),
+                       int(self.s["mbndry"]), msrf, mmid, mzero,
+                       lists["parm_srf"], lists["parm_mid"], lists["parm_zero"],
+                       lists["ijsrf"], lists["ijmid"], lists["ijzero"],
+                       lists["numsrf"], lists["nummid"], lists["numzero"])
+        if rc:
+            raise _lib.G3M3Error(f"writing {path} failed ({rc})")
+
+    def read_fluid(self, path):
+        """Restart from a fluid file: t, the 37 arrays and the planet lists
+        replace the run's; old and wrk are zeroed, as in a fresh reference
+        process, and the first time step is found again (init_dt)."""
+        torch = _torch()
+        self.sync_arrays()
+        host = [np.empty(tuple(t.shape), np.float32) for t in self._fluid_arrays()]
+        lists = {k: np.empty(tuple(self.a[k].shape), self._host(self.a[k]).dtype)
+                 for k in _LIST_KEYS}
+        msrf, mmid, mzero = self._list_caps()
+        t = _ct.c_double(0.0)
+        rc = _lib.call("fluid_read", path.encode() if isinstance(path, str) else path, t,
+                       self.nx, self.ny, self.nz, self.ngrd,
+                       (_ct.c_void_p * len(host))(*[h.ctypes.data for h in host]),
+                       int(self.s["mbndry"]), msrf, mmid, mzero,
+                       lists["parm_srf"], lists["parm_mid"], lists["parm_zero"],
+                       lists["ijsrf"], lists["ijmid"], lists["ijzero"],
+                       lists["numsrf"], lists["nummid"], lists["numzero"])
+        if rc:
+            raise _lib.G3M3Error(f"reading {path} failed ({rc})")
+        for k in ("numsrf", "nummid", "numzero"):
+            if not np.array_equal(lists[k], self._host(self.a[k])):
+                raise _lib.G3M3Error(f"{path}: planet list {k} differs from the run's; "
+                                     "create the run from the file's lists")
+        for dev, h in zip(self._fluid_arrays(), host):
+            dev.copy_(torch.from_numpy(h))
+        for k in _LIST_KEYS:
+            if hasattr(self.a[k], "copy_"):
+                self.a[k].copy_(torch.from_numpy(lists[k]))
+            else:
+                self.a[k][...] = lists[k]
+        for k in FIELDS:
+            self.a["old" + k].zero_()
+            self.a["wrk" + k].zero_()
+        torch.cuda.synchronize()
+        self._L.g3m3_run_set_time(_ct.c_void_p(self.h), _ct.c_double(t.value))
+        self.init_dt()
 
     def state_host(self, prefix=""):
         self.sync_arrays()

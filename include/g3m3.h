@@ -385,6 +385,29 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
                         const float* re_equiv, const int* nx, const int* ny, const int* nz,
                         const int* nq, const float* const* fields, float* scdata, int* gridpt);
 
+/* ---- fluid (restart) files ------------------------------------------------
+ * The reference's restart format (written at code/3d_main.f90:3310-3349,
+ * read at :972-1011): Fortran unformatted sequential records with 4-byte
+ * byte counts -- t (double), the 37 arrays q(nx,ny,nz,n_grids) in this
+ * order: per species q, h, o: rho, px, py, pz, presx, presy, presz,
+ * presxy, presxz, presyz; then bx, by, bz, epres, bx0, by0, bz0 -- and one
+ * record of the planet lists (parm_srf(mbndry,7,msrf), parm_mid, parm_zero,
+ * ijzero(mbndry,3,mzero), numzero(mbndry), ijmid, nummid, ijsrf, numsrf),
+ * msrf/mmid/mzero being the lists' declared sizes.  Host arrays; returns
+ * 0, 1 (cannot open) or 2 (write failed / record sizes do not match). */
+#define G3M3_FLUID_ARRAYS 37
+int g3m3_fluid_write(const char* path, const double* t, const int* nx, const int* ny,
+                     const int* nz, const int* n_grids, const float* const* fields,
+                     const int* mbndry, const int* msrf, const int* mmid, const int* mzero,
+                     const float* parm_srf, const float* parm_mid, const float* parm_zero,
+                     const int* ijsrf, const int* ijmid, const int* ijzero, const int* numsrf,
+                     const int* nummid, const int* numzero);
+int g3m3_fluid_read(const char* path, double* t, const int* nx, const int* ny, const int* nz,
+                    const int* n_grids, float* const* fields, const int* mbndry, const int* msrf,
+                    const int* mmid, const int* mzero, float* parm_srf, float* parm_mid,
+                    float* parm_zero, int* ijsrf, int* ijmid, int* ijzero, int* numsrf,
+                    int* nummid, int* numzero);
+
 /* ---- native runtime ----------------------------------------------------------
  * No reference counterpart: the whole run resident in HBM, advanced a full
  * step (code/3d_main.f90:1925-3095) per call with fused kernels.  Results
@@ -445,6 +468,7 @@ int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt
 int g3m3_run_full_step_async(void* run, const double* dt_override);
 int g3m3_run_finish(void* run);
 int g3m3_run_box_step(void* run, int box);
+void g3m3_run_set_time(void* run, double t);          /* after a restart read */
 int g3m3_run_sync_arrays(void* run);
 void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
                    double* t_new, long long* cell_updates, long long* box_steps);
