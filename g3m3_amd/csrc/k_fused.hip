@@ -673,6 +673,19 @@ __device__ inline float fct_flux_ax(const WA& W, const PA& P, int base, int m, i
     return fct_lim(dq, d1m, d1p, chi);
 }
 
+// fct_flux_ax without branches: the end shift and the d1p choice are
+// arithmetic and selects, all six reads always made (each index stays in
+// the tile + halo for every lane), so lanes never diverge
+template <class WA, class PA>
+__device__ inline float fct_flux_bf(const WA& W, const PA& P, int base, int m, int n, float chi) {
+    const int sh = (m == 0) ? 1 : ((m == n - 1) ? -1 : 0);
+    const int c = base + sh;
+    const float w0 = W(c), w1 = W(c + 1);
+    const float pm1 = P(c - 1), p0 = P(c), p1 = P(c + 1), p2 = P(c + 2);
+    const float d1p = (m + sh + 1 == n - 1) ? (p1 - p0) : (p2 - p1);
+    return fct_lim(w1 - w0, p0 - pm1, d1p, chi);
+}
+
 __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
     __shared__ FctLds L;
     const Box& B = F.B;
@@ -707,8 +720,10 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
             const int e = tid + 256 * m;
             const int ii = e % FPX, jj = e / FPX;
             const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-            rw[m] = (e < FP_N && i >= 0 && i < nx && j >= 0 && j < ny)
-                        ? wg[(size_t)k * plane + j * nx + i] : 0.0f;
+            const bool ok = e < FP_N && i >= 0 && i < nx && j >= 0 && j < ny;
+            const float v = wg[(size_t)k * plane + min(max(j, 0), ny - 1) * nx +
+                               min(max(i, 0), nx - 1)];
+            rw[m] = ok ? v : 0.0f;
         }
     };
     auto store_w = [&](float* dst) {
@@ -724,8 +739,10 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
             const int e = tid + 256 * m;
             const int ii = e % FOX, jj = e / FOX;
             const int i = i0 - 3 + ii, j = j0 - 3 + jj;
-            ro[m] = (e < FO_N && i >= 0 && i < nx && j >= 0 && j < ny)
-                        ? og[(size_t)k * plane + j * nx + i] : 0.0f;
+            const bool ok = e < FO_N && i >= 0 && i < nx && j >= 0 && j < ny;
+            const float v = og[(size_t)k * plane + min(max(j, 0), ny - 1) * nx +
+                               min(max(i, 0), nx - 1)];
+            ro[m] = ok ? v : 0.0f;
         }
     };
     auto store_o = [&](float* dst) {
@@ -749,18 +766,13 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
             if (e >= FP_N) break;
             const int ii = e % FPX, jj = e / FPX;
             const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-            float v = 0.0f;
-            if (i >= 0 && i < nx && j >= 0 && j < ny) {
-                const float wc = w[e];
-                if (kface || i == 0 || i == nx - 1 || j == 0 || j == ny - 1) {
-                    v = wc;
-                } else {
-                    const int oc0 = (jj + 1) * FOX + ii + 1;
-                    v = wc + chi * (oc[oc0 + 1] + oc[oc0 - 1] + oc[oc0 + FOX] + oc[oc0 - FOX] +
-                                    op[oc0] + om[oc0] - 6.0f * oc[oc0]);
-                }
-            }
-            dst[e] = v;
+            const bool inbox = i >= 0 && i < nx && j >= 0 && j < ny;
+            const bool face = kface || i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            const float wc = w[e];
+            const int oc0 = (jj + 1) * FOX + ii + 1;
+            const float vi = wc + chi * (oc[oc0 + 1] + oc[oc0 - 1] + oc[oc0 + FOX] +
+                                         oc[oc0 - FOX] + op[oc0] + om[oc0] - 6.0f * oc[oc0]);
+            dst[e] = inbox ? (face ? wc : vi) : 0.0f;
         }
     };
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -778,19 +790,13 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
             return;
         }
         // x: each lane's face flux, the lower one from the lane to the left
-        // (the tile's first lane computes its own)
-        float fx = 0.0f;
-        if (i < nx) {
-            auto Wx = [&](int o) { return wk[pc + o]; };
-            auto Px = [&](int o) { return pk[pc + o]; };
-            fx = fct_flux_ax(Wx, Px, 0, min(i, nx - 1), nx, chi);
-        }
-        float fxm = __shfl_up(fx, 1, 64);
-        if (tx == 0 && i > 0 && i < nx) {
-            auto Wx = [&](int o) { return wk[pc + o]; };
-            auto Px = [&](int o) { return pk[pc + o]; };
-            fxm = fct_flux_ax(Wx, Px, -1, i - 1, nx, chi);
-        }
+        // (the tile's first lane uses its own evaluation of it)
+        auto Wx = [&](int o) { return wk[pc + o]; };
+        auto Px = [&](int o) { return pk[pc + o]; };
+        const float fx = fct_flux_bf(Wx, Px, 0, i, nx, chi);
+        const float fxo = fct_flux_bf(Wx, Px, -1, i - 1, nx, chi);
+        const float fxl = __shfl_up(fx, 1, 64);
+        const float fxm = (tx == 0) ? fxo : fxl;
         // z: this cell's column, P(k-2..k+2) and wrk(k-1..k+1), in registers;
         // the flux below is the previous plane's flux above when it was made
         const float Wm1 = wslot(k + 3)[pc], W0 = wk[pc], W1 = wslot(k + 1)[pc];
@@ -808,18 +814,13 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
         }
         fz_prev = fz;
         fz_plane = k;
-        if (!mine) return;
-        float tv;
-        if (i > 0 && i < nx - 1) tv = pk[pc] - fx + fxm;
-        else tv = wk[pc];
-        if (j > 0 && j < ny - 1) {
-            auto Wy = [&](int o) { return wk[pc + o * FPX]; };
-            auto Py = [&](int o) { return pk[pc + o * FPX]; };
-            const float fy = fct_flux_ax(Wy, Py, 0, j, ny, chi);
-            const float fym = fct_flux_ax(Wy, Py, -1, j - 1, ny, chi);
-            tv = tv - fy + fym;
-        }
-        out[c] = tv - fz + fzm;
+        auto Wy = [&](int o) { return wk[pc + o * FPX]; };
+        auto Py = [&](int o) { return pk[pc + o * FPX]; };
+        const float fy = fct_flux_bf(Wy, Py, 0, j, ny, chi);
+        const float fym = fct_flux_bf(Wy, Py, -1, j - 1, ny, chi);
+        const float tx0 = (i > 0 && i < nx - 1) ? (pk[pc] - fx + fxm) : wk[pc];
+        const float ty0 = (j > 0 && j < ny - 1) ? (tx0 - fy + fym) : tx0;
+        if (mine) out[c] = ty0 - fz + fzm;
     };
     // P planes pstart..pend are made in order; stage s stores wrk(s) and
     // old(s+1) (fetched during stage s-1), makes P(s) and outputs plane
