@@ -12,6 +12,7 @@
 
 #include "../../include/g3m3.h"
 #include "g3m3_craft.h"
+#include "g3m3_fused.h"
 #include "g3m3_launch.h"
 
 using namespace g3m3;
@@ -75,7 +76,7 @@ struct Buf {
         return p;
     }
 };
-Buf g_fct, g_speed_list, g_small, g_divb_f, g_divb_d, g_tmp;
+Buf g_speed_list, g_small, g_divb_f, g_divb_d, g_tmp;
 
 inline Box box_of(const int* nx, const int* ny, const int* nz) { return Box{*nx, *ny, *nz}; }
 inline size_t off(const Box& B, const int* box) { return (size_t)(*box - 1) * B.V(); }
@@ -554,16 +555,15 @@ void g3m3_fnd_vtot(const float* qpx, const float* qpy, const float* qpz, const f
 
 static void fct_impl(const Box& B, float* const* out, const float* const* old,
                      const float* const* wrk, const float* chi, int nf, const int* box) {
-    FctPtrs F{};
+    FctParams F{};
+    F.B = B;
     for (int f = 0; f < nf; ++f) {
         F.out[f] = S(out[f], B, box);
         F.old[f] = S(old[f], B, box);
         F.wrk[f] = S(wrk[f], B, box);
         F.chi[f] = chi[f];
     }
-    float* scratch = (float*)g_fct.get(fct_scratch_floats(B.V(), nf) * sizeof(float));
-    if (!scratch) return;
-    launch_fct(B, F, nf, scratch, g_stream);
+    launch_fct_fused(F, nf, g_stream);
     check_launch("fct");
 }
 

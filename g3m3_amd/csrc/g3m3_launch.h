@@ -35,13 +35,6 @@ struct LapPtrs {
     float chi[10];
 };
 
-struct FctPtrs {
-    float* out[kMaxFields];
-    const float* old[kMaxFields];
-    const float* wrk[kMaxFields];
-    float chi[kMaxFields];
-};
-
 enum OuterKind { kRHO = 0, kPRES, kPX, kPY, kPZ, kOFF, kEP, kBX, kBY, kBZ };
 struct OuterPtrs {
     float* f[kMaxFields];
@@ -125,8 +118,6 @@ void launch_velocity(int n, const float* r, const float* p[3], float* v[3], floa
 void launch_vtot(int n, const float* mom[9], const float* rho[3], float* v[3], hipStream_t s);
 void launch_lapidus(const Box& B, const LapPtrs& L, int nf, const float* v[3], float delt,
                     hipStream_t s);
-void launch_fct(const Box& B, const FctPtrs& F, int nf, float* scratch, hipStream_t s);
-size_t fct_scratch_floats(int ncells, int nf);
 void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s);
 void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
                         const int num[3], int mbndry, int box, float xmin, float ymin, float dx,

@@ -626,25 +626,35 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 }
 
 // ---------------------------------------------------------------------------
-// FCT in one pass (k_fct_fused): a block owns a 64 x 4 (i, j) tile of one
-// field and a run of up to 16 k-planes, and marches it along k.  LDS rings
-// hold the planes the stencils need -- old with a 3-cell (i, j) halo (3
-// planes), wrk and the diffused field P with a 2-cell halo (4 and 5
-// planes) -- so P never leaves the CU and each input value comes from HBM
-// about once.  The global loads of the next plane are issued into
-// registers before the current plane is processed.  The arithmetic is
-// k_fct_diffuse2 + k_fct_limit's, expression for expression.
+// FCT in one pass (k_fct_fused): a block owns a 63 x 8 (i, j) tile of one
+// field and a run of up to 16 k-planes, and marches it along k.  Rings of
+// three LDS planes hold old (3-cell halo), wrk and the diffused field P
+// (2-cell halo), so P never leaves the CU and each input value comes from
+// HBM about once.  A wave covers two tile rows: lane tx owns the cell
+// i0 - 1 + tx of rows j0 + 2ty and j0 + 2ty + 1.  Lane 0 outputs nothing;
+// it supplies the flux at the tile's lower x face, so each x face flux is
+// evaluated once and handed to the lane above by a shuffle.  Each thread
+// keeps its own column of P (5 planes) and wrk (3 planes) in registers for
+// the z fluxes, and carries the z face flux from one plane to the next.
+// The arithmetic is the reference's (code/sub_fcsmooth.f90:44-239),
+// expression for expression.
+//
+// Values outside the box: tile loads are clamped into the box, so halo
+// cells beyond it hold copies of box cells.  They reach only lanes that
+// store nothing, or operands that a select discards (the limiter's far
+// difference at the last interior cell), so every stored value equals the
+// reference's.
 
-constexpr int FTX = 64, FTY = 4, FKC = 16;
-constexpr int FOX = FTX + 6, FOY = FTY + 6;    // old plane with halo 3
-constexpr int FPX = FTX + 4, FPY = FTY + 4;    // wrk / P planes with halo 2
+constexpr int FTX = 63, FTY = 8, FKC = 16;
+constexpr int FPX = FTX + 5, FPY = FTY + 4;   // wrk / P: cells i0-3 .. i0+64, j0-2 .. j0+9
+constexpr int FOX = FPX + 2, FOY = FPY + 2;   // old: one more cell on each side
 constexpr int FO_N = FOX * FOY, FP_N = FPX * FPY;
 constexpr int FO_PT = (FO_N + 255) / 256, FP_PT = (FP_N + 255) / 256;
 
 struct FctLds {
-    float o[3][FOY][FOX];
-    float w[4][FPY][FPX];
-    float p[5][FPY][FPX];
+    float o[3][FO_N];
+    float w[3][FP_N];
+    float p[3][FP_N];
 };
 
 // the limiter of code/sub_fcsmooth.f90 on one face: dq = wrk difference,
@@ -655,29 +665,13 @@ __device__ inline float fct_lim(float dq, float d1m, float d1p, float chi) {
 }
 
 // limited antidiffusive flux of one axis at the face above the cell `base`
-// cells along that axis from the output cell (m = its index along the
-// axis, n the extent); W(t) / P(t) read wrk / P t cells along the axis
+// cells along that axis from the output cell (m = the index along the axis
+// of that cell, n the extent); W(t) / P(t) read wrk / P t cells along the
+// axis.  At the two ends the reference shifts the stencil one cell inwards
+// (code/sub_fcsmooth.f90:87-239); the shift and the choice of the far
+// difference are selects, so lanes never diverge.
 template <class WA, class PA>
-__device__ inline float fct_flux_ax(const WA& W, const PA& P, int base, int m, int n, float chi) {
-    int c = base;
-    if (m == 0) {
-        c += 1;
-        m = 1;
-    } else if (m == n - 1) {
-        c -= 1;
-        m = n - 2;
-    }
-    const float dq = W(c + 1) - W(c);
-    const float d1m = P(c) - P(c - 1);
-    const float d1p = (m + 1 == n - 1) ? (P(c + 1) - P(c)) : (P(c + 2) - P(c + 1));
-    return fct_lim(dq, d1m, d1p, chi);
-}
-
-// fct_flux_ax without branches: the end shift and the d1p choice are
-// arithmetic and selects, all six reads always made (each index stays in
-// the tile + halo for every lane), so lanes never diverge
-template <class WA, class PA>
-__device__ inline float fct_flux_bf(const WA& W, const PA& P, int base, int m, int n, float chi) {
+__device__ inline float fct_flux(const WA& W, const PA& P, int base, int m, int n, float chi) {
     const int sh = (m == 0) ? 1 : ((m == n - 1) ? -1 : 0);
     const int c = base + sh;
     const float w0 = W(c), w1 = W(c + 1);
@@ -700,155 +694,209 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     const int t = r % (ntx * nty);
     const int i0 = (t % ntx) * FTX, j0 = (t / ntx) * FTY;
     const int kb = kc * FKC, ke = min(nz, kb + FKC);
-    const float* __restrict__ wg = F.wrk[f];
-    const float* __restrict__ og = F.old[f];
+    const char* __restrict__ wg = reinterpret_cast<const char*>(F.wrk[f]);
+    const char* __restrict__ og = reinterpret_cast<const char*>(F.old[f]);
     float* out = F.out[f];
     const float chi = F.chi[f];
-    const int tid = threadIdx.x + FTX * threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = tx + 64 * ty;
     const int plane = nx * ny;
+    const size_t plane_bytes = (size_t)plane * sizeof(float);
 
-    // plane rings: old(k) in L.o[k % 3], wrk(k) in L.w[k % 4], P(k) in
-    // L.p[k % 5]; slot pointers are formed once per stage
-    auto oslot = [&](int k) { return &L.o[k % 3][0][0]; };
-    auto wslot = [&](int k) { return &L.w[k & 3][0][0]; };
-    auto pslot = [&](int k) { return &L.p[k % 5][0][0]; };
-
+    // this thread's tile elements, fixed for the whole march: clamped
+    // in-plane byte offsets of the loads, the P element's centre in the old
+    // tile, and whether the P element lies on an x or y face of the box
+    unsigned offw[FP_PT], offo[FO_PT];
+    int ocw[FP_PT];
+    int xyface = 0;
+#pragma unroll
+    for (int m = 0; m < FP_PT; ++m) {
+        const int e = tid + 256 * m;
+        const int ii = e % FPX, jj = e / FPX;
+        const int gi = i0 - 3 + ii, gj = j0 - 2 + jj;
+        offw[m] = (unsigned)(min(max(gj, 0), ny - 1) * nx + min(max(gi, 0), nx - 1)) * 4u;
+        ocw[m] = (jj + 1) * FOX + ii + 1;
+        if (gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1) xyface |= 1 << m;
+    }
+#pragma unroll
+    for (int m = 0; m < FO_PT; ++m) {
+        const int e = tid + 256 * m;
+        const int ii = e % FOX, jj = e / FOX;
+        const int gi = i0 - 4 + ii, gj = j0 - 3 + jj;
+        offo[m] = (unsigned)(min(max(gj, 0), ny - 1) * nx + min(max(gi, 0), nx - 1)) * 4u;
+    }
+    auto ld = [](const char* base, unsigned off) {
+        return *reinterpret_cast<const float*>(base + off);
+    };
     float rw[FP_PT], ro[FO_PT];
     auto fetch_w = [&](int k) {
+        const char* b = wg + (size_t)k * plane_bytes;
 #pragma unroll
-        for (int m = 0; m < FP_PT; ++m) {
-            const int e = tid + 256 * m;
-            const int ii = e % FPX, jj = e / FPX;
-            const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-            const bool ok = e < FP_N && i >= 0 && i < nx && j >= 0 && j < ny;
-            const float v = wg[(size_t)k * plane + min(max(j, 0), ny - 1) * nx +
-                               min(max(i, 0), nx - 1)];
-            rw[m] = ok ? v : 0.0f;
-        }
+        for (int m = 0; m < FP_PT; ++m) rw[m] = ld(b, offw[m]);
+    };
+    auto fetch_o = [&](int k) {
+        const char* b = og + (size_t)k * plane_bytes;
+#pragma unroll
+        for (int m = 0; m < FO_PT; ++m) ro[m] = ld(b, offo[m]);
     };
     auto store_w = [&](float* dst) {
 #pragma unroll
-        for (int m = 0; m < FP_PT; ++m) {
-            const int e = tid + 256 * m;
-            if (e < FP_N) dst[e] = rw[m];
-        }
-    };
-    auto fetch_o = [&](int k) {
-#pragma unroll
-        for (int m = 0; m < FO_PT; ++m) {
-            const int e = tid + 256 * m;
-            const int ii = e % FOX, jj = e / FOX;
-            const int i = i0 - 3 + ii, j = j0 - 3 + jj;
-            const bool ok = e < FO_N && i >= 0 && i < nx && j >= 0 && j < ny;
-            const float v = og[(size_t)k * plane + min(max(j, 0), ny - 1) * nx +
-                               min(max(i, 0), nx - 1)];
-            ro[m] = ok ? v : 0.0f;
-        }
+        for (int m = 0; m < FP_PT; ++m)
+            if (m < FP_PT - 1 || tid + 256 * m < FP_N) dst[tid + 256 * m] = rw[m];
     };
     auto store_o = [&](float* dst) {
 #pragma unroll
-        for (int m = 0; m < FO_PT; ++m) {
-            const int e = tid + 256 * m;
-            if (e < FO_N) dst[e] = ro[m];
-        }
+        for (int m = 0; m < FO_PT; ++m)
+            if (m < FO_PT - 1 || tid + 256 * m < FO_N) dst[tid + 256 * m] = ro[m];
     };
-    // P of plane k on the tile + 2-cell halo (code/sub_fcsmooth.f90:44-83)
-    auto make_p = [&](int k) {
-        float* dst = pslot(k);
-        const float* w = wslot(k);
-        const float* om = oslot(k + 2);   // plane k - 1
-        const float* oc = oslot(k);
-        const float* op = oslot(k + 1);
-        const bool kface = (k == 0 || k == nz - 1);
-#pragma unroll
-        for (int m = 0; m < FP_PT; ++m) {
-            const int e = tid + 256 * m;
-            if (e >= FP_N) break;
-            const int ii = e % FPX, jj = e / FPX;
-            const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-            const bool inbox = i >= 0 && i < nx && j >= 0 && j < ny;
-            const bool face = kface || i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
-            const float wc = w[e];
-            const int oc0 = (jj + 1) * FOX + ii + 1;
-            const float vi = wc + chi * (oc[oc0 + 1] + oc[oc0 - 1] + oc[oc0 + FOX] +
-                                         oc[oc0 - FOX] + op[oc0] + om[oc0] - 6.0f * oc[oc0]);
-            dst[e] = inbox ? (face ? wc : vi) : 0.0f;
-        }
-    };
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = i0 + tx, j = j0 + ty;
-    const int pc = (ty + 2) * FPX + tx + 2;   // this thread's cell in a P / wrk plane
-    float fz_prev = 0.0f;   // flux at the face below the plane being output
-    int fz_plane = -1;      // the plane whose upper-face flux fz_prev is
-    auto output = [&](int k) {
-        const float* wk = wslot(k);
-        const float* pk = pslot(k);
-        const bool mine = i < nx && j < ny;
-        const size_t c = (size_t)k * plane + (size_t)j * nx + i;
-        if (k == 0 || k == nz - 1) {     // z faces keep w
-            if (mine) out[c] = wk[pc];
+
+    // the cells this thread outputs, and their place in a P / wrk plane
+    const int i = i0 - 1 + tx, ja = j0 + 2 * ty, jb = ja + 1;
+    const int pa = (2 * ty + 2) * FPX + tx + 2, pb = pa + FPX;
+    const bool xin = tx >= 1 && i < nx;
+    const bool oka = xin && ja < ny, okb = xin && jb < ny;
+    const bool xint = i > 0 && i < nx - 1;
+
+    // own columns: after stage s, Pc*[4] = P(s), ..., Pc*[0] = P(s-4);
+    // Wc*[2] = wrk(s-1), Wc*[1] = wrk(s-2), Wc*[0] = wrk(s-3)
+    float Pca[5] = {0, 0, 0, 0, 0}, Pcb[5] = {0, 0, 0, 0, 0};
+    float Wca[3] = {0, 0, 0}, Wcb[3] = {0, 0, 0};
+    float fza_prev = 0.0f, fzb_prev = 0.0f;   // flux above the last plane output
+    int fz_plane = -1;
+
+    // output plane k from the slot of its wrk / P plane; columns hold
+    // planes k-2 .. k+2 (P) and k-1 .. k+1 (wrk)
+    auto output = [&](int k, const float* wk, const float* pk) {
+        const size_t cpl = (size_t)k * plane + i;
+        if (k == 0 || k == nz - 1) {    // z faces keep w
+            if (oka) out[cpl + (size_t)ja * nx] = Wca[1];
+            if (okb) out[cpl + (size_t)jb * nx] = Wcb[1];
             return;
         }
-        // x: each lane's face flux, the lower one from the lane to the left
-        // (the tile's first lane uses its own evaluation of it)
-        auto Wx = [&](int o) { return wk[pc + o]; };
-        auto Px = [&](int o) { return pk[pc + o]; };
-        const float fx = fct_flux_bf(Wx, Px, 0, i, nx, chi);
-        const float fxo = fct_flux_bf(Wx, Px, -1, i - 1, nx, chi);
-        const float fxl = __shfl_up(fx, 1, 64);
-        const float fxm = (tx == 0) ? fxo : fxl;
-        // z: this cell's column, P(k-2..k+2) and wrk(k-1..k+1), in registers;
-        // the flux below is the previous plane's flux above when it was made
-        const float Wm1 = wslot(k + 3)[pc], W0 = wk[pc], W1 = wslot(k + 1)[pc];
-        const float Pm2 = pslot(k + 3)[pc], Pm1 = pslot(k + 4)[pc], P0 = pk[pc];
-        const float P1 = pslot(k + 1)[pc], P2 = pslot(k + 2)[pc];   // P2 unused at k = nz-2
-        // face above (fct_flux_ax, base 0, m = k: no end shift for 1 <= k <= nz-2)
-        const float fz = fct_lim(W1 - W0, P0 - Pm1, (k + 1 == nz - 1) ? (P1 - P0) : (P2 - P1), chi);
-        float fzm;
+        // x: the flux at the face above this lane's cell; the face below
+        // comes from the lane below
+        auto Wxa = [&](int o) { return wk[pa + o]; };
+        auto Pxa = [&](int o) { return pk[pa + o]; };
+        auto Wxb = [&](int o) { return wk[pb + o]; };
+        auto Pxb = [&](int o) { return pk[pb + o]; };
+        const float fxa = fct_flux(Wxa, Pxa, 0, i, nx, chi);
+        const float fxb = fct_flux(Wxb, Pxb, 0, i, nx, chi);
+        const float fxma = __shfl_up(fxa, 1, 64);
+        const float fxmb = __shfl_up(fxb, 1, 64);
+        // y: the faces below row a, between the rows and above row b
+        auto Wy = [&](int o) { return wk[pa + o * FPX]; };
+        auto Py = [&](int o) { return pk[pa + o * FPX]; };
+        const float fy0 = fct_flux(Wy, Py, -1, ja - 1, ny, chi);
+        const float fy1 = fct_flux(Wy, Py, 0, ja, ny, chi);
+        const float fy2 = fct_flux(Wy, Py, 1, jb, ny, chi);
+        // z, from the columns: the face above (no end shift for
+        // 1 <= k <= nz-2) and the face below
+        const bool top = (k + 1 == nz - 1);
+        const float fza = fct_lim(Wca[2] - Wca[1], Pca[2] - Pca[1],
+                                  top ? (Pca[3] - Pca[2]) : (Pca[4] - Pca[3]), chi);
+        const float fzb = fct_lim(Wcb[2] - Wcb[1], Pcb[2] - Pcb[1],
+                                  top ? (Pcb[3] - Pcb[2]) : (Pcb[4] - Pcb[3]), chi);
+        float fzma, fzmb;
         if (fz_plane == k - 1) {
-            fzm = fz_prev;
-        } else if (k == 1) {   // base -1, m = 0: shifted to the face above plane 1
-            fzm = fct_lim(W1 - W0, P0 - Pm1, (nz == 3) ? (P1 - P0) : (P2 - P1), chi);
-        } else {               // base -1, m = k - 1
-            fzm = fct_lim(W0 - Wm1, Pm1 - Pm2, P1 - P0, chi);
+            fzma = fza_prev;
+            fzmb = fzb_prev;
+        } else if (k == 1) {   // m = 0: shifted to the face above plane 1
+            const bool top1 = (nz == 3);
+            fzma = fct_lim(Wca[2] - Wca[1], Pca[2] - Pca[1],
+                           top1 ? (Pca[3] - Pca[2]) : (Pca[4] - Pca[3]), chi);
+            fzmb = fct_lim(Wcb[2] - Wcb[1], Pcb[2] - Pcb[1],
+                           top1 ? (Pcb[3] - Pcb[2]) : (Pcb[4] - Pcb[3]), chi);
+        } else {               // the face above plane k - 1
+            fzma = fct_lim(Wca[1] - Wca[0], Pca[1] - Pca[0], Pca[3] - Pca[2], chi);
+            fzmb = fct_lim(Wcb[1] - Wcb[0], Pcb[1] - Pcb[0], Pcb[3] - Pcb[2], chi);
         }
-        fz_prev = fz;
+        fza_prev = fza;
+        fzb_prev = fzb;
         fz_plane = k;
-        auto Wy = [&](int o) { return wk[pc + o * FPX]; };
-        auto Py = [&](int o) { return pk[pc + o * FPX]; };
-        const float fy = fct_flux_bf(Wy, Py, 0, j, ny, chi);
-        const float fym = fct_flux_bf(Wy, Py, -1, j - 1, ny, chi);
-        const float tx0 = (i > 0 && i < nx - 1) ? (pk[pc] - fx + fxm) : wk[pc];
-        const float ty0 = (j > 0 && j < ny - 1) ? (tx0 - fy + fym) : tx0;
-        if (mine) out[c] = ty0 - fz + fzm;
+        const float txa = xint ? (Pca[2] - fxa + fxma) : Wca[1];
+        const float txb = xint ? (Pcb[2] - fxb + fxmb) : Wcb[1];
+        const float tya = (ja > 0 && ja < ny - 1) ? (txa - fy1 + fy0) : txa;
+        const float tyb = (jb > 0 && jb < ny - 1) ? (txb - fy2 + fy1) : txb;
+        if (oka) out[cpl + (size_t)ja * nx] = tya - fza + fzma;
+        if (okb) out[cpl + (size_t)jb * nx] = tyb - fzb + fzmb;
     };
-    // P planes pstart..pend are made in order; stage s stores wrk(s) and
-    // old(s+1) (fetched during stage s-1), makes P(s) and outputs plane
-    // s-2, which needs P(s-4..s) and wrk(s-3..s-1)
+    // shift the columns by one plane: P(s) and wrk(s-1) come in
+    auto shift = [&](float pa_new, float pb_new, float wa_new, float wb_new) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Pca[q] = Pca[q + 1];
+            Pcb[q] = Pcb[q + 1];
+        }
+        Pca[4] = pa_new;
+        Pcb[4] = pb_new;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            Wca[q] = Wca[q + 1];
+            Wcb[q] = Wcb[q + 1];
+        }
+        Wca[2] = wa_new;
+        Wcb[2] = wb_new;
+    };
+
+    // P planes pstart..pend are made in order.  Plane p's wrk, P and old
+    // live in slot p % 3; during stage s, s0 / s1 / s2 are the slots of
+    // planes s / s-1 / s-2 (old(s+1) goes to s2).  Stage s stores wrk(s)
+    // and old(s+1) (fetched during stage s-1), makes P(s), starts the loads
+    // of the next plane and outputs plane s-2.
     const int pstart = max(0, kb - 2), pend = min(nz - 1, ke + 1);
+    int s0 = pstart % 3, s1 = (pstart + 2) % 3, s2 = (pstart + 1) % 3;
     if (pstart >= 1) {
         fetch_o(pstart - 1);
-        store_o(oslot(pstart - 1));
+        store_o(L.o[s1]);
     }
     fetch_o(pstart);
-    store_o(oslot(pstart));
+    store_o(L.o[s0]);
     fetch_w(pstart);
     if (pstart + 1 <= nz - 1) fetch_o(pstart + 1);
     for (int s = pstart; s <= pend; ++s) {
-        __syncthreads();   // readers of the ring slots reused below are done
-        store_w(wslot(s));
-        if (s + 1 <= nz - 1) store_o(oslot(s + 1));
-        if (s + 1 <= pend) {   // next plane's loads in flight during this one
-            fetch_w(s + 1);
-            if (s + 2 <= nz - 1) fetch_o(s + 2);
+        __syncthreads();   // readers of the slots reused below are done
+        store_w(L.w[s0]);
+        if (s + 1 <= nz - 1) store_o(L.o[s2]);
+        __syncthreads();
+        {   // P(s) = w + chi * Lap7(old) inside, w on the box faces
+            // (code/sub_fcsmooth.f90:44-83)
+            const float* om = L.o[s1];
+            const float* oc = L.o[s0];
+            const float* op = L.o[s2];
+            float* dst = L.p[s0];
+            const bool zface = (s == 0 || s == nz - 1);
+#pragma unroll
+            for (int m = 0; m < FP_PT; ++m) {
+                if (m == FP_PT - 1 && tid + 256 * m >= FP_N) break;
+                const float wc = rw[m];
+                const int o0 = ocw[m];
+                const float vi = wc + chi * (oc[o0 + 1] + oc[o0 - 1] + oc[o0 + FOX] +
+                                             oc[o0 - FOX] + op[o0] + om[o0] - 6.0f * oc[o0]);
+                dst[tid + 256 * m] = (zface || ((xyface >> m) & 1)) ? wc : vi;
+            }
         }
+        if (s + 1 <= pend) fetch_w(s + 1);   // in flight during the output
+        if (s + 2 <= nz - 1) fetch_o(s + 2);
         __syncthreads();
-        make_p(s);
-        __syncthreads();
+        shift(L.p[s0][pa], L.p[s0][pb], L.w[s1][pa], L.w[s1][pb]);
         const int k = s - 2;
-        if (k >= kb && k < ke) output(k);
+        if (k >= kb && k < ke) output(k, L.w[s2], L.p[s2]);
+        const int tmp = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = tmp;
     }
-    for (int k = max(kb, pend - 1); k < ke; ++k) output(k);   // planes past the last P made
+    // planes past the last P made (only at the top of the box: plane nz-2
+    // needs P up to nz-1, plane nz-1 is a z face)
+    for (int s = pend + 1; s - 2 < ke; ++s) {
+        shift(0.0f, 0.0f, L.w[s1][pa], L.w[s1][pb]);
+        const int k = s - 2;
+        if (k >= kb) output(k, L.w[s2], L.p[s2]);
+        const int tmp = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = tmp;
+    }
 }
 
 void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
@@ -858,7 +906,7 @@ void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
     G.nf = nf;
     const int nb = ntx * nty * nkc * nf;
     ProfScope prof_("k_fct_fused", s);
-    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(FTX, FTY), 0, s, G, ntx, nty, nkc);
+    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, 4), 0, s, G, ntx, nty, nkc);
 }
 
 // ---------------------------------------------------------------------------

@@ -200,6 +200,21 @@ def test_flux_correct(dev, world):
     assert_bits(wg, wo, keys)
 
 
+@pytest.mark.parametrize("shape", [(70, 19, 37), (64, 8, 16), (127, 9, 18), (5, 5, 5)])
+def test_flux_correct_tiles(dev, shape):
+    """The one-pass FCT works on 63 x 8 tiles and runs of 16 planes: boxes
+    that end one cell into a tile, fill tiles exactly, or span several in
+    each direction, down to the smallest box."""
+    w = World(*shape, 2, seed=7 + sum(shape))
+    w.s.update(box=2)
+    names = flux_correct_names()
+    wg, _ = gpu_call(dev, w, "flux_correct", names)
+    wo = oracle_call(w, "flux_correct", names)
+    keys = [k for k in names[:102] if not k.startswith(("wrk", "old"))] + \
+           ["epres", "bx", "by", "bz"]
+    assert_bits(wg, wo, keys)
+
+
 def test_bndry_outer(dev, world):
     world.s.update(srho=np.float32(25.), spress=np.float32(0.02395), spx=np.float32(10.),
                    spy=np.float32(0.5), spz=np.float32(-0.1), sbx_wind=np.float32(0.),
