@@ -31,6 +31,14 @@ void latch(int code, const char* what) {
     }
 }
 
+// true when e is hipSuccess; otherwise the error is latched
+bool hip_ok(hipError_t e, const char* where) {
+    if (e == hipSuccess) return true;
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    latch((int)e, m.c_str());
+    return false;
+}
+
 void check_launch(const char* where) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -64,7 +72,7 @@ struct Buf {
     size_t bytes = 0;
     void* get(size_t need) {
         if (need > bytes) {
-            if (p) hipFree(p);
+            if (p) hip_ok(hipFree(p), "workspace free");
             p = nullptr;
             if (hipMalloc(&p, need) != hipSuccess) {
                 latch(2, "workspace allocation failed");
@@ -95,13 +103,14 @@ long long g_prof_count = 0;
 
 void prof_drain() {
     for (auto& e : g_prof_events) {
-        hipEventSynchronize(e.second);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e.first, e.second);
-        g_prof_ms += ms;
-        ++g_prof_count;
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        if (hip_ok(hipEventSynchronize(e.second), "kernel timing") &&
+            hip_ok(hipEventElapsedTime(&ms, e.first, e.second), "kernel timing")) {
+            g_prof_ms += ms;
+            ++g_prof_count;
+        }
+        hip_ok(hipEventDestroy(e.first), "kernel timing");
+        hip_ok(hipEventDestroy(e.second), "kernel timing");
     }
     g_prof_events.clear();
 }
@@ -111,14 +120,17 @@ void prof_drain() {
 namespace g3m3 {
 void prof_begin(const char* kernel, hipStream_t s) {
     if (g_prof_name.empty() || g_prof_name != kernel) return;
-    hipEventCreate(&g_prof_open);
-    hipEventRecord(g_prof_open, s);
+    if (!hip_ok(hipEventCreate(&g_prof_open), "kernel timing")) {
+        g_prof_open = nullptr;
+        return;
+    }
+    hip_ok(hipEventRecord(g_prof_open, s), "kernel timing");
 }
 void prof_end(const char* kernel, hipStream_t s) {
     if (g_prof_name.empty() || g_prof_name != kernel || !g_prof_open) return;
     hipEvent_t stop;
-    hipEventCreate(&stop);
-    hipEventRecord(stop, s);
+    if (!hip_ok(hipEventCreate(&stop), "kernel timing")) return;
+    hip_ok(hipEventRecord(stop, s), "kernel timing");
     g_prof_events.emplace_back(g_prof_open, stop);
     g_prof_open = nullptr;
     if (g_prof_events.size() > 4096) prof_drain();
@@ -149,7 +161,7 @@ int g3m3_init(void) {
         latch(1, "no HIP device");
         return 1;
     }
-    hipSetDevice(0);
+    if (!hip_ok(hipSetDevice(0), "init")) return 1;
     return 0;
 }
 
@@ -182,8 +194,8 @@ int g3m3_register(void* host, long long bytes) {
 int g3m3_unregister(void* host) {
     auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
     if (it == g_reg.end()) return 1;
-    hipStreamSynchronize(g_stream);
-    hipFree(it->second.dev);
+    hip_ok(hipStreamSynchronize(g_stream), "unregister");
+    hip_ok(hipFree(it->second.dev), "unregister");
     g_reg.erase(it);
     return 0;
 }
@@ -191,15 +203,21 @@ int g3m3_unregister(void* host) {
 int g3m3_to_device(void* host) {
     auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
     if (it == g_reg.end()) return 1;
-    hipMemcpyAsync(it->second.dev, host, it->second.bytes, hipMemcpyHostToDevice, g_stream);
-    return 0;
+    return hip_ok(hipMemcpyAsync(it->second.dev, host, it->second.bytes, hipMemcpyHostToDevice,
+                                 g_stream),
+                  "to_device")
+               ? 0
+               : 2;
 }
 
 int g3m3_to_host(void* host) {
     auto it = g_reg.find(reinterpret_cast<uintptr_t>(host));
     if (it == g_reg.end()) return 1;
-    hipMemcpyAsync(host, it->second.dev, it->second.bytes, hipMemcpyDeviceToHost, g_stream);
-    hipStreamSynchronize(g_stream);
+    if (!hip_ok(hipMemcpyAsync(host, it->second.dev, it->second.bytes, hipMemcpyDeviceToHost,
+                               g_stream),
+                "to_host") ||
+        !hip_ok(hipStreamSynchronize(g_stream), "to_host"))
+        return 2;
     return 0;
 }
 
@@ -460,8 +478,10 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
     launch_set_speed(B, SP, Q, b0, b, R(btot), S(epres, B, box), maxima, list, count, g_stream);
     check_launch("set_speed_agrd");
     float h[6];
-    hipMemcpyAsync(h, maxima, sizeof(h), hipMemcpyDeviceToHost, g_stream);
-    if (hipStreamSynchronize(g_stream) != hipSuccess) latch(3, "set_speed_agrd sync failed");
+    if (!hip_ok(hipMemcpyAsync(h, maxima, sizeof(h), hipMemcpyDeviceToHost, g_stream),
+                "set_speed_agrd") ||
+        !hip_ok(hipStreamSynchronize(g_stream), "set_speed_agrd"))
+        return;
     *pxmax = h[0];
     *pymax = h[1];
     *pzmax = h[2];
@@ -843,7 +863,10 @@ void g3m3_banded_tpose(const int* nbands, const int* npts, const int* divB_band_
     }
     int* pos = (int*)g_tmp.get(64);
     if (!pos) return;
-    hipMemcpyAsync(pos, divB_band_pos, 7 * sizeof(int), hipMemcpyHostToDevice, g_stream);
+    if (!hip_ok(hipMemcpyAsync(pos, divB_band_pos, 7 * sizeof(int), hipMemcpyHostToDevice,
+                               g_stream),
+                "banded_tpose"))
+        return;
     launch_banded_matvec(*npts, pos, R(banded_matrix), R(x_vec), R(tp_result), g_stream);
     check_launch("banded_tpose");
 }
@@ -885,11 +908,14 @@ void g3m3_trilin_interp(const float* sxyz, const float* gridpts, const float* qt
     std::memcpy(h, sxyz, 3 * sizeof(float));
     std::memcpy(h + 3, gridpts, 6 * sizeof(float));
     std::memcpy(h + 9, qty, 8 * sizeof(float));
-    hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, g_stream);
+    if (!hip_ok(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, g_stream),
+                "trilin_interp"))
+        return;
     launch_trilin(1, d, d + 3, d + 9, d + 17, g_stream);
-    hipMemcpyAsync(scdata, d + 17, sizeof(float), hipMemcpyDeviceToHost, g_stream);
-    hipStreamSynchronize(g_stream);
     check_launch("trilin_interp");
+    if (hip_ok(hipMemcpyAsync(scdata, d + 17, sizeof(float), hipMemcpyDeviceToHost, g_stream),
+               "trilin_interp"))
+        hip_ok(hipStreamSynchronize(g_stream), "trilin_interp");
 }
 
 void g3m3_sparse_solver(const int* nbands, const int* npts, const int* divB_band_pos,
@@ -925,6 +951,195 @@ void g3m3_fnd_pres(const float* px, const float* py, const float* pz, const floa
     launch_fnd_pres(B.V(), p, v, b, S(p_para, B, box), S(p_perp, B, box), S(p_cross, B, box),
                     g_stream);
     check_launch("fnd_pres");
+}
+
+// ---- initialisation ------------------------------------------------------------
+
+static RotationGeom rotation_geom(const float* rot_angle, const float* xdip, const float* ydip,
+                                  const float* zdip, const float* sin_tilt,
+                                  const float* cos_tilt, const float* b0) {
+    RotationGeom g;
+    g.xdip = *xdip;
+    g.ydip = *ydip;
+    g.zdip = *zdip;
+    g.sin_rot = std::sin(*rot_angle);   // the C library's sinf / cosf
+    g.cos_rot = std::cos(*rot_angle);
+    g.sin_tilt = *sin_tilt;
+    g.cos_tilt = *cos_tilt;
+    g.b0 = *b0;
+    return g;
+}
+
+// code/sub_mak_dip_grd.f90:22-24 and code/3d_main.f90:1247-1252
+static InitGeom init_geom(int nx, int ny, int nz, float x0, float x1, float y0, float y1,
+                          float z0, float z1) {
+    InitGeom G;
+    G.xmin = x0;
+    G.ymin = y0;
+    G.zmin = z0;
+    G.dx = (x1 - x0) / (nx - 1.0f);
+    G.dy = (y1 - y0) / (ny - 1.0f);
+    G.dz = (z1 - z0) / (nz - 1.0f);
+    return G;
+}
+
+void g3m3_dipole(float* abx, float* aby, float* abz, const float* ax, const float* ay,
+                 const float* az, const float* rot_angle, const float* xdip, const float* ydip,
+                 const float* zdip, const float* sin_tilt, const float* cos_tilt, const float* b0) {
+    const RotationGeom rot = rotation_geom(rot_angle, xdip, ydip, zdip, sin_tilt, cos_tilt, b0);
+    dipole_host(*ax, *ay, *az, rot, *abx, *aby, *abz);
+}
+
+void g3m3_mak_dip_grd(float* bx0, float* by0, float* bz0, const int* nx, const int* ny,
+                      const int* nz, const int* n_grids, const int* mbndry, const int* box,
+                      const int* ijzero, const int* numzero, const int* mzero,
+                      const float* r_inner, const float* grd_xmin, const float* grd_xmax,
+                      const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+                      const float* grd_zmax, const float* rot_angle, const float* xdip,
+                      const float* ydip, const float* zdip, const float* sin_tilt,
+                      const float* cos_tilt, const float* b0) {
+    (void)n_grids;
+    const Box B = box_of(nx, ny, nz);
+    const int b = *box - 1;
+    const InitGeom G = init_geom(*nx, *ny, *nz, grd_xmin[b], grd_xmax[b], grd_ymin[b],
+                                 grd_ymax[b], grd_zmin[b], grd_zmax[b]);
+    const RotationGeom rot = rotation_geom(rot_angle, xdip, ydip, zdip, sin_tilt, cos_tilt, b0);
+    int nzero = 0;
+    if (*box <= *mbndry) {
+        nzero = numzero[b];
+        if (nzero < 0 || nzero > *mzero) {
+            latch(4, "mak_dip_grd: numzero(box) outside 0..mzero");
+            return;
+        }
+    }
+    launch_mak_dip_grd(B, S(bx0, B, box), S(by0, B, box), S(bz0, B, box), G, rot, *r_inner,
+                       R(ijzero), *mbndry, *box, nzero, g_stream);
+    check_launch("mak_dip_grd");
+}
+
+void g3m3_set_resist(float* rst, const int* nx, const int* ny, const int* nz, const int* mbndry,
+                     const float* resist, const int* ijzero, const int* numzero,
+                     const int* ijmid, const int* nummid, const int* ijsrf, const int* numsrf,
+                     const int* msrf, const int* mmid, const int* mzero, const float* b0) {
+    const Box B = box_of(nx, ny, nz);
+    const int mb = *mbndry;
+    if (mb < 1) {
+        latch(4, "set_resist: mbndry < 1");
+        return;
+    }
+    const int cnt[3] = {numsrf[mb - 1], nummid[mb - 1], numzero[mb - 1]};
+    const int cap[3] = {*msrf, *mmid, *mzero};
+    for (int q = 0; q < 3; ++q)
+        if (cnt[q] < 0 || cnt[q] > cap[q]) {
+            latch(4, "set_resist: a point count exceeds its list's size");
+            return;
+        }
+    const int most = std::max(cnt[0], std::max(cnt[1], cnt[2]));
+    launch_set_resist(B, R(rst), mb, *resist, *b0, R(ijzero), R(ijmid), R(ijsrf), nullptr,
+                      cnt, most, g_stream);
+    check_launch("set_resist");
+}
+
+Buf g_init_cls, g_init_cnt;
+
+void g3m3_init_plasma(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* qpresxy, float* qpresxz, float* qpresyz, float* hrho, float* hpresx, float* hpresy,
+    float* hpresz, float* hpx, float* hpy, float* hpz, float* hpresxy, float* hpresxz,
+    float* hpresyz, float* orho, float* opresx, float* opresy, float* opresz, float* opx,
+    float* opy, float* opz, float* opresxy, float* opresxz, float* opresyz, float* epres,
+    float* bx, float* by, float* bz, float* bx0, float* by0, float* bz0, float* resistive,
+    const int* nx, const int* ny, const int* nz, const int* n_grids, const int* mbndry,
+    const int* msrf, const int* mmid, const int* mzero, const float* grid_minvals,
+    const float* grid_maxvals, float* parm_srf, float* parm_mid, float* parm_zero, int* ijsrf,
+    int* ijmid, int* ijzero, int* numsrf, int* nummid, int* numzero, const float* rmassq,
+    const float* rmassh, const float* rmasso, const float* ti_te, const float* re_equiv,
+    const float* r_inner, const float* zero_bndry, const float* mid_bndry,
+    const float* srf_bndry, const float* alpha_e, const float* erho, const float* d_min,
+    const float* cs_inner, const float* srho, const float* spx, const float* spy,
+    const float* spz, const float* spress, const float* cs_wind, const float* gamma,
+    const float* rho_frac, const float* resist, const float* v_rot, const float* r_rot,
+    const float* rot_angle, const float* xdip, const float* ydip, const float* zdip,
+    const float* sin_tilt, const float* cos_tilt, const float* b0) {
+    const int ng = *n_grids, mb = *mbndry;
+    if (ng < 1 || ng > kMaxInitGrids || mb < 1 || mb > ng) {
+        latch(4, "init_plasma: n_grids or mbndry out of range");
+        return;
+    }
+    InitPlasma P{};
+    P.B = box_of(nx, ny, nz);
+    float* f[34] = {qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz,
+                    hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz, hpresxy, hpresxz, hpresyz,
+                    orho, opresx, opresy, opresz, opx, opy, opz, opresxy, opresxz, opresyz,
+                    epres, bx, by, bz};
+    for (int q = 0; q < 34; ++q) P.f[q] = R(f[q]);
+    P.b0f[0] = R(bx0);
+    P.b0f[1] = R(by0);
+    P.b0f[2] = R(bz0);
+    const size_t V = (size_t)P.B.V();
+    P.cls = (unsigned char*)g_init_cls.get(V * mb);
+    P.counts = (int*)g_init_cnt.get(sizeof(int) * (3 * mb + 1));
+    if (!P.cls || !P.counts) return;
+    if (!hip_ok(hipMemsetAsync(P.counts + 3 * mb, 0, sizeof(int), g_stream), "init_plasma"))
+        return;
+    P.ijsrf = R(ijsrf);
+    P.ijmid = R(ijmid);
+    P.ijzero = R(ijzero);
+    P.parm_srf = R(parm_srf);
+    P.parm_mid = R(parm_mid);
+    P.parm_zero = R(parm_zero);
+    P.mbndry = mb;
+    P.msrf = *msrf;
+    P.mmid = *mmid;
+    P.mzero = *mzero;
+    for (int b = 0; b < ng; ++b)
+        P.geom[b] = init_geom(*nx, *ny, *nz, grid_minvals[3 * b], grid_maxvals[3 * b],
+                              grid_minvals[3 * b + 1], grid_maxvals[3 * b + 1],
+                              grid_minvals[3 * b + 2], grid_maxvals[3 * b + 2]);
+    P.rot = rotation_geom(rot_angle, xdip, ydip, zdip, sin_tilt, cos_tilt, b0);
+    P.re_equiv = *re_equiv;
+    P.r_rot = *r_rot;
+    P.v_rot = *v_rot;
+    P.zero_bndry = *zero_bndry;
+    P.mid_bndry = *mid_bndry;
+    P.srf_bndry = *srf_bndry;
+    P.r_inner = *r_inner;
+    P.alpha_e = *alpha_e;
+    P.erho = *erho;
+    P.d_min = *d_min;
+    P.cs_inner = *cs_inner;
+    P.rmassq = *rmassq;
+    P.rmassh = *rmassh;
+    P.rmasso = *rmasso;
+    P.ti_te = *ti_te;
+    P.srho = *srho;
+    P.spx = *spx;
+    P.spy = *spy;
+    P.spz = *spz;
+    P.spress = *spress;
+    P.cs_wind = *cs_wind;
+    P.gamma = *gamma;
+    P.rho_frac = *rho_frac;
+    P.wind_bnd = *r_rot / *re_equiv;   // code/3d_main.f90:1467
+    launch_init_plasma(P, ng, g_stream);
+    // code/3d_main.f90:1624-1626
+    const int most = std::max(*msrf, std::max(*mmid, *mzero));
+    launch_set_resist(P.B, R(resistive), mb, *resist, 1.0f, P.ijzero, P.ijmid, P.ijsrf,
+                      P.counts + 3 * (mb - 1), nullptr, most, g_stream);
+    check_launch("init_plasma");
+    std::vector<int> h(3 * mb + 1, 0);
+    if (hipMemcpyAsync(h.data(), P.counts, sizeof(int) * h.size(), hipMemcpyDeviceToHost,
+                       g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess) {
+        latch(5, "init_plasma: reading the point counts failed");
+        return;
+    }
+    for (int b = 0; b < mb; ++b) {
+        numsrf[b] = h[3 * b];
+        nummid[b] = h[3 * b + 1];
+        numzero[b] = h[3 * b + 2];
+    }
+    if (h[3 * mb]) latch(4, "init_plasma: more boundary points than msrf/mmid/mzero; lists cut");
 }
 
 static bool craft_grid(CraftGrid& G, const int* n_grids, const float* grid_minvals,
@@ -1026,5 +1241,9 @@ G3M3_FORTRAN_ALIAS(fnd_pres)
 G3M3_FORTRAN_ALIAS(sparse_solver)
 G3M3_FORTRAN_ALIAS(findgrid)
 G3M3_FORTRAN_ALIAS(sample_fields)
+G3M3_FORTRAN_ALIAS(dipole)
+G3M3_FORTRAN_ALIAS(mak_dip_grd)
+G3M3_FORTRAN_ALIAS(set_resist)
+G3M3_FORTRAN_ALIAS(init_plasma)
 #undef G3M3_FORTRAN_ALIAS
 #endif

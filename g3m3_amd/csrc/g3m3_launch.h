@@ -69,6 +69,41 @@ struct FlankGeom {
 
 enum { kDivbRef = 0, kDivbF64 = 1, kDivbCG = 2, kDivbRefine = 3 };
 
+// one box's origin and cell size (code/3d_main.f90:1247-1252)
+struct InitGeom {
+    float xmin, ymin, zmin, dx, dy, dz;
+};
+// common /rotation/'s dipole centre, tilt and strength (code/3d_main.f90:
+// 633-634) with the sine and cosine of its rotation angle
+struct RotationGeom {
+    float xdip, ydip, zdip, sin_rot, cos_rot, sin_tilt, cos_tilt, b0;
+};
+constexpr int kMaxInitGrids = 32;
+// the initialisation block (code/3d_main.f90:1245-1627): fields in the
+// order q, h, o x (rho, presx, presy, presz, px, py, pz, presxy, presxz,
+// presyz), epres, bx, by, bz, each (nx,ny,nz,n_grids); the point lists of
+// boxes 1..mbndry as the reference lays them out, ij(mbndry,3,m) and
+// parm(mbndry,7,m); cls = a class byte per cell of boxes 1..mbndry;
+// counts = (srf, mid, zero) per box, cut at the capacity, then an
+// overflow flag
+struct InitPlasma {
+    Box B;
+    float* f[34];
+    float* b0f[3];
+    unsigned char* cls;
+    int* counts;
+    int *ijsrf, *ijmid, *ijzero;
+    float *parm_srf, *parm_mid, *parm_zero;
+    int mbndry, msrf, mmid, mzero;
+    InitGeom geom[kMaxInitGrids];
+    RotationGeom rot;
+    float re_equiv, r_rot, v_rot, zero_bndry, mid_bndry, srf_bndry, r_inner, alpha_e, erho, d_min,
+        cs_inner;
+    float rmassq, rmassh, rmasso, ti_te;
+    float srho, spx, spy, spz, spress, cs_wind, gamma, rho_frac, wind_bnd;
+};
+
+
 // Optional per-kernel timing: when the kernel named by g3m3_prof_select()
 // is launched, HIP events are recorded around it on its stream.
 void prof_begin(const char* kernel, hipStream_t s);
@@ -141,5 +176,16 @@ void launch_banded_matvec(int npts, const int* pos, const float* M, const float*
                           hipStream_t s);
 void launch_trilin(int npts, const float* sxyz, const float* gridpts, const float* qty,
                    float* out, hipStream_t s);
+void dipole_host(float ax, float ay, float az, const RotationGeom& R, float& abx, float& aby,
+                 float& abz);
+void launch_mak_dip_grd(const Box& B, float* bx0, float* by0, float* bz0, const InitGeom& G,
+                        const RotationGeom& R, float r_inner, const int* ijzero, int mbndry,
+                        int box, int numzero, hipStream_t s);
+// point counts (srf, mid, zero) from dcounts on the device, or from the
+// host array hcounts when dcounts is null
+void launch_set_resist(const Box& B, float* rst, int mbndry, float resist, float b0,
+                       const int* ijzero, const int* ijmid, const int* ijsrf, const int* dcounts,
+                       const int hcounts[3], int max_points, hipStream_t s);
+void launch_init_plasma(const InitPlasma& P, int n_grids, hipStream_t s);
 
 }  // namespace g3m3

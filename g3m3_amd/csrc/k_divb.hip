@@ -10,11 +10,12 @@
 //   kDivbRef  -- the reference's single-precision scalars (each reduced
 //                dot is rounded to float, scalar updates in float);
 //   kDivbF64  -- scalars and vector updates in double, stored as float.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "g3m3_dev.h"
 #include "g3m3_launch.h"
-
-#include <algorithm>
-#include <vector>
 
 namespace g3m3 {
 
@@ -57,6 +58,17 @@ struct DivbCtl {
 };
 
 enum { kScResd = 2, kScSumB = 3, kScErr2 = 8, kScR2 = 9 };
+
+// solve status when a HIP call fails (the error is also left in HIP's
+// last-error state for the caller's check)
+constexpr int kStatusHipFailed = 4;
+
+// device -> host copy of solver control data, waited for; false when HIP
+// reports an error
+static bool fetch(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
+           hipStreamSynchronize(s) == hipSuccess;
+}
 
 __device__ inline double as_mode_d(double v, int mode) {
     return mode == kDivbRef ? (double)(float)v : v;
@@ -284,15 +296,13 @@ struct DivbSolver {
     int status = 0;
 
     DivbCtl read_ctl() {
-        DivbCtl h;
-        hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        DivbCtl h{};
+        if (!fetch(&h, ctl, sizeof(h), s)) h.status = kStatusHipFailed;
         return h;
     }
     double read_sc(int q) {
         double v = 0.0;
-        hipMemcpyAsync(&v, sc + q, sizeof(double), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        if (!fetch(&v, sc + q, sizeof(double), s)) return std::nan("");
         return v;
     }
 
@@ -341,6 +351,10 @@ struct DivbSolver {
                 tail(L);
             }
             const DivbCtl c = read_ctl();
+            if (c.status == kStatusHipFailed) {
+                status = kStatusHipFailed;
+                return;
+            }
             iters = c.iters;
             level_done += iters - start;
             if (c.status == 0) continue;
@@ -362,6 +376,10 @@ struct DivbSolver {
             hipLaunchKernelGGL(k_resume, dim3(1), t, 0, s, sc, ctl);
             tail(L);
             const DivbCtl c2 = read_ctl();
+            if (c2.status == kStatusHipFailed) {
+                status = kStatusHipFailed;
+                return;
+            }
             if (c2.status == 1) {
                 status = 1;
                 return;
@@ -509,17 +527,15 @@ static int cg_solve(const Box& B, const float* rhs, float* soln, float* vec, dou
             cur ^= 1;
         }
         it += nbatch;
-        DivbCtl h;
-        hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        DivbCtl h{};
+        if (!fetch(&h, ctl, sizeof(h), s)) return kStatusHipFailed;
         if (h.status != 0) {
             if (iters_out) *iters_out = h.iters;
             return 1;
         }
     }
-    DivbCtl h;
-    hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    DivbCtl h{};
+    if (!fetch(&h, ctl, sizeof(h), s)) return kStatusHipFailed;
     if (iters_out) *iters_out = h.iters;
     return h.status == 1 ? 1 : 3;
 }
@@ -602,8 +618,10 @@ static int refine_solve(const Box& B, const float* rhs, float* soln, double* xd,
         hipLaunchKernelGGL(k_dres, g, t, 0, s, B, xd, rhs, rf, part, pass == 0 ? 1 : 0);
         hipLaunchKernelGGL(k_sum2, dim3(1), t, 0, s, part, nb, sc + kCgTmp);
         double h[2];
-        hipMemcpyAsync(h, sc + kCgTmp, sizeof(h), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        if (!fetch(h, sc + kCgTmp, sizeof(h), s)) {
+            status = kStatusHipFailed;
+            break;
+        }
         r2 = h[0];
         if (pass == 0) b2 = h[1];
         if (r2 <= b2 * err2) {
@@ -613,7 +631,11 @@ static int refine_solve(const Box& B, const float* rhs, float* soln, double* xd,
         if (total >= max_iters) break;
         // inner solve to 1e-4 of the current residual (FP32 vectors)
         long long it = 0;
-        cg_solve(B, rf, d, vec, sc, ctl, part, 1e-8, max_iters - total, &it, s);
+        if (cg_solve(B, rf, d, vec, sc, ctl, part, 1e-8, max_iters - total, &it, s) ==
+            kStatusHipFailed) {
+            status = kStatusHipFailed;
+            break;
+        }
         total += it;
         hipLaunchKernelGGL(k_dadd, g, t, 0, s, n, xd, d);
     }

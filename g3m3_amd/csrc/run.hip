@@ -70,12 +70,26 @@ struct Run {
     Set wrk(int box) const { return slice(d.wrk, box); }
 };
 
+// true when e is hipSuccess; otherwise the run's first error is recorded
+bool ok_hip(Run* R, hipError_t e, const char* where) {
+    if (e == hipSuccess) return true;
+    if (R->err == 0) {
+        R->err = (int)e;
+        fprintf(stderr, "g3m3 run: %s: %s\n", where, hipGetErrorString(e));
+    }
+    return false;
+}
+
 void check(Run* R, const char* where) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess && R->err == 0) {
         R->err = (int)e;
         fprintf(stderr, "g3m3 run: %s: %s\n", where, hipGetErrorString(e));
     }
+}
+
+bool in_box(const Box& B, int i, int j, int k) {
+    return i >= 0 && i < B.nx && j >= 0 && j < B.ny && k >= 0 && k < B.nz;
 }
 
 // x is +-2^n (so 1/x is exact and x / y == x * (1/y) bit for bit)
@@ -429,8 +443,12 @@ void box_step(Run* R, int box) {
 
 void read_tstepnew(Run* R, const std::vector<int>& boxes) {
     std::vector<float> h(5 * R->d.n_grids);
-    hipMemcpyAsync(h.data(), R->maxima, h.size() * sizeof(float), hipMemcpyDeviceToHost, R->s);
-    hipStreamSynchronize(R->s);
+    if (!ok_hip(R,
+                hipMemcpyAsync(h.data(), R->maxima, h.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, R->s),
+                "read CFL maxima") ||
+        !ok_hip(R, hipStreamSynchronize(R->s), "read CFL maxima"))
+        return;
     for (int box : boxes) {
         const float* m = &h[5 * (box - 1)];
         float f = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3] + m[4] * m[4]);
@@ -467,23 +485,29 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
         ok = hipMalloc(&R->rst_eff, sizeof(float) * (size_t)n) == hipSuccess;
         if (ok) {
             std::vector<float> h(n, 0.0f), rst(n);
-            hipMemcpy(rst.data(), desc->rst + (size_t)(desc->mbndry - 1) * n, sizeof(float) * n,
-                      hipMemcpyDeviceToHost);
+            ok = hipMemcpy(rst.data(), desc->rst + (size_t)(desc->mbndry - 1) * n,
+                           sizeof(float) * n, hipMemcpyDeviceToHost) == hipSuccess;
             const int mb = desc->mbndry, b = desc->mbndry - 1;
             const int* lists[2] = {desc->ijmid, desc->ijzero};
             const int nums[2] = {desc->nummid, desc->numzero};
-            for (int l = 0; l < 2; ++l) {
+            for (int l = 0; ok && l < 2; ++l) {
                 if (nums[l] <= 0) continue;
                 std::vector<int> ij((size_t)mb * 3 * nums[l]);
-                hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(), hipMemcpyDeviceToHost);
-                for (int p = 0; p < nums[l]; ++p) {
+                ok = hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(),
+                               hipMemcpyDeviceToHost) == hipSuccess;
+                for (int p = 0; ok && p < nums[l]; ++p) {
                     const int i = ij[b + mb * (0 + 3 * p)] - 1, j = ij[b + mb * (1 + 3 * p)] - 1,
                               k = ij[b + mb * (2 + 3 * p)] - 1;
+                    if (!in_box(R->B, i, j, k)) {
+                        ok = false;
+                        break;
+                    }
                     const int c = R->B.IX(i, j, k);
                     h[c] = rst[c];
                 }
             }
-            hipMemcpy(R->rst_eff, h.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+            ok = ok && hipMemcpy(R->rst_eff, h.data(), sizeof(float) * n,
+                                 hipMemcpyHostToDevice) == hipSuccess;
         }
     }
     if (ok && desc->mbndry >= 1) {
@@ -493,13 +517,17 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
         const int* lists[3] = {desc->ijsrf, desc->ijmid, desc->ijzero};
         const int nums[3] = {desc->numsrf, desc->nummid, desc->numzero};
         std::vector<int> cells;
-        for (int l = 0; l < 3; ++l) {
+        for (int l = 0; ok && l < 3; ++l) {
             if (nums[l] <= 0) continue;
             std::vector<int> ij((size_t)mb * 3 * nums[l]);
-            hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(), hipMemcpyDeviceToHost);
-            for (int p = 0; p < nums[l]; ++p)
-                cells.push_back(R->B.IX(ij[b + mb * (0 + 3 * p)] - 1, ij[b + mb * (1 + 3 * p)] - 1,
-                                        ij[b + mb * (2 + 3 * p)] - 1));
+            ok = hipMemcpy(ij.data(), lists[l], sizeof(int) * ij.size(),
+                           hipMemcpyDeviceToHost) == hipSuccess;
+            for (int p = 0; ok && p < nums[l]; ++p) {
+                const int i = ij[b + mb * (0 + 3 * p)] - 1, j = ij[b + mb * (1 + 3 * p)] - 1,
+                          k = ij[b + mb * (2 + 3 * p)] - 1;
+                ok = in_box(R->B, i, j, k);
+                if (ok) cells.push_back(R->B.IX(i, j, k));
+            }
         }
         std::sort(cells.begin(), cells.end());
         R->lists_disjoint = std::adjacent_find(cells.begin(), cells.end()) == cells.end();
@@ -512,13 +540,15 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
              hipMemcpy(R->shell[box - 1], h.data(), sizeof(ShellPt) * h.size(),
                        hipMemcpyHostToDevice) == hipSuccess;
     }
+    ok = ok && hipMemset(R->count, 0, 128) == hipSuccess &&
+         hipMemset(R->maxima, 0, sizeof(float) * 5 * ng) == hipSuccess;
     if (!ok) {
-        fprintf(stderr, "g3m3_run_create: device allocation failed\n");
+        fprintf(stderr,
+                "g3m3_run_create: device allocation or copy failed, or a planet list point "
+                "lies outside the box\n");
         g3m3_run_destroy(R);
         return nullptr;
     }
-    hipMemset(R->count, 0, 128);
-    hipMemset(R->maxima, 0, sizeof(float) * 5 * ng);
     R->t_old.assign(ng, 0.0);
     R->t_new.assign(ng, 0.0);
     R->t_step.assign(ng, 0.0);
@@ -530,12 +560,13 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
 void g3m3_run_destroy(void* run) {
     Run* R = (Run*)run;
     if (!R) return;
-    hipStreamSynchronize(R->s);
+    // teardown: errors here have no caller left to report them to
+    (void)hipStreamSynchronize(R->s);
     for (void* p : {(void*)R->scr, (void*)R->list, (void*)R->count,
                     (void*)R->partials, (void*)R->maxima, (void*)R->rst_eff})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (ShellPt* p : R->shell)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     delete R;
 }
 
@@ -645,7 +676,7 @@ int g3m3_run_sync_arrays(void* run) {
         R->rot[box - 1] = 0;
     }
     // the caller reads or writes the arrays next, possibly on another stream
-    hipStreamSynchronize(R->s);
+    ok_hip(R, hipStreamSynchronize(R->s), "sync_arrays");
     check(R, "sync_arrays");
     return R->err;
 }

@@ -338,7 +338,8 @@ void g3m3_divb_correct(const int* nx, const int* ny, const int* nz, const int* n
  *       below FP32's reach, e.g. 1e-8),
  * an iteration cap (0 = the reference's 10*npts), and the iteration count
  * and final squared residual |A x - b|^2 returned.  Returns the solver
- * status: 1 converged, 2 stopped after restarts, 3 iteration cap. */
+ * status: 1 converged, 2 stopped after restarts, 3 iteration cap, 4 a HIP call
+ * failed (the error is latched). */
 int g3m3_divb_correct_ex(const int* nx, const int* ny, const int* nz, const int* n_grids,
                          const int* box, const float* xspac, const float* divB_errlim,
                          float* bx, float* by, float* bz, const int* mode,
@@ -384,6 +385,70 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
                         const float* grid_minvals, const float* grid_maxvals, const float* xspac,
                         const float* re_equiv, const int* nx, const int* ny, const int* nz,
                         const int* nq, const float* const* fields, float* scdata, int* gridpt);
+
+/* ---- initialisation -----------------------------------------------------------
+ * common /rotation/ (code/3d_main.f90:633-634), which the reference's
+ * dipole and mak_dip_grd read, comes in as the trailing arguments rot_angle,
+ * xdip, ydip, zdip, sin_tilt, cos_tilt, b0; sin and cos of rot_angle are
+ * taken on the host with the C library's sinf / cosf, as a compiled
+ * reference takes them. */
+
+/* code/sub_dipole.f90:4 -- the dipole field at one point, on the host. */
+void g3m3_dipole(float* abx, float* aby, float* abz, const float* ax, const float* ay,
+                 const float* az, const float* rot_angle, const float* xdip, const float* ydip,
+                 const float* zdip, const float* sin_tilt, const float* cos_tilt, const float* b0);
+
+/* code/sub_mak_dip_grd.f90:4-6 -- the dipole on box `box`, zero inside
+ * r_inner-1.5 and at the box's ijzero points; numzero is a host array. */
+void g3m3_mak_dip_grd(float* bx0, float* by0, float* bz0, const int* nx, const int* ny,
+                      const int* nz, const int* n_grids, const int* mbndry, const int* box,
+                      const int* ijzero, const int* numzero, const int* mzero,
+                      const float* r_inner, const float* grd_xmin, const float* grd_xmax,
+                      const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+                      const float* grd_zmax, const float* rot_angle, const float* xdip,
+                      const float* ydip, const float* zdip, const float* sin_tilt,
+                      const float* cos_tilt, const float* b0);
+
+/* code/sub_set_resist.f90:10-13 -- rst(nx,ny,nz,mbndry); the point counts
+ * are host arrays, and (as in the reference) those of box mbndry are used
+ * for every box. */
+void g3m3_set_resist(float* rst, const int* nx, const int* ny, const int* nz, const int* mbndry,
+                     const float* resist, const int* ijzero, const int* numzero,
+                     const int* ijmid, const int* nummid, const int* ijsrf, const int* numsrf,
+                     const int* msrf, const int* mmid, const int* mzero, const float* b0);
+
+/* code/3d_main.f90:1245-1627 -- the main program's initialisation block,
+ * which has no subroutine of its own: the dipole field b0 of every box, the
+ * ambient ionospheric plasma, the inner-boundary point lists of boxes
+ * 1..mbndry with their saved values, the solar wind beyond r_rot/re_equiv,
+ * the speed check, and set_resist(resistive, ..., b0 = 1.) (:1624-1626).
+ * Fields come in the order q, h, o x (rho, presx, presy, presz, px, py, pz,
+ * presxy, presxz, presyz), then epres, bx, by, bz, bx0, by0, bz0, each
+ * (nx,ny,nz,n_grids); resistive is (nx,ny,nz,mbndry).  grid_minvals and
+ * grid_maxvals are the driver's (3,n_grids) host arrays.  The lists are
+ * written on the device; their counts num*(mbndry) are host outputs (the
+ * call synchronises).  A list longer than its capacity (msrf, mmid, mzero)
+ * is cut there and the error latch is set.  Scalars are the driver's
+ * variables of the same names; rot_angle is the one computed at :1217. */
+void g3m3_init_plasma(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* qpresxy, float* qpresxz, float* qpresyz, float* hrho, float* hpresx, float* hpresy,
+    float* hpresz, float* hpx, float* hpy, float* hpz, float* hpresxy, float* hpresxz,
+    float* hpresyz, float* orho, float* opresx, float* opresy, float* opresz, float* opx,
+    float* opy, float* opz, float* opresxy, float* opresxz, float* opresyz, float* epres,
+    float* bx, float* by, float* bz, float* bx0, float* by0, float* bz0, float* resistive,
+    const int* nx, const int* ny, const int* nz, const int* n_grids, const int* mbndry,
+    const int* msrf, const int* mmid, const int* mzero, const float* grid_minvals,
+    const float* grid_maxvals, float* parm_srf, float* parm_mid, float* parm_zero, int* ijsrf,
+    int* ijmid, int* ijzero, int* numsrf, int* nummid, int* numzero, const float* rmassq,
+    const float* rmassh, const float* rmasso, const float* ti_te, const float* re_equiv,
+    const float* r_inner, const float* zero_bndry, const float* mid_bndry,
+    const float* srf_bndry, const float* alpha_e, const float* erho, const float* d_min,
+    const float* cs_inner, const float* srho, const float* spx, const float* spy,
+    const float* spz, const float* spress, const float* cs_wind, const float* gamma,
+    const float* rho_frac, const float* resist, const float* v_rot, const float* r_rot,
+    const float* rot_angle, const float* xdip, const float* ydip, const float* zdip,
+    const float* sin_tilt, const float* cos_tilt, const float* b0);
 
 /* ---- fluid (restart) files ------------------------------------------------
  * The reference's restart format (written at code/3d_main.f90:3310-3349,

@@ -1215,3 +1215,279 @@ def divB_correct_f64(bx, by, bz, rx):
     out[1][1:-1, 1:-1, 1:-1] -= (phi[1:-1, 2:, 1:-1] - phi[1:-1, :-2, 1:-1]) / (2.0 * rx)
     out[2][1:-1, 1:-1, 1:-1] -= (phi[2:, 1:-1, 1:-1] - phi[:-2, 1:-1, 1:-1]) / (2.0 * rx)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Initialisation: the dipole field (code/sub_dipole.f90), mak_dip_grd
+# (code/sub_mak_dip_grd.f90), set_resist (code/sub_set_resist.f90) and the
+# main program's initialisation block (code/3d_main.f90:1245-1627).
+# sin / cos / pow of single-precision values are the C library's sinf,
+# cosf and powf, which a compiled reference calls; integer powers x**n are
+# expanded left to right, ((x*x)*x)*..., as flang expands them.
+
+import ctypes as _ct
+import ctypes.util as _ctu
+
+_LIBM = _ct.CDLL(_ctu.find_library("m"))
+for _fn in ("sinf", "cosf"):
+    getattr(_LIBM, _fn).restype = _ct.c_float
+    getattr(_LIBM, _fn).argtypes = [_ct.c_float]
+_LIBM.powf.restype = _ct.c_float
+_LIBM.powf.argtypes = [_ct.c_float, _ct.c_float]
+
+
+def _sinf(x):
+    return F(_LIBM.sinf(float(F(x))))
+
+
+def _cosf(x):
+    return F(_LIBM.cosf(float(F(x))))
+
+
+def _powf(a, e):
+    """Elementwise C powf of a float32 array (small grids only)."""
+    a = np.asarray(a, F)
+    out = np.fromiter((_LIBM.powf(float(v), float(F(e))) for v in a.ravel()), F, a.size)
+    return out.reshape(a.shape)
+
+
+def _ipow(x, n):
+    r = x
+    for _ in range(n - 1):
+        r = r * x
+    return r
+
+
+def _dipole_core(x1, y1, z1, sin_rot, cos_rot, sin_tilt, cos_tilt, b0):
+    """code/sub_dipole.f90:17-49 (and the same lines of mak_dip_grd); returns
+    (abx, aby, abz, ar)."""
+    xr = x1 * cos_rot + y1 * sin_rot
+    yr = -(x1 * sin_rot) + y1 * cos_rot
+    xp = xr * cos_tilt - z1 * sin_tilt
+    yp = yr
+    zp = xr * sin_tilt + z1 * cos_tilt
+    x2, y2, z2 = xp * xp, yp * yp, zp * zp
+    ar = np.sqrt(x2 + y2 + z2)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        bmag = -(b0 / _ipow(ar, 5))
+        dbx = -(F(3.) * bmag * xp * zp)
+        dby = -(F(3.) * bmag * yp * zp)
+        dbz = bmag * (x2 + y2 - F(2.) * z2)
+        rbx = dbx * cos_tilt + dbz * sin_tilt
+        rby = dby
+        rbz = -(dbx * sin_tilt) + dbz * cos_tilt
+        abx = rbx * cos_rot - rby * sin_rot
+        aby = rbx * sin_rot + rby * cos_rot
+    return abx, aby, rbz, ar
+
+
+def dipole(abx, aby, abz, ax, ay, az, rot_angle=0.0, xdip=0.0, ydip=0.0, zdip=0.0,
+           sin_tilt=0.0, cos_tilt=1.0, b0=1.0):
+    """code/sub_dipole.f90:4-52 at one point; abx/aby/abz are 1-element
+    arrays; the keyword arguments are common /rotation/."""
+    x1 = F(ax) - F(xdip)
+    y1 = F(ay) - F(ydip)
+    z1 = F(az) - F(zdip)
+    bx, by, bz, _ = _dipole_core(x1, y1, z1, _sinf(rot_angle), _cosf(rot_angle), F(sin_tilt),
+                                 F(cos_tilt), F(b0))
+    abx[0], aby[0], abz[0] = bx, by, bz
+
+
+def _box_axes(nx, ny, nz, x0, x1, y0, y1, z0, z1):
+    dx = (F(x1) - F(x0)) / (F(nx) - F(1.))
+    dy = (F(y1) - F(y0)) / (F(ny) - F(1.))
+    dz = (F(z1) - F(z0)) / (F(nz) - F(1.))
+    ax = (F(x0) + dx * np.arange(nx, dtype=F))[None, None, :]
+    ay = (F(y0) + dy * np.arange(ny, dtype=F))[None, :, None]
+    az = (F(z0) + dz * np.arange(nz, dtype=F))[:, None, None]
+    return ax, ay, az
+
+
+def mak_dip_grd(bx0, by0, bz0, nx, ny, nz, n_grids, mbndry, box, ijzero, numzero, mzero,
+                r_inner, grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax,
+                rot_angle=0.0, xdip=0.0, ydip=0.0, zdip=0.0, sin_tilt=0.0, cos_tilt=1.0,
+                b0=1.0):
+    """code/sub_mak_dip_grd.f90:4-112; the keyword arguments are common
+    /rotation/."""
+    b = box - 1
+    ax, ay, az = _box_axes(nx, ny, nz, grd_xmin[b], grd_xmax[b], grd_ymin[b], grd_ymax[b],
+                           grd_zmin[b], grd_zmax[b])
+    x1, y1, z1 = ax - F(xdip), ay - F(ydip), az - F(zdip)
+    bx, by, bz, ar = _dipole_core(x1, y1, z1, _sinf(rot_angle), _cosf(rot_angle),
+                                  F(sin_tilt), F(cos_tilt), F(b0))
+    keep = ar > F(r_inner) - F(1.5)
+    bx0[b] = np.where(keep, bx, F(0.))
+    by0[b] = np.where(keep, by, F(0.))
+    bz0[b] = np.where(keep, bz, F(0.))
+    if box <= mbndry:
+        for n in range(int(numzero[b])):
+            i, j, k = (int(ijzero[n, c, b]) - 1 for c in range(3))
+            bx0[b, k, j, i] = by0[b, k, j, i] = bz0[b, k, j, i] = F(0.)
+
+
+def set_resist(rst, nx, ny, nz, mbndry, resist, ijzero, numzero, ijmid, nummid, ijsrf, numsrf,
+               msrf, mmid, mzero, b0):
+    """code/sub_set_resist.f90:10-63 (the counts of box mbndry for every
+    box, as the reference takes them)."""
+    rst[...] = F(0.)
+    mb = mbndry
+    for lst, num, val in ((ijzero, numzero, F(b0) / F(resist)),
+                          (ijmid, nummid, F(0.5) * F(b0) / F(resist)),
+                          (ijsrf, numsrf, F(0.125) * F(b0) / F(resist))):
+        for n in range(int(num[mb - 1])):
+            for b in range(mb):
+                i, j, k = (int(lst[n, c, b]) - 1 for c in range(3))
+                rst[b, k, j, i] = val
+
+
+def init_plasma(f, bx0, by0, bz0, resistive, nx, ny, nz, n_grids, mbndry, msrf, mmid, mzero,
+                grid_minvals, grid_maxvals, parm_srf, parm_mid, parm_zero, ijsrf, ijmid, ijzero,
+                numsrf, nummid, numzero, s):
+    """code/3d_main.f90:1245-1627, the main program's initialisation block.
+    ``f`` maps the 34 state-field names to (n_grids, nz, ny, nx) arrays;
+    ``s`` holds the scalars by their reference names (rmassq ... b0, with
+    rot_angle and the common /rotation/ values).  Lists and counts are
+    filled in place.  Returns the list lengths found (before any cut)."""
+    g = {k: F(v) for k, v in s.items()}
+    sin_rot, cos_rot = _sinf(s["rot_angle"]), _cosf(s["rot_angle"])
+    st, ct = g["sin_tilt"], g["cos_tilt"]
+    numsrf[...] = 0
+    nummid[...] = 0
+    numzero[...] = 0
+    found = np.zeros((mbndry, 3), np.int64)
+    ri = g["r_inner"]
+    for b in range(n_grids):
+        ax, ay, az = _box_axes(nx, ny, nz, grid_minvals[b, 0], grid_maxvals[b, 0],
+                               grid_minvals[b, 1], grid_maxvals[b, 1], grid_minvals[b, 2],
+                               grid_maxvals[b, 2])
+        az, ay, ax = az - g["zdip"], ay - g["ydip"], ax - g["xdip"]
+        shp = (nz, ny, nx)
+        ax, ay, az = (np.broadcast_to(a, shp).astype(F) for a in (ax, ay, az))
+        xr = ax * cos_rot + ay * sin_rot
+        yr = -(ax * sin_rot) + ay * cos_rot
+        xp = xr * ct - az * st
+        yp = yr
+        zp = xr * st + az * ct
+        ar = np.sqrt(xp * xp + yp * yp + zp * zp)
+        bx, by, bz, _ = _dipole_core(ax - g["xdip"], ay - g["ydip"], az - g["zdip"], sin_rot,
+                                     cos_rot, st, ct, g["b0"])
+        inner = ar < g["zero_bndry"]
+        bx0[b] = np.where(inner, F(0.), bx)
+        by0[b] = np.where(inner, F(0.), by)
+        bz0[b] = np.where(inner, F(0.), bz)
+        rx = ax * g["re_equiv"]
+        ry = ay * g["re_equiv"]
+        rd = np.sqrt(rx * rx + ry * ry)
+        rr = g["r_rot"]
+        t = (F(2.) * (rr * rr)) / (rr * rr + rd * rd)
+        vfrac = np.where(rd < rr, F(1.), t * t)
+        rvy = rx * g["v_rot"] * vfrac
+        rvx = -(ry * g["v_rot"] * vfrac)
+        ar_iono = np.sqrt(xp * xp + ay * ay + F(1.25) * (zp * zp))
+        ar_iono = np.maximum(F(0.0001), ar_iono)
+        ra = _powf((ar_iono + F(0.5) * ri) / (F(1.5) * ri), -g["alpha_e"])
+        h15, h30 = F(1.5) * ri, F(3.0) * ri
+        zheight = np.maximum(F(1.), (zp * zp + h15 * h15) / (h30 * h30))
+        ra = ra / (zheight * zheight)
+        rho_iono = np.maximum(g["erho"] * ra, g["d_min"])
+        arho = np.maximum(rho_iono, g["d_min"])
+        vt = np.maximum(np.sqrt(rvy * rvy + rvx * rvx), g["cs_inner"])
+        qrho = arho * g["rmassq"] / zheight
+        qpres = arho * vt
+        hrho = F(0.05) * qrho * g["rmassh"] / g["rmassq"]
+        hpres = F(0.05) * qpres
+        orho = qrho * g["rmasso"] / g["rmassq"]
+        for sp, rho, pr in (("q", qrho, qpres), ("h", hrho, hpres), ("o", orho, qpres)):
+            f[sp + "rho"][b] = rho
+            for c in "xyz":
+                f[sp + "pres" + c][b] = pr
+            for c in ("xy", "xz", "yz"):
+                f[sp + "pres" + c][b] = F(0.)
+            f[sp + "px"][b] = rho * rvx
+            f[sp + "py"][b] = rho * rvy
+            f[sp + "pz"][b] = F(0.)
+        f["epres"][b] = (qpres + hpres + qpres) / g["ti_te"]
+        for c in "xyz":
+            f["b" + c][b] = F(0.)
+        if b < mbndry:
+            # code/3d_main.f90:1384-1455: loop order k, j, i = memory order
+            cls = np.zeros(shp, np.int8)
+            near = ar <= g["srf_bndry"]
+            cls[near] = 1
+            cls[near & (ar < g["mid_bndry"])] = 2
+            cls[near & (ar < g["zero_bndry"])] = 3
+            flat = cls.ravel()
+            vals = [f[k][b].ravel() for k in ("qrho", "hrho", "orho", "qpresx", "hpresx",
+                                              "opresx", "epres")]
+            for q, (ij, parm, cap) in enumerate(((ijsrf, parm_srf, msrf), (ijmid, parm_mid, mmid),
+                                                 (ijzero, parm_zero, mzero))):
+                idx = np.nonzero(flat == q + 1)[0]
+                found[b, q] = len(idx)
+                idx = idx[:cap]
+                kk, jj, ii = np.unravel_index(idx, shp)
+                ij[:len(idx), 0, b] = ii + 1
+                ij[:len(idx), 1, b] = jj + 1
+                ij[:len(idx), 2, b] = kk + 1
+                for a in range(7):
+                    parm[:len(idx), a, b] = vals[a][idx]
+                (numsrf, nummid, numzero)[q][b] = len(idx)
+    # solar wind (:1466-1590) and the speed check (:1592-1621)
+    wb = g["r_rot"] / g["re_equiv"]
+    ofrac = g["rho_frac"]
+    for b in range(n_grids):
+        ax, ay, az = _box_axes(nx, ny, nz, grid_minvals[b, 0], grid_maxvals[b, 0],
+                               grid_minvals[b, 1], grid_maxvals[b, 1], grid_minvals[b, 2],
+                               grid_maxvals[b, 2])
+        az, ay, ax = az - g["zdip"], ay - g["ydip"], ax - g["xdip"]
+        shp = (nz, ny, nx)
+        ax, ay, az = (np.broadcast_to(a, shp).astype(F) for a in (ax, ay, az))
+        ar = np.sqrt(ax * ax + ay * ay + az * az)
+        full = (ar >= F(1.5) * wb) & (ax < F(0.))
+        ramp = (ar > wb) & (ar < F(1.5) * wb) & (ax < F(0.))
+        d = ((ar - wb) / (F(0.5) * wb)).astype(F)
+
+        def sc(v, m):   # an added term, times dfrac on the ramp
+            return np.where(m, v * d, v) if np.ndim(v) else np.where(m, F(v) * d, F(v))
+
+        for m, on in ((full, full), (ramp, ramp)):
+            if not on.any():
+                continue
+            rm = m & ramp
+
+            def add(key, term):
+                f[key][b] = np.where(m, sc(term, rm) + f[key][b], f[key][b])
+
+            add("qrho", g["srho"])
+            add("qpx", g["spx"])
+            add("qpy", g["spy"])
+            add("qpz", g["spz"])
+            qr = f["qrho"][b]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                avx, avy, avz = f["qpx"][b] / qr, f["qpy"][b] / qr, f["qpz"][b] / qr
+            apres = g["cs_wind"] * g["cs_wind"] * qr / g["gamma"]
+            add("qpresx", F(0.5) * apres)
+            add("epres", F(0.5) * apres / g["ti_te"])
+            add("hrho", g["srho"] * g["rho_frac"] * g["rmassh"] / g["rmassq"])
+            add("hpresx", F(0.5) * g["spress"] * g["rho_frac"])
+            add("orho", g["srho"] * ofrac * g["rmasso"] / g["rmassq"])
+            add("opresx", F(0.5) * g["spress"] * ofrac)
+            for sp in "qho":
+                for c in "yz":
+                    f[sp + "pres" + c][b] = np.where(m, f[sp + "presx"][b],
+                                                     f[sp + "pres" + c][b])
+                for c in ("xy", "xz", "yz"):
+                    f[sp + "pres" + c][b] = np.where(m, F(0.), f[sp + "pres" + c][b])
+            for sp in "ho":
+                for c, av in zip("xyz", (avx, avy, avz)):
+                    f[sp + "p" + c][b] = np.where(m, av * f[sp + "rho"][b], f[sp + "p" + c][b])
+        for sp in "qho":
+            r = f[sp + "rho"][b]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                vx, vy, vz = f[sp + "px"][b] / r, f[sp + "py"][b] / r, f[sp + "pz"][b] / r
+                spd = np.sqrt(vx * vx + vy * vy + vz * vz)
+            fast = spd > F(1.)
+            for c in "xyz":
+                f[sp + "p" + c][b] = np.where(fast, F(0.), f[sp + "p" + c][b])
+    set_resist(resistive, nx, ny, nz, mbndry, s["resist"], ijzero, numzero, ijmid, nummid,
+               ijsrf, numsrf, msrf, mmid, mzero, F(1.))
+    return found

@@ -132,3 +132,52 @@ def max_rel(a, b):
     b = np.asarray(b, np.float64)
     scale = max(np.abs(b).max(), 1e-30)
     return float(np.abs(a - b).max() / scale)
+
+
+# --- the initialisation block (code/3d_main.f90:1245-1627) ----------------------
+
+INIT_SCALARS = ("rmassq", "rmassh", "rmasso", "ti_te", "re_equiv", "r_inner", "zero_bndry",
+                "mid_bndry", "srf_bndry", "alpha_e", "erho", "d_min", "cs_inner", "srho", "spx",
+                "spy", "spz", "spress", "cs_wind", "gamma", "rho_frac", "resist", "v_rot",
+                "r_rot", "rot_angle", "xdip", "ydip", "zdip", "sin_tilt", "cos_tilt", "b0")
+
+
+def init_case(nx=21, ny=19, nz=13, ngrd=2, erho=0.0, mcap=400):
+    """Inputs of the initialisation block for nested boxes centred on the
+    planet, with corotation, a tilted and rotated dipole and a solar wind
+    whose inner edge (r_rot/re_equiv = 6) lies inside the outer box."""
+    s = dict(rmassq=1.0, rmassh=1.0, rmasso=16.0, ti_te=1.0, re_equiv=0.2, r_inner=3.0,
+             zero_bndry=1.5, mid_bndry=2.5, srf_bndry=3.6, alpha_e=6.0, erho=erho, d_min=0.01,
+             cs_inner=0.05, srho=0.5, spx=0.505, spy=0.0, spz=0.01, spress=0.02, cs_wind=0.3,
+             gamma=1.6666, rho_frac=0.001, resist=64.0, v_rot=0.02, r_rot=1.2, rot_angle=0.3,
+             xdip=0.00001, ydip=0.0, zdip=0.0, sin_tilt=float(F(np.sin(0.2))),
+             cos_tilt=float(F(np.cos(0.2))), b0=270.0)
+    s = {k: float(F(v)) for k, v in s.items()}
+    xs = np.array([2.0 ** b for b in range(ngrd)], F)
+    gmin = np.stack([-(nx - 1) / 2 * xs, -(ny - 1) / 2 * xs, -(nz - 1) / 2 * xs], 1).astype(F)
+    gmax = np.stack([(nx - 1) / 2 * xs, (ny - 1) / 2 * xs, (nz - 1) / 2 * xs], 1).astype(F)
+    shp = (ngrd, nz, ny, nx)
+    f = {k: np.full(shp, 7.0, F) for k in FIELDS}
+    b0 = [np.full(shp, 7.0, F) for _ in range(3)]
+    mb = 1
+    rst = np.full((mb, nz, ny, nx), 7.0, F)
+    lists = dict(parm_srf=np.zeros((mcap, 7, mb), F), parm_mid=np.zeros((mcap, 7, mb), F),
+                 parm_zero=np.zeros((mcap, 7, mb), F), ijsrf=np.zeros((mcap, 3, mb), np.int32),
+                 ijmid=np.zeros((mcap, 3, mb), np.int32), ijzero=np.zeros((mcap, 3, mb), np.int32),
+                 numsrf=np.zeros(mb, np.int32), nummid=np.zeros(mb, np.int32),
+                 numzero=np.zeros(mb, np.int32))
+    return dict(nx=nx, ny=ny, nz=nz, ngrd=ngrd, mbndry=mb, mcap=mcap, s=s, gmin=gmin, gmax=gmax,
+                f=f, b0=b0, rst=rst, lists=lists)
+
+
+def init_oracle(c):
+    """Runs the oracle's init_plasma on a copy of an init_case; returns the
+    copy and the list lengths found."""
+    from oracle import restate as O
+    d = copy.deepcopy(c)
+    L = d["lists"]
+    found = O.init_plasma(d["f"], *d["b0"], d["rst"], d["nx"], d["ny"], d["nz"], d["ngrd"],
+                          d["mbndry"], d["mcap"], d["mcap"], d["mcap"], d["gmin"], d["gmax"],
+                          L["parm_srf"], L["parm_mid"], L["parm_zero"], L["ijsrf"], L["ijmid"],
+                          L["ijzero"], L["numsrf"], L["nummid"], L["numzero"], d["s"])
+    return d, found

@@ -193,3 +193,129 @@ def test_divb_fp64_oracle_matches_single_precision_solve():
         d = np.abs(wo.a[k][0].astype(np.float64) - r).max()
         assert d <= 1e-4 * np.abs(r).max(), (k, d)
         assert not np.allclose(r, w.a[k][0].astype(np.float64))   # the correction did something
+
+
+# --- the initialisation block (code/3d_main.f90:1245-1627) -----------------------
+
+def _positions(d, b):
+    F = np.float32
+    s = d["s"]
+    nx, ny, nz = d["nx"], d["ny"], d["nz"]
+    g0, g1 = d["gmin"][b], d["gmax"][b]
+    ax = (g0[0] + (g1[0] - g0[0]) / (F(nx) - F(1.)) * np.arange(nx, dtype=F)) - F(s["xdip"])
+    ay = (g0[1] + (g1[1] - g0[1]) / (F(ny) - F(1.)) * np.arange(ny, dtype=F)) - F(s["ydip"])
+    az = (g0[2] + (g1[2] - g0[2]) / (F(nz) - F(1.)) * np.arange(nz, dtype=F)) - F(s["zdip"])
+    return np.meshgrid(az, ay, ax, indexing="ij")[::-1]
+
+
+def test_init_dipole_closed_form():
+    """b0 against the textbook dipole (3(m.r)r - m r^2)/r^5 in float64, the
+    moment b0 along the tilted, rotated axis (sin t cos a, sin t sin a,
+    cos t), r from the doubly shifted centre the driver passes; zero
+    inside zero_bndry."""
+    from common import init_case, init_oracle
+    d, _ = init_oracle(init_case())
+    s = d["s"]
+    a, st, ct = s["rot_angle"], s["sin_tilt"], s["cos_tilt"]
+    m = s["b0"] * np.array([st * np.cos(a), st * np.sin(a), ct])
+    for b in range(d["ngrd"]):
+        ax, ay, az = (p.astype(np.float64) for p in _positions(d, b))
+        r = np.stack([ax - s["xdip"], ay - s["ydip"], az - s["zdip"]])
+        rr = np.sqrt((r * r).sum(0))
+        mr = np.tensordot(m, r, 1)
+        want = (3 * mr * r - m[:, None, None, None] * rr ** 2) / rr ** 5
+        got = np.stack([d["b0"][q][b] for q in range(3)]).astype(np.float64)
+        out = rr > s["zero_bndry"] + 0.2
+        assert np.all(got[:, rr < s["zero_bndry"] - 0.2] == 0)
+        err = np.abs(got - want)[:, out].max() / np.abs(want[:, out]).max()
+        assert err < 1e-5, err
+
+
+def test_init_lists_brute_force():
+    """The boundary-point lists: a scalar loop in the reference's k, j, i
+    order classifying each cell of box 1 by its dipole-space radius, and
+    each list entry's saved values equal to the final fields there (the
+    points lie far inside the solar-wind edge)."""
+    from common import init_case, init_oracle
+    d, found = init_oracle(init_case())
+    s = d["s"]
+    F = np.float32
+    sr, cr = O._sinf(s["rot_angle"]), O._cosf(s["rot_angle"])
+    ax, ay, az = _positions(d, 0)
+    want = {1: [], 2: [], 3: []}
+    nz, ny, nx = ax.shape
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                x, y, z = ax[k, j, i], ay[k, j, i], az[k, j, i]
+                xr = x * cr + y * sr
+                yr = -(x * sr) + y * cr
+                xp = xr * F(s["cos_tilt"]) - z * F(s["sin_tilt"])
+                zp = xr * F(s["sin_tilt"]) + z * F(s["cos_tilt"])
+                ar = np.sqrt(xp * xp + yr * yr + zp * zp)
+                if ar <= F(s["srf_bndry"]):
+                    q = 3 if ar < F(s["zero_bndry"]) else (2 if ar < F(s["mid_bndry"]) else 1)
+                    want[q].append((i + 1, j + 1, k + 1))
+    L = d["lists"]
+    keys = ("qrho", "hrho", "orho", "qpresx", "hpresx", "opresx", "epres")
+    for q, nm in ((1, "srf"), (2, "mid"), (3, "zero")):
+        n = int(L["num" + nm][0])
+        assert n == len(want[q]) > 0
+        got = [tuple(int(v) for v in L["ij" + nm][e, :, 0]) for e in range(n)]
+        assert got == want[q]
+        for e, (i, j, k) in enumerate(got):
+            vals = [d["f"][kk][0, k - 1, j - 1, i - 1] for kk in keys]
+            assert list(L["parm_" + nm][e, :, 0]) == vals
+    assert tuple(found[0]) == tuple(len(want[q]) for q in (1, 2, 3))
+
+
+def test_init_plasma_relations():
+    """Relations the block fixes: isotropic pressure with zero off-diagonal
+    terms; h and o densities 0.05 m_h/m_q and m_o/m_q of q away from the
+    wind; rigid corotation |v| = v_rot |r_xy| re_equiv inside r_rot; the
+    equatorial density d_min m_q with no ionospheric source (erho = 0); the
+    full-wind region adds exactly srho to the ambient density; no speed
+    above 1 survives the speed check."""
+    from common import init_case, init_oracle
+    d, _ = init_oracle(init_case(erho=0.0))
+    s = d["s"]
+    f = d["f"]
+    for sp in "qho":
+        assert np.array_equal(f[sp + "presx"], f[sp + "presy"])
+        assert np.array_equal(f[sp + "presx"], f[sp + "presz"])
+        for c in ("xy", "xz", "yz"):
+            assert not f[sp + "pres" + c].any()
+        v = np.sqrt(sum((f[sp + "p" + c] / f[sp + "rho"]).astype(np.float64) ** 2 for c in "xyz"))
+        assert v.max() <= 1.0
+    # the wind (0.505 / 0.5 upstream) trips the speed check in part of it
+    wind = f["qrho"] > 0.3
+    stopped = wind & (f["qpx"] == 0)
+    assert 0 < stopped.sum() < wind.sum()
+    wb = s["r_rot"] / s["re_equiv"]
+    for b in range(d["ngrd"]):
+        ax, ay, az = (p.astype(np.float64) for p in _positions(d, b))
+        r = np.sqrt(ax ** 2 + ay ** 2 + az ** 2)
+        amb = ~((r > wb) & (ax < 0))
+        q, h, o = (f[k][b].astype(np.float64) for k in ("qrho", "hrho", "orho"))
+        assert np.allclose(h[amb], 0.05 * q[amb], rtol=1e-6)
+        assert np.allclose(o[amb], 16.0 * q[amb], rtol=1e-6)
+        rd = np.sqrt(ax ** 2 + ay ** 2) * s["re_equiv"]
+        rig = amb & (rd < s["r_rot"] * 0.9)
+        vq = np.sqrt((f["qpx"][b] / f["qrho"][b]).astype(np.float64) ** 2 +
+                     (f["qpy"][b] / f["qrho"][b]).astype(np.float64) ** 2)
+        assert np.allclose(vq[rig], s["v_rot"] * rd[rig], rtol=1e-5, atol=1e-9)
+        eq = amb & (np.abs(az) < 0.2) & (np.abs(ax) < 2.0)
+        if eq.any():
+            assert np.allclose(q[eq], s["d_min"] * s["rmassq"], rtol=1e-6)
+        full = (r >= 1.5 * wb + 1e-3) & (ax < -1e-3)
+        if full.any():
+            d0, _ = init_oracle(init_case(erho=0.0) | {"s": dict(s, srho=0.0, spx=0.0, spz=0.0,
+                                                                   spress=0.0, cs_wind=0.0)})
+            q0 = d0["f"]["qrho"][b].astype(np.float64)
+            assert np.allclose(q[full] - q0[full], s["srho"], rtol=1e-5)
+            # the ramp wb < r < 1.5 wb adds srho (r - wb) / (wb / 2)
+            ramp = (r > wb + 1e-3) & (r < 1.5 * wb - 1e-3) & (ax < -1e-3)
+            if ramp.any():
+                dfrac = (r[ramp] - wb) / (0.5 * wb)
+                assert np.allclose(q[ramp] - q0[ramp], s["srho"] * dfrac, rtol=1e-4,
+                                   atol=1e-6)

@@ -287,3 +287,124 @@ def test_findgrid():
         lib().call("findgrid", p, ngrd, mn, mx, xs, a, re, R.Chars("craft01", 8))
         O.findgrid(p, ngrd, mn, mx, xs, b, re)
         assert np.array_equal(a, b), (p, a, b)
+
+
+# --- initialisation: dipole, mak_dip_grd, set_resist --------------------------
+
+ROT = dict(rot_angle=0.7, xdip=0.00001, ydip=-0.3, zdip=0.2,
+           sin_tilt=float(np.float32(np.sin(0.35))), cos_tilt=float(np.float32(np.cos(0.35))),
+           b0=270.0)
+
+
+def _set_rotation(**kw):
+    rot = lib().rotation()
+    for k, v in kw.items():
+        setattr(rot, k, v)
+
+
+def test_dipole_points():
+    """The dipole at 2000 points, rotated and tilted: bit for bit."""
+    _set_rotation(**ROT)
+    rng = np.random.default_rng(41)
+    pts = rng.uniform(-12, 12, (2000, 3)).astype(np.float32)
+    for p in pts:
+        r = [np.zeros(1, np.float32) for _ in range(3)]
+        o = [np.zeros(1, np.float32) for _ in range(3)]
+        lib().call("dipole", *r, p[0], p[1], p[2])
+        O.dipole(*o, p[0], p[1], p[2], **ROT)
+        assert [x[0] for x in r] == [x[0] for x in o], p
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_mak_dip_grd(box):
+    _set_rotation(**ROT)
+    nx, ny, nz, ng, mb = 13, 11, 9, 2, 1
+    w = World(nx, ny, nz, ng, seed=3, lists=False)
+    shp = (ng, nz, ny, nx)
+    ij = np.zeros((4, 3, mb), np.int32)
+    ij[:, :, 0] = [[7, 6, 5], [1, 1, 1], [13, 11, 9], [2, 3, 4]]
+    num = np.array([3], np.int32)
+    outs = []
+    for call in (lambda *a: lib().call("mak_dip_grd", *a), lambda *a: O.mak_dip_grd(*a, **ROT)):
+        b = [np.full(shp, 5.0, np.float32) for _ in range(3)]
+        call(*b, nx, ny, nz, ng, mb, box, ij, num, 4, np.float32(3.0),
+             w.a["grd_xmin"], w.a["grd_xmax"], w.a["grd_ymin"], w.a["grd_ymax"],
+             w.a["grd_zmin"], w.a["grd_zmax"])
+        outs.append(b)
+    for r, o in zip(*outs):
+        assert np.array_equal(r, o)
+    assert np.abs(outs[0][2][box - 1]).max() > 0
+    if box == 1:   # the first numzero listed points are zeroed, the fourth is not
+        assert outs[0][0][0, 4, 5, 6] == 0.0 and outs[0][0][0, 3, 2, 1] != 0.0
+
+
+def test_set_resist():
+    nx, ny, nz, mb = 9, 8, 7, 2
+    rng = np.random.default_rng(8)
+
+    def lst(n, m):
+        a = np.zeros((m, 3, mb), np.int32)
+        a[:, 0, :] = rng.integers(1, nx + 1, (m, mb))
+        a[:, 1, :] = rng.integers(1, ny + 1, (m, mb))
+        a[:, 2, :] = rng.integers(1, nz + 1, (m, mb))
+        return a
+
+    ijz, ijm, ijs = lst(0, 6), lst(0, 9), lst(0, 12)
+    nz_, nm_, ns_ = (np.array([2, 5], np.int32), np.array([9, 4], np.int32),
+                     np.array([1, 11], np.int32))
+    outs = []
+    for fn in (lambda *a: lib().call("set_resist", *a), O.set_resist):
+        rst = np.full((mb, nz, ny, nx), 3.0, np.float32)
+        fn(rst, nx, ny, nz, mb, np.float32(64.0), ijz, nz_, ijm, nm_, ijs, ns_, 12, 9, 6,
+           np.float32(1.0))
+        outs.append(rst)
+    assert np.array_equal(*outs)
+    assert (outs[0] != 0).sum() > 10
+
+
+def test_init_plasma_field_and_resist_vs_reference():
+    """The oracle's initialisation block against the reference's own pieces:
+    b0 of every box equals the reference dipole subroutine called as the
+    driver calls it (at positions already shifted by the dipole centre,
+    code/3d_main.f90:1256-1274) wherever r >= zero_bndry, and the
+    resistivity equals the reference set_resist on the lists produced."""
+    from common import init_case, init_oracle
+    c = init_case()
+    d, _ = init_oracle(c)
+    s = d["s"]
+    _set_rotation(**{k: s[k] for k in ("rot_angle", "xdip", "ydip", "zdip", "sin_tilt",
+                                        "cos_tilt", "b0")})
+    F = np.float32
+    nx, ny, nz = d["nx"], d["ny"], d["nz"]
+    rng = np.random.default_rng(2)
+    sr, cr = O._sinf(s["rot_angle"]), O._cosf(s["rot_angle"])
+    checked = 0
+    for b in range(d["ngrd"]):
+        g0, g1 = d["gmin"][b], d["gmax"][b]
+        dx, dy, dz = ((g1[a] - g0[a]) / (F(n) - F(1.)) for a, n in ((0, nx), (1, ny), (2, nz)))
+        for _ in range(300):
+            i, j, k = rng.integers(0, nx), rng.integers(0, ny), rng.integers(0, nz)
+            ax = g0[0] + dx * F(i) - F(s["xdip"])
+            ay = g0[1] + dy * F(j) - F(s["ydip"])
+            az = g0[2] + dz * F(k) - F(s["zdip"])
+            xr = ax * cr + ay * sr
+            yr = -(ax * sr) + ay * cr
+            xp = xr * F(s["cos_tilt"]) - az * F(s["sin_tilt"])
+            zp = xr * F(s["sin_tilt"]) + az * F(s["cos_tilt"])
+            ar = np.sqrt(xp * xp + yr * yr + zp * zp)
+            r = [np.zeros(1, F) for _ in range(3)]
+            lib().call("dipole", *r, ax, ay, az)
+            got = [d["b0"][q][b, k, j, i] for q in range(3)]
+            if ar < F(s["zero_bndry"]):
+                assert got == [0.0, 0.0, 0.0]
+            else:
+                assert got == [x[0] for x in r]
+                checked += 1
+    assert checked > 400
+    L = d["lists"]
+    rst = np.zeros_like(d["rst"])
+    lib().call("set_resist", rst, nx, ny, nz, 1, F(s["resist"]), L["ijzero"], L["numzero"],
+               L["ijmid"], L["nummid"], L["ijsrf"], L["numsrf"], d["mcap"], d["mcap"], d["mcap"],
+               F(1.0))
+    assert np.array_equal(rst, d["rst"])
+    assert L["numzero"][0] > 0 and L["nummid"][0] > 0 and L["numsrf"][0] > 0
