@@ -1122,7 +1122,7 @@ void g3m3_init_plasma(
     P.rho_frac = *rho_frac;
     P.wind_bnd = *r_rot / *re_equiv;   // code/3d_main.f90:1467
     launch_init_plasma(P, ng, g_stream);
-    // code/3d_main.f90:1624-1626
+    // code/3d_main.f90:1631-1633
     const int most = std::max(*msrf, std::max(*mmid, *mzero));
     launch_set_resist(P.B, R(resistive), mb, *resist, 1.0f, P.ijzero, P.ijmid, P.ijsrf,
                       P.counts + 3 * (mb - 1), nullptr, most, g_stream);

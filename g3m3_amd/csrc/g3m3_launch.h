@@ -79,7 +79,7 @@ struct RotationGeom {
     float xdip, ydip, zdip, sin_rot, cos_rot, sin_tilt, cos_tilt, b0;
 };
 constexpr int kMaxInitGrids = 32;
-// the initialisation block (code/3d_main.f90:1245-1627): fields in the
+// the initialisation block (code/3d_main.f90:1238-1633): fields in the
 // order q, h, o x (rho, presx, presy, presz, px, py, pz, presxy, presxz,
 // presyz), epres, bx, by, bz, each (nx,ny,nz,n_grids); the point lists of
 // boxes 1..mbndry as the reference lays them out, ij(mbndry,3,m) and

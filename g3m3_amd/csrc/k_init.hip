@@ -10,7 +10,7 @@
 // IEEE single precision with no fused multiply-add.  Integer powers x**n
 // are expanded left to right, ((x*x)*x)*..., as flang expands them.  The
 // one real power, the ionospheric profile ((r+r_i/2)/(1.5 r_i))**(-alpha_e)
-// (code/3d_main.f90:1304-1305), is evaluated in double precision and
+// (code/3d_main.f90:1311-1312), is evaluated in double precision and
 // rounded once; a C library's powf may differ from it by an ulp.
 #include "g3m3_dev.h"
 #include "g3m3_launch.h"
@@ -151,7 +151,7 @@ void launch_set_resist(const Box& B, float* rst, int mbndry, float resist, float
 }
 
 // ---------------------------------------------------------------------------
-// The initialisation block (code/3d_main.f90:1245-1627).  Three passes:
+// The initialisation block (code/3d_main.f90:1238-1633).  Three passes:
 // the ambient plasma and dipole on every cell of every box, with each cell
 // of a box <= mbndry classified as a zero / mid / surface boundary point;
 // the ordered lists of those points with their ambient values; the solar
@@ -178,12 +178,12 @@ __global__ void k_init_ambient(InitPlasma P) {
     const float yp = yr;
     const float zp = xr * R.sin_tilt + az * R.cos_tilt;
     const float ar = sqrtf(xp * xp + yp * yp + zp * zp);
-    // the dipole subroutine subtracts the dipole centre again (:1272-1273)
+    // the dipole subroutine subtracts the dipole centre again (:1279-1281)
     float abx, aby, abz;
     dipole_field(ax - R.xdip, ay - R.ydip, az - R.zdip, R.sin_rot, R.cos_rot, R.sin_tilt,
                  R.cos_tilt, R.b0, abx, aby, abz);
     if (ar < P.zero_bndry) abx = aby = abz = 0.0f;
-    // corotation (:1285-1296)
+    // corotation (:1293-1303)
     const float rx = ax * P.re_equiv;
     const float ry = ay * P.re_equiv;
     const float rd = sqrtf(rx * rx + ry * ry);
@@ -194,7 +194,7 @@ __global__ void k_init_ambient(InitPlasma P) {
     }
     const float rvy = rx * P.v_rot * vfrac;
     const float rvx = -(ry * P.v_rot * vfrac);
-    // ionosphere (:1300-1311); the sphere terms of :1313-1332 feed nothing
+    // ionosphere (:1306-1316); the sphere terms of :1318-1332 feed nothing
     float ar_iono = sqrtf(xp * xp + ay * ay + 1.25f * (zp * zp));
     ar_iono = fmaxf(0.0001f, ar_iono);
     float ra = powd((ar_iono + 0.5f * P.r_inner) / (1.5f * P.r_inner), -P.alpha_e);
@@ -204,7 +204,7 @@ __global__ void k_init_ambient(InitPlasma P) {
     const float rho_iono = fmaxf(P.erho * ra, P.d_min);
     const float arho = fmaxf(rho_iono, P.d_min);
     const float vt = fmaxf(sqrtf(rvy * rvy + rvx * rvx), P.cs_inner);
-    // species (:1341-1379)
+    // species (:1334-1384)
     const float qrho = arho * P.rmassq / zheight;
     const float qpres = arho * vt;
     const float hrho = 0.05f * qrho * P.rmassh / P.rmassq;
@@ -235,7 +235,7 @@ __global__ void k_init_ambient(InitPlasma P) {
     P.b0f[0][c] = abx;
     P.b0f[1][c] = aby;
     P.b0f[2][c] = abz;
-    // boundary points of boxes <= mbndry (:1384-1455)
+    // boundary points of boxes <= mbndry (:1389-1452)
     if (box < P.mbndry) {
         unsigned char cls = kClsNone;
         if (ar <= P.srf_bndry) cls = (ar < P.zero_bndry) ? kClsZero
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(1024) k_init_lists(InitPlasma P) {
     }
 }
 
-// solar wind (:1466-1590) and the speed check (:1592-1621)
+// solar wind (:1464-1587) and the speed check (:1589-1626)
 __global__ void k_init_wind(InitPlasma P) {
     int i, j, k;
     const Box& B = P.B;

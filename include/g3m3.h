@@ -417,11 +417,11 @@ void g3m3_set_resist(float* rst, const int* nx, const int* ny, const int* nz, co
                      const int* ijmid, const int* nummid, const int* ijsrf, const int* numsrf,
                      const int* msrf, const int* mmid, const int* mzero, const float* b0);
 
-/* code/3d_main.f90:1245-1627 -- the main program's initialisation block,
+/* code/3d_main.f90:1238-1633 -- the main program's initialisation block,
  * which has no subroutine of its own: the dipole field b0 of every box, the
  * ambient ionospheric plasma, the inner-boundary point lists of boxes
  * 1..mbndry with their saved values, the solar wind beyond r_rot/re_equiv,
- * the speed check, and set_resist(resistive, ..., b0 = 1.) (:1624-1626).
+ * the speed check, and set_resist(resistive, ..., b0 = 1.) (:1631-1633).
  * Fields come in the order q, h, o x (rho, presx, presy, presz, px, py, pz,
  * presxy, presxz, presyz), then epres, bx, by, bz, bx0, by0, bz0, each
  * (nx,ny,nz,n_grids); resistive is (nx,ny,nz,mbndry).  grid_minvals and
@@ -429,7 +429,7 @@ void g3m3_set_resist(float* rst, const int* nx, const int* ny, const int* nz, co
  * written on the device; their counts num*(mbndry) are host outputs (the
  * call synchronises).  A list longer than its capacity (msrf, mmid, mzero)
  * is cut there and the error latch is set.  Scalars are the driver's
- * variables of the same names; rot_angle is the one computed at :1217. */
+ * variables of the same names; rot_angle is the one computed at :1218. */
 void g3m3_init_plasma(
     float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
     float* qpresxy, float* qpresxz, float* qpresyz, float* hrho, float* hpresx, float* hpresy,

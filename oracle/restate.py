@@ -1220,7 +1220,7 @@ def divB_correct_f64(bx, by, bz, rx):
 # ---------------------------------------------------------------------------
 # Initialisation: the dipole field (code/sub_dipole.f90), mak_dip_grd
 # (code/sub_mak_dip_grd.f90), set_resist (code/sub_set_resist.f90) and the
-# main program's initialisation block (code/3d_main.f90:1245-1627).
+# main program's initialisation block (code/3d_main.f90:1238-1633).
 # sin / cos / pow of single-precision values are the C library's sinf,
 # cosf and powf, which a compiled reference calls; integer powers x**n are
 # expanded left to right, ((x*x)*x)*..., as flang expands them.
@@ -1343,7 +1343,7 @@ def set_resist(rst, nx, ny, nz, mbndry, resist, ijzero, numzero, ijmid, nummid, 
 def init_plasma(f, bx0, by0, bz0, resistive, nx, ny, nz, n_grids, mbndry, msrf, mmid, mzero,
                 grid_minvals, grid_maxvals, parm_srf, parm_mid, parm_zero, ijsrf, ijmid, ijzero,
                 numsrf, nummid, numzero, s):
-    """code/3d_main.f90:1245-1627, the main program's initialisation block.
+    """code/3d_main.f90:1238-1633, the main program's initialisation block.
     ``f`` maps the 34 state-field names to (n_grids, nz, ny, nx) arrays;
     ``s`` holds the scalars by their reference names (rmassq ... b0, with
     rot_angle and the common /rotation/ values).  Lists and counts are
@@ -1410,7 +1410,7 @@ def init_plasma(f, bx0, by0, bz0, resistive, nx, ny, nz, n_grids, mbndry, msrf, 
         for c in "xyz":
             f["b" + c][b] = F(0.)
         if b < mbndry:
-            # code/3d_main.f90:1384-1455: loop order k, j, i = memory order
+            # code/3d_main.f90:1389-1452: loop order k, j, i = memory order
             cls = np.zeros(shp, np.int8)
             near = ar <= g["srf_bndry"]
             cls[near] = 1
@@ -1431,7 +1431,7 @@ def init_plasma(f, bx0, by0, bz0, resistive, nx, ny, nz, n_grids, mbndry, msrf, 
                 for a in range(7):
                     parm[:len(idx), a, b] = vals[a][idx]
                 (numsrf, nummid, numzero)[q][b] = len(idx)
-    # solar wind (:1466-1590) and the speed check (:1592-1621)
+    # solar wind (:1464-1587) and the speed check (:1589-1626)
     wb = g["r_rot"] / g["re_equiv"]
     ofrac = g["rho_frac"]
     for b in range(n_grids):

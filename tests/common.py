@@ -134,7 +134,7 @@ def max_rel(a, b):
     return float(np.abs(a - b).max() / scale)
 
 
-# --- the initialisation block (code/3d_main.f90:1245-1627) ----------------------
+# --- the initialisation block (code/3d_main.f90:1238-1633) ----------------------
 
 INIT_SCALARS = ("rmassq", "rmassh", "rmasso", "ti_te", "re_equiv", "r_inner", "zero_bndry",
                 "mid_bndry", "srf_bndry", "alpha_e", "erho", "d_min", "cs_inner", "srho", "spx",

@@ -3,7 +3,7 @@ set_resist and the main program's initialisation block -- against the
 oracle on the same inputs.
 
 Bar: bit-exact, except the ionospheric profile's real power
-((r + r_i/2) / (1.5 r_i))**(-alpha_e) (code/3d_main.f90:1304-1305), which
+((r + r_i/2) / (1.5 r_i))**(-alpha_e) (code/3d_main.f90:1311-1312), which
 the device evaluates in double precision and rounds once while the
 reference calls the C library's powf: fields that depend on it agree to
 4 ulp (relative 5e-7); with no ionospheric source (erho = 0) the power

@@ -195,7 +195,7 @@ def test_divb_fp64_oracle_matches_single_precision_solve():
         assert not np.allclose(r, w.a[k][0].astype(np.float64))   # the correction did something
 
 
-# --- the initialisation block (code/3d_main.f90:1245-1627) -----------------------
+# --- the initialisation block (code/3d_main.f90:1238-1633) -----------------------
 
 def _positions(d, b):
     F = np.float32
