@@ -118,11 +118,26 @@ def batch_case(n: int = 32, steps: int = 20, streams: int = 4):
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     cu_batch = b.cell_updates
+    del b
+    # the same runs replaying captured step graphs, time step on the device
+    g = Batch(wls, streams=streams)
+    g.init_dt()
+    g.graph_steps(0)            # capture and instantiate, outside the timing
+    torch.cuda.synchronize()
+    cu0 = g.cell_updates
+    t0 = time.perf_counter()
+    g.graph_steps(steps)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    cu_graph = g.cell_updates - cu0
+    plain = cu_plain / t_plain
     return {"runs": n, "steps": steps, "streams": streams,
             "cells_min_max": [min(w.cells_per_box for w in wls), max(w.cells_per_box for w in wls)],
-            "plain_cell_updates_per_s": cu_plain / t_plain,
+            "plain_cell_updates_per_s": plain,
             "batched_cell_updates_per_s": cu_batch / t_batch,
-            "speedup": (cu_batch / t_batch) / (cu_plain / t_plain)}
+            "speedup": (cu_batch / t_batch) / plain,
+            "graph_cell_updates_per_s": cu_graph / t_graph,
+            "graph_speedup": (cu_graph / t_graph) / plain}
 
 
 def precision_case(fast_mode: int):

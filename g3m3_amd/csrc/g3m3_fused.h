@@ -51,6 +51,10 @@ struct PushParams {
     float xmin, ymin, zmin, ddx, ddy, ddz, re_equiv, grav, gamma, gamma1;
     int do_elec;
     int copy_shell;             // copy non-interior cells old -> out
+    // time step from device memory (captured steps): delt = dt_scale *
+    // *dt_dev, and bmax is formed from it; null: delt and bmax above
+    const float* dt_dev;
+    float dt_scale;
 };
 
 struct ValEntry {
@@ -82,6 +86,7 @@ struct LapParams {
     const float* b_src[3];
     float* b_dst[3];
     float chirho, chipxyz, chierg, delt, smallbit;
+    const float* dt_dev;     // non-null: delt = *dt_dev (captured steps)
 };
 
 struct FctParams {

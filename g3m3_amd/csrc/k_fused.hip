@@ -184,6 +184,12 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
 // power of two (P2), x / d is computed as x * (1/d): the reciprocal is
 // exact and both operations are correctly rounded, so the result is the
 // same float; otherwise a true division.
+// the push's time step: from device memory in captured steps (the same
+// float the host would pass: 0.5 dt or dt)
+__device__ inline float step_delt(const PushParams& P) {
+    return P.dt_dev ? P.dt_scale * *P.dt_dev : P.delt;
+}
+
 template <bool P2>
 __device__ inline float dvp(float x, float d, float r) {
     return P2 ? x * r : x / d;
@@ -197,12 +203,13 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     const SpeciesSet& W = P.src[s];
     const SpeciesSet& O = P.old[s];
     const float rmass = P.rmass[s], rrmass = P.rrmass[s];
-    const float delt = P.delt, reynolds = P.reynolds;
+    const float delt = step_delt(P), reynolds = P.reynolds;
     auto Dx = [&](float x) { return dvp<P2>(x, P.dxt, P.rdxt); };
     auto Dy = [&](float x) { return dvp<P2>(x, P.dyt, P.rdyt); };
     auto Dz = [&](float x) { return dvp<P2>(x, P.dzt, P.rdzt); };
     auto Dm = [&](float x) { return dvp<P2>(x, rmass, rrmass); };
-    const float bmax = P.bmax[s];    // 0.20*rmass/delt/reynolds (code/sub_push_ion.f90:64)
+    // 0.20*rmass/delt/reynolds (code/sub_push_ion.f90:64)
+    const float bmax = P.dt_dev ? 0.20f * rmass / delt / reynolds : P.bmax[s];
     const float* __restrict__ wr = W.f[0];
     const float* __restrict__ wx = W.f[1];
     const float* __restrict__ wy = W.f[2];
@@ -319,7 +326,7 @@ template <bool P2>
 __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k) {
     const Box& B = P.B;
     const int c = B.IX(i, j, k), sy = B.nx, sz = B.nx * B.ny;
-    const float delt = P.delt;
+    const float delt = step_delt(P);
     auto Dx = [&](float x) { return dvp<P2>(x, P.fdxt, P.rfdxt); };
     auto Dy = [&](float x) { return dvp<P2>(x, P.fdyt, P.rfdyt); };
     auto Dz = [&](float x) { return dvp<P2>(x, P.fdzt, P.rfdzt); };
@@ -595,7 +602,7 @@ __device__ inline void lap_body(const LapParams& L, int g, int i, int j, int k) 
     const float uym = fabsf(vy[0] - vy[4]);
     const float uzp = fabsf(vz[5] - vz[0]);
     const float uzm = fabsf(vz[0] - vz[6]);
-    const float delt = L.delt;
+    const float delt = L.dt_dev ? *L.dt_dev : L.delt;
     auto lap = [&](const float* __restrict__ q, float* w, float chi, int m) {
         const float q0 = q[c];
         const float v = q0 + chi * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +

@@ -59,6 +59,17 @@ struct Run {
     // flank shell tables of the inner boxes (geometry is fixed for a run)
     ShellPt* shell[G3M3_MAX_GRIDS] = {};
     int nshell = 0;
+    // captured full steps of a single-grid run: gexec[p] is one full step
+    // starting from rotation parity p; the time step and the run time live
+    // on the device (dt_dev[0] = the next step's dt, [1] = t_stepnew,
+    // [2] = the dt of the step last taken; t_dev = t)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    float* dt_dev = nullptr;
+    double* t_dev = nullptr;
+    float* host_buf = nullptr;    // pinned: t_stepnew and t read back
+    bool capturing = false;       // kernels read the time step from dt_dev
+    bool own_stream = false;
 
     Set slice(float* const* arrays, int box) const {
         Set S;
@@ -134,7 +145,7 @@ void efield(Run* R, const Set& src, int box) {
 }
 
 void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float delt,
-          int do_elec, int copy_shell) {
+          float dt_scale, int do_elec, int copy_shell) {
     const g3m3_run_desc& d = R->d;
     PushParams P;
     P.B = R->B;
@@ -193,6 +204,8 @@ void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float
     P.pow2 = p2 ? 1 : 0;
     P.do_elec = do_elec;
     P.copy_shell = copy_shell;
+    P.dt_dev = R->capturing ? R->dt_dev : nullptr;
+    P.dt_scale = dt_scale;
     launch_push(P, R->s);
 }
 
@@ -348,6 +361,7 @@ void lapidus(Run* R, int box, float delt) {
     L.chierg = d.chierg;
     L.delt = delt;
     L.smallbit = d.smallbit;
+    L.dt_dev = R->capturing ? R->dt_dev : nullptr;
     launch_lap(L, R->s);
 }
 
@@ -408,7 +422,7 @@ void box_step(Run* R, int box) {
     // push_elec only writes the state epres interior, which step 2
     // overwrites before any read, so it is not issued
     efield(R, ol, box);
-    push(R, wk, ol, ol, box, 0.5f * delt, 0, 0);
+    push(R, wk, ol, ol, box, 0.5f * delt, 0.5f, 0, 0);
     double tg = inner_box ? clamp_t(R->t_old[b] + (double)delt2) : 0.0;
     bounds(R, wk, wk, box, tg);
     validate(R, wk, box, false);
@@ -419,7 +433,7 @@ void box_step(Run* R, int box) {
     // So the call is not issued (results equal up to the sign of an exact
     // zero).
     efield(R, wk, box);
-    push(R, st, ol, wk, box, delt, 1, 1);
+    push(R, st, ol, wk, box, delt, 1.0f, 1, 1);
     tg = inner_box ? clamp_t(R->t_old[b] + (double)delt) : 0.0;
     bounds(R, st, wk, box, tg, false);
     validate(R, st, box, false);
@@ -456,6 +470,71 @@ void read_tstepnew(Run* R, const std::vector<int>& boxes) {
         const float st = R->d.stepsz * R->d.xspac[box - 1] / f;
         R->t_stepnew[box - 1] = (double)std::min(st, R->d.t_step_max);
     }
+}
+
+// The next time step of a single-grid run from the CFL maxima of its last
+// validation pass, as read_tstepnew forms it on the host, and the run time
+// advanced by the step just taken (code/3d_main.f90:1929-1968, 3038)
+__global__ void k_next_dt(const float* m, float stepsz, float xspac, float t_step_max,
+                          float* dt, double* t) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *t = *t + (double)dt[0];
+    dt[2] = dt[0];
+    float f = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3] + m[4] * m[4]);
+    f = (0.0f < f) ? f : 0.0f;              // std::max(0.0f, f)
+    const float st = stepsz * xspac / f;
+    const float nd = (t_step_max < st) ? t_step_max : st;   // std::min(st, t_step_max)
+    dt[1] = nd;
+    dt[0] = nd;
+}
+
+__global__ void k_set_dt(float* dt, double* t, float dt0, double t0) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    dt[0] = dt0;
+    dt[1] = dt0;
+    dt[2] = dt0;
+    *t = t0;
+}
+
+// Captures the two full steps (one per rotation parity) of a single-grid
+// run; host bookkeeping touched by the capture is restored
+bool build_graphs(Run* R) {
+    if (R->d.n_grids != 1) return false;
+    if (!R->s) {
+        if (!ok_hip(R, hipStreamCreate(&R->s), "graph stream")) return false;
+        R->own_stream = true;
+    }
+    if (!R->dt_dev) {
+        if (!ok_hip(R, hipMalloc(&R->dt_dev, 4 * sizeof(float)), "graph scalars") ||
+            !ok_hip(R, hipMalloc(&R->t_dev, sizeof(double)), "graph scalars") ||
+            !ok_hip(R, hipHostMalloc(&R->host_buf, 4 * sizeof(double)), "graph scalars"))
+            return false;
+    }
+    const std::vector<double> t_old = R->t_old, t_new = R->t_new;
+    const long long cu = R->cell_updates, bs = R->box_steps;
+    const int vpass = R->vpass;
+    bool ok = true;
+    for (int k = 0; ok && k < 2; ++k) {
+        const int p = R->rot[0];
+        R->capturing = true;
+        ok = ok_hip(R, hipStreamBeginCapture(R->s, hipStreamCaptureModeThreadLocal),
+                    "begin capture");
+        if (ok) {
+            box_step(R, 1);
+            hipLaunchKernelGGL(k_next_dt, dim3(1), dim3(64), 0, R->s, R->maxima, R->d.stepsz,
+                               R->d.xspac[0], R->d.t_step_max, R->dt_dev, R->t_dev);
+            ok = ok_hip(R, hipStreamEndCapture(R->s, &R->graph[p]), "end capture") &&
+                 ok_hip(R, hipGraphInstantiate(&R->gexec[p], R->graph[p], nullptr, nullptr, 0),
+                        "instantiate");
+        }
+        R->capturing = false;
+    }
+    R->t_old = t_old;
+    R->t_new = t_new;
+    R->cell_updates = cu;
+    R->box_steps = bs;
+    if (R->vpass != vpass) ok = false;   // a full step keeps the counter parity
+    return ok && R->err == 0;
 }
 
 }  // namespace
@@ -567,6 +646,14 @@ void g3m3_run_destroy(void* run) {
         if (p) (void)hipFree(p);
     for (ShellPt* p : R->shell)
         if (p) (void)hipFree(p);
+    for (int q = 0; q < 2; ++q) {
+        if (R->gexec[q]) (void)hipGraphExecDestroy(R->gexec[q]);
+        if (R->graph[q]) (void)hipGraphDestroy(R->graph[q]);
+    }
+    if (R->dt_dev) (void)hipFree(R->dt_dev);
+    if (R->t_dev) (void)hipFree(R->t_dev);
+    if (R->host_buf) (void)hipHostFree(R->host_buf);
+    if (R->own_stream) (void)hipStreamDestroy(R->s);
     delete R;
 }
 
@@ -659,6 +746,51 @@ int g3m3_run_finish(void* run) {
     read_tstepnew(R, R->stepped);
     R->stepped.clear();
     check(R, "finish");
+    return R->err;
+}
+
+int g3m3_run_graph_steps(void* run, int nsteps) {
+    Run* R = (Run*)run;
+    if (R->err) return R->err;
+    if (R->d.n_grids != 1) {
+        fprintf(stderr, "g3m3_run_graph_steps: single-grid runs only\n");
+        return R->err = -1;
+    }
+    if (!R->gexec[0] || !R->gexec[1]) {
+        if (!build_graphs(R)) {
+            if (!R->err) R->err = -1;
+            return R->err;
+        }
+    }
+    // this step's dt and the run time, as the host path would use them
+    const float dt0 = (float)R->t_stepnew[0];
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, R->s, R->dt_dev, R->t_dev, dt0, R->t);
+    for (int q = 0; q < nsteps; ++q) {
+        if (!ok_hip(R, hipGraphLaunch(R->gexec[(int)R->rot[0]], R->s), "graph launch")) break;
+        R->rot[0] ^= 1;
+        R->cell_updates += R->n;
+        R->box_steps += 1;
+    }
+    check(R, "graph_steps");
+    return R->err;
+}
+
+int g3m3_run_graph_finish(void* run) {
+    Run* R = (Run*)run;
+    if (!R->host_buf) return R->err;
+    double* h = (double*)R->host_buf;
+    if (ok_hip(R, hipMemcpyAsync(h, R->t_dev, sizeof(double), hipMemcpyDeviceToHost, R->s),
+               "graph finish") &&
+        ok_hip(R, hipMemcpyAsync(h + 1, R->dt_dev, 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                 R->s),
+               "graph finish") &&
+        ok_hip(R, hipStreamSynchronize(R->s), "graph finish")) {
+        R->t = h[0];
+        const float* f = (const float*)(h + 1);
+        R->t_stepnew[0] = (double)f[1];
+        R->t_step[0] = (double)f[2];
+    }
+    check(R, "graph_finish");
     return R->err;
 }
 

@@ -416,6 +416,18 @@ class FastRun(Run):
             raise _lib.G3M3Error("g3m3_run_finish failed")
         self._pull()
 
+    def graph_steps(self, nsteps: int):
+        """Issue `nsteps` full steps of a single-grid run as replays of a
+        captured HIP graph (time step kept on the device); graph_finish()
+        waits for them."""
+        if self._L.g3m3_run_graph_steps(_ct.c_void_p(self.h), int(nsteps)):
+            raise _lib.G3M3Error("g3m3_run_graph_steps failed")
+
+    def graph_finish(self):
+        if self._L.g3m3_run_graph_finish(_ct.c_void_p(self.h)):
+            raise _lib.G3M3Error("g3m3_run_graph_finish failed")
+        self._pull()
+
     def sync_arrays(self):
         """Put every box's state back in the state tensors and its old state
         in the old tensors (the runtime swaps the two per box-step)."""
@@ -523,6 +535,15 @@ class Batch:
             r.full_step_async()
         for r in self.runs:
             r.finish()
+
+    def graph_steps(self, nsteps: int):
+        """`nsteps` full steps of every run (single-grid runs), each run
+        replaying its captured step graph on its stream with no host wait
+        between steps; returns when all are done."""
+        for r in self.runs:
+            r.graph_steps(nsteps)
+        for r in self.runs:
+            r.graph_finish()
 
     @property
     def cell_updates(self) -> int:

@@ -532,6 +532,16 @@ int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt
  * that set the next step (the one host synchronisation of a full step). */
 int g3m3_run_full_step_async(void* run, const double* dt_override);
 int g3m3_run_finish(void* run);
+/* Single-grid runs (n_grids = 1): nsteps full steps replayed from HIP
+ * graphs captured on the first call (one per buffer-rotation parity).  The
+ * time step lives on the device: a kernel forms each next step from the
+ * step's CFL maxima exactly as g3m3_run_finish does on the host, so no step
+ * waits for the host and results equal g3m3_run_full_step's bit for bit.
+ * Returns once the steps are issued; g3m3_run_graph_finish waits and brings
+ * the run's times (t, t_step, t_stepnew) up to date.  A run without a
+ * stream gets one of its own. */
+int g3m3_run_graph_steps(void* run, int nsteps);
+int g3m3_run_graph_finish(void* run);
 int g3m3_run_box_step(void* run, int box);
 void g3m3_run_set_time(void* run, double t);          /* after a restart read */
 int g3m3_run_sync_arrays(void* run);

@@ -32,3 +32,51 @@ def test_batch_equals_runs_alone():
         bad = [k for k in FIELDS if not np.array_equal(ha[k], hb[k])]
         assert not bad, f"{wl.nx}x{wl.ny}x{wl.nz}: batched run differs on {bad}"
         assert ra.cell_updates == rb.cell_updates
+
+
+def _same_run(ra, rb, label):
+    for pre in ("", "old", "wrk"):
+        ha, hb = ra.state_host(pre), rb.state_host(pre)
+        bad = [k for k in FIELDS if not np.array_equal(ha[k], hb[k])]
+        assert not bad, f"{label}: {pre or 'state'} arrays differ on {bad}"
+    assert ra.t == rb.t and ra.cell_updates == rb.cell_updates
+    assert list(ra.t_step) == list(rb.t_step) and list(ra.t_stepnew) == list(rb.t_stepnew)
+
+
+@pytest.mark.parametrize("L", [10, 30])
+def test_graph_steps_equal_host_steps(L):
+    """Full steps replayed from captured graphs, with the time step formed
+    on the device, equal the host-driven steps bit for bit -- arrays, time,
+    time steps -- also when graph and host steps alternate."""
+    from g3m3_amd.driver import FastRun
+    wl = lambda: make_workload("single61", seed=77, L=L)
+    ra, rb = FastRun(wl()), FastRun(wl())
+    ra.init_dt()
+    rb.init_dt()
+    print(f"graph vs host steps, {ra.nx}x{ra.ny}x{ra.nz}", flush=True)
+    for _ in range(5):
+        ra.full_step()
+    rb.graph_steps(2)
+    rb.graph_finish()
+    rb.full_step()
+    rb.graph_steps(2)
+    rb.graph_finish()
+    _same_run(ra, rb, f"L={L}")
+
+
+def test_batch_graph_equals_runs_alone():
+    from g3m3_amd.driver import Batch, FastRun
+    specs = [dict(L=8, n_sw=3.0, v_sw=350.0, imf=(1.0, -2.0, -4.0)),
+             dict(L=11, n_sw=12.0, v_sw=700.0, imf=(-3.0, 0.5, 2.0)),
+             dict(L=14, n_sw=6.0, v_sw=500.0, imf=(0.0, 4.0, -1.0))]
+    mk = lambda: [make_workload("single61", seed=50 + i, **sp) for i, sp in enumerate(specs)]
+    batch = Batch(mk(), streams=2)
+    batch.init_dt()
+    print("batched graph steps, 3 runs on 2 streams", flush=True)
+    batch.graph_steps(4)
+    for wl, rb in zip(mk(), batch.runs):
+        ra = FastRun(wl)
+        ra.init_dt()
+        for _ in range(4):
+            ra.full_step()
+        _same_run(ra, rb, f"{wl.nx}x{wl.ny}x{wl.nz}")
