@@ -326,6 +326,10 @@ def main(argv=None):
                    "box_steps": run.box_steps - bs0, "cell_updates": cu},
         "roofline": roof, "cpu_baseline": cpu, "divb": divb,
     }
+    if hasattr(run, "fct_shape"):
+        kc, order, cands = run.fct_shape()
+        out["fct_autotune"] = {"k_planes_per_block": kc, "block_order": order,
+                               "candidates_us": cands}
     if prec is not None:
         out["divb_precision"] = prec
     if batch is not None:

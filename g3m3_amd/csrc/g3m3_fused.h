@@ -96,6 +96,8 @@ struct FctParams {
     const float* wrk[kMaxFields];
     float chi[kMaxFields];
     int nf;                  // fields in use (k_fct_fused)
+    int kc_len;              // k-planes per block (0: 16)
+    int no_remap;            // 1: plain block order instead of the XCD-contiguous one
 };
 
 // One cell of the 2-layer flank shell of a fine box with its precomputed

@@ -634,7 +634,8 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 
 // ---------------------------------------------------------------------------
 // FCT in one pass (k_fct_fused): a block owns a 63 x 8 (i, j) tile of one
-// field and a run of up to 16 k-planes, and marches it along k.  Rings of
+// field and a run of k-planes (16 unless the run's autotuner picked another
+// depth), and marches it along k.  Rings of
 // three LDS planes hold old (3-cell halo), wrk and the diffused field P
 // (2-cell halo), so P never leaves the CU and each input value comes from
 // HBM about once.  A wave covers two tile rows: lane tx owns the cell
@@ -652,7 +653,7 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 // difference at the last interior cell), so every stored value equals the
 // reference's.
 
-constexpr int FTX = 63, FTY = 8, FKC = 16;
+constexpr int FTX = 63, FTY = 8, FKC = 16;   // FKC: default k-planes per block
 constexpr int FPX = FTX + 5, FPY = FTY + 4;   // wrk / P: cells i0-3 .. i0+64, j0-2 .. j0+9
 constexpr int FOX = FPX + 2, FOY = FPY + 2;   // old: one more cell on each side
 constexpr int FO_N = FOX * FOY, FP_N = FPX * FPY;
@@ -693,14 +694,15 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     // logical block order: field, k-run, tile row, tile column (fastest)
     const int per_f = ntx * nty * nkc;
-    const int bl = xcd_block(blockIdx.x, gridDim.x);   // neighbouring tiles on one XCD
+    // neighbouring tiles on one XCD, unless tuned otherwise
+    const int bl = F.no_remap ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (bl >= per_f * F.nf) return;
     const int f = bl / per_f;
     const int r = bl % per_f;
     const int kc = r / (ntx * nty);
     const int t = r % (ntx * nty);
     const int i0 = (t % ntx) * FTX, j0 = (t / ntx) * FTY;
-    const int kb = kc * FKC, ke = min(nz, kb + FKC);
+    const int kb = kc * F.kc_len, ke = min(nz, kb + F.kc_len);
     const char* __restrict__ wg = reinterpret_cast<const char*>(F.wrk[f]);
     const char* __restrict__ og = reinterpret_cast<const char*>(F.old[f]);
     float* out = F.out[f];
@@ -907,10 +909,11 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
 }
 
 void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
-    const int ntx = (F.B.nx + FTX - 1) / FTX, nty = (F.B.ny + FTY - 1) / FTY;
-    const int nkc = (F.B.nz + FKC - 1) / FKC;
     FctParams G = F;
     G.nf = nf;
+    if (G.kc_len <= 0) G.kc_len = FKC;
+    const int ntx = (F.B.nx + FTX - 1) / FTX, nty = (F.B.ny + FTY - 1) / FTY;
+    const int nkc = (F.B.nz + G.kc_len - 1) / G.kc_len;
     const int nb = ntx * nty * nkc * nf;
     ProfScope prof_("k_fct_fused", s);
     hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, 4), 0, s, G, ntx, nty, nkc);

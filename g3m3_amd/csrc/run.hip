@@ -70,6 +70,10 @@ struct Run {
     float* host_buf = nullptr;    // pinned: t_stepnew and t read back
     bool capturing = false;       // kernels read the time step from dt_dev
     bool own_stream = false;
+    // the FCT's launch shape, picked by tune_fct: k-planes per block and
+    // block order, with the measured time of each candidate
+    int fct_kc = 16, fct_no_remap = 0;
+    float fct_us[6] = {};
 
     Set slice(float* const* arrays, int box) const {
         Set S;
@@ -376,7 +380,57 @@ void fct(Run* R, int box) {
         F.wrk[f] = wk.f[f];
         F.chi[f] = (f >= BX) ? d.diferg : d.difrho;
     }
+    F.kc_len = R->fct_kc;
+    F.no_remap = R->fct_no_remap;
     launch_fct_fused(F, NF, R->s);
+}
+
+// Autotuning of the FCT's launch shape: the k-planes per block (more
+// planes re-load less halo, fewer planes put more blocks in flight) and the
+// block order (XCD-contiguous or plain) are timed on the run's own box --
+// box 1's arrays as inputs, a scratch output -- and the fastest is kept.
+// Every candidate gives the same bits.
+constexpr int kFctDepths[3] = {8, 16, 32};
+
+void tune_fct(Run* R) {
+    float* out = nullptr;
+    if (hipMalloc(&out, sizeof(float) * (size_t)NF * R->n) != hipSuccess) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    const g3m3_run_desc& d = R->d;
+    FctParams F;
+    F.B = R->B;
+    const Set ol = R->old(1), wk = R->wrk(1);
+    for (int f = 0; f < NF; ++f) {
+        F.out[f] = out + (size_t)f * R->n;
+        F.old[f] = ol.f[f];
+        F.wrk[f] = wk.f[f];
+        F.chi[f] = (f >= BX) ? d.diferg : d.difrho;
+    }
+    float best = 1e30f;
+    for (int c = 0; ok && c < 3; ++c)
+        for (int rm = 0; ok && rm < 2; ++rm) {
+            F.kc_len = kFctDepths[c];
+            F.no_remap = rm;
+            launch_fct_fused(F, NF, R->s);   // warm-up
+            ok = hipEventRecord(e0, R->s) == hipSuccess;
+            for (int q = 0; q < 3; ++q) launch_fct_fused(F, NF, R->s);
+            float ms = 0.0f;
+            ok = ok && hipEventRecord(e1, R->s) == hipSuccess &&
+                 hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (!ok) break;
+            const float us = ms * 1000.0f / 3.0f;
+            R->fct_us[2 * c + rm] = us;
+            if (us < best) {
+                best = us;
+                R->fct_kc = kFctDepths[c];
+                R->fct_no_remap = rm;
+            }
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
 }
 
 void corer(Run* R, int box) {
@@ -633,6 +687,7 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
     R->t_step.assign(ng, 0.0);
     R->t_stepnew.assign(ng, 0.0);
     R->rot.assign(ng, 0);
+    tune_fct(R);
     return R;
 }
 
@@ -792,6 +847,29 @@ int g3m3_run_graph_finish(void* run) {
     }
     check(R, "graph_finish");
     return R->err;
+}
+
+int g3m3_run_fct_shape(void* run, int* kc_len, int* no_remap, float* cand_us) {
+    Run* R = (Run*)run;
+    if (kc_len) *kc_len = R->fct_kc;
+    if (no_remap) *no_remap = R->fct_no_remap;
+    if (cand_us)
+        for (int q = 0; q < 6; ++q) cand_us[q] = R->fct_us[q];
+    return 0;
+}
+
+int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap) {
+    Run* R = (Run*)run;
+    if (kc_len < 1) return 1;
+    R->fct_kc = kc_len;
+    R->fct_no_remap = no_remap ? 1 : 0;
+    for (int q = 0; q < 2; ++q) {   // captured steps hold the old shape
+        if (R->gexec[q]) (void)hipGraphExecDestroy(R->gexec[q]);
+        if (R->graph[q]) (void)hipGraphDestroy(R->graph[q]);
+        R->gexec[q] = nullptr;
+        R->graph[q] = nullptr;
+    }
+    return 0;
 }
 
 int g3m3_run_sync_arrays(void* run) {

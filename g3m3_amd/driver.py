@@ -423,6 +423,21 @@ class FastRun(Run):
         if self._L.g3m3_run_graph_steps(_ct.c_void_p(self.h), int(nsteps)):
             raise _lib.G3M3Error("g3m3_run_graph_steps failed")
 
+    def fct_shape(self):
+        """The FCT launch shape chosen by the run's autotuner: (k-planes per
+        block, block order 'xcd' or 'plain', {candidate: microseconds})."""
+        kc, rm = _ct.c_int(0), _ct.c_int(0)
+        us = (_ct.c_float * 6)()
+        self._L.g3m3_run_fct_shape(_ct.c_void_p(self.h), _ct.byref(kc), _ct.byref(rm), us)
+        cands = {f"{d}/{o}": round(float(us[2 * i + j]), 2)
+                 for i, d in enumerate((8, 16, 32)) for j, o in enumerate(("xcd", "plain"))}
+        return kc.value, ("plain" if rm.value else "xcd"), cands
+
+    def set_fct_shape(self, kc_len: int, order: str = "xcd"):
+        if self._L.g3m3_run_set_fct_shape(_ct.c_void_p(self.h), int(kc_len),
+                                          int(order == "plain")):
+            raise _lib.G3M3Error(f"bad FCT shape {kc_len}")
+
     def graph_finish(self):
         if self._L.g3m3_run_graph_finish(_ct.c_void_p(self.h)):
             raise _lib.G3M3Error("g3m3_run_graph_finish failed")

@@ -542,6 +542,13 @@ int g3m3_run_finish(void* run);
  * stream gets one of its own. */
 int g3m3_run_graph_steps(void* run, int nsteps);
 int g3m3_run_graph_finish(void* run);
+/* The FCT launch shape the run's autotuner chose at g3m3_run_create: the
+ * k-planes per block (8, 16 or 32) and the block order (0 XCD-contiguous,
+ * 1 plain), with the measured microseconds of the six candidates (depth
+ * major, order minor).  Every shape gives the same results. */
+int g3m3_run_fct_shape(void* run, int* kc_len, int* no_remap, float* cand_us);
+/* Overrides that choice (kc_len >= 1); returns nonzero on a bad value. */
+int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap);
 int g3m3_run_box_step(void* run, int box);
 void g3m3_run_set_time(void* run, double t);          /* after a restart read */
 int g3m3_run_sync_arrays(void* run);

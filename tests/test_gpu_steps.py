@@ -116,3 +116,28 @@ def test_fluid_restart_round_trip(tmp_path):
     ha, hb = a.state_host(), b.state_host()
     bad = [k for k in FIELDS if not np.array_equal(ha[k], hb[k])]
     assert not bad, bad
+
+
+def test_fct_launch_shapes_equal():
+    """Every FCT launch shape the autotuner chooses among (8, 16 or 32
+    k-planes per block; XCD-contiguous or plain block order) gives the same
+    bits over whole steps of a nested run; the tuner's pick is one of them."""
+    from g3m3_amd.driver import FastRun
+    ref = None
+    for kc in (8, 16, 32):
+        for order in ("xcd", "plain"):
+            r = FastRun(make_workload("small", eta=1e-3))
+            if ref is None:
+                kc0, order0, cands = r.fct_shape()
+                assert kc0 in (8, 16, 32) and order0 in ("xcd", "plain") and len(cands) == 6
+            r.set_fct_shape(kc, order)
+            r.init_dt()
+            print(f"FCT shape {kc}/{order}", flush=True)
+            for _ in range(2):
+                r.full_step()
+            h = r.state_host()
+            if ref is None:
+                ref = h
+            else:
+                bad = [k for k in FIELDS if not np.array_equal(h[k], ref[k])]
+                assert not bad, f"shape {kc}/{order} differs on {bad}"
