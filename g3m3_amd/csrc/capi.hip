@@ -1184,8 +1184,56 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
     }
     FieldPtrs F{};
     for (int q = 0; q < *nq; ++q) F.p[q] = const_cast<float*>(R(fields[q]));
-    launch_sample(*npts, R(sxyz), G, *nx, *ny, *nz, *nq, F, R(scdata), R(gridpt), g_stream);
+    // positions are only read here (no trajectories)
+    launch_sample(*npts, const_cast<float*>(R(sxyz)), nullptr, nullptr, 0.0f, G, *nx, *ny, *nz,
+                  *nq, F, R(scdata), R(gridpt), g_stream);
     check_launch("sample_fields");
+}
+
+void g3m3_sample_craft(const int* ncraft, const float* xcraft, const float* zcraft,
+                       const float* ut, const int* n_grids, const float* grid_minvals,
+                       const float* grid_maxvals, const float* xspac, const float* re_equiv,
+                       const int* nx, const int* ny, const int* nz, const int* nq,
+                       const float* const* fields, float* sxyz, float* scdata, int* gridpt) {
+    CraftGrid G;
+    if (!craft_grid(G, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv)) return;
+    if (*nq < 0 || *nq > kMaxFields) {
+        latch(4, "sample_craft: nq out of range");
+        return;
+    }
+    FieldPtrs F{};
+    for (int q = 0; q < *nq; ++q) F.p[q] = const_cast<float*>(R(fields[q]));
+    launch_sample(*ncraft, R(sxyz), R(xcraft), R(zcraft), *ut, G, *nx, *ny, *nz, *nq, F,
+                  R(scdata), R(gridpt), g_stream);
+    check_launch("sample_craft");
+}
+
+void g3m3_find_aux_cube(const char* cname, const int* i_craft, const int* scout,
+                        const int* n_grids, const float* grid_minvals, const float* grid_maxvals,
+                        const float* xspac, const float* re_equiv, const float* xcraftn,
+                        const float* zcraftn, const float* ut, int* ntimesn, int* recordingn,
+                        int* gridpt, float* sxyz, float* cube_vertices, size_t cname_len) {
+    (void)i_craft;
+    (void)scout;
+    (void)ntimesn;
+    CraftGrid G;
+    if (!craft_grid(G, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv)) return;
+    craft_position(xcraftn, zcraftn, *ut, sxyz);
+    g3m3_findgrid(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, gridpt, re_equiv, cname,
+                  cname_len);
+    if (gridpt[0] <= 0) {
+        *recordingn = 0;
+        const std::string name = cname ? std::string(cname, cname_len) : std::string();
+        printf(" Gridding problem with craft %s at UT = %g Future points skipped.\n",
+               name.c_str(), (double)*ut);
+        return;
+    }
+    float v0[3], v1[3];
+    craft_cube(G, gridpt, v0, v1);
+    for (int a = 0; a < 3; ++a) {
+        cube_vertices[a] = v0[a];       // cube_vertices(axis, 1)
+        cube_vertices[3 + a] = v1[a];   // cube_vertices(axis, 2)
+    }
 }
 
 }  // extern "C"
@@ -1241,6 +1289,8 @@ G3M3_FORTRAN_ALIAS(fnd_pres)
 G3M3_FORTRAN_ALIAS(sparse_solver)
 G3M3_FORTRAN_ALIAS(findgrid)
 G3M3_FORTRAN_ALIAS(sample_fields)
+G3M3_FORTRAN_ALIAS(sample_craft)
+G3M3_FORTRAN_ALIAS(find_aux_cube)
 G3M3_FORTRAN_ALIAS(dipole)
 G3M3_FORTRAN_ALIAS(mak_dip_grd)
 G3M3_FORTRAN_ALIAS(set_resist)

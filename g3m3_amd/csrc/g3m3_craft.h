@@ -1,6 +1,7 @@
 // Spacecraft sampling geometry shared by host and device code:
 // code/cft_findgrid.f90:4-60 (smallest box containing the point, lower
-// grid corner) and the cube vertices of code/cft_find_aux_cube.f90:62-65.
+// grid corner), the cube vertices of code/cft_find_aux_cube.f90:62-65 and
+// the craft position between trajectory points (code/cft_linterp.f90).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,6 +53,19 @@ __host__ __device__ inline void craft_cube(const CraftGrid& G, const int gridpt[
     for (int a = 0; a < 3; ++a) {
         v0[a] = (G.mn[b][a] + ((float)gridpt[a] - 1.0f) * G.xspac[b]) * G.re_equiv;
         v1[a] = v0[a] + G.xspac[b] * G.re_equiv;
+    }
+}
+
+// The craft position at time ut (code/cft_find_aux_cube.f90:43-47 with
+// code/cft_linterp.f90:17-27): the trajectory point xc(1:3) when the two
+// points share a time or ut has reached the second, otherwise the straight
+// line from xc to zc at ut; xc, zc are (x, y, z, t)
+__host__ __device__ inline void craft_position(const float xc[4], const float zc[4], float ut,
+                                               float s[3]) {
+    if (xc[3] == zc[3] || ut >= zc[3]) {
+        for (int a = 0; a < 3; ++a) s[a] = xc[a];
+    } else {
+        for (int a = 0; a < 3; ++a) s[a] = xc[a] + (zc[a] - xc[a]) / (zc[3] - xc[3]) * (ut - xc[3]);
     }
 }
 

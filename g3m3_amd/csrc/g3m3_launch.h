@@ -137,8 +137,9 @@ void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],
                      const float* const b[3], float* para, float* perp, float* cross,
                      hipStream_t s);
 struct CraftGrid;
-void launch_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
-                   int nq, const FieldPtrs& F, float* out, int* gridpt, hipStream_t s);
+void launch_sample(int npts, float* sxyz, const float* xcraft, const float* zcraft, float ut,
+                   const CraftGrid& G, int nx, int ny, int nz, int nq, const FieldPtrs& F,
+                   float* out, int* gridpt, hipStream_t s);
 void launch_swap_fields(int n, int nf, const FieldPtrs& a, const FieldPtrs& b, hipStream_t s);
 void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, float grav,
                      float re_equiv, float reynolds, const float gmin[3], const float gmax[3],

@@ -94,11 +94,24 @@ void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],
 // code/cft_record_aux_data.f90:27-30): findgrid, the aux cube, the 2x2x2
 // cube of every field and trilin_interp, one point per thread.  A point
 // whose cube would leave its box gets gridpt[0] = 0 and zero samples.
-__global__ void k_sample(int npts, const float* __restrict__ sxyz, CraftGrid G, int nx, int ny,
+// With trajectories (xcraft, zcraft: (x, y, z, t) per craft) each point is
+// first placed at time ut as find_aux_cube places it, and written to sxyz.
+__global__ void k_sample(int npts, float* __restrict__ sxyz, const float* __restrict__ xcraft,
+                         const float* __restrict__ zcraft, float ut, CraftGrid G, int nx, int ny,
                          int nz, int nq, FieldPtrs F, float* out, int* gridpt) {
     const int p = gtid();
     if (p >= npts) return;
-    const float s[3] = {sxyz[3 * p], sxyz[3 * p + 1], sxyz[3 * p + 2]};
+    float s[3];
+    if (xcraft) {
+        const float xc[4] = {xcraft[4 * p], xcraft[4 * p + 1], xcraft[4 * p + 2],
+                             xcraft[4 * p + 3]};
+        const float zc[4] = {zcraft[4 * p], zcraft[4 * p + 1], zcraft[4 * p + 2],
+                             zcraft[4 * p + 3]};
+        craft_position(xc, zc, ut, s);
+        for (int a = 0; a < 3; ++a) sxyz[3 * p + a] = s[a];
+    } else {
+        for (int a = 0; a < 3; ++a) s[a] = sxyz[3 * p + a];
+    }
     int gp[4];
     craft_findgrid(G, s, gp);
     const bool ok = gp[0] >= 1 && gp[0] <= nx - 1 && gp[1] >= 1 && gp[1] <= ny - 1 &&
@@ -130,11 +143,12 @@ __global__ void k_sample(int npts, const float* __restrict__ sxyz, CraftGrid G, 
     }
 }
 
-void launch_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
-                   int nq, const FieldPtrs& F, float* out, int* gridpt, hipStream_t s) {
+void launch_sample(int npts, float* sxyz, const float* xcraft, const float* zcraft, float ut,
+                   const CraftGrid& G, int nx, int ny, int nz, int nq, const FieldPtrs& F,
+                   float* out, int* gridpt, hipStream_t s) {
     if (npts <= 0) return;
-    hipLaunchKernelGGL(k_sample, dim3((npts + 255) / 256), dim3(256), 0, s, npts, sxyz, G, nx, ny,
-                       nz, nq, F, out, gridpt);
+    hipLaunchKernelGGL(k_sample, dim3((npts + 255) / 256), dim3(256), 0, s, npts, sxyz, xcraft,
+                       zcraft, ut, G, nx, ny, nz, nq, F, out, gridpt);
 }
 
 }  // namespace g3m3

@@ -386,6 +386,31 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
                         const float* re_equiv, const int* nx, const int* ny, const int* nz,
                         const int* nq, const float* const* fields, float* scdata, int* gridpt);
 
+/* code/cft_find_aux_cube.f90:4-8 -- host arrays: the craft's position at
+ * time ut between its trajectory points xcraftn and zcraftn, (x, y, z, t)
+ * each, then findgrid and the cube vertices cube_vertices(3,2).  As in the
+ * reference, a gridding problem (gridpt(1) <= 0) clears recordingn and
+ * prints a notice; closing the craft's output unit scout+i_craft is left to
+ * the Fortran caller.  cname_len is the CHARACTER*8 hidden length. */
+void g3m3_find_aux_cube(const char* cname, const int* i_craft, const int* scout,
+                        const int* n_grids, const float* grid_minvals, const float* grid_maxvals,
+                        const float* xspac, const float* re_equiv, const float* xcraftn,
+                        const float* zcraftn, const float* ut, int* ntimesn, int* recordingn,
+                        int* gridpt, float* sxyz, float* cube_vertices, size_t cname_len);
+
+/* No single reference counterpart: the aux-craft branch of craft_handling
+ * (code/cft_craft_handling.f90:127-200) for many craft in one launch.  Each
+ * craft n is placed at time ut between xcraft(:,n) and zcraft(:,n) (as
+ * find_aux_cube places it) and written to sxyz(3,ncraft); then findgrid,
+ * the aux cube and trilin_interp of the nq fields give scdata(nq,ncraft)
+ * (record_aux_data's samples, code/cft_record_aux_data.f90:27-30) and
+ * gridpt(4,ncraft).  Device arrays except the grid extents. */
+void g3m3_sample_craft(const int* ncraft, const float* xcraft, const float* zcraft,
+                       const float* ut, const int* n_grids, const float* grid_minvals,
+                       const float* grid_maxvals, const float* xspac, const float* re_equiv,
+                       const int* nx, const int* ny, const int* nz, const int* nq,
+                       const float* const* fields, float* sxyz, float* scdata, int* gridpt);
+
 /* ---- initialisation -----------------------------------------------------------
  * common /rotation/ (code/3d_main.f90:633-634), which the reference's
  * dipole and mak_dip_grd read, comes in as the trailing arguments rot_angle,

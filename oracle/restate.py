@@ -1148,6 +1148,51 @@ def aux_cube(gridpt, grid_minvals, xspac, re_equiv):
     return cv
 
 
+def linterp(xyzt1, xyzt2, time, xyz):
+    """code/cft_linterp.f90:7-30: xyz (3,) at `time` on the straight line
+    between the space-time points xyzt1 and xyzt2 (4,)."""
+    a, b = np.asarray(xyzt1, F), np.asarray(xyzt2, F)
+    if a[3] == b[3]:
+        xyz[:] = a[:3]
+    else:
+        t = F(time)
+        for ax in range(3):
+            xyz[ax] = a[ax] + (b[ax] - a[ax]) / (b[3] - a[3]) * (t - a[3])
+
+
+def craft_position(xcraftn, zcraftn, ut):
+    """code/cft_find_aux_cube.f90:43-47: the craft's position at time ut."""
+    s = np.zeros(3, F)
+    if F(xcraftn[3]) == F(zcraftn[3]) or F(ut) >= F(zcraftn[3]):
+        s[:] = np.asarray(xcraftn[:3], F)
+    else:
+        linterp(xcraftn, zcraftn, ut, s)
+    return s
+
+
+def find_aux_cube(cname, i_craft, scout, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv,
+                  xcraftn, zcraftn, ut, ntimesn, recordingn, gridpt, sxyz, cube_vertices):
+    """code/cft_find_aux_cube.f90:4-67 (the unit close of a gridding problem
+    is I/O and is left out); recordingn and cube_vertices are arrays (the
+    latter (2, 3), i.e. the Fortran (3,2))."""
+    sxyz[:] = craft_position(xcraftn, zcraftn, ut)
+    findgrid(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, gridpt, re_equiv)
+    if gridpt[0] <= 0:
+        recordingn[0] = 0
+        return
+    cube_vertices[...] = aux_cube(gridpt, grid_minvals, xspac, re_equiv)
+
+
+def sample_craft(xcraft, zcraft, ut, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv,
+                 fields):
+    """The aux-craft branch of craft_handling for many craft: positions at
+    ut from (ncraft, 4) trajectory points, then sample_fields.  Returns
+    (sxyz, scdata, gridpt)."""
+    sxyz = np.stack([craft_position(x, z, ut) for x, z in zip(xcraft, zcraft)]).astype(F)
+    sc, gp = sample_fields(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv, fields)
+    return sxyz, sc, gp
+
+
 def sample_fields(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv, fields):
     """Craft sampling (code/cft_craft_handling.f90:165-195 with
     code/cft_record_aux_data.f90:27-30): for each point, findgrid, the aux

@@ -495,3 +495,73 @@ def test_divb_refine_vs_fp64(dev):
               f"{err[mode]:.2e}", flush=True)
     assert err[3] <= 2e-6
     assert err[3] <= err[2]
+
+
+def test_find_aux_cube_host(dev):
+    """The host find_aux_cube entry point against the oracle: placement
+    between trajectory points, findgrid and the cube vertices."""
+    from g3m3_amd import _lib
+    import ctypes
+    ngrd, re = 3, np.float32(0.5)
+    xs = np.array([1.0, 2.0, 4.0], np.float32)
+    mn = np.ascontiguousarray(np.stack([-6 * xs, -6 * xs, -3 * xs], 1).astype(np.float32))
+    mx = np.ascontiguousarray(np.stack([6 * xs, 6 * xs, 3 * xs], 1).astype(np.float32))
+    rng = np.random.default_rng(17)
+    for q in range(60):
+        x = np.append(rng.uniform(-10, 10, 3), rng.uniform(0, 10)).astype(np.float32)
+        z = np.append(rng.uniform(-10, 10, 3), x[3] + rng.uniform(0.1, 5)).astype(np.float32)
+        if q % 10 == 0:
+            z[3] = x[3]
+        ut = np.float32(x[3] + rng.uniform(-1, 6))
+        res = []
+        for who in ("gpu", "oracle"):
+            nt, rec = np.zeros(2, np.int32), np.ones(1, np.int32)
+            gp, s, cv = np.zeros(4, np.int32), np.zeros(3, np.float32), np.zeros((2, 3), np.float32)
+            if who == "gpu":
+                name = ctypes.create_string_buffer(b"craft02 ", 8)
+                _lib.call("find_aux_cube", name, 2, 40, ngrd, mn, mx, xs, re, x, z, ut, nt, rec,
+                          gp, s, cv, _lib.ByValue(ctypes.c_size_t(8)))
+            else:
+                O.find_aux_cube("craft02", 2, 40, ngrd, mn, mx, xs, re, x, z, ut, nt, rec, gp, s,
+                                cv)
+            res.append((gp, s, cv, rec))
+        for u, v in zip(*res):
+            assert np.array_equal(u, v), (q, res)
+
+
+def test_sample_craft(dev):
+    """Many craft placed at time ut on their trajectories and sampled in one
+    launch, against the oracle's find_aux_cube placement + sampling."""
+    from g3m3_amd import _lib
+    import ctypes
+    nx, ny, nz, ngrd = 13, 13, 7, 3
+    re = np.float32(0.5)
+    xs = np.array([1.0, 2.0, 4.0], np.float32)
+    mn = np.ascontiguousarray(np.stack([-6 * xs, -6 * xs, -3 * xs], 1).astype(np.float32))
+    mx = np.ascontiguousarray(np.stack([6 * xs, 6 * xs, 3 * xs], 1).astype(np.float32))
+    rng = np.random.default_rng(23)
+    fields = [rng.uniform(-1, 1, (ngrd, nz, ny, nx)).astype(np.float32) for _ in range(3)]
+    nc = 400
+    xc = np.concatenate([rng.uniform(-12, 12, (nc, 3)), rng.uniform(0, 5, (nc, 1))], 1)
+    zc = np.concatenate([rng.uniform(-12, 12, (nc, 3)), xc[:, 3:] + rng.uniform(0.1, 5, (nc, 1))],
+                        1)
+    xc, zc = xc.astype(np.float32), zc.astype(np.float32)
+    xc[:, 2] *= 0.5
+    zc[:, 2] *= 0.5
+    zc[::17, 3] = xc[::17, 3]        # same-time pairs
+    ut = np.float32(4.0)
+    ws, wsc, wgp = O.sample_craft(xc, zc, ut, ngrd, mn, mx, xs, re, fields)
+    dfs = [dev.from_numpy(f).cuda() for f in fields]
+    parr = (ctypes.c_void_p * len(dfs))(*[f.data_ptr() for f in dfs])
+    dx, dz = dev.from_numpy(np.ascontiguousarray(xc)).cuda(), dev.from_numpy(
+        np.ascontiguousarray(zc)).cuda()
+    s = dev.zeros((nc, 3), dtype=dev.float32, device="cuda")
+    out = dev.zeros((nc, len(fields)), dtype=dev.float32, device="cuda")
+    gp = dev.zeros((nc, 4), dtype=dev.int32, device="cuda")
+    _lib.call("sample_craft", nc, dx, dz, ut, ngrd, mn, mx, xs, re, nx, ny, nz, len(fields),
+              parr, s, out, gp)
+    _lib.sync()
+    assert np.array_equal(s.cpu().numpy(), ws)
+    assert np.array_equal(gp.cpu().numpy(), wgp)
+    assert np.array_equal(out.cpu().numpy(), wsc)
+    assert (wgp[:, 0] > 0).sum() > 200

@@ -408,3 +408,49 @@ def test_init_plasma_field_and_resist_vs_reference():
                F(1.0))
     assert np.array_equal(rst, d["rst"])
     assert L["numzero"][0] > 0 and L["nummid"][0] > 0 and L["numsrf"][0] > 0
+
+
+# --- craft placement: linterp, find_aux_cube ------------------------------------
+
+def test_linterp():
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        a = rng.uniform(-20, 20, 4).astype(np.float32)
+        b = rng.uniform(-20, 20, 4).astype(np.float32)
+        if rng.uniform() < 0.1:
+            b[3] = a[3]
+        t = np.float32(rng.uniform(-20, 20))
+        r, o = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        lib().call("linterp", a, b, t, r)
+        O.linterp(a, b, t, o)
+        assert np.array_equal(r, o), (a, b, t)
+
+
+def test_find_aux_cube():
+    """Between trajectory points, at and past the second point's time, and
+    with both points at one time; cube vertices and gridpt bit for bit."""
+    ngrd, re = 3, np.float32(0.5)
+    xs, mn, mx = _grid_extents(ngrd)
+    rng = np.random.default_rng(13)
+    for q in range(120):
+        x = np.append(rng.uniform(-12, 12, 3), rng.uniform(0, 10)).astype(np.float32)
+        z = np.append(rng.uniform(-12, 12, 3), x[3] + rng.uniform(0.1, 5)).astype(np.float32)
+        if q % 10 == 0:
+            z[3] = x[3]
+        ut = np.float32(x[3] + rng.uniform(-1, 6))
+        outs = []
+        for call in ("ref", "oracle"):
+            nt = np.zeros(2, np.int32)
+            rec = np.ones(1, np.int32)
+            gp = np.zeros(4, np.int32)
+            s = np.zeros(3, np.float32)
+            cv = np.zeros((2, 3), np.float32)
+            if call == "ref":
+                lib().call("find_aux_cube", R.Chars("craft01", 8), 1, 40, ngrd, mn, mx, xs, re,
+                           x, z, ut, nt, rec, gp, s, cv)
+            else:
+                O.find_aux_cube("craft01", 1, 40, ngrd, mn, mx, xs, re, x, z, ut, nt, rec, gp, s,
+                                cv)
+            outs.append((gp, s, cv, rec))
+        for u, v in zip(*outs):
+            assert np.array_equal(u, v), (q, outs)
