@@ -1142,6 +1142,53 @@ void g3m3_init_plasma(
     if (h[3 * mb]) latch(4, "init_plasma: more boundary points than msrf/mmid/mzero; lists cut");
 }
 
+// code/sub_flux_counter.f90:5-158: the wall sums on the device, the
+// in/out bookkeeping on the host in the reference's order
+void g3m3_flux_counter(const float* qpx, const float* qpy, const float* qpz, const float* hpx,
+                       const float* hpy, const float* hpz, const float* opx, const float* opy,
+                       const float* opz, const int* nx, const int* ny, const int* nz,
+                       const int* n_grids, const int* box, const float* rmassq,
+                       const float* rmassh, const float* rmasso, const float* grid_spacing,
+                       float* qflux_in, float* qflux_out, float* hflux_in, float* hflux_out,
+                       float* oflux_in, float* oflux_out) {
+    (void)n_grids;
+    const Box B = box_of(nx, ny, nz);
+    FieldPtrs P{};
+    const float* mom[9] = {qpx, qpy, qpz, hpx, hpy, hpz, opx, opy, opz};
+    for (int q = 0; q < 9; ++q) P.p[q] = const_cast<float*>(S(mom[q], B, box));
+    float* d = (float*)g_tmp.get(256);
+    if (!d) return;
+    launch_flux_walls(B, P, d, g_stream);
+    check_launch("flux_counter");
+    float h[18];
+    if (!hip_ok(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, g_stream),
+                "flux_counter") ||
+        !hip_ok(hipStreamSynchronize(g_stream), "flux_counter"))
+        return;
+    const float gs = *grid_spacing;
+    const float rm[3] = {*rmassq, *rmassh, *rmasso};
+    float in[3] = {0.0f, 0.0f, 0.0f}, out[3] = {0.0f, 0.0f, 0.0f};
+    for (int axis = 0; axis < 3; ++axis) {
+        const float* f1 = h + 6 * axis;       // q, h, o on the low wall
+        const float* f2 = h + 6 * axis + 3;   // on the high wall
+        // the h species' flux decides the direction for all three (:48-67)
+        for (int s = 0; s < 3; ++s) {
+            if (f1[1] > 0) in[s] = in[s] + f1[s] * gs / rm[s];
+            else out[s] = out[s] - f1[s] * gs / rm[s];
+        }
+        for (int s = 0; s < 3; ++s) {
+            if (f2[1] < 0) in[s] = in[s] - f2[s] * gs / rm[s];
+            else out[s] = out[s] + f2[s] * gs / rm[s];
+        }
+    }
+    *qflux_in = in[0];
+    *qflux_out = out[0];
+    *hflux_in = in[1];
+    *hflux_out = out[1];
+    *oflux_in = in[2];
+    *oflux_out = out[2];
+}
+
 static bool craft_grid(CraftGrid& G, const int* n_grids, const float* grid_minvals,
                        const float* grid_maxvals, const float* xspac, const float* re_equiv) {
     if (*n_grids < 1 || *n_grids > kCraftMaxGrids) {
@@ -1291,6 +1338,7 @@ G3M3_FORTRAN_ALIAS(findgrid)
 G3M3_FORTRAN_ALIAS(sample_fields)
 G3M3_FORTRAN_ALIAS(sample_craft)
 G3M3_FORTRAN_ALIAS(find_aux_cube)
+G3M3_FORTRAN_ALIAS(flux_counter)
 G3M3_FORTRAN_ALIAS(dipole)
 G3M3_FORTRAN_ALIAS(mak_dip_grd)
 G3M3_FORTRAN_ALIAS(set_resist)

@@ -188,5 +188,7 @@ void launch_set_resist(const Box& B, float* rst, int mbndry, float resist, float
                        const int* ijzero, const int* ijmid, const int* ijsrf, const int* dcounts,
                        const int hcounts[3], int max_points, hipStream_t s);
 void launch_init_plasma(const InitPlasma& P, int n_grids, hipStream_t s);
+// P.p = q, h, o x (px, py, pz) of one box; sums[18] as k_flux_walls lays them out
+void launch_flux_walls(const Box& B, const FieldPtrs& P, float* sums, hipStream_t s);
 
 }  // namespace g3m3

@@ -152,3 +152,39 @@ void launch_sample(int npts, float* sxyz, const float* xcraft, const float* zcra
 }
 
 }  // namespace g3m3
+
+namespace g3m3 {
+
+// ---------------------------------------------------------------------------
+// flux_counter's wall sums (code/sub_flux_counter.f90:26-151): thread t sums
+// one species' momentum over one wall plane, in the reference's loop order
+// (sequential single precision, as the reference accumulates), into
+// sums[t]; t = 6*axis + 3*side + species, axis 0: px on i = 2 and nx-1,
+// 1: py on j = 2 and ny-1, 2: pz on k = 2 and nz-1 (1-based).
+__global__ void k_flux_walls(Box B, FieldPtrs P, float* sums) {
+    const int t = threadIdx.x;
+    if (t >= 18) return;
+    const int axis = t / 6, side = (t / 3) % 2, s = t % 3;
+    const float* p = P.p[3 * s + axis];
+    float acc = 0.0f;
+    if (axis == 0) {
+        const int i = side == 0 ? 1 : B.nx - 2;
+        for (int k = 0; k < B.nz; ++k)
+            for (int j = 0; j < B.ny; ++j) acc = acc + p[B.IX(i, j, k)];
+    } else if (axis == 1) {
+        const int j = side == 0 ? 1 : B.ny - 2;
+        for (int k = 0; k < B.nz; ++k)
+            for (int i = 0; i < B.nx; ++i) acc = acc + p[B.IX(i, j, k)];
+    } else {
+        const int k = side == 0 ? 1 : B.nz - 2;
+        for (int j = 0; j < B.ny; ++j)
+            for (int i = 0; i < B.nx; ++i) acc = acc + p[B.IX(i, j, k)];
+    }
+    sums[t] = acc;
+}
+
+void launch_flux_walls(const Box& B, const FieldPtrs& P, float* sums, hipStream_t s) {
+    hipLaunchKernelGGL(k_flux_walls, dim3(1), dim3(64), 0, s, B, P, sums);
+}
+
+}  // namespace g3m3

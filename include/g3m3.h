@@ -411,6 +411,18 @@ void g3m3_sample_craft(const int* ncraft, const float* xcraft, const float* zcra
                        const int* nx, const int* ny, const int* nz, const int* nq,
                        const float* const* fields, float* sxyz, float* scdata, int* gridpt);
 
+/* code/sub_flux_counter.f90:5-8 -- the species' particle fluxes through
+ * the six walls one cell inside box `box`; the six results are host
+ * scalars (the call synchronises).  Wall sums run in the reference's loop
+ * order in single precision. */
+void g3m3_flux_counter(const float* qpx, const float* qpy, const float* qpz, const float* hpx,
+                       const float* hpy, const float* hpz, const float* opx, const float* opy,
+                       const float* opz, const int* nx, const int* ny, const int* nz,
+                       const int* n_grids, const int* box, const float* rmassq,
+                       const float* rmassh, const float* rmasso, const float* grid_spacing,
+                       float* qflux_in, float* qflux_out, float* hflux_in, float* hflux_out,
+                       float* oflux_in, float* oflux_out);
+
 /* ---- initialisation -----------------------------------------------------------
  * common /rotation/ (code/3d_main.f90:633-634), which the reference's
  * dipole and mak_dip_grd read, comes in as the trailing arguments rot_angle,

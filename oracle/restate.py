@@ -1148,6 +1148,41 @@ def aux_cube(gridpt, grid_minvals, xspac, re_equiv):
     return cv
 
 
+# ---------------------------------------------------------------------------
+# code/sub_flux_counter.f90:5-158
+
+
+def flux_counter(qpx, qpy, qpz, hpx, hpy, hpz, opx, opy, opz, nx, ny, nz, n_grids, box, rmassq,
+                 rmassh, rmasso, grid_spacing, qflux_in, qflux_out, hflux_in, hflux_out,
+                 oflux_in, oflux_out):
+    """Outputs are 1-element arrays.  Wall sums are sequential in the
+    reference's loop order."""
+    b = box - 1
+    gs = F(grid_spacing)
+    rm = (F(rmassq), F(rmassh), F(rmasso))
+    inn = [F(0.)] * 3
+    out = [F(0.)] * 3
+    walls = (((qpx, hpx, opx), lambda a, i: a[b][:, :, i]),     # (k, j) rows: k outer, j inner
+             ((qpy, hpy, opy), lambda a, j: a[b][:, j, :]),     # k outer, i inner
+             ((qpz, hpz, opz), lambda a, k: a[b][k, :, :]))     # j outer, i inner
+    ext = (nx, ny, nz)
+    for axis, (arrs, cut) in enumerate(walls):
+        f1 = [seq_sum(cut(a, 1).ravel()) for a in arrs]
+        f2 = [seq_sum(cut(a, ext[axis] - 2).ravel()) for a in arrs]
+        for s in range(3):
+            if f1[1] > 0:
+                inn[s] = inn[s] + f1[s] * gs / rm[s]
+            else:
+                out[s] = out[s] - f1[s] * gs / rm[s]
+        for s in range(3):
+            if f2[1] < 0:
+                inn[s] = inn[s] - f2[s] * gs / rm[s]
+            else:
+                out[s] = out[s] + f2[s] * gs / rm[s]
+    qflux_in[0], hflux_in[0], oflux_in[0] = inn
+    qflux_out[0], hflux_out[0], oflux_out[0] = out
+
+
 def linterp(xyzt1, xyzt2, time, xyz):
     """code/cft_linterp.f90:7-30: xyz (3,) at `time` on the straight line
     between the space-time points xyzt1 and xyzt2 (4,)."""

@@ -565,3 +565,24 @@ def test_sample_craft(dev):
     assert np.array_equal(gp.cpu().numpy(), wgp)
     assert np.array_equal(out.cpu().numpy(), wsc)
     assert (wgp[:, 0] > 0).sum() > 200
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.25, -0.25])
+def test_flux_counter(dev, world, shift):
+    for sp in "qho":
+        for c in "xyz":
+            world.a[sp + "p" + c] += np.float32(shift)
+    world.s["grid_spacing"] = np.float32(2.0)
+    outs = ["qflux_in", "qflux_out", "hflux_in", "hflux_out", "oflux_in", "oflux_out"]
+    names = ["qpx", "qpy", "qpz", "hpx", "hpy", "hpz", "opx", "opy", "opz", "nx", "ny", "nz",
+             "n_grids", "box", "rmassq", "rmassh", "rmasso", "grid_spacing"] + outs
+    wo = world.copy()
+    for k in outs:
+        wo.a[k] = np.zeros(1, np.float32)
+    O.flux_counter(*wo.args(names))
+    from g3m3_amd import _lib
+    d = to_dev(dev, world)
+    res = [np.zeros(1, np.float32) for _ in outs]
+    _lib.call("flux_counter", *[d[n] if n in d else world.s[n] for n in names[:18]], *res)
+    _lib.check("flux_counter")
+    assert [r[0] for r in res] == [wo.a[k][0] for k in outs]

@@ -454,3 +454,21 @@ def test_find_aux_cube():
             outs.append((gp, s, cv, rec))
         for u, v in zip(*outs):
             assert np.array_equal(u, v), (q, outs)
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.25, -0.25])
+def test_flux_counter(world, shift):
+    """Wall sums in the reference's order and the in/out bookkeeping, with
+    the h flux positive, negative and mixed on the walls."""
+    for sp in "qho":
+        for c in "xyz":
+            world.a[sp + "p" + c] += np.float32(shift)
+    names = ["qpx", "qpy", "qpz", "hpx", "hpy", "hpz", "opx", "opy", "opz", "nx", "ny", "nz",
+             "n_grids", "box", "rmassq", "rmassh", "rmasso", "grid_spacing", "qflux_in",
+             "qflux_out", "hflux_in", "hflux_out", "oflux_in", "oflux_out"]
+    world.s["grid_spacing"] = np.float32(2.0)
+    for k in names[18:]:
+        world.a[k] = np.zeros(1, np.float32)
+    wr, wo = run_both(world, "flux_counter", names)
+    assert_same(wr, wo, names[18:])
+    assert any(wr.a[k][0] != 0 for k in names[18:])
