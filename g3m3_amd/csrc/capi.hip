@@ -1189,6 +1189,78 @@ void g3m3_flux_counter(const float* qpx, const float* qpy, const float* qpz, con
     *oflux_out = out[2];
 }
 
+Buf g_torus;
+
+void g3m3_torus_inject(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* qpresxy, float* qpresxz, float* qpresyz, float* hrho, float* hpresx, float* hpresy,
+    float* hpresz, float* hpx, float* hpy, float* hpz, float* hpresxy, float* hpresxz,
+    float* hpresyz, float* orho, float* opresx, float* opresy, float* opresz, float* opx,
+    float* opy, float* opz, float* opresxy, float* opresxz, float* opresyz, float* epres,
+    const float* bx, const float* by, const float* bz, const float* bx0, const float* by0,
+    const float* bz0, const int* nx, const int* ny, const int* nz, const int* n_grids,
+    const float* grid_minvals, const float* grid_maxvals, const float* xdip, const float* ydip,
+    const float* zdip, const float* re_equiv, const float* v_rot, const float* torus_inj_dist,
+    const float* torus_rad, const float* srf_bndry, const float* denq_torus,
+    const float* denh_torus, const float* deno_torus, const float* rmassq, const float* rmassh,
+    const float* rmasso, const float* t_torus, const float* aniso_factor, const float* reduct,
+    float* tot_q, float* tot_h, float* tot_o) {
+    (void)n_grids;
+    const Box B = box_of(nx, ny, nz);
+    const int box1 = 1;
+    TorusParams T{};
+    T.B = B;
+    float* f[30] = {qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz,
+                    hrho, hpresx, hpresy, hpresz, hpx, hpy, hpz, hpresxy, hpresxz, hpresyz,
+                    orho, opresx, opresy, opresz, opx, opy, opz, opresxy, opresxz, opresyz};
+    for (int sp = 0; sp < 3; ++sp)
+        for (int m = 0; m < 10; ++m) T.sp.f[sp][m] = S(f[10 * sp + m], B, &box1);
+    T.epres = S(epres, B, &box1);
+    T.bx = S(bx, B, &box1);
+    T.by = S(by, B, &box1);
+    T.bz = S(bz, B, &box1);
+    T.bx0 = S(bx0, B, &box1);
+    T.by0 = S(by0, B, &box1);
+    T.bz0 = S(bz0, B, &box1);
+    float* scratch = (float*)g_torus.get(sizeof(float) * (3 * (size_t)B.V() + 64));
+    if (!scratch) return;
+    T.contrib = scratch + 64;
+    // code/3d_main.f90:3155-3158
+    T.dx = (grid_maxvals[0] - grid_minvals[0]) / (*nx - 1.0f);
+    T.dy = (grid_maxvals[1] - grid_minvals[1]) / (*ny - 1.0f);
+    T.dz = (grid_maxvals[2] - grid_minvals[2]) / (*nz - 1.0f);
+    T.xmin = grid_minvals[0];
+    T.ymin = grid_minvals[1];
+    T.zmin = grid_minvals[2];
+    T.xdip = *xdip;
+    T.ydip = *ydip;
+    T.zdip = *zdip;
+    T.re_equiv = *re_equiv;
+    T.v_rot = *v_rot;
+    T.torus_inj_dist = *torus_inj_dist;
+    T.torus_rad = *torus_rad;
+    T.srf_bndry = *srf_bndry;
+    T.denq = *denq_torus;
+    T.denh = *denh_torus;
+    T.deno = *deno_torus;
+    T.rmassq = *rmassq;
+    T.rmassh = *rmassh;
+    T.rmasso = *rmasso;
+    T.t_torus = *t_torus;
+    T.aniso_factor = *aniso_factor;
+    T.reduct = *reduct;
+    launch_torus(T, scratch, g_stream);
+    check_launch("torus_inject");
+    float h[3];
+    if (!hip_ok(hipMemcpyAsync(h, scratch, sizeof(h), hipMemcpyDeviceToHost, g_stream),
+                "torus_inject") ||
+        !hip_ok(hipStreamSynchronize(g_stream), "torus_inject"))
+        return;
+    *tot_q = h[0];
+    *tot_h = h[1];
+    *tot_o = h[2];
+}
+
 static bool craft_grid(CraftGrid& G, const int* n_grids, const float* grid_minvals,
                        const float* grid_maxvals, const float* xspac, const float* re_equiv) {
     if (*n_grids < 1 || *n_grids > kCraftMaxGrids) {
@@ -1339,6 +1411,7 @@ G3M3_FORTRAN_ALIAS(sample_fields)
 G3M3_FORTRAN_ALIAS(sample_craft)
 G3M3_FORTRAN_ALIAS(find_aux_cube)
 G3M3_FORTRAN_ALIAS(flux_counter)
+G3M3_FORTRAN_ALIAS(torus_inject)
 G3M3_FORTRAN_ALIAS(dipole)
 G3M3_FORTRAN_ALIAS(mak_dip_grd)
 G3M3_FORTRAN_ALIAS(set_resist)

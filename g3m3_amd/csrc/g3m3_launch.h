@@ -188,6 +188,19 @@ void launch_set_resist(const Box& B, float* rst, int mbndry, float resist, float
                        const int* ijzero, const int* ijmid, const int* ijsrf, const int* dcounts,
                        const int hcounts[3], int max_points, hipStream_t s);
 void launch_init_plasma(const InitPlasma& P, int n_grids, hipStream_t s);
+// the torus source on box 1 (code/3d_main.f90:3149-3253); sp[s].f = rho,
+// presx, presy, presz, px, py, pz, presxy, presxz, presyz of species q, h, o
+struct TorusParams {
+    Box B;
+    SpeciesPtrs sp;
+    float* epres;
+    const float *bx0, *by0, *bz0, *bx, *by, *bz;
+    float* contrib;   // 3 * V scratch
+    float xmin, ymin, zmin, dx, dy, dz, xdip, ydip, zdip, re_equiv, v_rot;
+    float torus_inj_dist, torus_rad, srf_bndry, denq, denh, deno, rmassq, rmassh, rmasso;
+    float t_torus, aniso_factor, reduct;
+};
+void launch_torus(const TorusParams& T, float* tot, hipStream_t s);
 // P.p = q, h, o x (px, py, pz) of one box; sums[18] as k_flux_walls lays them out
 void launch_flux_walls(const Box& B, const FieldPtrs& P, float* sums, hipStream_t s);
 

@@ -374,3 +374,93 @@ void launch_init_plasma(const InitPlasma& P, int n_grids, hipStream_t s) {
 }
 
 }  // namespace g3m3
+
+namespace g3m3 {
+
+// ---------------------------------------------------------------------------
+// Torus source (code/3d_main.f90:3149-3253): plasma added to box 1 in a
+// torus of radius torus_inj_dist around the planet, outside srf_bndry,
+// with Gaussian radial and vertical profiles and a field-direction cut.
+// The exp profiles are evaluated in double and rounded once (a C library's
+// expf may differ by an ulp).  Each cell's injected q, h, o mass times the
+// cell volume goes to contrib[3][V] (zero elsewhere); the totals are summed
+// afterwards in the reference's loop order.
+
+__device__ inline float expd(float x) { return (float)exp((double)x); }
+
+__global__ void k_torus(TorusParams T) {
+    int i, j, k;
+    const Box& B = T.B;
+    if (!cell_of(B, gtid(), i, j, k)) return;
+    const size_t c = B.IX(i, j, k);   // box 1
+    const float az = T.zmin + T.dz * (float)k - T.zdip;
+    const float ay = T.ymin + T.dy * (float)j - T.ydip;
+    const float ax = T.xmin + T.dx * (float)i - T.xdip;
+    const float rx = ax * T.re_equiv;
+    const float ry = ay * T.re_equiv;
+    const float ar = sqrtf((ax + T.xdip) * (ax + T.xdip) + (ay + T.ydip) * (ay + T.ydip) +
+                           (az + T.zdip) * (az + T.zdip));
+    const float rvy = rx * T.v_rot;
+    const float rvx = -(ry * T.v_rot);
+    const float corotate = sqrtf(rvx * rvx + rvy * rvy);
+    const float ar_tmid = sqrtf(ax * ax + ay * ay) * T.re_equiv;
+    const float dr_tmid = fabsf(ar_tmid - T.torus_inj_dist);
+    float cq = 0.0f, ch = 0.0f, co = 0.0f;
+    if (fabsf(dr_tmid) < 2.0f * T.torus_rad && ar > T.srf_bndry) {
+        const float a1 = dr_tmid / (0.5f * T.torus_rad);
+        const float rscale = expd(-(a1 * a1));
+        const float a2 = (az * T.re_equiv) / (0.5f * T.torus_rad);
+        const float zscale = expd(-(a2 * a2));
+        const float bsx = T.bx0[c] + T.bx[c], bsy = T.by0[c] + T.by[c], bsz = T.bz0[c] + T.bz[c];
+        const float abtot = (bsx * bsx + bsy * bsy) / (bsz * bsz);
+        const float dscale = fmaxf(0.0f, 1.0f - 10.0f * abtot);
+        const float c2 = corotate * corotate;
+        // h, q, o in the reference's order; species 0 q, 1 h, 2 o
+        const int order[3] = {1, 0, 2};
+        const float den0[3] = {T.denq, T.denh, T.deno};
+        const float rm[3] = {T.rmassq, T.rmassh, T.rmasso};
+        float del[3] = {0.0f, 0.0f, 0.0f}, dens[3] = {0.0f, 0.0f, 0.0f};
+        for (int oi = 0; oi < 3; ++oi) {
+            const int s = order[oi];
+            float* const* f = T.sp.f[s];   // rho, presx, presy, presz, px, py, pz, presxy, ..
+            const float den = den0[s] * rm[s] * rscale * zscale * dscale;
+            f[0][c] = f[0][c] + den;
+            // h: hden*corotate^2*t_torus; q and o divide by their mass first
+            const float dp = (s == 1) ? den * c2 * T.t_torus : (den / rm[s]) * c2 * T.t_torus;
+            f[1][c] = f[1][c] + dp;
+            f[2][c] = f[2][c] + dp;
+            f[3][c] = f[3][c] + dp * T.aniso_factor;
+            f[7][c] = f[7][c] + dp;
+            f[4][c] = f[4][c] + T.reduct * den * rvx;
+            f[5][c] = f[5][c] + T.reduct * den * rvy;
+            del[s] = dp;
+            dens[s] = den;
+        }
+        T.epres[c] = T.epres[c] + (del[2] + del[1] + del[0]);
+        cq = dens[0] * T.dx * T.dy * T.dz;
+        ch = dens[1] * T.dx * T.dy * T.dz;
+        co = dens[2] * T.dx * T.dy * T.dz;
+    }
+    const size_t V = (size_t)B.V();
+    T.contrib[c] = cq;
+    T.contrib[V + c] = ch;
+    T.contrib[2 * V + c] = co;
+}
+
+// the injection totals, one thread per species, summed cell by cell in the
+// reference's loop order (k, j, i = memory order), in single precision
+__global__ void k_torus_totals(const float* contrib, int V, float* tot) {
+    const int s = threadIdx.x;
+    if (s >= 3) return;
+    const float* p = contrib + (size_t)s * V;
+    float acc = 0.0f;
+    for (int c = 0; c < V; ++c) acc = acc + p[c];
+    tot[s] = acc;
+}
+
+void launch_torus(const TorusParams& T, float* tot, hipStream_t s) {
+    hipLaunchKernelGGL(k_torus, dim3((T.B.V() + 255) / 256), dim3(256), 0, s, T);
+    hipLaunchKernelGGL(k_torus_totals, dim3(1), dim3(64), 0, s, T.contrib, T.B.V(), tot);
+}
+
+}  // namespace g3m3

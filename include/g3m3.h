@@ -423,6 +423,30 @@ void g3m3_flux_counter(const float* qpx, const float* qpy, const float* qpz, con
                        float* qflux_in, float* qflux_out, float* hflux_in, float* hflux_out,
                        float* oflux_in, float* oflux_out);
 
+/* code/3d_main.f90:3149-3253 -- the torus source, which has no subroutine
+ * of its own: h, q and o plasma added to box 1 in the torus around the
+ * planet.  Fields are (nx,ny,nz,n_grids); grid_minvals/grid_maxvals are
+ * the driver's (3,n_grids) host arrays; scalars are the driver's variables
+ * of the same names (denq/denh/deno_torus, t_torus, aniso_factor, reduct
+ * from its namelists).  tot_q/h/o (host) are the injected masses summed
+ * over the cells, sum of den*dx*dy*dz in the reference's loop order, before
+ * the driver's scaling to kg/s (:3255-3276); the call synchronises.  The
+ * Gaussian profiles' exp is evaluated in double and rounded once. */
+void g3m3_torus_inject(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* qpresxy, float* qpresxz, float* qpresyz, float* hrho, float* hpresx, float* hpresy,
+    float* hpresz, float* hpx, float* hpy, float* hpz, float* hpresxy, float* hpresxz,
+    float* hpresyz, float* orho, float* opresx, float* opresy, float* opresz, float* opx,
+    float* opy, float* opz, float* opresxy, float* opresxz, float* opresyz, float* epres,
+    const float* bx, const float* by, const float* bz, const float* bx0, const float* by0,
+    const float* bz0, const int* nx, const int* ny, const int* nz, const int* n_grids,
+    const float* grid_minvals, const float* grid_maxvals, const float* xdip, const float* ydip,
+    const float* zdip, const float* re_equiv, const float* v_rot, const float* torus_inj_dist,
+    const float* torus_rad, const float* srf_bndry, const float* denq_torus,
+    const float* denh_torus, const float* deno_torus, const float* rmassq, const float* rmassh,
+    const float* rmasso, const float* t_torus, const float* aniso_factor, const float* reduct,
+    float* tot_q, float* tot_h, float* tot_o);
+
 /* ---- initialisation -----------------------------------------------------------
  * common /rotation/ (code/3d_main.f90:633-634), which the reference's
  * dipole and mak_dip_grd read, comes in as the trailing arguments rot_angle,

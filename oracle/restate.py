@@ -1571,3 +1571,67 @@ def init_plasma(f, bx0, by0, bz0, resistive, nx, ny, nz, n_grids, mbndry, msrf, 
     set_resist(resistive, nx, ny, nz, mbndry, s["resist"], ijzero, numzero, ijmid, nummid,
                ijsrf, numsrf, msrf, mmid, mzero, F(1.))
     return found
+
+
+# ---------------------------------------------------------------------------
+# code/3d_main.f90:3149-3253, the torus source (inline in the main program)
+
+_LIBM.expf.restype = _ct.c_float
+_LIBM.expf.argtypes = [_ct.c_float]
+
+
+def _expf(a):
+    a = np.asarray(a, F)
+    out = np.fromiter((_LIBM.expf(float(v)) for v in a.ravel()), F, a.size)
+    return out.reshape(a.shape)
+
+
+def torus_inject(f, bx0, by0, bz0, nx, ny, nz, grid_minvals, grid_maxvals, s):
+    """Adds the torus plasma to box 1 of the fields ``f`` (names -> (n_grids,
+    nz, ny, nx) arrays, updated in place); ``s`` holds the driver's scalars
+    by name.  Returns (tot_q, tot_h, tot_o), sum of den*dx*dy*dz in the
+    reference's loop order."""
+    g = {k: F(v) for k, v in s.items()}
+    x0, x1 = grid_minvals[0], grid_maxvals[0]
+    ax, ay, az = _box_axes(nx, ny, nz, x0[0], x1[0], x0[1], x1[1], x0[2], x1[2])
+    dx = (F(x1[0]) - F(x0[0])) / (F(nx) - F(1.))
+    dy = (F(x1[1]) - F(x0[1])) / (F(ny) - F(1.))
+    dz = (F(x1[2]) - F(x0[2])) / (F(nz) - F(1.))
+    shp = (nz, ny, nx)
+    az, ay, ax = az - g["zdip"], ay - g["ydip"], ax - g["xdip"]
+    ax, ay, az = (np.broadcast_to(a, shp).astype(F) for a in (ax, ay, az))
+    rx, ry = ax * g["re_equiv"], ay * g["re_equiv"]
+    ar = np.sqrt((ax + g["xdip"]) * (ax + g["xdip"]) + (ay + g["ydip"]) * (ay + g["ydip"])
+                 + (az + g["zdip"]) * (az + g["zdip"]))
+    rvy = rx * g["v_rot"]
+    rvx = -(ry * g["v_rot"])
+    corotate = np.sqrt(rvx * rvx + rvy * rvy)
+    ar_tmid = np.sqrt(ax * ax + ay * ay) * g["re_equiv"]
+    dr_tmid = np.abs(ar_tmid - g["torus_inj_dist"])
+    m = (np.abs(dr_tmid) < F(2.) * g["torus_rad"]) & (ar > g["srf_bndry"])
+    a1 = dr_tmid / (F(0.5) * g["torus_rad"])
+    a2 = (az * g["re_equiv"]) / (F(0.5) * g["torus_rad"])
+    rscale = np.where(m, _expf(-(a1 * a1)), F(0.))
+    zscale = np.where(m, _expf(-(a2 * a2)), F(0.))
+    bsx, bsy, bsz = bx0[0] + f["bx"][0], by0[0] + f["by"][0], bz0[0] + f["bz"][0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        abtot = (bsx * bsx + bsy * bsy) / (bsz * bsz)
+    dscale = np.maximum(F(0.), F(1.) - F(10.) * abtot)
+    c2 = corotate * corotate
+    dels, dens = {}, {}
+    for sp, dn, rm in (("h", "denh_torus", "rmassh"), ("q", "denq_torus", "rmassq"),
+                       ("o", "deno_torus", "rmasso")):
+        den = g[dn] * g[rm] * rscale * zscale * dscale
+        dp = den * c2 * g["t_torus"] if sp == "h" else (den / g[rm]) * c2 * g["t_torus"]
+        for key, add in (("rho", den), ("presx", dp), ("presy", dp),
+                         ("presz", dp * g["aniso_factor"]), ("presxy", dp),
+                         ("px", g["reduct"] * den * rvx), ("py", g["reduct"] * den * rvy)):
+            f[sp + key][0] = np.where(m, f[sp + key][0] + add, f[sp + key][0])
+        dels[sp], dens[sp] = dp, den
+    f["epres"][0] = np.where(m, f["epres"][0] + (dels["o"] + dels["h"] + dels["q"]),
+                             f["epres"][0])
+    tots = []
+    for sp in "qho":
+        contrib = np.where(m, dens[sp] * dx * dy * dz, F(0.))
+        tots.append(seq_sum(contrib.ravel()))
+    return tuple(tots)

@@ -181,3 +181,38 @@ def init_oracle(c):
                           L["parm_srf"], L["parm_mid"], L["parm_zero"], L["ijsrf"], L["ijmid"],
                           L["ijzero"], L["numsrf"], L["nummid"], L["numzero"], d["s"])
     return d, found
+
+
+# --- the torus source (code/3d_main.f90:3149-3253) --------------------------------
+
+TORUS_SCALARS = ("xdip", "ydip", "zdip", "re_equiv", "v_rot", "torus_inj_dist", "torus_rad",
+                 "srf_bndry", "denq_torus", "denh_torus", "deno_torus", "rmassq", "rmassh",
+                 "rmasso", "t_torus", "aniso_factor", "reduct")
+
+
+def torus_case(nx=41, ny=41, nz=21, ngrd=2, seed=5, tilt=0.0):
+    """A box of spacing 0.5 around the planet, B along z (tilted by `tilt`
+    in x), and a torus of radius 5 and half-width 1 in the equatorial
+    plane, with corotation."""
+    rng = np.random.default_rng(seed)
+    s = dict(xdip=0.0, ydip=0.0, zdip=0.0, re_equiv=1.0, v_rot=0.1, torus_inj_dist=5.0,
+             torus_rad=1.0, srf_bndry=2.0, denq_torus=0.3, denh_torus=2.0, deno_torus=0.7,
+             rmassq=1.0, rmassh=1.0, rmasso=16.0, t_torus=0.5, aniso_factor=0.8, reduct=0.9)
+    s = {k: float(F(v)) for k, v in s.items()}
+    xs = np.array([0.5 * 2.0 ** b for b in range(ngrd)], F)
+    gmin = np.stack([-(nx - 1) / 2 * xs, -(ny - 1) / 2 * xs, -(nz - 1) / 2 * xs], 1).astype(F)
+    gmax = np.stack([(nx - 1) / 2 * xs, (ny - 1) / 2 * xs, (nz - 1) / 2 * xs], 1).astype(F)
+    shp = (ngrd, nz, ny, nx)
+    f = {k: rng.uniform(0.5, 1.5, shp).astype(F) for k in FIELDS}
+    for k in ("bx", "by", "bz"):
+        f[k] = np.zeros(shp, F)
+    b0 = [np.full(shp, F(tilt)), np.zeros(shp, F), np.ones(shp, F)]
+    return dict(nx=nx, ny=ny, nz=nz, ngrd=ngrd, s=s, gmin=gmin, gmax=gmax, f=f, b0=b0)
+
+
+def torus_oracle(c):
+    from oracle import restate as O
+    d = copy.deepcopy(c)
+    tot = O.torus_inject(d["f"], *d["b0"], d["nx"], d["ny"], d["nz"], d["gmin"], d["gmax"],
+                         d["s"])
+    return d, tot

@@ -167,3 +167,32 @@ def test_init_plasma_list_capacity(dev):
     assert got["numsrf"][0] == 10
     want = init_oracle(init_case(mcap=2000))[0]["lists"]
     assert np.array_equal(got["ijsrf"][:10], want["ijsrf"][:10])
+
+
+@pytest.mark.parametrize("tilt", [0.0, 0.2])
+def test_torus_inject(dev, tilt):
+    """The torus source against the oracle: untouched cells bit-exact, the
+    injected cells within 4 ulp (the Gaussians' exp: double on the device,
+    the C library's expf in the oracle), totals to 1e-6."""
+    from g3m3_amd import _lib
+    from common import TORUS_SCALARS, torus_case, torus_oracle
+    c = torus_case(tilt=tilt)
+    d, want_tot = torus_oracle(c)
+    f = {k: dev.from_numpy(v.copy()).cuda() for k, v in c["f"].items()}
+    b0 = [dev.from_numpy(v.copy()).cuda() for v in c["b0"]]
+    tot = [np.zeros(1, F) for _ in range(3)]
+    print(f"torus on {c['nx']}x{c['ny']}x{c['nz']}, tilt {tilt}", flush=True)
+    _lib.call("torus_inject", *[f[k] for k in FIELDS[:31]], f["bx"], f["by"], f["bz"], *b0,
+              c["nx"], c["ny"], c["nz"], c["ngrd"], c["gmin"], c["gmax"],
+              *[F(c["s"][k]) for k in TORUS_SCALARS], *tot)
+    _lib.check("torus_inject")
+    for k in FIELDS:
+        got, want = f[k].cpu().numpy(), d["f"][k]
+        same = c["f"][k] == want            # cells the source did not touch
+        assert np.array_equal(got[same], want[same]), k
+        a, b = got.astype(np.float64), want.astype(np.float64)
+        assert np.all(np.abs(a - b) <= 5e-7 * np.abs(b) + 1e-30), k
+    for g, w in zip(tot, want_tot):
+        assert abs(float(g[0]) - float(w)) <= 1e-6 * max(abs(float(w)), 1e-30)
+    if tilt == 0.0:
+        assert want_tot[1] > 0

@@ -319,3 +319,68 @@ def test_init_plasma_relations():
                 dfrac = (r[ramp] - wb) / (0.5 * wb)
                 assert np.allclose(q[ramp] - q0[ramp], s["srho"] * dfrac, rtol=1e-4,
                                    atol=1e-6)
+
+
+# --- the torus source (code/3d_main.f90:3149-3253) --------------------------------
+
+def test_torus_closed_forms():
+    """With B along z (no field-direction cut) the ring cell on the
+    equator (dr = 0, z = 0: both Gaussians exp(0) = 1) gains exactly
+    den*m of each species; pressures rise by the same amount in x, y, xy
+    and aniso times it in z; momentum gains reduct * corotation velocity;
+    cells away from the torus and inside srf_bndry are untouched; the
+    totals are the injected mass times the cell volume."""
+    from common import FIELDS, torus_case, torus_oracle
+    c = torus_case()
+    d, tot = torus_oracle(c)
+    s = d["s"]
+    f0, f1 = c["f"], d["f"]
+    nz, ny, nx = f0["qrho"].shape[1:]
+    # the ring cell (x = 5, y = 0, z = 0): i = 10 + (nx-1)/2, j, k at the centre
+    i, j, k = (nx - 1) // 2 + 10, (ny - 1) // 2, (nz - 1) // 2
+    for sp, dn, rm in (("q", "denq_torus", "rmassq"), ("h", "denh_torus", "rmassh"),
+                       ("o", "deno_torus", "rmasso")):
+        drho = float(f1[sp + "rho"][0, k, j, i]) - float(f0[sp + "rho"][0, k, j, i])
+        assert abs(drho - s[dn] * s[rm]) <= 1e-6 * s[dn] * s[rm]
+    changed = np.zeros(f0["qrho"][0].shape, bool)
+    for key in FIELDS:
+        changed |= f1[key][0] != f0[key][0]
+    g = np.meshgrid(*(np.arange(n) * 0.5 - (n - 1) * 0.25 for n in (nz, ny, nx)), indexing="ij")
+    zz, yy, xx = g
+    rad = np.sqrt(xx ** 2 + yy ** 2)
+    far = np.abs(rad - 5.0) > 2.01
+    assert not changed[far].any() and changed.any()
+    assert np.array_equal(f1["qrho"][1], f0["qrho"][1])            # box 2 untouched
+    inside = np.sqrt(xx ** 2 + yy ** 2 + zz ** 2) < 1.99
+    assert not changed[inside].any()
+    for sp in "qho":
+        dpx = (f1[sp + "presx"][0] - f0[sp + "presx"][0]).astype(np.float64)
+        assert np.allclose(f1[sp + "presy"][0] - f0[sp + "presy"][0], dpx, atol=1e-6)
+        assert np.allclose(f1[sp + "presz"][0] - f0[sp + "presz"][0], s["aniso_factor"] * dpx,
+                           atol=1e-6)
+        drho = (f1[sp + "rho"][0] - f0[sp + "rho"][0]).astype(np.float64)
+        dmx = (f1[sp + "px"][0] - f0[sp + "px"][0]).astype(np.float64)
+        ring = drho > 0.05
+        assert ring.any()
+        # rigid corotation: vx = -y v_rot (the differences carry the fields'
+        # single-precision rounding, hence the tolerance)
+        assert np.allclose(dmx[ring] / drho[ring], -s["reduct"] * yy[ring] * s["v_rot"],
+                           rtol=1e-3, atol=1e-5)
+    de = (f1["epres"][0] - f0["epres"][0]).astype(np.float64)
+    dsum = sum((f1[sp + "presx"][0] - f0[sp + "presx"][0]).astype(np.float64) for sp in "qho")
+    assert np.allclose(de, dsum, atol=2e-6)
+    vol = 0.5 ** 3
+    for q, sp in enumerate("qho"):
+        want = ((f1[sp + "rho"][0] - f0[sp + "rho"][0]).astype(np.float64) * vol).sum()
+        assert abs(float(tot[q]) - want) <= 1e-4 * abs(want)
+
+
+def test_torus_field_cut():
+    """A field tilted off z by more than 1/sqrt(10) (abtot >= 0.1) cuts the
+    injection off entirely."""
+    from common import FIELDS, torus_case, torus_oracle
+    c = torus_case(tilt=0.4)
+    d, tot = torus_oracle(c)
+    for key in FIELDS:
+        assert np.array_equal(d["f"][key], c["f"][key])
+    assert tot == (0.0, 0.0, 0.0)
