@@ -35,6 +35,8 @@ METRIC = "cell-updates/s, full step, 5 nested 121x121x61 grids; adjoint/forward 
 # interior cell, one field for the FCT kernels), counting each input and
 # output array once -- what the kernel must move if nothing is re-read.
 KERNEL_BYTES = {
+    # algorithmic bytes per unit (each input and output array counted once)
+    # native runtime:
     # per interior cell, 3 species + fields: per species wrk 10 + old 10 +
     # out 10; shared bs 3 + tv 3 + E 3 + ev 3, b old 3 + b out 3, epres
     # wrk/old/out 3 -> 3*30 + 12 + 6 + 3
@@ -45,20 +47,16 @@ KERNEL_BYTES = {
     "k_lap": (68 * 4, "interior"),
     # per cell and field: wrk, old read, out written (P stays in LDS)
     "k_fct_fused": (3 * 4, "all_x_fields"),
-    # per cell and field: w, old read, P written
-    "k_fct_diffuse2": (3 * 4, "all_x_fields"),
-    # per cell and field: P, w read, out written
-    "k_fct_limit": (3 * 4, "all_x_fields"),
+    # per cell: b 3, b0 3, 9 momenta, 3 densities, epres read; E, ev, tv, bs written
+    "k_efield": (31 * 4, "all"),
+    # per shell point and field: target read and written, and the coarse
+    # cells under the shell once (about 0.8 per point: two time levels)
+    "k_flanks_table": (3 * 4, "shell_x_fields"),
+    # per-subroutine path:
     # wrk 10 + old 10 + qv 3 + bs 3 + btot + E 3 + tv 3 reads, 10 writes
     "k_push_ion": (43 * 4, "interior"),
     # v 3 + 10 fields read, 10 written
     "k_lapidus": (23 * 4, "interior"),
-    # per field: P and w read, 3 fluxes written
-    "k_fct_flux": (5 * 4, "all_x_fields"),
-    # per field: P, w, 3 fluxes read, out written
-    "k_fct_apply": (6 * 4, "all_x_fields"),
-    # per field: w, old read, P written
-    "k_fct_diffuse": (3 * 4, "all_x_fields"),
     # 30 species fields read
     "k_speed_flag": (30 * 4, "interior"),
     "k_copy_fields": (2 * 4, "all_x_fields"),
@@ -71,6 +69,8 @@ def units_per_launch(kind, wl, nfields=34):
         return (nx - 2) * (ny - 2) * (nz - 2)
     if kind == "all":
         return nx * ny * nz
+    if kind == "shell_x_fields":   # the 2-layer shell of a box
+        return (4 * nx * ny + 4 * nx * (nz - 4) + 4 * (ny - 4) * (nz - 4)) * nfields
     return nx * ny * nz * nfields
 
 
