@@ -90,7 +90,7 @@ def measured_traffic(workload: str, kernel: str):
     return None, None
 
 
-def batch_case(n: int = 32, steps: int = 20, streams: int = 4):
+def batch_case(n: int = 32, steps: int = 20, streams: int = 8):
     """Throughput of n independent runs stepped one after another (each run
     alone on the GPU) and stepped together (g3m3_amd.driver.Batch)."""
     import torch
@@ -234,7 +234,7 @@ def main(argv=None):
     ap.add_argument("--batch", action="store_true",
                     help="also run the batching case: 32 independent single-grid runs "
                          "(41x41x21 .. 121x121x61), 20 full steps each, one at a time and "
-                         "then together on 4 streams")
+                         "then together on 8 streams")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)

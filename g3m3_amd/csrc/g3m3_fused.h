@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <vector>
+
 #include "g3m3_dev.h"
 #include "g3m3_launch.h"
 
@@ -105,16 +107,28 @@ struct FctParams {
 // the coarse cell (ib-1, jb-1, kb-1), dx/dy/dz = the fractional offsets,
 // flags = bits 0-5 panels bottom, top, front, back, left, right containing
 // the cell, bits 8-13 the same panels when the cell is on their inner layer.
+// cb, sy and sz index the array the coarse values are read from: the coarse
+// box itself (sy = nx, sz = nx*ny) or, after build_shell_slab, the gathered
+// slab (cb a slab index, sy/sz the strides of the cell's face block).
 struct ShellPt {
     int c, cb;
     float dx, dy, dz;
     int flags;
+    int sy, sz;
 };
 
 // the shell of a box in panel-slab order, weights computed as the kernel
 // of the reference computes them (host, IEEE single precision)
 void build_shell(const Box& B, const FlankGeom& G, ShellPt* out);
 int shell_count(const Box& B);
+// The coarse cells a shell reads, as six dense blocks (one per face of the
+// shell: the bounding box of its cells' trilinear corners, x fastest):
+// gather[n] = the coarse cell of slab entry n; the points' cb/sy/sz are
+// rewritten to index the slab.  False if a corner lies outside the box.
+bool build_shell_slab(const Box& B, ShellPt* pts, int npts, std::vector<int>& gather);
+// slab[(2f) n + t] = old[f][idx[t]], slab[(2f+1) n + t] = cur[f][idx[t]]
+void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, float* slab,
+                        hipStream_t s);
 void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const FlankPtrs& Fp, int nf,
                          float t1, float t2, hipStream_t s);
 

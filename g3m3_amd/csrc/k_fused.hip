@@ -11,6 +11,8 @@
 //   k_lap       Lapidus for the 3 species, electrons and B (launch group)
 //   k_fct_fused flux-corrected transport of all 34 fields, one pass
 //   k_flanks_table  coarse -> fine boundary on the 2-layer shell only
+#include <algorithm>
+
 #include "g3m3_dev.h"
 #include "g3m3_fused.h"
 
@@ -927,7 +929,7 @@ int shell_count(const Box& B) {
 }
 
 void build_shell(const Box& B, const FlankGeom& G, ShellPt* out) {
-    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const int nx = B.nx, ny = B.ny, nz = B.nz, Y = nx, Z = nx * ny;
     int n = 0;
     auto add = [&](int i, int j, int k) {
         ShellPt p;
@@ -953,6 +955,8 @@ void build_shell(const Box& B, const FlankGeom& G, ShellPt* out) {
         if (i <= 1) f |= 16 | (i == 1 ? 16 << 8 : 0);
         if (i >= nx - 2) f |= 32 | (i == nx - 2 ? 32 << 8 : 0);
         p.flags = f;
+        p.sy = Y;
+        p.sz = Z;
         out[n++] = p;
     };
     const int ks[4] = {0, 1, nz - 2, nz - 1};
@@ -971,13 +975,84 @@ void build_shell(const Box& B, const FlankGeom& G, ShellPt* out) {
             for (int s = 0; s < 4; ++s) add(is[s], j, k);
 }
 
+// The face a shell cell belongs to, in build_shell's order: the z faces
+// take every cell of their layers, the y faces the rest of theirs, the x
+// faces what remains.
+static int shell_face(int flags) {
+    return (flags & 1) ? 0 : (flags & 2) ? 1 : (flags & 4) ? 2 : (flags & 8) ? 3 : (flags & 16) ? 4 : 5;
+}
+
+bool build_shell_slab(const Box& B, ShellPt* pts, int npts, std::vector<int>& gather) {
+    const int nx = B.nx, ny = B.ny, nz = B.nz, big = 1 << 30;
+    int lo[6][3], hi[6][3];
+    for (int f = 0; f < 6; ++f)
+        for (int a = 0; a < 3; ++a) {
+            lo[f][a] = big;
+            hi[f][a] = -big;
+        }
+    auto split = [&](int c, int v[3]) {
+        v[0] = c % nx;
+        v[1] = (c / nx) % ny;
+        v[2] = c / (nx * ny);
+    };
+    for (int n = 0; n < npts; ++n) {
+        const int f = shell_face(pts[n].flags);
+        int v[3];
+        split(pts[n].cb, v);
+        for (int a = 0; a < 3; ++a) {
+            lo[f][a] = std::min(lo[f][a], v[a]);
+            hi[f][a] = std::max(hi[f][a], v[a] + 1);   // the upper trilinear corner
+        }
+    }
+    int off[6], dim[6][3];
+    gather.clear();
+    for (int f = 0; f < 6; ++f) {
+        off[f] = (int)gather.size();
+        if (lo[f][0] > hi[f][0]) continue;              // no cell on this face
+        if (lo[f][0] < 0 || lo[f][1] < 0 || lo[f][2] < 0 || hi[f][0] >= nx || hi[f][1] >= ny ||
+            hi[f][2] >= nz)
+            return false;
+        for (int a = 0; a < 3; ++a) dim[f][a] = hi[f][a] - lo[f][a] + 1;
+        for (int k = lo[f][2]; k <= hi[f][2]; ++k)
+            for (int j = lo[f][1]; j <= hi[f][1]; ++j)
+                for (int i = lo[f][0]; i <= hi[f][0]; ++i) gather.push_back(B.IX(i, j, k));
+    }
+    for (int n = 0; n < npts; ++n) {
+        ShellPt& p = pts[n];
+        const int f = shell_face(p.flags);
+        int v[3];
+        split(p.cb, v);
+        p.sy = dim[f][0];
+        p.sz = dim[f][0] * dim[f][1];
+        p.cb = off[f] + (v[0] - lo[f][0]) + p.sy * (v[1] - lo[f][1]) + p.sz * (v[2] - lo[f][2]);
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_slab_gather(const int* __restrict__ idx, int n,
+                                                     FlankPtrs Fp, float* __restrict__ slab) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int f = blockIdx.y, c = idx[t];
+    float* s = slab + (size_t)2 * f * n;
+    s[t] = Fp.old[f][c];
+    s[n + t] = Fp.cur[f][c];
+}
+
+void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, float* slab,
+                        hipStream_t s) {
+    if (n <= 0 || nf <= 0) return;
+    hipLaunchKernelGGL(k_slab_gather, dim3((n + 255) / 256, nf), dim3(256), 0, s, idx, n, Fp,
+                       slab);
+}
+
 __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict__ tab, int npts,
-                                                      FlankPtrs Fp, float t1, float t2, int X,
-                                                      int Y, int Z) {
+                                                      FlankPtrs Fp, float t1, float t2) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= npts) return;
     const int f = blockIdx.y;
     const ShellPt p = tab[t];
+    const int X = 1, Y = p.sy, Z = p.sz;
     const float dx = p.dx, dy = p.dy, dz = p.dz;
     const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
     const float* __restrict__ o = Fp.old[f];
@@ -1006,7 +1081,7 @@ void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const Flank
     if (npts <= 0 || nf <= 0) return;
     ProfScope prof_("k_flanks_table", s);
     hipLaunchKernelGGL(k_flanks_table, dim3((npts + 255) / 256, nf), dim3(256), 0, s, tab, npts,
-                       Fp, t1, t2, 1, B.nx, B.nx * B.ny);
+                       Fp, t1, t2);
 }
 
 

@@ -59,6 +59,15 @@ struct Run {
     // flank shell tables of the inner boxes (geometry is fixed for a run)
     ShellPt* shell[G3M3_MAX_GRIDS] = {};
     int nshell = 0;
+    // The coarse cells inner box b+1's shell reads from box b+2, gathered
+    // (old and state, every field) into a dense slab once per coarse
+    // change: the shell's x faces would otherwise fetch a cache line per
+    // 2-3 coarse values on each of the six flank passes a coarse step
+    // serves.  slab_ok[b] clears whenever box b+2's state or old may change.
+    float* slab[G3M3_MAX_GRIDS] = {};
+    int* slab_idx[G3M3_MAX_GRIDS] = {};
+    int nslab[G3M3_MAX_GRIDS] = {};
+    bool slab_ok[G3M3_MAX_GRIDS] = {};
     // captured full steps of a single-grid run: gexec[p] is one full step
     // starting from rotation parity p; the time step and the run time live
     // on the device (dt_dev[0] = the next step's dt, [1] = t_stepnew,
@@ -269,18 +278,35 @@ FlankGeom geom(const g3m3_run_desc& d, int box) {
     return G;
 }
 
+// box's state or old may have changed: the slab of the box inside it is stale
+void coarse_changed(Run* R, int box) {
+    if (box >= 2) R->slab_ok[box - 2] = false;
+}
+void all_coarse_changed(Run* R) {
+    for (bool& v : R->slab_ok) v = false;
+}
+
 void flanks(Run* R, const Set& tgt, int box, double t_grid) {
     const int mb = box;   // 0-based index of box+1
     const float t1 = (float)((R->t_new[mb] - t_grid) / (R->t_new[mb] - R->t_old[mb]));
     const float t2 = 1.0f - t1;
+    const int b = box - 1, n = R->nslab[b];
     FlankPtrs P{};
-    const Set cur = R->state(box + 1), old = R->old(box + 1);
+    if (!R->slab_ok[b]) {
+        const Set cur = R->state(box + 1), old = R->old(box + 1);
+        for (int f = 0; f < NF; ++f) {
+            P.cur[f] = cur.f[f];
+            P.old[f] = old.f[f];
+        }
+        launch_slab_gather(R->slab_idx[b], n, P, NF, R->slab[b], R->s);
+        R->slab_ok[b] = true;
+    }
     for (int f = 0; f < NF; ++f) {
         P.tgt[f] = tgt.f[f];
-        P.cur[f] = cur.f[f];
-        P.old[f] = old.f[f];
+        P.old[f] = R->slab[b] + (size_t)2 * f * n;
+        P.cur[f] = R->slab[b] + (size_t)(2 * f + 1) * n;
     }
-    launch_flanks_table(R->B, R->shell[box - 1], R->nshell, P, NF, t1, t2, R->s);
+    launch_flanks_table(R->B, R->shell[b], R->nshell, P, NF, t1, t2, R->s);
 }
 
 void inner_bc(Run* R, const Set& set) {
@@ -435,6 +461,7 @@ void tune_fct(Run* R) {
 
 void corer(Run* R, int box) {
     const g3m3_run_desc& d = R->d;
+    coarse_changed(R, box + 1);
     const int mb = box, ms = box - 1;
     const float sx = (d.grd_xmax[ms] - d.grd_xmin[ms]) / (d.nx - 1.0f);
     const float dx = (d.grd_xmax[mb] - d.grd_xmin[mb]) / (d.nx - 1.0f);
@@ -465,6 +492,7 @@ void box_step(Run* R, int box) {
     // step 2 writes the new state into the other slice (its push copies the
     // cells it does not compute from old)
     R->rot[b] ^= 1;
+    coarse_changed(R, box);
     const Set st = R->state(box), ol = R->old(box), wk = R->wrk(box);
     auto clamp_t = [&](double tg) {
         if (tg > R->t_new[b + 1]) tg = R->t_new[b + 1];
@@ -667,11 +695,21 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
     }
     R->nshell = shell_count(R->B);
     for (int box = 1; ok && box < ng; ++box) {
+        const int b = box - 1;
         std::vector<ShellPt> h(R->nshell);
+        std::vector<int> gather;
         build_shell(R->B, geom(R->d, box), h.data());
-        ok = hipMalloc(&R->shell[box - 1], sizeof(ShellPt) * h.size()) == hipSuccess &&
-             hipMemcpy(R->shell[box - 1], h.data(), sizeof(ShellPt) * h.size(),
-                       hipMemcpyHostToDevice) == hipSuccess;
+        ok = build_shell_slab(R->B, h.data(), R->nshell, gather);
+        if (!ok) fprintf(stderr, "g3m3_run_create: box %d's shell reads outside box %d\n", box,
+                         box + 1);
+        R->nslab[b] = (int)gather.size();
+        ok = ok && hipMalloc(&R->shell[b], sizeof(ShellPt) * h.size()) == hipSuccess &&
+             hipMemcpy(R->shell[b], h.data(), sizeof(ShellPt) * h.size(),
+                       hipMemcpyHostToDevice) == hipSuccess &&
+             hipMalloc(&R->slab_idx[b], sizeof(int) * gather.size()) == hipSuccess &&
+             hipMemcpy(R->slab_idx[b], gather.data(), sizeof(int) * gather.size(),
+                       hipMemcpyHostToDevice) == hipSuccess &&
+             hipMalloc(&R->slab[b], sizeof(float) * 2 * NF * gather.size()) == hipSuccess;
     }
     ok = ok && hipMemset(R->count, 0, 128) == hipSuccess &&
          hipMemset(R->maxima, 0, sizeof(float) * 5 * ng) == hipSuccess;
@@ -701,6 +739,10 @@ void g3m3_run_destroy(void* run) {
         if (p) (void)hipFree(p);
     for (ShellPt* p : R->shell)
         if (p) (void)hipFree(p);
+    for (float* p : R->slab)
+        if (p) (void)hipFree(p);
+    for (int* p : R->slab_idx)
+        if (p) (void)hipFree(p);
     for (int q = 0; q < 2; ++q) {
         if (R->gexec[q]) (void)hipGraphExecDestroy(R->gexec[q]);
         if (R->graph[q]) (void)hipGraphDestroy(R->graph[q]);
@@ -716,6 +758,7 @@ void g3m3_run_set_stream(void* run, void* stream) { ((Run*)run)->s = (hipStream_
 
 int g3m3_run_init_dt(void* run) {
     Run* R = (Run*)run;
+    all_coarse_changed(R);   // the caller may have written the arrays
     std::vector<int> boxes;
     for (int box = R->d.n_grids; box >= 1; --box) {
         R->t_old[box - 1] = 0.0;
@@ -736,6 +779,7 @@ int g3m3_run_full_step(void* run, const double* dt_override) {
 
 int g3m3_run_full_step_async(void* run, const double* dt_override) {
     Run* R = (Run*)run;
+    all_coarse_changed(R);   // the caller may have written the arrays
     const int n = R->d.n_grids;
     std::vector<double>& ts = R->t_step;
     // dt synchronisation (code/3d_main.f90:1929-1968)
@@ -790,6 +834,7 @@ int g3m3_run_full_step_async(void* run, const double* dt_override) {
     for (int box = 1; box < n; ++box) {
         corer(R, box);
         set_rho_only(R, R->state(box));
+        coarse_changed(R, box);
     }
     R->stepped = stepped;
     check(R, "full_step");
@@ -874,6 +919,7 @@ int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap) {
 
 int g3m3_run_sync_arrays(void* run) {
     Run* R = (Run*)run;
+    all_coarse_changed(R);   // the caller may have written the arrays
     for (int box = 1; box <= R->d.n_grids; ++box) {
         if (!R->rot[box - 1]) continue;
         const Set a = R->slice(R->d.state, box), o = R->slice(R->d.old, box);
@@ -895,6 +941,7 @@ void g3m3_run_set_time(void* run, double t) { ((Run*)run)->t = t; }
 
 int g3m3_run_box_step(void* run, int box) {
     Run* R = (Run*)run;
+    all_coarse_changed(R);   // the caller may have written the arrays
     box_step(R, box);
     return R->err;
 }

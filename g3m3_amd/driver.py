@@ -533,7 +533,7 @@ class FastRun(Run):
 class Batch:
     """Independent runs (any sizes) stepped together on `streams` streams."""
 
-    def __init__(self, workloads, streams: int = 4):
+    def __init__(self, workloads, streams: int = 8):
         torch = _torch()
         self.runs = [FastRun(wl) for wl in workloads]
         torch.cuda.synchronize()   # uploads done before other streams touch the arrays
