@@ -53,6 +53,7 @@ struct PushParams {
     float xmin, ymin, zmin, ddx, ddy, ddz, re_equiv, grav, gamma, gamma1;
     int do_elec;
     int copy_shell;             // copy non-interior cells old -> out
+    int march;                  // species by k_push_march; k_push does B, electrons, shells
     // time step from device memory (captured steps): delt = dt_scale *
     // *dt_dev, and bmax is formed from it; null: delt and bmax above
     const float* dt_dev;

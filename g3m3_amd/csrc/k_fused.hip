@@ -162,15 +162,17 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
         const float tx = (P.mom[0][c] + P.mom[3][c] + P.mom[6][c]) / tden;
         const float ty = (P.mom[1][c] + P.mom[4][c] + P.mom[7][c]) / tden;
         const float tz = (P.mom[2][c] + P.mom[5][c] + P.mom[8][c]) / tden;
+        const float bsx = P.b0[0][c] + P.b[0][c], bsy = P.b0[1][c] + P.b[1][c],
+                    bsz = P.b0[2][c] + P.b[2][c];
         P.ev[0][c] = tx - cx / tot / P.reynolds;
         P.ev[1][c] = ty - cy / tot / P.reynolds;
         P.ev[2][c] = tz - cz / tot / P.reynolds;
         P.tv[0][c] = tx;
         P.tv[1][c] = ty;
         P.tv[2][c] = tz;
-        P.bs[0][c] = P.b0[0][c] + P.b[0][c];
-        P.bs[1][c] = P.b0[1][c] + P.b[1][c];
-        P.bs[2][c] = P.b0[2][c] + P.b[2][c];
+        P.bs[0][c] = bsx;
+        P.bs[1][c] = bsy;
+        P.bs[2][c] = bsz;
     }
     P.e[0][c] = Ex;
     P.e[1][c] = Ey;
@@ -197,13 +199,40 @@ __device__ inline float dvp(float x, float d, float r) {
     return P2 ? x * r : x / d;
 }
 
-template <bool P2>
-__device__ inline void push_species(const PushParams& P, int s, int i, int j, int k) {
-    const Box& B = P.B;
-    const int c = B.IX(i, j, k), X = 1, Y = B.nx, Z = B.nx * B.ny;
-    auto put = [&](int m, float v) { P.out[s].f[m][c] = v; };
-    const SpeciesSet& W = P.src[s];
-    const SpeciesSet& O = P.old[s];
+// Stencil directions of the push: 0 the cell, 1 +x, 2 -x, 3 +y, 4 -y,
+// 5 +z, 6 -z.  An accessor gives w(m, d) = species array m (0 rho, 1-3
+// presx..presz, 4-6 px..pz, 7-9 presxy, presxz, presyz), v(q, d) = the
+// velocity component q (px/max(rho, 1e-6) etc.) and b(q, d) = total B
+// component q at direction d; push_species_core is the arithmetic, the same
+// for every accessor, so every kernel that uses it produces the same bits.
+// The operands the push reads at the cell only: the species' old arrays,
+// the v x B term and E.  Loaded before anything is stored: on this hardware
+// a wait for a load also waits for every store issued before it.
+struct PushCentre {
+    float o[10], tv[3], e[3];
+};
+
+__device__ inline void push_centre_load(const PushParams& P, int s, int c, PushCentre& C) {
+    const unsigned ob = (unsigned)c * 4u;
+    auto ld = [&](const float* base) {
+        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + ob);
+    };
+#pragma unroll
+    for (int m = 0; m < 10; ++m) C.o[m] = ld(P.old[s].f[m]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) C.tv[q] = ld(P.tv[q]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) C.e[q] = ld(P.e[q]);
+}
+
+template <bool P2, class A>
+__device__ inline void push_species_core(const PushParams& P, int s, int i, int j, int k, int c,
+                                         const A& a, const PushCentre& C) {
+    const float* o = C.o;
+    const float tx = C.tv[0], ty = C.tv[1], tz = C.tv[2];
+    const float ex = C.e[0], ey = C.e[1], ez = C.e[2];
+    float res[10];
+    auto put = [&](int m, float v) { res[m] = v; };
     const float rmass = P.rmass[s], rrmass = P.rrmass[s];
     const float delt = step_delt(P), reynolds = P.reynolds;
     auto Dx = [&](float x) { return dvp<P2>(x, P.dxt, P.rdxt); };
@@ -212,66 +241,46 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     auto Dm = [&](float x) { return dvp<P2>(x, rmass, rrmass); };
     // 0.20*rmass/delt/reynolds (code/sub_push_ion.f90:64)
     const float bmax = P.dt_dev ? 0.20f * rmass / delt / reynolds : P.bmax[s];
-    const float* __restrict__ wr = W.f[0];
-    const float* __restrict__ wx = W.f[1];
-    const float* __restrict__ wy = W.f[2];
-    const float* __restrict__ wz = W.f[3];
-    const float* __restrict__ wpx = W.f[4];
-    const float* __restrict__ wpy = W.f[5];
-    const float* __restrict__ wpz = W.f[6];
-    const float* __restrict__ wxy = W.f[7];
-    const float* __restrict__ wxz = W.f[8];
-    const float* __restrict__ wyz = W.f[9];
     // velocities at the centre and the six neighbours (code/sub_push_ion.f90:67-81)
-    float vx[7], vy[7], vz[7];
-    const int off[7] = {0, X, -X, Y, -Y, Z, -Z};
-#pragma unroll
-    for (int m = 0; m < 7; ++m) {
-        const int p = c + off[m];
-        const float a = fmaxf(wr[p], 1.e-6f);
-        vx[m] = wpx[p] / a;
-        vy[m] = wpy[p] / a;
-        vz[m] = wpz[p] / a;
-    }
-    // index into vx etc: 0 c, 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z
-    auto flux = [&](const float* __restrict__ w) {
-        return Dx(w[c + X] * vx[1] - w[c - X] * vx[2]) + Dy(w[c + Y] * vy[3] - w[c - Y] * vy[4]) +
-               Dz(w[c + Z] * vz[5] - w[c - Z] * vz[6]);
+    auto vx = [&](int d) { return a.v(0, d); };
+    auto vy = [&](int d) { return a.v(1, d); };
+    auto vz = [&](int d) { return a.v(2, d); };
+    auto flux = [&](int m) {
+        return Dx(a.w(m, 1) * vx(1) - a.w(m, 2) * vx(2)) +
+               Dy(a.w(m, 3) * vy(3) - a.w(m, 4) * vy(4)) +
+               Dz(a.w(m, 5) * vz(5) - a.w(m, 6) * vz(6));
     };
-    put(0, O.f[0][c] - delt * (Dx(wpx[c + X] - wpx[c - X]) + Dy(wpy[c + Y] - wpy[c - Y]) +
-                               Dz(wpz[c + Z] - wpz[c - Z])));
-    const float aqpx = O.f[4][c] - delt * flux(wpx);
-    const float aqpy = O.f[5][c] - delt * flux(wpy);
-    const float aqpz = O.f[6][c] - delt * flux(wpz);
-    const float* __restrict__ bsx = P.bs[0];
-    const float* __restrict__ bsy = P.bs[1];
-    const float* __restrict__ bsz = P.bs[2];
-    float abx = bsx[c], aby = bsy[c], abz = bsz[c];
+    enum { RHO = 0, WX = 1, WY = 2, WZ = 3, PX = 4, PY = 5, PZ = 6, WXY = 7, WXZ = 8, WYZ = 9 };
+    put(0, o[0] - delt * (Dx(a.w(PX, 1) - a.w(PX, 2)) + Dy(a.w(PY, 3) - a.w(PY, 4)) +
+                               Dz(a.w(PZ, 5) - a.w(PZ, 6))));
+    const float aqpx = o[4] - delt * flux(PX);
+    const float aqpy = o[5] - delt * flux(PY);
+    const float aqpz = o[6] - delt * flux(PZ);
+    float abx = a.b(0, 0), aby = a.b(1, 0), abz = a.b(2, 0);
     const float bmag0 = fmaxf(sqrtf(abx * abx + aby * aby + abz * abz), P.smallbit);
     const float weight = fmaxf(1.0f, bmag0 / bmax);
     const float skin_depth = reynolds / weight;
-    const float tx = P.tv[0][c], ty = P.tv[1][c], tz = P.tv[2][c];
     const float tcx = (ty * abz - tz * aby);
     const float tcy = -(tx * abz - tz * abx);
     const float tcz = (tx * aby - ty * abx);
-    const float qcx = (vy[0] * abz - vz[0] * aby);
-    const float qcy = -(vx[0] * abz - vz[0] * abx);
-    const float qcz = (vx[0] * aby - vy[0] * abx);
+    const float qcx = (vy(0) * abz - vz(0) * aby);
+    const float qcy = -(vx(0) * abz - vz(0) * abx);
+    const float qcz = (vx(0) * aby - vy(0) * abx);
     // the reference divides the y and z cross terms by dxt as well
     // (code/sub_push_ion.f90:155-164)
-    const float gpx = Dx(wx[c + X] - wx[c - X]) + Dx(wxy[c + Y] - wxy[c - Y]) +
-                      Dx(wxz[c + Z] - wxz[c - Z]);
-    const float gpy = Dx(wxy[c + X] - wxy[c - X]) + Dy(wy[c + Y] - wy[c - Y]) +
-                      Dx(wyz[c + Z] - wyz[c - Z]);
-    const float gpz = Dx(wxz[c + X] - wxz[c - X]) + Dy(wyz[c + Y] - wyz[c - Y]) +
-                      Dz(wz[c + Z] - wz[c - Z]);
-    const float dex = (P.e[0][c] + tcx) * reynolds;
-    const float dey = (P.e[1][c] + tcy) * reynolds;
-    const float dez = (P.e[2][c] + tcz) * reynolds;
+    const float gpx = Dx(a.w(WX, 1) - a.w(WX, 2)) + Dx(a.w(WXY, 3) - a.w(WXY, 4)) +
+                      Dx(a.w(WXZ, 5) - a.w(WXZ, 6));
+    const float gpy = Dx(a.w(WXY, 1) - a.w(WXY, 2)) + Dy(a.w(WY, 3) - a.w(WY, 4)) +
+                      Dx(a.w(WYZ, 5) - a.w(WYZ, 6));
+    const float gpz = Dx(a.w(WXZ, 1) - a.w(WXZ, 2)) + Dy(a.w(WYZ, 3) - a.w(WYZ, 4)) +
+                      Dz(a.w(WZ, 5) - a.w(WZ, 6));
+    const float dex = (ex + tcx) * reynolds;
+    const float dey = (ey + tcy) * reynolds;
+    const float dez = (ez + tcz) * reynolds;
     const float dvx = (qcx - tcx) * skin_depth;
     const float dvy = (qcy - tcy) * skin_depth;
     const float dvz = (qcz - tcz) * skin_depth;
-    const float wrc = wr[c];
+    const float wrc = a.w(RHO, 0);
     const float qden = Dm(wrc);
     const float px = aqpx + delt * (qden * (dex + dvx) - gpx);
     const float py = aqpy + delt * (qden * (dey + dvy) - gpy);
@@ -286,42 +295,237 @@ __device__ inline void push_species(const PushParams& P, int s, int i, int j, in
     put(5, py - g * ay / radius);
     put(6, pz - g * az / radius);
 
-    const float cx = wx[c], cy = wy[c], cz = wz[c];
-    const float cxy = wxy[c], cxz = wxz[c], cyz = wyz[c];
-    const float px0 = O.f[1][c] - delt * flux(wx) - Dx(2.0f * delt * cx * (vx[1] - vx[2]));
-    const float py0 = O.f[2][c] - delt * flux(wy) - Dy(2.0f * delt * cy * (vy[3] - vy[4]));
-    const float pz0 = O.f[3][c] - delt * flux(wz) - Dz(2.0f * delt * cz * (vz[5] - vz[6]));
-    abx = (6.0f * bsx[c] + bsx[c + X] + bsx[c - X] + bsx[c + Y] + bsx[c - Y] + bsx[c + Z] +
-           bsx[c - Z]) / 12.0f;
-    aby = (6.0f * bsy[c] + bsy[c + X] + bsy[c - X] + bsy[c + Y] + bsy[c - Y] + bsy[c + Z] +
-           bsy[c - Z]) / 12.0f;
-    abz = (6.0f * bsz[c] + bsz[c + X] + bsz[c - X] + bsz[c + Y] + bsz[c - Y] + bsz[c + Z] +
-           bsz[c - Z]) / 12.0f;
+    const float cx = a.w(WX, 0), cy = a.w(WY, 0), cz = a.w(WZ, 0);
+    const float cxy = a.w(WXY, 0), cxz = a.w(WXZ, 0), cyz = a.w(WYZ, 0);
+    const float px0 = o[1] - delt * flux(WX) - Dx(2.0f * delt * cx * (vx(1) - vx(2)));
+    const float py0 = o[2] - delt * flux(WY) - Dy(2.0f * delt * cy * (vy(3) - vy(4)));
+    const float pz0 = o[3] - delt * flux(WZ) - Dz(2.0f * delt * cz * (vz(5) - vz(6)));
+    abx = (6.0f * a.b(0, 0) + a.b(0, 1) + a.b(0, 2) + a.b(0, 3) + a.b(0, 4) + a.b(0, 5) +
+           a.b(0, 6)) / 12.0f;
+    aby = (6.0f * a.b(1, 0) + a.b(1, 1) + a.b(1, 2) + a.b(1, 3) + a.b(1, 4) + a.b(1, 5) +
+           a.b(1, 6)) / 12.0f;
+    abz = (6.0f * a.b(2, 0) + a.b(2, 1) + a.b(2, 2) + a.b(2, 3) + a.b(2, 4) + a.b(2, 5) +
+           a.b(2, 6)) / 12.0f;
     const float bmag = sqrtf(abx * abx + aby * aby + abz * abz);
     float sf = reynolds;
     if (bmag > bmax) sf = reynolds * (bmax / bmag);
     const float t2 = 2.0f * delt;
-    put(1, px0 - Dy(t2 * cxy * (vx[3] - vx[4])) - Dz(t2 * cxz * (vx[5] - vx[6])) +
+    put(1, px0 - Dy(t2 * cxy * (vx(3) - vx(4))) - Dz(t2 * cxz * (vx(5) - vx(6))) +
                Dm(t2 * sf) * (cxy * abz - cxz * aby));
-    put(2, py0 - Dx(t2 * cxy * (vy[1] - vy[2])) - Dz(t2 * cyz * (vy[5] - vy[6])) +
+    put(2, py0 - Dx(t2 * cxy * (vy(1) - vy(2))) - Dz(t2 * cyz * (vy(5) - vy(6))) +
                Dm(t2 * sf) * (cyz * abx - cxy * abz));
-    put(3, pz0 - Dx(t2 * cxz * (vz[1] - vz[2])) - Dy(t2 * cyz * (vz[3] - vz[4])) +
+    put(3, pz0 - Dx(t2 * cxz * (vz(1) - vz(2))) - Dy(t2 * cyz * (vz(3) - vz(4))) +
                Dm(t2 * sf) * (cxz * aby - cyz * abx));
-    const float pxy = O.f[7][c] - delt * flux(wxy);
-    put(7, pxy - delt * cxy * (Dx(vx[1] - vx[2]) + Dy(vy[3] - vy[4])) -
-               Dy(delt * cy * (vx[3] - vx[4])) - Dx(delt * cx * (vy[1] - vy[2])) -
-               Dz(delt * cyz * (vx[5] - vx[6])) - Dz(delt * cxz * (vy[5] - vy[6])) +
+    const float pxy = o[7] - delt * flux(WXY);
+    put(7, pxy - delt * cxy * (Dx(vx(1) - vx(2)) + Dy(vy(3) - vy(4))) -
+               Dy(delt * cy * (vx(3) - vx(4))) - Dx(delt * cx * (vy(1) - vy(2))) -
+               Dz(delt * cyz * (vx(5) - vx(6))) - Dz(delt * cxz * (vy(5) - vy(6))) +
                Dm(delt * sf) * (abz * (cy - cx) - aby * cyz + abx * cxz));
-    const float pxz = O.f[8][c] - delt * flux(wxz);
-    put(8, pxz - delt * cxz * (Dx(vx[1] - vx[2]) + Dz(vz[5] - vz[6])) -
-               Dz(delt * cz * (vx[5] - vx[6])) - Dx(delt * cx * (vz[1] - vz[2])) -
-               Dy(delt * cyz * (vx[3] - vx[4])) - Dy(delt * cxy * (vz[3] - vz[4])) +
+    const float pxz = o[8] - delt * flux(WXZ);
+    put(8, pxz - delt * cxz * (Dx(vx(1) - vx(2)) + Dz(vz(5) - vz(6))) -
+               Dz(delt * cz * (vx(5) - vx(6))) - Dx(delt * cx * (vz(1) - vz(2))) -
+               Dy(delt * cyz * (vx(3) - vx(4))) - Dy(delt * cxy * (vz(3) - vz(4))) +
                Dm(delt * sf) * (aby * (cx - cz) - abx * cxy + abz * cyz));
-    const float pyz = O.f[9][c] - delt * flux(wyz);
-    put(9, pyz - delt * cyz * (Dy(vy[3] - vy[4]) + Dz(vz[5] - vz[6])) -
-               Dy(delt * cy * (vz[3] - vz[4])) - Dz(delt * cz * (vy[5] - vy[6])) -
-               Dx(delt * cxy * (vz[1] - vz[2])) - Dx(delt * cxz * (vy[1] - vy[2])) +
+    const float pyz = o[9] - delt * flux(WYZ);
+    put(9, pyz - delt * cyz * (Dy(vy(3) - vy(4)) + Dz(vz(5) - vz(6))) -
+               Dy(delt * cy * (vz(3) - vz(4))) - Dz(delt * cz * (vy(5) - vy(6))) -
+               Dx(delt * cxy * (vz(1) - vz(2))) - Dx(delt * cxz * (vy(1) - vy(2))) +
                Dm(delt * sf) * (abx * (cz - cy) - abz * cxz + aby * cxy));
+    const unsigned ob = (unsigned)c * 4u;
+#pragma unroll
+    for (int m = 0; m < 10; ++m)
+        *reinterpret_cast<float*>(reinterpret_cast<char*>(P.out[s].f[m]) + ob) = res[m];
+}
+
+// The velocity of code/sub_push_ion.f90:67-81 at one point
+__device__ inline float push_vel(float p, float rho) { return p / fmaxf(rho, 1.e-6f); }
+
+// Reads every operand from global memory at the cell and its six
+// neighbours; the seven velocities are formed by each thread.
+struct PushGlobalAcc {
+    const float* const* wp;
+    const float* const* bs;
+    int c, off[7];
+    float vel[3][7];
+    __device__ PushGlobalAcc(const PushParams& P, int s, int c_)
+        : wp(P.src[s].f), bs(P.bs), c(c_) {
+        const int X = 1, Y = P.B.nx, Z = P.B.nx * P.B.ny;
+        off[0] = 0, off[1] = X, off[2] = -X, off[3] = Y, off[4] = -Y, off[5] = Z, off[6] = -Z;
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            const float r = wp[0][c + off[d]];
+            vel[0][d] = push_vel(wp[4][c + off[d]], r);
+            vel[1][d] = push_vel(wp[5][c + off[d]], r);
+            vel[2][d] = push_vel(wp[6][c + off[d]], r);
+        }
+    }
+    __device__ float w(int m, int d) const { return wp[m][c + off[d]]; }
+    __device__ float v(int q, int d) const { return vel[q][d]; }
+    __device__ float b(int q, int d) const { return bs[q][c + off[d]]; }
+};
+
+template <bool P2>
+__device__ inline void push_species(const PushParams& P, int s, int i, int j, int k) {
+    const int c = P.B.IX(i, j, k);
+    PushCentre C;
+    push_centre_load(P, s, c, C);
+    const PushGlobalAcc a(P, s, c);
+    push_species_core<P2>(P, s, i, j, k, c, a, C);
+}
+
+// ---------------------------------------------------------------------------
+// The species push marching along k.  A block (4 waves) owns a 62 x 4
+// (i, j) tile of one species and a run of MKC k-planes.  Lane l of wave w
+// holds cell (i0 - 1 + l, j0 + w): lanes 0 and 63 are the x halo, so a row
+// is one aligned 64-wide load and lanes 1..62 produce output.  Waves 0 and
+// 1 also load the tile's y-halo rows j0 - 1 and j0 + 4.
+//
+// Per plane, LDS holds the 15 arrays the stencil reads at x/y neighbours
+// (the 3 velocities, the 9 transported species arrays, the 3 B components)
+// on the 64 x 6 tile; a 2-slot ring holds planes k and k+1, so the +z
+// neighbour is the next slot and the -z neighbour is kept in registers.
+// Each point's velocity is formed once, by the thread that loads it.
+// Plane k+2 is loaded at the start of plane k's iteration and written into
+// plane k's slot after one barrier, so the loads stay in flight during the
+// arithmetic.  Loads are clamped into the box; lanes and rows outside the
+// interior store nothing.
+constexpr int MTX = 64, MOX = 62, MTY = 4, MRY = MTY + 2, MKC = 16, MNA = 15;
+constexpr int MNR = 13;     // values loaded per point: rho, arrays 1-9, 3 B components
+
+struct MarchLds {
+    float a[2][MNA][MRY][MTX];
+};
+
+// LDS index of species array m (1-9) and of B component q
+__device__ constexpr int mw(int m) { return m + 2; }
+__device__ constexpr int mb(int q) { return 12 + q; }
+
+struct PushMarchAcc {
+    const MarchLds& L;
+    int cur, nxt, r, l;
+    const float* prev;      // the 15 arrays at (i, j, k-1)
+    float rho;              // rho at the cell
+    __device__ float at(int x, int d) const {
+        switch (d) {
+            case 0: return L.a[cur][x][r][l];
+            case 1: return L.a[cur][x][r][l + 1];
+            case 2: return L.a[cur][x][r][l - 1];
+            case 3: return L.a[cur][x][r + 1][l];
+            case 4: return L.a[cur][x][r - 1][l];
+            case 5: return L.a[nxt][x][r][l];
+            default: return prev[x];
+        }
+    }
+    __device__ float w(int m, int d) const { return m == 0 ? rho : at(mw(m), d); }
+    __device__ float v(int q, int d) const { return at(q, d); }
+    __device__ float b(int q, int d) const { return at(mb(q), d); }
+};
+
+// base[byte offset]: with a block-uniform base the load takes the base
+// from scalar registers and only the 32-bit offset from a vector register
+__device__ inline float ldb(const float* base, unsigned off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+
+// the 13 values of one point (byte offset ob), in the order of MNR
+__device__ inline void march_load(const PushParams& P, int s, unsigned ob, float* v) {
+    const SpeciesSet& W = P.src[s];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) v[m] = ldb(W.f[m], ob);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[10 + q] = ldb(P.bs[q], ob);
+}
+
+__device__ inline void march_store(MarchLds& L, int slot, int row, int l, const float* v) {
+    L.a[slot][0][row][l] = push_vel(v[4], v[0]);
+    L.a[slot][1][row][l] = push_vel(v[5], v[0]);
+    L.a[slot][2][row][l] = push_vel(v[6], v[0]);
+#pragma unroll
+    for (int m = 1; m < 10; ++m) L.a[slot][mw(m)][row][l] = v[m];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) L.a[slot][mb(q)][row][l] = v[10 + q];
+}
+
+template <bool P2>
+__global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx, int nty, int nkc) {
+    __shared__ MarchLds L;
+    const Box B = P.B;
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    int t = xcd_block(blockIdx.x, gridDim.x);
+    const int tx = t % ntx;
+    t /= ntx;
+    const int ty = t % nty;
+    t /= nty;
+    const int kc = t % nkc;
+    const int s = t / nkc;
+    if (s >= 3) return;                      // grid padding: whole blocks
+    const int l = threadIdx.x, w = threadIdx.y;
+    const int i = tx * MOX + l, j0 = 1 + ty * MTY, j = j0 + w;
+    const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+    const bool out = l >= 1 && l <= MOX && i <= nx - 2 && j <= ny - 2;
+    // the y-halo row this wave loads (waves 0 and 1), and its LDS row
+    const bool halo = w < 2;
+    const int hj = w == 0 ? j0 - 1 : min(j0 + MTY, ny - 1), hr = w == 0 ? 0 : MRY - 1;
+    const int r = w + 1;
+    const int kb = 1 + kc * MKC, ke = min(kb + MKC, nz - 1);
+    const int Z = nx * ny;
+    const int own0 = ic + nx * jc, halo0 = ic + nx * hj;
+    float prev[MNA];
+    {
+        float v[MNR];
+        march_load(P, s, 4u * (own0 + Z * (kb - 1)), v);
+        prev[0] = push_vel(v[4], v[0]);
+        prev[1] = push_vel(v[5], v[0]);
+        prev[2] = push_vel(v[6], v[0]);
+#pragma unroll
+        for (int m = 1; m < 10; ++m) prev[mw(m)] = v[m];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) prev[mb(q)] = v[10 + q];
+    }
+    float rho_cur, rho_nxt;
+    {
+        float v[MNR];
+        march_load(P, s, 4u * (own0 + Z * kb), v);
+        march_store(L, 0, r, l, v);
+        rho_cur = v[0];
+        if (halo) {
+            march_load(P, s, 4u * (halo0 + Z * kb), v);
+            march_store(L, 0, hr, l, v);
+        }
+        march_load(P, s, 4u * (own0 + Z * (kb + 1)), v);
+        march_store(L, 1, r, l, v);
+        rho_nxt = v[0];
+        if (halo) {
+            march_load(P, s, 4u * (halo0 + Z * (kb + 1)), v);
+            march_store(L, 1, hr, l, v);
+        }
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = kb; k < ke; ++k) {
+        const int nxt = cur ^ 1;
+        // (prefetching these a plane ahead measured no faster: 92.7 against
+        // 91.3 us, for 34 more VGPRs)
+        PushCentre cc;
+        push_centre_load(P, s, own0 + Z * k, cc);
+        // plane k+2 (clamped into the box), in flight during plane k
+        const int k2 = min(k + 2, nz - 1);
+        float vo[MNR], vh[MNR];
+        march_load(P, s, 4u * (own0 + Z * k2), vo);
+        if (halo) march_load(P, s, 4u * (halo0 + Z * k2), vh);
+        if (out) {
+            const PushMarchAcc a{L, cur, nxt, r, l, prev, rho_cur};
+            push_species_core<P2>(P, s, i, j, k, own0 + Z * k, a, cc);
+        }
+#pragma unroll
+        for (int x = 0; x < MNA; ++x) prev[x] = L.a[cur][x][r][l];
+        __syncthreads();                     // plane k's slot is free
+        march_store(L, cur, r, l, vo);
+        if (halo) march_store(L, cur, hr, l, vh);
+        rho_cur = rho_nxt;
+        rho_nxt = vo[0];
+        cur = nxt;
+    }
 }
 
 template <bool P2>
@@ -335,10 +539,14 @@ __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k)
     const float* __restrict__ ex = P.e[0];
     const float* __restrict__ ey = P.e[1];
     const float* __restrict__ ez = P.e[2];
-    // code/sub_push_bfld.f90:87-97
-    P.bout[0][c] = P.bold[0][c] - delt * (Dy(ez[c + sy] - ez[c - sy]) - Dz(ey[c + sz] - ey[c - sz]));
-    P.bout[1][c] = P.bold[1][c] + delt * (Dx(ez[c + 1] - ez[c - 1]) - Dz(ex[c + sz] - ex[c - sz]));
-    P.bout[2][c] = P.bold[2][c] - delt * (Dx(ey[c + 1] - ey[c - 1]) - Dy(ex[c + sy] - ex[c - sy]));
+    // code/sub_push_bfld.f90:87-97; stored after every load (see push_shell_copy)
+    const float b0 =
+        P.bold[0][c] - delt * (Dy(ez[c + sy] - ez[c - sy]) - Dz(ey[c + sz] - ey[c - sz]));
+    const float b1 =
+        P.bold[1][c] + delt * (Dx(ez[c + 1] - ez[c - 1]) - Dz(ex[c + sz] - ex[c - sz]));
+    const float b2 =
+        P.bold[2][c] - delt * (Dx(ey[c + 1] - ey[c - 1]) - Dy(ex[c + sy] - ex[c - sy]));
+    float ep = 0.0f;
     if (P.do_elec) {   // code/sub_push_elec.f90:31-47
         const float* __restrict__ w = P.wrkepres;
         const float* __restrict__ evx = P.ev[0];
@@ -351,30 +559,73 @@ __device__ inline void push_field_elec(const PushParams& P, int i, int j, int k)
                           Dy(w[c + sy] * evy[c + sy] - w[c - sy] * evy[c - sy]) +
                           Dz(w[c + sz] * evz[c + sz] - w[c - sz] * evz[c - sz]);
         const float e = P.oldepres[c] - delt * P.gamma * div;
-        P.epres[c] = e + delt * P.gamma1 * (evx[c] * gx + evy[c] * gy + evz[c] * gz);
+        ep = e + delt * P.gamma1 * (evx[c] * gx + evy[c] * gy + evz[c] * gz);
     }
+    P.bout[0][c] = b0;
+    P.bout[1][c] = b1;
+    P.bout[2][c] = b2;
+    if (P.do_elec) P.epres[c] = ep;
 }
 
 // Cells the push does not write keep their pre-step values.  With the
 // runtime's buffer rotation the output buffer holds stale data there, so
 // when copy_shell is set the boundary threads copy them from old.
+// Every value is loaded before the first store: the compiler cannot prove
+// that the output arrays do not alias the inputs, so interleaved copies
+// would wait for each load in turn.
 __device__ inline void push_shell_copy(const PushParams& P, int s, int c) {
     if (s < 3) {
+        float v[10];
 #pragma unroll
-        for (int m = 0; m < 10; ++m) P.out[s].f[m][c] = P.old[s].f[m][c];
+        for (int m = 0; m < 10; ++m) v[m] = P.old[s].f[m][c];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) P.out[s].f[m][c] = v[m];
     } else {
+        float v[4];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) P.bout[m][c] = P.bold[m][c];
-        P.epres[c] = P.oldepres[c];
+        for (int m = 0; m < 3; ++m) v[m] = P.bold[m][c];
+        v[3] = P.oldepres[c];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) P.bout[m][c] = v[m];
+        P.epres[c] = v[3];
     }
 }
 
+// all four groups' shells of one cell (the march launch): 34 loads, then
+// 34 stores
+__device__ inline void push_shell_copy_all(const PushParams& P, int c) {
+    float v[34];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[10 * s + m] = P.old[s].f[m][c];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) v[30 + m] = P.bold[m][c];
+    v[33] = P.oldepres[c];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int m = 0; m < 10; ++m) P.out[s].f[m][c] = v[10 * s + m];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) P.bout[m][c] = v[30 + m];
+    P.epres[c] = v[33];
+}
+
+// With march set the grid has the B / electron group only, and its
+// boundary threads copy the species' shells as well.
 template <bool P2>
 __global__ void __launch_bounds__(256) k_push(PushParams P) {
     int i, j, k, s;
     if (!cell3(P.B, i, j, k, s)) return;
+    if (P.march) s = 3;
     if (!P.B.interior(i, j, k)) {
-        if (P.copy_shell) push_shell_copy(P, s, P.B.IX(i, j, k));
+        if (P.copy_shell) {
+            const int c = P.B.IX(i, j, k);
+            if (P.march)
+                push_shell_copy_all(P, c);
+            else
+                push_shell_copy(P, s, c);
+        }
         return;
     }
     if (s < 3) push_species<P2>(P, s, i, j, k);
@@ -542,7 +793,7 @@ __global__ void k_validate_apply(ValParams Q) {
     // is using it now) so no separate zeroing launch is needed
     if (blockIdx.x == 0 && threadIdx.x == 0) *Q.count_next = 0;
     for (int e = gtid(); e < n; e += gridDim.x * blockDim.x) {
-        const ValEntry& a = Q.list[e];
+        const ValEntry a = Q.list[e];   // whole entry loaded before the stores
         float* const* f = Q.sp[a.species].f;
 #pragma unroll
         for (int m = 0; m < 10; ++m) f[m][a.cell] = a.v[m];
@@ -577,18 +828,35 @@ __device__ inline void lap_body(const LapParams& L, int g, int i, int j, int k) 
     const Box& B = L.B;
     const int c = B.IX(i, j, k), X = 1, Y = B.nx, Z = B.nx * B.ny;
     const int off[7] = {0, X, -X, Y, -Y, Z, -Z};
+    const float delt = L.dt_dev ? *L.dt_dev : L.delt;
+    auto ldb = [](const float* base, int e) {
+        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) +
+                                               4u * (unsigned)e);
+    };
     float vx[7], vy[7], vz[7];
+    // every operand is loaded before the first store: on this hardware a
+    // wait for a load also waits for every store issued before it
+    float q[10][7];
     if (g < 3) {
         float* const* f = L.src[g].f;
 #pragma unroll
-        for (int m = 0; m < 7; ++m) {
-            const int p = c + off[m];
-            const float a = fmaxf(f[0][p], L.smallbit);
-            vx[m] = f[4][p] / a;
-            vy[m] = f[5][p] / a;
-            vz[m] = f[6][p] / a;
+        for (int m = 0; m < 10; ++m)
+#pragma unroll
+            for (int d = 0; d < 7; ++d) q[m][d] = ldb(f[m], c + off[d]);
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            const float a = fmaxf(q[0][d], L.smallbit);
+            vx[d] = q[4][d] / a;
+            vy[d] = q[5][d] / a;
+            vz[d] = q[6][d] / a;
         }
     } else {
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            q[0][d] = ldb(L.ep_src, c + off[d]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) q[1 + m][d] = ldb(L.b_src[m], c + off[d]);
+        }
 #pragma unroll
         for (int m = 0; m < 7; ++m) {
             const int p = c + off[m];
@@ -604,25 +872,31 @@ __device__ inline void lap_body(const LapParams& L, int g, int i, int j, int k) 
     const float uym = fabsf(vy[0] - vy[4]);
     const float uzp = fabsf(vz[5] - vz[0]);
     const float uzm = fabsf(vz[0] - vz[6]);
-    const float delt = L.dt_dev ? *L.dt_dev : L.delt;
-    auto lap = [&](const float* __restrict__ q, float* w, float chi, int m) {
-        const float q0 = q[c];
-        const float v = q0 + chi * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +
-                                    delt * (uyp * (q[c + Y] - q0) - uym * (q0 - q[c - Y])) +
-                                    delt * (uzp * (q[c + Z] - q0) - uzm * (q0 - q[c - Z])));
-        (void)m;
-        w[c] = v;
+    auto lap = [&](const float* t, float chi) {
+        const float q0 = t[0];
+        return q0 + chi * (delt * (uxp * (t[1] - q0) - uxm * (q0 - t[2])) +
+                           delt * (uyp * (t[3] - q0) - uym * (q0 - t[4])) +
+                           delt * (uzp * (t[5] - q0) - uzm * (q0 - t[6])));
+    };
+    const unsigned ob = 4u * (unsigned)c;
+    auto st = [&](float* base, float v) {
+        *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + ob) = v;
     };
     if (g < 3) {
-        lap(L.src[g].f[0], L.dst[g].f[0], L.chirho, 0);
-        // the reference's field order within lap_plasma does not matter
-        // (each output depends only on its own input)
+        float v[10];
+        v[0] = lap(q[0], L.chirho);
 #pragma unroll
-        for (int m = 1; m < 10; ++m) lap(L.src[g].f[m], L.dst[g].f[m], L.chipxyz, m);
+        for (int m = 1; m < 10; ++m) v[m] = lap(q[m], L.chipxyz);
+#pragma unroll
+        for (int m = 0; m < 10; ++m) st(L.dst[g].f[m], v[m]);
     } else {
-        lap(L.ep_src, L.ep_dst, L.chipxyz, 0);
+        float v[4];
+        v[0] = lap(q[0], L.chipxyz);
 #pragma unroll
-        for (int m = 0; m < 3; ++m) lap(L.b_src[m], L.b_dst[m], L.chierg, 1 + m);
+        for (int m = 0; m < 3; ++m) v[1 + m] = lap(q[1 + m], L.chierg);
+        st(L.ep_dst, v[0]);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) st(L.b_dst[m], v[1 + m]);
     }
 }
 
@@ -1035,8 +1309,9 @@ __global__ void __launch_bounds__(256) k_slab_gather(const int* __restrict__ idx
     if (t >= n) return;
     const int f = blockIdx.y, c = idx[t];
     float* s = slab + (size_t)2 * f * n;
-    s[t] = Fp.old[f][c];
-    s[n + t] = Fp.cur[f][c];
+    const float o = Fp.old[f][c], q = Fp.cur[f][c];
+    s[t] = o;
+    s[n + t] = q;
 }
 
 void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, float* slab,
@@ -1093,10 +1368,20 @@ void launch_efield(const EParams& P, hipStream_t s) {
 
 void launch_push(const PushParams& P, hipStream_t s) {
     ProfScope prof_("k_push", s);
+    if (P.march) {
+        const int ntx = (P.B.nx - 2 + MOX - 1) / MOX, nty = (P.B.ny - 2 + MTY - 1) / MTY;
+        const int nkc = (P.B.nz - 2 + MKC - 1) / MKC, nb = 3 * ntx * nty * nkc;
+        const dim3 g((nb + 7) & ~7), b(MTX, MTY);
+        if (P.pow2)
+            hipLaunchKernelGGL(k_push_march<true>, g, b, 0, s, P, ntx, nty, nkc);
+        else
+            hipLaunchKernelGGL(k_push_march<false>, g, b, 0, s, P, ntx, nty, nkc);
+    }
+    const int groups = P.march ? 1 : 4;
     if (P.pow2)
-        hipLaunchKernelGGL(k_push<true>, grid3(P.B, 4), block3(), 0, s, P);
+        hipLaunchKernelGGL(k_push<true>, grid3(P.B, groups), block3(), 0, s, P);
     else
-        hipLaunchKernelGGL(k_push<false>, grid3(P.B, 4), block3(), 0, s, P);
+        hipLaunchKernelGGL(k_push<false>, grid3(P.B, groups), block3(), 0, s, P);
 }
 
 int validate_blocks(const Box& B) {

@@ -42,6 +42,9 @@ __global__ void k_push_ion(Box B, IonPtrs P, float rmass, float delt, float grav
     const float* __restrict__ qvy = P.qv[1];
     const float* __restrict__ qvz = P.qv[2];
 
+    // outputs are stored after every load: on this hardware a wait for a
+    // load also waits for every store issued before it
+    float res[10];
     auto flux = [&](const float* __restrict__ w) {
         return ((w[c + X] * qvx[c + X] - w[c - X] * qvx[c - X]) / dxt) +
                ((w[c + Y] * qvy[c + Y] - w[c - Y] * qvy[c - Y]) / dyt) +
@@ -49,7 +52,7 @@ __global__ void k_push_ion(Box B, IonPtrs P, float rmass, float delt, float grav
     };
 
     // continuity (:107-110)
-    P.out[0][c] = P.old[0][c] - delt * (((wpx[c + X] - wpx[c - X]) / dxt) +
+    res[0] = P.old[0][c] - delt * (((wpx[c + X] - wpx[c - X]) / dxt) +
                                         ((wpy[c + Y] - wpy[c - Y]) / dyt) +
                                         ((wpz[c + Z] - wpz[c - Z]) / dzt));
     const float aqpx = P.old[4][c] - delt * flux(wpx);
@@ -94,9 +97,9 @@ __global__ void k_push_ion(Box B, IonPtrs P, float rmass, float delt, float grav
     const float radius = sqrtf(ax * ax + ay * ay + az * az) + 0.0000001f;
     const float tr = re_equiv * radius;
     const float g = delt * wrc * grav / (tr * tr);
-    P.out[4][c] = px - g * ax / radius;
-    P.out[5][c] = py - g * ay / radius;
-    P.out[6][c] = pz - g * az / radius;
+    res[4] = px - g * ax / radius;
+    res[5] = py - g * ay / radius;
+    res[6] = pz - g * az / radius;
 
     // diagonal pressures (:210-238)
     const float cx = wx[c], cy = wy[c], cz = wz[c];
@@ -120,38 +123,40 @@ __global__ void k_push_ion(Box B, IonPtrs P, float rmass, float delt, float grav
     if (bmag > bmax) sf = reynolds * (bmax / bmag);
     const float t2 = 2.0f * delt;
 
-    P.out[1][c] = px0 - t2 * cxy * (qvx[c + Y] - qvx[c - Y]) / dyt -
+    res[1] = px0 - t2 * cxy * (qvx[c + Y] - qvx[c - Y]) / dyt -
                   t2 * cxz * (qvx[c + Z] - qvx[c - Z]) / dzt +
                   t2 * sf / rmass * (cxy * abz - cxz * aby);
-    P.out[2][c] = py0 - t2 * cxy * (qvy[c + X] - qvy[c - X]) / dxt -
+    res[2] = py0 - t2 * cxy * (qvy[c + X] - qvy[c - X]) / dxt -
                   t2 * cyz * (qvy[c + Z] - qvy[c - Z]) / dzt +
                   t2 * sf / rmass * (cyz * abx - cxy * abz);
-    P.out[3][c] = pz0 - t2 * cxz * (qvz[c + X] - qvz[c - X]) / dxt -
+    res[3] = pz0 - t2 * cxz * (qvz[c + X] - qvz[c - X]) / dxt -
                   t2 * cyz * (qvz[c + Y] - qvz[c - Y]) / dyt +
                   t2 * sf / rmass * (cxz * aby - cyz * abx);
 
     // off-diagonal pressures (:285-356)
     const float pxy = P.old[7][c] - delt * flux(wxy);
-    P.out[7][c] = pxy - delt * cxy * ((qvx[c + X] - qvx[c - X]) / dxt + (qvy[c + Y] - qvy[c - Y]) / dyt) -
+    res[7] = pxy - delt * cxy * ((qvx[c + X] - qvx[c - X]) / dxt + (qvy[c + Y] - qvy[c - Y]) / dyt) -
                   delt * cy * (qvx[c + Y] - qvx[c - Y]) / dyt -
                   delt * cx * (qvy[c + X] - qvy[c - X]) / dxt -
                   delt * cyz * (qvx[c + Z] - qvx[c - Z]) / dzt -
                   delt * cxz * (qvy[c + Z] - qvy[c - Z]) / dzt +
                   delt * sf / rmass * (abz * (cy - cx) - aby * cyz + abx * cxz);
     const float pxz = P.old[8][c] - delt * flux(wxz);
-    P.out[8][c] = pxz - delt * cxz * ((qvx[c + X] - qvx[c - X]) / dxt + (qvz[c + Z] - qvz[c - Z]) / dzt) -
+    res[8] = pxz - delt * cxz * ((qvx[c + X] - qvx[c - X]) / dxt + (qvz[c + Z] - qvz[c - Z]) / dzt) -
                   delt * cz * (qvx[c + Z] - qvx[c - Z]) / dzt -
                   delt * cx * (qvz[c + X] - qvz[c - X]) / dxt -
                   delt * cyz * (qvx[c + Y] - qvx[c - Y]) / dyt -
                   delt * cxy * (qvz[c + Y] - qvz[c - Y]) / dyt +
                   delt * sf / rmass * (aby * (cx - cz) - abx * cxy + abz * cyz);
     const float pyz = P.old[9][c] - delt * flux(wyz);
-    P.out[9][c] = pyz - delt * cyz * ((qvy[c + Y] - qvy[c - Y]) / dyt + (qvz[c + Z] - qvz[c - Z]) / dzt) -
+    res[9] = pyz - delt * cyz * ((qvy[c + Y] - qvy[c - Y]) / dyt + (qvz[c + Z] - qvz[c - Z]) / dzt) -
                   delt * cy * (qvz[c + Y] - qvz[c - Y]) / dyt -
                   delt * cz * (qvy[c + Z] - qvy[c - Z]) / dzt -
                   delt * cxy * (qvz[c + X] - qvz[c - X]) / dxt -
                   delt * cxz * (qvy[c + X] - qvy[c - X]) / dxt +
                   delt * sf / rmass * (abx * (cz - cy) - abz * cxz + aby * cxy);
+#pragma unroll
+    for (int m = 0; m < 10; ++m) P.out[m][c] = res[m];
 }
 
 void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, float grav,

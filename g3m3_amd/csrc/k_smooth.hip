@@ -60,12 +60,23 @@ __global__ void k_lapidus(Box B, LapPtrs L, int nf, const float* __restrict__ vx
     const float uym = fabsf(vy[c] - vy[c - Y]);
     const float uzp = fabsf(vz[c + Z] - vz[c]);
     const float uzm = fabsf(vz[c] - vz[c - Z]);
-    for (int f = 0; f < nf; ++f) {
-        const float* __restrict__ q = L.src[f];
-        const float q0 = q[c];
-        L.dst[f][c] = q0 + L.chi[f] * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +
-                                       delt * (uyp * (q[c + Y] - q0) - uym * (q0 - q[c - Y])) +
-                                       delt * (uzp * (q[c + Z] - q0) - uzm * (q0 - q[c - Z])));
+    // five fields at a time, stored after their loads: on this hardware a
+    // wait for a load also waits for every store issued before it
+    constexpr int G = 5;
+    for (int f0 = 0; f0 < nf; f0 += G) {
+        float w[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (f0 + u >= nf) break;
+            const float* __restrict__ q = L.src[f0 + u];
+            const float q0 = q[c];
+            w[u] = q0 + L.chi[f0 + u] * (delt * (uxp * (q[c + X] - q0) - uxm * (q0 - q[c - X])) +
+                                         delt * (uyp * (q[c + Y] - q0) - uym * (q0 - q[c - Y])) +
+                                         delt * (uzp * (q[c + Z] - q0) - uzm * (q0 - q[c - Z])));
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (f0 + u < nf) L.dst[f0 + u][c] = w[u];
     }
 }
 

@@ -82,6 +82,8 @@ struct Run {
     // the FCT's launch shape, picked by tune_fct: k-planes per block and
     // block order, with the measured time of each candidate
     int fct_kc = 16, fct_no_remap = 0;
+    // species push: 1 = k_push_march, 0 = one thread per cell (k_push)
+    int push_march = 1;
     float fct_us[6] = {};
 
     Set slice(float* const* arrays, int box) const {
@@ -217,6 +219,7 @@ void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float
     P.pow2 = p2 ? 1 : 0;
     P.do_elec = do_elec;
     P.copy_shell = copy_shell;
+    P.march = R->push_march;
     P.dt_dev = R->capturing ? R->dt_dev : nullptr;
     P.dt_scale = dt_scale;
     launch_push(P, R->s);
