@@ -82,8 +82,10 @@ struct Run {
     // the FCT's launch shape, picked by tune_fct: k-planes per block and
     // block order, with the measured time of each candidate
     int fct_kc = 16, fct_no_remap = 0;
-    // species push: 1 = k_push_march, 0 = one thread per cell (k_push)
+    // species push, picked by tune_push: 1 = k_push_march, 0 = one thread
+    // per cell (k_push), with the measured time of each
     int push_march = 1;
+    float push_us[2] = {};
     float fct_us[6] = {};
 
     Set slice(float* const* arrays, int box) const {
@@ -421,6 +423,40 @@ void fct(Run* R, int box) {
 // Every candidate gives the same bits.
 constexpr int kFctDepths[3] = {8, 16, 32};
 
+// Times one step-2 push of box 1 (every output into scratch) with each
+// species kernel and keeps the faster.  Both give the same bits.
+void tune_push(Run* R) {
+    float* out = nullptr;
+    if (hipMalloc(&out, sizeof(float) * (size_t)NF * R->n) != hipSuccess) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    Set o;
+    for (int f = 0; f < NF; ++f) o.f[f] = out + (size_t)f * R->n;
+    const Set ol = R->old(1), wk = R->wrk(1);
+    float best = 1e30f;
+    const int keep = R->push_march;
+    for (int m = 0; ok && m < 2; ++m) {
+        R->push_march = m;
+        push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);   // warm-up
+        ok = hipEventRecord(e0, R->s) == hipSuccess;
+        for (int q = 0; q < 3; ++q) push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);
+        float ms = 0.0f;
+        ok = ok && hipEventRecord(e1, R->s) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (!ok) break;
+        R->push_us[m] = ms * 1000.0f / 3.0f;
+        if (R->push_us[m] < best) {
+            best = R->push_us[m];
+            R->push_march = m;
+        }
+    }
+    if (!ok) R->push_march = keep;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+}
+
 void tune_fct(Run* R) {
     float* out = nullptr;
     if (hipMalloc(&out, sizeof(float) * (size_t)NF * R->n) != hipSuccess) return;
@@ -729,6 +765,7 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
     R->t_stepnew.assign(ng, 0.0);
     R->rot.assign(ng, 0);
     tune_fct(R);
+    tune_push(R);
     return R;
 }
 
@@ -912,6 +949,27 @@ int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap) {
     R->fct_kc = kc_len;
     R->fct_no_remap = no_remap ? 1 : 0;
     for (int q = 0; q < 2; ++q) {   // captured steps hold the old shape
+        if (R->gexec[q]) (void)hipGraphExecDestroy(R->gexec[q]);
+        if (R->graph[q]) (void)hipGraphDestroy(R->graph[q]);
+        R->gexec[q] = nullptr;
+        R->graph[q] = nullptr;
+    }
+    return 0;
+}
+
+int g3m3_run_push_kernel(void* run, int* march, float* cand_us) {
+    Run* R = (Run*)run;
+    if (march) *march = R->push_march;
+    if (cand_us)
+        for (int q = 0; q < 2; ++q) cand_us[q] = R->push_us[q];
+    return 0;
+}
+
+int g3m3_run_set_push_kernel(void* run, int march) {
+    Run* R = (Run*)run;
+    if (march != 0 && march != 1) return 1;
+    R->push_march = march;
+    for (int q = 0; q < 2; ++q) {   // captured steps hold the old kernel
         if (R->gexec[q]) (void)hipGraphExecDestroy(R->gexec[q]);
         if (R->graph[q]) (void)hipGraphDestroy(R->graph[q]);
         R->gexec[q] = nullptr;

@@ -163,3 +163,28 @@ def test_fct_launch_shapes_equal():
             else:
                 bad = [k for k in FIELDS if not np.array_equal(h[k], ref[k])]
                 assert not bad, f"shape {kc}/{order} differs on {bad}"
+
+
+def test_push_kernels_equal():
+    """The two species push kernels the autotuner chooses between (k-marching
+    LDS tiles, one thread per cell) give the same bits over whole steps of a
+    nested run with the resistivity extension; the tuner's pick is one of
+    them."""
+    from g3m3_amd.driver import FastRun
+    ref = None
+    for kind in ("cell", "march"):
+        r = FastRun(make_workload("small", ngrd=3, eta=1e-3))
+        k0, cands = r.push_kernel()
+        assert k0 in ("cell", "march") and set(cands) == {"cell", "march"}
+        r.set_push_kernel(kind)
+        r.init_dt()
+        print(f"push kernel {kind}", flush=True)
+        for _ in range(3):
+            r.full_step()
+        h = {p: r.state_host(p) for p in ("", "old", "wrk")}
+        if ref is None:
+            ref = h
+        else:
+            for p in h:
+                bad = [k for k in FIELDS if not np.array_equal(h[p][k], ref[p][k])]
+                assert not bad, f"{p or 'state'} differs on {bad}"
