@@ -435,6 +435,7 @@ void tune_push(Run* R) {
     const Set ol = R->old(1), wk = R->wrk(1);
     float best = 1e30f;
     const int keep = R->push_march;
+    int pick = keep;
     for (int m = 0; ok && m < 2; ++m) {
         R->push_march = m;
         push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);   // warm-up
@@ -448,10 +449,10 @@ void tune_push(Run* R) {
         R->push_us[m] = ms * 1000.0f / 3.0f;
         if (R->push_us[m] < best) {
             best = R->push_us[m];
-            R->push_march = m;
+            pick = m;
         }
     }
-    if (!ok) R->push_march = keep;
+    R->push_march = ok ? pick : keep;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(out);
