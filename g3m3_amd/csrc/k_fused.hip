@@ -463,6 +463,13 @@ __global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx,
     const int i = tx * MOX + l, j0 = 1 + ty * MTY, j = j0 + w;
     const int ic = min(i, nx - 1), jc = min(j, ny - 1);
     const bool out = l >= 1 && l <= MOX && i <= nx - 2 && j <= ny - 2;
+    // step 2: the lanes on the box's x faces copy the species' old values
+    // there (loaded with the centre operands); k_push copies the rest of
+    // the shell.  Each such cell has one owner: lane 0 for i = 0, an output
+    // lane -- or lane 63 of the last tile -- for i = nx-1.
+    const bool xshell = P.copy_shell && j <= ny - 2 &&
+                        ((i == 0 && l == 0) ||
+                         (i == nx - 1 && l >= 1 && (l <= MOX || tx == ntx - 1)));
     // the y-halo row this wave loads (waves 0 and 1), and its LDS row
     const bool halo = w < 2;
     const int hj = w == 0 ? j0 - 1 : min(j0 + MTY, ny - 1), hr = w == 0 ? 0 : MRY - 1;
@@ -516,6 +523,11 @@ __global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx,
         if (out) {
             const PushMarchAcc a{L, cur, nxt, r, l, prev, rho_cur};
             push_species_core<P2>(P, s, i, j, k, own0 + Z * k, a, cc);
+        } else if (xshell) {
+            const unsigned ob = 4u * (unsigned)(own0 + Z * k);
+#pragma unroll
+            for (int m = 0; m < 10; ++m)
+                *reinterpret_cast<float*>(reinterpret_cast<char*>(P.out[s].f[m]) + ob) = cc.o[m];
         }
 #pragma unroll
         for (int x = 0; x < MNA; ++x) prev[x] = L.a[cur][x][r][l];
@@ -612,7 +624,8 @@ __device__ inline void push_shell_copy_all(const PushParams& P, int c) {
 }
 
 // With march set the grid has the B / electron group only, and its
-// boundary threads copy the species' shells as well.
+// boundary threads copy the species' shells as well -- except on the x
+// faces between the j and k faces, which k_push_march has copied.
 template <bool P2>
 __global__ void __launch_bounds__(256) k_push(PushParams P) {
     int i, j, k, s;
@@ -621,7 +634,8 @@ __global__ void __launch_bounds__(256) k_push(PushParams P) {
     if (!P.B.interior(i, j, k)) {
         if (P.copy_shell) {
             const int c = P.B.IX(i, j, k);
-            if (P.march)
+            const bool xface = j > 0 && j < P.B.ny - 1 && k > 0 && k < P.B.nz - 1;
+            if (P.march && !xface)
                 push_shell_copy_all(P, c);
             else
                 push_shell_copy(P, s, c);
