@@ -330,8 +330,8 @@ def main(argv=None):
         kc, order, cands = run.fct_shape()
         out["fct_autotune"] = {"k_planes_per_block": kc, "block_order": order,
                                "candidates_us": cands}
-        kind, pc = run.push_kernel()
-        out["push_autotune"] = {"kernel": kind, "candidates_us": pc}
+        kind, pc = run.species_kernels()
+        out["species_kernel_autotune"] = {"kernels": kind, "candidates_us": pc}
     if prec is not None:
         out["divb_precision"] = prec
     if batch is not None:

@@ -90,6 +90,7 @@ struct LapParams {
     float* b_dst[3];
     float chirho, chipxyz, chierg, delt, smallbit;
     const float* dt_dev;     // non-null: delt = *dt_dev (captured steps)
+    int march;               // species by k_lap_march; k_lap does epres and B
 };
 
 struct FctParams {

@@ -914,10 +914,134 @@ __device__ inline void lap_body(const LapParams& L, int g, int i, int j, int k) 
     }
 }
 
+// Lapidus of the three species marching along k, on the push's tiling: a
+// block owns a 62 x 4 (i, j) tile of one species and a run of MKC planes;
+// LDS holds, per plane, the 10 species arrays and the 3 velocity
+// components p / max(rho, smallbit) on the tile with its x and y halo, in
+// a 2-slot ring (planes k and k+1); -z comes from registers.  Each velocity
+// is formed once, by the thread that loads the point.  Same expression as
+// lap_body, so the same bits.
+constexpr int LNA = 13;
+struct LapMarchLds {
+    float a[2][LNA][MRY][MTX];
+};
+
+__device__ inline void lap_march_load(const LapParams& L, int s, unsigned ob, float* v) {
+#pragma unroll
+    for (int m = 0; m < 10; ++m) v[m] = ldb(L.src[s].f[m], ob);
+}
+
+__device__ inline void lap_march_store(LapMarchLds& S, int slot, int row, int l, const float* v,
+                                       float smallbit) {
+#pragma unroll
+    for (int m = 0; m < 10; ++m) S.a[slot][m][row][l] = v[m];
+    const float a = fmaxf(v[0], smallbit);
+    S.a[slot][10][row][l] = v[4] / a;
+    S.a[slot][11][row][l] = v[5] / a;
+    S.a[slot][12][row][l] = v[6] / a;
+}
+
+__global__ void __launch_bounds__(MTX * MTY) k_lap_march(LapParams L, int ntx, int nty, int nkc) {
+    __shared__ LapMarchLds S;
+    const Box B = L.B;
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    int t = xcd_block(blockIdx.x, gridDim.x);
+    const int tx = t % ntx;
+    t /= ntx;
+    const int ty = t % nty;
+    t /= nty;
+    const int kc = t % nkc;
+    const int s = t / nkc;
+    if (s >= 3) return;                      // grid padding: whole blocks
+    const int l = threadIdx.x, w = threadIdx.y;
+    const int i = tx * MOX + l, j0 = 1 + ty * MTY, j = j0 + w;
+    const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+    const bool out = l >= 1 && l <= MOX && i <= nx - 2 && j <= ny - 2;
+    const bool halo = w < 2;
+    const int hj = w == 0 ? j0 - 1 : min(j0 + MTY, ny - 1), hr = w == 0 ? 0 : MRY - 1;
+    const int r = w + 1;
+    const int kb = 1 + kc * MKC, ke = min(kb + MKC, nz - 1);
+    const int Z = nx * ny;
+    const int own0 = ic + nx * jc, halo0 = ic + nx * hj;
+    const float delt = L.dt_dev ? *L.dt_dev : L.delt, sb = L.smallbit;
+    float prev[LNA];
+    {
+        float v[10];
+        lap_march_load(L, s, 4u * (own0 + Z * (kb - 1)), v);
+#pragma unroll
+        for (int m = 0; m < 10; ++m) prev[m] = v[m];
+        const float a = fmaxf(v[0], sb);
+        prev[10] = v[4] / a;
+        prev[11] = v[5] / a;
+        prev[12] = v[6] / a;
+        lap_march_load(L, s, 4u * (own0 + Z * kb), v);
+        lap_march_store(S, 0, r, l, v, sb);
+        if (halo) {
+            lap_march_load(L, s, 4u * (halo0 + Z * kb), v);
+            lap_march_store(S, 0, hr, l, v, sb);
+        }
+        lap_march_load(L, s, 4u * (own0 + Z * (kb + 1)), v);
+        lap_march_store(S, 1, r, l, v, sb);
+        if (halo) {
+            lap_march_load(L, s, 4u * (halo0 + Z * (kb + 1)), v);
+            lap_march_store(S, 1, hr, l, v, sb);
+        }
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = kb; k < ke; ++k) {
+        const int nxt = cur ^ 1;
+        const int k2 = min(k + 2, nz - 1);
+        float vo[10], vh[10];
+        lap_march_load(L, s, 4u * (own0 + Z * k2), vo);
+        if (halo) lap_march_load(L, s, 4u * (halo0 + Z * k2), vh);
+        if (out) {
+            // operand x at direction d: 0 the cell, 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z
+            auto at = [&](int x, int d) {
+                switch (d) {
+                    case 0: return S.a[cur][x][r][l];
+                    case 1: return S.a[cur][x][r][l + 1];
+                    case 2: return S.a[cur][x][r][l - 1];
+                    case 3: return S.a[cur][x][r + 1][l];
+                    case 4: return S.a[cur][x][r - 1][l];
+                    case 5: return S.a[nxt][x][r][l];
+                    default: return prev[x];
+                }
+            };
+            const float uxp = fabsf(at(10, 1) - at(10, 0));
+            const float uxm = fabsf(at(10, 0) - at(10, 2));
+            const float uyp = fabsf(at(11, 3) - at(11, 0));
+            const float uym = fabsf(at(11, 0) - at(11, 4));
+            const float uzp = fabsf(at(12, 5) - at(12, 0));
+            const float uzm = fabsf(at(12, 0) - at(12, 6));
+            float res[10];
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+                const float chi = m == 0 ? L.chirho : L.chipxyz;
+                const float q0 = at(m, 0);
+                res[m] = q0 + chi * (delt * (uxp * (at(m, 1) - q0) - uxm * (q0 - at(m, 2))) +
+                                     delt * (uyp * (at(m, 3) - q0) - uym * (q0 - at(m, 4))) +
+                                     delt * (uzp * (at(m, 5) - q0) - uzm * (q0 - at(m, 6))));
+            }
+            const unsigned ob = 4u * (unsigned)(own0 + Z * k);
+#pragma unroll
+            for (int m = 0; m < 10; ++m)
+                *reinterpret_cast<float*>(reinterpret_cast<char*>(L.dst[s].f[m]) + ob) = res[m];
+        }
+#pragma unroll
+        for (int x = 0; x < LNA; ++x) prev[x] = S.a[cur][x][r][l];
+        __syncthreads();                     // plane k's slot is free
+        lap_march_store(S, cur, r, l, vo, sb);
+        if (halo) lap_march_store(S, cur, hr, l, vh, sb);
+        cur = nxt;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_lap(LapParams L) {
     int i, j, k, g;
     const Box& B = L.B;
     if (!cell3(B, i, j, k, g)) return;
+    if (L.march) g = 3;
     if (!B.interior(i, j, k)) return;
     lap_body(L, g, i, j, k);
 }
@@ -1417,7 +1541,13 @@ void launch_validate(const ValParams& Q, float* maxima, hipStream_t s) {
 
 void launch_lap(const LapParams& L, hipStream_t s) {
     ProfScope prof_("k_lap", s);
-    hipLaunchKernelGGL(k_lap, grid3(L.B, 4), block3(), 0, s, L);
+    if (L.march) {
+        const int ntx = (L.B.nx - 2 + MOX - 1) / MOX, nty = (L.B.ny - 2 + MTY - 1) / MTY;
+        const int nkc = (L.B.nz - 2 + MKC - 1) / MKC, nb = 3 * ntx * nty * nkc;
+        hipLaunchKernelGGL(k_lap_march, dim3((nb + 7) & ~7), dim3(MTX, MTY), 0, s, L, ntx, nty,
+                           nkc);
+    }
+    hipLaunchKernelGGL(k_lap, grid3(L.B, L.march ? 1 : 4), block3(), 0, s, L);
 }
 
 

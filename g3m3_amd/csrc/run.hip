@@ -376,11 +376,12 @@ void bounds(Run* R, const Set& set, const Set& flank_tgt, int box, double t_grid
     if (box <= R->d.mbndry) inner_bc(R, set);
 }
 
-void lapidus(Run* R, int box, float delt) {
+// into wrk, or into *dst (the autotuner's scratch)
+void lapidus(Run* R, int box, float delt, const Set* dst = nullptr) {
     const g3m3_run_desc& d = R->d;
     LapParams L;
     L.B = R->B;
-    const Set st = R->state(box), wk = R->wrk(box);
+    const Set st = R->state(box), wk = dst ? *dst : R->wrk(box);
     for (int s = 0; s < 3; ++s) {
         L.src[s] = species(st, s);
         L.dst[s] = species(wk, s);
@@ -397,6 +398,7 @@ void lapidus(Run* R, int box, float delt) {
     L.delt = delt;
     L.smallbit = d.smallbit;
     L.dt_dev = R->capturing ? R->dt_dev : nullptr;
+    L.march = R->push_march;
     launch_lap(L, R->s);
 }
 
@@ -423,8 +425,9 @@ void fct(Run* R, int box) {
 // Every candidate gives the same bits.
 constexpr int kFctDepths[3] = {8, 16, 32};
 
-// Times one step-2 push of box 1 (every output into scratch) with each
-// species kernel and keeps the faster.  Both give the same bits.
+// Times one step-2 push and one Lapidus pass of box 1 (every output into
+// scratch) with each family of species kernels and keeps the faster.  Both
+// give the same bits.
 void tune_push(Run* R) {
     float* out = nullptr;
     if (hipMalloc(&out, sizeof(float) * (size_t)NF * R->n) != hipSuccess) return;
@@ -439,8 +442,12 @@ void tune_push(Run* R) {
     for (int m = 0; ok && m < 2; ++m) {
         R->push_march = m;
         push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);   // warm-up
+        lapidus(R, 1, 1.0e-3f, &o);
         ok = hipEventRecord(e0, R->s) == hipSuccess;
-        for (int q = 0; q < 3; ++q) push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);
+        for (int q = 0; q < 3; ++q) {
+            push(R, o, ol, wk, 1, 1.0e-3f, 1.0f, 1, 1);
+            lapidus(R, 1, 1.0e-3f, &o);
+        }
         float ms = 0.0f;
         ok = ok && hipEventRecord(e1, R->s) == hipSuccess &&
              hipEventSynchronize(e1) == hipSuccess &&
@@ -958,7 +965,7 @@ int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap) {
     return 0;
 }
 
-int g3m3_run_push_kernel(void* run, int* march, float* cand_us) {
+int g3m3_run_species_kernels(void* run, int* march, float* cand_us) {
     Run* R = (Run*)run;
     if (march) *march = R->push_march;
     if (cand_us)
@@ -966,7 +973,7 @@ int g3m3_run_push_kernel(void* run, int* march, float* cand_us) {
     return 0;
 }
 
-int g3m3_run_set_push_kernel(void* run, int march) {
+int g3m3_run_set_species_kernels(void* run, int march) {
     Run* R = (Run*)run;
     if (march != 0 && march != 1) return 1;
     R->push_march = march;

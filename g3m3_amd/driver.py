@@ -438,19 +438,20 @@ class FastRun(Run):
                                           int(order == "plain")):
             raise _lib.G3M3Error(f"bad FCT shape {kc_len}")
 
-    def push_kernel(self):
-        """The species push kernel chosen by the run's autotuner: ('march'
-        or 'cell', {candidate: microseconds per push})."""
+    def species_kernels(self):
+        """The species push and Lapidus kernels chosen by the run's
+        autotuner: ('march' or 'cell', {candidate: microseconds of one push
+        plus one Lapidus pass})."""
         m = _ct.c_int(0)
         us = (_ct.c_float * 2)()
-        self._L.g3m3_run_push_kernel(_ct.c_void_p(self.h), _ct.byref(m), us)
+        self._L.g3m3_run_species_kernels(_ct.c_void_p(self.h), _ct.byref(m), us)
         return ("march" if m.value else "cell"), {"cell": round(float(us[0]), 2),
                                                    "march": round(float(us[1]), 2)}
 
-    def set_push_kernel(self, kind: str):
-        if kind not in ("march", "cell") or self._L.g3m3_run_set_push_kernel(
+    def set_species_kernels(self, kind: str):
+        if kind not in ("march", "cell") or self._L.g3m3_run_set_species_kernels(
                 _ct.c_void_p(self.h), int(kind == "march")):
-            raise _lib.G3M3Error(f"bad push kernel {kind!r}")
+            raise _lib.G3M3Error(f"bad species kernel family {kind!r}")
 
     def graph_finish(self):
         if self._L.g3m3_run_graph_finish(_ct.c_void_p(self.h)):

@@ -610,13 +610,14 @@ int g3m3_run_graph_finish(void* run);
 int g3m3_run_fct_shape(void* run, int* kc_len, int* no_remap, float* cand_us);
 /* Overrides that choice (kc_len >= 1); returns nonzero on a bad value. */
 int g3m3_run_set_fct_shape(void* run, int kc_len, int no_remap);
-/* The species push kernel the autotuner chose at g3m3_run_create (1: the
- * k-marching, LDS-tiled kernel; 0: one thread per cell), with the measured
- * microseconds of one push by each (index = the value above).  Both give
- * the same results (code/sub_push_ion.f90). */
-int g3m3_run_push_kernel(void* run, int* march, float* cand_us);
+/* The species push and Lapidus kernels the autotuner chose at
+ * g3m3_run_create (1: the k-marching, LDS-tiled kernels; 0: one thread per
+ * cell), with the measured microseconds of one push plus one Lapidus pass
+ * by each (index = the value above).  Both give the same results
+ * (code/sub_push_ion.f90, code/sub_lap_plasma.f90). */
+int g3m3_run_species_kernels(void* run, int* march, float* cand_us);
 /* Overrides that choice (0 or 1); returns nonzero on a bad value. */
-int g3m3_run_set_push_kernel(void* run, int march);
+int g3m3_run_set_species_kernels(void* run, int march);
 int g3m3_run_box_step(void* run, int box);
 void g3m3_run_set_time(void* run, double t);          /* after a restart read */
 int g3m3_run_sync_arrays(void* run);

@@ -165,21 +165,21 @@ def test_fct_launch_shapes_equal():
                 assert not bad, f"shape {kc}/{order} differs on {bad}"
 
 
-def test_push_kernels_equal():
-    """The two species push kernels the autotuner chooses between (k-marching
-    LDS tiles, one thread per cell) give the same bits over whole steps of a
-    nested run with the resistivity extension; the tuner's pick is one of
-    them."""
+def test_species_kernels_equal():
+    """The two families of species push and Lapidus kernels the autotuner
+    chooses between (k-marching LDS tiles, one thread per cell) give the same
+    bits over whole steps of a nested run with the resistivity extension;
+    the tuner keeps the faster."""
     from g3m3_amd.driver import FastRun
     ref = None
     for kind in ("cell", "march"):
         r = FastRun(make_workload("small", ngrd=3, eta=1e-3))
-        k0, cands = r.push_kernel()
+        k0, cands = r.species_kernels()
         assert k0 in ("cell", "march") and set(cands) == {"cell", "march"}
         assert cands[k0] == min(cands.values()), (k0, cands)   # the faster one
-        r.set_push_kernel(kind)
+        r.set_species_kernels(kind)
         r.init_dt()
-        print(f"push kernel {kind}", flush=True)
+        print(f"species kernels {kind}", flush=True)
         for _ in range(3):
             r.full_step()
         h = {p: r.state_host(p) for p in ("", "old", "wrk")}
