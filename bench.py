@@ -74,20 +74,30 @@ def units_per_launch(kind, wl, nfields=34):
     return nx * ny * nz * nfields
 
 
+# The kernels one call of an instrumented stage launches (the library times
+# the stage as one unit): the k-marching species kernels plus the per-cell
+# kernel for B and the electrons.
+STAGE_KERNELS = {"k_push": ("k_push_march", "k_push"), "k_lap": ("k_lap_march", "k_lap")}
+
+
 def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE counters, calibrated on a plain copy (the commands and the
-    calibration are in profiles/traffic_native_<workload>.md)."""
+    """HBM bytes per call of stage `kernel` (summed over the kernels it
+    launches) measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters,
+    calibrated on a plain copy (the commands and the calibration are in
+    profiles/traffic_native_<workload>.md)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"traffic_native_{workload}.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
         data = json.load(fh)
-    for name, rec in data.get("kernels", {}).items():
-        if name == kernel or name.startswith(kernel + "<"):
-            return rec["traffic_bytes"], f"profiles/traffic_native_{workload}.md"
-    return None, None
+    total, found = 0, False
+    for base in STAGE_KERNELS.get(kernel, (kernel,)):
+        for name, rec in data.get("kernels", {}).items():
+            if name == base or name.startswith(base + "<"):
+                total += rec["traffic_bytes"]
+                found = True
+    return (total, f"profiles/traffic_native_{workload}.md") if found else (None, None)
 
 
 def batch_case(n: int = 32, steps: int = 20, streams: int = 8):
@@ -235,6 +245,10 @@ def main(argv=None):
                     help="also run the batching case: 32 independent single-grid runs "
                          "(41x41x21 .. 121x121x61), 20 full steps each, one at a time and "
                          "then together on 8 streams")
+    ap.add_argument("--species-kernels", choices=("auto", "march", "cell"), default="auto",
+                    help="native driver: the species push / Lapidus kernels (auto: the run's "
+                         "autotuner picks; counter runs fix the family the timed runs pick, "
+                         "since counter collection changes the timings the tuner sees)")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -250,6 +264,8 @@ def main(argv=None):
 
     wl = make_workload(a.workload)
     run = (FastRun if a.driver == "native" else Run)(wl)
+    if a.driver == "native" and a.species_kernels != "auto":
+        run.set_species_kernels(a.species_kernels)
     run.init_dt()
     for _ in range(a.warmup):
         run.full_step()
@@ -284,7 +300,9 @@ def main(argv=None):
         per_launch = per_unit * units_per_launch(kind, wl)
         avg_s = ms.value / 1e3 / nl.value
         achieved = per_launch / avg_s / 1e9
-        roof = {"kernel": a.prof_kernel, "bound": "hbm", "achieved": round(achieved, 1),
+        roof = {"kernel": a.prof_kernel,
+                "kernels": list(STAGE_KERNELS.get(a.prof_kernel, (a.prof_kernel,))),
+                "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "traffic_source": tsrc, "bytes_per_launch": per_launch,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl.value}
