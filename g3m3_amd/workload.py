@@ -97,6 +97,16 @@ def make_workload(name: str = "nested121", seed: int = 20261203, **over) -> Work
     L, ngrd = int(p["L"]), int(p["ngrd"])
     nx, ny, nz = 2 * L + 1, 2 * L + 1, L + 1
     grid = _grid(L, ngrd)
+    if p.get("dims") is not None:
+        # a single box of any extents, centred on the planet, cell size `spacing`
+        if ngrd != 1:
+            raise ValueError("dims= needs a single grid")
+        nx, ny, nz = (int(d) for d in p["dims"])
+        h = F(p.get("spacing", 1.0))
+        grid = {"xspac": np.array([h], F)}
+        for ax, n in (("x", nx), ("y", ny), ("z", nz)):
+            grid[f"grd_{ax}min"] = np.array([-h * F((n - 1) / 2.0)], F)
+            grid[f"grd_{ax}max"] = np.array([h * F((n - 1) / 2.0)], F)
     _check_mating(grid, ngrd)
     rng = np.random.default_rng(seed)
 
