@@ -28,6 +28,10 @@ struct EParams {
     float* tv[3];
     float* bs[3];
     float rmq, rmh, rmo, reynolds, dxt, dyt, dzt, eta;
+    // reciprocals of the above, used when pow2: each of them is a power of
+    // two, so x * (1/d) is x / d exactly
+    float rrmq, rrmh, rrmo, rreynolds, rdxt, rdyt, rdzt;
+    int pow2;
 };
 
 struct PushParams {

@@ -41,23 +41,37 @@ inline dim3 grid3(const Box& B, int groups) {
 }
 inline dim3 block3() { return dim3(256, 1, 1); }
 
+// Division by a grid step, a species mass or the Reynolds number.  When
+// every such divisor is a power of two (P2, checked on the host), x / d is
+// computed as x * (1/d): the reciprocal is exact and both operations are
+// correctly rounded, so the result is the same float; otherwise a true
+// division.
+template <bool P2>
+__device__ inline float dvp(float x, float d, float r) {
+    return P2 ? x * r : x / d;
+}
+
 // ---------------------------------------------------------------------------
 // E field.  cur at a boundary cell is calcur's face/edge/corner rule; those
 // rules resolve to the curl at the clamped index, except the four corners
 // at i = nx that average three cells (code/sub_calcur.f90:268-289).  E on
 // the boundary is BandE's flank copy: E at the clamped index.
 
+template <bool P2>
 struct CurlCtx {
     Box B;
     const float* __restrict__ bx;
     const float* __restrict__ by;
     const float* __restrict__ bz;
-    float dxt, dyt, dzt;
+    float dxt, dyt, dzt, rdxt, rdyt, rdzt;
     __device__ void curl(int i, int j, int k, float& cx, float& cy, float& cz) const {
         const int c = B.IX(i, j, k), sx = 1, sy = B.nx, sz = B.nx * B.ny;
-        cx = (bz[c + sy] - bz[c - sy]) / dyt - (by[c + sz] - by[c - sz]) / dzt;
-        cy = (bx[c + sz] - bx[c - sz]) / dzt - (bz[c + sx] - bz[c - sx]) / dxt;
-        cz = (by[c + sx] - by[c - sx]) / dxt - (bx[c + sy] - bx[c - sy]) / dyt;
+        auto Dx = [&](float x) { return dvp<P2>(x, dxt, rdxt); };
+        auto Dy = [&](float x) { return dvp<P2>(x, dyt, rdyt); };
+        auto Dz = [&](float x) { return dvp<P2>(x, dzt, rdzt); };
+        cx = Dy(bz[c + sy] - bz[c - sy]) - Dz(by[c + sz] - by[c - sz]);
+        cy = Dz(bx[c + sz] - bx[c - sz]) - Dx(bz[c + sx] - bz[c - sx]);
+        cz = Dx(by[c + sx] - by[c - sx]) - Dy(bx[c + sy] - bx[c - sy]);
     }
     __device__ void cur(int i, int j, int k, float& cx, float& cy, float& cz) const {
         const int ci = clampi(i, 1, B.nx - 2), cj = clampi(j, 1, B.ny - 2),
@@ -72,25 +86,30 @@ struct CurlCtx {
 };
 
 // E at an interior cell (code/sub_BandE.f90:37-116 + uniform eta)
-__device__ inline void efield_interior(const EParams& P, const CurlCtx& C, int i, int j, int k,
+template <bool P2>
+__device__ inline void efield_interior(const EParams& P, const CurlCtx<P2>& C, int i, int j, int k,
                                        float& Ex, float& Ey, float& Ez, float* evo, float* tvo,
                                        float* bso) {
     const Box& B = P.B;
     const int c = B.IX(i, j, k);
+    auto Mq = [&](float x) { return dvp<P2>(x, P.rmq, P.rrmq); };
+    auto Mh = [&](float x) { return dvp<P2>(x, P.rmh, P.rrmh); };
+    auto Mo = [&](float x) { return dvp<P2>(x, P.rmo, P.rrmo); };
+    auto Re = [&](float x) { return dvp<P2>(x, P.reynolds, P.rreynolds); };
     float cx, cy, cz;
     C.curl(i, j, k, cx, cy, cz);
     const float q = P.rho[0][c], h = P.rho[1][c], o = P.rho[2][c];
-    const float qden = (q + 0.000001f) / P.rmq;
-    const float hden = (h + 0.000001f) / P.rmh;
-    const float oden = (o + 0.000001f) / P.rmo;
+    const float qden = Mq(q + 0.000001f);
+    const float hden = Mh(h + 0.000001f);
+    const float oden = Mo(o + 0.000001f);
     const float tot = qden + hden + oden;
     const float tden = q + h + o;
     const float tx = (P.mom[0][c] + P.mom[3][c] + P.mom[6][c]) / tden;
     const float ty = (P.mom[1][c] + P.mom[4][c] + P.mom[7][c]) / tden;
     const float tz = (P.mom[2][c] + P.mom[5][c] + P.mom[8][c]) / tden;
-    const float evx = tx - cx / tot / P.reynolds;
-    const float evy = ty - cy / tot / P.reynolds;
-    const float evz = tz - cz / tot / P.reynolds;
+    const float evx = tx - Re(cx / tot);
+    const float evy = ty - Re(cy / tot);
+    const float evz = tz - Re(cz / tot);
     const float abx = P.b0[0][c] + P.b[0][c], aby = P.b0[1][c] + P.b[1][c],
                 abz = P.b0[2][c] + P.b[2][c];
     Ex = -(evy * abz - evz * aby);
@@ -98,21 +117,21 @@ __device__ inline void efield_interior(const EParams& P, const CurlCtx& C, int i
     Ez = -(evx * aby - evy * abx);
     const int sy = B.nx, sz = B.nx * B.ny;
     const float* ep = P.ep;
-    const float arho = (q / P.rmq + h / P.rmh + o / P.rmo) * P.reynolds;
-    Ex = Ex - ((ep[c + 1] - ep[c - 1]) / P.dxt) / arho;
-    Ey = Ey - ((ep[c + sy] - ep[c - sy]) / P.dyt) / arho;
-    Ez = Ez - ((ep[c + sz] - ep[c - sz]) / P.dzt) / arho;
+    const float arho = (Mq(q) + Mh(h) + Mo(o)) * P.reynolds;
+    Ex = Ex - dvp<P2>(ep[c + 1] - ep[c - 1], P.dxt, P.rdxt) / arho;
+    Ey = Ey - dvp<P2>(ep[c + sy] - ep[c - sy], P.dyt, P.rdyt) / arho;
+    Ez = Ez - dvp<P2>(ep[c + sz] - ep[c - sz], P.dzt, P.rdzt) / arho;
     if (P.rst) {   // list points of box mbndry: rst is zero elsewhere
         const float r = P.rst[c];
         if (r != 0.0f) {
-            const float eden = q / P.rmq + h / P.rmh + o / P.rmo;
+            const float eden = Mq(q) + Mh(h) + Mo(o);
             Ex = Ex + r * cx / eden;
             Ey = Ey + r * cy / eden;
             Ez = Ez + r * cz / eden;
         }
     }
     if (P.eta != 0.0f) {
-        const float eden = q / P.rmq + h / P.rmh + o / P.rmo;
+        const float eden = Mq(q) + Mh(h) + Mo(o);
         Ex = Ex + P.eta * cx / eden;
         Ey = Ey + P.eta * cy / eden;
         Ez = Ez + P.eta * cz / eden;
@@ -130,12 +149,13 @@ __device__ inline void efield_interior(const EParams& P, const CurlCtx& C, int i
     }
 }
 
+template <bool P2>
 __global__ void __launch_bounds__(256) k_efield(EParams P) {
     int i, j, k, g;
     const Box& B = P.B;
     if (!cell3(B, i, j, k, g)) return;
     const int c = B.IX(i, j, k);
-    CurlCtx C{B, P.b[0], P.b[1], P.b[2], P.dxt, P.dyt, P.dzt};
+    const CurlCtx<P2> C{B, P.b[0], P.b[1], P.b[2], P.dxt, P.dyt, P.dzt, P.rdxt, P.rdyt, P.rdzt};
     float Ex, Ey, Ez;
     if (B.interior(i, j, k)) {
         float ev[3], tv[3], bs[3];
@@ -157,16 +177,18 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
         float cx, cy, cz;
         C.cur(i, j, k, cx, cy, cz);
         const float q = P.rho[0][c], h = P.rho[1][c], o = P.rho[2][c];
-        const float tot = (q + 0.000001f) / P.rmq + (h + 0.000001f) / P.rmh + (o + 0.000001f) / P.rmo;
+        const float tot = dvp<P2>(q + 0.000001f, P.rmq, P.rrmq) +
+                          dvp<P2>(h + 0.000001f, P.rmh, P.rrmh) +
+                          dvp<P2>(o + 0.000001f, P.rmo, P.rrmo);
         const float tden = q + h + o;
         const float tx = (P.mom[0][c] + P.mom[3][c] + P.mom[6][c]) / tden;
         const float ty = (P.mom[1][c] + P.mom[4][c] + P.mom[7][c]) / tden;
         const float tz = (P.mom[2][c] + P.mom[5][c] + P.mom[8][c]) / tden;
         const float bsx = P.b0[0][c] + P.b[0][c], bsy = P.b0[1][c] + P.b[1][c],
                     bsz = P.b0[2][c] + P.b[2][c];
-        P.ev[0][c] = tx - cx / tot / P.reynolds;
-        P.ev[1][c] = ty - cy / tot / P.reynolds;
-        P.ev[2][c] = tz - cz / tot / P.reynolds;
+        P.ev[0][c] = tx - dvp<P2>(cx / tot, P.reynolds, P.rreynolds);
+        P.ev[1][c] = ty - dvp<P2>(cy / tot, P.reynolds, P.rreynolds);
+        P.ev[2][c] = tz - dvp<P2>(cz / tot, P.reynolds, P.rreynolds);
         P.tv[0][c] = tx;
         P.tv[1][c] = ty;
         P.tv[2][c] = tz;
@@ -184,19 +206,10 @@ __global__ void __launch_bounds__(256) k_efield(EParams P) {
 // when P.do_elec).  The species velocity wrk p / max(wrk rho, 1e-6) is
 // evaluated where it is needed instead of stored (same bits).
 
-// Division by a grid step or a species mass.  When every such divisor is a
-// power of two (P2), x / d is computed as x * (1/d): the reciprocal is
-// exact and both operations are correctly rounded, so the result is the
-// same float; otherwise a true division.
 // the push's time step: from device memory in captured steps (the same
 // float the host would pass: 0.5 dt or dt)
 __device__ inline float step_delt(const PushParams& P) {
     return P.dt_dev ? P.dt_scale * *P.dt_dev : P.delt;
-}
-
-template <bool P2>
-__device__ inline float dvp(float x, float d, float r) {
-    return P2 ? x * r : x / d;
 }
 
 // Stencil directions of the push: 0 the cell, 1 +x, 2 -x, 3 +y, 4 -y,
@@ -1501,7 +1514,10 @@ void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const Flank
 // ---------------------------------------------------------------------------
 // host launchers
 void launch_efield(const EParams& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_efield, grid3(P.B, 1), block3(), 0, s, P);
+    if (P.pow2)
+        hipLaunchKernelGGL(k_efield<true>, grid3(P.B, 1), block3(), 0, s, P);
+    else
+        hipLaunchKernelGGL(k_efield<false>, grid3(P.B, 1), block3(), 0, s, P);
 }
 
 void launch_push(const PushParams& P, hipStream_t s) {

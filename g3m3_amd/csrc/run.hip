@@ -158,6 +158,14 @@ void efield(Run* R, const Set& src, int box) {
     P.dyt = 2.0f * rx;
     P.dzt = 2.0f * rx;
     P.eta = d.eta;
+    const float ds[7] = {P.rmq, P.rmh, P.rmo, P.reynolds, P.dxt, P.dyt, P.dzt};
+    float* rs[7] = {&P.rrmq, &P.rrmh, &P.rrmo, &P.rreynolds, &P.rdxt, &P.rdyt, &P.rdzt};
+    bool p2 = true;
+    for (int q = 0; q < 7; ++q) {
+        *rs[q] = 1.0f / ds[q];
+        p2 = p2 && is_pow2(ds[q]);
+    }
+    P.pow2 = p2 ? 1 : 0;
     launch_efield(P, R->s);
 }
 
