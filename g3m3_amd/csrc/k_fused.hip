@@ -814,22 +814,9 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
     }
 }
 
-__global__ void k_validate_apply(ValParams Q) {
-    const int n = *Q.count;
-    // the next pass appends to the other counter; clear it here (no pass
-    // is using it now) so no separate zeroing launch is needed
-    if (blockIdx.x == 0 && threadIdx.x == 0) *Q.count_next = 0;
-    for (int e = gtid(); e < n; e += gridDim.x * blockDim.x) {
-        const ValEntry a = Q.list[e];   // whole entry loaded before the stores
-        float* const* f = Q.sp[a.species].f;
-#pragma unroll
-        for (int m = 0; m < 10; ++m) f[m][a.cell] = a.v[m];
-    }
-}
-
-// final maxima of one validation pass -> maxima[0..4]; zero the list counter
-__global__ void k_validate_reduce(const float* __restrict__ partials, int nblocks, float* maxima,
-                                  int* count) {
+// The pass's CFL maxima from the per-block partials (one block)
+__device__ inline void validate_reduce_block(const float* __restrict__ partials, int nblocks,
+                                             float* maxima) {
     __shared__ float red[5][4];
     float m[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
@@ -845,6 +832,26 @@ __global__ void k_validate_reduce(const float* __restrict__ partials, int nblock
     if (threadIdx.x < 5) {
         const int q = threadIdx.x;
         maxima[q] = fmaxf(fmaxf(red[q][0], red[q][1]), fmaxf(red[q][2], red[q][3]));
+    }
+}
+
+// Blocks 0 .. kApplyBlocks-1 apply the pass's side-list; with maxima set,
+// one more block forms the pass's CFL maxima (independent work, one launch)
+constexpr int kApplyBlocks = 128;
+__global__ void __launch_bounds__(256) k_validate_apply(ValParams Q, float* maxima, int nblocks) {
+    if (blockIdx.x == kApplyBlocks) {
+        validate_reduce_block(Q.partials, nblocks, maxima);
+        return;
+    }
+    const int n = *Q.count;
+    // the next pass appends to the other counter; clear it here (no pass
+    // is using it now) so no separate zeroing launch is needed
+    if (blockIdx.x == 0 && threadIdx.x == 0) *Q.count_next = 0;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += kApplyBlocks * 256) {
+        const ValEntry a = Q.list[e];   // whole entry loaded before the stores
+        float* const* f = Q.sp[a.species].f;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) f[m][a.cell] = a.v[m];
     }
 }
 
@@ -1549,10 +1556,9 @@ void launch_validate(const ValParams& Q, float* maxima, hipStream_t s) {
         ProfScope prof_("k_validate", s);
         hipLaunchKernelGGL(k_validate, grid3(Q.B, 1), block3(), 0, s, Q);
     }
-    hipLaunchKernelGGL(k_validate_apply, dim3(128), dim3(256), 0, s, Q);
-    if (Q.partials && maxima)
-        hipLaunchKernelGGL(k_validate_reduce, dim3(1), dim3(256), 0, s, Q.partials, nb, maxima,
-                           Q.count);
+    const bool red = Q.partials && maxima;
+    hipLaunchKernelGGL(k_validate_apply, dim3(kApplyBlocks + (red ? 1 : 0)), dim3(256), 0, s, Q,
+                       maxima, nb);
 }
 
 void launch_lap(const LapParams& L, hipStream_t s) {
