@@ -108,6 +108,28 @@ def test_native_equals_per_subroutine_three_boxes():
     assert a.box_steps == b.box_steps
 
 
+def test_native_equals_per_subroutine_81x81x41_three_boxes():
+    """The forward run of the differentiation case's geometry (three nested
+    81x81x41 boxes, resistivity 1e-3): native runtime and per-subroutine
+    path bit-equal in every state, old and wrk array after each of 2 full
+    steps (14 box-steps)."""
+    from g3m3_amd.driver import FastRun, Run
+    wl = make_workload("nested121", L=40, ngrd=3, eta=1e-3)
+    assert (wl.nx, wl.ny, wl.nz, wl.ngrd) == (81, 81, 41, 3)
+    a, b = FastRun(wl), Run(wl)
+    a.init_dt()
+    b.init_dt()
+    for st in range(1, 3):
+        print(f"81x81x41 x3, step {st}", flush=True)
+        b.full_step()
+        a.full_step(dt_override=b.t_step.copy())
+        for pre in ("", "old", "wrk"):
+            ha, hb = a.state_host(pre), b.state_host(pre)
+            bad = [k for k in FIELDS if not np.array_equal(ha[k], hb[k])]
+            assert not bad, f"step {st} {pre or 'state'}: {bad}"
+    assert a.box_steps == b.box_steps
+
+
 def test_native_single61_100_steps():
     """The 61x61x31 single-grid case over 100 steps, every field checked
     after 1, 10 and 100 steps (Lapidus and FCT every step)."""
