@@ -695,18 +695,33 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
     if (cell3(B, i, j, k, g)) {
         const int c = B.IX(i, j, k);
         const bool inner = B.interior(i, j, k);
+        // every centre operand first: a list append (a store) inside one
+        // species' branch would otherwise hold back the next species' loads
+        float cv[3][7];
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int m = 0; m < 7; ++m) cv[s][m] = Q.sp[s].f[m][c];
+        const float e0 = Q.ep[c];
+        float bt[6] = {0, 0, 0, 0, 0, 0};
+        if (Q.partials && inner)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                bt[q] = Q.b0[q][c];
+                bt[3 + q] = Q.b[q][c];
+            }
         float rho_final[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             float* const* f = Q.sp[s].f;
-            const RhoFix cf = set_rho_cell(f[0][c], f[1][c], f[2][c], f[3][c], dmins[s]);
+            const RhoFix cf = set_rho_cell(cv[s][0], cv[s][1], cv[s][2], cv[s][3], dmins[s]);
             rho_final[s] = cf.r;
             bool avg = false;
             if (inner) {
                 const float arho = cf.r + Q.smallbit;
-                const float avx = fabsf(f[4][c] / arho);
-                const float avy = fabsf(f[5][c] / arho);
-                const float avz = fabsf(f[6][c] / arho);
+                const float avx = fabsf(cv[s][4] / arho);
+                const float avy = fabsf(cv[s][5] / arho);
+                const float avz = fabsf(cv[s][6] / arho);
                 const float v = sqrtf(avx * avx + avy * avy + avz * avz);
                 const float cs = sqrtf(0.33f * (cf.x + cf.y + cf.z) / arho);
                 mx = fmaxf(mx, avx);
@@ -768,24 +783,22 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
                 e.v[1] = cf.x;
                 e.v[2] = cf.y;
                 e.v[3] = cf.z;
-                e.v[4] = f[4][c];
-                e.v[5] = f[5][c];
-                e.v[6] = f[6][c];
+                e.v[4] = cv[s][4];
+                e.v[5] = cv[s][5];
+                e.v[6] = cv[s][6];
                 e.v[7] = e.v[8] = e.v[9] = 0.0f;
                 Q.list[atomicAdd(Q.count, 1)] = e;
             }
         }
         // electron pressure: |pe| everywhere, sound-speed cap and Alfven
         // maximum on the interior with the final densities (:404-426)
-        const float e0 = Q.ep[c];
         float e1 = fabsf(e0);
         if (inner) {
             const float arho = rho_final[0] + rho_final[1] + rho_final[2] + Q.smallbit;
             const float cs = sqrtf(e1 / arho);
             if (cs > Q.cslim) e1 = e1 * Q.cslim / cs;
             if (Q.partials) {
-                const float sx = Q.b0[0][c] + Q.b[0][c], sy = Q.b0[1][c] + Q.b[1][c],
-                            sz = Q.b0[2][c] + Q.b[2][c];
+                const float sx = bt[0] + bt[3], sy = bt[1] + bt[4], sz = bt[2] + bt[5];
                 const float bt = fmaxf(sqrtf(sx * sx + sy * sy + sz * sz), Q.smallbit);
                 ma = bt / sqrtf(arho);
             }
