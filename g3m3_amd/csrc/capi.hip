@@ -84,7 +84,7 @@ struct Buf {
         return p;
     }
 };
-Buf g_speed_list, g_small, g_divb_f, g_divb_d, g_tmp;
+Buf g_speed_list, g_speed_part, g_small, g_divb_f, g_divb_d, g_tmp;
 
 inline Box box_of(const int* nx, const int* ny, const int* nz) { return Box{*nx, *ny, *nz}; }
 inline size_t off(const Box& B, const int* box) { return (size_t)(*box - 1) * B.V(); }
@@ -471,11 +471,13 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
     const float* b0[3] = {S(bx0, B, box), S(by0, B, box), S(bz0, B, box)};
     const float* b[3] = {S(bx, B, box), S(by, B, box), S(bz, B, box)};
     void* list = g_speed_list.get(speed_list_bytes(B.V()));
+    float* part = (float*)g_speed_part.get(speed_partials_bytes(B.V()));
     char* small = (char*)g_small.get(256);
-    if (!list || !small) return;
+    if (!list || !part || !small) return;
     float* maxima = (float*)small;
     int* count = (int*)(small + 64);
-    launch_set_speed(B, SP, Q, b0, b, R(btot), S(epres, B, box), maxima, list, count, g_stream);
+    launch_set_speed(B, SP, Q, b0, b, R(btot), S(epres, B, box), maxima, list, count, part,
+                     g_stream);
     check_launch("set_speed_agrd");
     float h[6];
     if (!hip_ok(hipMemcpyAsync(h, maxima, sizeof(h), hipMemcpyDeviceToHost, g_stream),

@@ -69,10 +69,5 @@ struct FieldPtrs {
 
 __device__ inline float sgn1(float x) { return copysignf(1.0f, x); }
 
-// Non-negative float max via integer compare (maxima here are >= 0).
-__device__ inline void atomic_max_nonneg(float* addr, float v) {
-    if (!(v >= 0.0f)) return;
-    atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
-}
 
 }  // namespace g3m3

@@ -145,10 +145,12 @@ void launch_push_ion(const Box& B, const IonPtrs& P, float rmass, float delt, fl
                      float re_equiv, float reynolds, const float gmin[3], const float gmax[3],
                      hipStream_t s);
 void launch_set_rho(int n, const SpeciesPtrs& S, float* ep, hipStream_t s);
+// partials: speed_partials_bytes(V) of scratch for the per-block maxima
 void launch_set_speed(const Box& B, const SpeciesPtrs& S, const SpeedParams& Q, const float* b0[3],
                       const float* b[3], float* btot, float* ep, float* maxima, void* list,
-                      int* count, hipStream_t s);
+                      int* count, float* partials, hipStream_t s);
 size_t speed_list_bytes(int ncells);
+size_t speed_partials_bytes(int ncells);
 void launch_velocity(int n, const float* r, const float* p[3], float* v[3], float dmin,
                      hipStream_t s);
 void launch_vtot(int n, const float* mom[9], const float* rho[3], float* v[3], hipStream_t s);

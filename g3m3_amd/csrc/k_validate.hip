@@ -47,23 +47,42 @@ void launch_set_rho(int n, const SpeciesPtrs& S, float* ep, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Block-wide max of non-negative values into maxima[slot].
-__device__ inline void block_max_to(float v, float* dst) {
-    __shared__ float red[256];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomic_max_nonneg(dst, red[0]);
-    __syncthreads();
+// Maxima of non-negative values.  Each block writes its own (no
+// same-address atomics: thousands of blocks on one L2 line serialise), and
+// one block reduces them.  Blocks are 256 threads (4 waves).
+__device__ inline float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
-__global__ void k_zero_floats(float* p, int n) {
-    const int c = gtid();
-    if (c < n) p[c] = 0.0f;
+// dst[q] = the block's maximum of v[q]
+template <int N>
+__device__ inline void block_max_store(const float (&v)[N], float* dst) {
+    __shared__ float red[N][4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const float r = wave_maxf(v[q]);
+        if (l == 0) red[q][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        const int q = threadIdx.x;
+        dst[q] = fmaxf(fmaxf(red[q][0], red[q][1]), fmaxf(red[q][2], red[q][3]));
+    }
 }
+
+// maxima[q] = max over blocks of partials[5 b + q], q < 5 (one block)
+__global__ void __launch_bounds__(256) k_speed_reduce(const float* __restrict__ partials, int nb,
+                                                      float* maxima) {
+    float m[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += 256)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) m[q] = fmaxf(m[q], partials[5 * b + q]);
+    block_max_store<5>(m, maxima);
+}
+
 __global__ void k_zero_ints(int* p, int n) {
     const int c = gtid();
     if (c < n) p[c] = 0;
@@ -78,23 +97,30 @@ struct AvgEntry {
 
 // code/sub_set_speed_agrd.f90:88-184 (and the h, o copies :188-381)
 // maxima layout: [0]=pxmax [1]=pymax [2]=pzmax [3]=csmax [4]=alfmax
-__global__ void k_speed_flag(Box B, SpeciesPtrs S, SpeedParams Q, AvgEntry* list, int* count,
-                             float* maxima) {
+__global__ void __launch_bounds__(256) k_speed_flag(Box B, SpeciesPtrs S, SpeedParams Q,
+                                                    AvgEntry* list, int* count, float* partials) {
     int i, j, k;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, mc = 0.0f;
     if (cell_of(B, gtid(), i, j, k) && B.interior(i, j, k)) {
         const int c = B.IX(i, j, k);
         const int nb[6] = {c + 1, c - 1, c + B.nx, c - B.nx, c + B.nx * B.ny, c - B.nx * B.ny};
         const float one_m = 1.0f - Q.aniso;
-#pragma unroll 1
+        // every centre operand first: a list append (a store) inside one
+        // species' branch would otherwise hold back the next species' loads
+        float cv[3][7];
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int m = 0; m < 7; ++m) cv[s][m] = S.f[s][m][c];
+#pragma unroll
         for (int s = 0; s < 3; ++s) {
             float* const* f = S.f[s];
-            const float arho = f[0][c] + Q.smallbit;
-            const float avx = fabsf(f[4][c] / arho);
-            const float avy = fabsf(f[5][c] / arho);
-            const float avz = fabsf(f[6][c] / arho);
+            const float arho = cv[s][0] + Q.smallbit;
+            const float avx = fabsf(cv[s][4] / arho);
+            const float avy = fabsf(cv[s][5] / arho);
+            const float avz = fabsf(cv[s][6] / arho);
             const float v = sqrtf(avx * avx + avy * avy + avz * avz);
-            const float cs = sqrtf(0.33f * (f[1][c] + f[2][c] + f[3][c]) / arho);
+            const float cs = sqrtf(0.33f * (cv[s][1] + cv[s][2] + cv[s][3]) / arho);
             const float vl = Q.vlim * Q.svlim[s];
             if (v > vl || cs > Q.vlim * Q.svlim[s] * Q.sclim[s]) {
                 AvgEntry e;
@@ -146,10 +172,8 @@ __global__ void k_speed_flag(Box B, SpeciesPtrs S, SpeedParams Q, AvgEntry* list
             mc = fmaxf(mc, cs);
         }
     }
-    block_max_to(mx, maxima + 0);
-    block_max_to(my, maxima + 1);
-    block_max_to(mz, maxima + 2);
-    block_max_to(mc, maxima + 3);
+    const float m4[4] = {mx, my, mz, mc};
+    block_max_store<4>(m4, partials + 5 * blockIdx.x);
 }
 
 __global__ void k_speed_apply(SpeciesPtrs S, const AvgEntry* list, const int* count) {
@@ -169,7 +193,7 @@ __global__ void k_speed_elec(Box B, const float* __restrict__ bx0, const float* 
                              const float* __restrict__ by, const float* __restrict__ bz,
                              float* btot, const float* __restrict__ q, const float* __restrict__ h,
                              const float* __restrict__ o, float* ep, float smallbit, float cslim,
-                             float* maxima) {
+                             float* partials) {
     int i, j, k;
     float ma = 0.0f;
     if (cell_of(B, gtid(), i, j, k)) {
@@ -185,26 +209,28 @@ __global__ void k_speed_elec(Box B, const float* __restrict__ bx0, const float* 
             ma = bt / sqrtf(arho);
         }
     }
-    block_max_to(ma, maxima + 4);
+    const float m1[1] = {ma};
+    block_max_store<1>(m1, partials + 5 * blockIdx.x + 4);
 }
 
 void launch_set_speed(const Box& B, const SpeciesPtrs& S, const SpeedParams& Q, const float* b0[3],
                       const float* b[3], float* btot, float* ep, float* maxima, void* list,
-                      int* count, hipStream_t s) {
-    const int n = B.V();
-    hipLaunchKernelGGL(k_zero_floats, dim3(1), dim3(64), 0, s, maxima, 6);
+                      int* count, float* partials, hipStream_t s) {
+    const int n = B.V(), nb = (n + 255) / 256;
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, count, 1);
     {
         ProfScope prof_("k_speed_flag", s);
-        hipLaunchKernelGGL(k_speed_flag, dim3((n + 255) / 256), dim3(256), 0, s, B, S, Q,
-                       (AvgEntry*)list, count, maxima);
+        hipLaunchKernelGGL(k_speed_flag, dim3(nb), dim3(256), 0, s, B, S, Q, (AvgEntry*)list, count,
+                           partials);
     }
     hipLaunchKernelGGL(k_speed_apply, dim3(256), dim3(256), 0, s, S, (const AvgEntry*)list, count);
-    hipLaunchKernelGGL(k_speed_elec, dim3((n + 255) / 256), dim3(256), 0, s, B, b0[0], b0[1], b0[2],
-                       b[0], b[1], b[2], btot, S.f[0][0], S.f[1][0], S.f[2][0], ep, Q.smallbit,
-                       Q.cslim, maxima);
+    hipLaunchKernelGGL(k_speed_elec, dim3(nb), dim3(256), 0, s, B, b0[0], b0[1], b0[2], b[0], b[1],
+                       b[2], btot, S.f[0][0], S.f[1][0], S.f[2][0], ep, Q.smallbit, Q.cslim,
+                       partials);
+    hipLaunchKernelGGL(k_speed_reduce, dim3(1), dim3(256), 0, s, partials, nb, maxima);
 }
 
 size_t speed_list_bytes(int ncells) { return (size_t)3 * ncells * sizeof(AvgEntry); }
+size_t speed_partials_bytes(int ncells) { return (size_t)5 * ((ncells + 255) / 256) * sizeof(float); }
 
 }  // namespace g3m3
