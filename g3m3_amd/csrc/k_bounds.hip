@@ -188,10 +188,49 @@ void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], cons
 // coarse corners of each shell cell instead of over the whole coarse box;
 // the panels' blends are applied in the reference's order (bottom, top,
 // front, back, left, right), so edge cells see the same sequence.
+// Shell cell t of a box with every extent >= 5 -> (i, j, k): the four z
+// layers (k = 0, 1, nz-2, nz-1; all i, j), then per plane k = 2..nz-3 the
+// four y rows (j = 0, 1, ny-2, ny-1; all i), then per (k, j) with
+// j = 2..ny-3 the four x cells (i = 0, 1, nx-2, nx-1).  False past the end.
+__device__ inline bool shell2_cell(const Box& B, int t, int& i, int& j, int& k) {
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const int zl = nx * ny, yl = 4 * nx, xl = 4 * (ny - 4);
+    auto layer = [](int s, int n) { return s < 2 ? s : n - 4 + s; };
+    if (t < 4 * zl) {
+        const int s = t / zl, r = t - s * zl;
+        k = layer(s, nz);
+        j = r / nx;
+        i = r - j * nx;
+        return true;
+    }
+    t -= 4 * zl;
+    if (t < (nz - 4) * yl) {
+        k = 2 + t / yl;
+        const int r = t - (k - 2) * yl, s = r / nx;
+        i = r - s * nx;
+        j = layer(s, ny);
+        return true;
+    }
+    t -= (nz - 4) * yl;
+    if (t >= (nz - 4) * xl) return false;
+    k = 2 + t / xl;
+    const int r = t - (k - 2) * xl;
+    j = 2 + r / 4;
+    i = layer(r & 3, nx);
+    return true;
+}
+
+int shell2_count(const Box& B) {
+    return 4 * B.nx * B.ny + (B.nz - 4) * 4 * B.nx + (B.nz - 4) * 4 * (B.ny - 4);
+}
+
+// shell: threads enumerate the 2-layer shell (every extent >= 5) instead of
+// the whole box
+template <bool shell>
 __global__ void k_flanks(Box B, FlankPtrs Fp, FlankGeom G, float t1, float t2) {
     int i, j, k;
     const int f = blockIdx.y;
-    if (!cell_of(B, gtid(), i, j, k)) return;
+    if (shell ? !shell2_cell(B, gtid(), i, j, k) : !cell_of(B, gtid(), i, j, k)) return;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     const bool bot = k <= 1, top = k >= nz - 2, fro = j <= 1, bac = j >= ny - 2, lef = i <= 1,
                rig = i >= nx - 2;
@@ -242,7 +281,12 @@ void launch_flanks(const Box& B, const FlankPtrs& Fp, int nf, const FlankGeom& G
     const int n = B.V();
     {
         ProfScope prof_("k_flanks", s);
-        hipLaunchKernelGGL(k_flanks, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp, G, t1, t2);
+        if (B.nx >= 5 && B.ny >= 5 && B.nz >= 5)
+            hipLaunchKernelGGL(k_flanks<true>, dim3((shell2_count(B) + 255) / 256, nf), dim3(256),
+                               0, s, B, Fp, G, t1, t2);
+        else
+            hipLaunchKernelGGL(k_flanks<false>, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, Fp,
+                               G, t1, t2);
     }
 }
 
