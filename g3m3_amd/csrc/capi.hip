@@ -455,7 +455,7 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
     for (int s = 0; s < 3; ++s)
         for (int f = 0; f < 10; ++f) SP.f[s][f] = S(g[s][f], B, box);
     if (*isotropic) g_iso_flag = 0.0f;
-    SpeedParams Q;
+    SpeedParams Q{};
     Q.aniso = *aniso_limit;
     Q.smallbit = *smallbit;
     Q.vlim = *vlim;

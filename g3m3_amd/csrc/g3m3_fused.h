@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <cmath>
+
 #include <vector>
 
 #include "g3m3_dev.h"
@@ -58,6 +60,10 @@ struct PushParams {
     int do_elec;
     int copy_shell;             // copy non-interior cells old -> out
     int march;                  // species by k_push_march; k_push does B, electrons, shells
+    int nspecies;               // species k_push_march pushes (src/old/out[0 .. nspecies-1])
+    int march_kc;               // k-planes per k_push_march block (0: 16)
+    const float* btot;          // non-null: |B| of the species push read from here (the
+                                // reference's btot) instead of formed from bs
     // time step from device memory (captured steps): delt = dt_scale *
     // *dt_dev, and bmax is formed from it; null: delt and bmax above
     const float* dt_dev;
@@ -140,6 +146,14 @@ void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const Flank
 
 void launch_efield(const EParams& P, hipStream_t s);
 void launch_push(const PushParams& P, hipStream_t s);
+// the species of P (nspecies of them) by k_push_march alone
+void launch_push_species(const PushParams& P, hipStream_t s);
+
+// x is a finite nonzero power of two (then 1/x is exact)
+inline bool is_pow2(float x) {
+    int e = 0;
+    return x != 0.0f && std::isfinite(x) && std::fabs(std::frexp(x, &e)) == 0.5f;
+}
 void launch_validate(const ValParams& Q, float* maxima, hipStream_t s);
 int validate_blocks(const Box& B);
 void launch_lap(const LapParams& L, hipStream_t s);

@@ -222,7 +222,7 @@ __device__ inline float step_delt(const PushParams& P) {
 // the v x B term and E.  Loaded before anything is stored: on this hardware
 // a wait for a load also waits for every store issued before it.
 struct PushCentre {
-    float o[10], tv[3], e[3];
+    float o[10], tv[3], e[3], bt;
 };
 
 __device__ inline void push_centre_load(const PushParams& P, int s, int c, PushCentre& C) {
@@ -236,6 +236,7 @@ __device__ inline void push_centre_load(const PushParams& P, int s, int c, PushC
     for (int q = 0; q < 3; ++q) C.tv[q] = ld(P.tv[q]);
 #pragma unroll
     for (int q = 0; q < 3; ++q) C.e[q] = ld(P.e[q]);
+    C.bt = P.btot ? ld(P.btot) : 0.0f;
 }
 
 template <bool P2, class A>
@@ -270,7 +271,8 @@ __device__ inline void push_species_core(const PushParams& P, int s, int i, int 
     const float aqpy = o[5] - delt * flux(PY);
     const float aqpz = o[6] - delt * flux(PZ);
     float abx = a.b(0, 0), aby = a.b(1, 0), abz = a.b(2, 0);
-    const float bmag0 = fmaxf(sqrtf(abx * abx + aby * aby + abz * abz), P.smallbit);
+    const float bmag0 =
+        P.btot ? C.bt : fmaxf(sqrtf(abx * abx + aby * aby + abz * abz), P.smallbit);
     const float weight = fmaxf(1.0f, bmag0 / bmax);
     const float skin_depth = reynolds / weight;
     const float tcx = (ty * abz - tz * aby);
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx,
     t /= nty;
     const int kc = t % nkc;
     const int s = t / nkc;
-    if (s >= 3) return;                      // grid padding: whole blocks
+    if (s >= P.nspecies) return;             // grid padding: whole blocks
     const int l = threadIdx.x, w = threadIdx.y;
     const int i = tx * MOX + l, j0 = 1 + ty * MTY, j = j0 + w;
     const int ic = min(i, nx - 1), jc = min(j, ny - 1);
@@ -487,7 +489,8 @@ __global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx,
     const bool halo = w < 2;
     const int hj = w == 0 ? j0 - 1 : min(j0 + MTY, ny - 1), hr = w == 0 ? 0 : MRY - 1;
     const int r = w + 1;
-    const int kb = 1 + kc * MKC, ke = min(kb + MKC, nz - 1);
+    const int kcl = P.march_kc > 0 ? P.march_kc : MKC;
+    const int kb = 1 + kc * kcl, ke = min(kb + kcl, nz - 1);
     const int Z = nx * ny;
     const int own0 = ic + nx * jc, halo0 = ic + nx * hj;
     float prev[MNA];
@@ -1540,17 +1543,22 @@ void launch_efield(const EParams& P, hipStream_t s) {
         hipLaunchKernelGGL(k_efield<false>, grid3(P.B, 1), block3(), 0, s, P);
 }
 
+static void launch_march(const PushParams& P, hipStream_t s) {
+    const int kcl = P.march_kc > 0 ? P.march_kc : MKC;
+    const int ntx = (P.B.nx - 2 + MOX - 1) / MOX, nty = (P.B.ny - 2 + MTY - 1) / MTY;
+    const int nkc = (P.B.nz - 2 + kcl - 1) / kcl, nb = P.nspecies * ntx * nty * nkc;
+    const dim3 g((nb + 7) & ~7), b(MTX, MTY);
+    if (P.pow2)
+        hipLaunchKernelGGL(k_push_march<true>, g, b, 0, s, P, ntx, nty, nkc);
+    else
+        hipLaunchKernelGGL(k_push_march<false>, g, b, 0, s, P, ntx, nty, nkc);
+}
+
+void launch_push_species(const PushParams& P, hipStream_t s) { launch_march(P, s); }
+
 void launch_push(const PushParams& P, hipStream_t s) {
     ProfScope prof_("k_push", s);
-    if (P.march) {
-        const int ntx = (P.B.nx - 2 + MOX - 1) / MOX, nty = (P.B.ny - 2 + MTY - 1) / MTY;
-        const int nkc = (P.B.nz - 2 + MKC - 1) / MKC, nb = 3 * ntx * nty * nkc;
-        const dim3 g((nb + 7) & ~7), b(MTX, MTY);
-        if (P.pow2)
-            hipLaunchKernelGGL(k_push_march<true>, g, b, 0, s, P, ntx, nty, nkc);
-        else
-            hipLaunchKernelGGL(k_push_march<false>, g, b, 0, s, P, ntx, nty, nkc);
-    }
+    if (P.march) launch_march(P, s);
     const int groups = P.march ? 1 : 4;
     if (P.pow2)
         hipLaunchKernelGGL(k_push<true>, grid3(P.B, groups), block3(), 0, s, P);

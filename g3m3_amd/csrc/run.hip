@@ -121,10 +121,6 @@ bool in_box(const Box& B, int i, int j, int k) {
 }
 
 // x is +-2^n (so 1/x is exact and x / y == x * (1/y) bit for bit)
-bool is_pow2(float x) {
-    int e = 0;
-    return x != 0.0f && std::isfinite(x) && std::fabs(std::frexp(x, &e)) == 0.5f;
-}
 
 SpeciesSet species(const Set& S, int s) {
     SpeciesSet Q;
@@ -135,7 +131,7 @@ SpeciesSet species(const Set& S, int s) {
 // -- stage pieces ---------------------------------------------------------------
 void efield(Run* R, const Set& src, int box) {
     const g3m3_run_desc& d = R->d;
-    EParams P;
+    EParams P{};
     P.B = R->B;
     for (int c = 0; c < 3; ++c) {
         P.b[c] = src.f[BX + c];
@@ -172,7 +168,7 @@ void efield(Run* R, const Set& src, int box) {
 void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float delt,
           float dt_scale, int do_elec, int copy_shell) {
     const g3m3_run_desc& d = R->d;
-    PushParams P;
+    PushParams P{};
     P.B = R->B;
     for (int s = 0; s < 3; ++s) {
         P.src[s] = species(src, s);
@@ -230,6 +226,8 @@ void push(Run* R, const Set& out, const Set& old, const Set& src, int box, float
     P.do_elec = do_elec;
     P.copy_shell = copy_shell;
     P.march = R->push_march;
+    P.nspecies = 3;
+    P.btot = nullptr;
     P.dt_dev = R->capturing ? R->dt_dev : nullptr;
     P.dt_scale = dt_scale;
     launch_push(P, R->s);
@@ -341,7 +339,7 @@ void inner_bc(Run* R, const Set& set) {
 
 void validate(Run* R, const Set& set, int box, bool want_max) {
     const g3m3_run_desc& d = R->d;
-    ValParams Q;
+    ValParams Q{};
     Q.B = R->B;
     for (int s = 0; s < 3; ++s) Q.sp[s] = species(set, s);
     Q.ep = set.f[EPRES];
@@ -387,7 +385,7 @@ void bounds(Run* R, const Set& set, const Set& flank_tgt, int box, double t_grid
 // into wrk, or into *dst (the autotuner's scratch)
 void lapidus(Run* R, int box, float delt, const Set* dst = nullptr) {
     const g3m3_run_desc& d = R->d;
-    LapParams L;
+    LapParams L{};
     L.B = R->B;
     const Set st = R->state(box), wk = dst ? *dst : R->wrk(box);
     for (int s = 0; s < 3; ++s) {
@@ -412,7 +410,7 @@ void lapidus(Run* R, int box, float delt, const Set* dst = nullptr) {
 
 void fct(Run* R, int box) {
     const g3m3_run_desc& d = R->d;
-    FctParams F;
+    FctParams F{};
     F.B = R->B;
     const Set st = R->state(box), ol = R->old(box), wk = R->wrk(box);
     for (int f = 0; f < NF; ++f) {
@@ -479,7 +477,7 @@ void tune_fct(Run* R) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     const g3m3_run_desc& d = R->d;
-    FctParams F;
+    FctParams F{};
     F.B = R->B;
     const Set ol = R->old(1), wk = R->wrk(1);
     for (int f = 0; f < NF; ++f) {
