@@ -1,0 +1,23 @@
+"""bench.py's command line on the CPU: more than one GPU is refused with a
+message (this build measures one GPU), before anything touches a device."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_help_lists_the_legs():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    for flag in ("--steps", "--warmup", "--gpus", "--batch", "--precision", "--driver",
+                 "--species-kernels"):
+        assert flag in r.stdout
