@@ -313,7 +313,8 @@ def push_ion(qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz,
     aby = (F(6) * bsy[C] + bsy[XP] + bsy[XM] + bsy[YP] + bsy[YM] + bsy[ZP] + bsy[ZM]) / F(12)
     abz = (F(6) * bsz[C] + bsz[XP] + bsz[XM] + bsz[YP] + bsz[YM] + bsz[ZP] + bsz[ZM]) / F(12)
     bmag = np.sqrt(abx * abx + aby * aby + abz * abz)
-    sf = np.where(bmag > bmax, reynolds * (bmax / bmag), reynolds).astype(F)
+    with np.errstate(divide="ignore"):   # the quotient is used only where bmag > bmax
+        sf = np.where(bmag > bmax, reynolds * (bmax / bmag), reynolds).astype(F)
 
     cxy, cxz, cyz = wxy[C], wxz[C], wyz[C]
     cx, cy, cz = wx[C], wy[C], wz[C]
