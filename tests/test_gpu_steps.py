@@ -130,6 +130,29 @@ def test_native_equals_per_subroutine_81x81x41_three_boxes():
     assert a.box_steps == b.box_steps
 
 
+def test_native_equals_per_subroutine_nested121_50_steps():
+    """The benchmark configuration itself -- five nested 121x121x61 boxes
+    with resistivity, 50 full steps -- through the native runtime (fused
+    kernels, autotuned launch shapes, buffer rotation, gathered flank slabs)
+    and through the per-subroutine path: every state field bit-equal after
+    steps 1, 10 and 50, and old and wrk after step 50."""
+    from g3m3_amd.driver import FastRun, Run
+    wl = make_workload("nested121")
+    a, b = FastRun(wl), Run(wl)
+    a.init_dt()
+    b.init_dt()
+    for st in range(1, 51):
+        b.full_step()
+        a.full_step(dt_override=b.t_step.copy())
+        if st in (1, 10, 50):
+            print(f"nested121 step {st}", flush=True)
+            for pre in (("", "old", "wrk") if st == 50 else ("",)):
+                ha, hb = a.state_host(pre), b.state_host(pre)
+                bad = [k for k in FIELDS if not np.array_equal(ha[k], hb[k])]
+                assert not bad, f"step {st} {pre or 'state'}: {bad}"
+    assert a.box_steps == b.box_steps
+
+
 def test_native_single61_100_steps():
     """The 61x61x31 single-grid case over 100 steps, every field checked
     after 1, 10 and 100 steps (Lapidus and FCT every step)."""
