@@ -45,6 +45,10 @@ struct Run {
     float* rst_eff = nullptr;     // rst on BandE's list points of box mbndry
     double t = 0.0;
     std::vector<double> t_old, t_new, t_step, t_stepnew;
+    // the main loop's tilt and universal time, now and at the step's start
+    // (code/3d_main.f90:1966-1977)
+    float tilt = 0.0f, old_tilt = 0.0f;
+    double ut = 0.0, utold = 0.0;
     long long cell_updates = 0, box_steps = 0;
     float iso_flag = 1.0f;
     int err = 0;
@@ -533,6 +537,42 @@ void corer(Run* R, int box) {
     launch_corer(R->B, fine, coarse, NF, nskip, nbx1, nby1, nbz1, R->s);
 }
 
+// b0 of one box for the tilt and rotation at its half step
+// (code/3d_main.f90:2173-2187), in the reference's mixed precision
+void tilt_box(Run* R, int box, float delt2) {
+    const g3m3_run_desc& d = R->d;
+    const int b = box - 1;
+    const float pi = 3.14159274f;   // 4*atan(1.0) in single precision
+    const double tb = R->t_old[b] + (double)delt2;
+    const float atilt = (float)((double)R->old_tilt + tb * (double)d.dtilt);
+    const float sin_tilt = std::sin(atilt * pi / 180.0f);   // the C library's sinf / cosf
+    const float cos_tilt = std::cos(atilt * pi / 180.0f);
+    const float ut2 = (float)(R->utold + tb * (double)d.t_equiv / 3600.0);
+    const int nrot2 = (int)(ut2 / d.planet_per);
+    const float rot_hr2 = ut2 - (float)nrot2 * d.planet_per;
+    const float rot_angle = 2.0f * pi * rot_hr2 / d.planet_per;
+    RotationGeom rot;
+    rot.xdip = d.xdip;
+    rot.ydip = d.ydip;
+    rot.zdip = d.zdip;
+    rot.sin_rot = std::sin(rot_angle);
+    rot.cos_rot = std::cos(rot_angle);
+    rot.sin_tilt = sin_tilt;
+    rot.cos_tilt = cos_tilt;
+    rot.b0 = d.dip_b0;
+    InitGeom G;
+    G.xmin = d.grd_xmin[b];
+    G.ymin = d.grd_ymin[b];
+    G.zmin = d.grd_zmin[b];
+    G.dx = (d.grd_xmax[b] - d.grd_xmin[b]) / (d.nx - 1.0f);
+    G.dy = (d.grd_ymax[b] - d.grd_ymin[b]) / (d.ny - 1.0f);
+    G.dz = (d.grd_zmax[b] - d.grd_zmin[b]) / (d.nz - 1.0f);
+    const size_t off = (size_t)b * R->n;
+    launch_mak_dip_grd(R->B, const_cast<float*>(d.bx0) + off, const_cast<float*>(d.by0) + off,
+                       const_cast<float*>(d.bz0) + off, G, rot, d.r_inner, d.ijzero, d.mbndry,
+                       box, box <= d.mbndry ? d.numzero : 0, R->s);
+}
+
 // -- code/3d_main.f90:2163-3065 ----------------------------------------------------
 void box_step(Run* R, int box) {
     g3m3_run_desc& d = R->d;
@@ -541,6 +581,7 @@ void box_step(Run* R, int box) {
     R->t_new[b] = R->t_old[b] + R->t_step[b];
     const float delt = (float)R->t_step[b];
     const float delt2 = delt / 2.0f;
+    if (d.tilting) tilt_box(R, box, delt2);
     // store_array x34 (:2193-2229) by rotation: the state becomes old and
     // step 2 writes the new state into the other slice (its push copies the
     // cells it does not compute from old)
@@ -634,7 +675,9 @@ __global__ void k_set_dt(float* dt, double* t, float dt0, double t0) {
 // Captures the two full steps (one per rotation parity) of a single-grid
 // run; host bookkeeping touched by the capture is restored
 bool build_graphs(Run* R) {
-    if (R->d.n_grids != 1) return false;
+    // captured steps replay fixed kernel arguments: a tilting dipole's
+    // rotation is formed on the host per box-step
+    if (R->d.n_grids != 1 || R->d.tilting) return false;
     if (!R->s) {
         if (!ok_hip(R, hipStreamCreate(&R->s), "graph stream")) return false;
         R->own_stream = true;
@@ -777,6 +820,10 @@ void* g3m3_run_create(const g3m3_run_desc* desc) {
     R->t_new.assign(ng, 0.0);
     R->t_step.assign(ng, 0.0);
     R->t_stepnew.assign(ng, 0.0);
+    R->tilt = R->d.tilt;
+    R->old_tilt = R->d.tilt;
+    R->ut = R->d.ut;
+    R->utold = R->d.ut;
     R->rot.assign(ng, 0);
     tune_fct(R);
     tune_push(R);
@@ -869,7 +916,14 @@ int g3m3_run_full_step_async(void* run, const double* dt_override) {
     if (dt_override)
         for (int b = 0; b < n; ++b) ts[b] = dt_override[b];
     const int m_step = (int)(ts[n - 1] / ts[m_small - 1]);
-    R->t += (double)(float)ts[n - 1];
+    // the main loop's clock (code/3d_main.f90:1966-1977): delt is single
+    // precision there, t and ut double, tilt single
+    const float delt_main = (float)ts[n - 1];
+    R->utold = R->ut;
+    R->old_tilt = R->tilt;
+    R->t += (double)delt_main;
+    R->ut = R->t * (double)R->d.t_equiv / 3600.0;
+    R->tilt = R->tilt + R->d.dtilt * delt_main;
     std::vector<int> stepped;
     for (int ms = 0; ms < m_step; ++ms) {
         for (int box = n; box >= 1; --box) {
@@ -906,8 +960,8 @@ int g3m3_run_finish(void* run) {
 int g3m3_run_graph_steps(void* run, int nsteps) {
     Run* R = (Run*)run;
     if (R->err) return R->err;
-    if (R->d.n_grids != 1) {
-        fprintf(stderr, "g3m3_run_graph_steps: single-grid runs only\n");
+    if (R->d.n_grids != 1 || R->d.tilting) {
+        fprintf(stderr, "g3m3_run_graph_steps: single-grid runs without tilting only\n");
         return R->err = -1;
     }
     if (!R->gexec[0] || !R->gexec[1]) {

@@ -14,6 +14,9 @@ after its last validation pass (the reference reads them into
 """
 from __future__ import annotations
 
+import ctypes as _ct
+import ctypes.util as _ctu
+
 import numpy as np
 
 from . import _lib
@@ -30,6 +33,13 @@ def _torch():
     if not torch.cuda.is_available():
         raise _lib.G3M3Error("no GPU visible: the G3M3 hot path runs only on the device")
     return torch
+
+
+# the C library's single-precision sin / cos, as a compiled driver calls them
+_LIBM = _ct.CDLL(_ctu.find_library("m"))
+for _fn in ("sinf", "cosf"):
+    getattr(_LIBM, _fn).restype = _ct.c_float
+    getattr(_LIBM, _fn).argtypes = [_ct.c_float]
 
 
 class Run:
@@ -75,6 +85,9 @@ class Run:
         self.t_step = np.zeros(self.ngrd, np.float64)
         self.t_stepnew = np.zeros(self.ngrd, np.float64)
         self.t = 0.0
+        # the main loop's tilt and universal time (code/3d_main.f90:1966-1977)
+        self.tilt = self.old_tilt = F(s.get("tilt", 0.0))
+        self.ut = self.utold = float(s.get("ut", 0.0))
         self.cell_updates = 0
         self.box_steps = 0
         # host arrays of device pointers for the 34-field entry points
@@ -155,6 +168,8 @@ class Run:
         delt = F(self.t_step[b])
         delt2 = delt / F(2.)
         s["box"] = box
+        if s.get("tilting"):
+            self._tilt(box, delt2)
         self._store_old(b)
         s["rx"] = s["ry"] = s["rz"] = F(a["xspac"][b])
 
@@ -252,7 +267,13 @@ class Run:
         if dt_override is not None:
             ts[:] = dt_override
         m_step = int(ts[n - 1] / ts[m_small - 1])
-        self.t += float(F(ts[n - 1]))
+        # the main loop's clock (code/3d_main.f90:1966-1977): delt single,
+        # t and ut double, tilt single
+        delt = F(ts[n - 1])
+        self.utold, self.old_tilt = self.ut, self.tilt
+        self.t += float(delt)
+        self.ut = self.t * float(F(self.s.get("t_equiv", 1.0))) / 3600.0
+        self.tilt = F(self.tilt + F(F(self.s.get("dtilt", 0.0)) * delt))
         for _ in range(m_step):
             for box in range(n, 0, -1):
                 b = box - 1
@@ -268,6 +289,27 @@ class Run:
                       self.nz, self.ngrd, box, *[self.a[g] for g in sig.GRID])
             self.call("set_rho", sig.set_rho(""))
         _lib.check("full_step")
+
+    def _tilt(self, box, delt2):
+        """code/3d_main.f90:2173-2187: the box's b0 recomputed for the tilt
+        and the planet's rotation at its half step (t_old and utold double,
+        the rest single; sin / cos are the C library's sinf / cosf)."""
+        s, a = self.s, self.a
+        pi = F(np.pi)   # 4*atan(1.0) in single precision
+        tb = float(self.t_old[box - 1]) + float(F(delt2))
+        atilt = F(float(self.old_tilt) + tb * float(F(s["dtilt"])))
+        arg = F(F(atilt * pi) / F(180.0))
+        sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))
+        ut2 = F(self.utold + tb * float(F(s["t_equiv"])) / 3600.0)
+        per = F(s["planet_per"])
+        nrot2 = int(F(ut2 / per))
+        rot_hr2 = F(ut2 - F(F(nrot2) * per))
+        rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
+        _lib.call("mak_dip_grd", a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz,
+                  self.ngrd, int(s["mbndry"]), box, a["ijzero"], a["numzero"],
+                  int(s["mzero"]), F(s["r_inner"]), *[a[g] for g in sig.GRID],
+                  rot_angle, F(s["xdip"]), F(s["ydip"]), F(s["zdip"]), F(sin_tilt),
+                  F(cos_tilt), F(s["b0"]))
 
     def divb(self, box, errlim, mode=0, max_iters=0):
         """divB_correct on one box (code/sub_divB_correct.f90:25-175)."""
@@ -289,7 +331,6 @@ class Run:
 # Native runtime (g3m3_run_*): the same full step, issued from C++ with the
 # fused kernels; Python makes one call per full step.
 
-import ctypes as _ct
 
 _MAXG = 8
 _SCALARS = ("rmassq", "rmassh", "rmasso", "gamma", "gamma1", "reynolds", "resist", "grav",
@@ -311,7 +352,10 @@ class RunDesc(_ct.Structure):
                                                       "grd_ymax", "grd_zmin", "grd_zmax",
                                                       "xspac")]
                 + [(k, _ct.c_float) for k in _SCALARS]
-                + [("isotropic", _ct.c_int)])
+                + [("isotropic", _ct.c_int), ("tilting", _ct.c_int), ("tilt", _ct.c_float),
+                   ("dtilt", _ct.c_float), ("ut", _ct.c_double), ("t_equiv", _ct.c_float),
+                   ("planet_per", _ct.c_float), ("xdip", _ct.c_float), ("ydip", _ct.c_float),
+                   ("zdip", _ct.c_float), ("dip_b0", _ct.c_float), ("r_inner", _ct.c_float)])
 
 
 # the reference's fluid-file array order (code/3d_main.f90:3312-3348)
@@ -354,6 +398,11 @@ class FastRun(Run):
         for k in _SCALARS:
             setattr(d, k, float(self.s[k]))
         d.isotropic = 1 if self.s["isotropic"] else 0
+        d.tilting = 1 if self.s.get("tilting") else 0
+        for k in ("tilt", "dtilt", "t_equiv", "planet_per", "xdip", "ydip", "zdip", "r_inner"):
+            setattr(d, k, float(self.s.get(k, 0.0)))
+        d.ut = float(self.s.get("ut", 0.0))
+        d.dip_b0 = float(self.s["b0"])
         self._desc = d
         self.h = L.g3m3_run_create(_ct.byref(d))
         if not self.h:
@@ -463,6 +512,27 @@ class FastRun(Run):
         in the old tensors (the runtime swaps the two per box-step)."""
         if self._L.g3m3_run_sync_arrays(_ct.c_void_p(self.h)):
             raise _lib.G3M3Error("g3m3_run_sync_arrays failed")
+
+    def _tilt(self, box, delt2):
+        """code/3d_main.f90:2173-2187: the box's b0 recomputed for the tilt
+        and the planet's rotation at its half step (t_old and utold double,
+        the rest single; sin / cos are the C library's sinf / cosf)."""
+        s, a = self.s, self.a
+        pi = F(np.pi)   # 4*atan(1.0) in single precision
+        tb = float(self.t_old[box - 1]) + float(F(delt2))
+        atilt = F(float(self.old_tilt) + tb * float(F(s["dtilt"])))
+        arg = F(F(atilt * pi) / F(180.0))
+        sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))
+        ut2 = F(self.utold + tb * float(F(s["t_equiv"])) / 3600.0)
+        per = F(s["planet_per"])
+        nrot2 = int(F(ut2 / per))
+        rot_hr2 = F(ut2 - F(F(nrot2) * per))
+        rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
+        _lib.call("mak_dip_grd", a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz,
+                  self.ngrd, int(s["mbndry"]), box, a["ijzero"], a["numzero"],
+                  int(s["mzero"]), F(s["r_inner"]), *[a[g] for g in sig.GRID],
+                  rot_angle, F(s["xdip"]), F(s["ydip"]), F(s["zdip"]), F(sin_tilt),
+                  F(cos_tilt), F(s["b0"]))
 
     def divb(self, box, errlim, mode=0, max_iters=0):
         self.sync_arrays()

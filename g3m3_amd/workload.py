@@ -131,6 +131,11 @@ def make_workload(name: str = "nested121", seed: int = 20261203, **over) -> Work
         stepsz=F(0.25), t_step_max=F(20.), vlim_factor=F(0.6), mbndry=1,
         v_rot=F(0.), r_rot=F(0.), erho=F(0.), epress=F(0.), alpha_e=F(6.),
         r_inner=F(p["r_inner"]), eta=F(p["eta"]), b0=F(0.),
+        # dipole tilting during the run (code/3d_main.f90:1966-1977, 2173-2187)
+        tilting=bool(p.get("tilting", False)), tilt=F(p.get("tilt", 0.0)),
+        dtilt=F(p.get("dtilt", 0.0)), ut=float(p.get("ut", 0.0)),
+        t_equiv=F(p.get("t_equiv", 2.5)), planet_per=F(p.get("planet_per", 10.0)),
+        xdip=F(0.00001), ydip=F(0.0), zdip=F(0.0),
     )
     # dipole strength: |B| = 10 b_equiv on the equator at r_inner
     b0 = F(10.0 * float(p["r_inner"]) ** 3)

@@ -580,6 +580,17 @@ typedef struct {
     float srho, spx, spy, spz, spress, sbx_wind, sby_wind, sbz_wind, v_rot, r_rot;
     float eta;                         /* uniform resistivity (0 = reference) */
     int isotropic;
+    /* Dipole tilting during the run (code/3d_main.f90:1966-1977, 2173-2187).
+     * With tilting set, every box-step first recomputes its box's b0 with
+     * mak_dip_grd for the tilt and rotation at its half step, rewriting
+     * bx0..bz0 in place; tilt, ut advance once per full step from the
+     * values given here.  tilting = 0: b0 is left alone. */
+    int tilting;
+    float tilt, dtilt;                 /* tilt now (degrees), its rate per unit time */
+    double ut;                         /* universal time now (hours) */
+    float t_equiv, planet_per;         /* the time unit (s) and the planet's period (h) */
+    float xdip, ydip, zdip, dip_b0;    /* common /rotation/: dipole offset and strength */
+    float r_inner;
 } g3m3_run_desc;
 
 void* g3m3_run_create(const g3m3_run_desc* desc);   /* NULL on failure */
@@ -593,7 +604,7 @@ int g3m3_run_full_step(void* run, const double* dt_override);   /* NULL = own dt
  * that set the next step (the one host synchronisation of a full step). */
 int g3m3_run_full_step_async(void* run, const double* dt_override);
 int g3m3_run_finish(void* run);
-/* Single-grid runs (n_grids = 1): nsteps full steps replayed from HIP
+/* Single-grid runs (n_grids = 1, no tilting): nsteps full steps replayed from HIP
  * graphs captured on the first call (one per buffer-rotation parity).  The
  * time step lives on the device: a kernel forms each next step from the
  * step's CFL maxima exactly as g3m3_run_finish does on the host, so no step

@@ -74,6 +74,9 @@ class OracleRun:
         self.t_step = np.zeros(n, np.float64)
         self.t_stepnew = np.zeros(n, np.float64)
         self.t = 0.0
+        # the main loop's tilt and universal time (code/3d_main.f90:1966-1977)
+        self.tilt = self.old_tilt = F(s.get("tilt", 0.0))
+        self.ut = self.utold = float(s.get("ut", 0.0))
         self.cell_updates = 0
         self.box_steps = 0
         self.dt_log = []
@@ -96,6 +99,26 @@ class OracleRun:
             self.lib.call(name, *args)
 
     # -- pieces of the box-step ------------------------------------------------
+    def _tilt(self, box, delt2):
+        """The box's b0 for the tilt and rotation at its half step."""
+        s, a = self.s, self.a
+        sin_tilt, cos_tilt, rot_angle = O.tilt_rotation(
+            self.old_tilt, self.utold, a["t_old"][box - 1], delt2, s["dtilt"], s["t_equiv"],
+            s["planet_per"])
+        rot = dict(rot_angle=rot_angle, xdip=s["xdip"], ydip=s["ydip"], zdip=s["zdip"],
+                   sin_tilt=sin_tilt, cos_tilt=cos_tilt, b0=s["b0"])
+        args = [a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz, self.ngrd,
+                s["mbndry"], box, a["ijzero"], a["numzero"], s["mzero"], F(s["r_inner"]),
+                a["grd_xmin"], a["grd_xmax"], a["grd_ymin"], a["grd_ymax"], a["grd_zmin"],
+                a["grd_zmax"]]
+        if self.backend == "restate":
+            O.mak_dip_grd(*args, **rot)
+        else:
+            r = self.lib.rotation()
+            for k, v in rot.items():
+                setattr(r, k, float(v))
+            self.lib.call("mak_dip_grd", *args)
+
     def _btot(self, pre, b):
         a = self.a
         a["bsx"][...] = a["bx0"][b] + a[pre + "bx"][b]
@@ -182,6 +205,8 @@ class OracleRun:
         delt = F(self.t_step[b])
         delt2 = delt / F(2.)
         s["box"] = box
+        if s.get("tilting"):                          # :2173-2187
+            self._tilt(box, delt2)
         for k in FIELDS:                              # store_array x34
             a["old" + k][b] = a[k][b]
         s["rx"] = s["ry"] = s["rz"] = F(a["xspac"][b])
@@ -285,7 +310,9 @@ class OracleRun:
         m_step = int(ts[n - 1] / ts[m_small - 1])
         delt = F(ts[n - 1])
         self.dt_log.append(ts.copy())
-        self.t += float(delt)
+        self.t, self.ut, self.tilt, self.utold, self.old_tilt = O.main_clock(
+            self.t, self.ut, self.tilt, delt, self.s.get("t_equiv", 1.0),
+            self.s.get("dtilt", 0.0))
         for _ in range(m_step):
             for box in range(n, 0, -1):
                 b = box - 1

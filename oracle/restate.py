@@ -1384,6 +1384,33 @@ def _box_axes(nx, ny, nz, x0, x1, y0, y1, z0, z1):
     return ax, ay, az
 
 
+def tilt_rotation(old_tilt, utold, t_old_box, delt2, dtilt, t_equiv, planet_per):
+    """code/3d_main.f90:2173-2180: the dipole's tilt and the planet's rotation
+    angle at a box's half step, in the reference's mixed precision (t_old,
+    utold double; everything else single; pi = 4*atan(1.0) in single).
+    Returns (sin_tilt, cos_tilt, rot_angle) as float32."""
+    pi = F(np.pi)   # 4*atan(1.0) in single precision, 3.14159274
+    tb = float(t_old_box) + float(F(delt2))
+    atilt = F(float(F(old_tilt)) + tb * float(F(dtilt)))
+    arg = F(F(atilt * pi) / F(180.0))
+    sin_tilt, cos_tilt = _sinf(arg), _cosf(arg)
+    ut2 = F(float(utold) + tb * float(F(t_equiv)) / 3600.0)
+    nrot2 = int(F(ut2 / F(planet_per)))
+    rot_hr2 = F(ut2 - F(F(nrot2) * F(planet_per)))
+    rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / F(planet_per))
+    return sin_tilt, cos_tilt, rot_angle
+
+
+def main_clock(t, ut, tilt, delt, t_equiv, dtilt):
+    """code/3d_main.f90:1966-1977: the full step's clock advance; returns
+    (t, ut, tilt, utold, old_tilt) -- t and ut double, tilt single."""
+    utold, old_tilt = ut, F(tilt)
+    t = t + float(F(delt))
+    ut = t * float(F(t_equiv)) / 3600.0
+    tilt = F(F(tilt) + F(F(dtilt) * F(delt)))
+    return t, ut, tilt, utold, old_tilt
+
+
 def mak_dip_grd(bx0, by0, bz0, nx, ny, nz, n_grids, mbndry, box, ijzero, numzero, mzero,
                 r_inner, grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax,
                 rot_angle=0.0, xdip=0.0, ydip=0.0, zdip=0.0, sin_tilt=0.0, cos_tilt=1.0,

@@ -384,3 +384,39 @@ def test_torus_field_cut():
     for key in FIELDS:
         assert np.array_equal(d["f"][key], c["f"][key])
     assert tot == (0.0, 0.0, 0.0)
+
+
+# --- dipole tilting during the run (code/3d_main.f90:1966-1977, 2173-2187) ----
+
+def test_tilt_rotation_closed_forms():
+    """The half-step tilt and rotation angle: with no tilt rate the tilt is
+    the step's starting tilt; a universal time on a whole number of planet
+    periods gives rotation angle 0, a quarter period pi/2 (to an ulp); the
+    tilt enters sin / cos in degrees."""
+    F = np.float32
+    # no tilt rate: atilt = old_tilt exactly, so sin_tilt = sinf(old_tilt*pi/180)
+    st, ct, _ = O.tilt_rotation(F(30.0), 0.0, 1.25, F(0.05), F(0.0), F(2.5), F(10.0))
+    arg = F(F(F(30.0) * F(np.pi)) / F(180.0))
+    assert st == O._sinf(arg) and ct == O._cosf(arg)
+    assert abs(float(st) - 0.5) < 1e-6
+    # ut2 = utold + (t_old + delt2) * t_equiv / 3600 = 20 h = 2 periods -> angle 0
+    _, _, ang = O.tilt_rotation(F(0.0), 20.0, 0.0, F(0.0), F(0.0), F(2.5), F(10.0))
+    assert ang == F(0.0)
+    # a quarter period past: pi/2 within an ulp
+    _, _, ang = O.tilt_rotation(F(0.0), 22.5, 0.0, F(0.0), F(0.0), F(2.5), F(10.0))
+    assert abs(float(ang) - np.pi / 2) <= 2.5e-7
+    # the time argument advances ut by t_equiv/3600 hours per unit time:
+    # 3600 units at t_equiv 2.5 s is 2.5 h, a quarter of a 10 h period
+    _, _, ang = O.tilt_rotation(F(0.0), 0.0, 3600.0, F(0.0), F(0.0), F(2.5), F(10.0))
+    assert abs(float(ang) - np.pi / 2) <= 2.5e-7
+    # a tilt rate: atilt = old_tilt + (t_old + delt2) * dtilt
+    st, _, _ = O.tilt_rotation(F(10.0), 0.0, 19.5, F(0.5), F(1.0), F(2.5), F(10.0))
+    assert st == O._sinf(F(F(F(30.0) * F(np.pi)) / F(180.0)))
+
+
+def test_main_clock():
+    """The full step's clock: t += delt (single), ut = t t_equiv / 3600,
+    tilt += dtilt delt; the step's starting ut and tilt are returned."""
+    F = np.float32
+    t, ut, tilt, utold, old_tilt = O.main_clock(1.0, 7.0, F(5.0), F(0.25), F(3600.0), F(2.0))
+    assert t == 1.25 and ut == 1.25 and tilt == F(5.5) and utold == 7.0 and old_tilt == F(5.0)
