@@ -548,9 +548,14 @@ void tilt_box(Run* R, int box, float delt2) {
     const float sin_tilt = std::sin(atilt * pi / 180.0f);   // the C library's sinf / cosf
     const float cos_tilt = std::cos(atilt * pi / 180.0f);
     const float ut2 = (float)(R->utold + tb * (double)d.t_equiv / 3600.0);
-    const int nrot2 = (int)(ut2 / d.planet_per);
-    const float rot_hr2 = ut2 - (float)nrot2 * d.planet_per;
-    const float rot_angle = 2.0f * pi * rot_hr2 / d.planet_per;
+    // a planet without rotation (period 0) has no rotation angle; the
+    // reference's division by 0 there has no defined integer result
+    float rot_angle = 0.0f;
+    if (d.planet_per != 0.0f) {
+        const int nrot2 = (int)(ut2 / d.planet_per);
+        const float rot_hr2 = ut2 - (float)nrot2 * d.planet_per;
+        rot_angle = 2.0f * pi * rot_hr2 / d.planet_per;
+    }
     RotationGeom rot;
     rot.xdip = d.xdip;
     rot.ydip = d.ydip;

@@ -302,9 +302,11 @@ class Run:
         sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))
         ut2 = F(self.utold + tb * float(F(s["t_equiv"])) / 3600.0)
         per = F(s["planet_per"])
-        nrot2 = int(F(ut2 / per))
-        rot_hr2 = F(ut2 - F(F(nrot2) * per))
-        rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
+        rot_angle = F(0.0)   # a planet without rotation (period 0)
+        if per != 0:
+            nrot2 = int(F(ut2 / per))
+            rot_hr2 = F(ut2 - F(F(nrot2) * per))
+            rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
         _lib.call("mak_dip_grd", a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz,
                   self.ngrd, int(s["mbndry"]), box, a["ijzero"], a["numzero"],
                   int(s["mzero"]), F(s["r_inner"]), *[a[g] for g in sig.GRID],
@@ -525,9 +527,11 @@ class FastRun(Run):
         sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))
         ut2 = F(self.utold + tb * float(F(s["t_equiv"])) / 3600.0)
         per = F(s["planet_per"])
-        nrot2 = int(F(ut2 / per))
-        rot_hr2 = F(ut2 - F(F(nrot2) * per))
-        rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
+        rot_angle = F(0.0)   # a planet without rotation (period 0)
+        if per != 0:
+            nrot2 = int(F(ut2 / per))
+            rot_hr2 = F(ut2 - F(F(nrot2) * per))
+            rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
         _lib.call("mak_dip_grd", a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz,
                   self.ngrd, int(s["mbndry"]), box, a["ijzero"], a["numzero"],
                   int(s["mzero"]), F(s["r_inner"]), *[a[g] for g in sig.GRID],

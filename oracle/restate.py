@@ -1395,6 +1395,8 @@ def tilt_rotation(old_tilt, utold, t_old_box, delt2, dtilt, t_equiv, planet_per)
     arg = F(F(atilt * pi) / F(180.0))
     sin_tilt, cos_tilt = _sinf(arg), _cosf(arg)
     ut2 = F(float(utold) + tb * float(F(t_equiv)) / 3600.0)
+    if F(planet_per) == 0:   # no rotation (the reference divides by 0 here)
+        return sin_tilt, cos_tilt, F(0.0)
     nrot2 = int(F(ut2 / F(planet_per)))
     rot_hr2 = F(ut2 - F(F(nrot2) * F(planet_per)))
     rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / F(planet_per))
