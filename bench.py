@@ -150,6 +150,28 @@ def batch_case(n: int = 32, steps: int = 20, streams: int = 8):
             "graph_speedup": (cu_graph / t_graph) / plain}
 
 
+def gradient_case():
+    """The differentiation configuration (three nested 81x81x41 boxes, 50 full
+    steps, 8 craft sampled every 10 steps) against a target run with
+    perturbed inputs: the loss, its gradient by central differences, and
+    the device time of one forward run and of the gradient's 12."""
+    from g3m3_amd import misfit as M
+    cfg = M.MisfitConfig()
+    theta = M.THETA0.copy()
+    target_theta = np.array([1.0, -1.0, -6.0, 5.5, 420.0, 1e-3])
+    target, _ = M.forward(target_theta, cfg)
+    s, fwd = M.forward(theta, cfg)
+    g, secs = M.gradient_fd(theta, target, cfg)
+    return {"config": "3 nested 81x81x41 boxes, 50 full steps, 8 craft sampled every 10 steps",
+            "theta": dict(zip(M.THETA_NAMES, theta.tolist())),
+            "target_theta": dict(zip(M.THETA_NAMES, target_theta.tolist())),
+            "loss": M.loss(s, target), "gradient": dict(zip(M.THETA_NAMES, g.tolist())),
+            "method": "central differences, h = 1e-3 |theta_i|, two forward runs per "
+                      "parameter (no adjoint)",
+            "forward_s": round(fwd, 4), "gradient_s": round(secs, 4),
+            "gradient_over_forward": round(secs / fwd, 2)}
+
+
 def precision_case(fast_mode: int):
     """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
     FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
@@ -249,6 +271,9 @@ def main(argv=None):
                     help="native driver: the species push / Lapidus kernels (auto: the run's "
                          "autotuner picks; counter runs fix the family the timed runs pick, "
                          "since counter collection changes the timings the tuner sees)")
+    ap.add_argument("--gradient", action="store_true",
+                    help="also run the differentiation case: 3 nested 81x81x41 boxes, 50 "
+                         "steps, craft misfit and its central-difference gradient")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -331,6 +356,7 @@ def main(argv=None):
 
     prec = precision_case(a.divb_mode) if a.precision else None
     batch = batch_case() if a.batch else None
+    grad = gradient_case() if a.gradient else None
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -354,6 +380,8 @@ def main(argv=None):
         out["divb_precision"] = prec
     if batch is not None:
         out["batch"] = batch
+    if grad is not None:
+        out["gradient"] = grad
     print(json.dumps(out))
     return 0
 
