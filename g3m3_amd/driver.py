@@ -515,29 +515,6 @@ class FastRun(Run):
         if self._L.g3m3_run_sync_arrays(_ct.c_void_p(self.h)):
             raise _lib.G3M3Error("g3m3_run_sync_arrays failed")
 
-    def _tilt(self, box, delt2):
-        """code/3d_main.f90:2173-2187: the box's b0 recomputed for the tilt
-        and the planet's rotation at its half step (t_old and utold double,
-        the rest single; sin / cos are the C library's sinf / cosf)."""
-        s, a = self.s, self.a
-        pi = F(np.pi)   # 4*atan(1.0) in single precision
-        tb = float(self.t_old[box - 1]) + float(F(delt2))
-        atilt = F(float(self.old_tilt) + tb * float(F(s["dtilt"])))
-        arg = F(F(atilt * pi) / F(180.0))
-        sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))
-        ut2 = F(self.utold + tb * float(F(s["t_equiv"])) / 3600.0)
-        per = F(s["planet_per"])
-        rot_angle = F(0.0)   # a planet without rotation (period 0)
-        if per != 0:
-            nrot2 = int(F(ut2 / per))
-            rot_hr2 = F(ut2 - F(F(nrot2) * per))
-            rot_angle = F(F(F(F(2.0) * pi) * rot_hr2) / per)
-        _lib.call("mak_dip_grd", a["bx0"], a["by0"], a["bz0"], self.nx, self.ny, self.nz,
-                  self.ngrd, int(s["mbndry"]), box, a["ijzero"], a["numzero"],
-                  int(s["mzero"]), F(s["r_inner"]), *[a[g] for g in sig.GRID],
-                  rot_angle, F(s["xdip"]), F(s["ydip"]), F(s["zdip"]), F(sin_tilt),
-                  F(cos_tilt), F(s["b0"]))
-
     def divb(self, box, errlim, mode=0, max_iters=0):
         self.sync_arrays()
         return super().divb(box, errlim, mode, max_iters)

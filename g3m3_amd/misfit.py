@@ -10,8 +10,10 @@ sum((b - b*)^2 + (rho - rho*)^2) against a target run.
 
 The gradient with respect to theta = (IMF x, y, z, upstream n, upstream v,
 uniform resistivity) is formed by central differences at h = 1e-3 |theta_i|:
-two forward runs per parameter.  That is the check an adjoint would be held
-to, and its cost is what an adjoint (at most 4 forward runs) would replace.
+two forward runs per parameter, stepped together on several streams (a
+3 x 81x81x41 run alone leaves most of the GPU idle).  That is the check an
+adjoint would be held to, and its cost is what an adjoint (at most 4 forward
+runs) would replace.
 """
 from __future__ import annotations
 
@@ -60,43 +62,68 @@ def workload_for(theta, cfg: MisfitConfig):
                          n_sw=t[3], v_sw=t[4], eta=t[5])
 
 
+class _Sampler:
+    """The craft samples of one run: b x, y, z and the ion density."""
+
+    NAMES = ("bx", "by", "bz", "qrho", "hrho", "orho")
+
+    def __init__(self, run, wl, cfg: MisfitConfig):
+        import torch
+        self.run, self.wl, self.cfg = run, wl, cfg
+        self.pos = torch.from_numpy(craft_positions(cfg, float(wl.scal["re_equiv"]))).cuda()
+        nq = len(self.NAMES)
+        self.out = torch.zeros((cfg.ncraft, nq), dtype=torch.float32, device="cuda")
+        self.gp = torch.zeros((cfg.ncraft, 4), dtype=torch.int32, device="cuda")
+        self.ptrs = (_ct.c_void_p * nq)(*[run.a[k].data_ptr() for k in self.NAMES])
+        g = wl.grid
+        self.mn = np.ascontiguousarray(np.stack([g["grd_xmin"], g["grd_ymin"], g["grd_zmin"]],
+                                                1).astype(F))
+        self.mx = np.ascontiguousarray(np.stack([g["grd_xmax"], g["grd_ymax"], g["grd_zmax"]],
+                                                1).astype(F))
+        self.xs = np.ascontiguousarray(g["xspac"], dtype=F)
+        self.samples = []
+
+    def sample(self):
+        wl, cfg = self.wl, self.cfg
+        self.run.sync_arrays()     # the state back in the reference's arrays
+        _lib.call("sample_fields", cfg.ncraft, self.pos, cfg.ngrd, self.mn, self.mx, self.xs,
+                  F(wl.scal["re_equiv"]), wl.nx, wl.ny, wl.nz, len(self.NAMES), self.ptrs,
+                  self.out, self.gp)
+        h = self.out.cpu().numpy()
+        self.samples.append(np.stack([h[:, 0], h[:, 1], h[:, 2],
+                                      (h[:, 3] + h[:, 4]) + h[:, 5]], 1).astype(F))
+
+
 def forward(theta, cfg: MisfitConfig = MisfitConfig()):
     """Samples (steps // every, ncraft, 4): b x, y, z and the ion density at
     each craft after every ``every`` full steps; and the device time of the
     steps and samples (seconds, host set-up excluded)."""
+    return forward_many([theta], cfg, streams=1)
+
+
+def forward_many(thetas, cfg: MisfitConfig = MisfitConfig(), streams: int = 8):
+    """forward() of several parameter vectors at once: the runs step
+    together on ``streams`` streams (each run's results equal the run alone,
+    bit for bit).  Returns the list of samples (a single array for one run)
+    and the device time of all of them."""
     import torch
-    from .driver import FastRun
-    wl = workload_for(theta, cfg)
-    run = FastRun(wl)
-    run.init_dt()
-    pos = craft_positions(cfg, float(wl.scal["re_equiv"]))
-    dpos = torch.from_numpy(pos).cuda()
-    nq = 6
-    out = torch.zeros((cfg.ncraft, nq), dtype=torch.float32, device="cuda")
-    gp = torch.zeros((cfg.ncraft, 4), dtype=torch.int32, device="cuda")
-    names = ("bx", "by", "bz", "qrho", "hrho", "orho")
-    ptrs = (_ct.c_void_p * nq)(*[run.a[k].data_ptr() for k in names])
-    mn = np.ascontiguousarray(np.stack([wl.grid["grd_xmin"], wl.grid["grd_ymin"],
-                                        wl.grid["grd_zmin"]], 1).astype(F))
-    mx = np.ascontiguousarray(np.stack([wl.grid["grd_xmax"], wl.grid["grd_ymax"],
-                                        wl.grid["grd_zmax"]], 1).astype(F))
-    xs = np.ascontiguousarray(wl.grid["xspac"], dtype=F)
-    samples = []
+    from .driver import Batch
+    wls = [workload_for(t, cfg) for t in thetas]
+    batch = Batch(wls, streams=streams)
+    batch.init_dt()
+    samplers = [_Sampler(r, wl, cfg) for r, wl in zip(batch.runs, wls)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for st in range(1, cfg.steps + 1):
-        run.full_step()
+        batch.full_step()
         if st % cfg.every == 0:
-            run.sync_arrays()     # the state back in the reference's arrays
-            _lib.call("sample_fields", cfg.ncraft, dpos, cfg.ngrd, mn, mx, xs,
-                      F(wl.scal["re_equiv"]), wl.nx, wl.ny, wl.nz, nq, ptrs, out, gp)
-            h = out.cpu().numpy()
-            samples.append(np.stack([h[:, 0], h[:, 1], h[:, 2],
-                                     (h[:, 3] + h[:, 4]) + h[:, 5]], 1).astype(F))
+            for sm in samplers:
+                sm.sample()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     _lib.check("misfit forward")
-    return np.stack(samples), secs
+    out = [np.stack(sm.samples) for sm in samplers]
+    return (out[0] if len(out) == 1 else out), secs
 
 
 def loss(samples, target) -> float:
@@ -106,20 +133,29 @@ def loss(samples, target) -> float:
     return float(np.sum(d * d))
 
 
-def gradient_fd(theta, target, cfg: MisfitConfig = MisfitConfig(), rel: float = 1e-3):
+def gradient_fd(theta, target, cfg: MisfitConfig = MisfitConfig(), rel: float = 1e-3,
+                together: bool = True):
     """Central-difference gradient of the loss with respect to theta, step
-    rel * |theta_i| (rel where theta_i = 0); the device seconds of all the
-    forward runs it took."""
+    rel * |theta_i| (rel where theta_i = 0), and the device seconds its
+    forward runs took: all 2 len(theta) runs stepped together, or (together
+    False) one after another.  Both give the same gradient, bit for bit."""
     theta = np.asarray(theta, dtype=np.float64)
-    g = np.zeros_like(theta)
-    secs = 0.0
+    hs, runs = [], []
     for i in range(len(theta)):
         h = rel * abs(theta[i]) if theta[i] != 0.0 else rel
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h
         tm[i] -= h
-        sp, a = forward(tp, cfg)
-        sm, b = forward(tm, cfg)
-        secs += a + b
-        g[i] = (loss(sp, target) - loss(sm, target)) / (2.0 * h)
+        hs.append(h)
+        runs += [tp, tm]
+    if together:
+        samples, secs = forward_many(runs, cfg)
+    else:
+        samples, secs = [], 0.0
+        for t in runs:
+            sm, a = forward(t, cfg)
+            samples.append(sm)
+            secs += a
+    g = np.array([(loss(samples[2 * i], target) - loss(samples[2 * i + 1], target)) /
+                  (2.0 * hs[i]) for i in range(len(theta))])
     return g, secs
