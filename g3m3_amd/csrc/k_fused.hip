@@ -1083,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 }
 
 // ---------------------------------------------------------------------------
-// FCT in one pass (k_fct_fused): a block owns a 63 x 8 (i, j) tile of one
+// FCT in one pass (k_fct_fused): a block owns a 63 x FTY (i, j) tile of one
 // field and a run of k-planes (16 unless the run's autotuner picked another
 // depth), and marches it along k.  Rings of
 // three LDS planes hold old (3-cell halo), wrk and the diffused field P
@@ -1103,11 +1103,18 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 // difference at the last interior cell), so every stored value equals the
 // reference's.
 
-constexpr int FTX = 63, FTY = 8, FKC = 16;   // FKC: default k-planes per block
-constexpr int FPX = FTX + 5, FPY = FTY + 4;   // wrk / P: cells i0-3 .. i0+64, j0-2 .. j0+9
+// FTY rows per tile, two per wave: a taller tile spends less of its work on
+// the halo (P is made on (FTX + 5)(FTY + 4) cells for FTX * FTY outputs)
+#ifndef G3M3_FCT_TILE_ROWS
+#define G3M3_FCT_TILE_ROWS 8
+#endif
+constexpr int FTX = 63, FTY = G3M3_FCT_TILE_ROWS, FKC = 16;   // FKC: default k-planes per block
+constexpr int FNT = 32 * FTY;                 // threads per block
+constexpr int FPX = FTX + 5, FPY = FTY + 4;   // wrk / P: cells i0-3 .. i0+64, j0-2 .. j0+FTY+1
 constexpr int FOX = FPX + 2, FOY = FPY + 2;   // old: one more cell on each side
 constexpr int FO_N = FOX * FOY, FP_N = FPX * FPY;
-constexpr int FO_PT = (FO_N + 255) / 256, FP_PT = (FP_N + 255) / 256;
+constexpr int FO_PT = (FO_N + FNT - 1) / FNT, FP_PT = (FP_N + FNT - 1) / FNT;
+static_assert(FTY % 2 == 0 && FP_PT <= 32, "two rows per wave; xyface holds FP_PT bits");
 
 struct FctLds {
     float o[3][FO_N];
@@ -1138,7 +1145,7 @@ __device__ inline float fct_flux(const WA& W, const PA& P, int base, int m, int 
     return fct_lim(w1 - w0, p0 - pm1, d1p, chi);
 }
 
-__global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
+__global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
     __shared__ FctLds L;
     const Box& B = F.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
@@ -1170,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     int xyface = 0;
 #pragma unroll
     for (int m = 0; m < FP_PT; ++m) {
-        const int e = tid + 256 * m;
+        const int e = tid + FNT * m;
         const int ii = e % FPX, jj = e / FPX;
         const int gi = i0 - 3 + ii, gj = j0 - 2 + jj;
         offw[m] = (unsigned)(min(max(gj, 0), ny - 1) * nx + min(max(gi, 0), nx - 1)) * 4u;
@@ -1179,7 +1186,7 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     }
 #pragma unroll
     for (int m = 0; m < FO_PT; ++m) {
-        const int e = tid + 256 * m;
+        const int e = tid + FNT * m;
         const int ii = e % FOX, jj = e / FOX;
         const int gi = i0 - 4 + ii, gj = j0 - 3 + jj;
         offo[m] = (unsigned)(min(max(gj, 0), ny - 1) * nx + min(max(gi, 0), nx - 1)) * 4u;
@@ -1201,12 +1208,12 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
     auto store_w = [&](float* dst) {
 #pragma unroll
         for (int m = 0; m < FP_PT; ++m)
-            if (m < FP_PT - 1 || tid + 256 * m < FP_N) dst[tid + 256 * m] = rw[m];
+            if (m < FP_PT - 1 || tid + FNT * m < FP_N) dst[tid + FNT * m] = rw[m];
     };
     auto store_o = [&](float* dst) {
 #pragma unroll
         for (int m = 0; m < FO_PT; ++m)
-            if (m < FO_PT - 1 || tid + 256 * m < FO_N) dst[tid + 256 * m] = ro[m];
+            if (m < FO_PT - 1 || tid + FNT * m < FO_N) dst[tid + FNT * m] = ro[m];
     };
 
     // the cells this thread outputs, and their place in a P / wrk plane
@@ -1326,12 +1333,12 @@ __global__ void __launch_bounds__(256) k_fct_fused(FctParams F, int ntx, int nty
             const bool zface = (s == 0 || s == nz - 1);
 #pragma unroll
             for (int m = 0; m < FP_PT; ++m) {
-                if (m == FP_PT - 1 && tid + 256 * m >= FP_N) break;
+                if (m == FP_PT - 1 && tid + FNT * m >= FP_N) break;
                 const float wc = rw[m];
                 const int o0 = ocw[m];
                 const float vi = wc + chi * (oc[o0 + 1] + oc[o0 - 1] + oc[o0 + FOX] +
                                              oc[o0 - FOX] + op[o0] + om[o0] - 6.0f * oc[o0]);
-                dst[tid + 256 * m] = (zface || ((xyface >> m) & 1)) ? wc : vi;
+                dst[tid + FNT * m] = (zface || ((xyface >> m) & 1)) ? wc : vi;
             }
         }
         if (s + 1 <= pend) fetch_w(s + 1);   // in flight during the output
@@ -1366,7 +1373,7 @@ void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
     const int nkc = (F.B.nz + G.kc_len - 1) / G.kc_len;
     const int nb = ntx * nty * nkc * nf;
     ProfScope prof_("k_fct_fused", s);
-    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, 4), 0, s, G, ntx, nty, nkc);
+    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, FTY / 2), 0, s, G, ntx, nty, nkc);
 }
 
 // ---------------------------------------------------------------------------
