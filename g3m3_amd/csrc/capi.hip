@@ -86,6 +86,46 @@ struct Buf {
 };
 Buf g_speed_list, g_speed_part, g_small, g_divb_f, g_divb_d, g_tmp;
 
+}  // namespace
+
+// --- fixed scratch of the allocation-free entry point -----------------------------
+// g3m3_set_speed_agrd_noalloc keeps its side-list, per-block maxima, maxima
+// and count here instead of in grow-only workspace.  Layout (bytes): the 5
+// maxima at 0 and the count at 64; per-block maxima from kNoAllocPartOff;
+// the side-list (3 species x V entries of 48 bytes) from kNoAllocListOff.
+// Each device holds its own copy (18.9 MB) until the process ends.
+constexpr long long kNoAllocMaxCells = 131072;
+constexpr size_t kNoAllocPartBytes = 5 * ((kNoAllocMaxCells + 255) / 256) * sizeof(float);
+constexpr size_t kNoAllocListBytes = 3 * (size_t)kNoAllocMaxCells * 48;
+constexpr size_t kNoAllocPartOff = 256;
+constexpr size_t kNoAllocListOff = kNoAllocPartOff + (kNoAllocPartBytes + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocListOff + kNoAllocListBytes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_dev[kMaxDevices] = {};   // scratch's address on each device
+
+// The current device's address of `scratch`, looked up once per device
+// (g3m3_init does it for device 0).  The array's host-side name is not a
+// device address and is never passed to a kernel or a copy.
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "noalloc scratch")) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        latch(4, "noalloc scratch: device ordinal out of range");
+        return nullptr;
+    }
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "noalloc scratch address"))
+            return nullptr;
+        g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
 inline Box box_of(const int* nx, const int* ny, const int* nz) { return Box{*nx, *ny, *nz}; }
 inline size_t off(const Box& B, const int* box) { return (size_t)(*box - 1) * B.V(); }
 
@@ -162,6 +202,7 @@ int g3m3_init(void) {
         return 1;
     }
     if (!hip_ok(hipSetDevice(0), "init")) return 1;
+    if (!noalloc_scratch()) return 1;
     return 0;
 }
 
@@ -431,22 +472,34 @@ void g3m3_set_rho(float* qrho, float* qpresx, float* qpresy, float* qpresz, floa
     check_launch("set_rho");
 }
 
-void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int* n_grids,
-                         const int* box, const float* rmassq, const float* rmassh,
-                         const float* rmasso, const int* isotropic, const float* smallbit,
-                         const float* aniso_limit, const float* vlim, const float* qvlim,
-                         const float* hvlim, const float* ovlim, const float* qclim,
-                         const float* hclim, const float* oclim, const float* cslim,
-                         const float* bx, const float* by, const float* bz, const float* bx0,
-                         const float* by0, const float* bz0, float* qrho, float* qpx, float* qpy,
-                         float* qpz, float* qpresx, float* qpresy, float* qpresz, float* qpresxy,
-                         float* qpresxz, float* qpresyz, float* hrho, float* hpx, float* hpy,
-                         float* hpz, float* hpresx, float* hpresy, float* hpresz, float* hpresxy,
-                         float* hpresxz, float* hpresyz, float* orho, float* opx, float* opy,
-                         float* opz, float* opresx, float* opresy, float* opresz,
-                         float* opresxy, float* opresxz, float* opresyz, float* epres,
-                         float* btot, float* pxmax, float* pymax, float* pzmax, float* csmax,
-                         float* alfmax, float* fastest) {
+// The dummies of code/sub_set_speed_agrd.f90:6-22, in order, shared by the
+// entry point and its allocation-free variant.
+#define G3M3_SPEED_AGRD_PARAMS                                                                  \
+    const int *nx, const int *ny, const int *nz, const int *n_grids, const int *box,            \
+        const float *rmassq, const float *rmassh, const float *rmasso, const int *isotropic,    \
+        const float *smallbit, const float *aniso_limit, const float *vlim, const float *qvlim, \
+        const float *hvlim, const float *ovlim, const float *qclim, const float *hclim,         \
+        const float *oclim, const float *cslim, const float *bx, const float *by,               \
+        const float *bz, const float *bx0, const float *by0, const float *bz0, float *qrho,     \
+        float *qpx, float *qpy, float *qpz, float *qpresx, float *qpresy, float *qpresz,        \
+        float *qpresxy, float *qpresxz, float *qpresyz, float *hrho, float *hpx, float *hpy,    \
+        float *hpz, float *hpresx, float *hpresy, float *hpresz, float *hpresxy,                \
+        float *hpresxz, float *hpresyz, float *orho, float *opx, float *opy, float *opz,        \
+        float *opresx, float *opresy, float *opresz, float *opresxy, float *opresxz,            \
+        float *opresyz, float *epres, float *btot, float *pxmax, float *pymax, float *pzmax,    \
+        float *csmax, float *alfmax, float *fastest
+#define G3M3_SPEED_AGRD_ARGS                                                                    \
+    nx, ny, nz, n_grids, box, rmassq, rmassh, rmasso, isotropic, smallbit, aniso_limit, vlim,   \
+        qvlim, hvlim, ovlim, qclim, hclim, oclim, cslim, bx, by, bz, bx0, by0, bz0, qrho, qpx,  \
+        qpy, qpz, qpresx, qpresy, qpresz, qpresxy, qpresxz, qpresyz, hrho, hpx, hpy, hpz,       \
+        hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, orho, opx, opy, opz, opresx, opresy, \
+        opresz, opresxy, opresxz, opresyz, epres, btot, pxmax, pymax, pzmax, csmax, alfmax,     \
+        fastest
+
+// list: the averaging side-list (speed_list_bytes), part: per-block maxima
+// (speed_partials_bytes), small: the 5 maxima at byte 0 and the list count
+// at byte 64.
+static void speed_agrd_impl(G3M3_SPEED_AGRD_PARAMS, void* list, float* part, char* small) {
     const Box B = box_of(nx, ny, nz);
     SpeciesPtrs SP{};
     float* g[3][10] = {{qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, qpresxy, qpresxz, qpresyz},
@@ -470,10 +523,6 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
     Q.sclim[2] = *oclim;
     const float* b0[3] = {S(bx0, B, box), S(by0, B, box), S(bz0, B, box)};
     const float* b[3] = {S(bx, B, box), S(by, B, box), S(bz, B, box)};
-    void* list = g_speed_list.get(speed_list_bytes(B.V()));
-    float* part = (float*)g_speed_part.get(speed_partials_bytes(B.V()));
-    char* small = (char*)g_small.get(256);
-    if (!list || !part || !small) return;
     float* maxima = (float*)small;
     int* count = (int*)(small + 64);
     launch_set_speed(B, SP, Q, b0, b, R(btot), S(epres, B, box), maxima, list, count, part,
@@ -491,6 +540,35 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
     *alfmax = h[4];
     const float f = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2] + h[3] * h[3] + h[4] * h[4]);
     *fastest = std::max(0.0f, f);
+}
+
+void g3m3_set_speed_agrd(G3M3_SPEED_AGRD_PARAMS) {
+    const Box B = box_of(nx, ny, nz);
+    void* list = g_speed_list.get(speed_list_bytes(B.V()));
+    float* part = (float*)g_speed_part.get(speed_partials_bytes(B.V()));
+    char* small = (char*)g_small.get(256);
+    if (!list || !part || !small) return;
+    speed_agrd_impl(G3M3_SPEED_AGRD_ARGS, list, part, small);
+}
+
+// Allocation-free set_speed_agrd: the side-list, the per-block maxima, the
+// maxima and the count live in `scratch`, a fixed device array sized for
+// boxes of up to kNoAllocMaxCells cells.  Host code reaches the array only
+// through the per-device address noalloc_scratch() returns.
+void g3m3_set_speed_agrd_noalloc(G3M3_SPEED_AGRD_PARAMS) {
+    const Box B = box_of(nx, ny, nz);
+    const long long V = (long long)B.nx * B.ny * B.nz;
+    if (B.nx < 1 || B.ny < 1 || B.nz < 1 || V > kNoAllocMaxCells ||
+        speed_list_bytes((int)V) > kNoAllocListBytes ||
+        speed_partials_bytes((int)V) > kNoAllocPartBytes) {
+        latch(4, "set_speed_agrd_noalloc: the box exceeds the 131072-cell scratch (or has an "
+                 "empty dimension); call g3m3_set_speed_agrd");
+        return;
+    }
+    unsigned char* base = noalloc_scratch();
+    if (!base) return;
+    speed_agrd_impl(G3M3_SPEED_AGRD_ARGS, base + kNoAllocListOff,
+                    (float*)(base + kNoAllocPartOff), (char*)base);
 }
 
 void g3m3_species_velocity(const float* rho, const float* px, const float* py, const float* pz,

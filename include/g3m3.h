@@ -169,6 +169,28 @@ void g3m3_set_speed_agrd(const int* nx, const int* ny, const int* nz, const int*
                          float* btot, float* pxmax, float* pymax, float* pzmax, float* csmax,
                          float* alfmax, float* fastest);
 
+/* No counterpart in the reference.  g3m3_set_speed_agrd with the same
+ * arguments and results, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): it calls no hipMalloc/hipFree.  Its scratch is a fixed
+ * 18.9 MB device array per GPU, sized for boxes of at most 131072 cells
+ * (nx*ny*nz); a larger box latches an error and nothing runs.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (several
+ * streams or host threads) race. */
+void g3m3_set_speed_agrd_noalloc(
+    const int* nx, const int* ny, const int* nz, const int* n_grids, const int* box,
+    const float* rmassq, const float* rmassh, const float* rmasso, const int* isotropic,
+    const float* smallbit, const float* aniso_limit, const float* vlim, const float* qvlim,
+    const float* hvlim, const float* ovlim, const float* qclim, const float* hclim,
+    const float* oclim, const float* cslim, const float* bx, const float* by, const float* bz,
+    const float* bx0, const float* by0, const float* bz0, float* qrho, float* qpx, float* qpy,
+    float* qpz, float* qpresx, float* qpresy, float* qpresz, float* qpresxy, float* qpresxz,
+    float* qpresyz, float* hrho, float* hpx, float* hpy, float* hpz, float* hpresx,
+    float* hpresy, float* hpresz, float* hpresxy, float* hpresxz, float* hpresyz, float* orho,
+    float* opx, float* opy, float* opz, float* opresx, float* opresy, float* opresz,
+    float* opresxy, float* opresxz, float* opresyz, float* epres, float* btot, float* pxmax,
+    float* pymax, float* pzmax, float* csmax, float* alfmax, float* fastest);
+
 /* code/sub_lap_plasma.f90:4-10 -- Lapidus smoothing, one species */
 void g3m3_lap_plasma(const float* rho, const float* px, const float* py, const float* pz,
                      const float* presx, const float* presy, const float* presz,

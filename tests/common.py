@@ -216,3 +216,18 @@ def torus_oracle(c):
     tot = O.torus_inject(d["f"], *d["b0"], d["nx"], d["ny"], d["nz"], d["gmin"], d["gmax"],
                          d["s"])
     return d, tot
+
+
+def same_bits(a, b) -> bool:
+    """True when two float32 arrays hold the same bit patterns (so -0.0 and
+    0.0 differ, and NaNs compare by payload)."""
+    a = np.ascontiguousarray(a, dtype=F)
+    b = np.ascontiguousarray(b, dtype=F)
+    return a.shape == b.shape and bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+
+
+def bit_diffs(a, b) -> int:
+    """Number of elements whose float32 bit patterns differ."""
+    a = np.ascontiguousarray(a, dtype=F)
+    b = np.ascontiguousarray(b, dtype=F)
+    return int((a.view(np.uint32) != b.view(np.uint32)).sum())
