@@ -25,6 +25,10 @@ enum Fld {   // canonical field order
     QRHO = 0, HRHO = 10, ORHO = 20, EPRES = 30, BX = 31, BY = 32, BZ = 33
 };
 constexpr int NF = 34;
+// the main loop's smallest time step (code/3d_main.f90:116) and the
+// run error a smaller (or non-finite) one latches
+constexpr float kTStepMin = 1.e-6f;
+constexpr int kErrStepCollapsed = -2;
 
 // one box's slice of every field
 struct Set {
@@ -537,13 +541,14 @@ void corer(Run* R, int box) {
     launch_corer(R->B, fine, coarse, NF, nskip, nbx1, nby1, nbz1, R->s);
 }
 
-// b0 of one box for the tilt and rotation at its half step
-// (code/3d_main.f90:2173-2187), in the reference's mixed precision
-void tilt_box(Run* R, int box, float delt2) {
+// b0 of one box for the tilt and rotation at t_old + dt_off, in the
+// reference's mixed precision: before step 1 at the half step
+// (code/3d_main.f90:2173-2187), before step 2 at the full step (:2465-2478)
+void tilt_box(Run* R, int box, float dt_off) {
     const g3m3_run_desc& d = R->d;
     const int b = box - 1;
     const float pi = 3.14159274f;   // 4*atan(1.0) in single precision
-    const double tb = R->t_old[b] + (double)delt2;
+    const double tb = R->t_old[b] + (double)dt_off;
     const float atilt = (float)((double)R->old_tilt + tb * (double)d.dtilt);
     const float sin_tilt = std::sin(atilt * pi / 180.0f);   // the C library's sinf / cosf
     const float cos_tilt = std::cos(atilt * pi / 180.0f);
@@ -613,6 +618,7 @@ void box_step(Run* R, int box) {
     // to the same interpolant with weight 1 (w = 0*w + I0) before any read.
     // So the call is not issued (results equal up to the sign of an exact
     // zero).
+    if (d.tilting) tilt_box(R, box, delt);
     efield(R, wk, box);
     push(R, st, ol, wk, box, delt, 1.0f, 1, 1);
     tg = inner_box ? clamp_t(R->t_old[b] + (double)delt) : 0.0;
@@ -655,7 +661,9 @@ void read_tstepnew(Run* R, const std::vector<int>& boxes) {
 
 // The next time step of a single-grid run from the CFL maxima of its last
 // validation pass, as read_tstepnew forms it on the host, and the run time
-// advanced by the step just taken (code/3d_main.f90:1929-1968, 3038)
+// advanced by the step just taken (code/3d_main.f90:1929-1968, 3038).
+// dt[3] becomes 1 (and stays so) when a step is formed below t_step_min or
+// not finite, where the main loop would stop (:1969-1973).
 __global__ void k_next_dt(const float* m, float stepsz, float xspac, float t_step_max,
                           float* dt, double* t) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -667,6 +675,7 @@ __global__ void k_next_dt(const float* m, float stepsz, float xspac, float t_ste
     const float nd = (t_step_max < st) ? t_step_max : st;   // std::min(st, t_step_max)
     dt[1] = nd;
     dt[0] = nd;
+    if (!(nd >= kTStepMin && nd <= 3.0e38f)) dt[3] = 1.0f;
 }
 
 __global__ void k_set_dt(float* dt, double* t, float dt0, double t0) {
@@ -674,6 +683,7 @@ __global__ void k_set_dt(float* dt, double* t, float dt0, double t0) {
     dt[0] = dt0;
     dt[1] = dt0;
     dt[2] = dt0;
+    dt[3] = 0.0f;
     *t = t0;
 }
 
@@ -920,10 +930,19 @@ int g3m3_run_full_step_async(void* run, const double* dt_override) {
     }
     if (dt_override)
         for (int b = 0; b < n; ++b) ts[b] = dt_override[b];
-    const int m_step = (int)(ts[n - 1] / ts[m_small - 1]);
     // the main loop's clock (code/3d_main.f90:1966-1977): delt is single
     // precision there, t and ut double, tilt single
     const float delt_main = (float)ts[n - 1];
+    // the main loop stops on a step below t_step_min (:1969-1973); a step
+    // that is not a finite number stops the run too
+    if (!(std::isfinite(delt_main) && delt_main >= kTStepMin) ||
+        !(std::isfinite(ts[m_small - 1]) && ts[m_small - 1] > 0.0)) {
+        fprintf(stderr, "g3m3 run: time step %g below t_step_min %g or not finite\n",
+                (double)delt_main, (double)kTStepMin);
+        if (!R->err) R->err = kErrStepCollapsed;
+        return R->err;
+    }
+    const int m_step = (int)(ts[n - 1] / ts[m_small - 1]);
     R->utold = R->ut;
     R->old_tilt = R->tilt;
     R->t += (double)delt_main;
@@ -977,6 +996,10 @@ int g3m3_run_graph_steps(void* run, int nsteps) {
     }
     // this step's dt and the run time, as the host path would use them
     const float dt0 = (float)R->t_stepnew[0];
+    if (!(std::isfinite(dt0) && dt0 >= kTStepMin)) {
+        fprintf(stderr, "g3m3 run: time step %g below t_step_min or not finite\n", (double)dt0);
+        return R->err = kErrStepCollapsed;
+    }
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, R->s, R->dt_dev, R->t_dev, dt0, R->t);
     for (int q = 0; q < nsteps; ++q) {
         if (!ok_hip(R, hipGraphLaunch(R->gexec[(int)R->rot[0]], R->s), "graph launch")) break;
@@ -994,7 +1017,7 @@ int g3m3_run_graph_finish(void* run) {
     double* h = (double*)R->host_buf;
     if (ok_hip(R, hipMemcpyAsync(h, R->t_dev, sizeof(double), hipMemcpyDeviceToHost, R->s),
                "graph finish") &&
-        ok_hip(R, hipMemcpyAsync(h + 1, R->dt_dev, 3 * sizeof(float), hipMemcpyDeviceToHost,
+        ok_hip(R, hipMemcpyAsync(h + 1, R->dt_dev, 4 * sizeof(float), hipMemcpyDeviceToHost,
                                  R->s),
                "graph finish") &&
         ok_hip(R, hipStreamSynchronize(R->s), "graph finish")) {
@@ -1002,6 +1025,11 @@ int g3m3_run_graph_finish(void* run) {
         const float* f = (const float*)(h + 1);
         R->t_stepnew[0] = (double)f[1];
         R->t_step[0] = (double)f[2];
+        if (f[3] != 0.0f && !R->err) {
+            fprintf(stderr, "g3m3 run: a captured step formed a time step below t_step_min "
+                            "or not finite\n");
+            R->err = kErrStepCollapsed;
+        }
     }
     check(R, "graph_finish");
     return R->err;

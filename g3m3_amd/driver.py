@@ -187,6 +187,8 @@ class Run:
         self._bounds("wrk", box, "wrk", "", tg)
         self._validate("wrk", box)
         # step 2
+        if s.get("tilting"):
+            self._tilt(box, delt)
         self._efield("wrk")
         self._push("", "old", "wrk", delt)
         tg = clamp(self.t_old[b] + float(delt)) if box < self.ngrd else None
@@ -266,6 +268,12 @@ class Run:
                 ts[b] = ts[b - 1] * nsteps
         if dt_override is not None:
             ts[:] = dt_override
+        # the main loop stops on a step below t_step_min = 1e-6
+        # (code/3d_main.f90:116, 1969-1973); a step that is not finite too
+        dm = F(ts[n - 1])
+        if not (np.isfinite(dm) and dm >= F(1.e-6)) or not (np.isfinite(ts[m_small - 1])
+                                                             and ts[m_small - 1] > 0):
+            raise _lib.G3M3Error(f"time step {float(dm)} below t_step_min 1e-6 or not finite")
         m_step = int(ts[n - 1] / ts[m_small - 1])
         # the main loop's clock (code/3d_main.f90:1966-1977): delt single,
         # t and ut double, tilt single
@@ -290,13 +298,15 @@ class Run:
             self.call("set_rho", sig.set_rho(""))
         _lib.check("full_step")
 
-    def _tilt(self, box, delt2):
-        """code/3d_main.f90:2173-2187: the box's b0 recomputed for the tilt
-        and the planet's rotation at its half step (t_old and utold double,
-        the rest single; sin / cos are the C library's sinf / cosf)."""
+    def _tilt(self, box, dt_off):
+        """The box's b0 recomputed for the tilt and the planet's rotation at
+        t_old + dt_off: the half step before step 1 (code/3d_main.f90:
+        2173-2187), the full step before step 2 (:2465-2478).  t_old and
+        utold are double, the rest single; sin / cos are the C library's
+        sinf / cosf."""
         s, a = self.s, self.a
         pi = F(np.pi)   # 4*atan(1.0) in single precision
-        tb = float(self.t_old[box - 1]) + float(F(delt2))
+        tb = float(self.t_old[box - 1]) + float(F(dt_off))
         atilt = F(float(self.old_tilt) + tb * float(F(s["dtilt"])))
         arg = F(F(atilt * pi) / F(180.0))
         sin_tilt, cos_tilt = _LIBM.sinf(float(arg)), _LIBM.cosf(float(arg))

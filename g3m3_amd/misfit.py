@@ -46,14 +46,32 @@ class MisfitConfig:
     r_max: float = 20.0
 
 
-def craft_positions(cfg: MisfitConfig, re_equiv: float) -> np.ndarray:
-    """(ncraft, 3) positions in grid units: r ~ U[r_min, r_max] planet radii,
-    direction uniform on the sphere (a planet radius is 1/re_equiv units)."""
+def craft_positions(cfg: MisfitConfig, wl) -> np.ndarray:
+    """(ncraft, 3) positions in planet radii, the unit findgrid takes
+    (code/cft_findgrid.f90 divides by re_equiv to reach grid units): r ~
+    U[r_min, r_max], direction uniform on the sphere.  A draw that does not
+    lie inside the outer box at least two of its cells from every face is
+    drawn again, so every craft samples the field inside a box."""
+    g = wl.grid
+    b = cfg.ngrd - 1
+    re = float(wl.scal["re_equiv"])
+    margin = 2.0 * float(g["xspac"][b])
+    lo = np.array([g["grd_xmin"][b], g["grd_ymin"][b], g["grd_zmin"][b]], np.float64) + margin
+    hi = np.array([g["grd_xmax"][b], g["grd_ymax"][b], g["grd_zmax"][b]], np.float64) - margin
     rng = np.random.default_rng(cfg.seed)
-    d = rng.normal(size=(cfg.ncraft, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    r = rng.uniform(cfg.r_min, cfg.r_max, cfg.ncraft) / re_equiv
-    return np.ascontiguousarray((d * r[:, None]).astype(F))
+    out = []
+    for _ in range(10000 * cfg.ncraft):
+        if len(out) == cfg.ncraft:
+            break
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        p = d * rng.uniform(cfg.r_min, cfg.r_max)
+        u = p / re
+        if np.all(u >= lo) and np.all(u <= hi):
+            out.append(p)
+    if len(out) < cfg.ncraft:
+        raise ValueError("no craft position in [r_min, r_max] fits inside the outer box")
+    return np.ascontiguousarray(np.array(out).astype(F))
 
 
 def workload_for(theta, cfg: MisfitConfig):
@@ -70,7 +88,7 @@ class _Sampler:
     def __init__(self, run, wl, cfg: MisfitConfig):
         import torch
         self.run, self.wl, self.cfg = run, wl, cfg
-        self.pos = torch.from_numpy(craft_positions(cfg, float(wl.scal["re_equiv"]))).cuda()
+        self.pos = torch.from_numpy(craft_positions(cfg, wl)).cuda()
         nq = len(self.NAMES)
         self.out = torch.zeros((cfg.ncraft, nq), dtype=torch.float32, device="cuda")
         self.gp = torch.zeros((cfg.ncraft, 4), dtype=torch.int32, device="cuda")

@@ -99,11 +99,13 @@ class OracleRun:
             self.lib.call(name, *args)
 
     # -- pieces of the box-step ------------------------------------------------
-    def _tilt(self, box, delt2):
-        """The box's b0 for the tilt and rotation at its half step."""
+    def _tilt(self, box, dt_off):
+        """The box's b0 for the tilt and rotation at t_old + dt_off: the half
+        step before step 1 (code/3d_main.f90:2173-2187), the full step
+        before step 2 (:2465-2478)."""
         s, a = self.s, self.a
         sin_tilt, cos_tilt, rot_angle = O.tilt_rotation(
-            self.old_tilt, self.utold, a["t_old"][box - 1], delt2, s["dtilt"], s["t_equiv"],
+            self.old_tilt, self.utold, a["t_old"][box - 1], dt_off, s["dtilt"], s["t_equiv"],
             s["planet_per"])
         rot = dict(rot_angle=rot_angle, xdip=s["xdip"], ydip=s["ydip"], zdip=s["zdip"],
                    sin_tilt=sin_tilt, cos_tilt=cos_tilt, b0=s["b0"])
@@ -220,7 +222,9 @@ class OracleRun:
             self._set_t(tg)
         self._bounds("wrk", box, "wrk", "")
         self._validate("wrk", box)
-        # step 2 (:2476-2684)
+        # step 2 (:2465-2684)
+        if s.get("tilting"):                          # :2465-2478
+            self._tilt(box, delt)
         self._efield("wrk", box)
         self._push("", "old", "wrk", delt)
         if box < self.ngrd:
@@ -307,6 +311,12 @@ class OracleRun:
                 ts[b] = ts[b - 1] * nsteps
         if dt_override is not None:
             ts[:] = dt_override
+        # the main loop stops on a step below t_step_min = 1e-6
+        # (code/3d_main.f90:116, 1969-1973); a step that is not finite too
+        dm = F(ts[n - 1])
+        if not (np.isfinite(dm) and dm >= F(1.e-6)) or not (np.isfinite(ts[m_small - 1])
+                                                             and ts[m_small - 1] > 0):
+            raise FloatingPointError(f"time step {float(dm)} below t_step_min 1e-6 or not finite")
         m_step = int(ts[n - 1] / ts[m_small - 1])
         delt = F(ts[n - 1])
         self.dt_log.append(ts.copy())

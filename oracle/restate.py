@@ -1384,13 +1384,14 @@ def _box_axes(nx, ny, nz, x0, x1, y0, y1, z0, z1):
     return ax, ay, az
 
 
-def tilt_rotation(old_tilt, utold, t_old_box, delt2, dtilt, t_equiv, planet_per):
-    """code/3d_main.f90:2173-2180: the dipole's tilt and the planet's rotation
-    angle at a box's half step, in the reference's mixed precision (t_old,
+def tilt_rotation(old_tilt, utold, t_old_box, dt_off, dtilt, t_equiv, planet_per):
+    """The dipole's tilt and the planet's rotation angle at t_old + dt_off:
+    the half step before step 1 (code/3d_main.f90:2173-2180) or the full step
+    before step 2 (:2465-2472), in the reference's mixed precision (t_old,
     utold double; everything else single; pi = 4*atan(1.0) in single).
     Returns (sin_tilt, cos_tilt, rot_angle) as float32."""
     pi = F(np.pi)   # 4*atan(1.0) in single precision, 3.14159274
-    tb = float(t_old_box) + float(F(delt2))
+    tb = float(t_old_box) + float(F(dt_off))
     atilt = F(float(F(old_tilt)) + tb * float(F(dtilt)))
     arg = F(F(atilt * pi) / F(180.0))
     sin_tilt, cos_tilt = _sinf(arg), _cosf(arg)

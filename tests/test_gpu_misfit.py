@@ -28,3 +28,38 @@ def test_misfit_forward_loss_gradient():
     assert np.all(np.isfinite(g)) and np.any(g != 0.0) and secs > 0
     g2, _ = M.gradient_fd(theta, target, cfg, together=False)
     assert np.array_equal(g, g2)     # runs stepped together equal runs alone
+
+
+def test_craft_samples_equal_oracle():
+    """After two full steps, the device's craft samples (findgrid, aux cube
+    and trilin_interp of b and the three densities) equal the oracle's on
+    the same state bit for bit; every craft lies inside a box (gridpt box
+    >= 1, found rather than clamped) and each sampled density lies within
+    the range of that box's field."""
+    from common import same_bits
+    from g3m3_amd import misfit as M
+    from g3m3_amd.driver import Batch
+    from oracle import restate as O
+    cfg = M.MisfitConfig(steps=2, every=2)
+    wl = M.workload_for(M.THETA0, cfg)
+    print("craft samples after 2 steps of 3 x 81x81x41", flush=True)
+    batch = Batch([wl], streams=1)
+    batch.init_dt()
+    for _ in range(2):
+        batch.full_step()
+    run = batch.runs[0]
+    sm = M._Sampler(run, wl, cfg)
+    sm.sample()
+    got, gp = sm.out.cpu().numpy(), sm.gp.cpu().numpy()
+    st = run.state_host()
+    pos = M.craft_positions(cfg, wl)
+    want, wgp = O.sample_fields(pos, cfg.ngrd, sm.mn, sm.mx, sm.xs,
+                                np.float32(wl.scal["re_equiv"]), [st[k] for k in sm.NAMES])
+    assert np.all(gp[:, 3] >= 1) and np.all(gp[:, 0] >= 1), gp
+    assert np.array_equal(gp, wgp)
+    assert same_bits(got, want)
+    for c in range(cfg.ncraft):
+        b = gp[c, 3] - 1
+        for q, k in enumerate(sm.NAMES[3:], start=3):
+            f = st[k][b]
+            assert f.min() <= got[c, q] <= f.max(), (c, k)

@@ -309,3 +309,23 @@ def test_tilting_against_oracle():
         gh = g.state_host()
         errs = {k: rel_l2(gh[k], o.a[k]) for k in FIELDS}
         assert max(errs.values()) <= TOL, errs
+
+
+@pytest.mark.parametrize("native", [True, False])
+def test_collapsed_step_stops_the_run(native):
+    """A time step below t_step_min = 1e-6, or one that is not finite, stops
+    the run with an error before any box-step, as the main loop stops
+    (code/3d_main.f90:1969-1973); the state is unchanged."""
+    from g3m3_amd import _lib
+    from g3m3_amd.driver import FastRun, Run
+    wl = make_workload("tiny")
+    for dt in (1e-7, float("nan")):
+        g = (FastRun if native else Run)(wl)
+        g.init_dt()
+        before = g.state_host()
+        print(f"collapsed step dt={dt} native={native}", flush=True)
+        with pytest.raises(_lib.G3M3Error):
+            g.full_step(dt_override=[dt] * wl.ngrd)
+        after = Run.state_host(g)     # the arrays as they stand (no sync on a failed run)
+        assert g.box_steps == 0
+        assert all(np.array_equal(before[k], after[k]) for k in FIELDS)

@@ -9,13 +9,38 @@ from g3m3_amd import misfit as M
 
 def test_craft_positions_in_shell_and_seeded():
     cfg = M.MisfitConfig()
-    re = 0.2
-    p = M.craft_positions(cfg, re)
+    wl = M.workload_for(M.THETA0, cfg)
+    p = M.craft_positions(cfg, wl)
     assert p.shape == (cfg.ncraft, 3) and p.dtype == np.float32
-    r = np.linalg.norm(p.astype(np.float64), axis=1) * re   # planet radii
+    r = np.linalg.norm(p.astype(np.float64), axis=1)   # planet radii
     assert np.all(r >= cfg.r_min * (1 - 1e-6)) and np.all(r <= cfg.r_max * (1 + 1e-6))
-    assert np.array_equal(p, M.craft_positions(cfg, re))
-    assert not np.array_equal(p, M.craft_positions(M.MisfitConfig(seed=cfg.seed + 1), re))
+    assert np.array_equal(p, M.craft_positions(cfg, wl))
+    assert not np.array_equal(p, M.craft_positions(M.MisfitConfig(seed=cfg.seed + 1), wl))
+
+
+def test_craft_land_inside_a_box():
+    """Each craft, placed by the reference's findgrid (which divides by
+    re_equiv), falls inside a box, away from its faces, with trilinear
+    weights in [0, 1]."""
+    from oracle import restate as O
+    cfg = M.MisfitConfig()
+    wl = M.workload_for(M.THETA0, cfg)
+    g = wl.grid
+    mn = np.stack([g["grd_xmin"], g["grd_ymin"], g["grd_zmin"]], 1).astype(np.float32)
+    mx = np.stack([g["grd_xmax"], g["grd_ymax"], g["grd_zmax"]], 1).astype(np.float32)
+    xs = np.asarray(g["xspac"], np.float32)
+    re = np.float32(wl.scal["re_equiv"])
+    n = (wl.nx, wl.ny, wl.nz)
+    for p in M.craft_positions(cfg, wl):
+        gp = np.zeros(4, np.int32)
+        O.findgrid(p, cfg.ngrd, mn, mx, xs, gp, re)
+        b = int(gp[3]) - 1
+        xa = p / re
+        assert np.all(xa > mn[b]) and np.all(xa < mx[b]), p      # found, not clamped
+        assert all(1 <= gp[a] <= n[a] - 1 for a in range(3)), gp
+        cv = O.aux_cube(gp, mn, xs, re)
+        w = (p - cv[0]) / (cv[1] - cv[0])
+        assert np.all(w >= 0.0) and np.all(w <= 1.0), (p, w)
 
 
 def test_loss():
