@@ -63,6 +63,14 @@ struct Run {
     std::vector<char> rot;
     int vpass = 0;                // which validation counter the next pass uses
     std::vector<int> stepped;     // boxes stepped by the last issued full step
+    // the box-steps of the last issued full step in the order taken: box,
+    // t_old and t_new after the step's clock update (the speeds.dat record
+    // of code/3d_main.f90:2167-2168)
+    struct StepRec {
+        int box;
+        double t_old, t_new;
+    };
+    std::vector<StepRec> step_log;
     bool lists_disjoint = false;  // the planet lists share no cell: one launch for all three
     // flank shell tables of the inner boxes (geometry is fixed for a run)
     ShellPt* shell[G3M3_MAX_GRIDS] = {};
@@ -589,6 +597,7 @@ void box_step(Run* R, int box) {
     const int b = box - 1;
     R->t_old[b] = R->t_new[b];
     R->t_new[b] = R->t_old[b] + R->t_step[b];
+    R->step_log.push_back(Run::StepRec{box, R->t_old[b], R->t_new[b]});
     const float delt = (float)R->t_step[b];
     const float delt2 = delt / 2.0f;
     if (d.tilting) tilt_box(R, box, delt2);
@@ -943,6 +952,7 @@ int g3m3_run_full_step_async(void* run, const double* dt_override) {
         return R->err;
     }
     const int m_step = (int)(ts[n - 1] / ts[m_small - 1]);
+    R->step_log.clear();
     R->utold = R->ut;
     R->old_tilt = R->tilt;
     R->t += (double)delt_main;
@@ -1106,6 +1116,17 @@ int g3m3_run_box_step(void* run, int box) {
     all_coarse_changed(R);   // the caller may have written the arrays
     box_step(R, box);
     return R->err;
+}
+
+int g3m3_run_step_log(void* run, int cap, int* box, double* t_old, double* t_new) {
+    const Run* R = (const Run*)run;
+    const int n = (int)R->step_log.size();
+    for (int q = 0; q < n && q < cap; ++q) {
+        if (box) box[q] = R->step_log[q].box;
+        if (t_old) t_old[q] = R->step_log[q].t_old;
+        if (t_new) t_new[q] = R->step_log[q].t_new;
+    }
+    return n;
 }
 
 void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,

@@ -459,6 +459,18 @@ class FastRun(Run):
             raise _lib.G3M3Error("g3m3_run_full_step failed")
         self._pull()
 
+    def step_log(self):
+        """The box-steps the last full step took, in order: (box, t_old,
+        t_new) after each, as speeds.dat records them."""
+        L = self._L
+        L.g3m3_run_step_log.restype = _ct.c_int
+        n = L.g3m3_run_step_log(_ct.c_void_p(self.h), 0, None, None, None)
+        box = (_ct.c_int * max(n, 1))()
+        to = (_ct.c_double * max(n, 1))()
+        tn = (_ct.c_double * max(n, 1))()
+        L.g3m3_run_step_log(_ct.c_void_p(self.h), n, box, to, tn)
+        return [(int(box[q]), float(to[q]), float(tn[q])) for q in range(n)]
+
     def set_stream(self, stream):
         """Issue this run's kernels on `stream` (a torch.cuda.Stream)."""
         self._L.g3m3_run_set_stream(_ct.c_void_p(self.h), _ct.c_void_p(stream.cuda_stream))

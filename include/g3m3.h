@@ -656,6 +656,10 @@ void g3m3_run_set_time(void* run, double t);          /* after a restart read */
 int g3m3_run_sync_arrays(void* run);
 void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, double* t_old,
                    double* t_new, long long* cell_updates, long long* box_steps);
+/* The box-steps the last full step took, in order: box number, t_old and
+ * t_new of that box after its step (what code/3d_main.f90:2167-2168 writes
+ * to speeds.dat).  Fills at most cap entries; returns how many there were. */
+int g3m3_run_step_log(void* run, int cap, int* box, double* t_old, double* t_new);
 
 /* ---- Fortran implicit-interface names ------------------------------------------
  * A Fortran call with no interface block (the reference's own style) names

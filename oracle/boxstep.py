@@ -80,6 +80,7 @@ class OracleRun:
         self.cell_updates = 0
         self.box_steps = 0
         self.dt_log = []
+        self.step_log = []    # (box, t_old, t_new) per box-step, as speeds.dat
 
     # -- plumbing ------------------------------------------------------------
     def args(self, names):
@@ -204,6 +205,7 @@ class OracleRun:
         b = box - 1
         a["t_old"][b] = a["t_new"][b]
         a["t_new"][b] = a["t_old"][b] + self.t_step[b]
+        self.step_log.append((box, float(a["t_old"][b]), float(a["t_new"][b])))
         delt = F(self.t_step[b])
         delt2 = delt / F(2.)
         s["box"] = box

@@ -1325,9 +1325,26 @@ def _cosf(x):
     return F(_LIBM.cosf(float(F(x))))
 
 
+# How the initialisation's real power and the torus's exponentials are
+# evaluated: "libm" calls the C library's powf / expf, as the compiled
+# reference does; "double" evaluates them in double precision and rounds
+# once to single, as the device kernels do.  The two differ in the last bit
+# for a small fraction of arguments; everything else is unaffected.
+TRANSCENDENTALS = "libm"
+
+
+def set_transcendentals(kind: str) -> None:
+    global TRANSCENDENTALS
+    if kind not in ("libm", "double"):
+        raise ValueError(kind)
+    TRANSCENDENTALS = kind
+
+
 def _powf(a, e):
-    """Elementwise C powf of a float32 array (small grids only)."""
+    """Elementwise powf of a float32 array (small grids only)."""
     a = np.asarray(a, F)
+    if TRANSCENDENTALS == "double":
+        return np.power(a.astype(np.float64), float(F(e))).astype(F)
     out = np.fromiter((_LIBM.powf(float(v), float(F(e))) for v in a.ravel()), F, a.size)
     return out.reshape(a.shape)
 
@@ -1613,6 +1630,8 @@ _LIBM.expf.argtypes = [_ct.c_float]
 
 def _expf(a):
     a = np.asarray(a, F)
+    if TRANSCENDENTALS == "double":
+        return np.exp(a.astype(np.float64)).astype(F)
     out = np.fromiter((_LIBM.expf(float(v)) for v in a.ravel()), F, a.size)
     return out.reshape(a.shape)
 
