@@ -1260,12 +1260,14 @@ def sample_fields(sxyz, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv, fi
     return out, gps
 
 
-def divB_correct_f64(bx, by, bz, rx):
+def divB_correct_f64(bx, by, bz, rx, method="direct", rtol=1e-13):
     """All-FP64 divB_correct (code/sub_divB_correct.f90:130-172) of one box
-    slice (nz, ny, nx): the central-difference right side, an exact sparse
-    solve of the same 7-band system (scipy), phi = rx^2 * x and B -= grad
-    phi on the interior, all in double.  Returns the three corrected
-    fields (float64)."""
+    slice (nz, ny, nx): the central-difference right side, a solve of the
+    same 7-band system in double -- exact (scipy's sparse direct solve,
+    small grids) or by conjugate gradients to relative residual ``rtol``
+    (method "cg"; the system is symmetric negative definite) -- then phi =
+    rx^2 * x and B -= grad phi on the interior, all in double.  Returns the
+    three corrected fields (float64)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     nz, ny, nx = bx.shape
@@ -1290,7 +1292,13 @@ def divB_correct_f64(bx, by, bz, rx):
     rhs[1:-1, 1:-1, 1:-1] = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2])
                              + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
                              + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / (2.0 * rx)
-    phi = (spl.spsolve(A.tocsc(), rhs.ravel()) * rx * rx).reshape(nz, ny, nx)
+    if method == "direct":
+        x = spl.spsolve(A.tocsc(), rhs.ravel())
+    else:
+        x, info = spl.cg(-A, -rhs.ravel(), rtol=rtol, atol=0.0, maxiter=20 * (nx + ny + nz))
+        if info != 0:
+            raise RuntimeError(f"FP64 CG did not converge ({info})")
+    phi = (x * rx * rx).reshape(nz, ny, nx)
     out = [X.copy(), Y.copy(), Z.copy()]
     out[0][1:-1, 1:-1, 1:-1] -= (phi[1:-1, 1:-1, 2:] - phi[1:-1, 1:-1, :-2]) / (2.0 * rx)
     out[1][1:-1, 1:-1, 1:-1] -= (phi[1:-1, 2:, 1:-1] - phi[1:-1, :-2, 1:-1]) / (2.0 * rx)

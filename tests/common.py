@@ -219,11 +219,16 @@ def torus_oracle(c):
 
 
 def same_bits(a, b) -> bool:
-    """True when two float32 arrays hold the same bit patterns (so -0.0 and
-    0.0 differ, and NaNs compare by payload)."""
-    a = np.ascontiguousarray(a, dtype=F)
-    b = np.ascontiguousarray(b, dtype=F)
-    return a.shape == b.shape and bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+    """True when two arrays hold the same bits: float32 arrays compared as
+    uint32 views (so -0.0 and 0.0 differ, and NaNs compare by payload),
+    any other dtype element for element with the dtypes equal."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype == F and b.dtype == F:
+        return bool(np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                                   np.ascontiguousarray(b).view(np.uint32)))
+    return a.dtype == b.dtype and bool(np.array_equal(a, b))
 
 
 def bit_diffs(a, b) -> int:

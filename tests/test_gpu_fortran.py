@@ -9,6 +9,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from common import same_bits
+
 from oracle import restate as O
 
 pytestmark = pytest.mark.gpu
@@ -90,5 +92,5 @@ def test_fortran_driver_fragment(tmp_path):
                 w["efldy"], w["efldz"], nx, ny, nz, ng, box, delt, rx, rx, rx)
     want = dict(w)
     want.update({k: a[k] for k in outs4})
-    bad = [k for k in outs3 + outs4 if not np.array_equal(got[k], want[k])]
+    bad = [k for k in outs3 + outs4 if not same_bits(got[k], want[k])]
     assert not bad, f"Fortran-driven results differ from the oracle on {bad}"

@@ -7,6 +7,8 @@ legitimately not bit-exact the tolerance is written in the test."""
 import numpy as np
 import pytest
 
+from common import same_bits
+
 from common import (World, push_ion_names, set_rho_names, set_speed_names, bndry_outer_names,
                     bndry_inner_names, lap_plasma_names, flux_correct_names, flanks_names,
                     corer_names, GRID)
@@ -63,7 +65,7 @@ def assert_bits(wg, wo, keys, tol=0.0):
     for k in keys:
         a, b = wg.a[k], wo.a[k]
         if tol == 0.0:
-            ok = np.array_equal(a, b, equal_nan=True)
+            ok = same_bits(a, b)
         else:
             sc = max(float(np.abs(b).max()), 1e-30)
             ok = float(np.abs(a.astype(np.float64) - b).max()) <= tol * sc
@@ -264,7 +266,7 @@ def test_bndry_flanks(dev, nested, box, tgt, src):
     O.bndry_flanks(*wo.args(flanks_names(tgt, src, "old")))
     for k in sig.FLANK_ORDER:
         got = d[tgt + k].cpu().numpy()
-        assert np.array_equal(got, wo.a[tgt + k]), k
+        assert same_bits(got, wo.a[tgt + k]), k
 
 
 @pytest.mark.parametrize("box", [1, 2])
@@ -280,7 +282,7 @@ def test_bndry_corer(dev, nested, box):
     wo.s["box"] = box
     O.bndry_corer(*wo.args(corer_names("")))
     for k in sig.CORER_ORDER:
-        assert np.array_equal(d[k].cpu().numpy(), wo.a[k]), k
+        assert same_bits(d[k].cpu().numpy(), wo.a[k]), k
 
 
 @pytest.mark.parametrize("box", [1, 2])
@@ -359,7 +361,7 @@ def test_findgrid_host(dev):
         name = ctypes.create_string_buffer(b"craft01 ", 8)
         _lib.call("findgrid", p, ngrd, mn, mx, xs, a, re, name, _lib.ByValue(ctypes.c_size_t(8)))
         O.findgrid(p, ngrd, mn, mx, xs, b, re)
-        assert np.array_equal(a, b), (p, a, b)
+        assert same_bits(a, b), (p, a, b)
 
 
 def test_sample_fields(dev):
@@ -389,8 +391,8 @@ def test_sample_fields(dev):
               out, gp)
     _lib.sync()
     _lib.check("sample_fields")
-    assert np.array_equal(gp.cpu().numpy(), wgp)
-    assert np.array_equal(out.cpu().numpy(), want)
+    assert same_bits(gp.cpu().numpy(), wgp)
+    assert same_bits(out.cpu().numpy(), want)
 
 
 @pytest.mark.parametrize("errlim", [1e-4, 1e-6])
@@ -526,7 +528,7 @@ def test_find_aux_cube_host(dev):
                                 cv)
             res.append((gp, s, cv, rec))
         for u, v in zip(*res):
-            assert np.array_equal(u, v), (q, res)
+            assert same_bits(u, v), (q, res)
 
 
 def test_sample_craft(dev):
@@ -561,9 +563,9 @@ def test_sample_craft(dev):
     _lib.call("sample_craft", nc, dx, dz, ut, ngrd, mn, mx, xs, re, nx, ny, nz, len(fields),
               parr, s, out, gp)
     _lib.sync()
-    assert np.array_equal(s.cpu().numpy(), ws)
-    assert np.array_equal(gp.cpu().numpy(), wgp)
-    assert np.array_equal(out.cpu().numpy(), wsc)
+    assert same_bits(s.cpu().numpy(), ws)
+    assert same_bits(gp.cpu().numpy(), wgp)
+    assert same_bits(out.cpu().numpy(), wsc)
     assert (wgp[:, 0] > 0).sum() > 200
 
 

@@ -11,6 +11,8 @@ does not reach any field and everything is bit-exact."""
 import numpy as np
 import pytest
 
+from common import same_bits
+
 from common import FIELDS, INIT_SCALARS, World, init_case, init_oracle
 from oracle import restate as O
 
@@ -62,7 +64,7 @@ def test_mak_dip_grd(dev, box):
               *[F(ROT[k]) for k in ROT_ARGS])
     _lib.sync()
     for g, o in zip(got, want):
-        assert np.array_equal(g.cpu().numpy(), o)
+        assert same_bits(g.cpu().numpy(), o)
 
 
 def test_set_resist(dev):
@@ -87,7 +89,7 @@ def test_set_resist(dev):
     _lib.call("set_resist", got, nx, ny, nz, mb, F(64.0), d[0], cnt[0], d[1], cnt[1], d[2],
               cnt[2], 12, 9, 6, F(1.0))
     _lib.sync()
-    assert np.array_equal(got.cpu().numpy(), want)
+    assert same_bits(got.cpu().numpy(), want)
 
 
 def run_init(dev, c, check=True):
@@ -132,7 +134,7 @@ def test_init_plasma_bitexact(dev, shape):
     c = init_case(*shape, erho=0.0, mcap=2000)
     got = run_init(dev, c)
     want = oracle_out(init_oracle(c)[0])
-    bad = [k for k in want if not np.array_equal(got[k], want[k])]
+    bad = [k for k in want if not same_bits(got[k], want[k])]
     assert not bad, f"differs from the oracle: {bad}"
     assert got["numsrf"][0] > 0 and got["nummid"][0] > 0 and got["numzero"][0] > 0
 
@@ -146,7 +148,7 @@ def test_init_plasma_ionosphere(dev):
     exact = ["bx0", "by0", "bz0", "rst", "ijsrf", "ijmid", "ijzero", "numsrf", "nummid",
              "numzero"]
     for k in exact:
-        assert np.array_equal(got[k], want[k]), k
+        assert same_bits(got[k], want[k]), k
     for k in want:
         if k in exact:
             continue
@@ -166,7 +168,7 @@ def test_init_plasma_list_capacity(dev):
         _lib.check("init_plasma")
     assert got["numsrf"][0] == 10
     want = init_oracle(init_case(mcap=2000))[0]["lists"]
-    assert np.array_equal(got["ijsrf"][:10], want["ijsrf"][:10])
+    assert same_bits(got["ijsrf"][:10], want["ijsrf"][:10])
 
 
 @pytest.mark.parametrize("tilt", [0.0, 0.2])
@@ -189,7 +191,7 @@ def test_torus_inject(dev, tilt):
     for k in FIELDS:
         got, want = f[k].cpu().numpy(), d["f"][k]
         same = c["f"][k] == want            # cells the source did not touch
-        assert np.array_equal(got[same], want[same]), k
+        assert same_bits(got[same], want[same]), k
         a, b = got.astype(np.float64), want.astype(np.float64)
         assert np.all(np.abs(a - b) <= 5e-7 * np.abs(b) + 1e-30), k
     for g, w in zip(tot, want_tot):

@@ -5,6 +5,8 @@ see non-zero fields, and the gradient is finite and repeats."""
 import numpy as np
 import pytest
 
+from common import same_bits
+
 pytestmark = pytest.mark.gpu
 
 
@@ -16,7 +18,7 @@ def test_misfit_forward_loss_gradient():
     s1, _ = M.forward(theta, cfg)
     s2, _ = M.forward(theta, cfg)
     assert s1.shape == (2, cfg.ncraft, 4)
-    assert np.array_equal(s1, s2)
+    assert same_bits(s1, s2)
     assert M.loss(s1, s2) == 0.0
     assert (np.abs(s1[..., 3]) > 0).sum() >= cfg.ncraft   # densities seen
     target_theta = theta * np.array([1.0, 1.0, 1.2, 1.1, 1.05, 1.0]) + \
@@ -27,7 +29,7 @@ def test_misfit_forward_loss_gradient():
     g, secs = M.gradient_fd(theta, target, cfg)
     assert np.all(np.isfinite(g)) and np.any(g != 0.0) and secs > 0
     g2, _ = M.gradient_fd(theta, target, cfg, together=False)
-    assert np.array_equal(g, g2)     # runs stepped together equal runs alone
+    assert same_bits(g, g2)     # runs stepped together equal runs alone
 
 
 def test_craft_samples_equal_oracle():
@@ -36,7 +38,6 @@ def test_craft_samples_equal_oracle():
     the same state bit for bit; every craft lies inside a box (gridpt box
     >= 1, found rather than clamped) and each sampled density lies within
     the range of that box's field."""
-    from common import same_bits
     from g3m3_amd import misfit as M
     from g3m3_amd.driver import Batch
     from oracle import restate as O
@@ -56,7 +57,7 @@ def test_craft_samples_equal_oracle():
     want, wgp = O.sample_fields(pos, cfg.ngrd, sm.mn, sm.mx, sm.xs,
                                 np.float32(wl.scal["re_equiv"]), [st[k] for k in sm.NAMES])
     assert np.all(gp[:, 3] >= 1) and np.all(gp[:, 0] >= 1), gp
-    assert np.array_equal(gp, wgp)
+    assert same_bits(gp, wgp)
     assert same_bits(got, want)
     for c in range(cfg.ncraft):
         b = gp[c, 3] - 1
