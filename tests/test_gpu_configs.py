@@ -1,0 +1,128 @@
+"""The benchmark's configurations at their full sizes on the GPU.
+
+* Five nested 121x121x61 boxes with resistivity 1e-3 (the benchmark
+  workload): two full steps with the run's own time steps, against the
+  oracle's run of the same workload (tests/golden/oracle_nested121.json,
+  made by tests/golden/make_oracle_digests.py with the reference's compiled
+  subroutines): time steps equal, every field bit for bit.
+* 32 independent single-grid runs of 41x41x21 .. 121x121x61 (the batching
+  workload): two full steps replayed from captured graphs, all runs
+  together, each equal bit for bit to the same run stepped alone from the
+  host; the smallest and the largest run's first step bit for bit against
+  the oracle (tests/golden/oracle_batch_ends.json).
+* divB on the 241x241x121 dipole + 1% noise field: mode 3 (CG with FP32
+  vectors, FP64 dots, inside FP64 residual refinement) to relative residual
+  1e-8, against an all-FP64 solve of the same system (tests/fp64ref.py, CG to
+  1e-13): the achieved relative residual at most 1e-8 and each corrected
+  component within 1e-6 of the largest |b| (the float32 rounding of the
+  result is about 6e-8 of a value)."""
+import base64
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from g3m3_amd.workload import FIELDS
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def _check_fields(h, digest, dims, where):
+    from mainpin import sample_index
+    idx = sample_index(*dims) % (dims[0] * dims[1] * dims[2] * dims[3])
+    bad = []
+    for k in FIELDS:
+        x = np.ascontiguousarray(h[k], np.float32).ravel()
+        if hashlib.sha256(x.tobytes()).hexdigest() != digest[k]["sha256"]:
+            want = np.frombuffer(base64.b64decode(digest[k]["samples"]), np.float32)
+            nd = int((x[idx].view(np.uint32) != want.view(np.uint32)).sum())
+            bad.append(f"{k} ({nd} of {len(idx)} samples differ)")
+    assert not bad, f"{where}: not bit-equal to the oracle: {bad}"
+
+
+def _run_against(r, gold, label):
+    dims = tuple(gold["dims"])
+    assert list(r.t_stepnew) == gold["init_t_stepnew"]
+    for st, g in enumerate(gold["steps"], start=1):
+        print(f"{label}: step {st}", flush=True)
+        r.full_step()
+        assert list(r.t_step) == g["t_step"] and list(r.t_stepnew) == g["t_stepnew"], st
+        assert r.box_steps == g["box_steps"]
+        _check_fields(r.state_host(), g["fields"], dims, f"{label} step {st}")
+
+
+def test_nested121_against_oracle():
+    from g3m3_amd.driver import FastRun
+    from g3m3_amd.workload import make_workload
+    r = FastRun(make_workload("nested121"))
+    r.init_dt()
+    _run_against(r, _load("oracle_nested121.json"), "nested121")
+
+
+def test_batch_32_runs():
+    import torch
+    from g3m3_amd.driver import Batch, FastRun
+    from g3m3_amd.workload import make_batch_workloads
+    from common import same_bits
+    wls = make_batch_workloads(32)
+    print("32 runs, 2 captured steps together", flush=True)
+    batch = Batch(wls)
+    batch.init_dt()
+    batch.graph_steps(2)
+    got = [r.state_host() for r in batch.runs]
+    del batch
+    torch.cuda.empty_cache()
+    for i, wl in enumerate(wls):
+        alone = FastRun(wl)
+        alone.init_dt()
+        for _ in range(2):
+            alone.full_step()
+        h = alone.state_host()
+        bad = [k for k in FIELDS if not same_bits(h[k], got[i][k])]
+        assert not bad, f"run {i} ({wl.nx}x{wl.ny}x{wl.nz}): batched differs on {bad}"
+    ends = _load("oracle_batch_ends.json")
+    for label, g in ends.items():
+        r = FastRun(wls[g["index"]])
+        r.init_dt()
+        _run_against(r, g, f"batch {label} run {g['index']}")
+
+
+def test_divb_precision_241():
+    import ctypes
+    import torch
+    from fp64ref import divb_fp64
+    from g3m3_amd import _lib
+    from g3m3_amd.workload import precision_field
+    host = precision_field()
+    nz, ny, nx = host[0].shape[1:]
+    npts = nx * ny * nz
+    dev = [torch.from_numpy(h).cuda() for h in host]
+    print(f"divB mode 3 on {nx}x{ny}x{nz}", flush=True)
+    it, r2 = ctypes.c_longlong(0), ctypes.c_double(0)
+    st = _lib.call("divb_correct_ex", nx, ny, nz, 1, 1, np.array([1.0], np.float32),
+                   np.float32(1e-8 / np.sqrt(npts)), dev[0], dev[1], dev[2], 3,
+                   ctypes.c_longlong(0), it, r2)
+    _lib.check("divb mode 3")
+    assert st == 1, f"status {st} after {it.value} iterations"
+    print("FP64 reference solve", flush=True)
+    ref, ref_it = divb_fp64(*[torch.from_numpy(h[0]).cuda() for h in host], 1.0)
+    X, Y, Z = (torch.from_numpy(h[0]).cuda().double() for h in host)
+    rhs = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2]) + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
+           + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / 2.0
+    rel_res = float(np.sqrt(max(r2.value, 0.0) / float((rhs * rhs).sum())))
+    scale = max(float(r.abs().max()) for r in ref)
+    errs = {k: float((d[0].double() - r).abs().max()) / scale
+            for k, d, r in zip(("bx", "by", "bz"), dev, ref)}
+    print(f"mode 3: {it.value} iterations, relative residual {rel_res:.2e}; "
+          f"field error vs FP64 {errs} (FP64 CG {ref_it} iterations)", flush=True)
+    assert rel_res <= 1e-8
+    assert max(errs.values()) <= 1e-6, errs
