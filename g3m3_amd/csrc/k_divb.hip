@@ -399,6 +399,7 @@ struct DivbSolver {
 // search direction p = r + beta p is formed inside the next matrix product.
 
 enum { kCgRR = 0, kCgAlpha = 1, kCgBeta = 2, kCgB2 = 3, kCgErr2 = 8, kCgTmp = 9 };
+constexpr int kCgRestarts = 8;
 
 // p_new = r + beta p_old (r alone on the first iteration) at the 7 points,
 // q = M p_new, partial sums of p_new.q
@@ -500,19 +501,38 @@ __global__ void k_cg_start(const double* __restrict__ part, int nb, double* sc, 
     }
 }
 
-// CG solve of A soln = rhs from zero; work: x lives in soln, 4 more vectors
-// (r, p0, p1, q) in `vec`.  Returns 1 converged, 3 iteration cap.
+// Resuming from the current solution: r (= A soln - rhs, the residual of
+// M soln = c) is already formed and its partial sums are in part; |c|^2,
+// the tolerance and the iteration count carry on.
+__global__ void k_cg_resume(const double* __restrict__ part, int nb, double* sc, DivbCtl* ctl) {
+    const double rr = sum_parts(part, nb);
+    if (threadIdx.x == 0) {
+        sc[kCgRR] = rr;
+        ctl->status = (rr <= sc[kCgB2] * sc[kCgErr2] || rr == 0.0) ? 1 : 0;
+    }
+}
+
+// CG solve of A soln = rhs, from zero or (from_x) from the solution in soln;
+// work: x lives in soln, 4 more vectors (r, p0, p1, q) in `vec`.  Returns 1
+// when the recursive residual meets the criterion, 3 at the iteration cap;
+// iters_out is the count since the solve from zero began.
 static int cg_solve(const Box& B, const float* rhs, float* soln, float* vec, double* sc,
                     DivbCtl* ctl, double* part, double err2, long long max_iters,
-                    long long* iters_out, hipStream_t s) {
+                    long long* iters_out, bool from_x, hipStream_t s) {
     const int n = B.V();
     const int nb = (n + kRedThreads - 1) / kRedThreads;
     float* r = vec;
     float* p[2] = {vec + (size_t)n, vec + 2 * (size_t)n};
     float* q = vec + 3 * (size_t)n;
     const dim3 g(nb), t(kRedThreads);
-    hipLaunchKernelGGL(k_cg_init, g, t, 0, s, n, rhs, soln, r, part);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), t, 0, s, part, nb, sc, ctl, err2);
+    if (from_x) {
+        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, Bands{}, soln, r, rhs, part, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_cg_resume, dim3(1), t, 0, s, part, nb, sc, ctl);
+    } else {
+        hipLaunchKernelGGL(k_cg_init, g, t, 0, s, n, rhs, soln, r, part);
+        hipLaunchKernelGGL(k_cg_start, dim3(1), t, 0, s, part, nb, sc, ctl, err2);
+    }
     long long it = 0;
     int cur = 0;
     for (;;) {
@@ -631,7 +651,7 @@ static int refine_solve(const Box& B, const float* rhs, float* soln, double* xd,
         if (total >= max_iters) break;
         // inner solve to 1e-4 of the current residual (FP32 vectors)
         long long it = 0;
-        if (cg_solve(B, rf, d, vec, sc, ctl, part, 1e-8, max_iters - total, &it, s) ==
+        if (cg_solve(B, rf, d, vec, sc, ctl, part, 1e-8, max_iters - total, &it, false, s) ==
             kStatusHipFailed) {
             status = kStatusHipFailed;
             break;
@@ -688,15 +708,31 @@ int divb_correct(const Box& B, float* bx, float* by, float* bz, float rx, float 
         S.iters = it;
         S.residual2 = r2;
     } else if (mode == kDivbCG) {
-        // plain CG from zero; the true residual |A soln - b|^2 at the end
+        // plain CG from zero.  The recursive residual drifts from the true
+        // one in FP32 vectors, so convergence is decided on the true
+        // residual |A soln - b|^2: while it misses the criterion, CG resumes
+        // from the solution with the true residual (at most kCgRestarts
+        // times within the iteration cap; status 3 if it still misses).
+        const double target = (double)n * (double)errlim * (double)errlim;
         long long it = 0;
-        S.status = cg_solve(B, rhs, S.soln, work + 2 * (size_t)n, S.sc, S.ctl, S.part,
-                            (double)n * (double)errlim * (double)errlim, S.max_iters, &it, s);
+        float* vec = work + 2 * (size_t)n;
+        S.status = cg_solve(B, rhs, S.soln, vec, S.sc, S.ctl, S.part, target, S.max_iters, &it,
+                            false, s);
+        for (int restart = 0;; ++restart) {
+            hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, Bands{}, S.soln, S.lv[3].xi, rhs,
+                               S.part, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            hipLaunchKernelGGL(k_sum1, dim3(1), t, 0, s, S.part, nb, S.sc + kCgTmp);
+            S.residual2 = S.read_sc(kCgTmp);
+            const double b2 = S.read_sc(kCgB2);
+            if (S.status != 1 || S.residual2 <= b2 * target) break;
+            if (restart == kCgRestarts || it >= S.max_iters) {
+                S.status = 3;
+                break;
+            }
+            S.status = cg_solve(B, rhs, S.soln, vec, S.sc, S.ctl, S.part, target,
+                                S.max_iters - it, &it, true, s);
+        }
         S.iters = it;
-        hipLaunchKernelGGL(k_matvec, g, t, 0, s, B, Bands{}, S.soln, S.lv[3].xi, rhs, S.part,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-        hipLaunchKernelGGL(k_sum1, dim3(1), t, 0, s, S.part, nb, S.sc + kCgTmp);
-        S.residual2 = S.read_sc(kCgTmp);
     } else {
         // the reference's soln_vec is an uninitialised automatic array; start from 0
         hipLaunchKernelGGL(k_zero1, g, t, 0, s, n, S.soln);

@@ -588,3 +588,64 @@ def test_flux_counter(dev, world, shift):
     _lib.call("flux_counter", *[d[n] if n in d else world.s[n] for n in names[:18]], *res)
     _lib.check("flux_counter")
     assert [r[0] for r in res] == [wo.a[k][0] for k in outs]
+
+
+def test_divb_correct_reference_signature(dev):
+    """g3m3_divb_correct with the argument list of code/sub_divB_correct.f90:
+    25-27 (the reference's CGNR, single-precision scalars) on box 2 of two
+    nested boxes at errlim 1e-6 against the all-FP64 exact solve of the same
+    system: every corrected component within 5e-4 of the largest |b| (the
+    oracle's sequential-FP32 CGNR at the same errlim lands within 9.2e-5),
+    box 1 untouched; and the same solve through g3m3_divb_correct_ex reports
+    convergence with residual^2 <= |b|^2 npts errlim^2, the reference's
+    criterion (code/sub_sparse_solver.f90:53, 111)."""
+    import ctypes
+    from g3m3_amd import _lib
+    w = World(17, 15, 11, 2, seed=3, lists=False)
+    errlim = np.float32(1e-6)
+    pos = np.array([w.nx * w.ny, w.nx, 1, 0, -1, -w.nx, -w.nx * w.ny], np.int32)
+    d = to_dev(dev, w)
+    print("divb_correct (reference signature) on box 2 of 17x15x11x2", flush=True)
+    _lib.call("divb_correct", w.nx, w.ny, w.nz, 2, 2, 7, pos, w.a["xspac"], errlim,
+              d["bx"], d["by"], d["bz"])
+    _lib.sync()
+    rx = float(w.a["xspac"][1])
+    ref = O.divB_correct_f64(w.a["bx"][1], w.a["by"][1], w.a["bz"][1], rx)
+    scale = max(float(np.abs(r).max()) for r in ref)
+    for k, r in zip(("bx", "by", "bz"), ref):
+        got = d[k].cpu().numpy()
+        assert same_bits(got[0], w.a[k][0]), f"{k}: box 1 changed"
+        assert float(np.abs(got[1].astype(np.float64) - r).max()) <= 5e-4 * scale, k
+    d2 = to_dev(dev, w)
+    it, r2 = ctypes.c_longlong(0), ctypes.c_double(0)
+    st = _lib.call("divb_correct_ex", w.nx, w.ny, w.nz, 2, 2, w.a["xspac"], errlim,
+                   d2["bx"], d2["by"], d2["bz"], 0, ctypes.c_longlong(0), it, r2)
+    _lib.check("divb_correct_ex")
+    X, Y, Z = (w.a[k][1] for k in ("bx", "by", "bz"))
+    rhs = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2]) + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
+           + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / np.float32(2.0 * rx)
+    b2 = float((rhs.astype(np.float64) ** 2).sum())
+    npts = w.nx * w.ny * w.nz
+    assert st == 1 and it.value > 0
+    assert r2.value <= b2 * npts * float(errlim) ** 2 * (1 + 1e-6)
+
+
+def test_banded_tpose(dev):
+    """g3m3_banded_tpose (code/sub_banded_tpose.f90:20-29) bit for bit against
+    the oracle (itself bit-equal to the reference build), on device vectors
+    of the 7-band divB matrix and on a banded matrix of random bands."""
+    from g3m3_amd import _lib
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 13, 9, 7
+    npts = nx * ny * nz
+    pos = np.array([nx * ny, nx, 1, 0, -1, -nx, -nx * ny], np.int32)
+    for M in (O.divb_matrix(nx, ny, nz), rng.uniform(-1, 1, (npts, 7)).astype(np.float32)):
+        x = rng.uniform(-1, 1, npts).astype(np.float32)
+        want = np.zeros(npts, np.float32)
+        O.banded_tpose(7, npts, pos, M, x, want)
+        dM, dx = dev.from_numpy(M).cuda(), dev.from_numpy(x).cuda()
+        dy = dev.zeros(npts, dtype=dev.float32, device="cuda")
+        print(f"banded_tpose on {npts} points", flush=True)
+        _lib.call("banded_tpose", 7, npts, pos, dM, dx, dy)
+        _lib.sync()
+        assert same_bits(dy.cpu().numpy(), want)
