@@ -29,6 +29,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# compulsory FP32 traffic of one cell-update: the four stages separated by
+# grid-wide dependencies (predictor 71+33, corrector 71+34, Lapidus 34+34,
+# FCT 68+34 accesses of 4 bytes)
+WHOLE_STEP_BYTES = 1384
 METRIC = "cell-updates/s, full step, 5 nested 121x121x61 grids; adjoint/forward cost"
 
 # Algorithmic HBM bytes per unit of each instrumented kernel (unit: one
@@ -37,10 +41,12 @@ METRIC = "cell-updates/s, full step, 5 nested 121x121x61 grids; adjoint/forward 
 KERNEL_BYTES = {
     # algorithmic bytes per unit (each input and output array counted once)
     # native runtime:
-    # per interior cell, 3 species + fields: per species wrk 10 + old 10 +
-    # out 10; shared bs 3 + tv 3 + E 3 + ev 3, b old 3 + b out 3, epres
-    # wrk/old/out 3 -> 3*30 + 12 + 6 + 3
-    "k_push": (111 * 4, "interior"),
+    # the push stage: per cell the compulsory FP32 traffic of the two
+    # Lax-Wendroff stages it completes, averaged over its two calls per
+    # box-step -- predictor: state 34 + wrkepres + b0 3 read, 33 written
+    # (284 B); corrector: wrk 34 + old 34 + b0 3 read, 34 written (420 B).
+    # Scratch (E, ev, tv, bs) is overhead and not counted.
+    "k_push": ((284 + 420) // 2, "all"),
     # per cell: 30 species fields + epres + b 3 + b0 3 read (maxima pass)
     "k_validate": (37 * 4, "all"),
     # per interior cell: 34 fields read, 34 written
@@ -211,44 +217,87 @@ def precision_case(fast_mode: int):
     return res
 
 
-def cpu_baseline(workload: str, threads: int):
-    """The reference (OpenMP build) on one full step of the same workload."""
-    try:
-        from g3m3_amd.workload import make_workload
-        from oracle import ref
-        from oracle.boxstep import OracleRun
-    except Exception as e:  # pragma: no cover
-        return {"value": None, "error": str(e)}
+def _cpu_child(workload: str, threads: int, what: str) -> dict:
+    """Runs in a child process (no GPU): the reference's OpenMP build over
+    one full step (what = "step") or one box-step of the finest box (what =
+    "box") of the workload, timed."""
+    from g3m3_amd.workload import make_workload
+    from oracle import ref
+    from oracle.boxstep import OracleRun
     wl = make_workload(workload)
-    if ref.available(omp=True):
-        out = {}
+    out = {}
 
-        def go():
-            r = OracleRun(wl, "ref_omp", threads=threads)
-            r.init_dt()
+    def go():
+        r = OracleRun(wl, "ref_omp", threads=threads)
+        r.init_dt()
+        if what == "step":
             t0 = time.perf_counter()
             r.full_step()
-            dt = time.perf_counter() - t0
-            out.update(cu=r.cell_updates, dt=dt, bs=r.box_steps)
-        ref.with_big_stack(go)
-        return {"value": out["cu"] / out["dt"], "unit": "cell-updates/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"1 full step ({out['bs']} box-steps, {out['cu']} cell-updates) of "
-                          f"{workload}, reference Fortran sub_*.f90 built with flang -O2 "
-                          f"-fopenmp, {threads} threads, {out['dt']:.1f} s"}
-    r = OracleRun(wl, "restate")
-    r.init_dt()
-    r.t_step[:] = r.t_stepnew
-    r.t_step[:] = np.minimum.accumulate(r.t_step[::-1])[::-1]
-    t0 = time.perf_counter()
-    r.box_step(wl.ngrd)
-    dt = time.perf_counter() - t0
-    return {"value": r.cell_updates / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 box-step of the outermost box of {workload} by the NumPy "
-                      f"restatement, {dt:.1f} s"}
+        else:
+            r.t_step[:] = r.t_stepnew
+            r.t_step[:] = np.minimum.accumulate(r.t_step[::-1])[::-1]
+            t0 = time.perf_counter()
+            r.box_step(1)
+        out.update(cu=r.cell_updates, dt=time.perf_counter() - t0, bs=r.box_steps)
+    ref.with_big_stack(go)
+    return out
+
+
+def cpu_baseline(workload: str, threads: int):
+    """The reference's OpenMP build (flang -O3 -march=x86-64-v3 -fopenmp) on
+    this host: one full step of the same workload at `threads` threads, and
+    one box-step at 1 thread beside it.  Each runs in a child process, so a
+    failure there cannot end the benchmark; where that build is absent, the
+    NumPy restatement on one box-step (kind "port")."""
+    import subprocess
+    from oracle import ref
+    if not ref.available(omp=True):
+        from g3m3_amd.workload import make_workload
+        from oracle.boxstep import OracleRun
+        wl = make_workload(workload)
+        r = OracleRun(wl, "restate")
+        r.init_dt()
+        r.t_step[:] = r.t_stepnew
+        r.t_step[:] = np.minimum.accumulate(r.t_step[::-1])[::-1]
+        t0 = time.perf_counter()
+        r.box_step(wl.ngrd)
+        dt = time.perf_counter() - t0
+        return {"value": r.cell_updates / dt, "unit": "cell-updates/s", "cores": 1,
+                "kind": "port", "sample": f"1 box-step of the outermost box of {workload} by "
+                                          f"the NumPy restatement, {dt:.1f} s"}
+    res = {}
+    for label, th, what in (("multi", threads, "step"), ("single", 1, "box")):
+        env = dict(os.environ, OMP_NUM_THREADS=str(th), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", workload, str(th), what]
+        try:
+            p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=240)
+            res[label] = json.loads(p.stdout.strip().splitlines()[-1]) if p.returncode == 0 \
+                else {"error": f"exit {p.returncode}: {p.stderr[-300:]}"}
+        except Exception as e:   # a failed baseline is reported, never fatal
+            res[label] = {"error": str(e)}
+    m, s1 = res["multi"], res["single"]
+    if "error" in m:
+        return {"value": None, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                "error": m["error"]}
+    out = {"value": m["cu"] / m["dt"], "unit": "cell-updates/s", "cores": threads,
+           "kind": "reference",
+           "sample": f"1 full step ({m['bs']} box-steps, {m['cu']} cell-updates) of {workload}, "
+                     f"the reference's sub_*.f90 built with flang -O3 -march=x86-64-v3 "
+                     f"-fopenmp, {threads} threads, {m['dt']:.1f} s"}
+    if "error" not in s1:
+        out["single_thread"] = {
+            "value": s1["cu"] / s1["dt"], "cores": 1,
+            "sample": f"1 box-step of box 1 ({s1['cu']} cell-updates), 1 thread, "
+                      f"{s1['dt']:.1f} s"}
+    return out
 
 
 def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if argv[:1] == ["--cpu-child"]:
+        print(json.dumps(_cpu_child(argv[1], int(argv[2]), argv[3])))
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -263,20 +312,20 @@ def main(argv=None):
     ap.add_argument("--divb-mode", type=int, default=2,
                     help="0/1: the reference's CGNR (single / double scalars); 2: conjugate "
                          "gradients on the same system (FP64 dots)")
-    ap.add_argument("--precision", action="store_true",
-                    help="also run the divB precision case: 241x241x121 dipole + 1%% noise, "
-                         "mode 3 (FP64 residual refinement) to rel-residual 1e-8")
-    ap.add_argument("--batch", action="store_true",
-                    help="also run the batching case: 32 independent single-grid runs "
-                         "(41x41x21 .. 121x121x61), 20 full steps each, one at a time and "
-                         "then together on 8 streams")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the divB precision case (241x241x121 dipole + 1%% noise, "
+                         "mode 3, FP64 residual refinement, to rel-residual 1e-8)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batching case (32 independent single-grid runs, "
+                         "41x41x21 .. 121x121x61, 20 full steps each, one at a time, together "
+                         "on 8 streams, and replayed from captured graphs)")
     ap.add_argument("--species-kernels", choices=("auto", "march", "cell"), default="auto",
                     help="native driver: the species push / Lapidus kernels (auto: the run's "
                          "autotuner picks; counter runs fix the family the timed runs pick, "
                          "since counter collection changes the timings the tuner sees)")
-    ap.add_argument("--gradient", action="store_true",
-                    help="also run the differentiation case: 3 nested 81x81x41 boxes, 50 "
-                         "steps, craft misfit and its central-difference gradient")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the differentiation case (3 nested 81x81x41 boxes, 50 steps, "
+                         "craft misfit and its gradient)")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -335,31 +384,48 @@ def main(argv=None):
                 "traffic": traffic, "traffic_source": tsrc, "bytes_per_launch": per_launch,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl.value}
 
+    whole = {"bytes_per_cell_update": WHOLE_STEP_BYTES,
+             "achieved": round(value * WHOLE_STEP_BYTES / 1e9, 1), "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": round(value * WHOLE_STEP_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+             "ms_per_box_step": round(wall / (run.box_steps - bs0) * 1e3, 4)}
+
     divb = None
     if not a.no_divb:
         # reference criterion: res^2 <= |b|^2 * npts * errlim^2; rel-residual 1e-6
         npts = wl.nx * wl.ny * wl.nz
         errlim = 1e-6 / np.sqrt(npts)
+        info, b2 = [], []
+        if hasattr(run, "sync_arrays"):
+            run.sync_arrays()
+        for box in range(1, wl.ngrd + 1):   # |b|^2 of each box's system, before the solves
+            X, Y, Z = (run.a[k][box - 1].double() for k in ("bx", "by", "bz"))
+            rx = float(wl.grid["xspac"][box - 1])
+            rhs = ((X[1:-1, 1:-1, 2:] - X[1:-1, 1:-1, :-2]) + (Y[1:-1, 2:, 1:-1] - Y[1:-1, :-2, 1:-1])
+                   + (Z[2:, 1:-1, 1:-1] - Z[:-2, 1:-1, 1:-1])) / (2.0 * rx)
+            b2.append(float((rhs * rhs).sum()))
+            del X, Y, Z, rhs
         torch.cuda.synchronize()
         d0 = time.perf_counter()
-        info = []
         for box in range(1, wl.ngrd + 1):
             st, it, r2 = run.divb(box, errlim, mode=a.divb_mode, max_iters=20000)
-            info.append({"box": box, "status": st, "iterations": it})
+            info.append({"box": box, "status": st, "iterations": it,
+                         "achieved_rel_residual": float(np.sqrt(max(r2, 0.0) / b2[box - 1]))
+                         if b2[box - 1] > 0 else 0.0})
         torch.cuda.synchronize()
         dsec = time.perf_counter() - d0
         per_step = wall / a.steps
         modes = {0: "CGNR (the reference's solver), fp32 scalars",
                  1: "CGNR (the reference's solver), fp32 vectors, fp64 dots/scalars",
                  2: "CG on the same system, fp32 vectors, fp64 dots/scalars"}
-        divb = {"seconds_all_boxes": round(dsec, 3), "boxes": info, "rel_residual": 1e-6,
+        divb = {"seconds_all_boxes": round(dsec, 3), "boxes": info,
+                "rel_residual_target": 1e-6,
                 "mode": modes.get(a.divb_mode, str(a.divb_mode)),
                 "cell_updates_per_s_incl_divb_every_10_steps":
                     cu / a.steps * 10 / (per_step * 10 + dsec)}
 
-    prec = precision_case(a.divb_mode) if a.precision else None
-    batch = batch_case() if a.batch else None
-    grad = gradient_case() if a.gradient else None
+    prec = None if a.no_precision else precision_case(a.divb_mode)
+    batch = None if a.no_batch else batch_case()
+    grad = None if a.no_gradient else gradient_case()
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -371,7 +437,7 @@ def main(argv=None):
                                f"spacing {list(map(float, wl.grid['xspac']))}, "
                                f"eta={float(wl.scal['eta'])}",
                    "box_steps": run.box_steps - bs0, "cell_updates": cu},
-        "roofline": roof, "cpu_baseline": cpu, "divb": divb,
+        "roofline": roof, "whole_step": whole, "cpu_baseline": cpu, "divb": divb,
     }
     if hasattr(run, "fct_shape"):
         kc, order, cands = run.fct_shape()

@@ -18,6 +18,6 @@ def test_help_lists_the_legs():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
-    for flag in ("--steps", "--warmup", "--gpus", "--batch", "--precision", "--driver",
-                 "--species-kernels", "--gradient"):
+    for flag in ("--steps", "--warmup", "--gpus", "--no-batch", "--no-precision", "--driver",
+                 "--species-kernels", "--no-gradient"):
         assert flag in r.stdout
