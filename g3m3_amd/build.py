@@ -9,9 +9,9 @@ g3m3_amd/libg3m3.so.  Flags:
                          expression order);
   -target-feature -packed-fp32-ops   no packed FP32 instructions.
 
-When flang is present the Fortran driver fragment g3m3_amd/fortran/
-drive_fields.f90 (reference-style calls through implicit interfaces) is
-linked against the library as g3m3_amd/fortran/drive_fields.
+When flang is present the Fortran driver fragments g3m3_amd/fortran/
+drive_fields.f90 and drive_bounds.f90 (reference-style calls through
+implicit interfaces) are linked against the library beside their sources.
 """
 from __future__ import annotations
 
@@ -66,17 +66,18 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 
 def _build_fortran(verbose: bool) -> None:
-    src = PKG / "fortran" / "drive_fields.f90"
-    exe = src.with_suffix("")
-    if not (src.exists() and os.access(FLANG, os.X_OK)):
-        return
-    cmd = [FLANG, "-O2", str(src), "-L", str(PKG), "-lg3m3", "-Wl,-rpath,$ORIGIN/..",
-           "-o", str(exe)]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(OBJ))
-    if r.returncode != 0:
-        raise RuntimeError(f"flang failed on {src.name}:\n{r.stderr[-4000:]}")
-    if verbose:
-        print(f"built {exe}")
+    for name in ("drive_fields", "drive_bounds"):
+        src = PKG / "fortran" / (name + ".f90")
+        exe = src.with_suffix("")
+        if not (src.exists() and os.access(FLANG, os.X_OK)):
+            continue
+        cmd = [FLANG, "-O2", str(src), "-L", str(PKG), "-lg3m3", "-Wl,-rpath,$ORIGIN/..",
+               "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(OBJ))
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed on {src.name}:\n{r.stderr[-4000:]}")
+        if verbose:
+            print(f"built {exe}")
 
 
 if __name__ == "__main__":

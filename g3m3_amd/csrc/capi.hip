@@ -915,6 +915,69 @@ void g3m3_bndry_corer_v(float* const* fields, const int* nx, const int* ny, cons
                grd_zmax);
 }
 
+void g3m3_bndry_flanks(
+    float* wrkqrho, float* wrkqpx, float* wrkqpy, float* wrkqpz, float* wrkqpresx,
+    float* wrkqpresy, float* wrkqpresz, float* wrkqpresxy, float* wrkqpresxz, float* wrkqpresyz,
+    float* wrkhrho, float* wrkhpx, float* wrkhpy, float* wrkhpz, float* wrkhpresx,
+    float* wrkhpresy, float* wrkhpresz, float* wrkhpresxy, float* wrkhpresxz, float* wrkhpresyz,
+    float* wrkorho, float* wrkopx, float* wrkopy, float* wrkopz, float* wrkopresx,
+    float* wrkopresy, float* wrkopresz, float* wrkopresxy, float* wrkopresxz, float* wrkopresyz,
+    float* wrkepres, float* wrkbx, float* wrkby, float* wrkbz, const float* qrho,
+    const float* qpx, const float* qpy, const float* qpz, const float* qpresx,
+    const float* qpresy, const float* qpresz, const float* qpresxy, const float* qpresxz,
+    const float* qpresyz, const float* hrho, const float* hpx, const float* hpy,
+    const float* hpz, const float* hpresx, const float* hpresy, const float* hpresz,
+    const float* hpresxy, const float* hpresxz, const float* hpresyz, const float* orho,
+    const float* opx, const float* opy, const float* opz, const float* opresx,
+    const float* opresy, const float* opresz, const float* opresxy, const float* opresxz,
+    const float* opresyz, const float* epres, const float* bx, const float* by, const float* bz,
+    const float* oldqrho, const float* oldqpx, const float* oldqpy, const float* oldqpz,
+    const float* oldqpresx, const float* oldqpresy, const float* oldqpresz,
+    const float* oldqpresxy, const float* oldqpresxz, const float* oldqpresyz,
+    const float* oldhrho, const float* oldhpx, const float* oldhpy, const float* oldhpz,
+    const float* oldhpresx, const float* oldhpresy, const float* oldhpresz,
+    const float* oldhpresxy, const float* oldhpresxz, const float* oldhpresyz,
+    const float* oldorho, const float* oldopx, const float* oldopy, const float* oldopz,
+    const float* oldopresx, const float* oldopresy, const float* oldopresz,
+    const float* oldopresxy, const float* oldopresxz, const float* oldopresyz,
+    const float* oldepres, const float* oldbx, const float* oldby, const float* oldbz,
+    float* work, const int* nx, const int* ny, const int* nz, const int* n_grids, const int* box,
+    const double* t_old, const double* t_new, const double* t, const float* grd_xmin,
+    const float* grd_xmax, const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+    const float* grd_zmax) {
+    float* const tgt[34] = {wrkqrho, wrkqpx, wrkqpy, wrkqpz, wrkqpresx, wrkqpresy, wrkqpresz,
+        wrkqpresxy, wrkqpresxz, wrkqpresyz, wrkhrho, wrkhpx, wrkhpy, wrkhpz, wrkhpresx,
+        wrkhpresy, wrkhpresz, wrkhpresxy, wrkhpresxz, wrkhpresyz, wrkorho, wrkopx, wrkopy,
+        wrkopz, wrkopresx, wrkopresy, wrkopresz, wrkopresxy, wrkopresxz, wrkopresyz, wrkepres,
+        wrkbx, wrkby, wrkbz};
+    const float* const cur[34] = {qrho, qpx, qpy, qpz, qpresx, qpresy, qpresz, qpresxy, qpresxz,
+        qpresyz, hrho, hpx, hpy, hpz, hpresx, hpresy, hpresz, hpresxy, hpresxz, hpresyz, orho,
+        opx, opy, opz, opresx, opresy, opresz, opresxy, opresxz, opresyz, epres, bx, by, bz};
+    const float* const old[34] = {oldqrho, oldqpx, oldqpy, oldqpz, oldqpresx, oldqpresy,
+        oldqpresz, oldqpresxy, oldqpresxz, oldqpresyz, oldhrho, oldhpx, oldhpy, oldhpz,
+        oldhpresx, oldhpresy, oldhpresz, oldhpresxy, oldhpresxz, oldhpresyz, oldorho, oldopx,
+        oldopy, oldopz, oldopresx, oldopresy, oldopresz, oldopresxy, oldopresxz, oldopresyz,
+        oldepres, oldbx, oldby, oldbz};
+    g3m3_bndry_flanks_v(tgt, cur, old, work, nx, ny, nz, n_grids, box, t_old, t_new, t,
+                        grd_xmin, grd_xmax, grd_ymin, grd_ymax, grd_zmin, grd_zmax);
+}
+
+void g3m3_bndry_corer(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx, float* hpy, float* hpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opx, float* opy, float* opz,
+    float* epres, float* bx, float* by, float* bz, float* qpresxy, float* qpresxz,
+    float* qpresyz, float* hpresxy, float* hpresxz, float* hpresyz, float* opresxy,
+    float* opresxz, float* opresyz, const int* nx, const int* ny, const int* nz,
+    const int* n_grids, const int* box, const float* grd_xmin, const float* grd_xmax,
+    const float* grd_ymin, const float* grd_ymax, const float* grd_zmin, const float* grd_zmax) {
+    float* const fields[34] = {qrho, qpresx, qpresy, qpresz, qpx, qpy, qpz, hrho, hpresx, hpresy,
+        hpresz, hpx, hpy, hpz, orho, opresx, opresy, opresz, opx, opy, opz, epres, bx, by, bz,
+        qpresxy, qpresxz, qpresyz, hpresxy, hpresxz, hpresyz, opresxy, opresxz, opresyz};
+    g3m3_bndry_corer_v(fields, nx, ny, nz, n_grids, box, grd_xmin, grd_xmax, grd_ymin,
+                       grd_ymax, grd_zmin, grd_zmax);
+}
+
 void g3m3_flanks_synced(float* rho, const int* nx, const int* ny, const int* nz,
                         const int* n_grids, const int* box, const float* gx0, const float* gx1,
                         const float* gy0, const float* gy1, const float* gz0, const float* gz1) {
@@ -1476,9 +1539,11 @@ G3M3_FORTRAN_ALIAS(flux_correct)
 G3M3_FORTRAN_ALIAS(bndry_outer)
 G3M3_FORTRAN_ALIAS(bndry_inner)
 G3M3_FORTRAN_ALIAS(bndry_flanks_v)
+G3M3_FORTRAN_ALIAS(bndry_flanks)
 G3M3_FORTRAN_ALIAS(flanks)
 G3M3_FORTRAN_ALIAS(corer)
 G3M3_FORTRAN_ALIAS(bndry_corer_v)
+G3M3_FORTRAN_ALIAS(bndry_corer)
 G3M3_FORTRAN_ALIAS(flanks_synced)
 G3M3_FORTRAN_ALIAS(banded_tpose)
 G3M3_FORTRAN_ALIAS(divb_correct)

@@ -330,6 +330,55 @@ void g3m3_bndry_corer_v(float* const* fields, const int* nx, const int* ny, cons
                         const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
                         const float* grd_zmin, const float* grd_zmax);
 
+/* code/sub_bndry_flanks.f90:5-35 -- the reference's argument list: the 34
+ * target arrays (wrk..., in the reference's order), the 34 current and 34
+ * old arrays of the same fields, work, the box and times, the grid extents.
+ * The target may be the current array itself (code/3d_main.f90:2945-2976).
+ * The same kernel as g3m3_bndry_flanks_v. */
+void g3m3_bndry_flanks(
+    float* wrkqrho, float* wrkqpx, float* wrkqpy, float* wrkqpz, float* wrkqpresx,
+    float* wrkqpresy, float* wrkqpresz, float* wrkqpresxy, float* wrkqpresxz, float* wrkqpresyz,
+    float* wrkhrho, float* wrkhpx, float* wrkhpy, float* wrkhpz, float* wrkhpresx,
+    float* wrkhpresy, float* wrkhpresz, float* wrkhpresxy, float* wrkhpresxz, float* wrkhpresyz,
+    float* wrkorho, float* wrkopx, float* wrkopy, float* wrkopz, float* wrkopresx,
+    float* wrkopresy, float* wrkopresz, float* wrkopresxy, float* wrkopresxz, float* wrkopresyz,
+    float* wrkepres, float* wrkbx, float* wrkby, float* wrkbz, const float* qrho,
+    const float* qpx, const float* qpy, const float* qpz, const float* qpresx,
+    const float* qpresy, const float* qpresz, const float* qpresxy, const float* qpresxz,
+    const float* qpresyz, const float* hrho, const float* hpx, const float* hpy,
+    const float* hpz, const float* hpresx, const float* hpresy, const float* hpresz,
+    const float* hpresxy, const float* hpresxz, const float* hpresyz, const float* orho,
+    const float* opx, const float* opy, const float* opz, const float* opresx,
+    const float* opresy, const float* opresz, const float* opresxy, const float* opresxz,
+    const float* opresyz, const float* epres, const float* bx, const float* by, const float* bz,
+    const float* oldqrho, const float* oldqpx, const float* oldqpy, const float* oldqpz,
+    const float* oldqpresx, const float* oldqpresy, const float* oldqpresz,
+    const float* oldqpresxy, const float* oldqpresxz, const float* oldqpresyz,
+    const float* oldhrho, const float* oldhpx, const float* oldhpy, const float* oldhpz,
+    const float* oldhpresx, const float* oldhpresy, const float* oldhpresz,
+    const float* oldhpresxy, const float* oldhpresxz, const float* oldhpresyz,
+    const float* oldorho, const float* oldopx, const float* oldopy, const float* oldopz,
+    const float* oldopresx, const float* oldopresy, const float* oldopresz,
+    const float* oldopresxy, const float* oldopresxz, const float* oldopresyz,
+    const float* oldepres, const float* oldbx, const float* oldby, const float* oldbz,
+    float* work, const int* nx, const int* ny, const int* nz, const int* n_grids, const int* box,
+    const double* t_old, const double* t_new, const double* t, const float* grd_xmin,
+    const float* grd_xmax, const float* grd_ymin, const float* grd_ymax, const float* grd_zmin,
+    const float* grd_zmax);
+
+/* code/sub_bndry_corer.f90:5-14 -- the reference's argument list: the 34
+ * fields in its order, the box, the grid extents.  The same kernel as
+ * g3m3_bndry_corer_v. */
+void g3m3_bndry_corer(
+    float* qrho, float* qpresx, float* qpresy, float* qpresz, float* qpx, float* qpy, float* qpz,
+    float* hrho, float* hpresx, float* hpresy, float* hpresz, float* hpx, float* hpy, float* hpz,
+    float* orho, float* opresx, float* opresy, float* opresz, float* opx, float* opy, float* opz,
+    float* epres, float* bx, float* by, float* bz, float* qpresxy, float* qpresxz,
+    float* qpresyz, float* hpresxy, float* hpresxz, float* hpresyz, float* opresxy,
+    float* opresxz, float* opresyz, const int* nx, const int* ny, const int* nz,
+    const int* n_grids, const int* box, const float* grd_xmin, const float* grd_xmax,
+    const float* grd_ymin, const float* grd_ymax, const float* grd_zmin, const float* grd_zmax);
+
 /* code/sub_flanks_synced.f90:5-6 -- coarse -> fine faces after divB */
 void g3m3_flanks_synced(float* rho, const int* nx, const int* ny, const int* nz,
                         const int* n_grids, const int* box, const float* grd_xmin,

@@ -94,3 +94,51 @@ def test_fortran_driver_fragment(tmp_path):
     want.update({k: a[k] for k in outs4})
     bad = [k for k in outs3 + outs4 if not same_bits(got[k], want[k])]
     assert not bad, f"Fortran-driven results differ from the oracle on {bad}"
+
+
+BOUNDS_EXE = os.path.join(os.path.dirname(EXE), "drive_bounds")
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_fortran_boundary_calls(tmp_path, box):
+    """g3m3_amd/fortran/drive_bounds.f90 calls, through implicit interfaces
+    with the reference's full argument lists: bndry_flanks into wrk at t1,
+    bndry_flanks with the state as both target and source at t2 (the aliased
+    call of code/3d_main.f90:2945-2976), then bndry_corer
+    (code/sub_bndry_corer.f90:5-14).  Both wrk and the state must equal the
+    oracle's result of the same three calls bit for bit."""
+    from common import World
+    from g3m3_amd import signatures as sig
+    from oracle.argnames import GRID, corer_names, flanks_names
+    if not os.path.exists(BOUNDS_EXE):
+        pytest.skip("Fortran driver not built (no flang where the library was built)")
+    w = World(13, 13, 7, 3, seed=23, lists=False)
+    t_old = np.array([0.0, 0.1, 0.1], np.float64)
+    t_new = np.array([0.05, 0.3, 0.5], np.float64)
+    t1, t2 = 0.17, 0.29
+    order = sig.FLANK_ORDER
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(np.array([w.nx, w.ny, w.nz, w.ngrd, box], np.int32).tobytes())
+        f.write(np.concatenate([t_old, t_new, [t1, t2]]).astype(np.float64).tobytes())
+        for g in GRID:
+            f.write(np.asarray(w.a[g], F).tobytes())
+        for pre in ("", "old", "wrk"):
+            for k in order:
+                f.write(np.ascontiguousarray(w.a[pre + k], F).tobytes())
+    print(f"running {BOUNDS_EXE}, box {box}", flush=True)
+    r = subprocess.run([BOUNDS_EXE, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, F).reshape(2, len(order), w.ngrd, w.nz, w.ny, w.nx)
+    wo = w.copy()
+    wo.a["t_old"], wo.a["t_new"] = t_old, t_new
+    wo.s["box"] = box
+    wo.s["t"] = t1
+    O.bndry_flanks(*wo.args(flanks_names("wrk", "", "old")))
+    wo.s["t"] = t2
+    O.bndry_flanks(*wo.args(flanks_names("", "", "old")))
+    O.bndry_corer(*wo.args(corer_names("")))
+    bad = [("wrk" + k) for q, k in enumerate(order) if not same_bits(raw[0, q], wo.a["wrk" + k])]
+    bad += [k for q, k in enumerate(order) if not same_bits(raw[1, q], wo.a[k])]
+    assert not bad, f"Fortran-driven boundary calls differ from the oracle on {bad}"
