@@ -1096,6 +1096,98 @@ void g3m3_fnd_pres(const float* px, const float* py, const float* pz, const floa
     check_launch("fnd_pres");
 }
 
+Buf g_graph;
+
+// code/sub_write_graphing_data.f90:348-760: the whole-box calcs (calcur,
+// total B, fnd_evel, BandE with the resistivity list term; species
+// velocities and fnd_pres inside the cut kernel) and the tabulated
+// quantities on the three cut planes of one box
+void g3m3_graphing_cuts(
+    const float* qrho, const float* qpx, const float* qpy, const float* qpz, const float* qpresx,
+    const float* qpresy, const float* qpresz, const float* hrho, const float* hpx,
+    const float* hpy, const float* hpz, const float* hpresx, const float* hpresy,
+    const float* hpresz, const float* orho, const float* opx, const float* opy,
+    const float* opz, const float* opresx, const float* opresy, const float* opresz,
+    const float* bx, const float* by, const float* bz, const float* epres, const float* bx0,
+    const float* by0, const float* bz0, const float* resistive, const int* ijzero,
+    const int* numzero, const int* ijmid, const int* nummid, const int* nx, const int* ny,
+    const int* nz, const int* n_grids, const int* box, const int* mbndry, const int* mmid,
+    const int* mzero, const float* limit, const float* xspac, const float* grid_minvals,
+    const float* rmassq, const float* rmassh, const float* rmasso, const float* reynolds,
+    const float* resist, const float* smallbit, const float* re_equiv, const float* b_equiv,
+    const float* v_equiv, const float* rho_equiv, const float* planet_rad, float* out) {
+    const Box B = box_of(nx, ny, nz);
+    const int b = *box - 1;
+    // the cut planes (:386-395), 1-based; box 5 of the reference is offset
+    const int x_adj[5] = {0, 0, 0, 0, -15};
+    const int cv[3] = {(int)(*limit / 2.0f + 1.0f), (int)(*limit + 1.0f),
+                       (int)(*limit + 1.0f + (float)(*box <= 5 ? x_adj[b] : 0))};
+    if (cv[0] < 1 || cv[0] > B.nz || cv[1] < 1 || cv[1] > B.ny || cv[2] < 1 || cv[2] > B.nx) {
+        latch(4, "graphing_cuts: a cut plane lies outside the box");
+        return;
+    }
+    const size_t V = (size_t)B.V();
+    float* w = (float*)g_graph.get(16 * V * sizeof(float));
+    if (!w) return;
+    float *cur = w, *bt = w + 3 * V, *btot = w + 6 * V, *ev = w + 7 * V, *tv = w + 10 * V,
+          *e = w + 13 * V;
+    const float rx = xspac[b];
+    g3m3_calcur(bx, by, bz, nx, ny, nz, n_grids, box, cur, cur + V, cur + 2 * V, &rx, &rx, &rx);
+    g3m3_bfield_total(bx, by, bz, bx0, by0, bz0, bt, bt + V, bt + 2 * V, btot, nx, ny, nz,
+                      n_grids, box, smallbit);
+    g3m3_fnd_evel(qpx, qpy, qpz, qrho, hpx, hpy, hpz, hrho, opx, opy, opz, orho, cur, cur + V,
+                  cur + 2 * V, ev, ev + V, ev + 2 * V, tv, tv + V, tv + 2 * V, nx, ny, nz,
+                  n_grids, box, rmassq, rmassh, rmasso, reynolds);
+    g3m3_bande(e, e + V, e + 2 * V, bt, bt + V, bt + 2 * V, cur, cur + V, cur + 2 * V, ev,
+               ev + V, ev + 2 * V, btot, epres, qrho, hrho, orho, resistive, resist, reynolds, nx,
+               ny, nz, n_grids, box, rmassq, rmassh, rmasso, ijmid, nummid, ijzero, mbndry,
+               numzero, mmid, mzero, &rx, &rx, &rx);
+    GraphCuts G{};
+    G.B = B;
+    for (int a = 0; a < 3; ++a) G.cut[a] = cv[a] - 1;
+    const float* sp[3][7] = {{qrho, qpx, qpy, qpz, qpresx, qpresy, qpresz},
+                             {hrho, hpx, hpy, hpz, hpresx, hpresy, hpresz},
+                             {orho, opx, opy, opz, opresx, opresy, opresz}};
+    for (int q = 0; q < 3; ++q)
+        for (int f = 0; f < 7; ++f) G.sp[q][f] = S(sp[q][f], B, box);
+    G.epres = S(epres, B, box);
+    const float* bb[3] = {bx, by, bz};
+    const float* bb0[3] = {bx0, by0, bz0};
+    for (int a = 0; a < 3; ++a) {
+        G.b[a] = S(bb[a], B, box);
+        G.b0[a] = S(bb0[a], B, box);
+        G.cur[a] = cur + a * V;
+        G.tv[a] = tv + a * V;
+        G.e[a] = e + a * V;
+    }
+    G.rmass[0] = *rmassq;
+    G.rmass[1] = *rmassh;
+    G.rmass[2] = *rmasso;
+    G.smallbit = *smallbit;
+    G.v_equiv = *v_equiv;
+    G.b_equiv = *b_equiv;
+    G.rho_equiv = *rho_equiv;
+    G.re_equiv = *re_equiv;
+    // the unit factors (:205-209) and constants (code/mod_astrometry.f90:20-24),
+    // single precision in the reference's order
+    const float pi = 3.14159274f;
+    const float mu0 = 4.e-7f * pi;
+    const float m_prot = 1.67262e-27f, q_elec = 1.60218e-19f;
+    const float vv = *v_equiv * 1.e3f;
+    G.temp_equiv = m_prot * (vv * vv) / q_elec / 1.e3f;
+    G.pres_equiv = *rho_equiv * 1.e6f * m_prot * (vv * vv);
+    G.cur_equiv = *b_equiv / mu0 / (*planet_rad * 1.e3f * *re_equiv);
+    G.e_equiv = *b_equiv * *v_equiv * 1.e3f / 1.e6f;
+    G.mu0 = mu0;
+    G.m_prot = m_prot;
+    G.xmin = grid_minvals[3 * b];
+    G.ymin = grid_minvals[3 * b + 1];
+    G.zmin = grid_minvals[3 * b + 2];
+    G.xs = xspac[b];
+    launch_graph_cuts(G, R(out), g_stream);
+    check_launch("graphing_cuts");
+}
+
 // ---- initialisation ------------------------------------------------------------
 
 static RotationGeom rotation_geom(const float* rot_angle, const float* xdip, const float* ydip,

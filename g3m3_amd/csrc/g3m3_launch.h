@@ -133,6 +133,23 @@ void launch_push_elec(const Box& B, const float* w, const float* old, float* ep,
 void launch_push_bfld(const Box& B, float* b[3], const float* old[3], const float* e[3], float delt,
                       float rx, float ry, float rz, hipStream_t s);
 void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& src, hipStream_t s);
+// write_graphing_data's cut-plane quantities (k_graph_cuts in k_misc.hip)
+struct GraphCuts {
+    Box B;
+    int cut[3];                 // 0-based: k of the xy plane, j of xz, i of yz
+    const float* sp[3][7];      // per species rho, px, py, pz, presx, presy, presz
+    const float* epres;
+    const float* b[3];
+    const float* b0[3];
+    const float* cur[3];
+    const float* tv[3];
+    const float* e[3];
+    float rmass[3];
+    float smallbit, v_equiv, b_equiv, rho_equiv, re_equiv;
+    float temp_equiv, pres_equiv, cur_equiv, e_equiv, mu0, m_prot;
+    float xmin, ymin, zmin, xs;
+};
+void launch_graph_cuts(const GraphCuts& G, float* out, hipStream_t s);
 void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],
                      const float* const b[3], float* para, float* perp, float* cross,
                      hipStream_t s);

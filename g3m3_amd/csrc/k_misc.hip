@@ -57,6 +57,23 @@ void launch_trilin(int npts, const float* sxyz, const float* gridpts, const floa
 
 // ---------------------------------------------------------------------------
 // code/sub_fnd_pres.f90:29-62: parallel / perpendicular / cross pressures
+// of one cell from the tensor diagonal (x, y, z), the flow v and the field b
+__device__ inline void fnd_pres_point(float x, float y, float z, float avx, float avy, float avz,
+                                      float abx, float aby, float abz, float& pa, float& pp,
+                                      float& pc) {
+    const float presmag = sqrtf(x * x + y * y + z * z) + 1.e-11f;
+    const float bmag = sqrtf(abx * abx + aby * aby + abz * abz) + 1.e-12f;
+    const float vcbx = (avy * abz - avz * aby) / bmag;
+    const float vcby = -(avx * abz - avz * abx) / bmag;
+    const float vcbz = (avx * aby - avy * abx) / bmag;
+    const float vcmag = sqrtf(vcbx * vcbx + vcby * vcby + vcbz * vcbz) + 1.e-12f;
+    const float ax = abx * x, ay = aby * y, az = abz * z;
+    pa = sqrtf(ax * ax + ay * ay + az * az) / bmag;
+    const float cx = vcbx * x, cy = vcby * y, cz = vcbz * z;
+    pc = sqrtf(cx * cx + cy * cy + cz * cz) / vcmag;
+    pp = sqrtf(fabsf(presmag * presmag - (pa * pa + pc * pc)));
+}
+
 __global__ void k_fnd_pres(int n, const float* __restrict__ px, const float* __restrict__ py,
                            const float* __restrict__ pz, const float* __restrict__ vx,
                            const float* __restrict__ vy, const float* __restrict__ vz,
@@ -64,22 +81,134 @@ __global__ void k_fnd_pres(int n, const float* __restrict__ px, const float* __r
                            const float* __restrict__ bz, float* para, float* perp, float* cross) {
     const int c = gtid();
     if (c >= n) return;
-    const float x = px[c], y = py[c], z = pz[c];
-    const float presmag = sqrtf(x * x + y * y + z * z) + 1.e-11f;
-    const float abx = bx[c], aby = by[c], abz = bz[c];
-    const float bmag = sqrtf(abx * abx + aby * aby + abz * abz) + 1.e-12f;
-    const float avx = vx[c], avy = vy[c], avz = vz[c];
-    const float vcbx = (avy * abz - avz * aby) / bmag;
-    const float vcby = -(avx * abz - avz * abx) / bmag;
-    const float vcbz = (avx * aby - avy * abx) / bmag;
-    const float vcmag = sqrtf(vcbx * vcbx + vcby * vcby + vcbz * vcbz) + 1.e-12f;
-    const float ax = abx * x, ay = aby * y, az = abz * z;
-    const float pa = sqrtf(ax * ax + ay * ay + az * az) / bmag;
-    const float cx = vcbx * x, cy = vcby * y, cz = vcbz * z;
-    const float pc = sqrtf(cx * cx + cy * cy + cz * cz) / vcmag;
+    float pa, pp, pc;
+    fnd_pres_point(px[c], py[c], pz[c], vx[c], vy[c], vz[c], bx[c], by[c], bz[c], pa, pp, pc);
     para[c] = pa;
     cross[c] = pc;
-    perp[c] = sqrtf(fabsf(presmag * presmag - (pa * pa + pc * pc)));
+    perp[c] = pp;
+}
+
+// ---------------------------------------------------------------------------
+// The quantities code/sub_write_graphing_data.f90:348-760 tabulates on its
+// three cut planes of one box, in its physical units and column order.  One
+// thread per plane point: cut 0 the xy plane k = kc (points j, i), cut 1 the
+// xz plane j = jc (k, i), cut 2 the yz plane i = ic (k, j), as the
+// reference's loops run.  Per point 53 values: plas 8, flow 13, pres 10,
+// bande 11, model 11.  Single precision in the reference's expression
+// order; log10 is evaluated in double and rounded once.
+__device__ inline float log10r(float x) { return (float)log10((double)x); }
+
+__global__ void k_graph_cuts(GraphCuts G, float* __restrict__ out) {
+    const int p = gtid();
+    const Box& B = G.B;
+    const int n0 = B.nx * B.ny, n1 = B.nx * B.nz, n2 = B.ny * B.nz;
+    if (p >= n0 + n1 + n2) return;
+    int i, j, k;
+    if (p < n0) {
+        k = G.cut[0];
+        j = p / B.nx;
+        i = p - j * B.nx;
+    } else if (p < n0 + n1) {
+        j = G.cut[1];
+        k = (p - n0) / B.nx;
+        i = (p - n0) - k * B.nx;
+    } else {
+        i = G.cut[2];
+        k = (p - n0 - n1) / B.ny;
+        j = (p - n0 - n1) - k * B.ny;
+    }
+    const int c = B.IX(i, j, k);
+    float* o = out + (size_t)53 * p;
+    const float ve = G.v_equiv;
+    const float unetx = G.tv[0][c] * ve, unety = G.tv[1][c] * ve, unetz = G.tv[2][c] * ve;
+    const float net_flow = sqrtf(unetx * unetx + unety * unety + unetz * unetz);
+    const float curx_all = G.cur[0][c] * G.cur_equiv;
+    const float cury_all = G.cur[1][c] * G.cur_equiv;
+    const float curz_all = G.cur[2][c] * G.cur_equiv;
+    const float bt[3] = {G.b0[0][c] + G.b[0][c], G.b0[1][c] + G.b[1][c], G.b0[2][c] + G.b[2][c]};
+    float rho[3], apres[3], v[3][3], lp[3][3];
+    for (int s = 0; s < 3; ++s) {
+        rho[s] = G.sp[s][0][c];
+        const float px = G.sp[s][1][c], py = G.sp[s][2][c], pz = G.sp[s][3][c];
+        const float x = G.sp[s][4][c], y = G.sp[s][5][c], z = G.sp[s][6][c];
+        apres[s] = fmaxf(0.0000001f, 0.333f * (x + y + z));
+        const float den = fmaxf(rho[s], G.smallbit);
+        const float vx = px / den, vy = py / den, vz = pz / den;
+        float pa, pp, pc;
+        fnd_pres_point(x, y, z, vx, vy, vz, bt[0], bt[1], bt[2], pa, pp, pc);
+        lp[s][0] = log10r(pa * G.pres_equiv);
+        lp[s][1] = log10r(pp * G.pres_equiv);
+        lp[s][2] = log10r(pc * G.pres_equiv);
+        v[s][0] = vx * ve;
+        v[s][1] = vy * ve;
+        v[s][2] = vz * ve;
+    }
+    const float aepres = fmaxf(0.0000001f, G.epres[c]);
+    float qden = fmaxf(rho[0], 0.000001f);
+    float hden = fmaxf(0.0001f, rho[1]);
+    float oden = fmaxf(0.0001f, rho[2]);
+    const float tden = (rho[0] + rho[1] + rho[2]) * G.rho_equiv;
+    qden = qden / G.rmass[0] * G.rho_equiv;
+    hden = hden / G.rmass[1] * G.rho_equiv;
+    oden = oden / G.rmass[2] * G.rho_equiv;
+    const float eden = oden + hden + qden + 0.0001f;
+    const float te = G.temp_equiv;
+    o[0] = log10r(qden);
+    o[1] = fmaxf(0.00001f, apres[0] / qden) * te;
+    o[2] = log10r(hden);
+    o[3] = apres[1] / hden * te;
+    o[4] = log10r(oden);
+    o[5] = apres[2] / oden * te;
+    o[6] = log10r(eden);
+    o[7] = aepres / eden * te;
+    const float be = G.b_equiv;
+    const float abx = be * (G.b[0][c] + G.b0[0][c]);
+    const float aby = be * (G.b[1][c] + G.b0[1][c]);
+    const float abz = be * (G.b[2][c] + G.b0[2][c]);
+    const float bmag = sqrtf(abx * abx + aby * aby + abz * abz);
+    const float v_alf = bmag * 1.e-9f / sqrtf(G.mu0 * tden * 1.e6f * G.m_prot);
+    for (int s = 0; s < 3; ++s)
+        for (int a = 0; a < 3; ++a) o[8 + 3 * s + a] = v[s][a];
+    o[17] = unetx;
+    o[18] = unety;
+    o[19] = unetz;
+    o[20] = net_flow * 1.e3f / v_alf;
+    for (int s = 0; s < 3; ++s)
+        for (int a = 0; a < 3; ++a) o[21 + 3 * s + a] = lp[s][a];
+    o[30] = log10r(aepres * G.pres_equiv);
+    const float ee = G.e_equiv;
+    const float aex = G.e[0][c] * ee, aey = G.e[1][c] * ee, aez = G.e[2][c] * ee;
+    o[31] = abx;
+    o[32] = aby;
+    o[33] = abz;
+    o[34] = bmag;
+    o[35] = aex;
+    o[36] = aey;
+    o[37] = aez;
+    o[38] = sqrtf(aex * aex + aey * aey + aez * aez);
+    o[39] = curx_all;
+    o[40] = cury_all;
+    o[41] = curz_all;
+    const float ri = (G.xmin + G.xs * (float)i) * G.re_equiv;
+    const float rj = (G.ymin + G.xs * (float)j) * G.re_equiv;
+    const float rk = (G.zmin + G.xs * (float)k) * G.re_equiv;
+    o[42] = ri;
+    o[43] = rj;
+    o[44] = rk;
+    o[45] = sqrtf(ri * ri + rj * rj + rk * rk);
+    const float b0x = G.b0[0][c], b0y = G.b0[1][c], b0z = G.b0[2][c];
+    o[46] = be * G.b[0][c];
+    o[47] = be * G.b[1][c];
+    o[48] = be * G.b[2][c];
+    o[49] = be * b0x;
+    o[50] = be * b0y;
+    o[51] = be * b0z;
+    o[52] = sqrtf(b0x * b0x + b0y * b0y + b0z * b0z) * be;
+}
+
+void launch_graph_cuts(const GraphCuts& G, float* out, hipStream_t s) {
+    const int n = G.B.nx * G.B.ny + G.B.nx * G.B.nz + G.B.ny * G.B.nz;
+    hipLaunchKernelGGL(k_graph_cuts, dim3((n + 255) / 256), dim3(256), 0, s, G, out);
 }
 
 void launch_fnd_pres(int n, const float* const p[3], const float* const v[3],

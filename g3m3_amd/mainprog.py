@@ -14,8 +14,12 @@ runs the time loop (:1925-3095) through the native runtime:
 * a fluid (restart) file when ``t >= ts1`` (:3300-3361), followed by the
   divB clean-up when ``divb_lores`` (:3363-3408).
 
-Plotting (``write_graphing_data``), spacecraft, the ``input_fluid`` upstream
-files and restarts from a fluid file are not part of this loop.  The grid
+* the plotting tables when ``t >= tgraph`` (:3097-3144,
+  ``g3m3_graphing_cuts`` and g3m3_amd.graphing, which writes the reference's
+  files; the reference's Python plotting script is not started);
+
+Spacecraft, the ``input_fluid`` upstream files and restarts from a fluid
+file are not part of this loop.  The grid
 shape is the reference's compile-time one (5 boxes of 121 x 121 x 61,
 ``limit = 60``) unless given.
 
@@ -124,6 +128,7 @@ class MainSetup:
     deltinj: np.float32
     torus: dict = field(default_factory=dict)
     flux_scale: list = field(default_factory=list)
+    graph: dict = field(default_factory=dict)   # write_graphing_data's scalars
 
 
 def setup(inp: dict, limit: float = 60.0, n_grids: int = 5, mbndry: int = 1,
@@ -262,9 +267,16 @@ def setup(inp: dict, limit: float = 60.0, n_grids: int = 5, mbndry: int = 1,
     scale = [F(F(F(rho_equiv * v_equiv) * F(1.e5))
                * _ipow(F(F(F(xs[b] * P["planet_rad"]) * re_equiv) * F(1.e5)), 2))
              for b in range(n_grids)]
+    # b_equiv in nT (:907): v_equiv*1.e3 * sqrt(mu0 * rho_equiv*1.e6*m_prot) * 1.e9
+    mu0, m_prot = F(F(4.e-7) * PI), F(1.67262e-27)
+    b_equiv = F(F(F(v_equiv * F(1.e3)) * F(np.sqrt(F(F(F(mu0 * rho_equiv) * F(1.e6)) * m_prot))))
+                * F(1.e9))
+    graph = dict(run_name=str(g.get("run_name", "")), limit=limit, re_equiv=re_equiv,
+                 b_equiv=b_equiv, v_equiv=v_equiv, rho_equiv=rho_equiv,
+                 planet_rad=P["planet_rad"])
     return MainSetup(inp=inp, wl=wl, init=init, caps=(msrf, mmid, mzero), t=t, ut=ut,
                      tmax=tmax, ts1=ts1, tgraph=tgraph, tinj=tinj, deltg=deltg,
-                     deltinj=deltinj, torus=torus, flux_scale=scale)
+                     deltinj=deltinj, torus=torus, flux_scale=scale, graph=graph)
 
 
 _LIBM = _ct.CDLL("libm.so.6")
@@ -349,6 +361,7 @@ class MainRun:
         self.ts1, self.tmax = self.S.ts1, self.S.tmax
         self.tgraph, self.tinj = self.S.tgraph, self.S.tinj
         self.step_log, self.flux_log, self.torus_log, self.fluid_files = [], [], [], []
+        self.graph_log = []
         self.steps = 0
 
     def _grid_args(self):
@@ -388,6 +401,16 @@ class MainRun:
                   *[F(T[k]) for k in _TORUS_ORDER], *tot)
         _lib.sync()
         self.torus_log.append((r.t, *[float(x[0]) for x in tot]))
+
+    def _graph(self, ut):
+        """code/3d_main.f90:3097-3144: the plotting tables (write_graphing_data)
+        under out_dir/figures/data."""
+        from .graphing import write_graphing_data
+        G = self.S.graph
+        n = write_graphing_data(self.run, self.out_dir, G["run_name"], ut, G["limit"],
+                                G["re_equiv"], G["b_equiv"], G["v_equiv"], G["rho_equiv"],
+                                G["planet_rad"])
+        self.graph_log.append((self.run.t, n))
 
     def _write_fluid(self):
         """code/3d_main.f90:3304-3361."""
@@ -434,6 +457,9 @@ class MainRun:
             ut = r.t * t_equiv / 3600.0
             self.flux_log += [(ut, *row) for row in fluxes]
             self.step_log += [(box, r.t, t_old, t_new) for box, t_old, t_new in r.step_log()]
+            if r.t >= float(self.tgraph):
+                self._graph(ut)
+                self.tgraph = F(self.tgraph + self.S.deltg)
             if r.t >= float(self.tinj) and self.S.inp.get("ringo", False):
                 self._torus()
                 self.tinj = F(self.tinj + self.S.deltinj)

@@ -518,6 +518,32 @@ void g3m3_torus_inject(
     const float* rmasso, const float* t_torus, const float* aniso_factor, const float* reduct,
     float* tot_q, float* tot_h, float* tot_o);
 
+/* code/sub_write_graphing_data.f90:348-760 -- the quantities the reference
+ * tabulates for plotting on three cut planes of box `box`: the xy plane
+ * k = limit/2+1, the xz plane j = limit+1 and the yz plane i = limit+1
+ * (limit+1-15 for box 5), points in the reference's loop order.  For each
+ * point 53 values, in its physical units and column order: plas (log10
+ * densities per cm^3, temperatures) 8, flow (species and net velocities,
+ * Alfven Mach number) 13, pres (log10 parallel / perpendicular / cross and
+ * electron pressures) 10, bande (B, E, J) 11, model (position, perturbation
+ * and dipole B) 11.  `out` (device) holds (nx*ny + nx*nz + ny*nz) * 53
+ * floats.  The file writing and the plotting call stay with the caller
+ * (g3m3_amd.graphing writes the reference's files). */
+void g3m3_graphing_cuts(
+    const float* qrho, const float* qpx, const float* qpy, const float* qpz, const float* qpresx,
+    const float* qpresy, const float* qpresz, const float* hrho, const float* hpx,
+    const float* hpy, const float* hpz, const float* hpresx, const float* hpresy,
+    const float* hpresz, const float* orho, const float* opx, const float* opy,
+    const float* opz, const float* opresx, const float* opresy, const float* opresz,
+    const float* bx, const float* by, const float* bz, const float* epres, const float* bx0,
+    const float* by0, const float* bz0, const float* resistive, const int* ijzero,
+    const int* numzero, const int* ijmid, const int* nummid, const int* nx, const int* ny,
+    const int* nz, const int* n_grids, const int* box, const int* mbndry, const int* mmid,
+    const int* mzero, const float* limit, const float* xspac, const float* grid_minvals,
+    const float* rmassq, const float* rmassh, const float* rmasso, const float* reynolds,
+    const float* resist, const float* smallbit, const float* re_equiv, const float* b_equiv,
+    const float* v_equiv, const float* rho_equiv, const float* planet_rad, float* out);
+
 /* ---- initialisation -----------------------------------------------------------
  * common /rotation/ (code/3d_main.f90:633-634), which the reference's
  * dipole and mak_dip_grd read, comes in as the trailing arguments rot_angle,

@@ -44,7 +44,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 BIN = os.path.join(ROOT, "oracle", "_ref", "g3m3_main")
 RUNS = os.path.join(ROOT, "oracle", "_ref", "main_run")
-INPUTS = ("pin_base", "pin_tilt_torus")
+INPUTS = ("pin_base", "pin_tilt_torus", "pin_graph")
 
 NX, NY, NZ, NGRD = 121, 121, 61, 5        # the program's compile-time grid
 V = NX * NY * NZ
@@ -107,6 +107,24 @@ def digest_fluid(path: str) -> dict:
     return {"t": rows[0]["value"], "records": rows}
 
 
+def digest_graph(path: str, every: int = 211) -> dict:
+    """A plotting table (code/sub_write_graphing_data.f90:278-760): its
+    SHA-256 and line count, the header and column-title lines verbatim, and
+    every `every`-th data row (zlib-compressed text)."""
+    import zlib
+    with open(path, "rb") as f:
+        raw = f.read()
+    lines = raw.decode().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    head = 6                                     # 5 header lines and the titles
+    rows = {str(r): lines[head + r] for r in range(0, len(lines) - head, every)}
+    blob = zlib.compress(json.dumps(rows).encode(), 9)
+    return {"sha256": hashlib.sha256(raw).hexdigest(), "lines": len(lines),
+            "head": lines[:head], "every": every,
+            "rows": base64.b64encode(blob).decode()}
+
+
 _NUM = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[EeDd][-+]?\d+)?"
 
 
@@ -144,6 +162,7 @@ def run_one(name: str) -> dict:
         shutil.rmtree(wd)
     os.makedirs(wd)
     shutil.copy(src, os.path.join(wd, "input"))
+    os.makedirs(os.path.join(wd, "figures", "data"))     # write_graphing_data's directory
     env = dict(os.environ, OMP_NUM_THREADS="1", GIT_AUTHOR_NAME="g", GIT_AUTHOR_EMAIL="g@g",
                GIT_COMMITTER_NAME="g", GIT_COMMITTER_EMAIL="g@g")
     for cmd in (["git", "init", "-q", "."], ["git", "add", "input"],
@@ -161,6 +180,13 @@ def run_one(name: str) -> dict:
         print(f"  digesting {f}", flush=True)
         out["fluid"][f] = digest_fluid(os.path.join(wd, f))
         os.remove(os.path.join(wd, f))
+    gdir = os.path.join(wd, "figures", "data")
+    graphs = sorted(f for f in os.listdir(gdir) if f.endswith(".dat"))
+    if graphs:
+        out["graphing"] = {}
+        for f in graphs:
+            out["graphing"][f] = digest_graph(os.path.join(gdir, f))
+            os.remove(os.path.join(gdir, f))
     out["steps"] = parse_speeds(os.path.join(wd, "speeds.dat"))
     out["fluxes"] = parse_fluxes(os.path.join(wd, "fluxes.dat"))
     return out

@@ -227,3 +227,46 @@ def digest_records(recs, dims) -> dict:
             row["samples"] = base64.b64encode(a[idx].astype("<f4").tobytes()).decode()
         rows.append(row)
     return {"t": rows[0]["value"], "records": rows}
+
+
+def compare_graphs(data_dir: str, gold: dict, rel: float = 2e-6) -> dict:
+    """Plotting tables under data_dir against the reference's
+    (tests/golden/main_pin_graph.json): the same files, line counts and
+    header lines, and at the sampled rows every printed value within rel
+    of the reference's (the device's log10 and a few others may move the
+    7th printed digit).  Returns counts and the problems found."""
+    import re
+    import zlib
+    want = gold["graphing"]
+    have = sorted(f for f in os.listdir(data_dir) if f.endswith(".dat"))
+    out = {"files": len(have), "values": 0, "text_equal": 0, "problems": []}
+    if have != sorted(want):
+        out["problems"].append(f"files differ: {sorted(set(have) ^ set(want))[:5]}")
+        return out
+    for name in have:
+        w = want[name]
+        lines = open(os.path.join(data_dir, name)).read().split("\n")
+        if lines and lines[-1] == "":
+            lines.pop()
+        if len(lines) != w["lines"]:
+            out["problems"].append(f"{name}: {len(lines)} lines, reference {w['lines']}")
+            continue
+        if lines[:len(w["head"])] != w["head"]:
+            out["problems"].append(f"{name}: header differs")
+        rows = json.loads(zlib.decompress(base64.b64decode(w["rows"])))
+        head = len(w["head"])
+        for r, text in rows.items():
+            got = lines[head + int(r)]
+            wt, gt = text.split(), got.split()
+            if len(wt) != len(gt):
+                out["problems"].append(f"{name} row {r}: column count")
+                continue
+            for a, b in zip(gt, wt):
+                out["values"] += 1
+                if a == b:
+                    out["text_equal"] += 1
+                    continue
+                x, y = float(a), float(b)
+                if not (abs(x - y) <= rel * abs(y) or (np.isnan(x) and np.isnan(y))):
+                    out["problems"].append(f"{name} row {r}: {a} vs {b}")
+    return out

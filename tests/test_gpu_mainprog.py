@@ -78,3 +78,23 @@ def test_main_program_tilting_torus(tmp_path):
     assert not compare_fluxes(m.flux_log, dev), compare_fluxes(m.flux_log, dev)
     assert not compare_steps(m.step_log, gold, rel=1e-6), compare_steps(m.step_log, gold, 1e-6)
     assert not compare_fluxes(m.flux_log, gold, rel=1e-4), compare_fluxes(m.flux_log, gold, 1e-4)
+
+
+def test_main_program_plotting_tables(tmp_path):
+    """pin_graph (pin_base with the plotting output on, at the end of the
+    second step): the 75 plotting tables (5 boxes x 3 cut planes x plas,
+    flow, pres, bande, model) have the reference's names, line counts and
+    header lines, and every value at the sampled rows is within 2e-6 of the
+    reference's printed value (the device evaluates log10 in double and
+    rounds once where the reference calls the C library's log10f, which can
+    move the seventh printed digit)."""
+    from mainpin import compare_graphs
+    gold = golden("pin_graph")
+    m = _run("pin_graph", tmp_path)
+    assert len(m.graph_log) == 1 and not m.fluid_files
+    assert not compare_steps(m.step_log, gold), compare_steps(m.step_log, gold)
+    res = compare_graphs(str(tmp_path / "figures" / "data"), gold)
+    print(f"plotting tables: {res['files']} files, {res['text_equal']} of {res['values']} "
+          f"sampled values print identically", flush=True)
+    assert not res["problems"], res["problems"][:8]
+    assert res["files"] == 75 and res["values"] > 0
