@@ -1544,6 +1544,23 @@ void g3m3_sample_fields(const int* npts, const float* sxyz, const int* n_grids,
     check_launch("sample_fields");
 }
 
+void g3m3_sample_fields_adjoint(const int* npts, const float* sxyz, const int* n_grids,
+                                const float* grid_minvals, const float* grid_maxvals,
+                                const float* xspac, const float* re_equiv, const int* nx,
+                                const int* ny, const int* nz, const int* nq,
+                                float* const* fields_bar, const float* scdata_bar) {
+    CraftGrid G;
+    if (!craft_grid(G, n_grids, grid_minvals, grid_maxvals, xspac, re_equiv)) return;
+    if (*nq < 0 || *nq > kMaxFields) {
+        latch(4, "sample_fields_adjoint: nq out of range");
+        return;
+    }
+    FieldPtrs F{};
+    for (int q = 0; q < *nq; ++q) F.p[q] = R(fields_bar[q]);
+    launch_adj_sample(*npts, R(sxyz), G, *nx, *ny, *nz, *nq, F, R(scdata_bar), g_stream);
+    check_launch("sample_fields_adjoint");
+}
+
 void g3m3_sample_craft(const int* ncraft, const float* xcraft, const float* zcraft,
                        const float* ut, const int* n_grids, const float* grid_minvals,
                        const float* grid_maxvals, const float* xspac, const float* re_equiv,
@@ -1645,6 +1662,7 @@ G3M3_FORTRAN_ALIAS(fnd_pres)
 G3M3_FORTRAN_ALIAS(sparse_solver)
 G3M3_FORTRAN_ALIAS(findgrid)
 G3M3_FORTRAN_ALIAS(sample_fields)
+G3M3_FORTRAN_ALIAS(sample_fields_adjoint)
 G3M3_FORTRAN_ALIAS(sample_craft)
 G3M3_FORTRAN_ALIAS(find_aux_cube)
 G3M3_FORTRAN_ALIAS(flux_counter)

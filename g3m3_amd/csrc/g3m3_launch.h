@@ -133,6 +133,28 @@ void launch_push_elec(const Box& B, const float* w, const float* old, float* ep,
 void launch_push_bfld(const Box& B, float* b[3], const float* old[3], const float* e[3], float delt,
                       float rx, float ry, float rz, hipStream_t s);
 void launch_copy_fields(int n, int nf, const FieldPtrs& dst, const FieldPtrs& src, hipStream_t s);
+// adjoints of the field half of a box-step (k_adjoint.hip); box-local arrays
+void launch_adj_push_bfld(const Box& B, const float* bb[3], float* ob[3], float* eb[3],
+                          float delt, float rx, float ry, float rz, hipStream_t s);
+void launch_adj_bande(const Box& B, float* eb[3], const float* bs[3], const float* ev[3],
+                      const float* cur[3], const float* rho[3], float rmq, float rmh, float rmo,
+                      float eta, float* bsb[3], float* evb[3], float* curb[3], double* etab,
+                      hipStream_t s);
+void launch_adj_evel(const Box& B, const float* evb[3], const float* rho[3], float rmq, float rmh,
+                     float rmo, float reynolds, float* curb[3], hipStream_t s);
+void launch_adj_calcur(const Box& B, float* curb[3], float* bb[3], float rx, float ry, float rz,
+                       hipStream_t s);
+void launch_adj_outer_b(const Box& B, float* bb[3], double* sbb, hipStream_t s);
+void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float chi, float delt,
+                    float* qb[3], float* w1b[3], hipStream_t s);
+size_t adj_fct_scratch_floats(const Box& B);
+void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
+                    float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s);
+void launch_add3(int n, const float* x[3], float* y[3], hipStream_t s);
+struct CraftGrid;
+void launch_adj_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
+                       int nq, const FieldPtrs& Fb, const float* sb, hipStream_t s);
+
 // write_graphing_data's cut-plane quantities (k_graph_cuts in k_misc.hip)
 struct GraphCuts {
     Box B;

@@ -736,6 +736,52 @@ void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, doub
  * to speeds.dat).  Fills at most cap entries; returns how many there were. */
 int g3m3_run_step_log(void* run, int cap, int* box, double* t_old, double* t_new);
 
+/* ---- adjoint of the field update ------------------------------------------
+ * No reference counterpart: the magnetic-field half of a box-step on one
+ * outermost box (box = n_grids = 1, no planet lists) with the plasma held
+ * fixed, and its discrete adjoint, for gradients of a misfit at virtual
+ * spacecraft with respect to the upstream field and a uniform resistivity.
+ * One step, from b = (bx, by, bz) (three contiguous device arrays of
+ * nx*ny*nz cells), follows the field calls of code/3d_main.f90: calcur,
+ * total B, fnd_evel and BandE (plus eta J / n on the interior, after the
+ * list term) from b (:2243-2270), push_bfld with delt/2 from old = b
+ * (:2324) and bndry_outer (:2331); the same from that field with delt
+ * (:2486-2565, 2576); lap_bfld (:2765) over the stage-1 field and
+ * bndry_outer (:2774); fcsmooth with diferg (code/sub_flux_correct.f90:183)
+ * and bndry_outer (:2932).  The fixed plasma gives fnd_evel's ion velocity
+ * and lap_bfld's velocity (fnd_vtot).  ctl = {eta, sbx_wind, sby_wind,
+ * sbz_wind} (host).  Device arrays; the descriptor's pointers are kept. */
+typedef struct {
+    int nx, ny, nz;
+    const float* rho[3];      /* qrho, hrho, orho */
+    const float* mom[9];      /* qpx, qpy, qpz, hpx, hpy, hpz, opx, opy, opz */
+    const float* epres;
+    const float* b0[3];       /* bx0, by0, bz0 */
+    float rx, delt, rmassq, rmassh, rmasso, reynolds, smallbit, chierg, diferg;
+} g3m3_field_model_desc;
+void* g3m3_field_model_create(const g3m3_field_model_desc* desc);
+void g3m3_field_model_destroy(void* model);
+void g3m3_field_model_set_stream(void* model, void* hip_stream);
+/* floats of one step's tape (24 arrays of nx*ny*nz) */
+long long g3m3_field_model_tape_floats(void* model);
+/* b_out = one step from b_in (distinct arrays); tape (or NULL) receives
+ * what the adjoint of the step needs.  Returns 0 on success. */
+int g3m3_field_model_step(void* model, const float* b_in, float* b_out, float* tape,
+                          const float* ctl);
+/* From the step's tape and the adjoint of its output: bin_bar = the adjoint
+ * of its input (assigned); ctl_bar[0..3] (device doubles) += the adjoints
+ * of eta and sb.  bout_bar is not changed.  Returns 0 on success. */
+int g3m3_field_model_adjoint(void* model, const float* tape, const float* bout_bar,
+                             float* bin_bar, double* ctl_bar, const float* ctl);
+/* The adjoint of g3m3_sample_fields (code/cft_trilin_interp.f90:10-66 for
+ * each point): fields_bar[q] += the trilinear weights of point p's cube
+ * times scdata_bar(q,p).  Points outside every box add nothing. */
+void g3m3_sample_fields_adjoint(const int* npts, const float* sxyz, const int* n_grids,
+                                const float* grid_minvals, const float* grid_maxvals,
+                                const float* xspac, const float* re_equiv, const int* nx,
+                                const int* ny, const int* nz, const int* nq,
+                                float* const* fields_bar, const float* scdata_bar);
+
 /* ---- Fortran implicit-interface names ------------------------------------------
  * A Fortran call with no interface block (the reference's own style) names
  * g3m3_<name>_ and passes everything by reference.  Every subroutine entry
