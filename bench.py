@@ -181,6 +181,57 @@ def gradient_case():
             "gradient_runs_one_at_a_time_s": round(secs1, 4)}
 
 
+def adjoint_case(steps: int = 50, every: int = 10, ckpt: int = 10, ncraft: int = 8):
+    """The checkpointed adjoint of the field update (g3m3_amd.adjoint) on one
+    121x121x61 box upstream of the planet: a craft misfit over ``steps``
+    steps against a target run with other controls plus 5% noise, its gradient in
+    (eta, sbx, sby, sbz) and in the initial field, and the wall time of the
+    forward rollout and of the backward pass (recompute from checkpoints
+    every ``ckpt`` steps + adjoint kernels)."""
+    import torch
+    from g3m3_amd import adjoint as A
+    s = A.field_setup(nx=121, ny=121, nz=61, spacing=0.5)
+    m = A.FieldModel(s)
+    pos = torch.from_numpy(A.craft_positions(s, ncraft, seed=12)).cuda()
+    ctl_true = torch.tensor([1e-3, 0.04, 0.02, -0.20], dtype=torch.float64)
+    ctl0 = torch.tensor([2e-3, 0.0, 0.0, -0.24], dtype=torch.float64)
+    with torch.no_grad():
+        target = A.rollout(ctl_true, m.new_state(), m, steps, every, pos).clone()
+    # observation noise: 5% of the samples' rms (seeded)
+    gen = torch.Generator().manual_seed(13)
+    rms = float(target.pow(2).mean().sqrt())
+    target += (0.05 * rms) * torch.randn(target.shape, generator=gen).cuda()
+
+    def grad():
+        c = ctl0.clone().requires_grad_(True)
+        b0 = m.new_state().requires_grad_(True)
+        out = A.rollout(c, b0, m, steps, every, pos, ckpt)
+        L = ((out - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        return float(L.detach()), c.grad.numpy(), time.perf_counter() - t0
+
+    grad()                                   # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        A.rollout(ctl0, m.new_state(), m, steps, every, pos)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    L, g, bwd = grad()
+    m.close()
+    return {"config": f"one 121x121x61 box, frozen plasma, {steps} steps, {ncraft} craft "
+                      f"sampling b every {every} steps, checkpoints every {ckpt} steps",
+            "controls": dict(zip(A.CTL_NAMES, ctl0.tolist())),
+            "target_controls": dict(zip(A.CTL_NAMES, ctl_true.tolist())),
+            "loss": L, "gradient": dict(zip(A.CTL_NAMES, g.tolist())),
+            "forward_ms": round(fwd * 1e3, 2), "backward_ms": round(bwd * 1e3, 2),
+            "backward_over_forward": round(bwd / fwd, 2),
+            "method": "hand-written adjoint kernels, recompute from checkpoints"}
+
+
 def precision_case(fast_mode: int):
     """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
     FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
@@ -326,6 +377,9 @@ def main(argv=None):
     ap.add_argument("--no-gradient", action="store_true",
                     help="skip the differentiation case (3 nested 81x81x41 boxes, 50 steps, "
                          "craft misfit and its gradient)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the field-update adjoint leg (checkpointed adjoint of a "
+                         "craft misfit on one 121x121x61 box)")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -426,6 +480,7 @@ def main(argv=None):
     prec = None if a.no_precision else precision_case(a.divb_mode)
     batch = None if a.no_batch else batch_case()
     grad = None if a.no_gradient else gradient_case()
+    adj = None if a.no_adjoint else adjoint_case()
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -451,6 +506,8 @@ def main(argv=None):
         out["batch"] = batch
     if grad is not None:
         out["gradient"] = grad
+    if adj is not None:
+        out["adjoint"] = adj
     print(json.dumps(out))
     return 0
 
