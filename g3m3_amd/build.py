@@ -11,7 +11,9 @@ g3m3_amd/libg3m3.so.  Flags:
 
 When flang is present the Fortran driver fragments g3m3_amd/fortran/
 drive_fields.f90 and drive_bounds.f90 (reference-style calls through
-implicit interfaces) are linked against the library beside their sources.
+implicit interfaces) and drive_iface.f90 (the same calls through the bind(c)
+interface module g3m3_iface.f90) are linked against the library beside
+their sources.
 """
 from __future__ import annotations
 
@@ -66,12 +68,14 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 
 def _build_fortran(verbose: bool) -> None:
-    for name in ("drive_fields", "drive_bounds"):
+    iface = PKG / "fortran" / "g3m3_iface.f90"
+    for name in ("drive_fields", "drive_bounds", "drive_iface"):
         src = PKG / "fortran" / (name + ".f90")
         exe = src.with_suffix("")
         if not (src.exists() and os.access(FLANG, os.X_OK)):
             continue
-        cmd = [FLANG, "-O2", str(src), "-L", str(PKG), "-lg3m3", "-Wl,-rpath,$ORIGIN/..",
+        srcs = [str(iface), str(src)] if name == "drive_iface" else [str(src)]
+        cmd = [FLANG, "-O2", *srcs, "-L", str(PKG), "-lg3m3", "-Wl,-rpath,$ORIGIN/..",
                "-o", str(exe)]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(OBJ))
         if r.returncode != 0:

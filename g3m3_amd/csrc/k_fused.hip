@@ -1502,42 +1502,50 @@ void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, floa
                        slab);
 }
 
+// Fields per thread: each thread reads its 32-byte table entry once for a
+// group of fields (the table is ~3.6 MB at 121x121x61; read once per field
+// it was most of this kernel's HBM traffic).
+constexpr int kFlankGroup = 9;
+
 __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict__ tab, int npts,
-                                                      FlankPtrs Fp, float t1, float t2) {
+                                                      FlankPtrs Fp, int nf, float t1, float t2) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= npts) return;
-    const int f = blockIdx.y;
     const ShellPt p = tab[t];
     const int X = 1, Y = p.sy, Z = p.sz;
     const float dx = p.dx, dy = p.dy, dz = p.dz;
     const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
-    const float* __restrict__ o = Fp.old[f];
-    const float* __restrict__ q = Fp.cur[f];
     const int c0 = p.cb;
-    auto W = [&](int c) { return o[c] * t1 + q[c] * t2; };
-    const float I0 = W(c0) * ddx * ddy * ddz + W(c0 + Z) * ddx * ddy * dz +
-                     W(c0 + Y) * ddx * dy * ddz + W(c0 + Y + Z) * ddx * dy * dz +
-                     W(c0 + X) * dx * ddy * ddz + W(c0 + X + Z) * dx * ddy * dz +
-                     W(c0 + X + Y) * dx * dy * ddz + W(c0 + X + Y + Z) * dx * dy * dz;
-    float* tgt = Fp.tgt[f];
-    float w = tgt[p.c];
+    const int f0 = blockIdx.y * kFlankGroup, f1 = min(f0 + kFlankGroup, nf);
+    for (int f = f0; f < f1; ++f) {
+        const float* __restrict__ o = Fp.old[f];
+        const float* __restrict__ q = Fp.cur[f];
+        auto W = [&](int c) { return o[c] * t1 + q[c] * t2; };
+        const float I0 = W(c0) * ddx * ddy * ddz + W(c0 + Z) * ddx * ddy * dz +
+                         W(c0 + Y) * ddx * dy * ddz + W(c0 + Y + Z) * ddx * dy * dz +
+                         W(c0 + X) * dx * ddy * ddz + W(c0 + X + Z) * dx * ddy * dz +
+                         W(c0 + X + Y) * dx * dy * ddz + W(c0 + X + Y + Z) * dx * dy * dz;
+        float* tgt = Fp.tgt[f];
+        float w = tgt[p.c];
 #pragma unroll
-    for (int b = 0; b < 6; ++b) {   // bottom, top, front, back, left, right
-        if (p.flags & (1 << b)) {
-            const float frac = (p.flags & (256 << b)) ? 0.5f : 0.0f;
-            const float frac1 = 1.0f - frac;
-            w = frac * w + frac1 * I0;
+        for (int b = 0; b < 6; ++b) {   // bottom, top, front, back, left, right
+            if (p.flags & (1 << b)) {
+                const float frac = (p.flags & (256 << b)) ? 0.5f : 0.0f;
+                const float frac1 = 1.0f - frac;
+                w = frac * w + frac1 * I0;
+            }
         }
+        tgt[p.c] = w;
     }
-    tgt[p.c] = w;
 }
 
 void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const FlankPtrs& Fp, int nf,
                          float t1, float t2, hipStream_t s) {
     if (npts <= 0 || nf <= 0) return;
     ProfScope prof_("k_flanks_table", s);
-    hipLaunchKernelGGL(k_flanks_table, dim3((npts + 255) / 256, nf), dim3(256), 0, s, tab, npts,
-                       Fp, t1, t2);
+    const int ng = (nf + kFlankGroup - 1) / kFlankGroup;
+    hipLaunchKernelGGL(k_flanks_table, dim3((npts + 255) / 256, ng), dim3(256), 0, s, tab, npts,
+                       Fp, nf, t1, t2);
 }
 
 

@@ -2,7 +2,9 @@
 (g3m3_amd/fortran/drive_fields.f90) calls the library through implicit
 interfaces on registered host arrays -- calcur, B total, fnd_evel, BandE,
 push_elec, push_bfld in the order of code/3d_main.f90:2476-2590 -- and its
-results must equal the oracle's on the same inputs, bit for bit."""
+results must equal the oracle's on the same inputs, bit for bit.  The same
+fragment through the bind(c) interface module (drive_iface.f90 with
+g3m3_iface.f90) must write the same bytes."""
 import os
 import subprocess
 
@@ -18,6 +20,7 @@ pytestmark = pytest.mark.gpu
 F = np.float32
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "g3m3_amd",
                    "fortran", "drive_fields")
+EXE_IFACE = EXE.replace("drive_fields", "drive_iface")
 
 
 def test_fortran_driver_fragment(tmp_path):
@@ -53,6 +56,14 @@ def test_fortran_driver_fragment(tmp_path):
     r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, F)
+    # the same calls through the bind(c) interface module: the same bytes
+    if os.path.exists(EXE_IFACE):
+        fout2 = tmp_path / "out_iface.bin"
+        print(f"running {EXE_IFACE}", flush=True)
+        r = subprocess.run([EXE_IFACE, str(fin), str(fout2)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert fout2.read_bytes() == fout.read_bytes()
     n3 = nx * ny * nz
     outs3 = ["curx", "cury", "curz", "bsx", "bsy", "bsz", "btot", "evx", "evy", "evz",
              "tvx", "tvy", "tvz", "efldx", "efldy", "efldz"]
