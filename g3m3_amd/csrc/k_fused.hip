@@ -466,14 +466,18 @@ __global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx,
     __shared__ MarchLds L;
     const Box B = P.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
+    // logical block order: species fastest, so the blocks of one tile's
+    // species are neighbours in one XCD's run of blocks and share the
+    // tile's b_s, E, tv and btot reads through that XCD's L2
     int t = xcd_block(blockIdx.x, gridDim.x);
+    const int s = t % P.nspecies;
+    t /= P.nspecies;
     const int tx = t % ntx;
     t /= ntx;
     const int ty = t % nty;
     t /= nty;
-    const int kc = t % nkc;
-    const int s = t / nkc;
-    if (s >= P.nspecies) return;             // grid padding: whole blocks
+    const int kc = t;
+    if (kc >= nkc) return;                   // grid padding: whole blocks
     const int l = threadIdx.x, w = threadIdx.y;
     const int i = tx * MOX + l, j0 = 1 + ty * MTY, j = j0 + w;
     const int ic = min(i, nx - 1), jc = min(j, ny - 1);

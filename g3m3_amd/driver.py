@@ -156,8 +156,10 @@ class Run:
         self.call("set_speed_agrd", sig.set_speed(pre))
 
     def _store_old(self, b):
-        # store_array x34 (code/3d_main.f90:2193-2229): one copy of box b
-        self.sets["old"][:, b].copy_(self.sets[""][:, b])
+        # store_array x34 (code/3d_main.f90:2193-2229): old(:,:,:,box) =
+        # state(:,:,:,box), field by field, through the library's entry point
+        for k in FIELDS:
+            self.call("store_array", ["old" + k, k, "nx", "ny", "nz", "n_grids", "box"])
 
     # -- code/3d_main.f90:2163-3065 ---------------------------------------------------
     def box_step(self, box):

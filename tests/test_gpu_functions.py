@@ -649,3 +649,45 @@ def test_banded_tpose(dev):
         _lib.call("banded_tpose", 7, npts, pos, dM, dx, dy)
         _lib.sync()
         assert same_bits(dy.cpu().numpy(), want)
+
+
+def test_store_array(dev, world):
+    """code/sub_store_array.f90: bx(:,:,:,box) = bx0(:,:,:,box), the other
+    boxes untouched."""
+    names = ["bx", "bx0", "nx", "ny", "nz", "n_grids", "box"]
+    wg, _ = gpu_call(dev, world, "store_array", names)
+    assert_bits(wg, oracle_call(world, "store_array", names), ["bx"])
+
+
+def test_fcsmooth_single_field(dev, world):
+    """code/sub_fcsmooth.f90:7-242 on one field (the single-field entry point,
+    beside flux_correct's all-field one)."""
+    world.a["tmpx"] = np.zeros((world.nz, world.ny, world.nx), np.float32)
+    world.a["tmpy"] = np.zeros_like(world.a["tmpx"])
+    world.a["tmpz"] = np.zeros_like(world.a["tmpx"])
+    for field, chi in (("qrho", np.float32(0.008)), ("bz", np.float32(0.004))):
+        names = [field, "old" + field, "wrk" + field, "nx", "ny", "nz", "n_grids", "box",
+                 chi, "tmpx", "tmpy", "tmpz"]
+        wg, _ = gpu_call(dev, world, "fcsmooth", names)
+        wo = oracle_call(world, "fcsmooth", names)
+        assert_bits(wg, wo, [field])
+
+
+@pytest.mark.parametrize("box", [1, 2])
+def test_flanks_single_field(dev, nested, box):
+    """code/sub_flanks.f90:5-303 on one field (g3m3_flanks), target wrk,
+    coarse values from the next box's current and old arrays."""
+    from g3m3_amd import _lib
+    nested.s["box"] = box
+    t = 0.17
+    for field in ("qrho", "bx", "opresyz"):
+        d = to_dev(dev, nested)
+        _lib.call("flanks", d["wrk" + field], d[field], d["old" + field], nested.nx,
+                  nested.ny, nested.nz, nested.ngrd, box, nested.a["t_new"],
+                  nested.a["t_old"], _lib.F64(t), d["work"], *[nested.a[g] for g in GRID])
+        _lib.sync()
+        wo = nested.copy()
+        O.flanks(*wo.args(["wrk" + field, field, "old" + field, "nx", "ny", "nz", "n_grids",
+                           "box", "t_new", "t_old"]), t, wo.a["work"],
+                 *[wo.a[g] for g in GRID])
+        assert same_bits(d["wrk" + field].cpu().numpy(), wo.a["wrk" + field]), field
