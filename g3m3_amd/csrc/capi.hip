@@ -1561,6 +1561,99 @@ void g3m3_sample_fields_adjoint(const int* npts, const float* sxyz, const int* n
     check_launch("sample_fields_adjoint");
 }
 
+void g3m3_push_ion_adjoint(const float* const* src, const float* bsx, const float* bsy,
+                           const float* bsz, const float* btot, const float* efldx,
+                           const float* efldy, const float* efldz, const float* tvx,
+                           const float* tvy, const float* tvz, const float* const* out_bar,
+                           float* const* src_bar, float* const* old_bar, float* bsx_bar,
+                           float* bsy_bar, float* bsz_bar, float* btot_bar, float* efldx_bar,
+                           float* efldy_bar, float* efldz_bar, float* tvx_bar, float* tvy_bar,
+                           float* tvz_bar, const int* nx, const int* ny, const int* nz,
+                           const float* rmass, const float* delt, const float* grav,
+                           const float* re_equiv, const float* reynolds, const float* grd_xmin,
+                           const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                           const float* grd_zmin, const float* grd_zmax, float* work) {
+    IonAdj A{};
+    A.B = box_of(nx, ny, nz);
+    for (int f = 0; f < 10; ++f) {
+        A.src[f] = R(src[f]);
+        A.ob[f] = R(out_bar[f]);
+        A.sb[f] = R(src_bar[f]);
+        A.oldb[f] = R(old_bar[f]);
+    }
+    const float* bs[3] = {bsx, bsy, bsz};
+    const float* e[3] = {efldx, efldy, efldz};
+    const float* tv[3] = {tvx, tvy, tvz};
+    float* bsb[3] = {bsx_bar, bsy_bar, bsz_bar};
+    float* eb[3] = {efldx_bar, efldy_bar, efldz_bar};
+    float* tvb[3] = {tvx_bar, tvy_bar, tvz_bar};
+    for (int a = 0; a < 3; ++a) {
+        A.bs[a] = R(bs[a]);
+        A.e[a] = R(e[a]);
+        A.tv[a] = R(tv[a]);
+        A.bsb[a] = R(bsb[a]);
+        A.eb[a] = R(eb[a]);
+        A.tvb[a] = R(tvb[a]);
+    }
+    A.btot = R(btot);
+    A.btotb = R(btot_bar);
+    A.scr = R(work);
+    A.rmass = *rmass;
+    A.delt = *delt;
+    A.grav = *grav;
+    A.re_equiv = *re_equiv;
+    A.reynolds = *reynolds;
+    A.xmin = *grd_xmin;
+    A.ymin = *grd_ymin;
+    A.zmin = *grd_zmin;
+    A.ddx = (*grd_xmax - *grd_xmin) / (float)(*nx - 1);
+    A.ddy = (*grd_ymax - *grd_ymin) / (float)(*ny - 1);
+    A.ddz = (*grd_zmax - *grd_zmin) / (float)(*nz - 1);
+    A.dxt = 2.0f * A.ddx;
+    A.dyt = 2.0f * A.ddy;
+    A.dzt = 2.0f * A.ddz;
+    A.bmax = 0.20f * A.rmass / A.delt / A.reynolds;
+    launch_adj_push_ion(A, g_stream);
+    check_launch("push_ion_adjoint");
+}
+
+void g3m3_lap_plasma_adjoint(const float* const* q, const float* vx, const float* vy,
+                             const float* vz, const float* const* w_bar, float* const* q_bar,
+                             float* vx_bar, float* vy_bar, float* vz_bar, const int* nx,
+                             const int* ny, const int* nz, const float* chirho,
+                             const float* chipxyz, const float* delt, float* work) {
+    LapAdj L{};
+    L.B = box_of(nx, ny, nz);
+    for (int f = 0; f < 10; ++f) {
+        L.q[f] = R(q[f]);
+        L.wb[f] = R(w_bar[f]);
+        L.qb[f] = R(q_bar[f]);
+        L.chi[f] = f == 0 ? *chirho : *chipxyz;
+    }
+    L.v[0] = R(vx);
+    L.v[1] = R(vy);
+    L.v[2] = R(vz);
+    L.vb[0] = R(vx_bar);
+    L.vb[1] = R(vy_bar);
+    L.vb[2] = R(vz_bar);
+    L.delt = *delt;
+    L.scr = R(work);
+    launch_adj_lap_species(L, g_stream);
+    check_launch("lap_plasma_adjoint");
+}
+
+void g3m3_species_velocity_adjoint(const float* rho, const float* px, const float* py,
+                                   const float* pz, const float* vx_bar, const float* vy_bar,
+                                   const float* vz_bar, float* rho_bar, float* px_bar,
+                                   float* py_bar, float* pz_bar, const int* n,
+                                   const float* smallbit) {
+    const float* p[3] = {R(px), R(py), R(pz)};
+    const float* vb[3] = {R(vx_bar), R(vy_bar), R(vz_bar)};
+    float* pb[3] = {R(px_bar), R(py_bar), R(pz_bar)};
+    launch_adj_velocity(*n, R(rho), p, vb, *smallbit, R(rho_bar), pb, g_stream);
+    check_launch("species_velocity_adjoint");
+}
+
 void g3m3_sample_craft(const int* ncraft, const float* xcraft, const float* zcraft,
                        const float* ut, const int* n_grids, const float* grid_minvals,
                        const float* grid_maxvals, const float* xspac, const float* re_equiv,
@@ -1663,6 +1756,9 @@ G3M3_FORTRAN_ALIAS(sparse_solver)
 G3M3_FORTRAN_ALIAS(findgrid)
 G3M3_FORTRAN_ALIAS(sample_fields)
 G3M3_FORTRAN_ALIAS(sample_fields_adjoint)
+G3M3_FORTRAN_ALIAS(push_ion_adjoint)
+G3M3_FORTRAN_ALIAS(lap_plasma_adjoint)
+G3M3_FORTRAN_ALIAS(species_velocity_adjoint)
 G3M3_FORTRAN_ALIAS(sample_craft)
 G3M3_FORTRAN_ALIAS(find_aux_cube)
 G3M3_FORTRAN_ALIAS(flux_counter)

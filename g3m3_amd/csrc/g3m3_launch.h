@@ -155,6 +155,44 @@ struct CraftGrid;
 void launch_adj_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
                        int nq, const FieldPtrs& Fb, const float* sb, hipStream_t s);
 
+// adjoints of push_ion (one species) and of lap_plasma (k_adjoint_ion.hip);
+// box-local arrays, fields in IonPtrs order; all adjoints accumulate
+struct IonAdj {
+    Box B;
+    const float* src[10];
+    const float* bs[3];
+    const float* btot;
+    const float* e[3];
+    const float* tv[3];
+    const float* ob[10];     // adjoint of the output
+    float* sb[10];           // adjoint of the source (wrk) fields
+    float* oldb[10];         // adjoint of the old fields
+    float* bsb[3];
+    float* btotb;
+    float* eb[3];
+    float* tvb[3];
+    float* scr;              // adj_ion_scratch_floats
+    float rmass, delt, grav, re_equiv, reynolds;
+    float xmin, ymin, zmin, ddx, ddy, ddz, dxt, dyt, dzt, bmax;
+};
+struct LapAdj {
+    Box B;
+    const float* q[10];
+    const float* v[3];
+    const float* wb[10];     // adjoint of the smoothed fields
+    float* qb[10];
+    float* vb[3];
+    float chi[10];
+    float delt;
+    float* scr;              // adj_lapsp_scratch_floats
+};
+size_t adj_ion_scratch_floats(const Box& B);
+size_t adj_lapsp_scratch_floats(const Box& B);
+void launch_adj_push_ion(const IonAdj& A, hipStream_t s);
+void launch_adj_lap_species(const LapAdj& L, hipStream_t s);
+void launch_adj_velocity(int n, const float* rho, const float* p[3], const float* vb[3],
+                         float smallbit, float* rhob, float* pb[3], hipStream_t s);
+
 // write_graphing_data's cut-plane quantities (k_graph_cuts in k_misc.hip)
 struct GraphCuts {
     Box B;

@@ -1434,5 +1434,88 @@ module g3m3_iface
             type(c_ptr), intent(in) :: fields_bar(*)
             real(c_float), intent(in) :: scdata_bar(*)
         end subroutine
+        subroutine g3m3_push_ion_adjoint(src, bsx, bsy, bsz, btot, efldx, efldy, efldz, &
+            tvx, tvy, tvz, out_bar, src_bar, old_bar, bsx_bar, bsy_bar, bsz_bar, &
+            btot_bar, efldx_bar, efldy_bar, efldz_bar, tvx_bar, tvy_bar, tvz_bar, nx, &
+            ny, nz, rmass, delt, grav, re_equiv, reynolds, grd_xmin, grd_xmax, &
+            grd_ymin, grd_ymax, grd_zmin, grd_zmax, work) bind(c, name='g3m3_push_ion_adjoint')
+            import :: c_float, c_int, c_ptr
+            type(c_ptr), intent(in) :: src(*)
+            real(c_float), intent(in) :: bsx(*)
+            real(c_float), intent(in) :: bsy(*)
+            real(c_float), intent(in) :: bsz(*)
+            real(c_float), intent(in) :: btot(*)
+            real(c_float), intent(in) :: efldx(*)
+            real(c_float), intent(in) :: efldy(*)
+            real(c_float), intent(in) :: efldz(*)
+            real(c_float), intent(in) :: tvx(*)
+            real(c_float), intent(in) :: tvy(*)
+            real(c_float), intent(in) :: tvz(*)
+            type(c_ptr), intent(in) :: out_bar(*)
+            type(c_ptr), intent(in) :: src_bar(*)
+            type(c_ptr), intent(in) :: old_bar(*)
+            real(c_float), intent(inout) :: bsx_bar(*)
+            real(c_float), intent(inout) :: bsy_bar(*)
+            real(c_float), intent(inout) :: bsz_bar(*)
+            real(c_float), intent(inout) :: btot_bar(*)
+            real(c_float), intent(inout) :: efldx_bar(*)
+            real(c_float), intent(inout) :: efldy_bar(*)
+            real(c_float), intent(inout) :: efldz_bar(*)
+            real(c_float), intent(inout) :: tvx_bar(*)
+            real(c_float), intent(inout) :: tvy_bar(*)
+            real(c_float), intent(inout) :: tvz_bar(*)
+            integer(c_int), intent(in) :: nx
+            integer(c_int), intent(in) :: ny
+            integer(c_int), intent(in) :: nz
+            real(c_float), intent(in) :: rmass
+            real(c_float), intent(in) :: delt
+            real(c_float), intent(in) :: grav
+            real(c_float), intent(in) :: re_equiv
+            real(c_float), intent(in) :: reynolds
+            real(c_float), intent(in) :: grd_xmin
+            real(c_float), intent(in) :: grd_xmax
+            real(c_float), intent(in) :: grd_ymin
+            real(c_float), intent(in) :: grd_ymax
+            real(c_float), intent(in) :: grd_zmin
+            real(c_float), intent(in) :: grd_zmax
+            real(c_float), intent(inout) :: work(*)
+        end subroutine
+        subroutine g3m3_lap_plasma_adjoint(q, vx, vy, vz, w_bar, q_bar, vx_bar, vy_bar, &
+            vz_bar, nx, ny, nz, chirho, chipxyz, delt, work) bind(c, name='g3m3_lap_plasma_adjoint')
+            import :: c_float, c_int, c_ptr
+            type(c_ptr), intent(in) :: q(*)
+            real(c_float), intent(in) :: vx(*)
+            real(c_float), intent(in) :: vy(*)
+            real(c_float), intent(in) :: vz(*)
+            type(c_ptr), intent(in) :: w_bar(*)
+            type(c_ptr), intent(in) :: q_bar(*)
+            real(c_float), intent(inout) :: vx_bar(*)
+            real(c_float), intent(inout) :: vy_bar(*)
+            real(c_float), intent(inout) :: vz_bar(*)
+            integer(c_int), intent(in) :: nx
+            integer(c_int), intent(in) :: ny
+            integer(c_int), intent(in) :: nz
+            real(c_float), intent(in) :: chirho
+            real(c_float), intent(in) :: chipxyz
+            real(c_float), intent(in) :: delt
+            real(c_float), intent(inout) :: work(*)
+        end subroutine
+        subroutine g3m3_species_velocity_adjoint(rho, px, py, pz, vx_bar, vy_bar, &
+            vz_bar, rho_bar, px_bar, py_bar, pz_bar, n, smallbit) bind(c, name='g3m3_species_velocity_adjoint')
+            import :: c_float, c_int
+            real(c_float), intent(in) :: rho(*)
+            real(c_float), intent(in) :: px(*)
+            real(c_float), intent(in) :: py(*)
+            real(c_float), intent(in) :: pz(*)
+            real(c_float), intent(in) :: vx_bar(*)
+            real(c_float), intent(in) :: vy_bar(*)
+            real(c_float), intent(in) :: vz_bar(*)
+            real(c_float), intent(inout) :: rho_bar(*)
+            real(c_float), intent(inout) :: px_bar(*)
+            real(c_float), intent(inout) :: py_bar(*)
+            real(c_float), intent(inout) :: pz_bar(*)
+            integer(c_int), intent(in) :: n
+            real(c_float), intent(in) :: smallbit
+        end subroutine
     end interface
 end module g3m3_iface

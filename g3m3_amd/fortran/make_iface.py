@@ -34,6 +34,12 @@ EXTRA_ARRAYS = {
                               "scdata_bar"},
     "sample_craft": {"xcraft", "zcraft", "grid_minvals", "grid_maxvals", "xspac", "fields",
                      "sxyz", "scdata", "gridpt"},
+    "push_ion_adjoint": {"bsx", "bsy", "bsz", "btot", "efldx", "efldy", "efldz", "tvx", "tvy",
+                         "tvz", "bsx_bar", "bsy_bar", "bsz_bar", "btot_bar", "efldx_bar",
+                         "efldy_bar", "efldz_bar", "tvx_bar", "tvy_bar", "tvz_bar", "work"},
+    "lap_plasma_adjoint": {"vx", "vy", "vz", "vx_bar", "vy_bar", "vz_bar", "work"},
+    "species_velocity_adjoint": {"rho", "px", "py", "pz", "vx_bar", "vy_bar", "vz_bar",
+                                 "rho_bar", "px_bar", "py_bar", "pz_bar"},
 }
 # entry points with the same dummies as a reference subroutine
 SAME_AS = {"set_speed_agrd_noalloc": "set_speed_agrd", "bndry_flanks_v": None,

@@ -782,6 +782,48 @@ void g3m3_sample_fields_adjoint(const int* npts, const float* sxyz, const int* n
                                 const int* ny, const int* nz, const int* nq,
                                 float* const* fields_bar, const float* scdata_bar);
 
+/* No reference counterparts: the adjoints of single subroutines, for an
+ * adjoint driver that runs a step's calls backwards.  Box-local device
+ * arrays of nx*ny*nz cells; a species' ten fields are passed as host arrays
+ * of ten device pointers in the order rho, presx, presy, presz, px, py, pz,
+ * presxy, presxz, presyz.  Every adjoint ACCUMULATES (+=) into its array;
+ * the output adjoint is not changed.
+ *
+ * g3m3_push_ion_adjoint -- code/sub_push_ion.f90:4-362 for one species,
+ * transposed: from the adjoint of the ten outputs (interior cells; the
+ * boundary cells push_ion does not write carry nothing), the adjoints of
+ * the source (wrk) fields, of the old fields, of bs, btot, efld and tv.
+ * Scalars and grid extents are the box's (grd_*min/max of this box).
+ * work: 15*nx*ny*nz floats of scratch. */
+void g3m3_push_ion_adjoint(const float* const* src, const float* bsx, const float* bsy,
+                           const float* bsz, const float* btot, const float* efldx,
+                           const float* efldy, const float* efldz, const float* tvx,
+                           const float* tvy, const float* tvz, const float* const* out_bar,
+                           float* const* src_bar, float* const* old_bar, float* bsx_bar,
+                           float* bsy_bar, float* bsz_bar, float* btot_bar, float* efldx_bar,
+                           float* efldy_bar, float* efldz_bar, float* tvx_bar, float* tvy_bar,
+                           float* tvz_bar, const int* nx, const int* ny, const int* nz,
+                           const float* rmass, const float* delt, const float* grav,
+                           const float* re_equiv, const float* reynolds, const float* grd_xmin,
+                           const float* grd_xmax, const float* grd_ymin, const float* grd_ymax,
+                           const float* grd_zmin, const float* grd_zmax, float* work);
+/* code/sub_lap_plasma.f90:4-147 for one species, transposed: from the
+ * adjoint of the ten smoothed fields, the adjoints of the ten fields and of
+ * the velocity v (chirho for rho, chipxyz for the rest).  work: 6*nx*ny*nz
+ * floats. */
+void g3m3_lap_plasma_adjoint(const float* const* q, const float* vx, const float* vy,
+                             const float* vz, const float* const* w_bar, float* const* q_bar,
+                             float* vx_bar, float* vy_bar, float* vz_bar, const int* nx,
+                             const int* ny, const int* nz, const float* chirho,
+                             const float* chipxyz, const float* delt, float* work);
+/* v = p / max(rho, smallbit) (code/3d_main.f90:2698-2703), transposed over
+ * n cells. */
+void g3m3_species_velocity_adjoint(const float* rho, const float* px, const float* py,
+                                   const float* pz, const float* vx_bar, const float* vy_bar,
+                                   const float* vz_bar, float* rho_bar, float* px_bar,
+                                   float* py_bar, float* pz_bar, const int* n,
+                                   const float* smallbit);
+
 /* ---- Fortran implicit-interface names ------------------------------------------
  * A Fortran call with no interface block (the reference's own style) names
  * g3m3_<name>_ and passes everything by reference.  Every subroutine entry
