@@ -307,3 +307,258 @@ def rollout(ctl, b_start, model: FieldModel, steps: int, every: int, pos, ckpt: 
     if _FR is None:
         _FR = _function()
     return _FR.apply(ctl, b_start, model, steps, every, pos, ckpt)
+
+
+# --- the whole box-step -------------------------------------------------------
+#
+# ``g3m3_full_model_*`` (include/g3m3.h): one box-step of an outermost box
+# with every field evolving -- the ion pushes, push_elec, push_bfld,
+# Lapidus and flux correction of code/3d_main.f90:2163-3030 -- and its
+# adjoint.  The state is the 34 fields in the canonical order of
+# g3m3_amd.workload.FIELDS plus the electron pressure of the last Lapidus
+# pass (the reference's wrkepres, which the next step's push_elec and
+# stage-2 BandE read).  The controls are the resistivity and the nine
+# upstream values bndry_outer imposes on the inflow face.
+
+FULL_CTL_NAMES = ("eta", "sbx_wind", "sby_wind", "sbz_wind", "srho", "spx", "spy", "spz",
+                  "spress")
+NSTATE = 35
+#: state indices of the quantities the full model's craft sample
+FULL_SAMPLED = (0, 4, 5, 6, 30, 31, 32, 33)     # qrho, qpx, qpy, qpz, epres, bx, by, bz
+
+
+class FullModelDesc(_ct.Structure):
+    """Mirror of g3m3_full_model_desc (include/g3m3.h)."""
+    _fields_ = [("nx", _ct.c_int), ("ny", _ct.c_int), ("nz", _ct.c_int),
+                ("b0", _ct.c_void_p * 3)] + [
+        (k, _ct.c_float) for k in ("rx", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax",
+                                   "delt", "rmassq", "rmassh", "rmasso", "reynolds",
+                                   "smallbit", "gamma", "gamma1", "grav", "re_equiv",
+                                   "chirho", "chipxyz", "chierg", "difrho", "diferg",
+                                   "rho_frac", "o_conc", "ti_te")]
+
+
+@dataclass
+class FullSetup:
+    """One outermost box with all its fields, host arrays float32."""
+    nx: int
+    ny: int
+    nz: int
+    state: np.ndarray            # (35, nz, ny, nx)
+    b0: np.ndarray               # (3, nz, ny, nx)
+    ctl: np.ndarray              # (9,) FULL_CTL_NAMES
+    scal: dict                   # the float fields of FullModelDesc
+    grid: dict = field(default_factory=dict)   # grid_minvals/maxvals (1,3), xspac (1,)
+
+
+def full_setup(nx: int = 41, ny: int = 41, nz: int = 21, spacing: float = 0.5,
+               seed: int = 20261209, n_sw: float = 5.0, v_sw: float = 400.0,
+               t_sw: float = 10.0, imf=(0.0, 0.0, -5.0), eta: float = 1e-3,
+               standoff: float = 6.0, aniso: float = 0.05) -> FullSetup:
+    """The box of field_setup with every state field: each species' density
+    and flow as there (the h and o fractions of the reference's solar wind,
+    code/sub_bndry_outer.f90:50-103), diagonal pressures with +-10% smooth
+    variations, off-diagonal pressures of +-``aniso`` of the diagonal, the
+    electron pressure the ions' sum over ti_te, and the Lapidus electron
+    pressure of a previous step within +-2% of it.  The time step is a
+    quarter cell at the fastest of flow + Alfven + sound speed."""
+    fs = field_setup(nx, ny, nz, spacing, seed, n_sw, v_sw, t_sw, imf, eta, standoff)
+    rng = np.random.default_rng(seed + 1)
+    shape = (nz, ny, nx)
+    h = float(fs.scal["rx"])
+    rho_frac, o_conc, ti_te = 0.001, 0.125, 1.0
+    spress = n_sw * 1e6 * t_sw * EV / P_EQUIV_PA
+    pl = fs.plasma
+    st = np.zeros((NSTATE,) + shape, F)
+    pres_frac = {"q": 1.0, "h": rho_frac, "o": rho_frac * o_conc}
+    ptot = np.zeros(shape)
+    for si, s in enumerate("qho"):
+        base = 10 * si
+        st[base + 0] = pl[s + "rho"]
+        p = 0.5 * spress * pres_frac[s] * (1.0 + 0.1 * _modes(rng, shape, h))
+        for m in (1, 2, 3):
+            st[base + m] = p * (1.0 + 0.02 * _modes(rng, shape, h))
+        st[base + 4], st[base + 5], st[base + 6] = pl[s + "px"], pl[s + "py"], pl[s + "pz"]
+        for m in (7, 8, 9):
+            st[base + m] = aniso * p * _modes(rng, shape, h)
+        ptot += st[base + 1]
+    st[30] = (ptot / ti_te).astype(F)
+    st[31:34] = fs.b_init
+    st[34] = (st[30] * (1.0 + 0.02 * _modes(rng, shape, h))).astype(F)
+    dens = st[0].astype(np.float64)
+    v = np.sqrt(st[4].astype(np.float64) ** 2 + st[5] ** 2 + st[6] ** 2) / dens
+    btot = np.sqrt(((fs.b_init + fs.b0) ** 2).sum(0))
+    cs = np.sqrt(1.6666 * 2.0 * ptot / dens)
+    vmax = float(np.max(v + np.sqrt(btot ** 2 / dens + cs ** 2)))
+    g = fs.grid
+    mn, mx = g["grid_minvals"][0], g["grid_maxvals"][0]
+    sc = fs.scal
+    scal = dict(rx=sc["rx"], xmin=mn[0], xmax=mx[0], ymin=mn[1], ymax=mx[1], zmin=mn[2],
+                zmax=mx[2], delt=F(0.25 * h / vmax), rmassq=sc["rmassq"], rmassh=sc["rmassh"],
+                rmasso=sc["rmasso"], reynolds=sc["reynolds"], smallbit=sc["smallbit"],
+                gamma=F(1.6666), gamma1=F(0.6666), grav=F(0.0), re_equiv=F(1.0),
+                chirho=F(2.0), chipxyz=F(2.0), chierg=F(1.0), difrho=F(0.01),
+                diferg=F(0.0025), rho_frac=F(rho_frac), o_conc=F(o_conc), ti_te=F(ti_te))
+    srho = n_sw / RHO_EQUIV_CC
+    svel = v_sw / V_EQUIV_KMS
+    ctl = np.array([eta, fs.ctl[1], fs.ctl[2], fs.ctl[3], srho, srho * svel, 0.0, 0.0,
+                    spress], F)
+    return FullSetup(nx=nx, ny=ny, nz=nz, state=st, b0=fs.b0, ctl=ctl, scal=scal, grid=g)
+
+
+class FullModel:
+    """The device model of one FullSetup (g3m3_full_model_create)."""
+
+    def __init__(self, setup: FullSetup):
+        import torch
+        self.setup = setup
+        self.nx, self.ny, self.nz = setup.nx, setup.ny, setup.nz
+        self.V = setup.nx * setup.ny * setup.nz
+        self.b0 = torch.from_numpy(np.ascontiguousarray(setup.b0)).cuda()
+        d = FullModelDesc()
+        d.nx, d.ny, d.nz = setup.nx, setup.ny, setup.nz
+        for c in range(3):
+            d.b0[c] = self.b0[c].data_ptr()
+        for k, v in setup.scal.items():
+            setattr(d, k, float(v))
+        lib = _lib.lib()
+        lib.g3m3_full_model_create.restype = _ct.c_void_p
+        lib.g3m3_full_model_tape_floats.restype = _ct.c_longlong
+        lib.g3m3_full_model_tape_floats.argtypes = [_ct.c_void_p]
+        for fn in ("step", "adjoint"):
+            getattr(lib, "g3m3_full_model_" + fn).restype = _ct.c_int
+        self.h = lib.g3m3_full_model_create(_ct.byref(d))
+        if not self.h:
+            raise _lib.G3M3Error("g3m3_full_model_create failed")
+        self.tape_floats = int(lib.g3m3_full_model_tape_floats(_ct.c_void_p(self.h)))
+        self._lib = lib
+        g = setup.grid
+        self._grid = (np.ascontiguousarray(g["grid_minvals"]), np.ascontiguousarray(g["grid_maxvals"]),
+                      np.ascontiguousarray(g["xspac"]), F(g["re_equiv"]))
+
+    def close(self):
+        if self.h:
+            self._lib.g3m3_full_model_destroy(_ct.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ctl(ctl) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(ctl, dtype=F).reshape(len(FULL_CTL_NAMES)))
+
+    def new_state(self, st=None):
+        import torch
+        src = self.setup.state if st is None else st
+        return torch.from_numpy(np.ascontiguousarray(src, dtype=F)).cuda().reshape(NSTATE, self.V)
+
+    def new_tape(self):
+        import torch
+        return torch.empty(self.tape_floats, dtype=torch.float32, device="cuda")
+
+    def step(self, s_in, s_out, ctl, tape=None):
+        c = self._ctl(ctl)
+        rc = self._lib.g3m3_full_model_step(
+            _ct.c_void_p(self.h), _ct.c_void_p(s_in.data_ptr()), _ct.c_void_p(s_out.data_ptr()),
+            _ct.c_void_p(tape.data_ptr() if tape is not None else 0),
+            c.ctypes.data_as(_ct.c_void_p))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_step returned {rc}")
+
+    def adjoint(self, tape, sout_bar, sin_bar, ctl_bar, ctl):
+        c = self._ctl(ctl)
+        rc = self._lib.g3m3_full_model_adjoint(
+            _ct.c_void_p(self.h), _ct.c_void_p(tape.data_ptr()),
+            _ct.c_void_p(sout_bar.data_ptr()), _ct.c_void_p(sin_bar.data_ptr()),
+            _ct.c_void_p(ctl_bar.data_ptr()), c.ctypes.data_as(_ct.c_void_p))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_adjoint returned {rc}")
+
+    def _ptrs(self, st):
+        return (_ct.c_void_p * len(FULL_SAMPLED))(*[st[i].data_ptr() for i in FULL_SAMPLED])
+
+    def sample(self, st, pos, out, gp):
+        mn, mx, xs, re = self._grid
+        _lib.call("sample_fields", pos.shape[0], pos, 1, mn, mx, xs, re, self.nx, self.ny,
+                  self.nz, len(FULL_SAMPLED), self._ptrs(st), out, gp)
+
+    def sample_adjoint(self, pos, s_bar, st_bar):
+        mn, mx, xs, re = self._grid
+        _lib.call("sample_fields_adjoint", pos.shape[0], pos, 1, mn, mx, xs, re, self.nx,
+                  self.ny, self.nz, len(FULL_SAMPLED), self._ptrs(st_bar), s_bar)
+
+
+def _full_function():
+    import torch
+
+    class FullRollout(torch.autograd.Function):
+        """(ctl, state at step 0) -> the FULL_SAMPLED quantities at the craft
+        after every ``every``-th of ``steps`` steps, shape (steps // every,
+        npts, 8).  ctl is a float64 CPU tensor of 9 (rounded to float32 for
+        the device); the state a (35, V) float32 device tensor.  Backward:
+        the gradient in ctl (float64, accumulated in double on the device)
+        and in the state, by checkpointed recompute as FieldRollout."""
+
+        @staticmethod
+        def forward(ctx, ctl, s_start, model, steps, every, pos, ckpt):
+            c = ctl.detach().cpu().numpy()
+            ns = steps // every
+            nq = len(FULL_SAMPLED)
+            out = torch.empty((ns, pos.shape[0], nq), dtype=torch.float32, device="cuda")
+            gp = torch.empty((pos.shape[0], 4), dtype=torch.int32, device="cuda")
+            s = s_start.detach().clone()
+            ns_ = torch.empty_like(s)
+            ckpts = []
+            for t in range(steps):
+                if t % ckpt == 0:
+                    ckpts.append(s.clone())
+                model.step(s, ns_, c)
+                s, ns_ = ns_, s
+                if (t + 1) % every == 0:
+                    model.sample(s, pos, out[(t + 1) // every - 1], gp)
+            ctx.model, ctx.steps, ctx.every, ctx.ckpt = model, steps, every, ckpt
+            ctx.c, ctx.pos, ctx.ckpts = c, pos, ckpts
+            ctx.ctl_meta = (ctl.dtype, ctl.device, ctl.shape)
+            return out
+
+        @staticmethod
+        def backward(ctx, gs):
+            model, steps, every, ckpt = ctx.model, ctx.steps, ctx.every, ctx.ckpt
+            gs = gs.contiguous()
+            s_bar = torch.zeros((NSTATE, model.V), dtype=torch.float32, device="cuda")
+            tmp = torch.empty_like(s_bar)
+            ctl_bar = torch.zeros(len(FULL_CTL_NAMES), dtype=torch.float64, device="cuda")
+            tapes = [model.new_tape() for _ in range(min(ckpt, steps))]
+            for seg in reversed(range(0, steps, ckpt)):
+                end = min(seg + ckpt, steps)
+                s = ctx.ckpts[seg // ckpt].clone()
+                ns_ = torch.empty_like(s)
+                for t in range(seg, end):
+                    model.step(s, ns_, ctx.c, tapes[t - seg])
+                    s, ns_ = ns_, s
+                for t in reversed(range(seg, end)):
+                    if (t + 1) % every == 0:
+                        model.sample_adjoint(ctx.pos, gs[(t + 1) // every - 1], s_bar)
+                    model.adjoint(tapes[t - seg], s_bar, tmp, ctl_bar, ctx.c)
+                    s_bar, tmp = tmp, s_bar
+            dtype, device, shape = ctx.ctl_meta
+            g_ctl = ctl_bar.cpu().to(dtype).reshape(shape).to(device)
+            return g_ctl, s_bar, None, None, None, None, None
+
+    return FullRollout
+
+
+_FULL = None
+
+
+def full_rollout(ctl, s_start, model: FullModel, steps: int, every: int, pos, ckpt: int = 10):
+    """Craft samples of the full model's rollout, differentiable in ctl and
+    the initial state."""
+    global _FULL
+    if _FULL is None:
+        _FULL = _full_function()
+    return _FULL.apply(ctl, s_start, model, steps, every, pos, ckpt)

@@ -1,0 +1,659 @@
+// One box-step of the outermost box (box = n_grids = 1, no planet lists)
+// with every field evolving, and its discrete adjoint.
+//
+// Forward (code/3d_main.f90:2163-3030 for that box; the validation calls
+// set_rho/set_speed_agrd are left out -- on the model's states they change
+// nothing, which the parity test against the oracle's whole box-step
+// checks -- and the time step is fixed):
+//   stage 1: E-field from S (calcur, total B, fnd_evel, BandE + eta);
+//            W1 = push_ion x3 and push_bfld from S with delt/2 (W1's
+//            pressure keeps the stale WE: the reference's step-1 push_elec
+//            writes the state, which step 2 overwrites); bndry_outer(W1)
+//   stage 2: E-field from W1; S2 = push_ion x3, push_elec (source W1's
+//            pressure) and push_bfld with old = S and delt; bndry_outer(S2)
+//   Lapidus: W3 over W1 = lap_plasma per species with its velocity, and
+//            lap_elec / lap_bfld with the total ion velocity; bndry_outer(W3)
+//   FCT:     S' = fcsmooth(old = S, wrk = W3) on the 34 fields;
+//            bndry_outer(S'); WE' = W3's electron pressure.
+// The state is S (34 fields, canonical order) and WE (the stale wrk
+// electron pressure the reference carries from one step's Lapidus to the
+// next step's push_elec and step-2 BandE): 35 arrays of nx*ny*nz.
+// Controls ctl = {eta, sbx, sby, sbz, srho, spx, spy, spz, spress}: the
+// resistivity and the upstream values bndry_outer imposes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/g3m3.h"
+#include "g3m3_fused.h"
+
+using namespace g3m3;
+
+namespace {
+
+constexpr int NFLD = 34, NST = 35;       // fields, state arrays (fields + WE)
+constexpr int EP = 30, BX = 31, WE = 34;
+// tape: S (35) | W1 (34) | S2 (34) | W3 (34) | S' (34)
+constexpr int T_S = 0, T_W1 = 35, T_S2 = 69, T_W3 = 103, T_S4 = 137, T_N = 171;
+
+__global__ void k_add_bnd(Box B, FieldPtrs dst, FieldPtrs src) {
+    int i, j, k;
+    if (!cell_of(B, gtid(), i, j, k) || B.interior(i, j, k)) return;
+    const int c = B.IX(i, j, k), f = blockIdx.y;
+    dst.p[f][c] += src.p[f][c];
+}
+__global__ void k_add_all(int n, FieldPtrs dst, FieldPtrs src) {
+    const int c = gtid();
+    if (c >= n) return;
+    dst.p[blockIdx.y][c] += src.p[blockIdx.y][c];
+}
+
+struct EF {   // one stage's E-field intermediates
+    float *J[3], *bs[3], *btot, *ev[3], *tv[3], *E[3];
+};
+
+struct FullModel {
+    g3m3_full_model_desc d;
+    Box B;
+    int n = 0;
+    hipStream_t s = nullptr;
+    float* mem = nullptr;
+    EF ef;
+    float* qv[3];
+    float* vel[3];
+    float* own_tape = nullptr;
+    // adjoint scratch
+    float *A = nullptr, *W3B = nullptr, *S2B = nullptr, *W1B = nullptr;
+    float *EB[3], *EVB[3], *CURB[3], *BSB[3], *BTB, *TVB[3], *VB[3];
+    float *fct_scr, *lap_scr, *ion_scr, *ef_scr;
+};
+
+float* fld(float* base, int n, int f) { return base + (size_t)n * f; }
+
+bool launched(const char* where) {
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return true;
+    fprintf(stderr, "g3m3_full_model: %s: %s\n", where, hipGetErrorString(e));
+    return false;
+}
+
+void zero(FullModel* M, float* p, size_t floats) {
+    hipMemsetAsync(p, 0, sizeof(float) * floats, M->s);
+}
+void copyf(FullModel* M, float* dst, const float* src, int nf) {
+    hipMemcpyAsync(dst, src, sizeof(float) * (size_t)M->n * nf, hipMemcpyDeviceToDevice, M->s);
+}
+
+// inflow values of the 34 fields from the controls, as the runtime forms
+// them (code/sub_bndry_outer.f90:50-103), with each value's control index
+// and derivative
+struct Inflow {
+    float val[NFLD];
+    int kind[NFLD];
+    int ctl[NFLD];
+    float coef[NFLD];
+};
+Inflow inflow(const g3m3_full_model_desc& d, const float* c) {
+    Inflow I;
+    const float srho = c[4], spx = c[5], spy = c[6], spz = c[7], spress = c[8];
+    const float ofrac = d.rho_frac * d.o_conc;
+    const float frac_h = d.rmassh / d.rmassq, frac_o = d.rmasso / d.rmassq;
+    const float pq = 0.5f * spress, rf = d.rho_frac;
+    const float sval[3][10] = {
+        {srho, pq, pq, pq, spx, spy, spz, 0.f, 0.f, 0.f},
+        {srho * rf * frac_h, 0.5f * spress * rf, 0.5f * spress * rf, 0.5f * spress * rf,
+         spx * rf * frac_h, spy * rf * frac_h, spz * rf * frac_h, 0.f, 0.f, 0.f},
+        {srho * ofrac * frac_o, 0.5f * spress * ofrac, 0.5f * spress * ofrac,
+         0.5f * spress * ofrac, spx * ofrac * frac_o, spy * ofrac * frac_o,
+         spz * ofrac * frac_o, 0.f, 0.f, 0.f}};
+    const float scale[3] = {1.0f, rf * frac_h, ofrac * frac_o};
+    const float pscale[3] = {0.5f, 0.5f * rf, 0.5f * ofrac};
+    const int kinds[10] = {kRHO, kPRES, kPRES, kPRES, kPX, kPY, kPZ, kOFF, kOFF, kOFF};
+    const int ctls[10] = {4, 8, 8, 8, 5, 6, 7, -1, -1, -1};
+    for (int sp = 0; sp < 3; ++sp)
+        for (int m = 0; m < 10; ++m) {
+            const int f = 10 * sp + m;
+            I.val[f] = sval[sp][m];
+            I.kind[f] = kinds[m];
+            I.ctl[f] = ctls[m];
+            I.coef[f] = (m >= 1 && m <= 3) ? pscale[sp] : (m == 0 || m >= 4 && m <= 6)
+                                                              ? scale[sp] : 0.0f;
+        }
+    I.val[EP] = pq / d.ti_te;
+    I.kind[EP] = kEP;
+    I.ctl[EP] = 8;
+    I.coef[EP] = 0.5f / d.ti_te;
+    for (int a = 0; a < 3; ++a) {
+        I.val[BX + a] = c[1 + a];
+        I.kind[BX + a] = kBX + a;
+        I.ctl[BX + a] = 1 + a;
+        I.coef[BX + a] = 1.0f;
+    }
+    return I;
+}
+
+void outer(FullModel* M, float* set, const Inflow& I) {
+    OuterPtrs O{};
+    for (int f = 0; f < NFLD; ++f) {
+        O.f[f] = fld(set, M->n, f);
+        O.kind[f] = I.kind[f];
+        O.val[f] = I.val[f];
+    }
+    for (int a = 0; a < 3; ++a) O.b0[a] = M->d.b0[a];
+    launch_bndry_outer(M->B, O, NFLD, M->s);
+}
+
+void outer_adj(FullModel* M, float* bar, const float* post, const Inflow& I, double* ctlb) {
+    OuterAdj O{};
+    for (int f = 0; f < NFLD; ++f) {
+        O.f[f] = fld(bar, M->n, f);
+        O.val[f] = post + (size_t)M->n * f;
+        O.kind[f] = I.kind[f];
+        O.ctl[f] = I.ctl[f];
+        O.coef[f] = I.coef[f];
+    }
+    launch_adj_outer(M->B, O, NFLD, ctlb, M->s);
+}
+
+// E-field intermediates of a set (fields in canonical order)
+void efield(FullModel* M, const float* set, float eta) {
+    const g3m3_full_model_desc& d = M->d;
+    const int n = M->n;
+    const float r = d.rx;
+    EF& e = M->ef;
+    const float* b[3] = {fld((float*)set, n, BX), fld((float*)set, n, BX + 1),
+                         fld((float*)set, n, BX + 2)};
+    const float* rho[3];
+    const float* mom[9];
+    for (int sp = 0; sp < 3; ++sp) {
+        rho[sp] = fld((float*)set, n, 10 * sp);
+        for (int a = 0; a < 3; ++a) mom[3 * sp + a] = fld((float*)set, n, 10 * sp + 4 + a);
+    }
+    launch_calcur(M->B, b[0], b[1], b[2], e.J[0], e.J[1], e.J[2], r, r, r, M->s);
+    const float* J[3] = {e.J[0], e.J[1], e.J[2]};
+    const float* b0[3] = {d.b0[0], d.b0[1], d.b0[2]};
+    launch_bfield_evel(M->B, b0, b, e.bs, e.btot, mom, rho, J, e.ev, e.tv, d.rmassq, d.rmassh,
+                       d.rmasso, d.reynolds, d.smallbit, 1, 1, M->s);
+    const float* bs[3] = {e.bs[0], e.bs[1], e.bs[2]};
+    const float* ev[3] = {e.ev[0], e.ev[1], e.ev[2]};
+    launch_bande(M->B, e.E, bs, J, ev, fld((float*)set, n, EP), rho, nullptr, 0.0f, d.reynolds,
+                 1, d.rmassq, d.rmassh, d.rmasso, nullptr, 0, nullptr, 0, 0, r, r, r, eta, M->s);
+}
+
+void push_species(FullModel* M, float* out, const float* old, const float* src, float delt) {
+    const g3m3_full_model_desc& d = M->d;
+    const int n = M->n;
+    const float rm[3] = {d.rmassq, d.rmassh, d.rmasso};
+    const float gmin[3] = {d.xmin, d.ymin, d.zmin}, gmax[3] = {d.xmax, d.ymax, d.zmax};
+    for (int sp = 0; sp < 3; ++sp) {
+        IonPtrs P{};
+        for (int f = 0; f < 10; ++f) {
+            P.out[f] = fld(out, n, 10 * sp + f);
+            P.old[f] = fld((float*)old, n, 10 * sp + f);
+            P.wrk[f] = fld((float*)src, n, 10 * sp + f);
+        }
+        for (int a = 0; a < 3; ++a) {
+            P.bs[a] = M->ef.bs[a];
+            P.e[a] = M->ef.E[a];
+            P.qv[a] = M->qv[a];
+            P.tv[a] = M->ef.tv[a];
+        }
+        P.btot = M->ef.btot;
+        launch_push_ion(M->B, P, rm[sp], delt, d.grav, d.re_equiv, d.reynolds, gmin, gmax, M->s);
+    }
+}
+
+void push_b(FullModel* M, float* out, const float* old, float delt) {
+    const int n = M->n;
+    float* b[3] = {fld(out, n, BX), fld(out, n, BX + 1), fld(out, n, BX + 2)};
+    const float* o[3] = {fld((float*)old, n, BX), fld((float*)old, n, BX + 1),
+                         fld((float*)old, n, BX + 2)};
+    const float* E[3] = {M->ef.E[0], M->ef.E[1], M->ef.E[2]};
+    const float r = M->d.rx;
+    launch_push_bfld(M->B, b, o, E, delt, r, r, r, M->s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* g3m3_full_model_create(const g3m3_full_model_desc* desc) {
+    if (!desc || desc->nx < 5 || desc->ny < 5 || desc->nz < 5 ||
+        (long long)desc->nx * desc->ny * desc->nz >= kFastDivMax) {
+        fprintf(stderr, "g3m3_full_model_create: box extents out of range\n");
+        return nullptr;
+    }
+    FullModel* M = new FullModel();
+    M->d = *desc;
+    M->B = Box{desc->nx, desc->ny, desc->nz};
+    M->n = M->B.V();
+    const size_t n = M->n;
+    // E-field 16 | qv 3 | vel 3 | own tape T_N | A 34 | W3B, S2B, W1B 34 each |
+    // EB, EVB, CURB, BSB 3 each, BTB 1, TVB 3, VB 3 | fct 24 | lap 6 | ion 15 | ef 3
+    const size_t total = 16 + 3 + 3 + T_N + 34 * 4 + 12 + 1 + 6 +
+                         adj_fct_scratch_floats(M->B) / n + 6 + 15 + 3;
+    if (hipMalloc(&M->mem, sizeof(float) * total * n) != hipSuccess) {
+        fprintf(stderr, "g3m3_full_model_create: out of device memory\n");
+        delete M;
+        return nullptr;
+    }
+    float* p = M->mem;
+    auto take = [&](size_t k) {
+        float* r = p;
+        p += k * n;
+        return r;
+    };
+    for (int a = 0; a < 3; ++a) M->ef.J[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->ef.bs[a] = take(1);
+    M->ef.btot = take(1);
+    for (int a = 0; a < 3; ++a) M->ef.ev[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->ef.tv[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->ef.E[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->qv[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->vel[a] = take(1);
+    M->own_tape = take(T_N);
+    M->A = take(34);
+    M->W3B = take(34);
+    M->S2B = take(34);
+    M->W1B = take(34);
+    for (int a = 0; a < 3; ++a) M->EB[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->EVB[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->CURB[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->BSB[a] = take(1);
+    M->BTB = take(1);
+    for (int a = 0; a < 3; ++a) M->TVB[a] = take(1);
+    for (int a = 0; a < 3; ++a) M->VB[a] = take(1);
+    M->fct_scr = take(adj_fct_scratch_floats(M->B) / n);
+    M->lap_scr = take(6);
+    M->ion_scr = take(15);
+    M->ef_scr = take(3);
+    zero(M, M->mem, total * n);
+    if (!launched("create") || hipStreamSynchronize(M->s) != hipSuccess) {
+        hipFree(M->mem);
+        delete M;
+        return nullptr;
+    }
+    return M;
+}
+
+void g3m3_full_model_destroy(void* model) {
+    FullModel* M = (FullModel*)model;
+    if (!M) return;
+    hipStreamSynchronize(M->s);
+    hipFree(M->mem);
+    delete M;
+}
+
+void g3m3_full_model_set_stream(void* model, void* hip_stream) {
+    if (model) ((FullModel*)model)->s = (hipStream_t)hip_stream;
+}
+
+long long g3m3_full_model_tape_floats(void* model) {
+    return model ? (long long)T_N * ((FullModel*)model)->n : 0;
+}
+
+int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* tape,
+                         const float* ctl) {
+    FullModel* M = (FullModel*)model;
+    if (!M || !s_in || !s_out || !ctl || s_in == s_out) return 1;
+    const g3m3_full_model_desc& d = M->d;
+    const int n = M->n;
+    const float eta = ctl[0], delt = d.delt;
+    const Inflow I = inflow(d, ctl);
+    float* T = tape ? tape : M->own_tape;
+    float* S = fld(T, n, T_S);
+    float* W1 = fld(T, n, T_W1);
+    float* S2 = fld(T, n, T_S2);
+    float* W3 = fld(T, n, T_W3);
+    float* S4 = fld(T, n, T_S4);
+    copyf(M, S, s_in, NST);
+    // stage 1
+    efield(M, S, eta);
+    copyf(M, W1, S, NFLD);
+    copyf(M, fld(W1, n, EP), fld(S, n, WE), 1);
+    push_species(M, W1, S, S, 0.5f * delt);
+    push_b(M, W1, S, 0.5f * delt);
+    outer(M, W1, I);
+    // stage 2
+    efield(M, W1, eta);
+    copyf(M, S2, S, NFLD);
+    push_species(M, S2, S, W1, delt);
+    {
+        const float* ev[3] = {M->ef.ev[0], M->ef.ev[1], M->ef.ev[2]};
+        const float r = d.rx;
+        launch_push_elec(M->B, fld(W1, n, EP), fld(S, n, EP), fld(S2, n, EP), ev, d.gamma,
+                         d.gamma1, delt, r, r, r, M->s);
+    }
+    push_b(M, S2, S, delt);
+    outer(M, S2, I);
+    // Lapidus into a copy of W1
+    copyf(M, W3, W1, NFLD);
+    for (int sp = 0; sp < 3; ++sp) {
+        const float* pp[3] = {fld(S2, n, 10 * sp + 4), fld(S2, n, 10 * sp + 5),
+                              fld(S2, n, 10 * sp + 6)};
+        launch_velocity(n, fld(S2, n, 10 * sp), pp, M->vel, d.smallbit, M->s);
+        LapPtrs L{};
+        for (int f = 0; f < 10; ++f) {
+            L.src[f] = fld(S2, n, 10 * sp + f);
+            L.dst[f] = fld(W3, n, 10 * sp + f);
+            L.chi[f] = f == 0 ? d.chirho : d.chipxyz;
+        }
+        const float* v[3] = {M->vel[0], M->vel[1], M->vel[2]};
+        launch_lapidus(M->B, L, 10, v, delt, M->s);
+    }
+    {
+        const float* mom[9];
+        const float* rho[3];
+        for (int sp = 0; sp < 3; ++sp) {
+            rho[sp] = fld(S2, n, 10 * sp);
+            for (int a = 0; a < 3; ++a) mom[3 * sp + a] = fld(S2, n, 10 * sp + 4 + a);
+        }
+        launch_vtot(n, mom, rho, M->vel, M->s);
+        LapPtrs L{};
+        for (int f = 0; f < 4; ++f) {
+            L.src[f] = fld(S2, n, EP + f);
+            L.dst[f] = fld(W3, n, EP + f);
+            L.chi[f] = f == 0 ? d.chipxyz : d.chierg;
+        }
+        const float* v[3] = {M->vel[0], M->vel[1], M->vel[2]};
+        launch_lapidus(M->B, L, 4, v, delt, M->s);
+    }
+    outer(M, W3, I);
+    // flux correction
+    FctParams F{};
+    F.B = M->B;
+    for (int f = 0; f < NFLD; ++f) {
+        F.out[f] = fld(S4, n, f);
+        F.old[f] = fld(S, n, f);
+        F.wrk[f] = fld(W3, n, f);
+        F.chi[f] = f >= BX ? d.diferg : d.difrho;
+    }
+    launch_fct_fused(F, NFLD, M->s);
+    outer(M, S4, I);
+    copyf(M, s_out, S4, NFLD);
+    copyf(M, fld(s_out, n, WE), fld(W3, n, EP), 1);
+    return launched("step") ? 0 : 2;
+}
+
+namespace {
+
+// the adjoint of one stage's push and E-field: from the stage output's
+// adjoint `ob` (34 fields), accumulate into the source set's adjoint `sb`
+// and the old set's adjoint `oldb`; src is the stage's source set, old S
+void stage_adj(FullModel* M, const float* src, const float* old, float* ob, float* sb,
+               float* oldb, float delt, bool elec, double* ctl_bar, float eta) {
+    const g3m3_full_model_desc& d = M->d;
+    const int n = M->n;
+    const float r = d.rx;
+    hipStream_t s = M->s;
+    efield(M, src, eta);
+    for (int a = 0; a < 3; ++a) {
+        zero(M, M->BSB[a], n);
+        zero(M, M->TVB[a], n);
+        zero(M, M->CURB[a], n);
+    }
+    zero(M, M->BTB, n);
+    // push_bfld: old's adjoint, E's adjoint (assigned)
+    {
+        const float* bb[3] = {fld(ob, n, BX), fld(ob, n, BX + 1), fld(ob, n, BX + 2)};
+        float* obb[3] = {fld(oldb, n, BX), fld(oldb, n, BX + 1), fld(oldb, n, BX + 2)};
+        launch_adj_push_bfld(M->B, bb, obb, M->EB, delt, r, r, r, s);
+    }
+    // push_ion, each species
+    const float rm[3] = {d.rmassq, d.rmassh, d.rmasso};
+    for (int sp = 0; sp < 3; ++sp) {
+        IonAdj A{};
+        A.B = M->B;
+        for (int f = 0; f < 10; ++f) {
+            A.src[f] = fld((float*)src, n, 10 * sp + f);
+            A.ob[f] = fld(ob, n, 10 * sp + f);
+            A.sb[f] = fld(sb, n, 10 * sp + f);
+            A.oldb[f] = fld(oldb, n, 10 * sp + f);
+        }
+        for (int a = 0; a < 3; ++a) {
+            A.bs[a] = M->ef.bs[a];
+            A.e[a] = M->ef.E[a];
+            A.tv[a] = M->ef.tv[a];
+            A.bsb[a] = M->BSB[a];
+            A.eb[a] = M->EB[a];
+            A.tvb[a] = M->TVB[a];
+        }
+        A.btot = M->ef.btot;
+        A.btotb = M->BTB;
+        A.scr = M->ion_scr;
+        A.rmass = rm[sp];
+        A.delt = delt;
+        A.grav = d.grav;
+        A.re_equiv = d.re_equiv;
+        A.reynolds = d.reynolds;
+        A.xmin = d.xmin;
+        A.ymin = d.ymin;
+        A.zmin = d.zmin;
+        A.ddx = (d.xmax - d.xmin) / (float)(d.nx - 1);
+        A.ddy = (d.ymax - d.ymin) / (float)(d.ny - 1);
+        A.ddz = (d.zmax - d.zmin) / (float)(d.nz - 1);
+        A.dxt = 2.0f * A.ddx;
+        A.dyt = 2.0f * A.ddy;
+        A.dzt = 2.0f * A.ddz;
+        A.bmax = 0.20f * A.rmass / A.delt / A.reynolds;
+        launch_adj_push_ion(A, s);
+    }
+    // BandE's face copies, then its body with the plasma terms
+    launch_adj_bande_faces(M->B, M->EB, s);
+    EfAdj E{};
+    E.B = M->B;
+    for (int a = 0; a < 3; ++a) {
+        E.eb[a] = M->EB[a];
+        E.bs[a] = M->ef.bs[a];
+        E.ev[a] = M->ef.ev[a];
+        E.cur[a] = M->ef.J[a];
+        E.tvb[a] = M->TVB[a];
+        E.evb[a] = M->EVB[a];
+        E.bsb[a] = M->BSB[a];
+        E.curb[a] = M->CURB[a];
+    }
+    for (int sp = 0; sp < 3; ++sp) {
+        E.rho[sp] = fld((float*)src, n, 10 * sp);
+        E.rhob[sp] = fld(sb, n, 10 * sp);
+        for (int a = 0; a < 3; ++a) {
+            E.mom[3 * sp + a] = fld((float*)src, n, 10 * sp + 4 + a);
+            E.momb[3 * sp + a] = fld(sb, n, 10 * sp + 4 + a);
+        }
+    }
+    E.ep = fld((float*)src, n, EP);
+    E.epb = fld(sb, n, EP);
+    E.etab = ctl_bar;
+    E.scr = M->ef_scr;
+    E.eta = eta;
+    E.rmq = d.rmassq;
+    E.rmh = d.rmassh;
+    E.rmo = d.rmasso;
+    E.reynolds = d.reynolds;
+    E.dxt = E.dyt = E.dzt = 2.0f * r;
+    // BandE's body assigns ev's adjoint; push_elec's part is added to it
+    // before fnd_evel's transpose reads it
+    launch_adj_bande_plasma(E, s);
+    if (elec) {
+        PeAdj P{};
+        P.B = M->B;
+        P.w = fld((float*)src, n, EP);
+        for (int a = 0; a < 3; ++a) {
+            P.ev[a] = M->ef.ev[a];
+            P.evb[a] = M->EVB[a];
+        }
+        P.ob = fld(ob, n, EP);
+        P.oldb = fld(oldb, n, EP);
+        P.wb = fld(sb, n, EP);
+        P.delt = delt;
+        P.gamma = d.gamma;
+        P.gamma1 = d.gamma1;
+        P.dxt = P.dyt = P.dzt = 2.0f * r;
+        launch_adj_push_elec(P, s);
+    }
+    launch_adj_evel_plasma(E, s);
+    // total B: btot, then bs = b0 + b
+    const float* bsc[3] = {M->ef.bs[0], M->ef.bs[1], M->ef.bs[2]};
+    launch_adj_btot(n, bsc, M->BTB, d.smallbit, M->BSB, s);
+    float* bb[3] = {fld(sb, n, BX), fld(sb, n, BX + 1), fld(sb, n, BX + 2)};
+    const float* bsb[3] = {M->BSB[0], M->BSB[1], M->BSB[2]};
+    launch_add3(n, bsb, bb, s);
+    launch_adj_calcur(M->B, M->CURB, bb, r, r, r, s);
+}
+
+}  // namespace
+
+int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_bar,
+                            float* sin_bar, double* ctl_bar, const float* ctl) {
+    FullModel* M = (FullModel*)model;
+    if (!M || !tape || !sout_bar || !sin_bar || !ctl_bar || !ctl) return 1;
+    const g3m3_full_model_desc& d = M->d;
+    const int n = M->n;
+    const float eta = ctl[0], delt = d.delt;
+    const Inflow I = inflow(d, ctl);
+    hipStream_t s = M->s;
+    float* T = const_cast<float*>(tape);
+    const float* S = fld(T, n, T_S);
+    const float* W1 = fld(T, n, T_W1);
+    const float* S2 = fld(T, n, T_S2);
+    const float* W3 = fld(T, n, T_W3);
+    const float* S4 = fld(T, n, T_S4);
+    float* SB = sin_bar;
+    zero(M, SB, (size_t)NST * n);
+    zero(M, M->S2B, (size_t)NFLD * n);
+    zero(M, M->W1B, (size_t)NFLD * n);
+    copyf(M, M->A, sout_bar, NFLD);
+    // S' = bndry_outer(fcsmooth(S, W3))
+    outer_adj(M, M->A, S4, I, ctl_bar);
+    zero(M, M->W3B, (size_t)NFLD * n);
+    for (int f0 = 0; f0 < NFLD;) {
+        // groups of up to three fields of one coefficient
+        const int end = f0 < BX ? BX : NFLD;
+        const int nf = end - f0 < 3 ? end - f0 : 3;
+        const float* old[3];
+        const float* wrk[3];
+        float *ob[3], *wb[3], *oldb[3];
+        for (int q = 0; q < 3; ++q) {
+            const int f = f0 + (q < nf ? q : 0);
+            old[q] = fld((float*)S, n, f);
+            wrk[q] = fld((float*)W3, n, f);
+            ob[q] = fld(M->A, n, f);
+            wb[q] = fld(M->W3B, n, f);
+            oldb[q] = fld(SB, n, f);
+        }
+        launch_adj_fct(M->B, old, wrk, ob, wb, oldb, f0 >= BX ? d.diferg : d.difrho,
+                       M->fct_scr, s, nf);
+        f0 += nf;
+    }
+    // WE' = W3's pressure after its boundary pass
+    {
+        FieldPtrs dst{}, src{};
+        dst.p[0] = fld(M->W3B, n, EP);
+        src.p[0] = const_cast<float*>(fld((float*)sout_bar, n, WE));
+        hipLaunchKernelGGL(k_add_all, dim3((n + 255) / 256, 1), dim3(256), 0, s, n, dst, src);
+    }
+    outer_adj(M, M->W3B, W3, I, ctl_bar);
+    // W3 = Lapidus of S2 on the interior, W1 on the boundary
+    {
+        FieldPtrs dst{}, src{};
+        for (int f = 0; f < NFLD; ++f) {
+            dst.p[f] = fld(M->W1B, n, f);
+            src.p[f] = fld(M->W3B, n, f);
+        }
+        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, NFLD), dim3(256), 0, s, M->B, dst,
+                           src);
+    }
+    for (int sp = 0; sp < 3; ++sp) {
+        const float* pp[3] = {fld((float*)S2, n, 10 * sp + 4), fld((float*)S2, n, 10 * sp + 5),
+                              fld((float*)S2, n, 10 * sp + 6)};
+        launch_velocity(n, fld((float*)S2, n, 10 * sp), pp, M->vel, d.smallbit, s);
+        for (int a = 0; a < 3; ++a) zero(M, M->VB[a], n);
+        LapAdj L{};
+        L.B = M->B;
+        for (int f = 0; f < 10; ++f) {
+            L.q[f] = fld((float*)S2, n, 10 * sp + f);
+            L.wb[f] = fld(M->W3B, n, 10 * sp + f);
+            L.qb[f] = fld(M->S2B, n, 10 * sp + f);
+            L.chi[f] = f == 0 ? d.chirho : d.chipxyz;
+        }
+        for (int a = 0; a < 3; ++a) {
+            L.v[a] = M->vel[a];
+            L.vb[a] = M->VB[a];
+        }
+        L.delt = delt;
+        L.scr = M->lap_scr;
+        L.nf = 10;
+        launch_adj_lap_species(L, s);
+        const float* vb[3] = {M->VB[0], M->VB[1], M->VB[2]};
+        float* pb[3] = {fld(M->S2B, n, 10 * sp + 4), fld(M->S2B, n, 10 * sp + 5),
+                        fld(M->S2B, n, 10 * sp + 6)};
+        launch_adj_velocity(n, fld((float*)S2, n, 10 * sp), pp, vb, d.smallbit,
+                            fld(M->S2B, n, 10 * sp), pb, s);
+    }
+    {
+        VtotAdj V{};
+        const float* mom[9];
+        const float* rho[3];
+        for (int sp = 0; sp < 3; ++sp) {
+            rho[sp] = fld((float*)S2, n, 10 * sp);
+            V.rho[sp] = rho[sp];
+            V.rhob[sp] = fld(M->S2B, n, 10 * sp);
+            for (int a = 0; a < 3; ++a) {
+                mom[3 * sp + a] = fld((float*)S2, n, 10 * sp + 4 + a);
+                V.mom[3 * sp + a] = mom[3 * sp + a];
+                V.momb[3 * sp + a] = fld(M->S2B, n, 10 * sp + 4 + a);
+            }
+        }
+        launch_vtot(n, mom, rho, M->vel, s);
+        for (int a = 0; a < 3; ++a) zero(M, M->VB[a], n);
+        LapAdj L{};
+        L.B = M->B;
+        for (int f = 0; f < 4; ++f) {
+            L.q[f] = fld((float*)S2, n, EP + f);
+            L.wb[f] = fld(M->W3B, n, EP + f);
+            L.qb[f] = fld(M->S2B, n, EP + f);
+            L.chi[f] = f == 0 ? d.chipxyz : d.chierg;
+        }
+        for (int a = 0; a < 3; ++a) {
+            L.v[a] = M->vel[a];
+            L.vb[a] = M->VB[a];
+            V.vb[a] = M->VB[a];
+        }
+        L.delt = delt;
+        L.scr = M->lap_scr;
+        L.nf = 4;
+        launch_adj_lap_species(L, s);
+        launch_adj_vtot(n, V, s);
+    }
+    // S2 = bndry_outer(push from W1 with old = S)
+    outer_adj(M, M->S2B, S2, I, ctl_bar);
+    {
+        FieldPtrs dst{}, src{};
+        for (int f = 0; f < NFLD; ++f) {
+            dst.p[f] = fld(SB, n, f);
+            src.p[f] = fld(M->S2B, n, f);
+        }
+        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, EP + 1), dim3(256), 0, s, M->B, dst,
+                           src);
+    }
+    stage_adj(M, W1, S, M->S2B, M->W1B, SB, delt, true, ctl_bar, eta);
+    // W1 = bndry_outer(push from S with delt/2; its pressure is WE)
+    outer_adj(M, M->W1B, W1, I, ctl_bar);
+    {
+        FieldPtrs dst{}, src{};
+        dst.p[0] = fld(SB, n, WE);
+        src.p[0] = fld(M->W1B, n, EP);
+        hipLaunchKernelGGL(k_add_all, dim3((n + 255) / 256, 1), dim3(256), 0, s, n, dst, src);
+        zero(M, fld(M->W1B, n, EP), n);
+        for (int f = 0; f < NFLD; ++f) {
+            dst.p[f] = fld(SB, n, f);
+            src.p[f] = fld(M->W1B, n, f);
+        }
+        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, EP + 1), dim3(256), 0, s, M->B, dst,
+                           src);
+    }
+    stage_adj(M, S, S, M->W1B, SB, SB, 0.5f * delt, false, ctl_bar, eta);
+    return launched("adjoint") ? 0 : 2;
+}
+
+}  // extern "C"

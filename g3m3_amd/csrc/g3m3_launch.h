@@ -142,6 +142,8 @@ void launch_adj_bande(const Box& B, float* eb[3], const float* bs[3], const floa
                       hipStream_t s);
 void launch_adj_evel(const Box& B, const float* evb[3], const float* rho[3], float rmq, float rmh,
                      float rmo, float reynolds, float* curb[3], hipStream_t s);
+// BandE's face copies, transposed (scatter to the clamped interior cell, zero)
+void launch_adj_bande_faces(const Box& B, float* eb[3], hipStream_t s);
 void launch_adj_calcur(const Box& B, float* curb[3], float* bb[3], float rx, float ry, float rz,
                        hipStream_t s);
 void launch_adj_outer_b(const Box& B, float* bb[3], double* sbb, hipStream_t s);
@@ -149,7 +151,8 @@ void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float c
                     float* qb[3], float* w1b[3], hipStream_t s);
 size_t adj_fct_scratch_floats(const Box& B);
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
-                    float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s);
+                    float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
+                    int nf = 3);
 void launch_add3(int n, const float* x[3], float* y[3], hipStream_t s);
 struct CraftGrid;
 void launch_adj_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
@@ -185,6 +188,7 @@ struct LapAdj {
     float chi[10];
     float delt;
     float* scr;              // adj_lapsp_scratch_floats
+    int nf = 10;             // fields in use
 };
 size_t adj_ion_scratch_floats(const Box& B);
 size_t adj_lapsp_scratch_floats(const Box& B);
@@ -192,6 +196,62 @@ void launch_adj_push_ion(const IonAdj& A, hipStream_t s);
 void launch_adj_lap_species(const LapAdj& L, hipStream_t s);
 void launch_adj_velocity(int n, const float* rho, const float* p[3], const float* vb[3],
                          float smallbit, float* rhob, float* pb[3], hipStream_t s);
+
+// adjoints of the plasma couplings of a box-step (k_adjoint_full.hip)
+struct EfAdj {
+    Box B;
+    const float* eb[3];      // E's adjoint after BandE's face copies are transposed
+    const float* bs[3];
+    const float* ev[3];
+    const float* cur[3];
+    const float* rho[3];
+    const float* mom[9];     // q px,py,pz, h ..., o ...
+    const float* ep;
+    const float* tvb[3];     // tv's adjoint from its other uses (the ion push)
+    float* evb[3];           // scratch: ev's adjoint
+    float* bsb[3];
+    float* curb[3];
+    float* rhob[3];
+    float* momb[9];
+    float* epb;
+    double* etab;
+    float* scr;              // 3 * V
+    float eta, rmq, rmh, rmo, reynolds, dxt, dyt, dzt;
+};
+struct PeAdj {
+    Box B;
+    const float* w;          // push_elec's source pressure
+    const float* ev[3];
+    const float* ob;         // adjoint of the output pressure
+    float* oldb;
+    float* wb;
+    float* evb[3];
+    float delt, gamma, gamma1, dxt, dyt, dzt;
+};
+struct VtotAdj {
+    const float* rho[3];
+    const float* mom[9];
+    const float* vb[3];
+    float* rhob[3];
+    float* momb[9];
+};
+struct OuterAdj {
+    float* f[kMaxFields];          // adjoint arrays (transposed in place)
+    const float* val[kMaxFields];   // the field after the forward pass (signs of |.|)
+    int kind[kMaxFields];
+    int ctl[kMaxFields];            // control index of the inflow value, -1 for none
+    float coef[kMaxFields];         // d(inflow value)/d(control)
+};
+// BandE's body with its plasma terms (assigns ev's adjoint), then fnd_evel's
+// transpose of ev's adjoint plus tv's; between them a caller adds
+// push_elec's part of ev's adjoint
+void launch_adj_bande_plasma(const EfAdj& A, hipStream_t s);
+void launch_adj_evel_plasma(const EfAdj& A, hipStream_t s);
+void launch_adj_btot(int n, const float* b[3], const float* btb, float smallbit, float* bb[3],
+                     hipStream_t s);
+void launch_adj_vtot(int n, const VtotAdj& A, hipStream_t s);
+void launch_adj_push_elec(const PeAdj& A, hipStream_t s);
+void launch_adj_outer(const Box& B, const OuterAdj& O, int nf, double* ctlb, hipStream_t s);
 
 // write_graphing_data's cut-plane quantities (k_graph_cuts in k_misc.hip)
 struct GraphCuts {

@@ -621,6 +621,12 @@ void launch_adj_bande(const Box& B, float* eb[3], const float* bs[3], const floa
                        curb[1], curb[2], etab);
 }
 
+void launch_adj_bande_faces(const Box& B, float* eb[3], hipStream_t s) {
+    const int n = B.V();
+    hipLaunchKernelGGL(k_adj_clamp_copy, dim3((n + 255) / 256), dim3(256), 0, s, B, eb[0], eb[1],
+                       eb[2]);
+}
+
 void launch_adj_evel(const Box& B, const float* evb[3], const float* rho[3], float rmq, float rmh,
                      float rmo, float reynolds, float* curb[3], hipStream_t s) {
     const int n = B.V();
@@ -662,12 +668,13 @@ void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float c
 size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 8 * B.V(); }
 
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
-                    float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s) {
+                    float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
+                    int nf) {
     const int n = B.V();
     FctAdj A;
     A.B = B;
     A.chi = chi;
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < nf; ++f) {
         float* base = scratch + (size_t)8 * n * f;
         A.old[f] = old[f];
         A.wrk[f] = wrk[f];
@@ -681,7 +688,7 @@ void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], floa
         A.wb[f] = wb[f];
         A.oldb[f] = oldb[f];
     }
-    const dim3 g((n + 255) / 256, 3), t(256);
+    const dim3 g((n + 255) / 256, nf), t(256);
     hipLaunchKernelGGL(k_adj_fct_diffuse, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_faces, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_cells, g, t, 0, s, A);

@@ -328,7 +328,7 @@ __global__ void k_adj_lapsp_coef(LapAdj L) {
         const float* v = L.v[d];
         const float sp = sgn(v[c + st[d]] - v[c]), sm = sgn(v[c] - v[c - st[d]]);
         float ap = 0.0f, am = 0.0f;
-        for (int f = 0; f < 10; ++f) {
+        for (int f = 0; f < L.nf; ++f) {
             const float g = L.chi[f] * L.delt * L.wb[f][c];
             const float* q = L.q[f];
             ap += g * (q[c + st[d]] - q[c]);
@@ -361,7 +361,7 @@ __global__ void k_adj_lapsp_gather(LapAdj L) {
         if (pos[d] + 1 < len[d]) up[d] = fabsf(v[m + st[d]] - v[m]);
         if (pos[d] - 1 >= 0) um[d] = fabsf(v[m] - v[m - st[d]]);
     }
-    for (int f = 0; f < 10; ++f) {
+    for (int f = 0; f < L.nf; ++f) {
         const float g = L.chi[f] * L.delt;
         const float* wb = L.wb[f];
         float acc = 0.0f;

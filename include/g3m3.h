@@ -773,6 +773,51 @@ int g3m3_field_model_step(void* model, const float* b_in, float* b_out, float* t
  * of eta and sb.  bout_bar is not changed.  Returns 0 on success. */
 int g3m3_field_model_adjoint(void* model, const float* tape, const float* bout_bar,
                              float* bin_bar, double* ctl_bar, const float* ctl);
+/* ---- adjoint of the whole box-step ----------------------------------------
+ * No reference counterpart: one box-step of the outermost box (box =
+ * n_grids = 1, no planet lists, fixed delt, the validation calls set_rho
+ * and set_speed_agrd left out) with every field evolving, and its discrete
+ * adjoint, for gradients of a misfit at virtual spacecraft with respect to
+ * the upstream plasma and field and a uniform resistivity.  The step
+ * follows code/3d_main.f90:2163-3030 for that box: stage 1 (:2243-2340:
+ * E-field, push_ion for q, h, o and push_bfld with delt/2, bndry_outer),
+ * stage 2 (:2486-2580: the same from the stage-1 set with delt, plus
+ * push_elec), the Lapidus smoothing (:2698-2778) and flux_correct on every
+ * field (:2896-2935, code/sub_flux_correct.f90) with bndry_outer after each.
+ * The state is 35 contiguous device arrays of nx*ny*nz cells: q, h, o (ten
+ * each: rho, presx, presy, presz, px, py, pz, presxy, presxz, presyz),
+ * epres, bx, by, bz, and the electron pressure of the last Lapidus pass,
+ * which the reference keeps in wrkepres and reads as the source of the next
+ * step's push_elec and stage-2 BandE.  ctl (host, 9 floats) = {eta,
+ * sbx_wind, sby_wind, sbz_wind, srho, spx, spy, spz, spress}: the
+ * resistivity and the upstream values of code/sub_bndry_outer.f90:50-103
+ * (the h and o values follow from them through rho_frac, o_conc and the
+ * masses, the electron pressure through ti_te).  b0 points at device
+ * arrays kept by the model. */
+typedef struct {
+    int nx, ny, nz;
+    const float* b0[3];       /* bx0, by0, bz0 of the box */
+    float rx;                 /* xspac of the box */
+    float xmin, xmax, ymin, ymax, zmin, zmax;   /* grd_* of the box */
+    float delt, rmassq, rmassh, rmasso, reynolds, smallbit, gamma, gamma1, grav, re_equiv;
+    float chirho, chipxyz, chierg, difrho, diferg, rho_frac, o_conc, ti_te;
+} g3m3_full_model_desc;
+void* g3m3_full_model_create(const g3m3_full_model_desc* desc);
+void g3m3_full_model_destroy(void* model);
+void g3m3_full_model_set_stream(void* model, void* hip_stream);
+/* floats of one step's tape (171 arrays of nx*ny*nz) */
+long long g3m3_full_model_tape_floats(void* model);
+/* s_out = one step from s_in (35 arrays each, distinct); tape (or NULL)
+ * receives what the adjoint of the step needs.  Returns 0 on success. */
+int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* tape,
+                         const float* ctl);
+/* From the step's tape and the adjoint of its output: sin_bar = the
+ * adjoint of its input (assigned, 35 arrays); ctl_bar[0..8] (device
+ * doubles) += the adjoints of the controls.  sout_bar is not changed.
+ * Returns 0 on success. */
+int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_bar,
+                            float* sin_bar, double* ctl_bar, const float* ctl);
+
 /* The adjoint of g3m3_sample_fields (code/cft_trilin_interp.f90:10-66 for
  * each point): fields_bar[q] += the trilinear weights of point p's cube
  * times scdata_bar(q,p).  Points outside every box add nothing. */
