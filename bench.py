@@ -232,6 +232,65 @@ def adjoint_case(steps: int = 50, every: int = 10, ckpt: int = 10, ncraft: int =
             "method": "hand-written adjoint kernels, recompute from checkpoints"}
 
 
+def full_adjoint_case(steps: int = 50, every: int = 10, ckpt: int = 10, ncraft: int = 8):
+    """The checkpointed adjoint of the whole box-step (g3m3_amd.adjoint
+    full model) on one 121x121x61 box upstream of the planet: a misfit of
+    density, momentum, electron pressure and field at ``ncraft`` craft near
+    the inflow face over ``steps`` steps, against a run with other upstream
+    values and resistivity plus 5% noise; its gradient in the nine controls
+    and the initial state, and the wall time of the forward rollout and of
+    the backward pass (recompute from checkpoints + adjoint kernels)."""
+    import torch
+    from g3m3_amd import adjoint as A
+    s = A.full_setup(nx=121, ny=121, nz=61, spacing=0.5)
+    m = A.FullModel(s)
+    pos_h = A.craft_positions(s, ncraft, seed=14)
+    h = float(s.grid["xspac"][0])
+    x0 = float(s.grid["grid_minvals"][0][0])
+    pos_h[:, 0] = (x0 + h * np.random.default_rng(5).uniform(2.0, 12.0, ncraft)).astype(np.float32)
+    pos = torch.from_numpy(pos_h).cuda()
+    ctl0 = torch.tensor(s.ctl.astype(np.float64))
+    ctl_true = ctl0.clone()
+    ctl_true[0], ctl_true[3] = 5e-4, -0.20
+    ctl_true[4] *= 1.05
+    ctl_true[5] *= 0.97
+    with torch.no_grad():
+        target = A.full_rollout(ctl_true, m.new_state(), m, steps, every, pos).clone()
+    gen = torch.Generator().manual_seed(15)
+    rms = target.pow(2).mean(dim=(0, 1)).sqrt()
+    target += 0.05 * rms * torch.randn(target.shape, generator=gen).cuda()
+    wt = 1.0 / rms
+
+    def grad():
+        c = ctl0.clone().requires_grad_(True)
+        s0 = m.new_state().requires_grad_(True)
+        out = A.full_rollout(c, s0, m, steps, every, pos, ckpt)
+        L = (((out - target) * wt) ** 2).sum()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        return float(L.detach()), c.grad.numpy(), time.perf_counter() - t0
+
+    grad()                                   # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        A.full_rollout(ctl0, m.new_state(), m, steps, every, pos)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    L, g, bwd = grad()
+    m.close()
+    return {"config": f"one 121x121x61 box, every field evolving, {steps} steps, {ncraft} craft "
+                      f"sampling 8 quantities every {every} steps, checkpoints every {ckpt} steps",
+            "controls": dict(zip(A.FULL_CTL_NAMES, ctl0.tolist())),
+            "target_controls": dict(zip(A.FULL_CTL_NAMES, ctl_true.tolist())),
+            "loss": L, "gradient": dict(zip(A.FULL_CTL_NAMES, g.tolist())),
+            "forward_ms": round(fwd * 1e3, 2), "backward_ms": round(bwd * 1e3, 2),
+            "backward_over_forward": round(bwd / fwd, 2),
+            "method": "hand-written adjoint kernels, recompute from checkpoints"}
+
+
 def precision_case(fast_mode: int):
     """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
     FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
@@ -481,6 +540,8 @@ def main(argv=None):
     batch = None if a.no_batch else batch_case()
     grad = None if a.no_gradient else gradient_case()
     adj = None if a.no_adjoint else adjoint_case()
+    if adj is not None:
+        adj = {"field": adj, "full": full_adjoint_case()}
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,

@@ -34,14 +34,43 @@ namespace {
 
 constexpr int NFLD = 34, NST = 35;       // fields, state arrays (fields + WE)
 constexpr int EP = 30, BX = 31, WE = 34;
-// tape: S (35) | W1 (34) | S2 (34) | W3 (34) | S' (34)
-constexpr int T_S = 0, T_W1 = 35, T_S2 = 69, T_W3 = 103, T_S4 = 137, T_N = 171;
+// tape: S (35) | W1 (34) | S2 (34) | W3 (34) | S' (34) | E-field of stage 1 (16) | of stage 2
+constexpr int T_S = 0, T_W1 = 35, T_S2 = 69, T_W3 = 103, T_S4 = 137, T_EF1 = 171, T_EF2 = 187,
+              T_N = 203;
 
+// boundary cell t of a box (0 <= t < bnd_cells): the k faces, then the j
+// faces of interior k, then the i faces of interior j and k
+__host__ __device__ inline int bnd_cells(const Box& B) {
+    return 2 * B.nx * B.ny + 2 * B.nx * (B.nz - 2) + 2 * (B.ny - 2) * (B.nz - 2);
+}
+__device__ inline int bnd_cell(const Box& B, int t) {
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    const int a = nx * ny, b = nx * (nz - 2), d = (ny - 2) * (nz - 2);
+    if (t < 2 * a) {
+        const int k = t < a ? 0 : nz - 1, r = t < a ? t : t - a;
+        return B.IX(r % nx, r / nx, k);
+    }
+    t -= 2 * a;
+    if (t < 2 * b) {
+        const int j = t < b ? 0 : ny - 1, r = t < b ? t : t - b;
+        return B.IX(r % nx, j, 1 + r / nx);
+    }
+    t -= 2 * b;
+    const int i = t < d ? 0 : nx - 1, r = t < d ? t : t - d;
+    return B.IX(i, 1 + r % (ny - 2), 1 + r / (ny - 2));
+}
+
+// dst += src on the boundary cells (blockIdx.y = field)
 __global__ void k_add_bnd(Box B, FieldPtrs dst, FieldPtrs src) {
-    int i, j, k;
-    if (!cell_of(B, gtid(), i, j, k) || B.interior(i, j, k)) return;
-    const int c = B.IX(i, j, k), f = blockIdx.y;
+    const int t = gtid();
+    if (t >= bnd_cells(B)) return;
+    const int c = bnd_cell(B, t), f = blockIdx.y;
     dst.p[f][c] += src.p[f][c];
+}
+// zero n cells of each listed array (blockIdx.y = array)
+__global__ void k_zero_list(int n, FieldPtrs p) {
+    const int c = gtid();
+    if (c < n) p.p[blockIdx.y][c] = 0.0f;
 }
 __global__ void k_add_all(int n, FieldPtrs dst, FieldPtrs src) {
     const int c = gtid();
@@ -49,9 +78,21 @@ __global__ void k_add_all(int n, FieldPtrs dst, FieldPtrs src) {
     dst.p[blockIdx.y][c] += src.p[blockIdx.y][c];
 }
 
-struct EF {   // one stage's E-field intermediates
+struct EF {   // one stage's E-field intermediates (16 arrays)
     float *J[3], *bs[3], *btot, *ev[3], *tv[3], *E[3];
 };
+EF ef_at(float* base, int n) {
+    EF e;
+    int q = 0;
+    auto take = [&]() { return base + (size_t)n * q++; };
+    for (int a = 0; a < 3; ++a) e.J[a] = take();
+    for (int a = 0; a < 3; ++a) e.bs[a] = take();
+    e.btot = take();
+    for (int a = 0; a < 3; ++a) e.ev[a] = take();
+    for (int a = 0; a < 3; ++a) e.tv[a] = take();
+    for (int a = 0; a < 3; ++a) e.E[a] = take();
+    return e;
+}
 
 struct FullModel {
     g3m3_full_model_desc d;
@@ -80,6 +121,11 @@ bool launched(const char* where) {
 
 void zero(FullModel* M, float* p, size_t floats) {
     hipMemsetAsync(p, 0, sizeof(float) * floats, M->s);
+}
+void zero_vb(FullModel* M) {
+    FieldPtrs z{};
+    for (int a = 0; a < 3; ++a) z.p[a] = M->VB[a];
+    hipLaunchKernelGGL(k_zero_list, dim3((M->n + 255) / 256, 3), dim3(256), 0, M->s, M->n, z);
 }
 void copyf(FullModel* M, float* dst, const float* src, int nf) {
     hipMemcpyAsync(dst, src, sizeof(float) * (size_t)M->n * nf, hipMemcpyDeviceToDevice, M->s);
@@ -229,9 +275,9 @@ void* g3m3_full_model_create(const g3m3_full_model_desc* desc) {
     M->B = Box{desc->nx, desc->ny, desc->nz};
     M->n = M->B.V();
     const size_t n = M->n;
-    // E-field 16 | qv 3 | vel 3 | own tape T_N | A 34 | W3B, S2B, W1B 34 each |
-    // EB, EVB, CURB, BSB 3 each, BTB 1, TVB 3, VB 3 | fct 24 | lap 6 | ion 15 | ef 3
-    const size_t total = 16 + 3 + 3 + T_N + 34 * 4 + 12 + 1 + 6 +
+    // qv 3 | vel 3 | own tape T_N | A 34 | W3B, S2B, W1B 34 each |
+    // EB, EVB, CURB, BSB 3 each, BTB 1, TVB 3, VB 3 | fct | lap 6 | ion 15 | ef 3
+    const size_t total = 3 + 3 + T_N + 34 * 4 + 12 + 1 + 6 +
                          adj_fct_scratch_floats(M->B) / n + 6 + 15 + 3;
     if (hipMalloc(&M->mem, sizeof(float) * total * n) != hipSuccess) {
         fprintf(stderr, "g3m3_full_model_create: out of device memory\n");
@@ -244,12 +290,6 @@ void* g3m3_full_model_create(const g3m3_full_model_desc* desc) {
         p += k * n;
         return r;
     };
-    for (int a = 0; a < 3; ++a) M->ef.J[a] = take(1);
-    for (int a = 0; a < 3; ++a) M->ef.bs[a] = take(1);
-    M->ef.btot = take(1);
-    for (int a = 0; a < 3; ++a) M->ef.ev[a] = take(1);
-    for (int a = 0; a < 3; ++a) M->ef.tv[a] = take(1);
-    for (int a = 0; a < 3; ++a) M->ef.E[a] = take(1);
     for (int a = 0; a < 3; ++a) M->qv[a] = take(1);
     for (int a = 0; a < 3; ++a) M->vel[a] = take(1);
     M->own_tape = take(T_N);
@@ -309,6 +349,7 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
     float* S4 = fld(T, n, T_S4);
     copyf(M, S, s_in, NST);
     // stage 1
+    M->ef = ef_at(fld(T, n, T_EF1), n);
     efield(M, S, eta);
     copyf(M, W1, S, NFLD);
     copyf(M, fld(W1, n, EP), fld(S, n, WE), 1);
@@ -316,6 +357,7 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
     push_b(M, W1, S, 0.5f * delt);
     outer(M, W1, I);
     // stage 2
+    M->ef = ef_at(fld(T, n, T_EF2), n);
     efield(M, W1, eta);
     copyf(M, S2, S, NFLD);
     push_species(M, S2, S, W1, delt);
@@ -381,19 +423,22 @@ namespace {
 // the adjoint of one stage's push and E-field: from the stage output's
 // adjoint `ob` (34 fields), accumulate into the source set's adjoint `sb`
 // and the old set's adjoint `oldb`; src is the stage's source set, old S
-void stage_adj(FullModel* M, const float* src, const float* old, float* ob, float* sb,
-               float* oldb, float delt, bool elec, double* ctl_bar, float eta) {
+void stage_adj(FullModel* M, const EF& ef, const float* src, const float* old, float* ob,
+               float* sb, float* oldb, float delt, bool elec, double* ctl_bar, float eta) {
     const g3m3_full_model_desc& d = M->d;
     const int n = M->n;
     const float r = d.rx;
     hipStream_t s = M->s;
-    efield(M, src, eta);
-    for (int a = 0; a < 3; ++a) {
-        zero(M, M->BSB[a], n);
-        zero(M, M->TVB[a], n);
-        zero(M, M->CURB[a], n);
+    {
+        FieldPtrs z{};
+        for (int a = 0; a < 3; ++a) {
+            z.p[a] = M->BSB[a];
+            z.p[3 + a] = M->TVB[a];
+            z.p[6 + a] = M->CURB[a];
+        }
+        z.p[9] = M->BTB;
+        hipLaunchKernelGGL(k_zero_list, dim3((n + 255) / 256, 10), dim3(256), 0, s, n, z);
     }
-    zero(M, M->BTB, n);
     // push_bfld: old's adjoint, E's adjoint (assigned)
     {
         const float* bb[3] = {fld(ob, n, BX), fld(ob, n, BX + 1), fld(ob, n, BX + 2)};
@@ -412,14 +457,14 @@ void stage_adj(FullModel* M, const float* src, const float* old, float* ob, floa
             A.oldb[f] = fld(oldb, n, 10 * sp + f);
         }
         for (int a = 0; a < 3; ++a) {
-            A.bs[a] = M->ef.bs[a];
-            A.e[a] = M->ef.E[a];
-            A.tv[a] = M->ef.tv[a];
+            A.bs[a] = ef.bs[a];
+            A.e[a] = ef.E[a];
+            A.tv[a] = ef.tv[a];
             A.bsb[a] = M->BSB[a];
             A.eb[a] = M->EB[a];
             A.tvb[a] = M->TVB[a];
         }
-        A.btot = M->ef.btot;
+        A.btot = ef.btot;
         A.btotb = M->BTB;
         A.scr = M->ion_scr;
         A.rmass = rm[sp];
@@ -445,9 +490,9 @@ void stage_adj(FullModel* M, const float* src, const float* old, float* ob, floa
     E.B = M->B;
     for (int a = 0; a < 3; ++a) {
         E.eb[a] = M->EB[a];
-        E.bs[a] = M->ef.bs[a];
-        E.ev[a] = M->ef.ev[a];
-        E.cur[a] = M->ef.J[a];
+        E.bs[a] = ef.bs[a];
+        E.ev[a] = ef.ev[a];
+        E.cur[a] = ef.J[a];
         E.tvb[a] = M->TVB[a];
         E.evb[a] = M->EVB[a];
         E.bsb[a] = M->BSB[a];
@@ -479,7 +524,7 @@ void stage_adj(FullModel* M, const float* src, const float* old, float* ob, floa
         P.B = M->B;
         P.w = fld((float*)src, n, EP);
         for (int a = 0; a < 3; ++a) {
-            P.ev[a] = M->ef.ev[a];
+            P.ev[a] = ef.ev[a];
             P.evb[a] = M->EVB[a];
         }
         P.ob = fld(ob, n, EP);
@@ -493,7 +538,7 @@ void stage_adj(FullModel* M, const float* src, const float* old, float* ob, floa
     }
     launch_adj_evel_plasma(E, s);
     // total B: btot, then bs = b0 + b
-    const float* bsc[3] = {M->ef.bs[0], M->ef.bs[1], M->ef.bs[2]};
+    const float* bsc[3] = {ef.bs[0], ef.bs[1], ef.bs[2]};
     launch_adj_btot(n, bsc, M->BTB, d.smallbit, M->BSB, s);
     float* bb[3] = {fld(sb, n, BX), fld(sb, n, BX + 1), fld(sb, n, BX + 2)};
     const float* bsb[3] = {M->BSB[0], M->BSB[1], M->BSB[2]};
@@ -525,8 +570,7 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
     copyf(M, M->A, sout_bar, NFLD);
     // S' = bndry_outer(fcsmooth(S, W3))
     outer_adj(M, M->A, S4, I, ctl_bar);
-    zero(M, M->W3B, (size_t)NFLD * n);
-    for (int f0 = 0; f0 < NFLD;) {
+    for (int f0 = 0; f0 < NFLD;) {   // (assigns W3B)
         // groups of up to three fields of one coefficient
         const int end = f0 < BX ? BX : NFLD;
         const int nf = end - f0 < 3 ? end - f0 : 3;
@@ -560,14 +604,14 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
             dst.p[f] = fld(M->W1B, n, f);
             src.p[f] = fld(M->W3B, n, f);
         }
-        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, NFLD), dim3(256), 0, s, M->B, dst,
-                           src);
+        hipLaunchKernelGGL(k_add_bnd, dim3((bnd_cells(M->B) + 255) / 256, NFLD), dim3(256), 0,
+                           s, M->B, dst, src);
     }
     for (int sp = 0; sp < 3; ++sp) {
         const float* pp[3] = {fld((float*)S2, n, 10 * sp + 4), fld((float*)S2, n, 10 * sp + 5),
                               fld((float*)S2, n, 10 * sp + 6)};
         launch_velocity(n, fld((float*)S2, n, 10 * sp), pp, M->vel, d.smallbit, s);
-        for (int a = 0; a < 3; ++a) zero(M, M->VB[a], n);
+        zero_vb(M);
         LapAdj L{};
         L.B = M->B;
         for (int f = 0; f < 10; ++f) {
@@ -605,7 +649,7 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
             }
         }
         launch_vtot(n, mom, rho, M->vel, s);
-        for (int a = 0; a < 3; ++a) zero(M, M->VB[a], n);
+        zero_vb(M);
         LapAdj L{};
         L.B = M->B;
         for (int f = 0; f < 4; ++f) {
@@ -633,10 +677,11 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
             dst.p[f] = fld(SB, n, f);
             src.p[f] = fld(M->S2B, n, f);
         }
-        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, EP + 1), dim3(256), 0, s, M->B, dst,
-                           src);
+        hipLaunchKernelGGL(k_add_bnd, dim3((bnd_cells(M->B) + 255) / 256, EP + 1), dim3(256), 0,
+                           s, M->B, dst, src);
     }
-    stage_adj(M, W1, S, M->S2B, M->W1B, SB, delt, true, ctl_bar, eta);
+    stage_adj(M, ef_at(fld(T, n, T_EF2), n), W1, S, M->S2B, M->W1B, SB, delt, true, ctl_bar,
+              eta);
     // W1 = bndry_outer(push from S with delt/2; its pressure is WE)
     outer_adj(M, M->W1B, W1, I, ctl_bar);
     {
@@ -649,10 +694,11 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
             dst.p[f] = fld(SB, n, f);
             src.p[f] = fld(M->W1B, n, f);
         }
-        hipLaunchKernelGGL(k_add_bnd, dim3((n + 255) / 256, EP + 1), dim3(256), 0, s, M->B, dst,
-                           src);
+        hipLaunchKernelGGL(k_add_bnd, dim3((bnd_cells(M->B) + 255) / 256, EP + 1), dim3(256), 0,
+                           s, M->B, dst, src);
     }
-    stage_adj(M, S, S, M->W1B, SB, SB, 0.5f * delt, false, ctl_bar, eta);
+    stage_adj(M, ef_at(fld(T, n, T_EF1), n), S, S, M->W1B, SB, SB, 0.5f * delt, false, ctl_bar,
+              eta);
     return launched("adjoint") ? 0 : 2;
 }
 

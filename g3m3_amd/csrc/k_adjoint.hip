@@ -429,8 +429,7 @@ struct FctAdj {
     const float* wrk[3];
     float* ob[3];        // the output's adjoint (read)
     float* D[3];         // scratch: the diffused field
-    float* P[3][3];      // scratch: d1's adjoint per axis
-    float* Q[3][3];      // scratch: d's adjoint per axis
+    unsigned char* code[3];   // scratch: the limiter's branch per axis (2 bits each)
     float* Db[3];        // scratch: D's adjoint
     float* wb[3];        // wrk's adjoint (assigned)
     float* oldb[3];      // old's adjoint (accumulated)
@@ -474,61 +473,78 @@ __device__ inline int fct_branch(const float* D, const float* W, int c0, int st,
     return mn > 0.0f ? br : 0;
 }
 
-__global__ void k_adj_fct_faces(FctAdj A) {
+// the branch of every interior index of the three lines through a cell,
+// packed two bits per axis (0 outside 1..n-2)
+__global__ void k_adj_fct_codes(FctAdj A) {
     int i, j, k;
     if (!cell_of(A.B, gtid(), i, j, k)) return;
     const int f = blockIdx.y, c = A.B.IX(i, j, k);
-    const int nx = A.B.nx, ny = A.B.ny, nz = A.B.nz;
-    const float* D = A.D[f];
-    const float* W = A.wrk[f];
-    const float* ob = A.ob[f];
-    const int pos[3] = {i, j, k}, len[3] = {nx, ny, nz}, st[3] = {1, nx, nx * ny};
-    const bool kin = k > 0 && k < nz - 1;
+    const int pos[3] = {i, j, k}, len[3] = {A.B.nx, A.B.ny, A.B.nz};
+    const int st[3] = {1, A.B.nx, A.B.nx * A.B.ny};
+    unsigned v = 0;
     for (int ax = 0; ax < 3; ++ax) {
-        const int n = len[ax], s = st[ax], t = pos[ax];
-        const int c0 = c - t * s;
-        // U: the output's adjoint where the flux differences act (interior k)
-        auto U = [&](int u) {
-            if (ax == 2) return (u > 0 && u < nz - 1) ? ob[c0 + u * s] : 0.0f;
-            return kin ? ob[c0 + u * s] : 0.0f;
-        };
-        // adjoint of fn(u) at interior u, the copy fn(0) = fn(1) folded in
-        auto g = [&](int u) { return (u >= 2 ? -U(u) : 0.0f) + (u + 1 <= n - 2 ? U(u + 1) : 0.0f); };
-        float p = 0.0f, q = 0.0f;
-        if (t <= n - 2) {
-            if (t + 1 <= n - 2 && fct_branch(D, W, c0, s, n, t + 1, A.chi) == 1) p += g(t + 1);
-            if (t - 1 >= 1 && fct_branch(D, W, c0, s, n, t - 1, A.chi) == 3) p += g(t - 1);
-            if (t >= 1) {
-                const int br = fct_branch(D, W, c0, s, n, t, A.chi);
-                if (br == 2) q = A.chi * g(t);
-                if (t == n - 2 && br == 3) p += g(t);   // d1(n-1) = d1(n-2)
-            }
-        }
-        A.P[f][ax][c] = p;
-        A.Q[f][ax][c] = q;
+        const int t = pos[ax], n = len[ax];
+        if (t >= 1 && t <= n - 2)
+            v |= (unsigned)fct_branch(A.D[f], A.wrk[f], c - t * st[ax], st[ax], n, t, A.chi)
+                 << (2 * ax);
     }
+    A.code[f][c] = (unsigned char)v;
 }
 
+// per cell: D's and wrk's adjoints.  Along each axis the flux fn(u) of
+// interior index u carries g(u) = -U(u) + U(u+1) (U: the output's adjoint
+// where the axis's flux differences act; the copy fn(0) = fn(1) folded in)
+// to d1(u-1) (branch 1), chi d(u) (branch 2) or d1(u+1) (branch 3, which
+// is d1(n-2) for u = n-2); d1(t) = D(t+1) - D(t), d(t) = wrk(t+1) - wrk(t).
+// The output's own terms: the x pass starts from D on interior i and wrk on
+// the i faces, interior k; the k faces keep wrk.
 __global__ void k_adj_fct_cells(FctAdj A) {
     int i, j, k;
     if (!cell_of(A.B, gtid(), i, j, k)) return;
     const int f = blockIdx.y, c = A.B.IX(i, j, k);
     const int nx = A.B.nx, ny = A.B.ny, nz = A.B.nz;
     const int pos[3] = {i, j, k}, len[3] = {nx, ny, nz}, st[3] = {1, nx, nx * ny};
+    const float* ob = A.ob[f];
+    const unsigned char* code = A.code[f];
+    const float chi = A.chi;
+    const bool kin = k > 0 && k < nz - 1;
     float db = 0.0f, wbar = 0.0f;
+#pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
+        if (ax != 2 && !kin) continue;       // the x and y fluxes act on interior k only
         const int t = pos[ax], s = st[ax], n = len[ax];
+        // U(u), u = t-2 .. t+2; g(u) and the branch br(u), u = t-2 .. t+1
+        float U[5], G[4];
+        int br[4];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int u = t - 2 + q;
+            const bool ok = ax == 2 ? (u > 0 && u < n - 1) : (u >= 0 && u < n);
+            U[q] = ok ? ob[c + (q - 2) * s] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int u = t - 2 + q;
+            G[q] = (u >= 2 ? -U[q] : 0.0f) + (u + 1 <= n - 2 ? U[q + 1] : 0.0f);
+            br[q] = (u >= 1 && u <= n - 2) ? (int)((code[c + (q - 2) * s] >> (2 * ax)) & 3u) : 0;
+        }
+        // adjoints of d1(t-1), d1(t) (p) and of d(t-1), d(t) (q); index q = u - t + 2
+        const float p0 = (br[2] == 1 ? G[2] : 0.0f) + (br[0] == 3 ? G[0] : 0.0f) +
+                         (t - 1 == n - 2 && br[1] == 3 ? G[1] : 0.0f);
+        const float p1 = (br[3] == 1 ? G[3] : 0.0f) + (br[1] == 3 ? G[1] : 0.0f) +
+                         (t == n - 2 && br[2] == 3 ? G[2] : 0.0f);
+        const float q0 = br[1] == 2 ? chi * G[1] : 0.0f;
+        const float q1 = br[2] == 2 ? chi * G[2] : 0.0f;
         if (t >= 1) {
-            db += A.P[f][ax][c - s];
-            wbar += A.Q[f][ax][c - s];
+            db += p0;
+            wbar += q0;
         }
         if (t <= n - 2) {
-            db -= A.P[f][ax][c];
-            wbar -= A.Q[f][ax][c];
+            db -= p1;
+            wbar -= q1;
         }
     }
-    const float o = A.ob[f][c];
-    const bool kin = k > 0 && k < nz - 1;
+    const float o = ob[c];
     if (kin) {
         if (i > 0 && i < nx - 1) db += o;     // the x pass's D term
         else wbar += o;                       // its i-face cells keep wrk
@@ -665,7 +681,7 @@ void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float c
                        v[0], v[1], v[2], chi, delt, qb[0], qb[1], qb[2], w1b[0], w1b[1], w1b[2]);
 }
 
-size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 8 * B.V(); }
+size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 3 * B.V(); }
 
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
                     float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
@@ -675,22 +691,19 @@ void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], floa
     A.B = B;
     A.chi = chi;
     for (int f = 0; f < nf; ++f) {
-        float* base = scratch + (size_t)8 * n * f;
+        float* base = scratch + (size_t)3 * n * f;
         A.old[f] = old[f];
         A.wrk[f] = wrk[f];
         A.ob[f] = ob[f];
         A.D[f] = base;
-        for (int ax = 0; ax < 3; ++ax) {
-            A.P[f][ax] = base + (size_t)n * (1 + ax);
-            A.Q[f][ax] = base + (size_t)n * (4 + ax);
-        }
-        A.Db[f] = base + (size_t)n * 7;
+        A.code[f] = reinterpret_cast<unsigned char*>(base + n);
+        A.Db[f] = base + (size_t)n * 2;
         A.wb[f] = wb[f];
         A.oldb[f] = oldb[f];
     }
     const dim3 g((n + 255) / 256, nf), t(256);
     hipLaunchKernelGGL(k_adj_fct_diffuse, g, t, 0, s, A);
-    hipLaunchKernelGGL(k_adj_fct_faces, g, t, 0, s, A);
+    hipLaunchKernelGGL(k_adj_fct_codes, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_cells, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_old, g, t, 0, s, A);
 }

@@ -805,7 +805,7 @@ typedef struct {
 void* g3m3_full_model_create(const g3m3_full_model_desc* desc);
 void g3m3_full_model_destroy(void* model);
 void g3m3_full_model_set_stream(void* model, void* hip_stream);
-/* floats of one step's tape (171 arrays of nx*ny*nz) */
+/* floats of one step's tape (203 arrays of nx*ny*nz) */
 long long g3m3_full_model_tape_floats(void* model);
 /* s_out = one step from s_in (35 arrays each, distinct); tape (or NULL)
  * receives what the adjoint of the step needs.  Returns 0 on success. */
