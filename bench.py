@@ -437,8 +437,11 @@ def main(argv=None):
                     help="skip the differentiation case (3 nested 81x81x41 boxes, 50 steps, "
                          "craft misfit and its gradient)")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the field-update adjoint leg (checkpointed adjoint of a "
-                         "craft misfit on one 121x121x61 box)")
+                    help="skip the adjoint legs (checkpointed adjoints of craft misfits on "
+                         "one 121x121x61 box: the field update, and the whole box-step)")
+    ap.add_argument("--only-adjoint", choices=("field", "full"),
+                    help="run only that adjoint leg and print its JSON (for a kernel trace "
+                         "of the adjoint alone)")
     ap.add_argument("--cpu-threads", type=int,
                     default=min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
     a = ap.parse_args(argv)
@@ -446,6 +449,11 @@ def main(argv=None):
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU runs are not part of it", file=sys.stderr)
         return 2
+
+    if a.only_adjoint:
+        leg = adjoint_case() if a.only_adjoint == "field" else full_adjoint_case()
+        print(json.dumps({"adjoint": {a.only_adjoint: leg}}), flush=True)
+        return 0
 
     import torch
     from g3m3_amd import _lib

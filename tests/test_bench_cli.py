@@ -19,5 +19,5 @@ def test_help_lists_the_legs():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
     for flag in ("--steps", "--warmup", "--gpus", "--no-batch", "--no-precision", "--driver",
-                 "--species-kernels", "--no-gradient", "--no-adjoint"):
+                 "--species-kernels", "--no-gradient", "--no-adjoint", "--only-adjoint"):
         assert flag in r.stdout
