@@ -1508,8 +1508,12 @@ void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, floa
 
 // Fields per thread: each thread reads its 32-byte table entry once for a
 // group of fields (the table is ~3.6 MB at 121x121x61; read once per field
-// it was most of this kernel's HBM traffic).
-constexpr int kFlankGroup = 9;
+// it was most of this kernel's HBM traffic).  Every load of the group (16
+// slab values and the target per field) is issued before the first store:
+// on this hardware a wait for a load also waits for every store issued
+// before it, so a load-compute-store loop over the fields would serialise
+// one memory latency per field.
+constexpr int kFlankGroup = 6;
 
 __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict__ tab, int npts,
                                                       FlankPtrs Fp, int nf, float t1, float t2) {
@@ -1520,17 +1524,36 @@ __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict_
     const float dx = p.dx, dy = p.dy, dz = p.dz;
     const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
     const int c0 = p.cb;
-    const int f0 = blockIdx.y * kFlankGroup, f1 = min(f0 + kFlankGroup, nf);
-    for (int f = f0; f < f1; ++f) {
-        const float* __restrict__ o = Fp.old[f];
-        const float* __restrict__ q = Fp.cur[f];
-        auto W = [&](int c) { return o[c] * t1 + q[c] * t2; };
-        const float I0 = W(c0) * ddx * ddy * ddz + W(c0 + Z) * ddx * ddy * dz +
-                         W(c0 + Y) * ddx * dy * ddz + W(c0 + Y + Z) * ddx * dy * dz +
-                         W(c0 + X) * dx * ddy * ddz + W(c0 + X + Z) * dx * ddy * dz +
-                         W(c0 + X + Y) * dx * dy * ddz + W(c0 + X + Y + Z) * dx * dy * dz;
-        float* tgt = Fp.tgt[f];
-        float w = tgt[p.c];
+    const int f0 = blockIdx.y * kFlankGroup;
+    const int cn[8] = {c0, c0 + Z, c0 + Y, c0 + Y + Z, c0 + X, c0 + X + Z, c0 + X + Y,
+                       c0 + X + Y + Z};
+    float vo[kFlankGroup][8], vq[kFlankGroup][8], vw[kFlankGroup];
+#pragma unroll
+    for (int g = 0; g < kFlankGroup; ++g) {
+        const int f = f0 + g;
+        if (f < nf) {
+            const float* __restrict__ o = Fp.old[f];
+            const float* __restrict__ q = Fp.cur[f];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                vo[g][n] = o[cn[n]];
+                vq[g][n] = q[cn[n]];
+            }
+            vw[g] = Fp.tgt[f][p.c];
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kFlankGroup; ++g) {
+        const int f = f0 + g;
+        if (f >= nf) break;
+        float W[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) W[n] = vo[g][n] * t1 + vq[g][n] * t2;
+        const float I0 = W[0] * ddx * ddy * ddz + W[1] * ddx * ddy * dz +
+                         W[2] * ddx * dy * ddz + W[3] * ddx * dy * dz +
+                         W[4] * dx * ddy * ddz + W[5] * dx * ddy * dz +
+                         W[6] * dx * dy * ddz + W[7] * dx * dy * dz;
+        float w = vw[g];
 #pragma unroll
         for (int b = 0; b < 6; ++b) {   // bottom, top, front, back, left, right
             if (p.flags & (1 << b)) {
@@ -1539,7 +1562,7 @@ __global__ void __launch_bounds__(256) k_flanks_table(const ShellPt* __restrict_
                 w = frac * w + frac1 * I0;
             }
         }
-        tgt[p.c] = w;
+        Fp.tgt[f][p.c] = w;
     }
 }
 
@@ -1548,7 +1571,11 @@ void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const Flank
     if (npts <= 0 || nf <= 0) return;
     ProfScope prof_("k_flanks_table", s);
     const int ng = (nf + kFlankGroup - 1) / kFlankGroup;
-    hipLaunchKernelGGL(k_flanks_table, dim3((npts + 255) / 256, ng), dim3(256), 0, s, tab, npts,
+    // a multiple of 8 blocks per field group: workgroups are dealt to the 8
+    // XCDs round-robin, so every field group of a table block runs on the
+    // same XCD, whose L2 then holds that block's table entries
+    const int nbx = (((npts + 255) / 256) + 7) & ~7;
+    hipLaunchKernelGGL(k_flanks_table, dim3(nbx, ng), dim3(256), 0, s, tab, npts,
                        Fp, nf, t1, t2);
 }
 
