@@ -12,6 +12,7 @@
 //   k_fct_fused flux-corrected transport of all 34 fields, one pass
 //   k_flanks_table  coarse -> fine boundary on the 2-layer shell only
 #include <algorithm>
+#include <type_traits>
 
 #include "g3m3_dev.h"
 #include "g3m3_fused.h"
@@ -1112,6 +1113,9 @@ __global__ void __launch_bounds__(256) k_lap(LapParams L) {
 #ifndef G3M3_FCT_TILE_ROWS
 #define G3M3_FCT_TILE_ROWS 8
 #endif
+// Word 3 of a raw buffer resource on gfx950: data format 32 bits (bits
+// 15-18 = 4), every other field zero (no swizzle, no index stride).
+constexpr unsigned kBufferWord3 = 0x00020000u;
 constexpr int FTX = 63, FTY = G3M3_FCT_TILE_ROWS, FKC = 16;   // FKC: default k-planes per block
 constexpr int FNT = 32 * FTY;                 // threads per block
 constexpr int FPX = FTX + 5, FPY = FTY + 4;   // wrk / P: cells i0-3 .. i0+64, j0-2 .. j0+FTY+1
@@ -1149,7 +1153,7 @@ __device__ inline float fct_flux(const WA& W, const PA& P, int base, int m, int 
     return fct_lim(w1 - w0, p0 - pm1, d1p, chi);
 }
 
-__global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
+__global__ void __launch_bounds__(FNT, 4) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
     __shared__ FctLds L;
     const Box& B = F.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
@@ -1164,14 +1168,22 @@ __global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty
     const int t = r % (ntx * nty);
     const int i0 = (t % ntx) * FTX, j0 = (t / ntx) * FTY;
     const int kb = kc * F.kc_len, ke = min(nz, kb + F.kc_len);
-    const char* __restrict__ wg = reinterpret_cast<const char*>(F.wrk[f]);
-    const char* __restrict__ og = reinterpret_cast<const char*>(F.old[f]);
+    // wrk and old through buffer resources: the plane's byte offset goes in
+    // a scalar register and each element's in-plane offset in a vector
+    // register, so no per-plane address arithmetic runs on the VALU
+    const int nbytes = nx * ny * nz * (int)sizeof(float);
+    const __amdgpu_buffer_rsrc_t wg =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(F.wrk[f]), (short)0, nbytes,
+                                          kBufferWord3);
+    const __amdgpu_buffer_rsrc_t og =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(F.old[f]), (short)0, nbytes,
+                                          kBufferWord3);
     float* out = F.out[f];
     const float chi = F.chi[f];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = tx + 64 * ty;
     const int plane = nx * ny;
-    const size_t plane_bytes = (size_t)plane * sizeof(float);
+    const int plane_bytes = plane * (int)sizeof(float);
 
     // this thread's tile elements, fixed for the whole march: clamped
     // in-plane byte offsets of the loads, the P element's centre in the old
@@ -1195,19 +1207,18 @@ __global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty
         const int gi = i0 - 4 + ii, gj = j0 - 3 + jj;
         offo[m] = (unsigned)(min(max(gj, 0), ny - 1) * nx + min(max(gi, 0), nx - 1)) * 4u;
     }
-    auto ld = [](const char* base, unsigned off) {
-        return *reinterpret_cast<const float*>(base + off);
-    };
     float rw[FP_PT], ro[FO_PT];
     auto fetch_w = [&](int k) {
-        const char* b = wg + (size_t)k * plane_bytes;
+        const int b = k * plane_bytes;
 #pragma unroll
-        for (int m = 0; m < FP_PT; ++m) rw[m] = ld(b, offw[m]);
+        for (int m = 0; m < FP_PT; ++m)
+            rw[m] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(wg, (int)offw[m], b, 0));
     };
     auto fetch_o = [&](int k) {
-        const char* b = og + (size_t)k * plane_bytes;
+        const int b = k * plane_bytes;
 #pragma unroll
-        for (int m = 0; m < FO_PT; ++m) ro[m] = ld(b, offo[m]);
+        for (int m = 0; m < FO_PT; ++m)
+            ro[m] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(og, (int)offo[m], b, 0));
     };
     auto store_w = [&](float* dst) {
 #pragma unroll
@@ -1309,31 +1320,33 @@ __global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty
     };
 
     // P planes pstart..pend are made in order.  Plane p's wrk, P and old
-    // live in slot p % 3; during stage s, s0 / s1 / s2 are the slots of
-    // planes s / s-1 / s-2 (old(s+1) goes to s2).  Stage s stores wrk(s)
-    // and old(s+1) (fetched during stage s-1), makes P(s), starts the loads
-    // of the next plane and outputs plane s-2.
+    // live in slot (p - pstart) % 3; during stage s, S0 / S1 / S2 are the
+    // slots of planes s / s-1 / s-2 (old(s+1) goes to S2).  Stage s stores
+    // wrk(s) and old(s+1) (fetched during stage s-1), makes P(s), starts the
+    // loads of the next plane and outputs plane s-2.  The march is unrolled
+    // three stages at a time, so every slot is a compile-time constant and
+    // each LDS access is a fixed offset from a per-thread base.
     const int pstart = max(0, kb - 2), pend = min(nz - 1, ke + 1);
-    int s0 = pstart % 3, s1 = (pstart + 2) % 3, s2 = (pstart + 1) % 3;
     if (pstart >= 1) {
         fetch_o(pstart - 1);
-        store_o(L.o[s1]);
+        store_o(L.o[2]);
     }
     fetch_o(pstart);
-    store_o(L.o[s0]);
+    store_o(L.o[0]);
     fetch_w(pstart);
     if (pstart + 1 <= nz - 1) fetch_o(pstart + 1);
-    for (int s = pstart; s <= pend; ++s) {
+    auto stage = [&](auto rc, int s) {
+        constexpr int S0 = decltype(rc)::value, S1 = (S0 + 2) % 3, S2 = (S0 + 1) % 3;
         __syncthreads();   // readers of the slots reused below are done
-        store_w(L.w[s0]);
-        if (s + 1 <= nz - 1) store_o(L.o[s2]);
+        store_w(L.w[S0]);
+        if (s + 1 <= nz - 1) store_o(L.o[S2]);
         __syncthreads();
         {   // P(s) = w + chi * Lap7(old) inside, w on the box faces
             // (code/sub_fcsmooth.f90:44-83)
-            const float* om = L.o[s1];
-            const float* oc = L.o[s0];
-            const float* op = L.o[s2];
-            float* dst = L.p[s0];
+            const float* om = L.o[S1];
+            const float* oc = L.o[S0];
+            const float* op = L.o[S2];
+            float* dst = L.p[S0];
             const bool zface = (s == 0 || s == nz - 1);
 #pragma unroll
             for (int m = 0; m < FP_PT; ++m) {
@@ -1348,24 +1361,32 @@ __global__ void __launch_bounds__(FNT) k_fct_fused(FctParams F, int ntx, int nty
         if (s + 1 <= pend) fetch_w(s + 1);   // in flight during the output
         if (s + 2 <= nz - 1) fetch_o(s + 2);
         __syncthreads();
-        shift(L.p[s0][pa], L.p[s0][pb], L.w[s1][pa], L.w[s1][pb]);
+        shift(L.p[S0][pa], L.p[S0][pb], L.w[S1][pa], L.w[S1][pb]);
         const int k = s - 2;
-        if (k >= kb && k < ke) output(k, L.w[s2], L.p[s2]);
-        const int tmp = s2;
-        s2 = s1;
-        s1 = s0;
-        s0 = tmp;
-    }
+        if (k >= kb && k < ke) output(k, L.w[S2], L.p[S2]);
+    };
     // planes past the last P made (only at the top of the box: plane nz-2
     // needs P up to nz-1, plane nz-1 is a z face)
-    for (int s = pend + 1; s - 2 < ke; ++s) {
-        shift(0.0f, 0.0f, L.w[s1][pa], L.w[s1][pb]);
+    auto tail = [&](auto rc, int s) {
+        constexpr int S0 = decltype(rc)::value, S1 = (S0 + 2) % 3, S2 = (S0 + 1) % 3;
+        (void)S0;
+        shift(0.0f, 0.0f, L.w[S1][pa], L.w[S1][pb]);
         const int k = s - 2;
-        if (k >= kb) output(k, L.w[s2], L.p[s2]);
-        const int tmp = s2;
-        s2 = s1;
-        s1 = s0;
-        s0 = tmp;
+        if (k >= kb) output(k, L.w[S2], L.p[S2]);
+    };
+    using R0 = std::integral_constant<int, 0>;
+    using R1 = std::integral_constant<int, 1>;
+    using R2 = std::integral_constant<int, 2>;
+    for (int s = pstart; s <= pend; s += 3) {
+        stage(R0{}, s);
+        if (s + 1 <= pend) stage(R1{}, s + 1);
+        if (s + 2 <= pend) stage(R2{}, s + 2);
+    }
+    for (int s = pend + 1; s - 2 < ke; ++s) {
+        const int r = (s - pstart) % 3;
+        if (r == 0) tail(R0{}, s);
+        else if (r == 1) tail(R1{}, s);
+        else tail(R2{}, s);
     }
 }
 
