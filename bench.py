@@ -146,6 +146,24 @@ def batch_case(n: int = 32, steps: int = 20, streams: int = 8):
     torch.cuda.synchronize()
     t_graph = time.perf_counter() - t0
     cu_graph = g.cell_updates - cu0
+    del g
+    # the same runs in lockstep groups: one multi-run launch per stage and
+    # group, one captured graph per step and group, the groups on their own
+    # streams (set-up and recording outside the timing); every group count
+    # tried, the fastest reported
+    fused = {}
+    for ng in (1, 2, 4, 8):
+        fb = Batch(wls, streams=1)
+        fb.init_dt()
+        fb.fused_steps(0, groups=ng)
+        torch.cuda.synchronize()
+        cu0 = fb.cell_updates
+        t0 = time.perf_counter()
+        fb.fused_steps(steps, groups=ng)
+        torch.cuda.synchronize()
+        fused[ng] = ((fb.cell_updates - cu0) / (time.perf_counter() - t0), fb.fused_launches)
+        del fb
+    best = max(fused, key=lambda g: fused[g][0])
     plain = cu_plain / t_plain
     return {"runs": n, "steps": steps, "streams": streams,
             "cells_min_max": [min(w.cells_per_box for w in wls), max(w.cells_per_box for w in wls)],
@@ -153,7 +171,13 @@ def batch_case(n: int = 32, steps: int = 20, streams: int = 8):
             "batched_cell_updates_per_s": cu_batch / t_batch,
             "speedup": (cu_batch / t_batch) / plain,
             "graph_cell_updates_per_s": cu_graph / t_graph,
-            "graph_speedup": (cu_graph / t_graph) / plain}
+            "graph_speedup": (cu_graph / t_graph) / plain,
+            "fused_cell_updates_per_s": fused[best][0],
+            "fused_speedup": fused[best][0] / plain,
+            "fused_groups": best,
+            "fused_launches_per_step": fused[best][1],
+            "fused_candidates": {str(g): {"cell_updates_per_s": v[0], "launches_per_step": v[1]}
+                                 for g, v in fused.items()}}
 
 
 def gradient_case():

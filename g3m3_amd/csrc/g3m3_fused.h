@@ -5,6 +5,7 @@
 
 #include <cmath>
 
+#include <cstring>
 #include <vector>
 
 #include "g3m3_dev.h"
@@ -143,6 +144,112 @@ void launch_slab_gather(const int* idx, int n, const FlankPtrs& Fp, int nf, floa
                         hipStream_t s);
 void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const FlankPtrs& Fp, int nf,
                          float t1, float t2, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Several runs in one launch.  Run r owns the logical blocks [start[r],
+// start[r + 1]) of the launch (start[nruns] = total); the grid is the total
+// padded to a multiple of 8, taken in the XCD-aware order, so each XCD
+// sweeps a contiguous run of the concatenated blocks.  Per-run arguments
+// live in device memory, one struct per run.
+struct MultiRun {
+    const int* start;
+    int nruns, total;
+};
+// atomicAdd on a counter in device memory, issued as a global atomic.  A
+// pointer read from device memory (a multi-run launch's per-run arguments)
+// is not known to the compiler to point there, and it would otherwise
+// issue a flat atomic, which runs many times slower here on a contended
+// counter.
+__device__ inline int atomic_add_global(int* p, int v) {
+    return __hip_atomic_fetch_add((__attribute__((address_space(1))) int*)p, v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The run of this block of a multi-run launch and the block's logical
+// index within that run's blocks; false for the grid's padding blocks
+__device__ inline bool multi_block(const MultiRun& M, int& r, int& lb) {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int g = (b & 7) * (nb >> 3) + (b >> 3);   // the XCD-aware order
+    if (g >= M.total) return false;
+    int lo = 0, hi = M.nruns - 1;   // the last run whose first block is <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (M.start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    r = lo;
+    lb = g - M.start[lo];
+    return true;
+}
+// the arguments of the stage kernels that take more than their parameter
+// struct (as one struct, the form a multi-run launch reads per run)
+struct MarchArgs {
+    PushParams P;
+    int ntx, nty, nkc;
+};
+struct LapMarchArgs {
+    LapParams L;
+    int ntx, nty, nkc;
+};
+struct FctArgs {
+    FctParams F;
+    int ntx, nty, nkc;
+};
+struct ApplyArgs {
+    ValParams Q;
+    float* maxima;
+    int nblocks;
+};
+struct OuterArgs {
+    Box B;
+    OuterPtrs O;
+    int pass;
+};
+// which = -1: the three lists of Ls (blockIdx.y = the list); 0..2 one list
+struct InnerArgs {
+    Box B;
+    InnerPtrs I;
+    InnerLists Ls;
+    int which, mbndry, b;
+    float xmin, ymin, dx, dy, re_equiv, v_rot;
+};
+struct NextDtArgs {
+    const float* m;
+    float stepsz, xspac, t_step_max;
+    float* dt;
+    double* t;
+};
+// the kernels a single-grid box-step launches, as recorded launches
+enum KernelId {
+    kKEfield = 0, kKEfieldP2, kKPushMarch, kKPushMarchP2, kKPush, kKPushP2, kKValidate,
+    kKValidateApply, kKLapMarch, kKLap, kKFct, kKOuter, kKInner, kKNextDt, kKernelIds
+};
+// One launch as the one-run path would issue it: the kernel, its grid and
+// block, and its arguments (the struct above, or the parameter struct)
+struct LaunchRec {
+    int id;
+    dim3 grid, block;
+    std::vector<unsigned char> args;
+};
+struct Recorder {
+    std::vector<LaunchRec> recs;
+    template <class A>
+    void add(int id, dim3 grid, dim3 block, const A& a) {
+        LaunchRec r{id, grid, block, {}};
+        r.args.resize(sizeof(A));
+        std::memcpy(r.args.data(), &a, sizeof(A));
+        recs.push_back(std::move(r));
+    }
+};
+// Non-null: the launchers of the stage kernels append to it instead of
+// launching (the host thread's own recorder)
+extern thread_local Recorder* g_rec;
+size_t kernel_args_size(int id);
+// every run's arguments for kernel id (nruns structs of kernel_args_size)
+void launch_multi(int id, const void* args, const MultiRun& M, dim3 block, unsigned gy,
+                  hipStream_t s);
+void launch_multi_bounds(int id, const void* args, const MultiRun& M, dim3 block, unsigned gy,
+                         hipStream_t s);
 
 void launch_efield(const EParams& P, hipStream_t s);
 void launch_push(const PushParams& P, hipStream_t s);

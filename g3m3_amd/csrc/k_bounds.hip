@@ -5,6 +5,7 @@
 //   corer         (code/sub_corer.f90:5-55)         fine -> coarse core
 //   flanks_synced (code/sub_flanks_synced.f90:5-141) coarse -> fine faces
 #include "g3m3_dev.h"
+#include "g3m3_fused.h"
 #include "g3m3_launch.h"
 
 namespace g3m3 {
@@ -12,9 +13,9 @@ namespace g3m3 {
 // ---------------------------------------------------------------------------
 // bndry_outer as eight ordered passes over all fields (blockIdx.y = field).
 // Pass p covers a 2-D index range (a, b); see outer_pass_extent().
-__global__ void k_outer_pass(Box B, OuterPtrs O, int pass) {
+__device__ __forceinline__ void outer_pass_block(const Box& B, const OuterPtrs& O, int pass, int lb) {
     const int f = blockIdx.y;
-    const int t = gtid();
+    const int t = lb * 256 + threadIdx.x;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     float* q = O.f[f];
     const int kind = O.kind[f];
@@ -87,12 +88,19 @@ __global__ void k_outer_pass(Box B, OuterPtrs O, int pass) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_outer_pass(Box B, OuterPtrs O, int pass) {
+    outer_pass_block(B, O, pass, blockIdx.x);
+}
+
 void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s) {
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     const int ext[8] = {ny * nz, ny * nz, (nx - 1) * nz, (nx - 1) * ny, (nx - 1) * ny,
                         nz - 2, ny - 2, nx - 1};
-    for (int p = 0; p < 8; ++p)
-        hipLaunchKernelGGL(k_outer_pass, dim3((ext[p] + 255) / 256, nf), dim3(256), 0, s, B, O, p);
+    for (int p = 0; p < 8; ++p) {
+        const dim3 g((ext[p] + 255) / 256, nf);
+        if (g_rec) g_rec->add(kKOuter, g, dim3(256), OuterArgs{B, O, p});
+        else hipLaunchKernelGGL(k_outer_pass, g, dim3(256), 0, s, B, O, p);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -139,47 +147,76 @@ __device__ inline void inner_point(const Box& B, const InnerPtrs& I, int which,
     }
 }
 
-__global__ void k_inner_list(Box B, InnerPtrs I, int which, const int* __restrict__ ij,
-                             const float* __restrict__ parm, int num, int mbndry, int b,
-                             float xmin, float ymin, float dx, float dy, float re_equiv,
-                             float v_rot) {
-    const int n = gtid();
-    if (n >= num) return;
-    inner_point(B, I, which, ij, parm, n, mbndry, b, xmin, ymin, dx, dy, re_equiv, v_rot);
+// which >= 0: list `which` of A.Ls; which = -1: the three lists in one
+// launch (blockIdx.y = list), only for lists with no cell in common, where
+// their order does not matter
+__device__ __forceinline__ void inner_block(const InnerArgs& A, int lb) {
+    const int w = A.which >= 0 ? A.which : (int)blockIdx.y;
+    const int n = lb * 256 + threadIdx.x;
+    if (n >= A.Ls.num[w]) return;
+    inner_point(A.B, A.I, w, A.Ls.ij[w], A.Ls.parm[w], n, A.mbndry, A.b, A.xmin, A.ymin, A.dx,
+                A.dy, A.re_equiv, A.v_rot);
 }
 
-// the three lists in one launch (blockIdx.y = list); only for lists with
-// no cell in common, where their order does not matter
-__global__ void k_inner_lists(Box B, InnerPtrs I, InnerLists Ls, int mbndry, int b, float xmin,
-                              float ymin, float dx, float dy, float re_equiv, float v_rot) {
-    const int w = blockIdx.y;
-    const int n = gtid();
-    if (n >= Ls.num[w]) return;
-    inner_point(B, I, w, Ls.ij[w], Ls.parm[w], n, mbndry, b, xmin, ymin, dx, dy, re_equiv, v_rot);
-}
+__global__ void __launch_bounds__(256) k_inner_lists(InnerArgs A) { inner_block(A, blockIdx.x); }
 
 void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], const float* parm[3],
                         const int num[3], int mbndry, int box, float xmin, float ymin, float dx,
                         float dy, float re_equiv, float v_rot, hipStream_t s, bool disjoint) {
+    InnerArgs A{};
+    A.B = B;
+    A.I = I;
+    int nmax = 0;
+    for (int w = 0; w < 3; ++w) {
+        A.Ls.ij[w] = ij[w];
+        A.Ls.parm[w] = parm[w];
+        A.Ls.num[w] = num[w] > 0 ? num[w] : 0;
+        nmax = num[w] > nmax ? num[w] : nmax;
+    }
+    A.mbndry = mbndry;
+    A.b = box - 1;
+    A.xmin = xmin;
+    A.ymin = ymin;
+    A.dx = dx;
+    A.dy = dy;
+    A.re_equiv = re_equiv;
+    A.v_rot = v_rot;
+    auto go = [&](dim3 g) {
+        if (g_rec) g_rec->add(kKInner, g, dim3(256), A);
+        else hipLaunchKernelGGL(k_inner_lists, g, dim3(256), 0, s, A);
+    };
     if (disjoint) {
-        InnerLists Ls;
-        int nmax = 0;
-        for (int w = 0; w < 3; ++w) {
-            Ls.ij[w] = ij[w];
-            Ls.parm[w] = parm[w];
-            Ls.num[w] = num[w] > 0 ? num[w] : 0;
-            nmax = num[w] > nmax ? num[w] : nmax;
-        }
-        if (nmax > 0)
-            hipLaunchKernelGGL(k_inner_lists, dim3((nmax + 255) / 256, 3), dim3(256), 0, s, B, I,
-                               Ls, mbndry, box - 1, xmin, ymin, dx, dy, re_equiv, v_rot);
+        A.which = -1;
+        if (nmax > 0) go(dim3((nmax + 255) / 256, 3));
         return;
     }
     for (int w = 0; w < 3; ++w)
-        if (num[w] > 0)
-            hipLaunchKernelGGL(k_inner_list, dim3((num[w] + 255) / 256), dim3(256), 0, s, B, I, w,
-                               ij[w], parm[w], num[w], mbndry, box - 1, xmin, ymin, dx, dy,
-                               re_equiv, v_rot);
+        if (num[w] > 0) {
+            A.which = w;
+            go(dim3((num[w] + 255) / 256));
+        }
+}
+
+__global__ void __launch_bounds__(256) k_outer_pass_runs(const OuterArgs* __restrict__ A,
+                                                         MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb)) return;
+    const OuterArgs& a = A[r];
+    outer_pass_block(a.B, a.O, a.pass, lb);
+}
+
+__global__ void __launch_bounds__(256) k_inner_runs(const InnerArgs* __restrict__ A, MultiRun M) {
+    int r, lb;
+    if (multi_block(M, r, lb)) inner_block(A[r], lb);
+}
+
+void launch_multi_bounds(int id, const void* args, const MultiRun& M, dim3 block, unsigned gy,
+                         hipStream_t s) {
+    const dim3 g((unsigned)((M.total + 7) & ~7), gy);
+    if (id == kKOuter)
+        hipLaunchKernelGGL(k_outer_pass_runs, g, block, 0, s, (const OuterArgs*)args, M);
+    else if (id == kKInner)
+        hipLaunchKernelGGL(k_inner_runs, g, block, 0, s, (const InnerArgs*)args, M);
 }
 
 // ---------------------------------------------------------------------------

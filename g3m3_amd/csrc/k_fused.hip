@@ -32,10 +32,13 @@ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > 
 // reads in its own L2.  The grid is padded to a multiple of 8 blocks.
 __device__ inline int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
 
-__device__ inline bool cell3(const Box& B, int& i, int& j, int& k, int& g) {
+// the cells of logical block lb (blockIdx.y = the group)
+__device__ inline bool cell3(const Box& B, int lb, int& i, int& j, int& k, int& g) {
     g = blockIdx.y;
-    return cell_of_fast(B, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x, i, j, k);
+    return cell_of_fast(B, lb * 256 + threadIdx.x, i, j, k);
 }
+// this launch's logical block: the XCD-aware order of a one-run launch
+__device__ inline int lblock() { return xcd_block(blockIdx.x, gridDim.x); }
 inline dim3 grid3(const Box& B, int groups) {
     const int nb = (B.V() + 255) / 256;
     return dim3((nb + 7) & ~7, groups, 1);
@@ -151,10 +154,10 @@ __device__ inline void efield_interior(const EParams& P, const CurlCtx<P2>& C, i
 }
 
 template <bool P2>
-__global__ void __launch_bounds__(256) k_efield(EParams P) {
+__device__ __forceinline__ void efield_block(const EParams& P, int lb) {
     int i, j, k, g;
     const Box& B = P.B;
-    if (!cell3(B, i, j, k, g)) return;
+    if (!cell3(B, lb, i, j, k, g)) return;
     const int c = B.IX(i, j, k);
     const CurlCtx<P2> C{B, P.b[0], P.b[1], P.b[2], P.dxt, P.dyt, P.dzt, P.rdxt, P.rdyt, P.rdzt};
     float Ex, Ey, Ez;
@@ -463,14 +466,14 @@ __device__ inline void march_store(MarchLds& L, int slot, int row, int l, const 
 }
 
 template <bool P2>
-__global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx, int nty, int nkc) {
+__device__ __forceinline__ void push_march_block(const PushParams& P, int ntx, int nty, int nkc, int lb) {
     __shared__ MarchLds L;
     const Box B = P.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     // logical block order: species fastest, so the blocks of one tile's
     // species are neighbours in one XCD's run of blocks and share the
     // tile's b_s, E, tv and btot reads through that XCD's L2
-    int t = xcd_block(blockIdx.x, gridDim.x);
+    int t = lb;
     const int s = t % P.nspecies;
     t /= P.nspecies;
     const int tx = t % ntx;
@@ -648,9 +651,9 @@ __device__ inline void push_shell_copy_all(const PushParams& P, int c) {
 // boundary threads copy the species' shells as well -- except on the x
 // faces between the j and k faces, which k_push_march has copied.
 template <bool P2>
-__global__ void __launch_bounds__(256) k_push(PushParams P) {
+__device__ __forceinline__ void push_block(const PushParams& P, int lb) {
     int i, j, k, s;
-    if (!cell3(P.B, i, j, k, s)) return;
+    if (!cell3(P.B, lb, i, j, k, s)) return;
     if (P.march) s = 3;
     if (!P.B.interior(i, j, k)) {
         if (P.copy_shell) {
@@ -695,12 +698,13 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_validate(ValParams Q) {
+// lb: the logical block (its cells); pb: the block's slot in Q.partials
+__device__ __forceinline__ void validate_block(const ValParams& Q, int lb, int pb) {
     const Box& B = Q.B;
     int i, j, k, g;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, mc = 0.0f, ma = 0.0f;
     const float dmins[3] = {2.e-4f, 1.e-4f, 2.e-5f};
-    if (cell3(B, i, j, k, g)) {
+    if (cell3(B, lb, i, j, k, g)) {
         const int c = B.IX(i, j, k);
         const bool inner = B.interior(i, j, k);
         // every centre operand first: a list append (a store) inside one
@@ -782,7 +786,7 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
                 e.v[8] = a6i(sxz);
                 e.v[9] = a6i(syz);
                 rho_final[s] = e.v[0];
-                Q.list[atomicAdd(Q.count, 1)] = e;
+                Q.list[atomic_add_global(Q.count, 1)] = e;
             } else if (cf.hit) {
                 ValEntry e;
                 e.cell = c;
@@ -795,7 +799,7 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
                 e.v[5] = cv[s][5];
                 e.v[6] = cv[s][6];
                 e.v[7] = e.v[8] = e.v[9] = 0.0f;
-                Q.list[atomicAdd(Q.count, 1)] = e;
+                Q.list[atomic_add_global(Q.count, 1)] = e;
             }
         }
         // electron pressure: |pe| everywhere, sound-speed cap and Alfven
@@ -830,7 +834,7 @@ __global__ void __launch_bounds__(256) k_validate(ValParams Q) {
         if (threadIdx.x < 5) {
             const int q = threadIdx.x;
             const float m = fmaxf(fmaxf(red[q][0], red[q][1]), fmaxf(red[q][2], red[q][3]));
-            Q.partials[blockIdx.x * 5 + q] = m;
+            Q.partials[pb * 5 + q] = m;
         }
     }
 }
@@ -859,16 +863,17 @@ __device__ inline void validate_reduce_block(const float* __restrict__ partials,
 // Blocks 0 .. kApplyBlocks-1 apply the pass's side-list; with maxima set,
 // one more block forms the pass's CFL maxima (independent work, one launch)
 constexpr int kApplyBlocks = 128;
-__global__ void __launch_bounds__(256) k_validate_apply(ValParams Q, float* maxima, int nblocks) {
-    if (blockIdx.x == kApplyBlocks) {
+__device__ __forceinline__ void validate_apply_block(const ValParams& Q, float* maxima, int nblocks,
+                                            int lb) {
+    if (lb == kApplyBlocks) {
         validate_reduce_block(Q.partials, nblocks, maxima);
         return;
     }
     const int n = *Q.count;
     // the next pass appends to the other counter; clear it here (no pass
     // is using it now) so no separate zeroing launch is needed
-    if (blockIdx.x == 0 && threadIdx.x == 0) *Q.count_next = 0;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += kApplyBlocks * 256) {
+    if (lb == 0 && threadIdx.x == 0) *Q.count_next = 0;
+    for (int e = lb * 256 + threadIdx.x; e < n; e += kApplyBlocks * 256) {
         const ValEntry a = Q.list[e];   // whole entry loaded before the stores
         float* const* f = Q.sp[a.species].f;
 #pragma unroll
@@ -982,11 +987,11 @@ __device__ inline void lap_march_store(LapMarchLds& S, int slot, int row, int l,
     S.a[slot][12][row][l] = v[6] / a;
 }
 
-__global__ void __launch_bounds__(MTX * MTY) k_lap_march(LapParams L, int ntx, int nty, int nkc) {
+__device__ __forceinline__ void lap_march_block(const LapParams& L, int ntx, int nty, int nkc, int lb) {
     __shared__ LapMarchLds S;
     const Box B = L.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
-    int t = xcd_block(blockIdx.x, gridDim.x);
+    int t = lb;
     const int tx = t % ntx;
     t /= ntx;
     const int ty = t % nty;
@@ -1078,10 +1083,10 @@ __global__ void __launch_bounds__(MTX * MTY) k_lap_march(LapParams L, int ntx, i
     }
 }
 
-__global__ void __launch_bounds__(256) k_lap(LapParams L) {
+__device__ __forceinline__ void lap_block(const LapParams& L, int lb) {
     int i, j, k, g;
     const Box& B = L.B;
-    if (!cell3(B, i, j, k, g)) return;
+    if (!cell3(B, lb, i, j, k, g)) return;
     if (L.march) g = 3;
     if (!B.interior(i, j, k)) return;
     lap_body(L, g, i, j, k);
@@ -1153,14 +1158,12 @@ __device__ inline float fct_flux(const WA& W, const PA& P, int base, int m, int 
     return fct_lim(w1 - w0, p0 - pm1, d1p, chi);
 }
 
-__global__ void __launch_bounds__(FNT, 4) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
+__device__ __forceinline__ void fct_block(const FctParams& F, int ntx, int nty, int nkc, int bl) {
     __shared__ FctLds L;
     const Box& B = F.B;
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     // logical block order: field, k-run, tile row, tile column (fastest)
     const int per_f = ntx * nty * nkc;
-    // neighbouring tiles on one XCD, unless tuned otherwise
-    const int bl = F.no_remap ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (bl >= per_f * F.nf) return;
     const int f = bl / per_f;
     const int r = bl % per_f;
@@ -1390,16 +1393,6 @@ __global__ void __launch_bounds__(FNT, 4) k_fct_fused(FctParams F, int ntx, int 
     }
 }
 
-void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
-    FctParams G = F;
-    G.nf = nf;
-    if (G.kc_len <= 0) G.kc_len = FKC;
-    const int ntx = (F.B.nx + FTX - 1) / FTX, nty = (F.B.ny + FTY - 1) / FTY;
-    const int nkc = (F.B.nz + G.kc_len - 1) / G.kc_len;
-    const int nb = ntx * nty * nkc * nf;
-    ProfScope prof_("k_fct_fused", s);
-    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, FTY / 2), 0, s, G, ntx, nty, nkc);
-}
 
 // ---------------------------------------------------------------------------
 // flanks from a precomputed shell table: one thread per (shell cell, field)
@@ -1602,8 +1595,176 @@ void launch_flanks_table(const Box& B, const ShellPt* tab, int npts, const Flank
 
 
 // ---------------------------------------------------------------------------
+// The stage kernels: one-run launches take the logical block from the
+// XCD-aware order; the multi-run forms (k_*_runs) find their run and its
+// logical block in the launch's concatenation of the runs' blocks.
+thread_local Recorder* g_rec = nullptr;
+
+template <bool P2>
+__global__ void __launch_bounds__(256) k_efield(EParams P) {
+    efield_block<P2>(P, lblock());
+}
+template <bool P2>
+__global__ void __launch_bounds__(256) k_efield_runs(const EParams* __restrict__ A, MultiRun M) {
+    int r, lb;
+    if (multi_block(M, r, lb)) efield_block<P2>(A[r], lb);
+}
+
+template <bool P2>
+__global__ void __launch_bounds__(MTX * MTY) k_push_march(PushParams P, int ntx, int nty, int nkc) {
+    push_march_block<P2>(P, ntx, nty, nkc, lblock());
+}
+template <bool P2>
+__global__ void __launch_bounds__(MTX * MTY) k_push_march_runs(const MarchArgs* __restrict__ A,
+                                                               MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb)) return;
+    const MarchArgs& a = A[r];
+    push_march_block<P2>(a.P, a.ntx, a.nty, a.nkc, lb);
+}
+
+template <bool P2>
+__global__ void __launch_bounds__(256) k_push(PushParams P) {
+    push_block<P2>(P, lblock());
+}
+template <bool P2>
+__global__ void __launch_bounds__(256) k_push_runs(const PushParams* __restrict__ A, MultiRun M) {
+    int r, lb;
+    if (multi_block(M, r, lb)) push_block<P2>(A[r], lb);
+}
+
+__global__ void __launch_bounds__(256) k_validate(ValParams Q) {
+    validate_block(Q, lblock(), blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_validate_runs(const ValParams* __restrict__ A,
+                                                       MultiRun M) {
+    int r, lb;
+    if (multi_block(M, r, lb)) validate_block(A[r], lb, lb);
+}
+
+__global__ void __launch_bounds__(256) k_validate_apply(ValParams Q, float* maxima, int nblocks) {
+    validate_apply_block(Q, maxima, nblocks, blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_validate_apply_runs(const ApplyArgs* __restrict__ A,
+                                                             MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb)) return;
+    const ApplyArgs& a = A[r];
+    validate_apply_block(a.Q, a.maxima, a.nblocks, lb);
+}
+
+__global__ void __launch_bounds__(MTX * MTY) k_lap_march(LapParams L, int ntx, int nty, int nkc) {
+    lap_march_block(L, ntx, nty, nkc, lblock());
+}
+__global__ void __launch_bounds__(MTX * MTY) k_lap_march_runs(const LapMarchArgs* __restrict__ A,
+                                                              MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb)) return;
+    const LapMarchArgs& a = A[r];
+    lap_march_block(a.L, a.ntx, a.nty, a.nkc, lb);
+}
+
+__global__ void __launch_bounds__(256) k_lap(LapParams L) { lap_block(L, lblock()); }
+__global__ void __launch_bounds__(256) k_lap_runs(const LapParams* __restrict__ A, MultiRun M) {
+    int r, lb;
+    if (multi_block(M, r, lb)) lap_block(A[r], lb);
+}
+
+// neighbouring tiles on one XCD, unless tuned otherwise
+__global__ void __launch_bounds__(FNT, 4) k_fct_fused(FctParams F, int ntx, int nty, int nkc) {
+    fct_block(F, ntx, nty, nkc, F.no_remap ? (int)blockIdx.x : lblock());
+}
+__global__ void __launch_bounds__(FNT, 4) k_fct_fused_runs(const FctArgs* __restrict__ A,
+                                                           MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb)) return;
+    const FctArgs& a = A[r];
+    fct_block(a.F, a.ntx, a.nty, a.nkc, lb);
+}
+
+size_t kernel_args_size(int id) {
+    switch (id) {
+    case kKEfield:
+    case kKEfieldP2: return sizeof(EParams);
+    case kKPushMarch:
+    case kKPushMarchP2: return sizeof(MarchArgs);
+    case kKPush:
+    case kKPushP2: return sizeof(PushParams);
+    case kKValidate: return sizeof(ValParams);
+    case kKValidateApply: return sizeof(ApplyArgs);
+    case kKLapMarch: return sizeof(LapMarchArgs);
+    case kKLap: return sizeof(LapParams);
+    case kKFct: return sizeof(FctArgs);
+    case kKOuter: return sizeof(OuterArgs);
+    case kKInner: return sizeof(InnerArgs);
+    case kKNextDt: return sizeof(NextDtArgs);
+    default: return 0;
+    }
+}
+
+void launch_multi(int id, const void* args, const MultiRun& M, dim3 block, unsigned gy,
+                  hipStream_t s) {
+    const dim3 g((unsigned)((M.total + 7) & ~7), gy);
+    switch (id) {
+    case kKEfield:
+        hipLaunchKernelGGL(k_efield_runs<false>, g, block, 0, s, (const EParams*)args, M);
+        break;
+    case kKEfieldP2:
+        hipLaunchKernelGGL(k_efield_runs<true>, g, block, 0, s, (const EParams*)args, M);
+        break;
+    case kKPushMarch:
+        hipLaunchKernelGGL(k_push_march_runs<false>, g, block, 0, s, (const MarchArgs*)args, M);
+        break;
+    case kKPushMarchP2:
+        hipLaunchKernelGGL(k_push_march_runs<true>, g, block, 0, s, (const MarchArgs*)args, M);
+        break;
+    case kKPush:
+        hipLaunchKernelGGL(k_push_runs<false>, g, block, 0, s, (const PushParams*)args, M);
+        break;
+    case kKPushP2:
+        hipLaunchKernelGGL(k_push_runs<true>, g, block, 0, s, (const PushParams*)args, M);
+        break;
+    case kKValidate:
+        hipLaunchKernelGGL(k_validate_runs, g, block, 0, s, (const ValParams*)args, M);
+        break;
+    case kKValidateApply:
+        hipLaunchKernelGGL(k_validate_apply_runs, g, block, 0, s, (const ApplyArgs*)args, M);
+        break;
+    case kKLapMarch:
+        hipLaunchKernelGGL(k_lap_march_runs, g, block, 0, s, (const LapMarchArgs*)args, M);
+        break;
+    case kKLap:
+        hipLaunchKernelGGL(k_lap_runs, g, block, 0, s, (const LapParams*)args, M);
+        break;
+    case kKFct:
+        hipLaunchKernelGGL(k_fct_fused_runs, g, block, 0, s, (const FctArgs*)args, M);
+        break;
+    default: launch_multi_bounds(id, args, M, block, gy, s); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
+void launch_fct_fused(const FctParams& F, int nf, hipStream_t s) {
+    FctParams G = F;
+    G.nf = nf;
+    if (G.kc_len <= 0) G.kc_len = FKC;
+    const int ntx = (F.B.nx + FTX - 1) / FTX, nty = (F.B.ny + FTY - 1) / FTY;
+    const int nkc = (F.B.nz + G.kc_len - 1) / G.kc_len;
+    const int nb = ntx * nty * nkc * nf;
+    if (g_rec) {
+        g_rec->add(kKFct, dim3((nb + 7) & ~7), dim3(64, FTY / 2), FctArgs{G, ntx, nty, nkc});
+        return;
+    }
+    ProfScope prof_("k_fct_fused", s);
+    hipLaunchKernelGGL(k_fct_fused, dim3((nb + 7) & ~7), dim3(64, FTY / 2), 0, s, G, ntx, nty, nkc);
+}
+
 void launch_efield(const EParams& P, hipStream_t s) {
+    if (g_rec) {
+        g_rec->add(P.pow2 ? kKEfieldP2 : kKEfield, grid3(P.B, 1), block3(), P);
+        return;
+    }
     if (P.pow2)
         hipLaunchKernelGGL(k_efield<true>, grid3(P.B, 1), block3(), 0, s, P);
     else
@@ -1615,6 +1776,10 @@ static void launch_march(const PushParams& P, hipStream_t s) {
     const int ntx = (P.B.nx - 2 + MOX - 1) / MOX, nty = (P.B.ny - 2 + MTY - 1) / MTY;
     const int nkc = (P.B.nz - 2 + kcl - 1) / kcl, nb = P.nspecies * ntx * nty * nkc;
     const dim3 g((nb + 7) & ~7), b(MTX, MTY);
+    if (g_rec) {
+        g_rec->add(P.pow2 ? kKPushMarchP2 : kKPushMarch, g, b, MarchArgs{P, ntx, nty, nkc});
+        return;
+    }
     if (P.pow2)
         hipLaunchKernelGGL(k_push_march<true>, g, b, 0, s, P, ntx, nty, nkc);
     else
@@ -1627,6 +1792,10 @@ void launch_push(const PushParams& P, hipStream_t s) {
     ProfScope prof_("k_push", s);
     if (P.march) launch_march(P, s);
     const int groups = P.march ? 1 : 4;
+    if (g_rec) {
+        g_rec->add(P.pow2 ? kKPushP2 : kKPush, grid3(P.B, groups), block3(), P);
+        return;
+    }
     if (P.pow2)
         hipLaunchKernelGGL(k_push<true>, grid3(P.B, groups), block3(), 0, s, P);
     else
@@ -1640,11 +1809,17 @@ int validate_blocks(const Box& B) {
 
 void launch_validate(const ValParams& Q, float* maxima, hipStream_t s) {
     const int nb = validate_blocks(Q.B);
+    const bool red = Q.partials && maxima;
+    if (g_rec) {
+        g_rec->add(kKValidate, grid3(Q.B, 1), block3(), Q);
+        g_rec->add(kKValidateApply, dim3(kApplyBlocks + (red ? 1 : 0)), dim3(256),
+                   ApplyArgs{Q, maxima, nb});
+        return;
+    }
     {
         ProfScope prof_("k_validate", s);
         hipLaunchKernelGGL(k_validate, grid3(Q.B, 1), block3(), 0, s, Q);
     }
-    const bool red = Q.partials && maxima;
     hipLaunchKernelGGL(k_validate_apply, dim3(kApplyBlocks + (red ? 1 : 0)), dim3(256), 0, s, Q,
                        maxima, nb);
 }
@@ -1654,8 +1829,16 @@ void launch_lap(const LapParams& L, hipStream_t s) {
     if (L.march) {
         const int ntx = (L.B.nx - 2 + MOX - 1) / MOX, nty = (L.B.ny - 2 + MTY - 1) / MTY;
         const int nkc = (L.B.nz - 2 + MKC - 1) / MKC, nb = 3 * ntx * nty * nkc;
-        hipLaunchKernelGGL(k_lap_march, dim3((nb + 7) & ~7), dim3(MTX, MTY), 0, s, L, ntx, nty,
+        if (g_rec)
+            g_rec->add(kKLapMarch, dim3((nb + 7) & ~7), dim3(MTX, MTY),
+                       LapMarchArgs{L, ntx, nty, nkc});
+        else
+            hipLaunchKernelGGL(k_lap_march, dim3((nb + 7) & ~7), dim3(MTX, MTY), 0, s, L, ntx, nty,
                            nkc);
+    }
+    if (g_rec) {
+        g_rec->add(kKLap, grid3(L.B, L.march ? 1 : 4), block3(), L);
+        return;
     }
     hipLaunchKernelGGL(k_lap, grid3(L.B, L.march ? 1 : 4), block3(), 0, s, L);
 }

@@ -696,6 +696,14 @@ __global__ void k_set_dt(float* dt, double* t, float dt0, double t0) {
     *t = t0;
 }
 
+// the device time step and run time of captured or batched steps
+bool graph_scalars(Run* R) {
+    if (R->dt_dev) return true;
+    return ok_hip(R, hipMalloc(&R->dt_dev, 4 * sizeof(float)), "graph scalars") &&
+           ok_hip(R, hipMalloc(&R->t_dev, sizeof(double)), "graph scalars") &&
+           ok_hip(R, hipHostMalloc(&R->host_buf, 4 * sizeof(double)), "graph scalars");
+}
+
 // Captures the two full steps (one per rotation parity) of a single-grid
 // run; host bookkeeping touched by the capture is restored
 bool build_graphs(Run* R) {
@@ -706,12 +714,7 @@ bool build_graphs(Run* R) {
         if (!ok_hip(R, hipStreamCreate(&R->s), "graph stream")) return false;
         R->own_stream = true;
     }
-    if (!R->dt_dev) {
-        if (!ok_hip(R, hipMalloc(&R->dt_dev, 4 * sizeof(float)), "graph scalars") ||
-            !ok_hip(R, hipMalloc(&R->t_dev, sizeof(double)), "graph scalars") ||
-            !ok_hip(R, hipHostMalloc(&R->host_buf, 4 * sizeof(double)), "graph scalars"))
-            return false;
-    }
+    if (!graph_scalars(R)) return false;
     const std::vector<double> t_old = R->t_old, t_new = R->t_new;
     const long long cu = R->cell_updates, bs = R->box_steps;
     const int vpass = R->vpass;
@@ -737,6 +740,234 @@ bool build_graphs(Run* R) {
     R->box_steps = bs;
     if (R->vpass != vpass) ok = false;   // a full step keeps the counter parity
     return ok && R->err == 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Many single-grid runs stepped in lockstep, one launch per stage for all
+// of them.  At creation every run's box-step is recorded, for each rotation
+// parity, as the launches (kernel, grid, block, arguments) its own
+// captured step would issue; launch i of every run then becomes one
+// multi-run launch whose grid concatenates the runs' grids (runs whose
+// kernel differs at a position -- power-of-two spacings or not -- form one
+// launch per kernel).  Each run's blocks read that run's own arguments and
+// touch only its own arrays and scratch, so a run's results are those of
+// its own captured step, bit for bit.
+
+__global__ void k_next_dt_runs(const NextDtArgs* __restrict__ A, MultiRun M) {
+    int r, lb;
+    if (!multi_block(M, r, lb) || threadIdx.x != 0) return;
+    const NextDtArgs& a = A[r];
+    float* dt = a.dt;
+    double* t = a.t;
+    const float* m = a.m;
+    *t = *t + (double)dt[0];
+    dt[2] = dt[0];
+    float f = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3] + m[4] * m[4]);
+    f = (0.0f < f) ? f : 0.0f;
+    const float st = a.stepsz * a.xspac / f;
+    const float nd = (a.t_step_max < st) ? a.t_step_max : st;
+    dt[1] = nd;
+    dt[0] = nd;
+    if (!(nd >= kTStepMin && nd <= 3.0e38f)) dt[3] = 1.0f;
+}
+
+struct MultiNode {
+    int id = 0;
+    dim3 block;
+    unsigned gy = 1;
+    int nruns = 0, total = 0;
+    int* d_start = nullptr;        // nruns + 1
+    unsigned char* d_args[2] = {nullptr, nullptr};   // per batch phase
+};
+
+struct FusedBatch {
+    std::vector<Run*> runs;
+    std::vector<int> rot0;         // each run's rotation parity at creation
+    std::vector<MultiNode> nodes;
+    hipStream_t s = nullptr;
+    bool own_stream = false;
+    int phase = 0;                 // steps taken since creation, mod 2
+    bool issued = false;           // steps issued since the last finish
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int err = 0;
+};
+
+bool batch_hip(FusedBatch* F, hipError_t e, const char* where) {
+    if (e == hipSuccess) return true;
+    if (!F->err) {
+        F->err = (int)e;
+        fprintf(stderr, "g3m3 batch: %s: %s\n", where, hipGetErrorString(e));
+    }
+    return false;
+}
+
+// one box-step of run R from rotation parity p, recorded (nothing runs)
+bool record_step(Run* R, int p, std::vector<LaunchRec>& out) {
+    const std::vector<double> t_old = R->t_old, t_new = R->t_new;
+    const long long cu = R->cell_updates, bs = R->box_steps;
+    const int vpass = R->vpass;
+    const size_t nlog = R->step_log.size();
+    const char rot = R->rot[0];
+    const int march = R->push_march;
+    Recorder rec;
+    R->rot[0] = (char)p;
+    R->push_march = 1;             // every run on the same kernel family (same bits)
+    R->capturing = true;
+    g_rec = &rec;
+    box_step(R, 1);
+    rec.add(kKNextDt, dim3(1), dim3(64),
+            NextDtArgs{R->maxima, R->d.stepsz, R->d.xspac[0], R->d.t_step_max, R->dt_dev,
+                       R->t_dev});
+    g_rec = nullptr;
+    R->capturing = false;
+    R->push_march = march;
+    R->rot[0] = rot;
+    R->t_old = t_old;
+    R->t_new = t_new;
+    R->cell_updates = cu;
+    R->box_steps = bs;
+    R->step_log.resize(nlog);
+    const bool same_vpass = R->vpass == vpass;
+    R->vpass = vpass;
+    out = std::move(rec.recs);
+    return same_vpass && R->err == 0;
+}
+
+void launch_node(const MultiNode& N, int phase, hipStream_t s) {
+    const MultiRun M{N.d_start, N.nruns, N.total};
+    if (N.id == kKNextDt)
+        hipLaunchKernelGGL(k_next_dt_runs, dim3((unsigned)((N.total + 7) & ~7)), N.block, 0, s,
+                           (const NextDtArgs*)N.d_args[phase], M);
+    else
+        launch_multi(N.id, N.d_args[phase], M, N.block, N.gy, s);
+}
+
+void free_batch(FusedBatch* F) {
+    for (MultiNode& N : F->nodes) {
+        if (N.d_start) (void)hipFree(N.d_start);
+        for (unsigned char* a : N.d_args)
+            if (a) (void)hipFree(a);
+    }
+    for (int q = 0; q < 2; ++q) {
+        if (F->gexec[q]) (void)hipGraphExecDestroy(F->gexec[q]);
+        if (F->graph[q]) (void)hipGraphDestroy(F->graph[q]);
+    }
+    if (F->own_stream && F->s) (void)hipStreamDestroy(F->s);
+    delete F;
+}
+
+FusedBatch* build_batch(Run** runs, int n, hipStream_t stream) {
+    FusedBatch* F = new FusedBatch();
+    F->s = stream;
+    if (!F->s) {
+        if (!batch_hip(F, hipStreamCreate(&F->s), "stream")) {
+            free_batch(F);
+            return nullptr;
+        }
+        F->own_stream = true;
+    }
+    // recs[p][r]: run r's launches from its parity at batch phase p
+    std::vector<std::vector<std::vector<LaunchRec>>> recs(2);
+    for (int r = 0; r < n; ++r) {
+        Run* R = runs[r];
+        if (!R || R->err || R->d.n_grids != 1 || R->d.tilting || !graph_scalars(R)) {
+            fprintf(stderr, "g3m3_batch_create: run %d is not a single-grid run without "
+                            "tilting, or has failed\n", r);
+            free_batch(F);
+            return nullptr;
+        }
+        F->runs.push_back(R);
+        F->rot0.push_back(R->rot[0]);
+        for (int ph = 0; ph < 2; ++ph) {
+            std::vector<LaunchRec> v;
+            if (!record_step(R, (R->rot[0] + ph) & 1, v)) {
+                fprintf(stderr, "g3m3_batch_create: recording run %d failed\n", r);
+                free_batch(F);
+                return nullptr;
+            }
+            recs[ph].push_back(std::move(v));
+        }
+    }
+    // the same sequence of stage launches in every run and phase
+    const size_t nl = recs[0][0].size();
+    for (int ph = 0; ph < 2; ++ph)
+        for (int r = 0; r < n; ++r) {
+            const std::vector<LaunchRec>& v = recs[ph][r];
+            bool same = v.size() == nl;
+            for (size_t i = 0; same && i < nl; ++i) {
+                const LaunchRec &a = v[i], &b = recs[0][0][i];
+                same = a.block.x == b.block.x && a.block.y == b.block.y &&
+                       a.grid.y == b.grid.y && kernel_args_size(a.id) == kernel_args_size(b.id);
+            }
+            if (!same) {
+                fprintf(stderr, "g3m3_batch_create: run %d issues another launch sequence\n", r);
+                free_batch(F);
+                return nullptr;
+            }
+        }
+    for (size_t i = 0; i < nl; ++i) {
+        // one node per kernel met at this position, runs in batch order
+        std::vector<int> ids;
+        for (int r = 0; r < n; ++r)
+            if (std::find(ids.begin(), ids.end(), recs[0][r][i].id) == ids.end())
+                ids.push_back(recs[0][r][i].id);
+        for (int id : ids) {
+            MultiNode N;
+            N.id = id;
+            N.block = recs[0][0][i].block;
+            N.gy = recs[0][0][i].grid.y;
+            std::vector<int> start{0};
+            std::vector<unsigned char> blob[2];
+            const size_t sz = kernel_args_size(id);
+            for (int r = 0; r < n; ++r) {
+                if (recs[0][r][i].id != id) continue;
+                for (int ph = 0; ph < 2; ++ph) {
+                    const LaunchRec& L = recs[ph][r][i];
+                    if (L.id != id || L.args.size() != sz) {
+                        fprintf(stderr, "g3m3_batch_create: run %d changes kernel between "
+                                        "parities\n", r);
+                        free_batch(F);
+                        return nullptr;
+                    }
+                    blob[ph].insert(blob[ph].end(), L.args.begin(), L.args.end());
+                }
+                start.push_back(start.back() + (int)recs[0][r][i].grid.x);
+                ++N.nruns;
+            }
+            N.total = start.back();
+            bool ok = batch_hip(F, hipMalloc(&N.d_start, sizeof(int) * start.size()), "alloc") &&
+                      batch_hip(F, hipMemcpy(N.d_start, start.data(), sizeof(int) * start.size(),
+                                             hipMemcpyHostToDevice), "copy");
+            for (int ph = 0; ok && ph < 2; ++ph)
+                ok = batch_hip(F, hipMalloc(&N.d_args[ph], blob[ph].size()), "alloc") &&
+                     batch_hip(F, hipMemcpy(N.d_args[ph], blob[ph].data(), blob[ph].size(),
+                                            hipMemcpyHostToDevice), "copy");
+            F->nodes.push_back(N);
+            if (!ok) {
+                free_batch(F);
+                return nullptr;
+            }
+        }
+    }
+    // one captured graph per phase: every node of one lockstep box-step
+    for (int ph = 0; ph < 2; ++ph) {
+        bool ok = batch_hip(F, hipStreamBeginCapture(F->s, hipStreamCaptureModeThreadLocal),
+                            "begin capture");
+        if (ok) {
+            for (const MultiNode& N : F->nodes) launch_node(N, ph, F->s);
+            ok = batch_hip(F, hipStreamEndCapture(F->s, &F->graph[ph]), "end capture") &&
+                 batch_hip(F, hipGraphInstantiate(&F->gexec[ph], F->graph[ph], nullptr, nullptr,
+                                                  0),
+                           "instantiate");
+        }
+        if (!ok) {
+            free_batch(F);
+            return nullptr;
+        }
+    }
+    return F;
 }
 
 }  // namespace
@@ -1142,6 +1373,101 @@ void g3m3_run_info(void* run, double* t, double* t_step, double* t_stepnew, doub
     }
     if (cell_updates) *cell_updates = R->cell_updates;
     if (box_steps) *box_steps = R->box_steps;
+}
+
+
+void* g3m3_batch_create(void* const* runs, int nruns, void* stream) {
+    if (!runs || nruns < 1) return nullptr;
+    std::vector<Run*> rs(nruns);
+    for (int r = 0; r < nruns; ++r) {
+        rs[r] = (Run*)runs[r];
+        // each run's own pending work is done before the batch reads it
+        if (rs[r] && rs[r]->s) (void)hipStreamSynchronize(rs[r]->s);
+    }
+    return build_batch(rs.data(), nruns, (hipStream_t)stream);
+}
+
+int g3m3_batch_steps(void* batch, int nsteps) {
+    FusedBatch* F = (FusedBatch*)batch;
+    if (!F) return -1;
+    if (F->err) return F->err;
+    for (size_t r = 0; r < F->runs.size(); ++r) {
+        Run* R = F->runs[r];
+        if (R->err) return F->err = R->err;
+        // the recorded arguments follow each run's buffer rotation from the
+        // batch's creation: a run stepped on its own since then no longer
+        // matches them
+        if (R->rot[0] != (char)(F->rot0[r] ^ F->phase)) {
+            fprintf(stderr, "g3m3 batch: run %d was stepped outside the batch\n", (int)r);
+            return F->err = -1;
+        }
+        // the device already holds the step and time of steps issued since
+        // the last finish; otherwise they come from the host
+        if (F->issued) continue;
+        const float dt0 = (float)R->t_stepnew[0];
+        if (!(std::isfinite(dt0) && dt0 >= kTStepMin)) {
+            fprintf(stderr, "g3m3 batch: time step %g below t_step_min or not finite\n",
+                    (double)dt0);
+            return F->err = kErrStepCollapsed;
+        }
+        if (R->s && R->s != F->s) (void)hipStreamSynchronize(R->s);
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, F->s, R->dt_dev, R->t_dev, dt0, R->t);
+    }
+    F->issued = true;
+    for (int q = 0; q < nsteps; ++q) {
+        if (!batch_hip(F, hipGraphLaunch(F->gexec[F->phase], F->s), "graph launch")) break;
+        F->phase ^= 1;
+        for (Run* R : F->runs) {
+            R->rot[0] ^= 1;
+            R->cell_updates += R->n;
+            R->box_steps += 1;
+        }
+    }
+    batch_hip(F, hipGetLastError(), "batch steps");
+    return F->err;
+}
+
+int g3m3_batch_finish(void* batch) {
+    FusedBatch* F = (FusedBatch*)batch;
+    if (!F) return -1;
+    if (F->err) return F->err;
+    for (Run* R : F->runs) {
+        double* h = (double*)R->host_buf;
+        if (!batch_hip(F, hipMemcpyAsync(h, R->t_dev, sizeof(double), hipMemcpyDeviceToHost, F->s),
+                       "finish") ||
+            !batch_hip(F, hipMemcpyAsync(h + 1, R->dt_dev, 4 * sizeof(float),
+                                         hipMemcpyDeviceToHost, F->s),
+                       "finish"))
+            return F->err;
+    }
+    if (!batch_hip(F, hipStreamSynchronize(F->s), "finish")) return F->err;
+    F->issued = false;
+    for (Run* R : F->runs) {
+        const double* h = (const double*)R->host_buf;
+        const float* f = (const float*)(h + 1);
+        R->t = h[0];
+        R->t_stepnew[0] = (double)f[1];
+        R->t_step[0] = (double)f[2];
+        if (f[3] != 0.0f && !R->err) {
+            fprintf(stderr, "g3m3 batch: a step formed a time step below t_step_min or not "
+                            "finite\n");
+            R->err = kErrStepCollapsed;
+        }
+        if (R->err && !F->err) F->err = R->err;
+    }
+    return F->err;
+}
+
+int g3m3_batch_launches(void* batch) {
+    FusedBatch* F = (FusedBatch*)batch;
+    return F ? (int)F->nodes.size() : 0;
+}
+
+void g3m3_batch_destroy(void* batch) {
+    FusedBatch* F = (FusedBatch*)batch;
+    if (!F) return;
+    (void)hipStreamSynchronize(F->s);
+    free_batch(F);
 }
 
 }  // extern "C"

@@ -650,6 +650,64 @@ class Batch:
         for r in self.runs:
             r.graph_finish()
 
+    def fused_steps(self, nsteps: int, groups: int = 4):
+        """`nsteps` full steps of every run (single-grid runs) in lockstep
+        groups: the runs are dealt into `groups` groups of about equal cells
+        (largest first, in snake order), each group steps with one launch
+        per stage for all its runs -- a grid concatenating the runs' own
+        grids (g3m3_batch_*) -- replayed from one captured graph per step on
+        the group's own stream, so one group's small stages overlap another's
+        large ones; returns when all are done.  Each run's results equal its
+        own graph_steps' bit for bit."""
+        L = _lib.lib()
+        if getattr(self, "_fused", None) is None:
+            L.g3m3_batch_create.restype = _ct.c_void_p
+            for nm in ("g3m3_batch_steps", "g3m3_batch_finish", "g3m3_batch_launches"):
+                getattr(L, nm).restype = _ct.c_int
+            torch = _torch()
+            torch.cuda.synchronize()
+            ng = max(1, min(int(groups), len(self.runs)))
+            order = sorted(range(len(self.runs)), key=lambda i: -self.runs[i].nx * self.runs[i].ny
+                           * self.runs[i].nz)
+            members = [[] for _ in range(ng)]
+            for rank, i in enumerate(order):
+                g = rank % (2 * ng)
+                members[g if g < ng else 2 * ng - 1 - g].append(i)
+            self._fused, self._fused_streams = [], []
+            for m in members:
+                if not m:
+                    continue
+                st = torch.cuda.Stream()
+                hs = (_ct.c_void_p * len(m))(*[self.runs[i].h for i in sorted(m)])
+                h = L.g3m3_batch_create(hs, len(m), _ct.c_void_p(st.cuda_stream))
+                if not h:
+                    raise _lib.G3M3Error("g3m3_batch_create failed")
+                self._fused.append(h)
+                self._fused_streams.append(st)
+        for h in self._fused:
+            if L.g3m3_batch_steps(_ct.c_void_p(h), int(nsteps)):
+                raise _lib.G3M3Error("g3m3_batch_steps failed")
+        for h in self._fused:
+            if L.g3m3_batch_finish(_ct.c_void_p(h)):
+                raise _lib.G3M3Error("g3m3_batch_finish failed")
+        for r in self.runs:
+            r._pull()
+
+    @property
+    def fused_launches(self) -> int:
+        """Multi-run launches per lockstep step, summed over the groups (0
+        before fused_steps)."""
+        hs = getattr(self, "_fused", None) or []
+        return sum(int(_lib.lib().g3m3_batch_launches(_ct.c_void_p(h))) for h in hs)
+
+    def __del__(self):
+        hs = getattr(self, "_fused", None)
+        L = getattr(_lib, "_lib", None)     # None once the interpreter tears the module down
+        if hs and L is not None:
+            for h in hs:
+                L.g3m3_batch_destroy(_ct.c_void_p(h))
+            self._fused = None
+
     @property
     def cell_updates(self) -> int:
         return sum(r.cell_updates for r in self.runs)

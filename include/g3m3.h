@@ -711,6 +711,21 @@ int g3m3_run_finish(void* run);
  * stream gets one of its own. */
 int g3m3_run_graph_steps(void* run, int nsteps);
 int g3m3_run_graph_finish(void* run);
+/* Many single-grid runs (g3m3_run_create handles, no tilting) stepped in
+ * lockstep: one launch per stage for all of them, the concatenation of the
+ * runs' own grids, each run's blocks reading that run's own arguments, and
+ * one captured graph per step.  A run's results equal its own
+ * g3m3_run_graph_steps / g3m3_run_full_step results bit for bit.  No
+ * counterpart in the reference (one run per process there).  stream: the
+ * HIP stream the batch runs on (NULL: its own).  g3m3_batch_steps issues
+ * nsteps full steps of every run and returns; g3m3_batch_finish waits and
+ * brings each run's time, time step and error back to its handle (call it
+ * before using a run on its own again).  The runs must outlive the batch. */
+void* g3m3_batch_create(void* const* runs, int nruns, void* stream);   /* NULL on failure */
+int g3m3_batch_steps(void* batch, int nsteps);
+int g3m3_batch_finish(void* batch);
+int g3m3_batch_launches(void* batch);    /* multi-run launches per lockstep step */
+void g3m3_batch_destroy(void* batch);
 /* The FCT launch shape the run's autotuner chose at g3m3_run_create: the
  * k-planes per block (8, 16 or 32) and the block order (0 XCD-contiguous,
  * 1 plain), with the measured microseconds of the six candidates (depth

@@ -82,3 +82,44 @@ def test_batch_graph_equals_runs_alone():
         for _ in range(4):
             ra.full_step()
         _same_run(ra, rb, f"{wl.nx}x{wl.ny}x{wl.nz}")
+
+
+def test_fused_batch_equals_runs_alone():
+    """Runs stepped in lockstep by one multi-run launch per stage (their
+    grids concatenated, a captured graph per step) reach exactly the arrays,
+    time and time steps each run reaches alone -- runs of several sizes,
+    one of them a step ahead (the other buffer parity) when the batch is
+    made, and the steps issued in two calls."""
+    from g3m3_amd.driver import Batch, FastRun
+    specs = [dict(L=8, n_sw=3.0, v_sw=350.0, imf=(1.0, -2.0, -4.0)),
+             dict(L=9, n_sw=12.0, v_sw=700.0, imf=(-3.0, 0.5, 2.0)),
+             dict(L=11, n_sw=6.0, v_sw=500.0, imf=(0.0, 4.0, -1.0)),
+             dict(L=14, n_sw=1.5, v_sw=420.0, imf=(2.0, 2.0, 2.0))]
+    mk = lambda: [make_workload("single61", seed=60 + i, **sp) for i, sp in enumerate(specs)]
+    batch = Batch(mk(), streams=2)
+    batch.init_dt()
+    batch.runs[1].full_step()
+    print("fused batch steps, 4 runs", flush=True)
+    batch.fused_steps(3)
+    batch.fused_steps(2)
+    assert batch.fused_launches > 0
+    for i, (wl, rb) in enumerate(zip(mk(), batch.runs)):
+        ra = FastRun(wl)
+        ra.init_dt()
+        for _ in range(5 + (i == 1)):
+            ra.full_step()
+        _same_run(ra, rb, f"{wl.nx}x{wl.ny}x{wl.nz}")
+
+
+def test_fused_batch_refuses_a_run_stepped_alone():
+    """A run stepped on its own after the batch was made no longer matches
+    the batch's recorded buffer rotation: the next lockstep step is refused."""
+    from g3m3_amd import _lib
+    from g3m3_amd.driver import Batch
+    wls = [make_workload("single61", seed=70 + i, L=8) for i in range(2)]
+    batch = Batch(wls, streams=1)
+    batch.init_dt()
+    batch.fused_steps(1)
+    batch.runs[0].full_step()
+    with pytest.raises(_lib.G3M3Error):
+        batch.fused_steps(1)
