@@ -203,7 +203,6 @@ struct ApplyArgs {
 struct OuterArgs {
     Box B;
     OuterPtrs O;
-    int pass;
 };
 // which = -1: the three lists of Ls (blockIdx.y = the list); 0..2 one list
 struct InnerArgs {

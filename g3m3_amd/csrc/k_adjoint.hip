@@ -274,7 +274,7 @@ __global__ void k_adj_calcur_int(Box B, const float* __restrict__ cxb,
 }
 
 // bndry_outer (code/sub_bndry_outer.f90:50-728) on bx, by, bz, transposed.
-// The forward kernel k_outer_pass runs eight ordered passes; within a pass
+// The forward bndry_outer runs eight ordered passes; within a pass
 // the cells written and the cells read are disjoint, so the adjoint of a
 // pass moves each written cell's adjoint to the cells it was formed from
 // and zeroes it, and the passes run in reverse.  Pass 0 (the inflow face,

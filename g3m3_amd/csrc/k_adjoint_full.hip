@@ -199,7 +199,7 @@ __global__ void k_adj_push_elec(PeAdj A) {
     A.wb[m] += wB;
 }
 
-// bndry_outer on any field kind, transposed (k_outer_pass's eight passes in
+// bndry_outer on any field kind, transposed (bndry_outer's eight passes in
 // reverse; within a pass written and read cells are disjoint).  Pass 0 (the
 // inflow face) sends its adjoint, times the field's coefficient, to its
 // control (sums in double).  blockIdx.y = field.

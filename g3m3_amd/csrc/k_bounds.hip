@@ -11,95 +11,142 @@
 namespace g3m3 {
 
 // ---------------------------------------------------------------------------
-// bndry_outer as eight ordered passes over all fields (blockIdx.y = field).
-// Pass p covers a 2-D index range (a, b); see outer_pass_extent().
-__device__ __forceinline__ void outer_pass_block(const Box& B, const OuterPtrs& O, int pass, int lb) {
-    const int f = blockIdx.y;
-    const int t = lb * 256 + threadIdx.x;
+// bndry_outer (code/sub_bndry_outer.f90:5-731): eight ordered passes over
+// the faces, edges and corner lines of each field.
+// The eight passes in five phases: 0 and 1 (the i = 1 and i = nx faces)
+// touch disjoint cells and read nothing the other writes, and so do 3 and 4
+// (z top / bottom) and 6 and 7 (back-wall lines j = 2..ny-1 / x-edge lines
+// j = 1, ny); every other pass reads what the one before it wrote.  Within
+// a phase no cell is both read and written, so its cells go in any order.
+constexpr int kOuterPhases = 5;
+__host__ __device__ inline int outer_phase_cells(const Box& B, int p) {
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    switch (p) {
+    case 0: return ny * nz;
+    case 1: return (nx - 1) * nz;
+    case 2: return (nx - 1) * ny;
+    case 3: return nz - 2;
+    default: return (ny - 2 > nx - 1) ? ny - 2 : nx - 1;
+    }
+}
+
+// Cells t0, t0 + stride, ... of phase p of field f.  The cells' loads are
+// issued kOuterUnroll at a time before their stores, so a thread with
+// several cells waits about one memory latency for them.
+constexpr int kOuterUnroll = 8;
+__device__ __forceinline__ void outer_phase(const Box& B, const OuterPtrs& O, int f, int p,
+                                            int t0, int stride) {
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     float* q = O.f[f];
     const int kind = O.kind[f];
     auto at = [&](int i, int j, int k) -> float& { return q[B.IX(i, j, k)]; };
-    switch (pass) {
-    case 0: {  // inflow face i=1, all j,k (:50-103)
-        if (t >= ny * nz) return;
-        const int j = t % ny, k = t / ny;
-        if (kind >= kBX) at(0, j, k) = O.val[f] - O.b0[kind - kBX][B.IX(0, j, k)];
-        else at(0, j, k) = O.val[f];
+    constexpr int U = kOuterUnroll;
+    auto sweep = [&](int n, auto load, auto store) {
+        for (int b = t0; b < n; b += stride * U) {
+            decltype(load(0)) v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = b + u * stride;
+                if (t < n) v[u] = load(t);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = b + u * stride;
+                if (t < n) store(t, v[u]);
+            }
+        }
+    };
+    switch (p) {
+    case 0:   // passes 0 (:50-103) and 1 (:107-154), all j, k
+        sweep(ny * nz,
+              [&](int t) {
+                  const int j = t % ny, k = t / ny;
+                  const float v0 =
+                      kind >= kBX ? O.val[f] - O.b0[kind - kBX][B.IX(0, j, k)] : O.val[f];
+                  return float2{v0, at(nx - 2, j, k)};
+              },
+              [&](int t, float2 v) {
+                  const int j = t % ny, k = t / ny;
+                  at(0, j, k) = v.x;
+                  at(nx - 1, j, k) = (kind == kPX) ? fabsf(v.y) : v.y;
+              });
         break;
-    }
-    case 1: {  // back wall i=nx (:107-154)
-        if (t >= ny * nz) return;
-        const int j = t % ny, k = t / ny;
-        const float v = at(nx - 2, j, k);
-        at(nx - 1, j, k) = (kind == kPX) ? fabsf(v) : v;
+    case 1:   // pass 2: y faces, i=2..nx, all k (:158-252)
+        sweep((nx - 1) * nz,
+              [&](int t) {
+                  const int i = 1 + t % (nx - 1), k = t / (nx - 1);
+                  return float2{at(i, 1, k), at(i, ny - 2, k)};
+              },
+              [&](int t, float2 v) {
+                  const int i = 1 + t % (nx - 1), k = t / (nx - 1);
+                  at(i, 0, k) = (kind == kPY) ? -fabsf(v.x) : v.x;
+                  at(i, ny - 1, k) = (kind == kPY) ? fabsf(v.y) : v.y;
+              });
         break;
-    }
-    case 2: {  // y faces, i=2..nx, all k (:158-252)
-        if (t >= (nx - 1) * nz) return;
-        const int i = 1 + t % (nx - 1), k = t / (nx - 1);
-        const float lo = at(i, 1, k), hi = at(i, ny - 2, k);
-        at(i, 0, k) = (kind == kPY) ? -fabsf(lo) : lo;
-        at(i, ny - 1, k) = (kind == kPY) ? fabsf(hi) : hi;
+    case 2:   // passes 3 (z top, :256-307) and 4 (z bottom, :311-359), i=2..nx, all j
+        sweep((nx - 1) * ny,
+              [&](int t) {
+                  const int i = 1 + t % (nx - 1), j = t / (nx - 1);
+                  return float2{at(i, j, nz - 2), at(i, j, 1)};
+              },
+              [&](int t, float2 v) {
+                  const int i = 1 + t % (nx - 1), j = t / (nx - 1);
+                  at(i, j, nz - 1) = (kind == kPZ) ? fabsf(v.x) : v.x;
+                  at(i, j, 0) = (kind == kPZ) ? -fabsf(v.y) : v.y;
+              });
         break;
-    }
-    case 3: {  // z top, i=2..nx, all j (:256-307)
-        if (t >= (nx - 1) * ny) return;
-        const int i = 1 + t % (nx - 1), j = t / (nx - 1);
-        const float v = at(i, j, nz - 2);
-        at(i, j, nz - 1) = (kind == kPZ) ? fabsf(v) : v;
+    case 3:   // pass 5: corner lines of the back wall, k=2..nz-1 (:363-466)
+        for (int t = t0; t < nz - 2; t += stride) {
+            const int k = 1 + t;
+            if (kind == kBX) {
+                at(nx - 1, 0, k) = at(nx - 2, 0, k);
+                at(nx - 1, ny - 1, k) = at(nx - 2, ny - 1, k);
+            } else {
+                at(nx - 1, 0, k) = (at(nx - 2, 0, k) + at(nx - 1, 1, k)) / 2.0f;
+                at(nx - 1, ny - 1, k) = (at(nx - 2, ny - 1, k) + at(nx - 1, ny - 2, k)) / 2.0f;
+            }
+        }
         break;
-    }
-    case 4: {  // z bottom (:311-359)
-        if (t >= (nx - 1) * ny) return;
-        const int i = 1 + t % (nx - 1), j = t / (nx - 1);
-        const float v = at(i, j, 1);
-        at(i, j, 0) = (kind == kPZ) ? -fabsf(v) : v;
-        break;
-    }
-    case 5: {  // corner lines of the back wall, k=2..nz-1 (:363-466)
-        if (t >= nz - 2) return;
-        const int k = 1 + t;
-        if (kind == kBX) {
-            at(nx - 1, 0, k) = at(nx - 2, 0, k);
-            at(nx - 1, ny - 1, k) = at(nx - 2, ny - 1, k);
-        } else {
-            at(nx - 1, 0, k) = (at(nx - 2, 0, k) + at(nx - 1, 1, k)) / 2.0f;
-            at(nx - 1, ny - 1, k) = (at(nx - 2, ny - 1, k) + at(nx - 1, ny - 2, k)) / 2.0f;
+    default:   // passes 6 (back-wall lines j=2..ny-1, :470-566) and 7 (x-edge lines, :570-728)
+        for (int t = t0; t < ny - 2; t += stride) {
+            const int j = 1 + t;
+            at(nx - 1, j, 0) = at(nx - 1, j, 1);
+            if (kind != kEP) at(nx - 1, j, nz - 1) = at(nx - 2, j, nz - 1);
+        }
+        for (int t = t0; t < nx - 1; t += stride) {
+            const int i = 1 + t;
+            at(i, 0, 0) = at(i, 0, 1);
+            at(i, ny - 1, 0) = at(i, ny - 1, 1);
+            at(i, 0, nz - 1) = at(i, 0, nz - 2);
+            at(i, ny - 1, nz - 1) = at(i, ny - 1, nz - 2);
         }
         break;
     }
-    case 6: {  // top/bottom lines of the back wall, j=2..ny-1 (:470-566)
-        if (t >= ny - 2) return;
-        const int j = 1 + t;
-        at(nx - 1, j, 0) = at(nx - 1, j, 1);
-        if (kind != kEP) at(nx - 1, j, nz - 1) = at(nx - 2, j, nz - 1);
-        break;
-    }
-    case 7: {  // x-directed edge lines, i=2..nx (:570-728)
-        if (t >= nx - 1) return;
-        const int i = 1 + t;
-        at(i, 0, 0) = at(i, 0, 1);
-        at(i, ny - 1, 0) = at(i, ny - 1, 1);
-        at(i, 0, nz - 1) = at(i, 0, nz - 2);
-        at(i, ny - 1, nz - 1) = at(i, ny - 1, nz - 2);
-        break;
-    }
-    }
 }
 
-__global__ void __launch_bounds__(256) k_outer_pass(Box B, OuterPtrs O, int pass) {
-    outer_pass_block(B, O, pass, blockIdx.x);
+// One run: a launch per phase, blockIdx.y = the field, one cell per thread.
+__global__ void __launch_bounds__(256) k_outer_phase(Box B, OuterPtrs O, int p) {
+    outer_phase(B, O, blockIdx.y, p, blockIdx.x * 256 + threadIdx.x, gridDim.x * 256);
+}
+
+// Many runs (multi-run launch): one block per (run, field) runs the five
+// phases with barriers between them -- one launch instead of five.
+constexpr int kOuterThreads = 1024;
+__device__ __forceinline__ void outer_all_block(const Box& B, const OuterPtrs& O, int f) {
+    for (int p = 0; p < kOuterPhases; ++p) {
+        outer_phase(B, O, f, p, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
 }
 
 void launch_bndry_outer(const Box& B, const OuterPtrs& O, int nf, hipStream_t s) {
-    const int nx = B.nx, ny = B.ny, nz = B.nz;
-    const int ext[8] = {ny * nz, ny * nz, (nx - 1) * nz, (nx - 1) * ny, (nx - 1) * ny,
-                        nz - 2, ny - 2, nx - 1};
-    for (int p = 0; p < 8; ++p) {
-        const dim3 g((ext[p] + 255) / 256, nf);
-        if (g_rec) g_rec->add(kKOuter, g, dim3(256), OuterArgs{B, O, p});
-        else hipLaunchKernelGGL(k_outer_pass, g, dim3(256), 0, s, B, O, p);
+    if (g_rec) {
+        g_rec->add(kKOuter, dim3(nf), dim3(kOuterThreads), OuterArgs{B, O});
+        return;
+    }
+    for (int p = 0; p < kOuterPhases; ++p) {
+        const int n = outer_phase_cells(B, p);
+        hipLaunchKernelGGL(k_outer_phase, dim3((n + 255) / 256, nf), dim3(256), 0, s, B, O, p);
     }
 }
 
@@ -197,12 +244,12 @@ void launch_bndry_inner(const Box& B, const InnerPtrs& I, const int* ij[3], cons
         }
 }
 
-__global__ void __launch_bounds__(256) k_outer_pass_runs(const OuterArgs* __restrict__ A,
-                                                         MultiRun M) {
+__global__ void __launch_bounds__(kOuterThreads) k_outer_all_runs(const OuterArgs* __restrict__ A,
+                                                                  MultiRun M) {
     int r, lb;
     if (!multi_block(M, r, lb)) return;
     const OuterArgs& a = A[r];
-    outer_pass_block(a.B, a.O, a.pass, lb);
+    outer_all_block(a.B, a.O, lb);
 }
 
 __global__ void __launch_bounds__(256) k_inner_runs(const InnerArgs* __restrict__ A, MultiRun M) {
@@ -214,7 +261,7 @@ void launch_multi_bounds(int id, const void* args, const MultiRun& M, dim3 block
                          hipStream_t s) {
     const dim3 g((unsigned)((M.total + 7) & ~7), gy);
     if (id == kKOuter)
-        hipLaunchKernelGGL(k_outer_pass_runs, g, block, 0, s, (const OuterArgs*)args, M);
+        hipLaunchKernelGGL(k_outer_all_runs, g, block, 0, s, (const OuterArgs*)args, M);
     else if (id == kKInner)
         hipLaunchKernelGGL(k_inner_runs, g, block, 0, s, (const InnerArgs*)args, M);
 }

@@ -83,7 +83,7 @@ class FullRef(FieldRef):
         return q + h + o + [pq / S(sc["ti_te"]), ctl[1], ctl[2], ctl[3]]
 
     def outer_all(self, X, vals):
-        """bndry_outer's eight passes (k_outer_pass order) on the 34 fields."""
+        """bndry_outer's eight passes, in order, on the 34 fields."""
         out = []
         for f in range(34):
             kind = KINDS[f]
