@@ -152,11 +152,13 @@ def loss(samples, target) -> float:
 
 
 def gradient_fd(theta, target, cfg: MisfitConfig = MisfitConfig(), rel: float = 1e-3,
-                together: bool = True):
+                together: bool = True, samples_out: list | None = None):
     """Central-difference gradient of the loss with respect to theta, step
     rel * |theta_i| (rel where theta_i = 0), and the device seconds its
     forward runs took: all 2 len(theta) runs stepped together, or (together
-    False) one after another.  Both give the same gradient, bit for bit."""
+    False) one after another.  Both give the same gradient, bit for bit.
+    samples_out: a list that receives the runs' samples (theta_0 + h, theta_0
+    - h, theta_1 + h, ...)."""
     theta = np.asarray(theta, dtype=np.float64)
     hs, runs = [], []
     for i in range(len(theta)):
@@ -176,4 +178,6 @@ def gradient_fd(theta, target, cfg: MisfitConfig = MisfitConfig(), rel: float = 
             secs += a
     g = np.array([(loss(samples[2 * i], target) - loss(samples[2 * i + 1], target)) /
                   (2.0 * hs[i]) for i in range(len(theta))])
+    if samples_out is not None:
+        samples_out.extend(samples)
     return g, secs
