@@ -203,25 +203,10 @@ __global__ void k_adj_push_elec(PeAdj A) {
 // reverse; within a pass written and read cells are disjoint).  Pass 0 (the
 // inflow face) sends its adjoint, times the field's coefficient, to its
 // control (sums in double).  blockIdx.y = field.
-__global__ void k_adj_outer_inflow_k(Box B, OuterAdj O, double* ctlb) {
-    const int f = blockIdx.y;
-    float* q = O.f[f];
-    const int t = gtid();
-    double v = 0.0;
-    if (t < B.ny * B.nz) {
-        const int c = B.IX(0, t % B.ny, t / B.ny);
-        v = q[c];
-        q[c] = 0.0f;
-    }
-    const double s = full_block_sum(v);
-    if (threadIdx.x == 0 && O.ctl[f] >= 0) atomicAdd(ctlb + O.ctl[f], s * (double)O.coef[f]);
-}
-
-__global__ void k_adj_outer_pass_k(Box B, OuterAdj O, int pass) {
-    const int f = blockIdx.y;
+// element t of pass `pass` (1..7) of field f
+__device__ inline void adj_outer_pass(const Box& B, const OuterAdj& O, int f, int pass, int t) {
     float* q = O.f[f];
     const int kind = O.kind[f];
-    const int t = gtid();
     const int nx = B.nx, ny = B.ny, nz = B.nz;
     auto take = [&](int i, int j, int k) {
         const int c = B.IX(i, j, k);
@@ -298,6 +283,48 @@ __global__ void k_adj_outer_pass_k(Box B, OuterAdj O, int pass) {
     }
 }
 
+// The passes in reverse, in the forward's five phases: 7 and 6, then 5,
+// then 4 and 3, then 2, then 1 and 0 (the inflow face, whose adjoint goes
+// to its control, summed in double).  The two passes of a phase take and
+// give disjoint cells, so they run together and every sum keeps its order.
+// blockIdx.y = field.
+__global__ void k_adj_outer_phase_k(Box B, OuterAdj O, int phase, double* ctlb) {
+    const int f = blockIdx.y;
+    const int t = gtid();
+    const int nx = B.nx, ny = B.ny, nz = B.nz;
+    switch (phase) {
+    case 0:
+        if (t < nx - 1) adj_outer_pass(B, O, f, 7, t);
+        if (t < ny - 2) adj_outer_pass(B, O, f, 6, t);
+        break;
+    case 1:
+        if (t < nz - 2) adj_outer_pass(B, O, f, 5, t);
+        break;
+    case 2:
+        if (t < (nx - 1) * ny) {
+            adj_outer_pass(B, O, f, 4, t);
+            adj_outer_pass(B, O, f, 3, t);
+        }
+        break;
+    case 3:
+        if (t < (nx - 1) * nz) adj_outer_pass(B, O, f, 2, t);
+        break;
+    default: {
+        double v = 0.0;
+        if (t < ny * nz) {
+            adj_outer_pass(B, O, f, 1, t);
+            float* q = O.f[f];
+            const int c = B.IX(0, t % ny, t / ny);
+            v = q[c];
+            q[c] = 0.0f;
+        }
+        const double s = full_block_sum(v);
+        if (threadIdx.x == 0 && O.ctl[f] >= 0) atomicAdd(ctlb + O.ctl[f], s * (double)O.coef[f]);
+        break;
+    }
+    }
+}
+
 // --- launches ---------------------------------------------------------------
 
 void launch_adj_bande_plasma(const EfAdj& A, hipStream_t s) {
@@ -329,13 +356,11 @@ void launch_adj_push_elec(const PeAdj& A, hipStream_t s) {
 
 void launch_adj_outer(const Box& B, const OuterAdj& O, int nf, double* ctlb, hipStream_t s) {
     const int nx = B.nx, ny = B.ny, nz = B.nz;
-    const int ext[8] = {ny * nz, ny * nz, (nx - 1) * nz, (nx - 1) * ny, (nx - 1) * ny,
-                        nz - 2, ny - 2, nx - 1};
-    for (int p = 7; p >= 1; --p)
-        hipLaunchKernelGGL(k_adj_outer_pass_k, dim3((ext[p] + 255) / 256, nf), dim3(256), 0, s, B,
-                           O, p);
-    hipLaunchKernelGGL(k_adj_outer_inflow_k, dim3((ext[0] + 255) / 256, nf), dim3(256), 0, s, B,
-                       O, ctlb);
+    const int ext[5] = {(ny - 2 > nx - 1) ? ny - 2 : nx - 1, nz - 2, (nx - 1) * ny,
+                        (nx - 1) * nz, ny * nz};
+    for (int p = 0; p < 5; ++p)
+        hipLaunchKernelGGL(k_adj_outer_phase_k, dim3((ext[p] + 255) / 256, nf), dim3(256), 0, s,
+                           B, O, p, ctlb);
 }
 
 }  // namespace g3m3

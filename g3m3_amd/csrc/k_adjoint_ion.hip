@@ -46,11 +46,24 @@ __global__ void k_adj_ion_local(IonAdj A) {
     }
     const int st[3] = {1, B.nx, B.nx * B.ny};
     const float dt = A.delt, dd[3] = {A.dxt, A.dyt, A.dzt};
-    float ob[10];
+    // Every operand, and every adjoint this cell accumulates into, is loaded
+    // before the first store: on this hardware a wait for a load also waits
+    // for every store issued before it.  The old set's and the source set's
+    // adjoints may be one array (a stage whose old is its source): then the
+    // two increments land on it in the order old, source.
+    float ob[10], olb[10], sbv[10];
     for (int f = 0; f < 10; ++f) {
         ob[f] = A.ob[f][c];
-        A.oldb[f][c] += ob[f];           // out = old + ... on the interior
+        olb[f] = A.oldb[f][c];
+        sbv[f] = A.sb[f] == A.oldb[f] ? 0.0f : A.sb[f][c];
     }
+    float ebv[3], bsbv[3], tvbv[3];
+    for (int a = 0; a < 3; ++a) {
+        ebv[a] = A.eb[a][c];
+        bsbv[a] = A.bsb[a][c];
+        tvbv[a] = A.tvb[a][c];
+    }
+    const float btbv = A.btotb[c];
     const float r = A.src[R_][c];
     const float x = A.src[X_][c], y = A.src[Y_][c], z = A.src[Z_][c];
     const float xy = A.src[XY_][c], xz = A.src[XZ_][c], yz = A.src[YZ_][c];
@@ -96,7 +109,7 @@ __global__ void k_adj_ion_local(IonAdj A) {
         const float PA = ob[PX_ + a];
         const float de = (e[a] + tc[a]) * re, dv = (qc[a] - tc[a]) * skin;
         const float deB = dt * qden * PA;
-        A.eb[a][c] += deB * re;
+        ebv[a] += deB * re;
         tcB[a] = deB * re - deB * skin;
         qcB[a] = deB * skin;
         skinB += deB * (qc[a] - tc[a]);
@@ -104,7 +117,8 @@ __global__ void k_adj_ion_local(IonAdj A) {
         rB -= PA * gfac * pos[a] / radius;
     }
     rB += qdenB / rm;
-    if (bmag / bmax > 1.0f) A.btotb[c] += -skinB * re / (weight * weight) / bmax;
+    const bool btot_on = bmag / bmax > 1.0f;
+    const float btb_new = btot_on ? btbv + -skinB * re / (weight * weight) / bmax : btbv;
     // tc = t x b, qc = v x b
     tB[0] = b[1] * tcB[2] - b[2] * tcB[1];
     tB[1] = b[2] * tcB[0] - b[0] * tcB[2];
@@ -219,20 +233,28 @@ __global__ void k_adj_ion_local(IonAdj A) {
         const float bm2B = -sfB * re * bmax / (bm2 * bm2);
         for (int a = 0; a < 3; ++a) abB[a] += bm2B * ab[a] / bm2;
     }
-    A.sb[R_][c] += rB;
-    A.sb[X_][c] += xB;
-    A.sb[Y_][c] += yB;
-    A.sb[Z_][c] += zB;
-    A.sb[XY_][c] += xyB;
-    A.sb[XZ_][c] += xzB;
-    A.sb[YZ_][c] += yzB;
+    // out = old + ... on the interior: old's adjoint first
+    for (int f = 0; f < 10; ++f) olb[f] += ob[f];
+    const float srcB[10] = {rB, xB, yB, zB, 0.0f, 0.0f, 0.0f, xyB, xzB, yzB};
+    for (int f = 0; f < 10; ++f) {
+        if (f >= PX_ && f <= PZ_) {
+            A.oldb[f][c] = olb[f];       // the momenta's source adjoint comes in the gather
+        } else if (A.sb[f] == A.oldb[f]) {
+            A.oldb[f][c] = olb[f] + srcB[f];
+        } else {
+            A.oldb[f][c] = olb[f];
+            A.sb[f][c] = sbv[f] + srcB[f];
+        }
+    }
     for (int a = 0; a < 3; ++a) {
-        A.bsb[a][c] += bB[a];
-        A.tvb[a][c] += tB[a];
+        A.eb[a][c] = ebv[a];
+        A.bsb[a][c] = bsbv[a] + bB[a];
+        A.tvb[a][c] = tvbv[a] + tB[a];
         for (int d = 0; d < 3; ++d) D[(size_t)(3 * a + d) * n + c] = Dc[a][d];
         S[(size_t)a * n + c] = abB[a] / 12.0f;
         QL[(size_t)a * n + c] = vB[a];
     }
+    if (btot_on) A.btotb[c] = btb_new;
 }
 
 __global__ void k_adj_ion_gather(IonAdj A) {
@@ -297,17 +319,21 @@ __global__ void k_adj_ion_gather(IonAdj A) {
         rB -= qvB[a] * qv[a] / aq;
     }
     if (A.src[R_][m] > kDmin) add[R_] += rB;
-    for (int f = 0; f < 10; ++f) A.sb[f][m] += add[f];
-    // the smoothed field of the gyrotropic terms: (6 b[c] + 6 neighbours)/12
+    // the smoothed field of the gyrotropic terms: (6 b[c] + 6 neighbours)/12,
+    // read (with the accumulators) before the first store
+    float acc[3], sbv[10], bsbv[3];
     for (int a = 0; a < 3; ++a) {
         const float* s = S + (size_t)a * n;
-        float acc = 6.0f * s[m];
+        acc[a] = 6.0f * s[m];
         for (int d = 0; d < 3; ++d) {
-            if (inb(d, -1)) acc += s[m - st[d]];
-            if (inb(d, 1)) acc += s[m + st[d]];
+            if (inb(d, -1)) acc[a] += s[m - st[d]];
+            if (inb(d, 1)) acc[a] += s[m + st[d]];
         }
-        A.bsb[a][m] += acc;
+        bsbv[a] = A.bsb[a][m];
     }
+    for (int f = 0; f < 10; ++f) sbv[f] = A.sb[f][m];
+    for (int f = 0; f < 10; ++f) A.sb[f][m] = sbv[f] + add[f];
+    for (int a = 0; a < 3; ++a) A.bsb[a][m] = bsbv[a] + acc[a];
 }
 
 // lap_plasma, pass 1: per interior cell c, the adjoints the six |dv| carry
