@@ -89,17 +89,20 @@ def test_fused_batch_equals_runs_alone():
     grids concatenated, a captured graph per step) reach exactly the arrays,
     time and time steps each run reaches alone -- runs of several sizes,
     one of them a step ahead (the other buffer parity) when the batch is
-    made, and the steps issued in two calls."""
+    made, one with a cell size that is not a power of two (its stage kernels
+    divide where the others multiply by exact reciprocals, so that stage's
+    launch splits by kernel), and the steps issued in two calls."""
     from g3m3_amd.driver import Batch, FastRun
     specs = [dict(L=8, n_sw=3.0, v_sw=350.0, imf=(1.0, -2.0, -4.0)),
              dict(L=9, n_sw=12.0, v_sw=700.0, imf=(-3.0, 0.5, 2.0)),
              dict(L=11, n_sw=6.0, v_sw=500.0, imf=(0.0, 4.0, -1.0)),
-             dict(L=14, n_sw=1.5, v_sw=420.0, imf=(2.0, 2.0, 2.0))]
+             dict(L=14, n_sw=1.5, v_sw=420.0, imf=(2.0, 2.0, 2.0)),
+             dict(dims=(19, 17, 13), spacing=0.75, n_sw=4.0, v_sw=450.0, imf=(0.5, -1.0, 3.0))]
     mk = lambda: [make_workload("single61", seed=60 + i, **sp) for i, sp in enumerate(specs)]
     batch = Batch(mk(), streams=2)
     batch.init_dt()
     batch.runs[1].full_step()
-    print("fused batch steps, 4 runs", flush=True)
+    print("fused batch steps, 5 runs", flush=True)
     batch.fused_steps(3)
     batch.fused_steps(2)
     assert batch.fused_launches > 0
