@@ -74,6 +74,17 @@ def _modes(rng, shape, xs, n=3):
     return out / n
 
 
+def dipole_b0(ax, ay, az) -> np.ndarray:
+    """(3, nz, ny, nx) float32: the dipole of make_workload's planet
+    (|B| = 10 b_equiv on the equator at r = 3) at the float32 node
+    coordinates ax (1, 1, nx), ay (1, ny, 1), az (nz, 1, 1)."""
+    ar = np.sqrt(ax * ax + ay * ay + az * az).astype(F)
+    b0m = F(10.0 * 27.0)
+    bmag = -b0m / ar ** 5
+    return np.stack([-3.0 * bmag * ax * az, -3.0 * bmag * ay * az,
+                     bmag * (ax * ax + ay * ay - 2.0 * az * az)]).astype(F)
+
+
 def field_setup(nx: int = 41, ny: int = 41, nz: int = 21, spacing: float = 0.5,
                 seed: int = 20261207, n_sw: float = 5.0, v_sw: float = 400.0,
                 t_sw: float = 10.0, imf=(0.0, 0.0, -5.0), eta: float = 1e-3,
@@ -95,11 +106,7 @@ def field_setup(nx: int = 41, ny: int = 41, nz: int = 21, spacing: float = 0.5,
     ax = (xmin + h * np.arange(nx, dtype=F))[None, None, :]
     ay = (ymin + h * np.arange(ny, dtype=F))[None, :, None]
     az = (zmin + h * np.arange(nz, dtype=F))[:, None, None]
-    ar = np.sqrt(ax * ax + ay * ay + az * az).astype(F)
-    b0m = F(10.0 * 27.0)
-    bmag = -b0m / ar ** 5
-    b0 = np.stack([-3.0 * bmag * ax * az, -3.0 * bmag * ay * az,
-                   bmag * (ax * ax + ay * ay - 2.0 * az * az)]).astype(F)
+    b0 = dipole_b0(ax, ay, az)
     srho = n_sw / RHO_EQUIV_CC
     svel = v_sw / V_EQUIV_KMS
     spress = n_sw * 1e6 * t_sw * EV / P_EQUIV_PA
@@ -477,6 +484,55 @@ class FullModel:
             _ct.c_void_p(ctl_bar.data_ptr()), c.ctypes.data_as(_ct.c_void_p))
         if rc != 0:
             raise _lib.G3M3Error(f"g3m3_full_model_adjoint returned {rc}")
+
+    # --- as the inner box of a nested pair (g3m3_full_model_set_parent) ---
+    def set_parent(self, parent: FullSetup):
+        """Bound this box by flanks from ``parent`` (same extents in cells,
+        a whole multiple of this box's spacing) and restrict it into the
+        parent's core."""
+        geom = np.array([self.setup.scal[k] for k in ("xmin", "xmax", "ymin", "ymax", "zmin",
+                                                     "zmax")]
+                        + [parent.scal[k] for k in ("xmin", "xmax", "ymin", "ymax", "zmin",
+                                                    "zmax")], F)
+        rc = self._lib.g3m3_full_model_set_parent(_ct.c_void_p(self.h),
+                                                  geom.ctypes.data_as(_ct.c_void_p))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_set_parent returned {rc}")
+
+    def step_inner(self, s_in, s_out, ctl, p_old, p_cur, t12, tape=None):
+        c = self._ctl(ctl)
+        t = np.ascontiguousarray(np.asarray(t12, F).reshape(4))
+        rc = self._lib.g3m3_full_model_step_inner(
+            _ct.c_void_p(self.h), _ct.c_void_p(s_in.data_ptr()), _ct.c_void_p(s_out.data_ptr()),
+            _ct.c_void_p(tape.data_ptr() if tape is not None else 0),
+            c.ctypes.data_as(_ct.c_void_p), _ct.c_void_p(p_old.data_ptr()),
+            _ct.c_void_p(p_cur.data_ptr()), t.ctypes.data_as(_ct.c_void_p))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_step_inner returned {rc}")
+
+    def adjoint_inner(self, tape, sout_bar, sin_bar, ctl_bar, ctl, t12, p_old_bar, p_cur_bar):
+        c = self._ctl(ctl)
+        t = np.ascontiguousarray(np.asarray(t12, F).reshape(4))
+        rc = self._lib.g3m3_full_model_adjoint_inner(
+            _ct.c_void_p(self.h), _ct.c_void_p(tape.data_ptr()),
+            _ct.c_void_p(sout_bar.data_ptr()), _ct.c_void_p(sin_bar.data_ptr()),
+            _ct.c_void_p(ctl_bar.data_ptr()), c.ctypes.data_as(_ct.c_void_p),
+            t.ctypes.data_as(_ct.c_void_p), _ct.c_void_p(p_old_bar.data_ptr()),
+            _ct.c_void_p(p_cur_bar.data_ptr()))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_adjoint_inner returned {rc}")
+
+    def corer(self, st, p_state):
+        rc = self._lib.g3m3_full_model_corer(_ct.c_void_p(self.h), _ct.c_void_p(st.data_ptr()),
+                                             _ct.c_void_p(p_state.data_ptr()))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_corer returned {rc}")
+
+    def corer_adjoint(self, s_bar, p_bar):
+        rc = self._lib.g3m3_full_model_corer_adjoint(
+            _ct.c_void_p(self.h), _ct.c_void_p(s_bar.data_ptr()), _ct.c_void_p(p_bar.data_ptr()))
+        if rc != 0:
+            raise _lib.G3M3Error(f"g3m3_full_model_corer_adjoint returned {rc}")
 
     def _ptrs(self, st):
         return (_ct.c_void_p * len(FULL_SAMPLED))(*[st[i].data_ptr() for i in FULL_SAMPLED])

@@ -20,10 +20,20 @@
 // next step's push_elec and step-2 BandE): 35 arrays of nx*ny*nz.
 // Controls ctl = {eta, sbx, sby, sbz, srho, spx, spy, spz, spress}: the
 // resistivity and the upstream values bndry_outer imposes.
+//
+// An inner box of a nested pair (g3m3_full_model_set_parent) bounds each
+// stage by the flanks from its parent instead (code/sub_flanks.f90), as the
+// reference's box-step does for box < n_grids: step 1's set at step 1's
+// t_grid; the Lapidus set and the FCT output at the full step's t_grid.
+// Step 2's flanks go to wrk, where the Lapidus pass and its own flanks
+// overwrite them before any read, so the stage-2 set keeps the state's
+// shell: no boundary pass there.  The corer (code/sub_corer.f90) restricts
+// the box into its parent's core.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "../../include/g3m3.h"
 #include "g3m3_fused.h"
@@ -108,6 +118,23 @@ struct FullModel {
     float *A = nullptr, *W3B = nullptr, *S2B = nullptr, *W1B = nullptr;
     float *EB[3], *EVB[3], *CURB[3], *BSB[3], *BTB, *TVB[3], *VB[3];
     float *fct_scr, *lap_scr, *ion_scr, *ef_scr;
+    // as an inner box: its 2-layer shell's cells with their parent-box
+    // corners and weights, and the corer's geometry
+    ShellPt* shell = nullptr;
+    int nshell = 0;
+    int c_nskip = 0, c_nbx1 = 0, c_nby1 = 0, c_nbz1 = 0;
+};
+
+// the flanks of one inner-box step: the parent's states (34 fields each,
+// canonical order) and the time weights of step 1's t_grid (t1a, t2a) and
+// of the later stages' (t1b, t2b); for the adjoint, the parent adjoints the
+// flanks add into
+struct Flank {
+    const float* p_old;
+    const float* p_cur;
+    float t1a, t2a, t1b, t2b;
+    float* p_old_bar;
+    float* p_cur_bar;
 };
 
 float* fld(float* base, int n, int f) { return base + (size_t)n * f; }
@@ -177,6 +204,90 @@ Inflow inflow(const g3m3_full_model_desc& d, const float* c) {
         I.coef[BX + a] = 1.0f;
     }
     return I;
+}
+
+// The flanks' adjoint (code/sub_flanks.f90, transposed): per shell cell
+// and field, w = frac w + (1 - frac) I0 for each panel containing the cell
+// (bottom, top, front, back, left, right), I0 = the trilinear weights times
+// (old t1 + cur t2) at the eight parent corners.  The cell's adjoint keeps
+// the product of the fracs; the parent corners receive t1 / t2 times their
+// weight times the adjoint of I0 (atomics: neighbouring shell cells share
+// corners).  blockIdx.y = field.
+__global__ void k_adj_flanks(const ShellPt* __restrict__ tab, int npts, FieldPtrs xb,
+                             FieldPtrs ob, FieldPtrs cb, float t1, float t2) {
+    const int t = gtid(), f = blockIdx.y;
+    if (t >= npts) return;
+    const ShellPt p = tab[t];
+    float pw = 1.0f, pI = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+        if (p.flags & (1 << b)) {
+            const float frac = (p.flags & (256 << b)) ? 0.5f : 0.0f;
+            pI = frac * pI + (1.0f - frac);
+            pw *= frac;
+        }
+    const float wb = xb.p[f][p.c];
+    xb.p[f][p.c] = pw * wb;
+    const float I0b = pI * wb;
+    const float dx = p.dx, dy = p.dy, dz = p.dz;
+    const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
+    const int X = 1, Y = p.sy, Z = p.sz, c0 = p.cb;
+    const int cn[8] = {c0, c0 + Z, c0 + Y, c0 + Y + Z, c0 + X, c0 + X + Z, c0 + X + Y,
+                       c0 + X + Y + Z};
+    const float w8[8] = {ddx * ddy * ddz, ddx * ddy * dz, ddx * dy * ddz, ddx * dy * dz,
+                         dx * ddy * ddz,  dx * ddy * dz,  dx * dy * ddz,  dx * dy * dz};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        atomicAdd(ob.p[f] + cn[n], t1 * w8[n] * I0b);
+        atomicAdd(cb.p[f] + cn[n], t2 * w8[n] * I0b);
+    }
+}
+
+// The corer's adjoint (code/sub_corer.f90:34-53, transposed): each parent
+// core cell's adjoint goes to the seven fine cells of its average (6/12 to
+// the centre, 1/12 to each neighbour), then the parent cell's adjoint is
+// zeroed (the corer overwrote it).  blockIdx.y = field.
+__global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, int nskip, int nbx1, int nby1,
+                            int nbz1, int ni, int nj, int nk) {
+    const int f = blockIdx.y, t = gtid();
+    if (t >= ni * nj * nk) return;
+    const int a = t % ni, bq = (t / ni) % nj, cq = t / (ni * nj);
+    const int is = 1 + nskip + a * nskip, js = 1 + nskip + bq * nskip, ks = 1 + nskip + cq * nskip;
+    const int ib = nbx1 + (is - 1) / nskip, jb = nby1 + (js - 1) / nskip,
+              kb = nbz1 + (ks - 1) / nskip;
+    const int cc = B.IX(ib - 1, jb - 1, kb - 1);
+    const float v = cb.p[f][cc];
+    cb.p[f][cc] = 0.0f;
+    const int c = B.IX(is - 1, js - 1, ks - 1), X = 1, Y = B.nx, Z = B.nx * B.ny;
+    float* r = fb.p[f];
+    atomicAdd(r + c, 6.0f * v / 12.0f);
+    const int nb[6] = {c + X, c - X, c + Y, c - Y, c + Z, c - Z};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) atomicAdd(r + nb[q], v / 12.0f);
+}
+
+// corer counts: fine indices 1+nskip, 1+2 nskip, ... <= n - nskip (1-based)
+int corer_count(int n, int nskip) { return (n - nskip - (1 + nskip)) / nskip + 1; }
+
+void flanks(FullModel* M, float* set, const Flank& Fk, float t1, float t2) {
+    FlankPtrs P{};
+    for (int f = 0; f < NFLD; ++f) {
+        P.tgt[f] = fld(set, M->n, f);
+        P.old[f] = fld((float*)Fk.p_old, M->n, f);
+        P.cur[f] = fld((float*)Fk.p_cur, M->n, f);
+    }
+    launch_flanks_table(M->B, M->shell, M->nshell, P, NFLD, t1, t2, M->s);
+}
+
+void flanks_adj(FullModel* M, float* bar, const Flank& Fk, float t1, float t2) {
+    FieldPtrs xb{}, ob{}, cb{};
+    for (int f = 0; f < NFLD; ++f) {
+        xb.p[f] = fld(bar, M->n, f);
+        ob.p[f] = fld(Fk.p_old_bar, M->n, f);
+        cb.p[f] = fld(Fk.p_cur_bar, M->n, f);
+    }
+    hipLaunchKernelGGL(k_adj_flanks, dim3((M->nshell + 255) / 256, NFLD), dim3(256), 0, M->s,
+                       M->shell, M->nshell, xb, ob, cb, t1, t2);
 }
 
 void outer(FullModel* M, float* set, const Inflow& I) {
@@ -322,6 +433,7 @@ void g3m3_full_model_destroy(void* model) {
     if (!M) return;
     hipStreamSynchronize(M->s);
     hipFree(M->mem);
+    if (M->shell) hipFree(M->shell);
     delete M;
 }
 
@@ -333,10 +445,12 @@ long long g3m3_full_model_tape_floats(void* model) {
     return model ? (long long)T_N * ((FullModel*)model)->n : 0;
 }
 
-int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* tape,
-                         const float* ctl) {
+// one step, outermost (Fk null: bndry_outer) or inner (the flanks of Fk)
+static int step_impl(void* model, const float* s_in, float* s_out, float* tape,
+                     const float* ctl, const Flank* Fk) {
     FullModel* M = (FullModel*)model;
     if (!M || !s_in || !s_out || !ctl || s_in == s_out) return 1;
+    if (Fk && (!M->shell || !Fk->p_old || !Fk->p_cur)) return 1;
     const g3m3_full_model_desc& d = M->d;
     const int n = M->n;
     const float eta = ctl[0], delt = d.delt;
@@ -355,7 +469,8 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
     copyf(M, fld(W1, n, EP), fld(S, n, WE), 1);
     push_species(M, W1, S, S, 0.5f * delt);
     push_b(M, W1, S, 0.5f * delt);
-    outer(M, W1, I);
+    if (Fk) flanks(M, W1, *Fk, Fk->t1a, Fk->t2a);
+    else outer(M, W1, I);
     // stage 2
     M->ef = ef_at(fld(T, n, T_EF2), n);
     efield(M, W1, eta);
@@ -368,7 +483,7 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
                          d.gamma1, delt, r, r, r, M->s);
     }
     push_b(M, S2, S, delt);
-    outer(M, S2, I);
+    if (!Fk) outer(M, S2, I);
     // Lapidus into a copy of W1
     copyf(M, W3, W1, NFLD);
     for (int sp = 0; sp < 3; ++sp) {
@@ -401,7 +516,8 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
         const float* v[3] = {M->vel[0], M->vel[1], M->vel[2]};
         launch_lapidus(M->B, L, 4, v, delt, M->s);
     }
-    outer(M, W3, I);
+    if (Fk) flanks(M, W3, *Fk, Fk->t1b, Fk->t2b);
+    else outer(M, W3, I);
     // flux correction
     FctParams F{};
     F.B = M->B;
@@ -412,7 +528,8 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
         F.chi[f] = f >= BX ? d.diferg : d.difrho;
     }
     launch_fct_fused(F, NFLD, M->s);
-    outer(M, S4, I);
+    if (Fk) flanks(M, S4, *Fk, Fk->t1b, Fk->t2b);
+    else outer(M, S4, I);
     copyf(M, s_out, S4, NFLD);
     copyf(M, fld(s_out, n, WE), fld(W3, n, EP), 1);
     return launched("step") ? 0 : 2;
@@ -548,10 +665,11 @@ void stage_adj(FullModel* M, const EF& ef, const float* src, const float* old, f
 
 }  // namespace
 
-int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_bar,
-                            float* sin_bar, double* ctl_bar, const float* ctl) {
+static int adjoint_impl(void* model, const float* tape, const float* sout_bar, float* sin_bar,
+                        double* ctl_bar, const float* ctl, const Flank* Fk) {
     FullModel* M = (FullModel*)model;
     if (!M || !tape || !sout_bar || !sin_bar || !ctl_bar || !ctl) return 1;
+    if (Fk && (!M->shell || !Fk->p_old_bar || !Fk->p_cur_bar)) return 1;
     const g3m3_full_model_desc& d = M->d;
     const int n = M->n;
     const float eta = ctl[0], delt = d.delt;
@@ -568,8 +686,9 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
     zero(M, M->S2B, (size_t)NFLD * n);
     zero(M, M->W1B, (size_t)NFLD * n);
     copyf(M, M->A, sout_bar, NFLD);
-    // S' = bndry_outer(fcsmooth(S, W3))
-    outer_adj(M, M->A, S4, I, ctl_bar);
+    // S' = bndry_outer(fcsmooth(S, W3)) (or the flanks)
+    if (Fk) flanks_adj(M, M->A, *Fk, Fk->t1b, Fk->t2b);
+    else outer_adj(M, M->A, S4, I, ctl_bar);
     for (int f0 = 0; f0 < NFLD;) {   // (assigns W3B)
         // groups of up to three fields of one coefficient
         const int end = f0 < BX ? BX : NFLD;
@@ -596,7 +715,8 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
         src.p[0] = const_cast<float*>(fld((float*)sout_bar, n, WE));
         hipLaunchKernelGGL(k_add_all, dim3((n + 255) / 256, 1), dim3(256), 0, s, n, dst, src);
     }
-    outer_adj(M, M->W3B, W3, I, ctl_bar);
+    if (Fk) flanks_adj(M, M->W3B, *Fk, Fk->t1b, Fk->t2b);
+    else outer_adj(M, M->W3B, W3, I, ctl_bar);
     // W3 = Lapidus of S2 on the interior, W1 on the boundary
     {
         FieldPtrs dst{}, src{};
@@ -669,8 +789,8 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
         launch_adj_lap_species(L, s);
         launch_adj_vtot(n, V, s);
     }
-    // S2 = bndry_outer(push from W1 with old = S)
-    outer_adj(M, M->S2B, S2, I, ctl_bar);
+    // S2 = bndry_outer(push from W1 with old = S) (an inner box: no boundary pass)
+    if (!Fk) outer_adj(M, M->S2B, S2, I, ctl_bar);
     {
         FieldPtrs dst{}, src{};
         for (int f = 0; f < NFLD; ++f) {
@@ -682,8 +802,9 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
     }
     stage_adj(M, ef_at(fld(T, n, T_EF2), n), W1, S, M->S2B, M->W1B, SB, delt, true, ctl_bar,
               eta);
-    // W1 = bndry_outer(push from S with delt/2; its pressure is WE)
-    outer_adj(M, M->W1B, W1, I, ctl_bar);
+    // W1 = bndry_outer(push from S with delt/2; its pressure is WE) (or the flanks)
+    if (Fk) flanks_adj(M, M->W1B, *Fk, Fk->t1a, Fk->t2a);
+    else outer_adj(M, M->W1B, W1, I, ctl_bar);
     {
         FieldPtrs dst{}, src{};
         dst.p[0] = fld(SB, n, WE);
@@ -700,6 +821,114 @@ int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_ba
     stage_adj(M, ef_at(fld(T, n, T_EF1), n), S, S, M->W1B, SB, SB, 0.5f * delt, false, ctl_bar,
               eta);
     return launched("adjoint") ? 0 : 2;
+}
+
+int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* tape,
+                         const float* ctl) {
+    return step_impl(model, s_in, s_out, tape, ctl, nullptr);
+}
+
+int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_bar,
+                            float* sin_bar, double* ctl_bar, const float* ctl) {
+    return adjoint_impl(model, tape, sout_bar, sin_bar, ctl_bar, ctl, nullptr);
+}
+
+int g3m3_full_model_set_parent(void* model, const float* geom) {
+    FullModel* M = (FullModel*)model;
+    if (!M || !geom) return 1;
+    const g3m3_full_model_desc& d = M->d;
+    FlankGeom G;
+    // this box (fine, s*) and its parent (coarse, b*): origin and spacing
+    G.sxmin = geom[0];
+    G.symin = geom[2];
+    G.szmin = geom[4];
+    G.sx = (geom[1] - geom[0]) / (d.nx - 1.0f);
+    G.sy = (geom[3] - geom[2]) / (d.ny - 1.0f);
+    G.sz = (geom[5] - geom[4]) / (d.nz - 1.0f);
+    G.bxmin = geom[6];
+    G.bymin = geom[8];
+    G.bzmin = geom[10];
+    G.delx = (geom[7] - geom[6]) / (d.nx - 1.0f);
+    G.dely = (geom[9] - geom[8]) / (d.ny - 1.0f);
+    G.delz = (geom[11] - geom[10]) / (d.nz - 1.0f);
+    const int n = shell_count(M->B);
+    std::vector<ShellPt> h(n);
+    build_shell(M->B, G, h.data());
+    for (const ShellPt& p : h) {   // every trilinear corner inside the parent box
+        const int x = p.cb % d.nx, y = (p.cb / d.nx) % d.ny, z = p.cb / (d.nx * d.ny);
+        if (p.cb < 0 || x + 1 >= d.nx || y + 1 >= d.ny || z + 1 >= d.nz) {
+            fprintf(stderr, "g3m3_full_model_set_parent: the shell reads outside the parent\n");
+            return 1;
+        }
+    }
+    // the corer's geometry (code/sub_corer.f90:17-32)
+    M->c_nskip = (int)(G.delx / G.sx);
+    M->c_nbx1 = (int)(1.0f + (G.sxmin - G.bxmin) / G.delx + 0.5f);
+    M->c_nby1 = (int)(1.0f + (G.symin - G.bymin) / G.dely + 0.5f);
+    M->c_nbz1 = (int)(1.0f + (G.szmin - G.bzmin) / G.delz + 0.5f);
+    if (M->c_nskip < 1) return 1;
+    {   // the corer's parent cells nb1 + 1 .. nb1 + count (1-based) inside the parent
+        const int nb1[3] = {M->c_nbx1, M->c_nby1, M->c_nbz1}, ext[3] = {d.nx, d.ny, d.nz};
+        for (int a = 0; a < 3; ++a) {
+            const int cnt = corer_count(ext[a], M->c_nskip);
+            if (cnt > 0 && (nb1[a] < 0 || nb1[a] + cnt > ext[a])) {
+                fprintf(stderr, "g3m3_full_model_set_parent: the core lies outside the parent\n");
+                return 1;
+            }
+        }
+    }
+    if (M->shell) (void)hipFree(M->shell);
+    M->shell = nullptr;
+    if (hipMalloc(&M->shell, sizeof(ShellPt) * n) != hipSuccess ||
+        hipMemcpy(M->shell, h.data(), sizeof(ShellPt) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return 2;
+    M->nshell = n;
+    return 0;
+}
+
+int g3m3_full_model_step_inner(void* model, const float* s_in, float* s_out, float* tape,
+                               const float* ctl, const float* p_old, const float* p_cur,
+                               const float* t12) {
+    if (!t12) return 1;
+    const Flank Fk{p_old, p_cur, t12[0], t12[1], t12[2], t12[3], nullptr, nullptr};
+    return step_impl(model, s_in, s_out, tape, ctl, &Fk);
+}
+
+int g3m3_full_model_adjoint_inner(void* model, const float* tape, const float* sout_bar,
+                                  float* sin_bar, double* ctl_bar, const float* ctl,
+                                  const float* t12, float* p_old_bar, float* p_cur_bar) {
+    if (!t12) return 1;
+    const Flank Fk{nullptr, nullptr, t12[0], t12[1], t12[2], t12[3], p_old_bar, p_cur_bar};
+    return adjoint_impl(model, tape, sout_bar, sin_bar, ctl_bar, ctl, &Fk);
+}
+
+int g3m3_full_model_corer(void* model, const float* s, float* p_state) {
+    FullModel* M = (FullModel*)model;
+    if (!M || !s || !p_state || !M->shell) return 1;
+    FieldPtrs fine{}, coarse{};
+    for (int f = 0; f < NFLD; ++f) {
+        fine.p[f] = fld((float*)s, M->n, f);
+        coarse.p[f] = fld(p_state, M->n, f);
+    }
+    launch_corer(M->B, fine, coarse, NFLD, M->c_nskip, M->c_nbx1, M->c_nby1, M->c_nbz1, M->s);
+    return launched("corer") ? 0 : 2;
+}
+
+int g3m3_full_model_corer_adjoint(void* model, float* s_bar, float* p_bar) {
+    FullModel* M = (FullModel*)model;
+    if (!M || !s_bar || !p_bar || !M->shell) return 1;
+    FieldPtrs fb{}, cb{};
+    for (int f = 0; f < NFLD; ++f) {
+        fb.p[f] = fld(s_bar, M->n, f);
+        cb.p[f] = fld(p_bar, M->n, f);
+    }
+    const int k = M->c_nskip;
+    const int ni = corer_count(M->B.nx, k), nj = corer_count(M->B.ny, k),
+              nk = corer_count(M->B.nz, k);
+    if (ni > 0 && nj > 0 && nk > 0)
+        hipLaunchKernelGGL(k_adj_corer, dim3((ni * nj * nk + 255) / 256, NFLD), dim3(256), 0, M->s,
+                           M->B, fb, cb, k, M->c_nbx1, M->c_nby1, M->c_nbz1, ni, nj, nk);
+    return launched("corer adjoint") ? 0 : 2;
 }
 
 }  // extern "C"

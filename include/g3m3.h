@@ -832,6 +832,27 @@ int g3m3_full_model_step(void* model, const float* s_in, float* s_out, float* ta
  * Returns 0 on success. */
 int g3m3_full_model_adjoint(void* model, const float* tape, const float* sout_bar,
                             float* sin_bar, double* ctl_bar, const float* ctl);
+/* The model as the inner box of a nested pair (box < n_grids of the
+ * reference; code/sub_flanks.f90, code/sub_corer.f90): geom = this box's
+ * xmin, xmax, ymin, ymax, zmin, zmax, then its parent's (the parent has
+ * the same extents in cells and twice the spacing, or any whole multiple).
+ * Its steps then bound each stage by the flanks from the parent's states
+ * p_old and p_cur (34 arrays each, canonical order) with the time weights
+ * t12 = (t1, t2) at step 1's t_grid and (t1, t2) at the later stages'
+ * (t1 = (t_new - t_grid) / (t_new - t_old) of the parent,
+ * code/sub_flanks.f90:21-22); their adjoint adds the flanks' adjoints into
+ * p_old_bar and p_cur_bar.  g3m3_full_model_corer sets the parent's core
+ * from this box's state; its adjoint moves the parent core's adjoint into
+ * this box's (then zeroes it).  Returns 0 on success. */
+int g3m3_full_model_set_parent(void* model, const float* geom);
+int g3m3_full_model_step_inner(void* model, const float* s_in, float* s_out, float* tape,
+                               const float* ctl, const float* p_old, const float* p_cur,
+                               const float* t12);
+int g3m3_full_model_adjoint_inner(void* model, const float* tape, const float* sout_bar,
+                                  float* sin_bar, double* ctl_bar, const float* ctl,
+                                  const float* t12, float* p_old_bar, float* p_cur_bar);
+int g3m3_full_model_corer(void* model, const float* s, float* p_state);
+int g3m3_full_model_corer_adjoint(void* model, float* s_bar, float* p_bar);
 
 /* The adjoint of g3m3_sample_fields (code/cft_trilin_interp.f90:10-66 for
  * each point): fields_bar[q] += the trilinear weights of point p's cube

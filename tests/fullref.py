@@ -199,24 +199,36 @@ class FullRef(FieldRef):
                          + d * (uzp * (I(q, 1) - c) - uzm * (c - I(q, -1))))
         return _set(w, C3, val)
 
-    def step(self, st, ctl, codes=None, record=None):
+    def step(self, st, ctl, codes=None, record=None, flank=None):
         """One box-step.  codes / record: fcsmooth's limiter branches per
-        field (34 lists of three per-axis tensors), see FieldRef.fcsmooth."""
+        field (34 lists of three per-axis tensors), see FieldRef.fcsmooth.
+        flank: an inner box's step -- dict(old=, cur=) the parent's two
+        states (lists of >= 34 tensors), t12 the four time weights, geom
+        from flank_geom -- whose stages are bounded by the flanks (step 1's
+        set at (t1, t2) = t12[0:2], the Lapidus set and the output at
+        t12[2:4]) and whose stage-2 set keeps the state's shell."""
         sc = self.sc
         S = list(st[:34])
         eta = ctl[0]
         vals = self.inflow(ctl)
         delt = sc["delt"]
+        if flank is None:
+            b1 = b3 = lambda X: self.outer_all(X, vals)
+        else:
+            t = [self.S(x) for x in flank["t12"]]
+            b1 = lambda X: self.flanks(X, flank, t[0], t[1])
+            b3 = lambda X: self.flanks(X, flank, t[2], t[3])
         # stage 1: push from S with delt/2; the pressure keeps the stale WE
         ef = self.efield_set(S, eta)
         W1 = self._push_stage(S, S, ef, np.float32(0.5) * delt)
         W1[EP] = st[WE]
-        W1 = self.outer_all(W1, vals)
+        W1 = b1(W1)
         # stage 2
         ef = self.efield_set(W1, eta)
         S2 = self._push_stage(S, W1, ef, delt)
         S2[EP] = self.push_elec(W1[EP], S[EP], ef["ev"], self.delt)
-        S2 = self.outer_all(S2, vals)
+        if flank is None:
+            S2 = self.outer_all(S2, vals)
         # Lapidus into W1
         W3 = list(W1)
         for si in range(3):
@@ -230,7 +242,7 @@ class FullRef(FieldRef):
         W3[EP] = self._lap1(S2[EP], W1[EP], vt, self.S(sc["chipxyz"]))
         for c in range(3):
             W3[BX + c] = self._lap1(S2[BX + c], W1[BX + c], vt, self.chierg)
-        W3 = self.outer_all(W3, vals)
+        W3 = b3(W3)
         # flux correction
         out = []
         for f in range(34):
@@ -241,8 +253,62 @@ class FullRef(FieldRef):
             chi = self.diferg if f >= BX else self.difrho
             out.append(self.fcsmooth(S[f], W3[f], chi,
                                      codes=None if codes is None else codes[f], record=rec))
-        out = self.outer_all(out, vals)
+        out = b3(out)
         return out + [W3[EP]]
+
+    # --- nested boxes: code/sub_flanks.f90:5-303, code/sub_corer.f90:5-55 ---
+    def flanks(self, X, flank, t1, t2):
+        """The flanks on the 34 fields of X: per field, work = old t1 +
+        cur t2 of the parent, its trilinear interpolation I0 at each node,
+        and each face's two layers blended w = frac w + (1 - frac) I0
+        (frac 0.5 on the inner layer, 0 on the outer one), the panels in the
+        reference's order -- bottom, top, front, back, left, right."""
+        g = {k: v if k in "ijk" else v.to(self.dt) for k, v in flank["geom"].items()}
+        kb, jb, ib = g["k"][:, None, None], g["j"][None, :, None], g["i"][None, None, :]
+        dz, ddz = g["dz"][:, None, None], g["ddz"][:, None, None]
+        dy, ddy = g["dy"][None, :, None], g["ddy"][None, :, None]
+        dx, ddx = g["dx"][None, None, :], g["ddx"][None, None, :]
+        nz, ny, nx = X[0].shape
+        half, zero, one = self.S(0.5), self.S(0.0), self.S(1.0)
+        out = []
+        for f in range(34):
+            w = flank["old"][f] * t1 + flank["cur"][f] * t2
+            I0 = (w[kb, jb, ib] * ddx * ddy * ddz
+                  + w[kb + 1, jb, ib] * ddx * ddy * dz
+                  + w[kb, jb + 1, ib] * ddx * dy * ddz
+                  + w[kb + 1, jb + 1, ib] * ddx * dy * dz
+                  + w[kb, jb, ib + 1] * dx * ddy * ddz
+                  + w[kb + 1, jb, ib + 1] * dx * ddy * dz
+                  + w[kb, jb + 1, ib + 1] * dx * dy * ddz
+                  + w[kb + 1, jb + 1, ib + 1] * dx * dy * dz)
+            a = X[f].clone()
+            for axis, n in ((0, nz), (1, ny), (2, nx)):
+                for lay in ((1, 0), (n - 2, n - 1)):
+                    for idx, frac in zip(lay, (half, zero)):
+                        sel = [slice(None)] * 3
+                        sel[axis] = idx
+                        sel = tuple(sel)
+                        a[sel] = frac * a[sel] + (one - frac) * I0[sel]
+            out.append(a)
+        return out
+
+    def corer(self, fine, coarse, geom):
+        """coarse (the parent's state) with the core of its 34 fields
+        replaced by the average of fine's (6/12 the centre, 1/12 each of its
+        six neighbours) at every second fine node."""
+        K, J, I = geom["fine"]
+        KB, JB, IB = geom["coarse"]
+        out = list(coarse)
+        twelve = self.S(12.0)
+        for f in range(34):
+            q = fine[f]
+            v = (self.six * q[K, J, I] + q[K, J, I + 1] + q[K, J, I - 1] + q[K, J + 1, I]
+                 + q[K, J - 1, I] + q[K + 1, J, I] + q[K - 1, J, I]) / twelve
+            a = coarse[f].clone()
+            a[KB, JB, IB] = v
+            out[f] = a
+        return out
+
 
     def sample_full(self, st, plan, idx):
         return self.sample([st[i] for i in idx], plan)
@@ -254,6 +320,79 @@ class FullRef(FieldRef):
             if (t + 1) % every == 0:
                 out.append(self.sample_full(st, plan, idx))
         return torch.stack(out)
+
+
+class NestedRef:
+    """The restatement of a nested stack (innermost first, as
+    g3m3_amd.adjoint_nested.nested_setup builds it) on the full step's
+    schedule (nested_schedule): each box a FullRef, inner boxes bounded by
+    FullRef.flanks from their parent and restricted into it by
+    FullRef.corer."""
+
+    def __init__(self, setups, dtype=torch.float64):
+        from g3m3_amd.adjoint_nested import nested_schedule
+        self.setups = list(setups)
+        self.n = len(self.setups)
+        self.refs = [FullRef(s, dtype) for s in self.setups]
+        self.fg = [flank_geom(self.setups[b], self.setups[b + 1]) for b in range(self.n - 1)]
+        self.cg = [corer_geom(self.setups[b], self.setups[b + 1]) for b in range(self.n - 1)]
+        self.events = nested_schedule([s.scal["delt"] for s in self.setups])
+
+    def states(self, sts=None):
+        sts = [s.state for s in self.setups] if sts is None else sts
+        return [r.state(st) for r, st in zip(self.refs, sts)]
+
+    def step(self, b, st, ctl, p_old=None, p_cur=None, t12=None, codes=None, record=None):
+        fl = None if t12 is None else dict(old=p_old, cur=p_cur, t12=t12, geom=self.fg[b])
+        return self.refs[b].step(st, ctl, codes=codes, record=record, flank=fl)
+
+    def corer(self, b, child, parent):
+        return self.refs[b].corer(child, parent, self.cg[b])
+
+    def full_step(self, cur, ctl):
+        cur, old = list(cur), [None] * self.n
+        for ev in self.events:
+            if ev[0] == "step":
+                _, b, t12 = ev
+                out = self.step(b, cur[b], ctl, *((old[b + 1], cur[b + 1], t12)
+                                                  if t12 is not None else ()))
+                old[b], cur[b] = cur[b], out
+            else:
+                b = ev[1]
+                cur[b + 1] = self.corer(b, cur[b], cur[b + 1])
+        return cur
+
+
+def flank_geom(setup, parent):
+    """The flanks' per-axis parent indices and weights of an inner box
+    (code/sub_flanks.f90:24-30, 50-55), as torch tensors."""
+    from oracle.restate import _axis_weights
+    g = {}
+    for ax, key, n in (("x", "i", setup.nx), ("y", "j", setup.ny), ("z", "k", setup.nz)):
+        lo, d, dd = _axis_weights(setup.scal[ax + "min"], setup.scal[ax + "max"],
+                                  parent.scal[ax + "min"], parent.scal[ax + "max"], n)
+        g[key] = torch.from_numpy(lo)
+        g["d" + ax] = torch.from_numpy(np.asarray(d, np.float64))
+        g["dd" + ax] = torch.from_numpy(np.asarray(dd, np.float64))
+    return g
+
+
+def corer_geom(setup, parent):
+    """The corer's fine and parent index grids (code/sub_corer.f90:17-32)."""
+    F = np.float32
+    sx = (F(setup.scal["xmax"]) - F(setup.scal["xmin"])) / F(setup.nx - 1.)
+    d = {ax: (F(parent.scal[ax + "max"]) - F(parent.scal[ax + "min"])) / F(n - 1.)
+         for ax, n in (("x", setup.nx), ("y", setup.ny), ("z", setup.nz))}
+    nskip = int(d["x"] / sx)
+    fine, coarse = [], []
+    for ax, n in (("z", setup.nz), ("y", setup.ny), ("x", setup.nx)):
+        nb1 = int(F(1.) + (F(setup.scal[ax + "min"]) - F(parent.scal[ax + "min"])) / d[ax]
+                  + F(.5))
+        s = np.arange(1 + nskip, n - nskip + 1, nskip)
+        fine.append(s - 1)
+        coarse.append(nb1 + (s - 1) // nskip - 1)
+    mesh = lambda a: tuple(torch.from_numpy(x) for x in np.ix_(*a))
+    return {"fine": mesh(fine), "coarse": mesh(coarse)}
 
 
 def oracle_run(setup, ctl):
@@ -294,6 +433,51 @@ def oracle_run(setup, ctl):
     r = OracleRun(wl, "restate")
     r.a["wrkepres"][0] = setup.state[WE]
     return r
+
+
+def nested_oracle_run(setups, ctl):
+    """An OracleRun of a nested stack (innermost first, as
+    g3m3_amd.adjoint_nested.nested_setup builds it): n_grids boxes with
+    each setup's state, b0, wrkepres and grid, mbndry = 0, the controls of
+    the outermost box."""
+    from g3m3_amd.workload import FIELDS, Workload
+    from oracle.boxstep import OracleRun
+
+    one = oracle_run(setups[-1], ctl)
+    n = len(setups)
+    nx, ny, nz = setups[0].nx, setups[0].ny, setups[0].nz
+    fields = {k: np.ascontiguousarray(np.stack([st.state[i] for st in setups]))
+              for i, k in enumerate(FIELDS)}
+    for i, k in enumerate(("bx0", "by0", "bz0")):
+        fields[k] = np.ascontiguousarray(np.stack([st.b0[i] for st in setups]))
+    fields["rst"] = np.zeros((n, nz, ny, nx), np.float32)
+    grid = {"xspac": np.array([st.scal["rx"] for st in setups], np.float32)}
+    for ax in "xyz":
+        grid[f"grd_{ax}min"] = np.array([st.scal[ax + "min"] for st in setups], np.float32)
+        grid[f"grd_{ax}max"] = np.array([st.scal[ax + "max"] for st in setups], np.float32)
+    lists = {}
+    for nm in ("zero", "mid", "srf"):
+        lists["ij" + nm] = np.zeros((n, 3, 1), np.int32)
+        lists["num" + nm] = np.zeros(n, np.int32)
+        lists["parm_" + nm] = np.zeros((n, 7, 1), np.float32)
+        lists["m" + nm] = 1
+    wl = Workload(name="fullnest", nx=nx, ny=ny, nz=nz, ngrd=n, steps=1, divb_every=0,
+                  divb_tol=1e-6, grid=grid, scal=dict(one.wl.scal), fields=fields,
+                  lists=lists)
+    r = OracleRun(wl, "restate")
+    for b, st in enumerate(setups):
+        r.a["wrkepres"][b] = st.state[WE]
+    return r
+
+
+def nested_oracle_full_step(r, setups):
+    """One full step of the nested OracleRun with every box's time step its
+    setup's delt; returns the boxes' new 35-array states."""
+    from g3m3_amd.workload import FIELDS
+    r.t_stepnew[:] = [float(np.float32(st.scal["delt"])) for st in setups]
+    r.full_step()
+    return [np.stack([r.a[k][b] for k in FIELDS] + [r.a["wrkepres"][b]])
+            for b in range(len(setups))]
 
 
 def oracle_step(r, delt):

@@ -1,0 +1,446 @@
+"""The whole-box-step adjoint through nested boxes on the device
+(g3m3_full_model_set_parent, _step_inner, _adjoint_inner, _corer,
+_corer_adjoint; g3m3_amd.adjoint_nested).
+
+* The device's nested full step is the oracle's (OracleRun.full_step on the
+  same stack) bit for bit, every array of every box, over two full steps of
+  two and of three boxes.
+* One inner box-step's adjoint -- into its input state and into the
+  parent's two states the flanks read -- and the corer's adjoint equal the
+  float64 vector-Jacobian products of the restatement (tests/fullref.py
+  NestedRef, pinned to the oracle by tests/test_nested_adjoint_host.py)
+  linearised at the float32 values (fullref.Float32Values and the float32
+  step's limiter branches): relative error 1e-5, or 3x what torch's float32
+  autograd of the restatement reaches.
+* The gradient of a craft misfit over a checkpointed nested rollout, in the
+  controls and in every box's initial state, matches float64
+  vector-Jacobian products chained back along the device's own state
+  versions to the same bar (3e-5 for the controls, whose gradients are
+  cancelling sums over the two boxes' craft); float64 central differences of the
+  restatement in eta, the upstream density and pressure within 1e-3 plus the
+  difference between the float64 gradients linearised at float32 and at
+  float64 values (held below 1e-2: measured 8e-3 for eta); and it does not
+  depend on the checkpoint interval.
+"""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from g3m3_amd.adjoint import FULL_CTL_NAMES, FULL_SAMPLED, NSTATE, craft_positions
+from g3m3_amd.adjoint_nested import NestedFullModel, nested_rollout, nested_setup
+from fullref import (Float32Values, NestedRef, nested_oracle_full_step, nested_oracle_run)
+
+DIMS = dict(nx=20, ny=16, nz=12, spacing=1.0)
+GROUPS = {"q": range(0, 10), "h": range(10, 20), "o": range(20, 30), "epres": [30],
+          "b": range(31, 34), "wrkepres": [34]}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bits(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _clist(c, dtype):
+    return [c[i] for i in range(len(FULL_CTL_NAMES))] if torch.is_tensor(c) else \
+        [torch.tensor(float(x), dtype=dtype) for x in c]
+
+
+def _host(m, t):
+    return t.cpu().numpy().reshape(NSTATE, m.nz, m.ny, m.nx)
+
+
+@pytest.mark.parametrize("nbox", [2, 3])
+def test_nested_full_step_equals_oracle(nbox):
+    _say("nested full step vs oracle, boxes:", nbox)
+    ss = nested_setup(nbox, **DIMS)
+    m = NestedFullModel(ss)
+    r = nested_oracle_run(ss, ss[-1].ctl)
+    cur = m.new_states()
+    for step in range(2):
+        cur = m.full_step(cur, ss[-1].ctl)
+        want = nested_oracle_full_step(r, ss)
+        for b in range(nbox):
+            got = _host(m, cur[b])
+            bad = {i: int(np.count_nonzero(_bits(got[i]) != _bits(want[b][i])))
+                   for i in range(NSTATE)}
+            bad = {k: v for k, v in bad.items() if v}
+            assert not bad, f"full step {step + 1}, box {b + 1}: differing cells {bad}"
+    m.close()
+
+
+class _Vjp:
+    """Vector-Jacobian products of the restatement's box-steps and corer at
+    given float32 states: float64 linearised at the float32 values, or
+    torch's float32 autograd."""
+
+    def __init__(self, ss, ctl32):
+        self.ss = ss
+        self.ref = {torch.float64: NestedRef(ss, torch.float64),
+                    torch.float32: NestedRef(ss, torch.float32)}
+        self.ctl32 = np.asarray(ctl32, np.float32)
+
+    def _branches(self, b, s_in, po, pc, t12):
+        r32 = self.ref[torch.float32]
+        rec = []
+        kw = {} if t12 is None else dict(p_old=r32.refs[b + 1].state(po),
+                                         p_cur=r32.refs[b + 1].state(pc), t12=t12)
+        r32.step(b, r32.refs[b].state(s_in), _clist(self.ctl32, torch.float32), record=rec, **kw)
+        return rec
+
+    def step(self, dtype, b, s_in, po, pc, t12, lam):
+        """(ctl grad, [s_in grads], [p_old grads], [p_cur grads]) of
+        sum(step(...) * lam); host states (35, nz, ny, nx)."""
+        ref = self.ref[dtype]
+        cc = torch.tensor(self.ctl32.astype(np.float64), dtype=dtype, requires_grad=True)
+        xs = [a.requires_grad_(True) for a in ref.refs[b].state(s_in)]
+        xo = xc = []
+        kw = {}
+        if t12 is not None:
+            xo = [a.requires_grad_(True) for a in ref.refs[b + 1].state(po)]
+            xc = [a.requires_grad_(True) for a in ref.refs[b + 1].state(pc)]
+            kw = dict(p_old=xo, p_cur=xc, t12=t12)
+        codes = self._branches(b, s_in, po, pc, t12)
+        if dtype == torch.float64:
+            with Float32Values():
+                y = ref.step(b, xs, _clist(cc, dtype), codes=codes, **kw)
+        else:
+            y = ref.step(b, xs, _clist(cc, dtype), codes=codes, **kw)
+        ins = [cc, *xs, *xo, *xc]
+        g = torch.autograd.grad(sum((a * l).sum() for a, l in zip(y, lam)), ins,
+                                allow_unused=True)
+        z = lambda a, x: torch.zeros_like(x) if a is None else a
+        g = [z(a, x) for a, x in zip(g, ins)]
+        n = len(xs)
+        return g[0], g[1:1 + n], g[1 + n:1 + n + len(xo)], g[1 + n + len(xo):]
+
+    def corer(self, dtype, b, child, parent, lam):
+        ref = self.ref[dtype]
+        xc = [a.requires_grad_(True) for a in ref.refs[b].state(child)]
+        xp = [a.requires_grad_(True) for a in ref.refs[b + 1].state(parent)]
+        y = ref.corer(b, xc, xp)
+        g = torch.autograd.grad(sum((a * l).sum() for a, l in zip(y, lam)), xc + xp,
+                                allow_unused=True)
+        g = [torch.zeros_like(x) if a is None else a for a, x in zip(g, xc + xp)]
+        return g[:len(xc)], g[len(xc):]
+
+
+def _stack(gs):
+    return np.stack([g.detach().double().numpy().reshape(-1) for g in gs])
+
+
+def _group_errs(got, want, label):
+    return {f"{label}{k}": _rel(got[list(v)], want[list(v)]) for k, v in GROUPS.items()
+            if np.linalg.norm(want[list(v)]) > 0 or np.linalg.norm(got[list(v)]) > 0}
+
+
+def test_inner_step_and_corer_adjoints_equal_float64_vjp():
+    _say("inner box-step and corer adjoints vs float64 vjps")
+    ss = nested_setup(2, **DIMS)
+    m = NestedFullModel(ss)
+    ctl = ss[-1].ctl.copy()
+    cur = m.full_step(m.new_states(), ctl)
+    rec = []
+    m.full_step(cur, ctl, rec)
+    # the first inner step: both of its flank sets weigh the parent's two states
+    _, b, s_in, s_out, po, pc, t12, tape = [r for r in rec if r[0] == "step" and r[1] == 0][0]
+    assert min(t12) > 0.0
+    g = torch.Generator().manual_seed(21)
+    w = torch.randn((NSTATE, m.V), generator=g)
+    sb = torch.empty_like(s_in)
+    pob, pcb = torch.zeros_like(po), torch.zeros_like(pc)
+    cb = torch.zeros(len(FULL_CTL_NAMES), dtype=torch.float64, device="cuda")
+    m.boxes[0].adjoint_inner(tape, w.cuda(), sb, cb, ctl, t12, pob, pcb)
+    torch.cuda.synchronize()
+    V = _Vjp(ss, ctl)
+    hs = [_host(m, t) for t in (s_in, po, pc)]
+    errs, errs32 = {}, {}
+    res = {}
+    for dt in (torch.float64, torch.float32):
+        lam = [x.to(dt) for x in w.reshape(NSTATE, m.nz, m.ny, m.nx)]
+        res[dt] = V.step(dt, 0, *hs, t12, lam)
+    gc, gx, go, gp = res[torch.float64]
+    gc32, gx32, go32, gp32 = res[torch.float32]
+    for label, dev, want, w32 in (("in.", sb, gx, gx32), ("p_old.", pob, go, go32),
+                                  ("p_cur.", pcb, gp, gp32)):
+        got = dev.cpu().double().numpy().reshape(NSTATE, -1)
+        errs.update(_group_errs(got, _stack(want), label))
+        errs32.update(_group_errs(_stack(w32), _stack(want), label))
+    e_eta = abs(float(cb[0]) - float(gc[0])) / abs(float(gc[0]))
+    _say("rel err by group:", {k: f"{v:.1e}" for k, v in errs.items()})
+    _say("torch float32 autograd:", {k: f"{v:.1e}" for k, v in errs32.items()})
+    _say(f"eta: {e_eta:.1e}; other controls (no inflow in an inner box):",
+         [float(x) for x in cb[1:].cpu()])
+    bad = {k: v for k, v in errs.items() if v > max(1e-5, 3.0 * errs32.get(k, 0.0))}
+    assert not bad, (bad, errs32)
+    assert e_eta < 1e-5
+    assert not torch.any(cb[1:] != 0)
+
+    # the corer's adjoint: the child's state and the parent's it overwrites
+    child, parent = s_out, pc
+    wp = torch.randn((NSTATE, m.V), generator=g)
+    cbar, pbar = torch.zeros_like(child), wp.cuda()
+    m.boxes[0].corer_adjoint(cbar, pbar)
+    torch.cuda.synchronize()
+    lam = [x.double() for x in wp.reshape(NSTATE, m.nz, m.ny, m.nx)]
+    gch, gpa = V.corer(torch.float64, 0, _host(m, child), _host(m, parent), lam)
+    ec = _rel(cbar.cpu().double().numpy().reshape(NSTATE, -1), _stack(gch))
+    ep = _rel(pbar.cpu().double().numpy().reshape(NSTATE, -1), _stack(gpa))
+    _say(f"corer adjoint: child {ec:.1e}, parent {ep:.1e}")
+    assert ec < 1e-6 and ep < 1e-7
+    m.close()
+
+
+def _misfit_case(nbox=2, steps=8, every=4):
+    ss = nested_setup(nbox, **DIMS)
+    m = NestedFullModel(ss)
+    outer, inner = ss[-1], ss[0]
+    # four craft inside the innermost box, four in the outer box between
+    # 1.5 and 4 of its cells from the inflow face (upstream of the inner box)
+    p_in = craft_positions(inner, 4, seed=31, margin=2.0)
+    p_out = craft_positions(outer, 4, seed=32, margin=2.0)
+    h = float(outer.grid["xspac"][0])
+    x0 = float(outer.grid["grid_minvals"][0][0])
+    p_out[:, 0] = (x0 + h * np.random.default_rng(33).uniform(1.5, 4.0, size=4)).astype(np.float32)
+    pos_h = np.ascontiguousarray(np.concatenate([p_in, p_out]))
+    pos = torch.from_numpy(pos_h).cuda()
+    ctl_true = torch.tensor(outer.ctl.astype(np.float64))
+    ctl_true[0] = 5e-4
+    ctl_true[3] = -0.20
+    ctl_true[4] *= 1.05
+    ctl_true[5] *= 0.97
+    with torch.no_grad():
+        target = nested_rollout(ctl_true, m.new_states(), m, steps, every, pos).clone()
+    g = torch.Generator().manual_seed(34)
+    rms = target.pow(2).mean(dim=(0, 1)).sqrt()
+    target += 0.05 * rms * torch.randn(target.shape, generator=g).cuda()
+    wt = (1.0 / rms).cuda()
+    return ss, m, pos_h, pos, target, wt
+
+
+def _device_grad(m, pos, target, wt, ctl0, steps, every, ckpt):
+    ctl = ctl0.clone().requires_grad_(True)
+    s0 = [s.requires_grad_(True) for s in m.new_states()]
+    out = nested_rollout(ctl, s0, m, steps, every, pos, ckpt)
+    L = (((out - target) * wt) ** 2).sum()
+    L.backward()
+    torch.cuda.synchronize()
+    return (float(L.detach()), ctl.grad.numpy().copy(),
+            [s.grad.cpu().numpy().copy() for s in s0])
+
+
+def test_nested_rollout_gradient():
+    _say("nested rollout gradient: chained float64 vjps and central differences")
+    steps, every = 8, 4
+    ss, m, pos_h, pos, target, wt = _misfit_case(steps=steps, every=every)
+    n = m.n
+    ctl0 = torch.tensor(ss[-1].ctl.astype(np.float64))
+    L, g_ctl, g_s = _device_grad(m, pos, target, wt, ctl0, steps, every, 3)
+    c32 = ctl0.numpy().astype(np.float32)
+    # the device's own state versions, full step by full step
+    cur = m.new_states()
+    first = list(cur)
+    steps_rec, samples = [], []
+    gp = torch.empty((pos.shape[0], 4), dtype=torch.int32, device="cuda")
+    for t in range(steps):
+        rec = []
+        cur = m.full_step(cur, c32, rec)
+        steps_rec.append((rec, cur))
+        if (t + 1) % every == 0:
+            o = torch.empty((pos.shape[0], len(FULL_SAMPLED)), device="cuda")
+            m.sample(cur, pos, o, gp)
+            samples.append(o)
+    boxes = gp[:, 3].cpu().numpy()      # gridpt(4, p): the box holding craft p
+    assert set(boxes.tolist()) == {1, 2}, boxes
+    res = 2.0 * (torch.stack(samples).cpu().double() - target.cpu().double()) * \
+        wt.cpu().double() ** 2
+    V = _Vjp(ss, c32)
+    shp = (m.nz, m.ny, m.nx)
+
+    def chain(dtype):
+        lam, gc = {}, torch.zeros(len(FULL_CTL_NAMES), dtype=torch.float64)
+
+        def get(t):
+            if id(t) not in lam:
+                lam[id(t)] = [torch.zeros(shp, dtype=dtype) for _ in range(NSTATE)]
+            return lam[id(t)]
+
+        def add(t, gs):
+            cur_l = get(t)
+            for i, g in enumerate(gs):
+                cur_l[i] = cur_l[i] + g
+
+        for t in reversed(range(steps)):
+            rec, finals = steps_rec[t]
+            if (t + 1) % every == 0:
+                for b in range(n):
+                    sel = np.nonzero(boxes == b + 1)[0]
+                    if not len(sel):
+                        continue
+                    r = V.ref[dtype].refs[b]
+                    xs = [a.requires_grad_(True) for a in r.state(_host(m, finals[b]))]
+                    plan = r.sample_plan(pos_h[sel])
+                    sm = r.sample_full(xs, plan, FULL_SAMPLED)
+                    gs = torch.autograd.grad((sm * res[(t + 1) // every - 1][sel].to(dtype)).sum(),
+                                             [xs[i] for i in FULL_SAMPLED])
+                    full = [torch.zeros(shp, dtype=dtype) for _ in range(NSTATE)]
+                    for i, g in zip(FULL_SAMPLED, gs):
+                        full[i] = g
+                    add(finals[b], full)
+            for r_ in reversed(rec):
+                if r_[0] == "step":
+                    _, b, s_in, s_out, po, pc, t12, _tape = r_
+                    lo = get(s_out)
+                    hs = [_host(m, s_in)] + ([_host(m, po), _host(m, pc)] if t12 else [None, None])
+                    g1, gx, go, gp_ = V.step(dtype, b, hs[0], hs[1], hs[2], t12, lo)
+                    gc += g1.double()
+                    del lam[id(s_out)]
+                    add(s_in, gx)
+                    if t12 is not None:
+                        add(po, list(go))
+                        add(pc, list(gp_))
+                else:
+                    _, b, child, p_in, p_out = r_
+                    lo = get(p_out)
+                    gch, gpa = V.corer(dtype, b, _host(m, child), _host(m, p_in), lo)
+                    del lam[id(p_out)]
+                    add(child, gch)
+                    add(p_in, gpa)
+        return gc.numpy(), [_stack(get(first[b])) for b in range(n)]
+
+    gc64, lam64 = chain(torch.float64)
+    gc32, lam32 = chain(torch.float32)
+
+    def errs(gs, gc):
+        es = {}
+        for b in range(n):
+            es.update({f"box{b + 1}.{k}": v for k, v in
+                       _group_errs(gs[b].reshape(NSTATE, -1), lam64[b], "").items()})
+        ec = {FULL_CTL_NAMES[i]: abs(gc[i] - gc64[i]) / abs(gc64[i])
+              for i in range(len(FULL_CTL_NAMES)) if gc64[i] != 0.0}
+        return es, ec
+
+    es, ec = errs(g_s, g_ctl)
+    es32, ec32 = errs(lam32, gc32)
+    _say("device vs chained float64 vjps: state", {k: f"{v:.1e}" for k, v in es.items()})
+    _say("  ctl", {k: f"{v:.1e}" for k, v in ec.items()})
+    _say("float32 autograd chain: state", {k: f"{v:.1e}" for k, v in es32.items()})
+    _say("  ctl", {k: f"{v:.1e}" for k, v in ec32.items()})
+    _say("ctl gradients: device", g_ctl.tolist(), "float64 chain", gc64.tolist(),
+         "float32 chain", gc32.tolist())
+    # Bar: 1e-5, or 3x the float32 autograd chain's error, per box and
+    # state group; for the controls 3e-5 or 3x the float32 chain's.  The
+    # inflow controls' gradients are sums over both boxes' craft whose terms
+    # cancel: a float32 chain leaves a residual set by the terms, not the sum
+    # (torch's float32 chain reaches 6e-5 on sbz_wind here, and with the
+    # inner box's craft left out the device is within 1.3e-6 on srho and spx,
+    # whose errors reach 1.2e-5 with them).
+    bad = {k: v for k, v in es.items() if v > max(1e-5, 3.0 * es32[k])}
+    bad.update({k: v for k, v in ec.items() if v > max(3e-5, 3.0 * ec32[k])})
+    assert not bad, bad
+
+    # float64 central differences of the restatement, and its float64
+    # autograd gradient, in eta and the upstream density
+    ref = NestedRef(ss, torch.float64)
+    tgt, wt64 = target.cpu().double(), wt.cpu().double()
+    plans = [(np.nonzero(boxes == b + 1)[0], ref.refs[b].sample_plan(pos_h[boxes == b + 1]))
+             for b in range(n)]
+
+    def loss(c):
+        st = ref.states()
+        out = []
+        for t in range(steps):
+            st = ref.full_step(st, _clist(c, torch.float64))
+            if (t + 1) % every == 0:
+                rows = torch.zeros((pos_h.shape[0], len(FULL_SAMPLED)), dtype=torch.float64)
+                for b, (sel, plan) in enumerate(plans):
+                    if len(sel):
+                        rows = rows.index_put((torch.from_numpy(sel),),
+                                              ref.refs[b].sample_full(st[b], plan, FULL_SAMPLED))
+                out.append(rows)
+        return (((torch.stack(out) - tgt) * wt64) ** 2).sum()
+
+    c64 = ctl0.clone().requires_grad_(True)
+    (g64,) = torch.autograd.grad(loss(c64), [c64])
+    g64 = g64.numpy()
+    with torch.no_grad():
+        for i in (0, 4, 8):
+            h = 1e-6 * max(abs(float(ctl0[i])), 0.1)
+            cp, cm = ctl0.clone(), ctl0.clone()
+            cp[i] += h
+            cm[i] -= h
+            fd = (float(loss(cp)) - float(loss(cm))) / (2 * h)
+            model = abs(gc64[i] - g64[i])
+            _say(f"control {FULL_CTL_NAMES[i]}: device {g_ctl[i]:.9g} fd {fd:.9g} "
+                 f"float64 autograd {g64[i]:.9g} float32-valued float64 {gc64[i]:.9g} "
+                 f"| device-fd {abs(g_ctl[i] - fd) / abs(fd):.1e} models {model / abs(fd):.1e}")
+            assert abs(g64[i] - fd) <= 2e-4 * abs(fd), (FULL_CTL_NAMES[i], g64[i], fd)
+            # (the two models linearise at trajectories that differ by
+            # float32 rounding over 24 box-steps, eta entering every one;
+            # measured up to 8e-3 of eta's gradient)
+            assert model <= 1e-2 * abs(fd), (FULL_CTL_NAMES[i], gc64[i], g64[i])
+            assert abs(g_ctl[i] - fd) <= 1e-3 * abs(fd) + 2.0 * model, \
+                (FULL_CTL_NAMES[i], g_ctl[i], fd)
+    m.close()
+
+
+def test_nested_gradient_independent_of_checkpoint_interval():
+    _say("nested checkpoint intervals")
+    steps, every = 8, 4
+    ss, m, pos_h, pos, target, wt = _misfit_case(steps=steps, every=every)
+    ctl0 = torch.tensor(ss[-1].ctl.astype(np.float64))
+    res = [_device_grad(m, pos, target, wt, ctl0, steps, every, k) for k in (1, 3, 8)]
+    L0, c0, g0 = res[0]
+    for L, c, g in res[1:]:
+        assert L == L0
+        assert np.allclose(c, c0, rtol=1e-5, atol=0)
+        for a, b in zip(g, g0):
+            assert _rel(a, b) < 1e-5
+    m.close()
+
+
+def test_nested_adjoint_cost_ratio():
+    _say("nested adjoint cost")
+    steps, every = 8, 4
+    ss = nested_setup(3, nx=41, ny=41, nz=21, spacing=0.5)
+    m = NestedFullModel(ss)
+    pos = torch.from_numpy(np.concatenate([craft_positions(s, 2, seed=40 + i)
+                                           for i, s in enumerate(ss)])).cuda()
+    ctl0 = torch.tensor(ss[-1].ctl.astype(np.float64))
+
+    def grad():
+        ctl = ctl0.clone().requires_grad_(True)
+        out = nested_rollout(ctl, m.new_states(), m, steps, every, pos, 4)
+        out.pow(2).sum().backward()
+        torch.cuda.synchronize()
+        return ctl.grad
+
+    grad()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        nested_rollout(ctl0, m.new_states(), m, steps, every, pos)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = grad()
+    t2 = time.perf_counter()
+    fwd, both = t1 - t0, t2 - t1
+    _say(f"{m.box_steps} box-steps per full step; forward {fwd * 1e3:.2f} ms, "
+         f"forward+backward {both * 1e3:.2f} ms, backward/forward {(both - fwd) / fwd:.2f}")
+    assert torch.all(torch.isfinite(g))
+    m.close()
