@@ -45,7 +45,7 @@ EXTRA_ARRAYS = {
 SAME_AS = {"set_speed_agrd_noalloc": "set_speed_agrd", "bndry_flanks_v": None,
            "bndry_corer_v": None}
 # not bound here: handles, descriptors, host-side C strings and the run API
-SKIP_PREFIX = ("run_", "field_model_", "full_model_", "prof_", "fluid_", "init_plasma",
+SKIP_PREFIX = ("run_", "batch_", "field_model_", "full_model_", "prof_", "fluid_", "init_plasma",
                "graphing_cuts", "torus_inject", "bndry_flanks_v", "bndry_corer_v", "set_stream", "last_error_msg")
 
 # dummies the reference leaves implicitly typed (INTEGER by their initial
