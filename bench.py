@@ -326,6 +326,76 @@ def full_adjoint_case(steps: int = 50, every: int = 10, ckpt: int = 10, ncraft: 
             "method": "hand-written adjoint kernels, recompute from checkpoints"}
 
 
+def nested_adjoint_case(nbox: int = 3, steps: int = 10, every: int = 5, ckpt: int = 5,
+                        ncraft: int = 8):
+    """The checkpointed adjoint of the whole box-step through ``nbox`` nested
+    121x121x61 boxes (g3m3_amd.adjoint_nested; inner boxes bounded by their
+    parent's flanks and restricted into its core, on the subcycling
+    schedule): a misfit of density, momentum, electron pressure and field at
+    ``ncraft`` craft, half inside the innermost box and half near the outer
+    box's inflow face, over ``steps`` full steps; its gradient in the nine
+    controls and every box's initial state, and the wall time of the forward
+    rollout and of the backward pass."""
+    import torch
+    from g3m3_amd import adjoint as A
+    from g3m3_amd import adjoint_nested as N
+    ss = N.nested_setup(nbox, nx=121, ny=121, nz=61, spacing=1.0)
+    m = N.NestedFullModel(ss)
+    outer = ss[-1]
+    half = ncraft // 2
+    p_in = A.craft_positions(ss[0], half, seed=31, margin=3.0)
+    p_out = A.craft_positions(outer, ncraft - half, seed=32, margin=3.0)
+    h = float(outer.grid["xspac"][0])
+    x0 = float(outer.grid["grid_minvals"][0][0])
+    p_out[:, 0] = (x0 + h * np.random.default_rng(33).uniform(2.0, 12.0, ncraft - half)
+                   ).astype(np.float32)
+    pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([p_in, p_out]))).cuda()
+    ctl0 = torch.tensor(outer.ctl.astype(np.float64))
+    ctl_true = ctl0.clone()
+    ctl_true[0], ctl_true[3] = 5e-4, -0.20
+    ctl_true[4] *= 1.05
+    ctl_true[5] *= 0.97
+    with torch.no_grad():
+        target = N.nested_rollout(ctl_true, m.new_states(), m, steps, every, pos).clone()
+    gen = torch.Generator().manual_seed(34)
+    rms = target.pow(2).mean(dim=(0, 1)).sqrt()
+    target += 0.05 * rms * torch.randn(target.shape, generator=gen).cuda()
+    wt = 1.0 / rms
+
+    def grad():
+        c = ctl0.clone().requires_grad_(True)
+        s0 = [s.requires_grad_(True) for s in m.new_states()]
+        out = N.nested_rollout(c, s0, m, steps, every, pos, ckpt)
+        L = (((out - target) * wt) ** 2).sum()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        return float(L.detach()), c.grad.numpy(), time.perf_counter() - t0
+
+    grad()                                   # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        N.nested_rollout(ctl0, m.new_states(), m, steps, every, pos)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    L, g, bwd = grad()
+    nbs = m.box_steps
+    m.close()
+    return {"config": f"{nbox} nested 121x121x61 boxes, spacing "
+                      f"{[float(s.scal['rx']) for s in ss]}, every field evolving, {steps} full "
+                      f"steps ({nbs} box-steps each), {ncraft} craft sampling 8 quantities "
+                      f"every {every} full steps, checkpoints every {ckpt} full steps",
+            "controls": dict(zip(A.FULL_CTL_NAMES, ctl0.tolist())),
+            "target_controls": dict(zip(A.FULL_CTL_NAMES, ctl_true.tolist())),
+            "loss": L, "gradient": dict(zip(A.FULL_CTL_NAMES, g.tolist())),
+            "forward_ms": round(fwd * 1e3, 2), "backward_ms": round(bwd * 1e3, 2),
+            "backward_over_forward": round(bwd / fwd, 2),
+            "method": "hand-written adjoint kernels (box-steps, flanks, corer, sampling), "
+                      "recompute from checkpoints"}
+
+
 def precision_case(fast_mode: int):
     """divB on a 241x241x121 dipole + 1% noise field to rel-residual 1e-8 with
     FP32 vectors, FP64 dots and FP64 residual refinement (mode 3); the
@@ -474,7 +544,7 @@ def main(argv=None):
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint legs (checkpointed adjoints of craft misfits on "
                          "one 121x121x61 box: the field update, and the whole box-step)")
-    ap.add_argument("--only-adjoint", choices=("field", "full"),
+    ap.add_argument("--only-adjoint", choices=("field", "full", "nested"),
                     help="run only that adjoint leg and print its JSON (for a kernel trace "
                          "of the adjoint alone)")
     ap.add_argument("--cpu-threads", type=int,
@@ -486,7 +556,8 @@ def main(argv=None):
         return 2
 
     if a.only_adjoint:
-        leg = adjoint_case() if a.only_adjoint == "field" else full_adjoint_case()
+        leg = {"field": adjoint_case, "full": full_adjoint_case,
+               "nested": nested_adjoint_case}[a.only_adjoint]()
         print(json.dumps({"adjoint": {a.only_adjoint: leg}}), flush=True)
         return 0
 
@@ -584,7 +655,7 @@ def main(argv=None):
     grad = None if a.no_gradient else gradient_case()
     adj = None if a.no_adjoint else adjoint_case()
     if adj is not None:
-        adj = {"field": adj, "full": full_adjoint_case()}
+        adj = {"field": adj, "full": full_adjoint_case(), "nested": nested_adjoint_case()}
     cpu = None if a.no_cpu else cpu_baseline(a.workload, a.cpu_threads)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,

@@ -32,7 +32,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/g3m3.h"
@@ -123,6 +125,16 @@ struct FullModel {
     ShellPt* shell = nullptr;
     int nshell = 0;
     int c_nskip = 0, c_nbx1 = 0, c_nby1 = 0, c_nbz1 = 0;
+    // the flanks' adjoint as a gather: the parent cells the shell's
+    // trilinear corners touch (fc_cell), and for each its (shell point,
+    // weight) pairs fc_p/fc_w[fc_start[i] .. fc_start[i+1]); fc_I0b holds
+    // each shell point's interpolation adjoint per field
+    int* fc_cell = nullptr;
+    int* fc_start = nullptr;
+    int* fc_p = nullptr;
+    float* fc_w = nullptr;
+    float* fc_I0b = nullptr;
+    int nfc = 0;
 };
 
 // the flanks of one inner-box step: the parent's states (34 fields each,
@@ -206,16 +218,17 @@ Inflow inflow(const g3m3_full_model_desc& d, const float* c) {
     return I;
 }
 
-// The flanks' adjoint (code/sub_flanks.f90, transposed): per shell cell
-// and field, w = frac w + (1 - frac) I0 for each panel containing the cell
-// (bottom, top, front, back, left, right), I0 = the trilinear weights times
-// (old t1 + cur t2) at the eight parent corners.  The cell's adjoint keeps
-// the product of the fracs; the parent corners receive t1 / t2 times their
-// weight times the adjoint of I0 (atomics: neighbouring shell cells share
-// corners).  blockIdx.y = field.
-__global__ void k_adj_flanks(const ShellPt* __restrict__ tab, int npts, FieldPtrs xb,
-                             FieldPtrs ob, FieldPtrs cb, float t1, float t2) {
-    const int t = gtid(), f = blockIdx.y;
+// The flanks' adjoint (code/sub_flanks.f90, transposed), first pass: per
+// shell cell, for each field w = frac w + (1 - frac) I0 over the panels
+// containing the cell (bottom, top, front, back, left, right) makes the
+// cell's adjoint the product of the fracs times its own, and I0's adjoint
+// the cell's adjoint times sum_j (1 - frac_j) prod_{l > j} frac_l, stored
+// point-major: I0b[t * NFLD + f].
+constexpr int kFcGroup = 6;   // fields per thread of the gather pass
+
+__global__ void k_adj_flanks_cells(const ShellPt* __restrict__ tab, int npts, FieldPtrs xb,
+                                   float* __restrict__ I0b) {
+    const int t = gtid();
     if (t >= npts) return;
     const ShellPt p = tab[t];
     float pw = 1.0f, pI = 0.0f;
@@ -226,44 +239,110 @@ __global__ void k_adj_flanks(const ShellPt* __restrict__ tab, int npts, FieldPtr
             pI = frac * pI + (1.0f - frac);
             pw *= frac;
         }
-    const float wb = xb.p[f][p.c];
-    xb.p[f][p.c] = pw * wb;
-    const float I0b = pI * wb;
-    const float dx = p.dx, dy = p.dy, dz = p.dz;
-    const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
-    const int X = 1, Y = p.sy, Z = p.sz, c0 = p.cb;
-    const int cn[8] = {c0, c0 + Z, c0 + Y, c0 + Y + Z, c0 + X, c0 + X + Z, c0 + X + Y,
-                       c0 + X + Y + Z};
-    const float w8[8] = {ddx * ddy * ddz, ddx * ddy * dz, ddx * dy * ddz, ddx * dy * dz,
-                         dx * ddy * ddz,  dx * ddy * dz,  dx * dy * ddz,  dx * dy * dz};
+    float wb[NFLD];
 #pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        atomicAdd(ob.p[f] + cn[n], t1 * w8[n] * I0b);
-        atomicAdd(cb.p[f] + cn[n], t2 * w8[n] * I0b);
+    for (int f = 0; f < NFLD; ++f) wb[f] = xb.p[f][p.c];
+    float* __restrict__ dst = I0b + (size_t)t * NFLD;
+#pragma unroll
+    for (int f = 0; f < NFLD; f += 2)
+        *reinterpret_cast<float2*>(dst + f) = make_float2(pI * wb[f], pI * wb[f + 1]);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) xb.p[f][p.c] = pw * wb[f];
+}
+
+// Second pass, per parent cell the shell's corners touch (ascending) and
+// group of kFcGroup fields: the adjoint of work = old t1 + cur t2 at that
+// cell is the sum over the shell points that interpolate from it of their
+// trilinear weight times I0's adjoint; old's adjoint gains t1 times it,
+// cur's t2 times it.  blockIdx.y = field group.
+__global__ void __launch_bounds__(64) k_adj_flanks_gather(
+    const int* __restrict__ cell, const int* __restrict__ start, const int* __restrict__ pt,
+    const float* __restrict__ wt, int ncell, const float* __restrict__ I0b, FieldPtrs ob,
+    FieldPtrs cb, float t1, float t2) {
+    const int i = gtid(), f0 = blockIdx.y * kFcGroup;
+    if (i >= ncell) return;
+    float acc[kFcGroup];
+#pragma unroll
+    for (int g = 0; g < kFcGroup; ++g) acc[g] = 0.0f;
+    const int e1 = start[i + 1];
+    for (int e = start[i]; e < e1; ++e) {
+        const float w = wt[e];
+        const float* __restrict__ src = I0b + (size_t)pt[e] * NFLD + f0;
+#pragma unroll
+        for (int g = 0; g < kFcGroup; g += 2) {
+            if (f0 + g < NFLD) {
+                const float2 v = *reinterpret_cast<const float2*>(src + g);
+                acc[g] += w * v.x;
+                acc[g + 1] += w * v.y;
+            }
+        }
+    }
+    const int c = cell[i];
+#pragma unroll
+    for (int g = 0; g < kFcGroup; ++g) {
+        const int f = f0 + g;
+        if (f < NFLD) {
+            ob.p[f][c] += t1 * acc[g];
+            cb.p[f][c] += t2 * acc[g];
+        }
     }
 }
 
-// The corer's adjoint (code/sub_corer.f90:34-53, transposed): each parent
-// core cell's adjoint goes to the seven fine cells of its average (6/12 to
-// the centre, 1/12 to each neighbour), then the parent cell's adjoint is
-// zeroed (the corer overwrote it).  blockIdx.y = field.
-__global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, int nskip, int nbx1, int nby1,
-                            int nbz1, int ni, int nj, int nk) {
-    const int f = blockIdx.y, t = gtid();
-    if (t >= ni * nj * nk) return;
-    const int a = t % ni, bq = (t / ni) % nj, cq = t / (ni * nj);
-    const int is = 1 + nskip + a * nskip, js = 1 + nskip + bq * nskip, ks = 1 + nskip + cq * nskip;
-    const int ib = nbx1 + (is - 1) / nskip, jb = nby1 + (js - 1) / nskip,
-              kb = nbz1 + (ks - 1) / nskip;
-    const int cc = B.IX(ib - 1, jb - 1, kb - 1);
-    const float v = cb.p[f][cc];
-    cb.p[f][cc] = 0.0f;
-    const int c = B.IX(is - 1, js - 1, ks - 1), X = 1, Y = B.nx, Z = B.nx * B.ny;
-    float* r = fb.p[f];
-    atomicAdd(r + c, 6.0f * v / 12.0f);
-    const int nb[6] = {c + X, c - X, c + Y, c - Y, c + Z, c - Z};
+// The corer's adjoint (code/sub_corer.f90:34-53, transposed), as a gather
+// over this box's cells: the parent core cell fed by a fine centre (0-based
+// fine index a nskip, 1 <= a <= count on each axis) hands 6/12 of its
+// adjoint to the centre and 1/12 to each of the centre's six neighbours, so
+// a fine cell collects 6/12 of its own centre's adjoint (if it is one) and
+// 1/12 of each centre one step away along one axis.  The parent core's
+// adjoint is zeroed afterwards (k_zero_core: the corer overwrote it).
+// blockIdx.y = field.
+struct CorerGeom {
+    int nskip, nb[3], cnt[3];
+};
+__device__ inline bool corer_centre(const CorerGeom& G, int ax, int q, int& a) {
+    if (q < G.nskip || q % G.nskip) return false;
+    a = q / G.nskip;
+    return a <= G.cnt[ax];
+}
+__global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, CorerGeom G) {
+    int x[3];
+    if (!cell_of(B, gtid(), x[0], x[1], x[2])) return;
+    const int f = blockIdx.y;
+    const float* __restrict__ src = cb.p[f];
+    int a[3];
+    bool al[3];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) atomicAdd(r + nb[q], v / 12.0f);
+    for (int d = 0; d < 3; ++d) al[d] = corer_centre(G, d, x[d], a[d]);
+    // the parent cell of the centre with fine indices (a0, a1, a2) nskip:
+    // nb1 + a (1-based) on each axis
+    auto parent = [&](int i, int j, int k) {
+        return B.IX(G.nb[0] + i - 1, G.nb[1] + j - 1, G.nb[2] + k - 1);
+    };
+    float acc = 0.0f;
+    if (al[0] && al[1] && al[2]) acc += 6.0f * src[parent(a[0], a[1], a[2])] / 12.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int e = (d + 1) % 3, g = (d + 2) % 3;
+        if (!(al[e] && al[g])) continue;
+#pragma unroll
+        for (int s = -1; s <= 1; s += 2) {
+            int c;
+            if (!corer_centre(G, d, x[d] + s, c)) continue;
+            int idx[3];
+            idx[d] = c;
+            idx[e] = a[e];
+            idx[g] = a[g];
+            acc += src[parent(idx[0], idx[1], idx[2])] / 12.0f;
+        }
+    }
+    if (acc != 0.0f) fb.p[f][B.IX(x[0], x[1], x[2])] += acc;
+}
+__global__ void k_zero_core(Box B, FieldPtrs cb, CorerGeom G) {
+    const int t = gtid(), f = blockIdx.y;
+    const int ni = G.cnt[0], nj = G.cnt[1], nk = G.cnt[2];
+    if (t >= ni * nj * nk) return;
+    const int i = 1 + t % ni, j = 1 + (t / ni) % nj, k = 1 + t / (ni * nj);
+    cb.p[f][B.IX(G.nb[0] + i - 1, G.nb[1] + j - 1, G.nb[2] + k - 1)] = 0.0f;
 }
 
 // corer counts: fine indices 1+nskip, 1+2 nskip, ... <= n - nskip (1-based)
@@ -286,8 +365,21 @@ void flanks_adj(FullModel* M, float* bar, const Flank& Fk, float t1, float t2) {
         ob.p[f] = fld(Fk.p_old_bar, M->n, f);
         cb.p[f] = fld(Fk.p_cur_bar, M->n, f);
     }
-    hipLaunchKernelGGL(k_adj_flanks, dim3((M->nshell + 255) / 256, NFLD), dim3(256), 0, M->s,
-                       M->shell, M->nshell, xb, ob, cb, t1, t2);
+    hipLaunchKernelGGL(k_adj_flanks_cells, dim3((M->nshell + 255) / 256), dim3(256), 0, M->s,
+                       M->shell, M->nshell, xb, M->fc_I0b);
+    hipLaunchKernelGGL(k_adj_flanks_gather, dim3((M->nfc + 63) / 64, (NFLD + kFcGroup - 1) / kFcGroup),
+                       dim3(64), 0, M->s, M->fc_cell, M->fc_start, M->fc_p, M->fc_w, M->nfc,
+                       M->fc_I0b, ob, cb, t1, t2);
+}
+
+void free_parent(FullModel* M) {
+    void* p[] = {M->shell, M->fc_cell, M->fc_start, M->fc_p, M->fc_w, M->fc_I0b};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    M->shell = nullptr;
+    M->fc_cell = M->fc_start = M->fc_p = nullptr;
+    M->fc_w = M->fc_I0b = nullptr;
+    M->nshell = M->nfc = 0;
 }
 
 void outer(FullModel* M, float* set, const Inflow& I) {
@@ -433,7 +525,7 @@ void g3m3_full_model_destroy(void* model) {
     if (!M) return;
     hipStreamSynchronize(M->s);
     hipFree(M->mem);
-    if (M->shell) hipFree(M->shell);
+    free_parent(M);
     delete M;
 }
 
@@ -877,12 +969,61 @@ int g3m3_full_model_set_parent(void* model, const float* geom) {
             }
         }
     }
-    if (M->shell) (void)hipFree(M->shell);
-    M->shell = nullptr;
-    if (hipMalloc(&M->shell, sizeof(ShellPt) * n) != hipSuccess ||
-        hipMemcpy(M->shell, h.data(), sizeof(ShellPt) * n, hipMemcpyHostToDevice) != hipSuccess)
+    // the gather table of the flanks' adjoint: each touched parent cell with
+    // its (shell point, trilinear weight) pairs, the weights as the forward
+    // kernel forms them
+    std::vector<std::vector<std::pair<int, float>>> by_cell;
+    std::vector<int> slot((size_t)M->n, -1), cells;
+    for (int t = 0; t < n; ++t) {
+        const ShellPt& p = h[t];
+        const float dx = p.dx, dy = p.dy, dz = p.dz;
+        const float ddx = 1.0f - dx, ddy = 1.0f - dy, ddz = 1.0f - dz;
+        const int X = 1, Y = p.sy, Z = p.sz, c0 = p.cb;
+        const int cn[8] = {c0, c0 + Z, c0 + Y, c0 + Y + Z, c0 + X, c0 + X + Z, c0 + X + Y,
+                           c0 + X + Y + Z};
+        const float w8[8] = {ddx * ddy * ddz, ddx * ddy * dz, ddx * dy * ddz, ddx * dy * dz,
+                             dx * ddy * ddz,  dx * ddy * dz,  dx * dy * ddz,  dx * dy * dz};
+        for (int q = 0; q < 8; ++q) {
+            if (slot[cn[q]] < 0) {
+                slot[cn[q]] = (int)cells.size();
+                cells.push_back(cn[q]);
+                by_cell.emplace_back();
+            }
+            by_cell[slot[cn[q]]].push_back({t, w8[q]});
+        }
+    }
+    // parent cells in ascending order: neighbouring threads of the gather
+    // update neighbouring cells
+    std::vector<int> order(cells.size());
+    for (size_t i = 0; i < cells.size(); ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] < cells[b]; });
+    std::vector<int> start(cells.size() + 1, 0), pts, sorted_cells(cells.size());
+    std::vector<float> wts;
+    for (size_t i = 0; i < cells.size(); ++i) {
+        sorted_cells[i] = cells[order[i]];
+        for (const auto& e : by_cell[order[i]]) {
+            pts.push_back(e.first);
+            wts.push_back(e.second);
+        }
+        start[i + 1] = (int)pts.size();
+    }
+    cells.swap(sorted_cells);
+    free_parent(M);
+    auto up = [](void** dst, const void* src, size_t bytes) {
+        return hipMalloc(dst, bytes) == hipSuccess &&
+               hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up((void**)&M->shell, h.data(), sizeof(ShellPt) * n) ||
+        !up((void**)&M->fc_cell, cells.data(), sizeof(int) * cells.size()) ||
+        !up((void**)&M->fc_start, start.data(), sizeof(int) * start.size()) ||
+        !up((void**)&M->fc_p, pts.data(), sizeof(int) * pts.size()) ||
+        !up((void**)&M->fc_w, wts.data(), sizeof(float) * wts.size()) ||
+        hipMalloc(&M->fc_I0b, sizeof(float) * (size_t)n * NFLD) != hipSuccess) {
+        free_parent(M);
         return 2;
+    }
     M->nshell = n;
+    M->nfc = (int)cells.size();
     return 0;
 }
 
@@ -922,12 +1063,20 @@ int g3m3_full_model_corer_adjoint(void* model, float* s_bar, float* p_bar) {
         fb.p[f] = fld(s_bar, M->n, f);
         cb.p[f] = fld(p_bar, M->n, f);
     }
-    const int k = M->c_nskip;
-    const int ni = corer_count(M->B.nx, k), nj = corer_count(M->B.ny, k),
-              nk = corer_count(M->B.nz, k);
-    if (ni > 0 && nj > 0 && nk > 0)
-        hipLaunchKernelGGL(k_adj_corer, dim3((ni * nj * nk + 255) / 256, NFLD), dim3(256), 0, M->s,
-                           M->B, fb, cb, k, M->c_nbx1, M->c_nby1, M->c_nbz1, ni, nj, nk);
+    CorerGeom G;
+    G.nskip = M->c_nskip;
+    G.nb[0] = M->c_nbx1;
+    G.nb[1] = M->c_nby1;
+    G.nb[2] = M->c_nbz1;
+    G.cnt[0] = corer_count(M->B.nx, G.nskip);
+    G.cnt[1] = corer_count(M->B.ny, G.nskip);
+    G.cnt[2] = corer_count(M->B.nz, G.nskip);
+    if (G.cnt[0] > 0 && G.cnt[1] > 0 && G.cnt[2] > 0) {
+        hipLaunchKernelGGL(k_adj_corer, dim3((M->n + 255) / 256, NFLD), dim3(256), 0, M->s, M->B,
+                           fb, cb, G);
+        hipLaunchKernelGGL(k_zero_core, dim3((G.cnt[0] * G.cnt[1] * G.cnt[2] + 255) / 256, NFLD),
+                           dim3(256), 0, M->s, M->B, cb, G);
+    }
     return launched("corer adjoint") ? 0 : 2;
 }
 
