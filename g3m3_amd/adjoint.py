@@ -15,9 +15,10 @@ Virtual spacecraft sample b (the perturbation field, as the reference's
 craft do, code/cft_craft_handling.f90:178-189) every ``every`` steps through
 ``g3m3_sample_fields`` (findgrid + trilin_interp).  ``FieldRollout`` is a
 ``torch.autograd.Function`` from (ctl, b at step 0) to the samples: the
-forward pass keeps b every ``ckpt`` steps; the backward pass recomputes each
-segment from its checkpoint with tapes (what the adjoint of a step needs)
-and runs the hand-written adjoint kernels back through it
+forward pass keeps b every ``ckpt`` steps, and when a gradient is wanted
+the tapes (what the adjoint of a step needs) of the last segment; the
+backward pass recomputes every earlier segment from its checkpoint with
+tapes and runs the hand-written adjoint kernels back through each
 (``g3m3_field_model_adjoint``, ``g3m3_sample_fields_adjoint``).
 """
 from __future__ import annotations
@@ -258,7 +259,7 @@ def _function():
         ctl (float64, accumulated in double on the device) and in b."""
 
         @staticmethod
-        def forward(ctx, ctl, b_start, model, steps, every, pos, ckpt):
+        def forward(ctx, ctl, b_start, model, steps, every, pos, ckpt, keep):
             c = ctl.detach().cpu().numpy()
             ns = steps // every
             out = torch.empty((ns, pos.shape[0], 3), dtype=torch.float32, device="cuda")
@@ -266,15 +267,17 @@ def _function():
             b = b_start.detach().clone()
             nb = torch.empty_like(b)
             ckpts = []
+            last = (steps - 1) // ckpt * ckpt
+            tapes = [model.new_tape() for _ in range(last, steps)] if keep else None
             for t in range(steps):
                 if t % ckpt == 0:
                     ckpts.append(b.clone())
-                model.step(b, nb, c)
+                model.step(b, nb, c, tapes[t - last] if keep and t >= last else None)
                 b, nb = nb, b
                 if (t + 1) % every == 0:
                     model.sample(b, pos, out[(t + 1) // every - 1], gp)
             ctx.model, ctx.steps, ctx.every, ctx.ckpt = model, steps, every, ckpt
-            ctx.c, ctx.pos, ctx.ckpts = c, pos, ckpts
+            ctx.c, ctx.pos, ctx.ckpts, ctx.last_tapes = c, pos, ckpts, tapes
             ctx.ctl_meta = (ctl.dtype, ctl.device, ctl.shape)
             return out
 
@@ -285,22 +288,28 @@ def _function():
             b_bar = torch.zeros((3, model.V), dtype=torch.float32, device="cuda")
             tmp = torch.empty_like(b_bar)
             ctl_bar = torch.zeros(4, dtype=torch.float64, device="cuda")
-            tapes = [model.new_tape() for _ in range(min(ckpt, steps))]
             for seg in reversed(range(0, steps, ckpt)):
                 end = min(seg + ckpt, steps)
-                b = ctx.ckpts[seg // ckpt].clone()
-                nb = torch.empty_like(b)
-                for t in range(seg, end):
-                    model.step(b, nb, ctx.c, tapes[t - seg])
-                    b, nb = nb, b
+                tapes = ctx.last_tapes
+                ctx.last_tapes = None
+                if tapes is None:
+                    # recompute the segment from its checkpoint (the last
+                    # segment's tapes come from the forward pass)
+                    tapes = [model.new_tape() for _ in range(seg, end)]
+                    b = ctx.ckpts[seg // ckpt].clone()
+                    nb = torch.empty_like(b)
+                    for t in range(seg, end):
+                        model.step(b, nb, ctx.c, tapes[t - seg])
+                        b, nb = nb, b
                 for t in reversed(range(seg, end)):
                     if (t + 1) % every == 0:
                         model.sample_adjoint(ctx.pos, gs[(t + 1) // every - 1], b_bar)
                     model.adjoint(tapes[t - seg], b_bar, tmp, ctl_bar, ctx.c)
                     b_bar, tmp = tmp, b_bar
+                del tapes
             dtype, device, shape = ctx.ctl_meta
             g_ctl = ctl_bar.cpu().to(dtype).reshape(shape).to(device)
-            return g_ctl, b_bar, None, None, None, None, None
+            return g_ctl, b_bar, None, None, None, None, None, None
 
     return FieldRollout
 
@@ -310,10 +319,12 @@ _FR = None
 
 def rollout(ctl, b_start, model: FieldModel, steps: int, every: int, pos, ckpt: int = 10):
     """Samples of the model's rollout, differentiable in ctl and b_start."""
+    import torch
     global _FR
     if _FR is None:
         _FR = _function()
-    return _FR.apply(ctl, b_start, model, steps, every, pos, ckpt)
+    keep = torch.is_grad_enabled() and (ctl.requires_grad or b_start.requires_grad)
+    return _FR.apply(ctl, b_start, model, steps, every, pos, ckpt, keep)
 
 
 # --- the whole box-step -------------------------------------------------------
@@ -560,7 +571,7 @@ def _full_function():
         and in the state, by checkpointed recompute as FieldRollout."""
 
         @staticmethod
-        def forward(ctx, ctl, s_start, model, steps, every, pos, ckpt):
+        def forward(ctx, ctl, s_start, model, steps, every, pos, ckpt, keep):
             c = ctl.detach().cpu().numpy()
             ns = steps // every
             nq = len(FULL_SAMPLED)
@@ -569,15 +580,17 @@ def _full_function():
             s = s_start.detach().clone()
             ns_ = torch.empty_like(s)
             ckpts = []
+            last = (steps - 1) // ckpt * ckpt
+            tapes = [model.new_tape() for _ in range(last, steps)] if keep else None
             for t in range(steps):
                 if t % ckpt == 0:
                     ckpts.append(s.clone())
-                model.step(s, ns_, c)
+                model.step(s, ns_, c, tapes[t - last] if keep and t >= last else None)
                 s, ns_ = ns_, s
                 if (t + 1) % every == 0:
                     model.sample(s, pos, out[(t + 1) // every - 1], gp)
             ctx.model, ctx.steps, ctx.every, ctx.ckpt = model, steps, every, ckpt
-            ctx.c, ctx.pos, ctx.ckpts = c, pos, ckpts
+            ctx.c, ctx.pos, ctx.ckpts, ctx.last_tapes = c, pos, ckpts, tapes
             ctx.ctl_meta = (ctl.dtype, ctl.device, ctl.shape)
             return out
 
@@ -588,22 +601,28 @@ def _full_function():
             s_bar = torch.zeros((NSTATE, model.V), dtype=torch.float32, device="cuda")
             tmp = torch.empty_like(s_bar)
             ctl_bar = torch.zeros(len(FULL_CTL_NAMES), dtype=torch.float64, device="cuda")
-            tapes = [model.new_tape() for _ in range(min(ckpt, steps))]
             for seg in reversed(range(0, steps, ckpt)):
                 end = min(seg + ckpt, steps)
-                s = ctx.ckpts[seg // ckpt].clone()
-                ns_ = torch.empty_like(s)
-                for t in range(seg, end):
-                    model.step(s, ns_, ctx.c, tapes[t - seg])
-                    s, ns_ = ns_, s
+                tapes = ctx.last_tapes
+                ctx.last_tapes = None
+                if tapes is None:
+                    # recompute the segment from its checkpoint (the last
+                    # segment's tapes come from the forward pass)
+                    tapes = [model.new_tape() for _ in range(seg, end)]
+                    s = ctx.ckpts[seg // ckpt].clone()
+                    ns_ = torch.empty_like(s)
+                    for t in range(seg, end):
+                        model.step(s, ns_, ctx.c, tapes[t - seg])
+                        s, ns_ = ns_, s
                 for t in reversed(range(seg, end)):
                     if (t + 1) % every == 0:
                         model.sample_adjoint(ctx.pos, gs[(t + 1) // every - 1], s_bar)
                     model.adjoint(tapes[t - seg], s_bar, tmp, ctl_bar, ctx.c)
                     s_bar, tmp = tmp, s_bar
+                del tapes
             dtype, device, shape = ctx.ctl_meta
             g_ctl = ctl_bar.cpu().to(dtype).reshape(shape).to(device)
-            return g_ctl, s_bar, None, None, None, None, None
+            return g_ctl, s_bar, None, None, None, None, None, None
 
     return FullRollout
 
@@ -614,7 +633,9 @@ _FULL = None
 def full_rollout(ctl, s_start, model: FullModel, steps: int, every: int, pos, ckpt: int = 10):
     """Craft samples of the full model's rollout, differentiable in ctl and
     the initial state."""
+    import torch
     global _FULL
     if _FULL is None:
         _FULL = _full_function()
-    return _FULL.apply(ctl, s_start, model, steps, every, pos, ckpt)
+    keep = torch.is_grad_enabled() and (ctl.requires_grad or s_start.requires_grad)
+    return _FULL.apply(ctl, s_start, model, steps, every, pos, ckpt, keep)

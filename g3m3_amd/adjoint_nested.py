@@ -15,10 +15,11 @@ the innermost one (``nested_schedule``).
 ``NestedRollout`` is a ``torch.autograd.Function`` from (ctl, the boxes'
 states at step 0) to the craft samples every ``every`` full steps (findgrid
 picks the finest box holding each craft, code/cft_find_aux_cube.f90).  The
-forward pass keeps the boxes' states every ``ckpt`` full steps; the backward
-pass recomputes each segment from its checkpoint, keeping every state version
-and a tape per box-step, and runs the hand-written adjoints back through the
-schedule: a box-step's adjoint (``g3m3_full_model_adjoint`` /
+forward pass keeps the boxes' states every ``ckpt`` full steps, and when a
+gradient is wanted the last segment's state versions and tapes; the
+backward pass recomputes every earlier segment from its checkpoint, keeping
+every state version and a tape per box-step, and runs the hand-written
+adjoints back through the schedule: a box-step's adjoint (``g3m3_full_model_adjoint`` /
 ``g3m3_full_model_adjoint_inner``, which also adds the flanks' adjoints into
 the parent's two states), the corer's (``g3m3_full_model_corer_adjoint``)
 and the sampling's (``g3m3_sample_fields_adjoint``).
@@ -286,7 +287,7 @@ def _nested_function():
         double on the device) and in each box's state."""
 
         @staticmethod
-        def forward(ctx, ctl, model, steps, every, pos, ckpt, *states):
+        def forward(ctx, ctl, model, steps, every, pos, ckpt, keep, *states):
             c = ctl.detach().cpu().numpy()
             ns = steps // every
             out = torch.empty((ns, pos.shape[0], len(FULL_SAMPLED)), dtype=torch.float32,
@@ -294,14 +295,20 @@ def _nested_function():
             gp = torch.empty((pos.shape[0], 4), dtype=torch.int32, device="cuda")
             cur = [s.detach() for s in states]
             ckpts = []
+            last = (steps - 1) // ckpt * ckpt
+            recs = [] if keep else None
             for t in range(steps):
                 if t % ckpt == 0:
                     ckpts.append([s.clone() for s in cur])
-                cur = model.full_step(cur, c)
+                rec = [] if keep and t >= last else None
+                nxt = model.full_step(cur, c, rec)
+                if rec is not None:
+                    recs.append((rec, nxt))
+                cur = nxt
                 if (t + 1) % every == 0:
                     model.sample(cur, pos, out[(t + 1) // every - 1], gp)
             ctx.model, ctx.steps, ctx.every, ctx.ckpt = model, steps, every, ckpt
-            ctx.c, ctx.pos, ctx.ckpts = c, pos, ckpts
+            ctx.c, ctx.pos, ctx.ckpts, ctx.last_recs = c, pos, ckpts, recs
             ctx.ctl_meta = (ctl.dtype, ctl.device, ctl.shape)
             return out
 
@@ -314,13 +321,18 @@ def _nested_function():
             ctl_bar = torch.zeros(len(FULL_CTL_NAMES), dtype=torch.float64, device="cuda")
             for seg in reversed(range(0, steps, ckpt)):
                 end = min(seg + ckpt, steps)
-                cur = ctx.ckpts[seg // ckpt]
-                recs = []
-                for t in range(seg, end):
-                    rec = []
-                    nxt = model.full_step(cur, ctx.c, rec)
-                    recs.append((rec, nxt))
-                    cur = nxt
+                recs = ctx.last_recs
+                ctx.last_recs = None
+                if recs is None:
+                    # recompute the segment from its checkpoint (the last
+                    # segment's records come from the forward pass)
+                    cur = ctx.ckpts[seg // ckpt]
+                    recs = []
+                    for t in range(seg, end):
+                        rec = []
+                        nxt = model.full_step(cur, ctx.c, rec)
+                        recs.append((rec, nxt))
+                        cur = nxt
                 for t in reversed(range(seg, end)):
                     rec, finals = recs.pop()
                     if (t + 1) % every == 0:
@@ -329,7 +341,7 @@ def _nested_function():
                     del rec, finals
             dtype, device, shape = ctx.ctl_meta
             g_ctl = ctl_bar.cpu().to(dtype).reshape(shape).to(device)
-            return (g_ctl, None, None, None, None, None, *bars)
+            return (g_ctl, None, None, None, None, None, None, *bars)
 
     return NestedRollout
 
@@ -341,7 +353,9 @@ def nested_rollout(ctl, states, model: NestedFullModel, steps: int, every: int, 
                    ckpt: int = 5):
     """Craft samples of the nested stack's rollout, differentiable in ctl
     and every box's initial state."""
+    import torch
     global _NESTED
     if _NESTED is None:
         _NESTED = _nested_function()
-    return _NESTED.apply(ctl, model, steps, every, pos, ckpt, *states)
+    keep = torch.is_grad_enabled() and (ctl.requires_grad or any(s.requires_grad for s in states))
+    return _NESTED.apply(ctl, model, steps, every, pos, ckpt, keep, *states)

@@ -79,6 +79,13 @@ __global__ void k_add_bnd(Box B, FieldPtrs dst, FieldPtrs src) {
     const int c = bnd_cell(B, t), f = blockIdx.y;
     dst.p[f][c] += src.p[f][c];
 }
+// dst = src on the boundary cells (blockIdx.y = field)
+__global__ void k_copy_bnd(Box B, FieldPtrs dst, FieldPtrs src) {
+    const int t = gtid();
+    if (t >= bnd_cells(B)) return;
+    const int c = bnd_cell(B, t), f = blockIdx.y;
+    dst.p[f][c] = src.p[f][c];
+}
 // zero n cells of each listed array (blockIdx.y = array)
 __global__ void k_zero_list(int n, FieldPtrs p) {
     const int c = gtid();
@@ -169,6 +176,18 @@ void zero_vb(FullModel* M) {
 void copyf(FullModel* M, float* dst, const float* src, int nf) {
     hipMemcpyAsync(dst, src, sizeof(float) * (size_t)M->n * nf, hipMemcpyDeviceToDevice, M->s);
 }
+// the boundary cells of nf fields: a set whose interior the next pass
+// writes whole (the pushes, the Lapidus passes) takes only its boundary
+// from the set it starts from
+void copy_bnd(FullModel* M, float* dst, const float* src, int nf) {
+    FieldPtrs d{}, s{};
+    for (int f = 0; f < nf; ++f) {
+        d.p[f] = dst + (size_t)M->n * f;
+        s.p[f] = const_cast<float*>(src) + (size_t)M->n * f;
+    }
+    hipLaunchKernelGGL(k_copy_bnd, dim3((bnd_cells(M->B) + 255) / 256, nf), dim3(256), 0, M->s,
+                       M->B, d, s);
+}
 
 // inflow values of the 34 fields from the controls, as the runtime forms
 // them (code/sub_bndry_outer.f90:50-103), with each value's control index
@@ -224,8 +243,6 @@ Inflow inflow(const g3m3_full_model_desc& d, const float* c) {
 // cell's adjoint the product of the fracs times its own, and I0's adjoint
 // the cell's adjoint times sum_j (1 - frac_j) prod_{l > j} frac_l, stored
 // point-major: I0b[t * NFLD + f].
-constexpr int kFcGroup = 6;   // fields per thread of the gather pass
-
 __global__ void k_adj_flanks_cells(const ShellPt* __restrict__ tab, int npts, FieldPtrs xb,
                                    float* __restrict__ I0b) {
     const int t = gtid();
@@ -250,41 +267,43 @@ __global__ void k_adj_flanks_cells(const ShellPt* __restrict__ tab, int npts, Fi
     for (int f = 0; f < NFLD; ++f) xb.p[f][p.c] = pw * wb[f];
 }
 
-// Second pass, per parent cell the shell's corners touch (ascending) and
-// group of kFcGroup fields: the adjoint of work = old t1 + cur t2 at that
-// cell is the sum over the shell points that interpolate from it of their
-// trilinear weight times I0's adjoint; old's adjoint gains t1 times it,
-// cur's t2 times it.  blockIdx.y = field group.
+// Second pass, per parent cell the shell's corners touch (ascending), all
+// fields: the adjoint of work = old t1 + cur t2 at that cell is the sum over
+// the shell points that interpolate from it of their trilinear weight times
+// I0's adjoint (each point's row of NFLD values read once); old's adjoint
+// gains t1 times it, cur's t2 times it.
 __global__ void __launch_bounds__(64) k_adj_flanks_gather(
     const int* __restrict__ cell, const int* __restrict__ start, const int* __restrict__ pt,
     const float* __restrict__ wt, int ncell, const float* __restrict__ I0b, FieldPtrs ob,
     FieldPtrs cb, float t1, float t2) {
-    const int i = gtid(), f0 = blockIdx.y * kFcGroup;
+    const int i = gtid();
     if (i >= ncell) return;
-    float acc[kFcGroup];
+    float acc[NFLD];
 #pragma unroll
-    for (int g = 0; g < kFcGroup; ++g) acc[g] = 0.0f;
+    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0f;
     const int e1 = start[i + 1];
     for (int e = start[i]; e < e1; ++e) {
         const float w = wt[e];
-        const float* __restrict__ src = I0b + (size_t)pt[e] * NFLD + f0;
+        const float2* __restrict__ src =
+            reinterpret_cast<const float2*>(I0b + (size_t)pt[e] * NFLD);
 #pragma unroll
-        for (int g = 0; g < kFcGroup; g += 2) {
-            if (f0 + g < NFLD) {
-                const float2 v = *reinterpret_cast<const float2*>(src + g);
-                acc[g] += w * v.x;
-                acc[g + 1] += w * v.y;
-            }
+        for (int h = 0; h < NFLD / 2; ++h) {
+            const float2 v = src[h];
+            acc[2 * h] += w * v.x;
+            acc[2 * h + 1] += w * v.y;
         }
     }
     const int c = cell[i];
+    float o[NFLD], q[NFLD];
 #pragma unroll
-    for (int g = 0; g < kFcGroup; ++g) {
-        const int f = f0 + g;
-        if (f < NFLD) {
-            ob.p[f][c] += t1 * acc[g];
-            cb.p[f][c] += t2 * acc[g];
-        }
+    for (int f = 0; f < NFLD; ++f) {
+        o[f] = ob.p[f][c];
+        q[f] = cb.p[f][c];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) {
+        ob.p[f][c] = o[f] + t1 * acc[f];
+        cb.p[f][c] = q[f] + t2 * acc[f];
     }
 }
 
@@ -295,20 +314,18 @@ __global__ void __launch_bounds__(64) k_adj_flanks_gather(
 // a fine cell collects 6/12 of its own centre's adjoint (if it is one) and
 // 1/12 of each centre one step away along one axis.  The parent core's
 // adjoint is zeroed afterwards (k_zero_core: the corer overwrote it).
-// blockIdx.y = field.
+// One thread per cell, all fields.
 struct CorerGeom {
     int nskip, nb[3], cnt[3];
 };
 __device__ inline bool corer_centre(const CorerGeom& G, int ax, int q, int& a) {
-    if (q < G.nskip || q % G.nskip) return false;
-    a = q / G.nskip;
-    return a <= G.cnt[ax];
+    if (q < G.nskip) return false;
+    a = udiv_small(q, G.nskip);
+    return a * G.nskip == q && a <= G.cnt[ax];
 }
 __global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, CorerGeom G) {
     int x[3];
-    if (!cell_of(B, gtid(), x[0], x[1], x[2])) return;
-    const int f = blockIdx.y;
-    const float* __restrict__ src = cb.p[f];
+    if (!cell_of_fast(B, gtid(), x[0], x[1], x[2])) return;
     int a[3];
     bool al[3];
 #pragma unroll
@@ -318,8 +335,15 @@ __global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, CorerGeom G) {
     auto parent = [&](int i, int j, int k) {
         return B.IX(G.nb[0] + i - 1, G.nb[1] + j - 1, G.nb[2] + k - 1);
     };
-    float acc = 0.0f;
-    if (al[0] && al[1] && al[2]) acc += 6.0f * src[parent(a[0], a[1], a[2])] / 12.0f;
+    // the parent cells this cell gathers from and their weights: its own
+    // centre's (6/12), each centre one step away along one axis (1/12)
+    int pc[7];
+    float pw[7];
+    int np = 0;
+    if (al[0] && al[1] && al[2]) {
+        pc[np] = parent(a[0], a[1], a[2]);
+        pw[np++] = 6.0f;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int e = (d + 1) % 3, g = (d + 2) % 3;
@@ -332,10 +356,19 @@ __global__ void k_adj_corer(Box B, FieldPtrs fb, FieldPtrs cb, CorerGeom G) {
             idx[d] = c;
             idx[e] = a[e];
             idx[g] = a[g];
-            acc += src[parent(idx[0], idx[1], idx[2])] / 12.0f;
+            pc[np] = parent(idx[0], idx[1], idx[2]);
+            pw[np++] = 1.0f;
         }
     }
-    if (acc != 0.0f) fb.p[f][B.IX(x[0], x[1], x[2])] += acc;
+    if (np == 0) return;
+    const int m = B.IX(x[0], x[1], x[2]);
+    for (int f = 0; f < NFLD; ++f) {
+        const float* __restrict__ src = cb.p[f];
+        float acc = 0.0f;
+        for (int q = 0; q < np; ++q)
+            acc += pw[q] == 6.0f ? 6.0f * src[pc[q]] / 12.0f : src[pc[q]] / 12.0f;
+        fb.p[f][m] += acc;
+    }
 }
 __global__ void k_zero_core(Box B, FieldPtrs cb, CorerGeom G) {
     const int t = gtid(), f = blockIdx.y;
@@ -367,9 +400,9 @@ void flanks_adj(FullModel* M, float* bar, const Flank& Fk, float t1, float t2) {
     }
     hipLaunchKernelGGL(k_adj_flanks_cells, dim3((M->nshell + 255) / 256), dim3(256), 0, M->s,
                        M->shell, M->nshell, xb, M->fc_I0b);
-    hipLaunchKernelGGL(k_adj_flanks_gather, dim3((M->nfc + 63) / 64, (NFLD + kFcGroup - 1) / kFcGroup),
-                       dim3(64), 0, M->s, M->fc_cell, M->fc_start, M->fc_p, M->fc_w, M->nfc,
-                       M->fc_I0b, ob, cb, t1, t2);
+    hipLaunchKernelGGL(k_adj_flanks_gather, dim3((M->nfc + 63) / 64), dim3(64), 0, M->s,
+                       M->fc_cell, M->fc_start, M->fc_p, M->fc_w, M->nfc, M->fc_I0b, ob, cb, t1,
+                       t2);
 }
 
 void free_parent(FullModel* M) {
@@ -557,7 +590,7 @@ static int step_impl(void* model, const float* s_in, float* s_out, float* tape,
     // stage 1
     M->ef = ef_at(fld(T, n, T_EF1), n);
     efield(M, S, eta);
-    copyf(M, W1, S, NFLD);
+    copy_bnd(M, W1, S, NFLD);
     copyf(M, fld(W1, n, EP), fld(S, n, WE), 1);
     push_species(M, W1, S, S, 0.5f * delt);
     push_b(M, W1, S, 0.5f * delt);
@@ -566,7 +599,7 @@ static int step_impl(void* model, const float* s_in, float* s_out, float* tape,
     // stage 2
     M->ef = ef_at(fld(T, n, T_EF2), n);
     efield(M, W1, eta);
-    copyf(M, S2, S, NFLD);
+    copy_bnd(M, S2, S, NFLD);
     push_species(M, S2, S, W1, delt);
     {
         const float* ev[3] = {M->ef.ev[0], M->ef.ev[1], M->ef.ev[2]};
@@ -577,7 +610,7 @@ static int step_impl(void* model, const float* s_in, float* s_out, float* tape,
     push_b(M, S2, S, delt);
     if (!Fk) outer(M, S2, I);
     // Lapidus into a copy of W1
-    copyf(M, W3, W1, NFLD);
+    copy_bnd(M, W3, W1, NFLD);
     for (int sp = 0; sp < 3; ++sp) {
         const float* pp[3] = {fld(S2, n, 10 * sp + 4), fld(S2, n, 10 * sp + 5),
                               fld(S2, n, 10 * sp + 6)};
@@ -1072,8 +1105,8 @@ int g3m3_full_model_corer_adjoint(void* model, float* s_bar, float* p_bar) {
     G.cnt[1] = corer_count(M->B.ny, G.nskip);
     G.cnt[2] = corer_count(M->B.nz, G.nskip);
     if (G.cnt[0] > 0 && G.cnt[1] > 0 && G.cnt[2] > 0) {
-        hipLaunchKernelGGL(k_adj_corer, dim3((M->n + 255) / 256, NFLD), dim3(256), 0, M->s, M->B,
-                           fb, cb, G);
+        hipLaunchKernelGGL(k_adj_corer, dim3((M->n + 255) / 256), dim3(256), 0, M->s, M->B, fb,
+                           cb, G);
         hipLaunchKernelGGL(k_zero_core, dim3((G.cnt[0] * G.cnt[1] * G.cnt[2] + 255) / 256, NFLD),
                            dim3(256), 0, M->s, M->B, cb, G);
     }

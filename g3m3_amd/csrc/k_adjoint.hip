@@ -429,7 +429,6 @@ struct FctAdj {
     const float* wrk[3];
     float* ob[3];        // the output's adjoint (read)
     float* D[3];         // scratch: the diffused field
-    unsigned char* code[3];   // scratch: the limiter's branch per axis (2 bits each)
     float* Db[3];        // scratch: D's adjoint
     float* wb[3];        // wrk's adjoint (assigned)
     float* oldb[3];      // old's adjoint (accumulated)
@@ -473,25 +472,8 @@ __device__ inline int fct_branch(const float* D, const float* W, int c0, int st,
     return mn > 0.0f ? br : 0;
 }
 
-// the branch of every interior index of the three lines through a cell,
-// packed two bits per axis (0 outside 1..n-2)
-__global__ void k_adj_fct_codes(FctAdj A) {
-    int i, j, k;
-    if (!cell_of(A.B, gtid(), i, j, k)) return;
-    const int f = blockIdx.y, c = A.B.IX(i, j, k);
-    const int pos[3] = {i, j, k}, len[3] = {A.B.nx, A.B.ny, A.B.nz};
-    const int st[3] = {1, A.B.nx, A.B.nx * A.B.ny};
-    unsigned v = 0;
-    for (int ax = 0; ax < 3; ++ax) {
-        const int t = pos[ax], n = len[ax];
-        if (t >= 1 && t <= n - 2)
-            v |= (unsigned)fct_branch(A.D[f], A.wrk[f], c - t * st[ax], st[ax], n, t, A.chi)
-                 << (2 * ax);
-    }
-    A.code[f][c] = (unsigned char)v;
-}
-
-// per cell: D's and wrk's adjoints.  Along each axis the flux fn(u) of
+// per cell: D's and wrk's adjoints (the limiter's branches recomputed from
+// D and wrk where they are used).  Along each axis the flux fn(u) of
 // interior index u carries g(u) = -U(u) + U(u+1) (U: the output's adjoint
 // where the axis's flux differences act; the copy fn(0) = fn(1) folded in)
 // to d1(u-1) (branch 1), chi d(u) (branch 2) or d1(u+1) (branch 3, which
@@ -505,7 +487,8 @@ __global__ void k_adj_fct_cells(FctAdj A) {
     const int nx = A.B.nx, ny = A.B.ny, nz = A.B.nz;
     const int pos[3] = {i, j, k}, len[3] = {nx, ny, nz}, st[3] = {1, nx, nx * ny};
     const float* ob = A.ob[f];
-    const unsigned char* code = A.code[f];
+    const float* D = A.D[f];
+    const float* W = A.wrk[f];
     const float chi = A.chi;
     const bool kin = k > 0 && k < nz - 1;
     float db = 0.0f, wbar = 0.0f;
@@ -526,7 +509,7 @@ __global__ void k_adj_fct_cells(FctAdj A) {
         for (int q = 0; q < 4; ++q) {
             const int u = t - 2 + q;
             G[q] = (u >= 2 ? -U[q] : 0.0f) + (u + 1 <= n - 2 ? U[q + 1] : 0.0f);
-            br[q] = (u >= 1 && u <= n - 2) ? (int)((code[c + (q - 2) * s] >> (2 * ax)) & 3u) : 0;
+            br[q] = (u >= 1 && u <= n - 2) ? fct_branch(D, W, c - t * s, s, n, u, chi) : 0;
         }
         // adjoints of d1(t-1), d1(t) (p) and of d(t-1), d(t) (q); index q = u - t + 2
         const float p0 = (br[2] == 1 ? G[2] : 0.0f) + (br[0] == 3 ? G[0] : 0.0f) +
@@ -681,7 +664,7 @@ void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float c
                        v[0], v[1], v[2], chi, delt, qb[0], qb[1], qb[2], w1b[0], w1b[1], w1b[2]);
 }
 
-size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 3 * B.V(); }
+size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 2 * B.V(); }
 
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
                     float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
@@ -691,19 +674,17 @@ void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], floa
     A.B = B;
     A.chi = chi;
     for (int f = 0; f < nf; ++f) {
-        float* base = scratch + (size_t)3 * n * f;
+        float* base = scratch + (size_t)2 * n * f;
         A.old[f] = old[f];
         A.wrk[f] = wrk[f];
         A.ob[f] = ob[f];
         A.D[f] = base;
-        A.code[f] = reinterpret_cast<unsigned char*>(base + n);
-        A.Db[f] = base + (size_t)n * 2;
+        A.Db[f] = base + (size_t)n;
         A.wb[f] = wb[f];
         A.oldb[f] = oldb[f];
     }
     const dim3 g((n + 255) / 256, nf), t(256);
     hipLaunchKernelGGL(k_adj_fct_diffuse, g, t, 0, s, A);
-    hipLaunchKernelGGL(k_adj_fct_codes, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_cells, g, t, 0, s, A);
     hipLaunchKernelGGL(k_adj_fct_old, g, t, 0, s, A);
 }

@@ -221,7 +221,7 @@ __device__ inline void adj_outer_pass(const Box& B, const OuterAdj& O, int f, in
             const float x = O.val[f][c];
             v *= (float)sgn_abs * (float)((x > 0.0f) - (x < 0.0f));
         }
-        atomicAdd(q + c, v);
+        q[c] += v;    // no two threads of a phase give to one cell
     };
     switch (pass) {
     case 1: {
