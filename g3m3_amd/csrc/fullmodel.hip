@@ -806,9 +806,9 @@ static int adjoint_impl(void* model, const float* tape, const float* sout_bar, f
     const float* S2 = fld(T, n, T_S2);
     const float* W3 = fld(T, n, T_W3);
     const float* S4 = fld(T, n, T_S4);
+    // SB's 34 fields are first written (assigned) by the FCT's adjoint, its
+    // WE by the last pass; S2B's by the Lapidus passes' adjoints
     float* SB = sin_bar;
-    zero(M, SB, (size_t)NST * n);
-    zero(M, M->S2B, (size_t)NFLD * n);
     zero(M, M->W1B, (size_t)NFLD * n);
     copyf(M, M->A, sout_bar, NFLD);
     // S' = bndry_outer(fcsmooth(S, W3)) (or the flanks)
@@ -830,7 +830,7 @@ static int adjoint_impl(void* model, const float* tape, const float* sout_bar, f
             oldb[q] = fld(SB, n, f);
         }
         launch_adj_fct(M->B, old, wrk, ob, wb, oldb, f0 >= BX ? d.diferg : d.difrho,
-                       M->fct_scr, s, nf);
+                       M->fct_scr, s, nf, true);
         f0 += nf;
     }
     // WE' = W3's pressure after its boundary pass
@@ -872,6 +872,7 @@ static int adjoint_impl(void* model, const float* tape, const float* sout_bar, f
         L.delt = delt;
         L.scr = M->lap_scr;
         L.nf = 10;
+        L.assign = true;
         launch_adj_lap_species(L, s);
         const float* vb[3] = {M->VB[0], M->VB[1], M->VB[2]};
         float* pb[3] = {fld(M->S2B, n, 10 * sp + 4), fld(M->S2B, n, 10 * sp + 5),
@@ -911,6 +912,7 @@ static int adjoint_impl(void* model, const float* tape, const float* sout_bar, f
         L.delt = delt;
         L.scr = M->lap_scr;
         L.nf = 4;
+        L.assign = true;
         launch_adj_lap_species(L, s);
         launch_adj_vtot(n, V, s);
     }
@@ -932,9 +934,7 @@ static int adjoint_impl(void* model, const float* tape, const float* sout_bar, f
     else outer_adj(M, M->W1B, W1, I, ctl_bar);
     {
         FieldPtrs dst{}, src{};
-        dst.p[0] = fld(SB, n, WE);
-        src.p[0] = fld(M->W1B, n, EP);
-        hipLaunchKernelGGL(k_add_all, dim3((n + 255) / 256, 1), dim3(256), 0, s, n, dst, src);
+        copyf(M, fld(SB, n, WE), fld(M->W1B, n, EP), 1);
         zero(M, fld(M->W1B, n, EP), n);
         for (int f = 0; f < NFLD; ++f) {
             dst.p[f] = fld(SB, n, f);

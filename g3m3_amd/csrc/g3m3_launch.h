@@ -150,9 +150,10 @@ void launch_adj_outer_b(const Box& B, float* bb[3], double* sbb, hipStream_t s);
 void launch_adj_lap(const Box& B, const float* wb[3], const float* v[3], float chi, float delt,
                     float* qb[3], float* w1b[3], hipStream_t s);
 size_t adj_fct_scratch_floats(const Box& B);
+// old's adjoint is accumulated, or with assign_old assigned (its first write)
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
                     float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
-                    int nf = 3);
+                    int nf = 3, bool assign_old = false);
 void launch_add3(int n, const float* x[3], float* y[3], hipStream_t s);
 struct CraftGrid;
 void launch_adj_sample(int npts, const float* sxyz, const CraftGrid& G, int nx, int ny, int nz,
@@ -189,6 +190,7 @@ struct LapAdj {
     float delt;
     float* scr;              // adj_lapsp_scratch_floats
     int nf = 10;             // fields in use
+    bool assign = false;     // qb assigned (=) instead of accumulated
 };
 size_t adj_ion_scratch_floats(const Box& B);
 size_t adj_lapsp_scratch_floats(const Box& B);

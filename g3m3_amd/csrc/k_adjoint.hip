@@ -431,8 +431,9 @@ struct FctAdj {
     float* D[3];         // scratch: the diffused field
     float* Db[3];        // scratch: D's adjoint
     float* wb[3];        // wrk's adjoint (assigned)
-    float* oldb[3];      // old's adjoint (accumulated)
+    float* oldb[3];      // old's adjoint (accumulated, or assigned with assign_old)
     float chi;
+    bool assign_old;
 };
 
 __global__ void k_adj_fct_diffuse(FctAdj A) {
@@ -552,7 +553,7 @@ __global__ void k_adj_fct_old(FctAdj A) {
     if (k > 0) acc += Db[c - Z];
     if (k < A.B.nz - 1) acc += Db[c + Z];
     acc -= 6.0f * Db[c];
-    A.oldb[f][c] += A.chi * acc;
+    A.oldb[f][c] = (A.assign_old ? 0.0f : A.oldb[f][c]) + A.chi * acc;
 }
 
 // y += x on three fields of n cells
@@ -668,11 +669,12 @@ size_t adj_fct_scratch_floats(const Box& B) { return (size_t)3 * 2 * B.V(); }
 
 void launch_adj_fct(const Box& B, const float* old[3], const float* wrk[3], float* ob[3],
                     float* wb[3], float* oldb[3], float chi, float* scratch, hipStream_t s,
-                    int nf) {
+                    int nf, bool assign_old) {
     const int n = B.V();
     FctAdj A;
     A.B = B;
     A.chi = chi;
+    A.assign_old = assign_old;
     for (int f = 0; f < nf; ++f) {
         float* base = scratch + (size_t)2 * n * f;
         A.old[f] = old[f];

@@ -397,7 +397,7 @@ __global__ void k_adj_lapsp_gather(LapAdj L) {
             if (in(d, -1)) acc += g * um[d] * wb[m - st[d]];
             if (in(d, 1)) acc += g * up[d] * wb[m + st[d]];
         }
-        L.qb[f][m] += acc;
+        L.qb[f][m] = (L.assign ? 0.0f : L.qb[f][m]) + acc;
     }
     for (int d = 0; d < 3; ++d) {
         const float* P = L.scr + (size_t)(2 * d) * n;       // |v(c+d) - v(c)|
