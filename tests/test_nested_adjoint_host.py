@@ -13,6 +13,9 @@ g3m3_amd.adjoint_nested.nested_schedule) against the oracle, on the CPU.
 * In float64 the gradient of a sum over the boxes' states after one full
   step matches central differences of itself in eta and in the upstream
   density (through the outer box's inflow, the flanks and the corer).
+* The schedule's shape for four boxes (visits, corer calls, time weights),
+  its refusal of other time-step ratios, and the interpolation that seeds
+  the inner boxes (against scipy's first-order map_coordinates).
 """
 from __future__ import annotations
 
@@ -90,3 +93,42 @@ def test_float64_nested_gradient_matches_central_differences():
             cm[i] -= h
             fd = (float(loss(cp)) - float(loss(cm))) / (2 * h)
             assert abs(float(gr[i]) - fd) <= 1e-5 * abs(fd), (i, float(gr[i]), fd)
+
+
+def test_schedule_shape_four_boxes():
+    """Four boxes, each time step twice the inner one: 8 visits per full
+    step; box b+1 steps 2^(3-b) times, every box but the outermost catches
+    up with its parent after each of the parent's steps (1 + 2 + 4 corer
+    calls), then every inner box is restricted once more."""
+    ev = nested_schedule([0.125, 0.25, 0.5, 1.0])
+    steps = [e[1] for e in ev if e[0] == "step"]
+    assert [steps.count(b) for b in range(4)] == [8, 4, 2, 1]
+    corers = [e[1] for e in ev if e[0] == "corer"]
+    assert [corers.count(b) for b in range(3)] == [4 + 1, 2 + 1, 1 + 1]
+    assert ev[0] == ("step", 3, None)
+    # the flanks' time weights stay in [0, 1] and sum to 1
+    for e in ev:
+        if e[0] == "step" and e[2] is not None:
+            t1a, t2a, t1b, t2b = (float(x) for x in e[2])
+            assert 0.0 <= t1a <= 1.0 and 0.0 <= t1b <= 1.0
+            assert t1a + t2a == 1.0 and t1b + t2b == 1.0
+
+
+def test_schedule_refuses_other_ratios():
+    with pytest.raises(ValueError):
+        nested_schedule([0.1, 0.3])
+
+
+def test_inner_box_interpolation_matches_scipy():
+    """nested_setup's trilinear interpolation of the parent state (before
+    its smooth variations) is scipy's first-order map_coordinates."""
+    from scipy.ndimage import map_coordinates
+    from g3m3_amd.adjoint_nested import _trilinear
+    rng = np.random.default_rng(9)
+    src = rng.normal(size=(2, 9, 8, 7))
+    x, y, z = np.linspace(0.5, 5.5, 6), np.linspace(1.0, 6.0, 5), np.linspace(0.0, 8.0, 4)
+    got = _trilinear(src, x, y, z)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    for a in range(2):
+        want = map_coordinates(src[a], [Z, Y, X], order=1)
+        assert np.allclose(got[a], want, rtol=0, atol=1e-12)
