@@ -326,7 +326,7 @@ def full_adjoint_case(steps: int = 50, every: int = 10, ckpt: int = 10, ncraft: 
             "method": "hand-written adjoint kernels, recompute from checkpoints"}
 
 
-def nested_adjoint_case(nbox: int = 3, steps: int = 10, every: int = 5, ckpt: int = 5,
+def nested_adjoint_case(nbox: int = 5, steps: int = 4, every: int = 2, ckpt: int = 2,
                         ncraft: int = 8):
     """The checkpointed adjoint of the whole box-step through ``nbox`` nested
     121x121x61 boxes (g3m3_amd.adjoint_nested; inner boxes bounded by their
