@@ -12,8 +12,9 @@ _corer_adjoint; g3m3_amd.adjoint_nested).
   linearised at the float32 values (fullref.Float32Values and the float32
   step's limiter branches): relative error 1e-5, or 3x what torch's float32
   autograd of the restatement reaches.
-* The gradient of a craft misfit over a checkpointed nested rollout, in the
-  controls and in every box's initial state, matches float64
+* The gradient of a craft misfit over a checkpointed nested rollout of two
+  and of three boxes, in the controls and in every box's initial state,
+  matches float64
   vector-Jacobian products chained back along the device's own state
   versions to the same bar (3e-5 for the controls, whose gradients are
   cancelling sums over the two boxes' craft); float64 central differences of the
@@ -244,10 +245,15 @@ def _device_grad(m, pos, target, wt, ctl0, steps, every, ckpt):
             [s.grad.cpu().numpy().copy() for s in s0])
 
 
-def test_nested_rollout_gradient():
-    _say("nested rollout gradient: chained float64 vjps and central differences")
-    steps, every = 8, 4
-    ss, m, pos_h, pos, target, wt = _misfit_case(steps=steps, every=every)
+@pytest.mark.parametrize("nbox", [2, 3])
+def test_nested_rollout_gradient(nbox):
+    """Two boxes: 8 full steps, chained vector-Jacobian products and
+    central differences.  Three boxes (a middle box that both receives its
+    child's flank adjoints and sends its own to its parent, and corers in
+    a chain): 4 full steps, chained vector-Jacobian products."""
+    _say("nested rollout gradient, boxes:", nbox)
+    steps, every = (8, 4) if nbox == 2 else (4, 2)
+    ss, m, pos_h, pos, target, wt = _misfit_case(nbox=nbox, steps=steps, every=every)
     n = m.n
     ctl0 = torch.tensor(ss[-1].ctl.astype(np.float64))
     L, g_ctl, g_s = _device_grad(m, pos, target, wt, ctl0, steps, every, 3)
@@ -266,7 +272,7 @@ def test_nested_rollout_gradient():
             m.sample(cur, pos, o, gp)
             samples.append(o)
     boxes = gp[:, 3].cpu().numpy()      # gridpt(4, p): the box holding craft p
-    assert set(boxes.tolist()) == {1, 2}, boxes
+    assert set(boxes.tolist()) == {1, nbox}, boxes
     res = 2.0 * (torch.stack(samples).cpu().double() - target.cpu().double()) * \
         wt.cpu().double() ** 2
     V = _Vjp(ss, c32)
@@ -353,6 +359,9 @@ def test_nested_rollout_gradient():
     bad = {k: v for k, v in es.items() if v > max(1e-5, 3.0 * es32[k])}
     bad.update({k: v for k, v in ec.items() if v > max(3e-5, 3.0 * ec32[k])})
     assert not bad, bad
+    if nbox != 2:
+        m.close()
+        return
 
     # float64 central differences of the restatement, and its float64
     # autograd gradient, in eta and the upstream density
